@@ -1,0 +1,27 @@
+"""flowpanel_amd -- MI355X-native (gfx950) hot path of the FLOWPanel.jl panel method.
+
+Host-side mirror of the reference's API over the C ABI of
+include/flowpanel_hip.h (libflowpanel_hip.so, built by ``build.py``):
+
+    grid, shedding = geometry.loft_wing(...)
+    body = RigidWakeBody(grid, shedding)
+    solve(body, Uinfs, Das, Dbs, solver=solve_ludiv)
+    Us = calcfield_U(body, body)
+    Cps = calcfield_Cp(body, magUinf)
+"""
+from . import geometry
+from .geometry import Grid, direction
+from .bodies import (ConstantDoublet, ConstantSource, MultiBody, NonLiftingBody, RigidWakeBody, VortexRing,
+                     calc_areas, calc_controlpoints, calc_normals)
+from .solvers import (assemble, calc_bc_noflowthrough, getrf, solve, solve_backslash, solve_gmres,
+                      solve_gmres_ir, solve_lu32_ir, solve_ludiv, last_stats)
+from .postprocess import Uind, phi, calcfield_U, calcfield_Cp, calcfield_F, calcfield_LDS
+from ._native import FPError, context
+
+__all__ = [
+    "geometry", "Grid", "direction", "ConstantSource", "ConstantDoublet", "VortexRing", "NonLiftingBody",
+    "RigidWakeBody", "MultiBody", "calc_normals", "calc_controlpoints", "calc_areas", "assemble",
+    "calc_bc_noflowthrough", "getrf", "solve", "solve_ludiv", "solve_backslash", "solve_gmres", "solve_lu32_ir",
+    "solve_gmres_ir", "last_stats", "Uind", "phi", "calcfield_U", "calcfield_Cp", "calcfield_F", "calcfield_LDS",
+    "FPError", "context",
+]

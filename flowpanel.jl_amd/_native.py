@@ -1,0 +1,202 @@
+"""ctypes binding of libflowpanel_hip.so (the C ABI of include/flowpanel_hip.h).
+
+The library is the only compute path: if it is missing or no GPU is visible,
+every call raises; nothing falls back to a CPU implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libflowpanel_hip.so")
+
+FP_OK = 0
+FP_ERR_ARG = 1
+FP_ERR_SHAPE = 2
+FP_ERR_NONUNIT_DIRECTION = 3
+FP_ERR_BAD_DIRECTIONS = 4
+FP_ERR_UNSOLVED = 5
+FP_ERR_SINGULAR = 6
+FP_ERR_NOT_CONVERGED = 7
+FP_ERR_DEVICE = 8
+FP_ERR_NOMEM = 9
+FP_ERR_UNSUPPORTED = 10
+
+FP_ELEM_SOURCE = 0
+FP_ELEM_VORTEXRING = 1
+FP_F64 = 0
+FP_F32 = 1
+FP_SOLVER_LU = 0
+FP_SOLVER_LU32_IR = 1
+FP_SOLVER_GMRES_IR = 2
+FP_SOLVER_GMRES = 3
+FP_KUTTA_NONE = 0
+FP_KUTTA_REFERENCE = 1
+FP_KUTTA_PAIR = 2
+CHARLEN = {"unitary": 0, "bbox": 1, "maxdist": 2, "sqrtarea": 3}
+
+EXPORTS = [
+    "fp_solve_opts_default", "fp_ctx_create", "fp_ctx_destroy", "fp_ctx_last_error", "fp_ctx_synchronize",
+    "fp_body_create", "fp_body_destroy", "fp_body_ncells", "fp_body_nsheddings", "fp_body_geometry",
+    "fp_body_set_freestream", "fp_body_set_strength", "fp_body_get_strength", "fp_solve_bodies",
+    "fp_assemble", "fp_mat_upload", "fp_mat_size", "fp_mat_download", "fp_mat_destroy", "fp_solve_mat",
+    "fp_mat_getrf", "fp_uind", "fp_phi", "fp_calcfield_U", "fp_calcfield_cp", "fp_ctx_profile",
+    "fp_ctx_profile_get",
+]
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [("solver", ctypes.c_int), ("tol", ctypes.c_double), ("inner_tol", ctypes.c_double),
+                ("restart", ctypes.c_int), ("max_iters", ctypes.c_int), ("max_refine", ctypes.c_int),
+                ("precond_block", ctypes.c_int)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("refinements", ctypes.c_int), ("iterations", ctypes.c_int), ("residual", ctypes.c_double),
+                ("ms_assemble", ctypes.c_double), ("ms_factor", ctypes.c_double), ("ms_solve", ctypes.c_double),
+                ("info", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class FPError(RuntimeError):
+    """Error raised by the HIP library; ``code`` is the FP_ERR_* status."""
+
+    def __init__(self, code, msg):
+        super().__init__("[flowpanel_hip %d] %s" % (code, msg))
+        self.code = code
+
+
+_LIB = None
+_CTX = None
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i64 = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError("libflowpanel_hip.so is not built (run flowpanel.jl_amd/build.py); "
+                           "there is no CPU fallback")
+    L = ctypes.CDLL(LIBPATH)
+    i64, dbl, cint = ctypes.c_int64, ctypes.c_double, ctypes.c_int
+    pp = ctypes.POINTER(_vp)
+    L.fp_solve_opts_default.argtypes = [ctypes.POINTER(SolveOpts), cint]
+    L.fp_solve_opts_default.restype = None
+    L.fp_ctx_create.argtypes = [cint, pp]
+    L.fp_ctx_destroy.argtypes = [_vp]
+    L.fp_ctx_destroy.restype = None
+    L.fp_ctx_last_error.argtypes = [_vp]
+    L.fp_ctx_last_error.restype = ctypes.c_char_p
+    L.fp_ctx_synchronize.argtypes = [_vp]
+    L.fp_body_create.argtypes = [_vp, _d, i64, _i64, _i64, i64, cint, cint, _i64, i64, dbl, cint, dbl, dbl, pp]
+    L.fp_body_destroy.argtypes = [_vp]
+    L.fp_body_destroy.restype = None
+    L.fp_body_ncells.argtypes = [_vp]
+    L.fp_body_ncells.restype = i64
+    L.fp_body_nsheddings.argtypes = [_vp]
+    L.fp_body_nsheddings.restype = i64
+    L.fp_body_geometry.argtypes = [_vp, _vp, _d, _d, _d]
+    L.fp_body_set_freestream.argtypes = [_vp, _vp, _d, _d, _d]
+    L.fp_body_set_strength.argtypes = [_vp, _vp, _d]
+    L.fp_body_get_strength.argtypes = [_vp, _vp, _d]
+    L.fp_solve_bodies.argtypes = [_vp, pp, cint, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats)]
+    L.fp_assemble.argtypes = [_vp, pp, cint, cint, pp]
+    L.fp_mat_upload.argtypes = [_vp, i64, _d, pp]
+    L.fp_mat_size.argtypes = [_vp]
+    L.fp_mat_size.restype = i64
+    L.fp_mat_download.argtypes = [_vp, _vp, _d]
+    L.fp_mat_destroy.argtypes = [_vp]
+    L.fp_mat_destroy.restype = None
+    L.fp_solve_mat.argtypes = [_vp, _vp, _d, i64, _d, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats)]
+    L.fp_mat_getrf.argtypes = [_vp, _vp, cint, _d, _i64]
+    L.fp_uind.argtypes = [_vp, pp, cint, _d, i64, _d]
+    L.fp_phi.argtypes = [_vp, pp, cint, _d, i64, _d]
+    L.fp_calcfield_U.argtypes = [_vp, pp, cint, _d]
+    L.fp_calcfield_cp.argtypes = [_vp, _d, i64, dbl, _i64, i64, cint, cint, _d]
+    L.fp_ctx_profile.argtypes = [_vp, cint]
+    L.fp_ctx_profile_get.argtypes = [_vp, ctypes.c_char_p, _i64, _d]
+    _LIB = L
+    return L
+
+
+class Context:
+    """One HIP device + stream (fp_ctx)."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        st = lib().fp_ctx_create(device, ctypes.byref(h))
+        if st != FP_OK:
+            raise FPError(st, "fp_ctx_create(%d) failed: no usable GPU" % device)
+        self.handle = h
+
+    def check(self, st):
+        if st != FP_OK:
+            msg = lib().fp_ctx_last_error(self.handle)
+            raise FPError(st, msg.decode() if msg else "")
+
+    def synchronize(self):
+        self.check(lib().fp_ctx_synchronize(self.handle))
+
+    def profile(self, enable=True):
+        """Start (clearing) or stop per-kernel device timing."""
+        self.check(lib().fp_ctx_profile(self.handle, int(bool(enable))))
+
+    def profile_get(self, kernel):
+        """(launches, total device ms) recorded for one kernel class."""
+        n = np.zeros(1, dtype=np.int64)
+        ms = np.zeros(1)
+        self.check(lib().fp_ctx_profile_get(self.handle, kernel.encode(), iptr(n), dptr(ms)))
+        return int(n[0]), float(ms[0])
+
+    def __del__(self):
+        try:
+            if self.handle and _LIB is not None:
+                _LIB.fp_ctx_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def context():
+    """Process-wide default context on device 0."""
+    global _CTX
+    if _CTX is None:
+        _CTX = Context(0)
+    return _CTX
+
+
+def dptr(a):
+    return a.ctypes.data_as(_d) if a is not None else None
+
+
+def iptr(a):
+    return a.ctypes.data_as(_i64) if a is not None else None
+
+
+def f64(a):
+    """Column-major contiguous float64 (a 3 x n array becomes x,y,z per point)."""
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def handles(objs):
+    arr = (_vp * len(objs))(*[o.handle for o in objs])
+    return arr
+
+
+def opts(solver, **kw):
+    o = SolveOpts()
+    lib().fp_solve_opts_default(ctypes.byref(o), solver)
+    for k, v in kw.items():
+        if v is None:
+            continue
+        if not hasattr(o, k):
+            raise TypeError("unknown solver option %r" % k)
+        setattr(o, k, v)
+    return o

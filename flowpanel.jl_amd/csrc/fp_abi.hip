@@ -1,0 +1,1215 @@
+// fp_abi.hip -- implementation of the C ABI declared in include/flowpanel_hip.h.
+//
+// Host-side orchestration only: argument checks that mirror the reference's
+// error() cases, device allocations, and the solver drivers (LU, FP32 LU +
+// FP64 refinement, block-Jacobi GMRES inside FP64 refinement, FP64 GMRES).
+// All arithmetic runs in the kernels of fp_influence.hip and fp_dense.hip.
+#include "fp_internal.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+
+// vector helpers from fp_dense.hip
+int fp_mdot(fp_ctx *ctx, const double *V, int64_t ldv, int nv, const double *w, int64_t n, double *part,
+            double *out);
+int fp_vupdate(fp_ctx *ctx, const double *V, int64_t ldv, int nv, const double *h, double *w, int64_t n);
+int fp_axpby(fp_ctx *ctx, double a, const double *x, double b, double *y, int64_t n);
+int fp_vcombine(fp_ctx *ctx, const double *V, int64_t ldv, int nv, const double *z, double *x, int64_t n);
+
+int fp_fail(fp_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int fp_hip_fail(fp_ctx *ctx, hipError_t e, const char *what)
+{
+    char buf[512];
+    snprintf(buf, sizeof buf, "HIP error %d (%s) in %s", (int)e, hipGetErrorString(e), what);
+    return fp_fail(ctx, e == hipErrorOutOfMemory ? FP_ERR_NOMEM : FP_ERR_DEVICE, buf);
+}
+
+namespace {
+
+// device buffer owned by a host scope
+struct DBuf {
+    void *p = nullptr;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    template <typename T> T *as() const { return (T *)p; }
+};
+
+int dalloc(fp_ctx *ctx, DBuf &b, size_t bytes)
+{
+    if (b.p) { (void)hipFree(b.p); b.p = nullptr; }
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) { b.p = nullptr; return fp_hip_fail(ctx, e, "hipMalloc"); }
+    return FP_OK;
+}
+
+int64_t round_ld(int64_t n) { return std::max<int64_t>(64, (n + 63) / 64 * 64); }
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipStream_t s;
+    explicit Timer(hipStream_t st) : s(st)
+    {
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        (void)hipEventRecord(a, s);
+    }
+    double stop()
+    {
+        (void)hipEventRecord(b, s);
+        (void)hipEventSynchronize(b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, a, b);
+        return ms;
+    }
+    void restart() { (void)hipEventRecord(a, s); }
+    ~Timer()
+    {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+bool check_unit(const double *d) { return std::fabs(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] - 1.0) <= 2 * DBL_EPSILON; }
+
+// Combined control points + normals of several bodies (SoA, M = sum N).
+struct Targets {
+    DBuf buf;
+    TargetView tv{};
+    int64_t M = 0;
+};
+
+int gather_targets(fp_ctx *ctx, fp_body *const *bodies, int nb, Targets &t)
+{
+    int64_t M = 0;
+    for (int b = 0; b < nb; ++b) M += bodies[b]->N;
+    t.M = M;
+    if (nb == 1) {
+        const fp_body *b = bodies[0];
+        const int64_t N = b->N;
+        t.tv.nx = b->d_geo; t.tv.ny = b->d_geo + N; t.tv.nz = b->d_geo + 2 * N;
+        t.tv.x = b->d_geo + 3 * N; t.tv.y = b->d_geo + 4 * N; t.tv.z = b->d_geo + 5 * N;
+        t.tv.M = N;
+        return FP_OK;
+    }
+    int st = dalloc(ctx, t.buf, 6 * M * sizeof(double));
+    if (st) return st;
+    double *base = t.buf.as<double>();
+    int64_t off = 0;
+    for (int b = 0; b < nb; ++b) {
+        const fp_body *bd = bodies[b];
+        for (int c = 0; c < 6; ++c)
+            FP_HIP(ctx, hipMemcpyAsync(base + c * M + off, bd->d_geo + c * bd->N, bd->N * sizeof(double),
+                                       hipMemcpyDeviceToDevice, ctx->stream));
+        off += bd->N;
+    }
+    t.tv.nx = base; t.tv.ny = base + M; t.tv.nz = base + 2 * M;
+    t.tv.x = base + 3 * M; t.tv.y = base + 4 * M; t.tv.z = base + 5 * M;
+    t.tv.M = M;
+    return FP_OK;
+}
+
+int check_bodies(fp_ctx *ctx, fp_body *const *bodies, int nb)
+{
+    if (!ctx) return FP_ERR_ARG;
+    if (!bodies || nb <= 0) return fp_fail(ctx, FP_ERR_ARG, "no bodies given");
+    for (int b = 0; b < nb; ++b)
+        if (!bodies[b] || bodies[b]->ctx != ctx) return fp_fail(ctx, FP_ERR_ARG, "invalid body handle");
+    return FP_OK;
+}
+
+// ----------------------------------------------------------------------------
+// operators for the iterative solvers
+struct Operator {
+    virtual int apply(const double *x, double *y) = 0;  // y = A x
+    virtual ~Operator() {}
+};
+
+template <typename T>
+struct DenseOp : Operator {
+    fp_ctx *ctx; const T *A; int64_t n, ld;
+    DenseOp(fp_ctx *c, const T *a, int64_t n_, int64_t ld_) : ctx(c), A(a), n(n_), ld(ld_) {}
+    int apply(const double *x, double *y) override { return fp_gemv<T>(ctx, A, n, ld, x, y, 1.0, 0.0); }
+};
+
+// y = G x recomputed from the geometry in FP64 (the normal velocity induced
+// at every control point by strengths x): no stored matrix.
+struct MatrixFreeOp : Operator {
+    fp_ctx *ctx; std::vector<fp_body *> bodies; TargetView tv; DBuf part; size_t part_bytes = 0; int64_t n;
+    MatrixFreeOp(fp_ctx *c, const std::vector<fp_body *> &b, const TargetView &t) : ctx(c), bodies(b), tv(t), n(t.M) {}
+    int init()
+    {
+        for (auto *b : bodies) part_bytes = std::max(part_bytes, fp_field_partial_bytes(tv.M, b->N));
+        return part_bytes ? dalloc(ctx, part, part_bytes) : FP_OK;
+    }
+    int apply(const double *x, double *y) override
+    {
+        FP_HIP(ctx, hipMemsetAsync(y, 0, n * sizeof(double), ctx->stream));
+        int64_t col = 0;
+        for (auto *b : bodies) {
+            int st = fp_launch_field(ctx, b, x + col, tv, 1, y, part.as<double>(), part_bytes);
+            if (st) return st;
+            col += b->N;
+        }
+        return FP_OK;
+    }
+};
+
+struct Precond {
+    virtual int apply(const double *x, double *y) = 0;
+    virtual ~Precond() {}
+};
+struct BJPrecond : Precond {
+    fp_ctx *ctx; const float *BJ; const int *piv; int64_t n; int block;
+    BJPrecond(fp_ctx *c, const float *b, const int *p, int64_t n_, int bl) : ctx(c), BJ(b), piv(p), n(n_), block(bl) {}
+    int apply(const double *x, double *y) override { return fp_bj_apply(ctx, BJ, piv, n, block, x, y); }
+};
+
+double dnorm_host(fp_ctx *ctx, const double *x, int64_t n, double *part, double *dout, int *st)
+{
+    *st = fp_mdot(ctx, x, n, 1, x, n, part, dout);
+    if (*st) return 0;
+    double v = 0;
+    hipError_t e = hipMemcpyAsync(&v, dout, sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) { *st = fp_hip_fail(ctx, e, "norm download"); return 0; }
+    return std::sqrt(v);
+}
+
+// Restarted GMRES with right preconditioning (x0 = 0 on entry when
+// `zero_start`).  Stops when ||b - A x|| <= tol * ||b|| (Arnoldi estimate)
+// or after maxit iterations.  CGS with one re-orthogonalisation.
+struct Gmres {
+    fp_ctx *ctx; int64_t n; int m;
+    DBuf V, Z, w, h, part, tmp;
+    int init(fp_ctx *c, int64_t n_, int m_)
+    {
+        ctx = c; n = n_; m = m_;
+        int st;
+        if ((st = dalloc(ctx, V, (size_t)n * (m + 1) * sizeof(double)))) return st;
+        if ((st = dalloc(ctx, w, n * sizeof(double)))) return st;
+        if ((st = dalloc(ctx, tmp, n * sizeof(double)))) return st;
+        if ((st = dalloc(ctx, Z, n * sizeof(double)))) return st;
+        if ((st = dalloc(ctx, h, (m + 2) * 2 * sizeof(double)))) return st;
+        if ((st = dalloc(ctx, part, (size_t)256 * (m + 2) * sizeof(double)))) return st;
+        return FP_OK;
+    }
+    // solve A x = b; x overwritten. Returns iterations via *its, final relative residual estimate via *rel.
+    int solve(Operator &A, Precond *M, const double *b, double *x, double tol, int maxit, int *its, double *rel)
+    {
+        int st = FP_OK;
+        double *Vp = V.as<double>(), *wp = w.as<double>(), *hp = h.as<double>(), *pp = part.as<double>();
+        double *tp = tmp.as<double>(), *zp = Z.as<double>();
+        const double bnorm = dnorm_host(ctx, b, n, pp, hp, &st);
+        if (st) return st;
+        *its = 0;
+        *rel = 0;
+        FP_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
+        if (bnorm == 0) return FP_OK;
+        std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hv(m + 2);
+        bool first = true;
+        for (;;) {
+            // r = b - A x
+            if (first) {
+                FP_HIP(ctx, hipMemcpyAsync(wp, b, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            } else {
+                if ((st = A.apply(x, wp))) return st;
+                if ((st = fp_residual_update(ctx, b, wp, n))) return st;
+            }
+            first = false;
+            const double beta = dnorm_host(ctx, wp, n, pp, hp, &st);
+            if (st) return st;
+            *rel = beta / bnorm;
+            if (*rel <= tol || *its >= maxit) return FP_OK;
+            if ((st = fp_axpby(ctx, 1.0 / beta, wp, 0.0, Vp, n))) return st;
+            std::fill(g.begin(), g.end(), 0.0);
+            g[0] = beta;
+            int j = 0;
+            for (; j < m && *its < maxit; ++j) {
+                const double *vj = Vp + (int64_t)j * n;
+                const double *zin = vj;
+                if (M) {
+                    if ((st = M->apply(vj, zp))) return st;
+                    zin = zp;
+                }
+                if ((st = A.apply(zin, wp))) return st;
+                // CGS2
+                if ((st = fp_mdot(ctx, Vp, n, j + 1, wp, n, pp, hp))) return st;
+                if ((st = fp_vupdate(ctx, Vp, n, j + 1, hp, wp, n))) return st;
+                if ((st = fp_mdot(ctx, Vp, n, j + 1, wp, n, pp, hp + (m + 2)))) return st;
+                if ((st = fp_vupdate(ctx, Vp, n, j + 1, hp + (m + 2), wp, n))) return st;
+                if ((st = fp_mdot(ctx, wp, n, 1, wp, n, pp, hp + 2 * (m + 2) - 1))) return st;
+                std::vector<double> hh(2 * (m + 2));
+                FP_HIP(ctx, hipMemcpyAsync(hh.data(), hp, 2 * (m + 2) * sizeof(double), hipMemcpyDeviceToHost,
+                                           ctx->stream));
+                FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                for (int i = 0; i <= j; ++i) hv[i] = hh[i] + hh[(m + 2) + i];
+                const double hn = std::sqrt(std::max(0.0, hh[2 * (m + 2) - 1]));
+                // apply previous rotations
+                for (int i = 0; i < j; ++i) {
+                    const double t = cs[i] * hv[i] + sn[i] * hv[i + 1];
+                    hv[i + 1] = -sn[i] * hv[i] + cs[i] * hv[i + 1];
+                    hv[i] = t;
+                }
+                const double den = std::hypot(hv[j], hn);
+                cs[j] = den > 0 ? hv[j] / den : 1.0;
+                sn[j] = den > 0 ? hn / den : 0.0;
+                hv[j] = den;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] = cs[j] * g[j];
+                for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = hv[i];
+                ++*its;
+                if (hn > 0) {
+                    if ((st = fp_axpby(ctx, 1.0 / hn, wp, 0.0, Vp + (int64_t)(j + 1) * n, n))) return st;
+                }
+                *rel = std::fabs(g[j + 1]) / bnorm;
+                if (*rel <= tol || hn == 0) { ++j; break; }
+            }
+            // y = H^-1 g
+            for (int i = j - 1; i >= 0; --i) {
+                double acc = g[i];
+                for (int q = i + 1; q < j; ++q) acc -= H[(size_t)i * m + q] * y[q];
+                y[i] = acc / H[(size_t)i * m + i];
+            }
+            FP_HIP(ctx, hipMemcpyAsync(hp, y.data(), j * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            if ((st = fp_vcombine(ctx, Vp, n, j, hp, tp, n))) return st;
+            if (M) {
+                if ((st = M->apply(tp, zp))) return st;
+                if ((st = fp_axpby(ctx, 1.0, zp, 1.0, x, n))) return st;
+            } else {
+                if ((st = fp_axpby(ctx, 1.0, tp, 1.0, x, n))) return st;
+            }
+            FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            if (*rel <= tol || *its >= maxit) {
+                // report the true residual of the final iterate
+                if ((st = A.apply(x, wp))) return st;
+                if ((st = fp_residual_update(ctx, b, wp, n))) return st;
+                const double rn = dnorm_host(ctx, wp, n, pp, hp, &st);
+                if (st) return st;
+                *rel = rn / bnorm;
+                return FP_OK;
+            }
+        }
+    }
+};
+
+// LAPACK ipiv (0-based swap sequence) -> row permutation, on the host.
+int ipiv_to_perm(fp_ctx *ctx, const int *d_ipiv, int64_t n, int *d_perm)
+{
+    std::vector<int> ipiv(n), perm(n);
+    FP_HIP(ctx, hipMemcpyAsync(ipiv.data(), d_ipiv, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i < n; ++i) perm[i] = (int)i;
+    for (int64_t i = 0; i < n; ++i) {
+        const int p = ipiv[i];
+        if (p < 0 || p >= n) return fp_fail(ctx, FP_ERR_DEVICE, "corrupt pivot from getrf");
+        if (p != i) std::swap(perm[i], perm[p]);
+    }
+    FP_HIP(ctx, hipMemcpyAsync(d_perm, perm.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    return FP_OK;
+}
+
+template <typename T>
+int factor(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *d_ipiv, int *d_perm, int *info_out)
+{
+    DBuf work, info;
+    int st;
+    if ((st = dalloc(ctx, work, fp_getrf_work_bytes(n, sizeof(T))))) return st;
+    if ((st = dalloc(ctx, info, sizeof(int)))) return st;
+    if ((st = fp_getrf<T>(ctx, A, n, ld, d_ipiv, info.as<int>(), work.p, fp_getrf_work_bytes(n, sizeof(T)))))
+        return st;
+    int hinfo = 0;
+    FP_HIP(ctx, hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    if ((st = ipiv_to_perm(ctx, d_ipiv, n, d_perm))) return st;
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (info_out) *info_out = hinfo;
+    if (hinfo) {
+        char buf[128];
+        snprintf(buf, sizeof buf, "singular matrix: zero pivot at column %d", hinfo);
+        return fp_fail(ctx, FP_ERR_SINGULAR, buf);
+    }
+    return FP_OK;
+}
+
+// Core solver over an operator for the FP64 residual and (optionally) a
+// stored matrix of the right precision.  b, x device FP64 vectors.
+struct SolveArgs {
+    int solver;
+    int64_t n, ld;
+    double *A64 = nullptr;   // FP64 matrix (LU path, or residual for uploaded matrices)
+    float *A32 = nullptr;    // FP32 matrix (LU32_IR / GMRES_IR)
+    Operator *resid = nullptr;  // FP64 residual operator
+};
+
+int solve_core(fp_ctx *ctx, const SolveArgs &a, const fp_solve_opts &o, const double *b, double *x,
+               fp_solve_stats &stats)
+{
+    const int64_t n = a.n;
+    int st;
+    DBuf ipiv, perm, work, r, d, part, dout;
+    if ((st = dalloc(ctx, part, 256 * 4 * sizeof(double)))) return st;
+    if ((st = dalloc(ctx, dout, 4 * sizeof(double)))) return st;
+    Timer tm(ctx->stream);
+    if (a.solver == FP_SOLVER_LU) {
+        if ((st = dalloc(ctx, ipiv, n * sizeof(int)))) return st;
+        if ((st = dalloc(ctx, perm, n * sizeof(int)))) return st;
+        if ((st = factor<double>(ctx, a.A64, n, a.ld, ipiv.as<int>(), perm.as<int>(), &stats.info))) return st;
+        stats.ms_factor = tm.stop();
+        tm.restart();
+        if ((st = dalloc(ctx, work, fp_getrs_work_bytes(n, 1)))) return st;
+        FP_HIP(ctx, hipMemcpyAsync(x, b, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        if ((st = fp_getrs<double>(ctx, a.A64, n, a.ld, perm.as<int>(), x, 1, n, work.p, fp_getrs_work_bytes(n, 1))))
+            return st;
+        stats.ms_solve = tm.stop();
+        stats.refinements = 0;
+        if (a.resid) {  // report the FP64 residual
+            if ((st = dalloc(ctx, r, n * sizeof(double)))) return st;
+            if ((st = a.resid->apply(x, r.as<double>()))) return st;
+            if ((st = fp_residual_update(ctx, b, r.as<double>(), n))) return st;
+            const double bn = dnorm_host(ctx, b, n, part.as<double>(), dout.as<double>(), &st);
+            const double rn = dnorm_host(ctx, r.as<double>(), n, part.as<double>(), dout.as<double>(), &st);
+            stats.residual = bn > 0 ? rn / bn : 0;
+        }
+        return FP_OK;
+    }
+    if (a.solver == FP_SOLVER_GMRES) {
+        // FP64 GMRES on the FP64 operator (solve_gmres!), optional block-Jacobi
+        Gmres g;
+        if ((st = g.init(ctx, n, std::max(1, o.restart)))) return st;
+        std::unique_ptr<BJPrecond> M;
+        DBuf bj, bjp, info;
+        if (o.precond_block > 0) {
+            const int blk = std::min<int64_t>(o.precond_block, n);
+            const int64_t nbk = (n + blk - 1) / blk;
+            if ((st = dalloc(ctx, bj, (size_t)nbk * blk * blk * sizeof(float)))) return st;
+            if ((st = dalloc(ctx, bjp, (size_t)nbk * blk * sizeof(int)))) return st;
+            if ((st = dalloc(ctx, info, sizeof(int)))) return st;
+            if (a.A64) st = fp_bj_factor<double>(ctx, a.A64, n, a.ld, blk, bj.as<float>(), bjp.as<int>(), info.as<int>());
+            else st = fp_bj_factor<float>(ctx, a.A32, n, a.ld, blk, bj.as<float>(), bjp.as<int>(), info.as<int>());
+            if (st) return st;
+            M.reset(new BJPrecond(ctx, bj.as<float>(), bjp.as<int>(), n, blk));
+        }
+        stats.ms_factor = tm.stop();
+        tm.restart();
+        int its = 0;
+        double rel = 0;
+        if ((st = g.solve(*a.resid, M.get(), b, x, o.tol, o.max_iters, &its, &rel))) return st;
+        stats.ms_solve = tm.stop();
+        stats.iterations = its;
+        stats.residual = rel;
+        if (rel > o.tol) return fp_fail(ctx, FP_ERR_NOT_CONVERGED, "GMRES did not reach the tolerance");
+        return FP_OK;
+    }
+    // refinement paths: FP32 inner solver, FP64 residual and update
+    if (!a.A32) return fp_fail(ctx, FP_ERR_ARG, "internal: no FP32 matrix");
+    std::unique_ptr<Gmres> g;
+    std::unique_ptr<BJPrecond> M;
+    std::unique_ptr<DenseOp<float>> A32op;
+    DBuf bj, bjp, info;
+    if (a.solver == FP_SOLVER_LU32_IR) {
+        if ((st = dalloc(ctx, ipiv, n * sizeof(int)))) return st;
+        if ((st = dalloc(ctx, perm, n * sizeof(int)))) return st;
+        if ((st = factor<float>(ctx, a.A32, n, a.ld, ipiv.as<int>(), perm.as<int>(), &stats.info))) return st;
+        if ((st = dalloc(ctx, work, fp_getrs_work_bytes(n, 1)))) return st;
+    } else if (a.solver == FP_SOLVER_GMRES_IR) {
+        const int blk = (int)std::min<int64_t>(o.precond_block > 0 ? o.precond_block : 512, n);
+        const int64_t nbk = (n + blk - 1) / blk;
+        if ((st = dalloc(ctx, bj, (size_t)nbk * blk * blk * sizeof(float)))) return st;
+        if ((st = dalloc(ctx, bjp, (size_t)nbk * blk * sizeof(int)))) return st;
+        if ((st = dalloc(ctx, info, sizeof(int)))) return st;
+        if ((st = fp_bj_factor<float>(ctx, a.A32, n, a.ld, blk, bj.as<float>(), bjp.as<int>(), info.as<int>())))
+            return st;
+        M.reset(new BJPrecond(ctx, bj.as<float>(), bjp.as<int>(), n, blk));
+        A32op.reset(new DenseOp<float>(ctx, a.A32, n, a.ld));
+        g.reset(new Gmres());
+        if ((st = g->init(ctx, n, std::max(1, o.restart)))) return st;
+    } else {
+        return fp_fail(ctx, FP_ERR_ARG, "unknown solver");
+    }
+    stats.ms_factor = tm.stop();
+    tm.restart();
+    if ((st = dalloc(ctx, r, n * sizeof(double)))) return st;
+    if ((st = dalloc(ctx, d, n * sizeof(double)))) return st;
+    double *rp = r.as<double>(), *dp = d.as<double>();
+    const double bn = dnorm_host(ctx, b, n, part.as<double>(), dout.as<double>(), &st);
+    if (st) return st;
+    FP_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
+    FP_HIP(ctx, hipMemcpyAsync(rp, b, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    double rel = bn > 0 ? 1.0 : 0.0;
+    int k = 0;
+    stats.iterations = 0;
+    while (rel > o.tol && k < o.max_refine) {
+        // correction d ~ A^-1 r in the low-precision solver
+        if (a.solver == FP_SOLVER_LU32_IR) {
+            FP_HIP(ctx, hipMemcpyAsync(dp, rp, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            if ((st = fp_getrs<float>(ctx, a.A32, n, a.ld, perm.as<int>(), dp, 1, n, work.p,
+                                      fp_getrs_work_bytes(n, 1))))
+                return st;
+        } else {
+            int its = 0;
+            double irel = 0;
+            const int left = std::max(1, o.max_iters - stats.iterations);
+            if ((st = g->solve(*A32op, M.get(), rp, dp, o.inner_tol, left, &its, &irel))) return st;
+            stats.iterations += its;
+        }
+        if ((st = fp_axpby(ctx, 1.0, dp, 1.0, x, n))) return st;
+        // FP64 residual
+        if ((st = a.resid->apply(x, rp))) return st;
+        if ((st = fp_residual_update(ctx, b, rp, n))) return st;
+        const double rn = dnorm_host(ctx, rp, n, part.as<double>(), dout.as<double>(), &st);
+        if (st) return st;
+        rel = bn > 0 ? rn / bn : 0.0;
+        ++k;
+    }
+    stats.ms_solve = tm.stop();
+    stats.refinements = k;
+    stats.residual = rel;
+    if (rel > o.tol) return fp_fail(ctx, FP_ERR_NOT_CONVERGED, "iterative refinement did not reach the tolerance");
+    return FP_OK;
+}
+
+// host AoS 3 x n -> device SoA
+int upload_soa3(fp_ctx *ctx, const double *h, int64_t n, double *d)
+{
+    std::vector<double> tmp(3 * n);
+    for (int64_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) tmp[c * n + i] = h[3 * i + c];
+    FP_HIP(ctx, hipMemcpy(d, tmp.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
+    return FP_OK;
+}
+
+int build_horseshoes(fp_ctx *ctx, fp_body *b)
+{
+    if (b->d_hh) { (void)hipFree(b->d_hh); b->d_hh = nullptr; }
+    if (b->d_hcol) { (void)hipFree(b->d_hcol); b->d_hcol = nullptr; }
+    b->nh = 0;
+    if (b->ns == 0) return FP_OK;
+    std::vector<double> rows;  // per horseshoe 12 values
+    std::vector<int> cols;
+    for (int64_t e = 0; e < b->ns; ++e) {
+        const int64_t *s = &b->shedding[6 * e];
+        const double *da = &b->das[3 * e], *db = &b->dbs[3 * e];
+        for (int side = 0; side < 2; ++side) {
+            const int64_t p = side == 0 ? s[0] : s[3];
+            if (p == -1) continue;
+            const int64_t na = side == 0 ? s[1] : s[4], nbv = side == 0 ? s[2] : s[5];
+            const int64_t ia = b->cell_idx[b->cell_ptr[p] + na], ib = b->cell_idx[b->cell_ptr[p] + nbv];
+            const double *d1 = side == 0 ? da : db, *d2 = side == 0 ? db : da;
+            double h[12];
+            for (int c = 0; c < 3; ++c) {
+                h[c] = b->nodes[3 * ia + c];
+                h[3 + c] = b->nodes[3 * ib + c];
+                h[6 + c] = d1[c];
+                h[9 + c] = d2[c];
+            }
+            rows.insert(rows.end(), h, h + 12);
+            cols.push_back((int)p);
+        }
+    }
+    const int nh = (int)cols.size();
+    std::vector<double> soa((size_t)12 * nh);
+    for (int h = 0; h < nh; ++h)
+        for (int c = 0; c < 12; ++c) soa[(size_t)c * nh + h] = rows[(size_t)h * 12 + c];
+    FP_HIP(ctx, hipMalloc((void **)&b->d_hh, soa.size() * sizeof(double)));
+    FP_HIP(ctx, hipMalloc((void **)&b->d_hcol, nh * sizeof(int)));
+    FP_HIP(ctx, hipMemcpy(b->d_hh, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
+    FP_HIP(ctx, hipMemcpy(b->d_hcol, cols.data(), nh * sizeof(int), hipMemcpyHostToDevice));
+    b->nh = nh;
+    return FP_OK;
+}
+
+__global__ void k_aos_add_soa(const double *__restrict__ soa, int64_t n, double *__restrict__ aos)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    aos[3 * i] += soa[i];
+    aos[3 * i + 1] += soa[n + i];
+    aos[3 * i + 2] += soa[2 * n + i];
+}
+
+template <typename T>
+__global__ void k_download_cm(const T *__restrict__ src, int64_t n, int64_t ld, double *__restrict__ dst)
+{
+    // dst column-major: dst[j*n + i] = src[i*ld + j]; one block row per column j
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (i < n) dst[j * n + i] = (double)src[i * ld + j];
+}
+
+// Cp with Kutta averaging (src/FLOWPanel_postprocess.jl:1059-1112)
+__global__ void k_cp(const double *__restrict__ U, int64_t n, double uref, double *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double u = sqrt(U[3 * i] * U[3 * i] + U[3 * i + 1] * U[3 * i + 1] + U[3 * i + 2] * U[3 * i + 2]) / uref;
+    out[i] += 1 - u * u;
+}
+
+__global__ void k_kutta(const int64_t *__restrict__ sh, int64_t ns, int mode, double *__restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int64_t e = 0; e < ns; ++e) {
+        const int64_t pi = sh[6 * e], pj = sh[6 * e + 3];
+        if (mode == FP_KUTTA_REFERENCE) {
+            if (pj != -1) {
+                const double ave = (out[pi] + out[pi + 1] + out[pj] + out[pj - 1]) / 4;
+                out[pi] = ave; out[pi + 1] = ave; out[pj] = ave; out[pj - 1] = ave;
+            } else {
+                const double ave = (out[pi] + out[pi + 1]) / 2;
+                out[pi] = ave; out[pi + 1] = ave;
+            }
+        } else if (pj != -1) {
+            const double ave = (out[pi] + out[pj]) / 2;
+            out[pi] = ave; out[pj] = ave;
+        }
+    }
+}
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+void fp_solve_opts_default(fp_solve_opts *o, int solver)
+{
+    if (!o) return;
+    o->solver = solver;
+    o->restart = 50;
+    o->max_iters = 2000;
+    o->max_refine = 20;
+    o->precond_block = 0;
+    o->inner_tol = 1e-5;
+    o->tol = 1e-12;
+    if (solver == FP_SOLVER_GMRES) {
+        o->tol = std::sqrt(DBL_EPSILON);  // Krylov.gmres atol = rtol = sqrt(eps) (src/FLOWPanel_linearsolver.jl:136-139)
+    } else if (solver == FP_SOLVER_GMRES_IR) {
+        o->precond_block = 512;
+    }
+}
+
+int fp_ctx_create(int device, fp_ctx **out)
+{
+    if (!out) return FP_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return FP_ERR_DEVICE;
+    if (device < 0 || device >= ndev) return FP_ERR_ARG;
+    std::unique_ptr<fp_ctx> c(new fp_ctx());
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) return FP_ERR_DEVICE;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return FP_ERR_DEVICE;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) == hipSuccess) c->num_cu = p.multiProcessorCount;
+    *out = c.release();
+    return FP_OK;
+}
+
+void fp_ctx_destroy(fp_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    for (auto &r : ctx->prof_pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    delete ctx;
+}
+
+const char *fp_ctx_last_error(const fp_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+static void prof_fold(fp_ctx *ctx)
+{
+    if (ctx->prof_pending.empty()) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &r : ctx->prof_pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            auto &acc = ctx->prof_acc[r.name];
+            acc.first += 1;
+            acc.second += ms;
+        }
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    ctx->prof_pending.clear();
+}
+
+int fp_ctx_profile(fp_ctx *ctx, int enable)
+{
+    if (!ctx) return FP_ERR_ARG;
+    prof_fold(ctx);
+    if (enable) ctx->prof_acc.clear();
+    ctx->prof = enable != 0;
+    return FP_OK;
+}
+
+int fp_ctx_profile_get(fp_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms)
+{
+    if (!ctx || !kernel || !launches || !total_ms) return FP_ERR_ARG;
+    prof_fold(ctx);
+    auto it = ctx->prof_acc.find(kernel);
+    *launches = it == ctx->prof_acc.end() ? 0 : it->second.first;
+    *total_ms = it == ctx->prof_acc.end() ? 0.0 : it->second.second;
+    return FP_OK;
+}
+
+int fp_ctx_synchronize(fp_ctx *ctx)
+{
+    if (!ctx) return FP_ERR_ARG;
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FP_OK;
+}
+
+int fp_body_create(fp_ctx *ctx, const double *nodes, int64_t nnodes, const int64_t *cell_ptr, const int64_t *cell_idx,
+                   int64_t ncells, int index_base, int element, const int64_t *shedding, int64_t ns,
+                   double cp_offset, int charlen_kind, double kernel_offset, double kernel_cutoff, fp_body **out)
+{
+    if (!ctx || !out) return FP_ERR_ARG;
+    *out = nullptr;
+    if (!nodes || !cell_ptr || !cell_idx || nnodes <= 0 || ncells <= 0)
+        return fp_fail(ctx, FP_ERR_ARG, "empty or null geometry");
+    if (index_base != 0 && index_base != 1) return fp_fail(ctx, FP_ERR_ARG, "index_base must be 0 or 1");
+    if (element != FP_ELEM_SOURCE && element != FP_ELEM_VORTEXRING)
+        return fp_fail(ctx, FP_ERR_ARG, "unknown element type");
+    if (charlen_kind < 0 || charlen_kind > 3) return fp_fail(ctx, FP_ERR_ARG, "unknown characteristic length");
+    if (ns < 0 || (ns > 0 && !shedding)) return fp_fail(ctx, FP_ERR_ARG, "null shedding matrix");
+    if (element == FP_ELEM_SOURCE && ns > 0)
+        return fp_fail(ctx, FP_ERR_ARG, "a constant-source body sheds no wake");
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    std::unique_ptr<fp_body> b(new fp_body());
+    b->ctx = ctx;
+    b->element = element;
+    b->N = ncells;
+    b->kernel_offset = kernel_offset;
+    b->kernel_cutoff = kernel_cutoff;
+    b->cp_offset = cp_offset;
+    b->charlen = charlen_kind;
+    b->nodes.assign(nodes, nodes + 3 * nnodes);
+    b->cell_ptr.resize(ncells + 1);
+    const int64_t p0 = cell_ptr[0];
+    int maxnv = 0;
+    for (int64_t c = 0; c <= ncells; ++c) b->cell_ptr[c] = cell_ptr[c] - p0;
+    for (int64_t c = 0; c < ncells; ++c) {
+        const int64_t nv = b->cell_ptr[c + 1] - b->cell_ptr[c];
+        if (nv < 3 || nv > FP_MAX_NV) return fp_fail(ctx, FP_ERR_UNSUPPORTED, "panels need 3..8 vertices");
+        maxnv = std::max<int>(maxnv, (int)nv);
+    }
+    const int64_t nidx = b->cell_ptr[ncells];
+    b->cell_idx.resize(nidx);
+    for (int64_t k = 0; k < nidx; ++k) {
+        const int64_t v = cell_idx[k] - index_base;
+        if (v < 0 || v >= nnodes) return fp_fail(ctx, FP_ERR_ARG, "cell vertex index out of range");
+        b->cell_idx[k] = v;
+    }
+    b->ns = ns;
+    b->shedding.resize(6 * ns);
+    for (int64_t e = 0; e < ns; ++e) {
+        for (int k = 0; k < 6; ++k) {
+            const int64_t v = shedding[6 * e + k];
+            b->shedding[6 * e + k] = (k == 3 && v == -1) ? -1 : v - index_base;
+        }
+        const int64_t *s = &b->shedding[6 * e];
+        for (int side = 0; side < 2; ++side) {
+            const int64_t p = s[3 * side];
+            if (side == 1 && p == -1) continue;
+            if (p < 0 || p >= ncells) return fp_fail(ctx, FP_ERR_ARG, "shedding panel index out of range");
+            const int64_t nv = b->cell_ptr[p + 1] - b->cell_ptr[p];
+            for (int k = 1; k < 3; ++k)
+                if (s[3 * side + k] < 0 || s[3 * side + k] >= nv)
+                    return fp_fail(ctx, FP_ERR_ARG, "shedding vertex index out of range");
+        }
+    }
+    b->nvp = maxnv <= 4 ? 4 : 8;
+    const int nvp = b->nvp;
+    const int64_t N = ncells;
+    std::vector<double> pv((size_t)3 * nvp * N);
+    std::vector<int> nvv(N);
+    for (int64_t j = 0; j < N; ++j) {
+        const int64_t *pc = &b->cell_idx[b->cell_ptr[j]];
+        const int nv = (int)(b->cell_ptr[j + 1] - b->cell_ptr[j]);
+        nvv[j] = nv;
+        for (int k = 0; k < nvp; ++k) {
+            const int64_t v = pc[k < nv ? k : 0];
+            for (int d = 0; d < 3; ++d) pv[(size_t)(3 * k + d) * N + j] = b->nodes[3 * v + d];
+        }
+    }
+    FP_HIP(ctx, hipMalloc((void **)&b->d_pv, pv.size() * sizeof(double)));
+    FP_HIP(ctx, hipMalloc((void **)&b->d_nv, N * sizeof(int)));
+    FP_HIP(ctx, hipMalloc((void **)&b->d_hs, (size_t)FP_HS_STRIDE(nvp) * N * sizeof(double)));
+    FP_HIP(ctx, hipMalloc((void **)&b->d_geo, (size_t)FP_GEO_STRIDE * N * sizeof(double)));
+    FP_HIP(ctx, hipMalloc((void **)&b->d_uinf, (size_t)3 * N * sizeof(double)));
+    FP_HIP(ctx, hipMalloc((void **)&b->d_strength, N * sizeof(double)));
+    FP_HIP(ctx, hipMemcpy(b->d_pv, pv.data(), pv.size() * sizeof(double), hipMemcpyHostToDevice));
+    FP_HIP(ctx, hipMemcpy(b->d_nv, nvv.data(), N * sizeof(int), hipMemcpyHostToDevice));
+    FP_HIP(ctx, hipMemset(b->d_strength, 0, N * sizeof(double)));
+    int st = fp_launch_geometry(ctx, b.get());
+    if (st) return st;
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *out = b.release();
+    return FP_OK;
+}
+
+void fp_body_destroy(fp_body *b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    for (void *p : {(void *)b->d_pv, (void *)b->d_nv, (void *)b->d_hs, (void *)b->d_geo, (void *)b->d_uinf,
+                    (void *)b->d_hh, (void *)b->d_hcol, (void *)b->d_strength})
+        if (p) (void)hipFree(p);
+    delete b;
+}
+
+int64_t fp_body_ncells(const fp_body *b) { return b ? b->N : -1; }
+int64_t fp_body_nsheddings(const fp_body *b) { return b ? b->ns : -1; }
+
+int fp_body_geometry(fp_ctx *ctx, const fp_body *b, double *normals, double *cps, double *areas)
+{
+    if (!ctx || !b || b->ctx != ctx) return FP_ERR_ARG;
+    const int64_t N = b->N;
+    std::vector<double> g((size_t)FP_GEO_STRIDE * N);
+    FP_HIP(ctx, hipMemcpyAsync(g.data(), b->d_geo, g.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int64_t j = 0; j < N; ++j) {
+        for (int c = 0; c < 3; ++c) {
+            if (normals) normals[3 * j + c] = g[c * N + j];
+            if (cps) cps[3 * j + c] = g[(3 + c) * N + j];
+        }
+        if (areas) areas[j] = g[6 * N + j];
+    }
+    return FP_OK;
+}
+
+int fp_body_set_freestream(fp_ctx *ctx, fp_body *b, const double *Uinfs, const double *Das, const double *Dbs)
+{
+    if (!ctx || !b || b->ctx != ctx) return FP_ERR_ARG;
+    if (!Uinfs) return fp_fail(ctx, FP_ERR_SHAPE, "Invalid Uinfs; expected size (3, ncells)");
+    if (b->ns > 0 && (!Das || !Dbs)) return fp_fail(ctx, FP_ERR_SHAPE, "Invalid Das/Dbs; expected size (3, nsheddings)");
+    for (int64_t e = 0; e < b->ns; ++e)
+        if (!check_unit(Das + 3 * e) || !check_unit(Dbs + 3 * e))
+            return fp_fail(ctx, FP_ERR_NONUNIT_DIRECTION, "Found non-unitary semi-infinite direction");
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    int st = upload_soa3(ctx, Uinfs, b->N, b->d_uinf);
+    if (st) return st;
+    b->das.assign(Das ? Das : Uinfs, Das ? Das + 3 * b->ns : Uinfs);
+    b->dbs.assign(Dbs ? Dbs : Uinfs, Dbs ? Dbs + 3 * b->ns : Uinfs);
+    if ((st = build_horseshoes(ctx, b))) return st;
+    b->has_uinf = true;
+    b->has_wake = true;
+    return FP_OK;
+}
+
+int fp_body_set_strength(fp_ctx *ctx, fp_body *b, const double *s)
+{
+    if (!ctx || !b || b->ctx != ctx || !s) return FP_ERR_ARG;
+    if (b->ns > 0 && !b->has_wake) return fp_fail(ctx, FP_ERR_ARG, "set the wake directions first");
+    FP_HIP(ctx, hipMemcpy(b->d_strength, s, b->N * sizeof(double), hipMemcpyHostToDevice));
+    b->solved = true;
+    return FP_OK;
+}
+
+int fp_body_get_strength(fp_ctx *ctx, const fp_body *b, double *s)
+{
+    if (!ctx || !b || b->ctx != ctx || !s) return FP_ERR_ARG;
+    FP_HIP(ctx, hipMemcpyAsync(s, b->d_strength, b->N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FP_OK;
+}
+
+static int assemble_into(fp_ctx *ctx, fp_body *const *bodies, int nb, const TargetView &tv, int precision, void *A,
+                         int64_t ld)
+{
+    int64_t col = 0;
+    for (int k = 0; k < nb; ++k) {
+        const fp_body *b = bodies[k];
+        if (b->element == FP_ELEM_VORTEXRING && b->ns > 0 && !b->has_wake)
+            return fp_fail(ctx, FP_ERR_ARG, "wake directions not set (call fp_body_set_freestream)");
+        if (b->kernel_offset != bodies[0]->kernel_offset || b->kernel_cutoff != bodies[0]->kernel_cutoff) {
+            // each body assembles with its own scalars; nothing to check
+        }
+        int st = precision == FP_F64 ? fp_launch_assemble<double>(ctx, b, tv, (double *)A, ld, col)
+                                     : fp_launch_assemble<float>(ctx, b, tv, (float *)A, ld, col);
+        if (st) return st;
+        col += b->N;
+    }
+    return FP_OK;
+}
+
+int fp_solve_bodies(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_solve_opts *opts, fp_solve_stats *stats)
+{
+    int st = check_bodies(ctx, bodies, nb);
+    if (st) return st;
+    for (int k = 0; k < nb; ++k)
+        if (!bodies[k]->has_uinf)
+            return fp_fail(ctx, FP_ERR_SHAPE, "Invalid Uinfs; call fp_body_set_freestream before solve");
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    fp_solve_opts o;
+    if (opts) o = *opts;
+    else fp_solve_opts_default(&o, FP_SOLVER_LU);
+    fp_solve_stats s{};
+    Targets t;
+    if ((st = gather_targets(ctx, bodies, nb, t))) return st;
+    const int64_t n = t.M, ld = round_ld(n);
+    DBuf A, rhs, x;
+    if ((st = dalloc(ctx, rhs, n * sizeof(double)))) return st;
+    if ((st = dalloc(ctx, x, n * sizeof(double)))) return st;
+    int64_t col = 0;
+    for (int k = 0; k < nb; ++k) {
+        if ((st = fp_rhs_kernel(ctx, bodies[k], rhs.as<double>() + col))) return st;
+        col += bodies[k]->N;
+    }
+    std::vector<fp_body *> bl(bodies, bodies + nb);
+    MatrixFreeOp mf(ctx, bl, t.tv);
+    if ((st = mf.init())) return st;
+    SolveArgs a;
+    a.solver = o.solver;
+    a.n = n;
+    a.ld = ld;
+    a.resid = &mf;
+    Timer tm(ctx->stream);
+    const bool f64 = o.solver == FP_SOLVER_LU || o.solver == FP_SOLVER_GMRES;
+    if ((st = dalloc(ctx, A, (size_t)n * ld * (f64 ? sizeof(double) : sizeof(float))))) return st;
+    if ((st = assemble_into(ctx, bodies, nb, t.tv, f64 ? FP_F64 : FP_F32, A.p, ld))) return st;
+    s.ms_assemble = tm.stop();
+    if (f64) a.A64 = A.as<double>();
+    else a.A32 = A.as<float>();
+    if (o.solver == FP_SOLVER_GMRES) {
+        DenseOp<double> op(ctx, a.A64, n, ld);
+        a.resid = &op;
+        st = solve_core(ctx, a, o, rhs.as<double>(), x.as<double>(), s);
+    } else {
+        st = solve_core(ctx, a, o, rhs.as<double>(), x.as<double>(), s);
+    }
+    if (stats) *stats = s;
+    if (st) return st;
+    col = 0;
+    for (int k = 0; k < nb; ++k) {
+        FP_HIP(ctx, hipMemcpyAsync(bodies[k]->d_strength, x.as<double>() + col, bodies[k]->N * sizeof(double),
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+        bodies[k]->solved = true;
+        col += bodies[k]->N;
+    }
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FP_OK;
+}
+
+int fp_assemble(fp_ctx *ctx, fp_body *const *bodies, int nb, int precision, fp_mat **out)
+{
+    int st = check_bodies(ctx, bodies, nb);
+    if (st) return st;
+    if (!out) return fp_fail(ctx, FP_ERR_ARG, "null output");
+    if (precision != FP_F64 && precision != FP_F32) return fp_fail(ctx, FP_ERR_ARG, "unknown precision");
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    Targets t;
+    if ((st = gather_targets(ctx, bodies, nb, t))) return st;
+    std::unique_ptr<fp_mat> m(new fp_mat());
+    m->ctx = ctx;
+    m->n = t.M;
+    m->ld = round_ld(t.M);
+    m->precision = precision;
+    const size_t es = precision == FP_F64 ? sizeof(double) : sizeof(float);
+    FP_HIP(ctx, hipMalloc(&m->A, (size_t)m->n * m->ld * es));
+    if ((st = assemble_into(ctx, bodies, nb, t.tv, precision, m->A, m->ld))) { (void)hipFree(m->A); return st; }
+    m->bodies.assign(bodies, bodies + nb);
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *out = m.release();
+    return FP_OK;
+}
+
+int fp_mat_upload(fp_ctx *ctx, int64_t n, const double *Ah, fp_mat **out)
+{
+    if (!ctx || !out || !Ah || n <= 0) return FP_ERR_ARG;
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    std::unique_ptr<fp_mat> m(new fp_mat());
+    m->ctx = ctx;
+    m->n = n;
+    m->ld = round_ld(n);
+    m->precision = FP_F64;
+    DBuf tmp;
+    int st = dalloc(ctx, tmp, (size_t)n * n * sizeof(double));
+    if (st) return st;
+    FP_HIP(ctx, hipMemcpy(tmp.p, Ah, (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
+    FP_HIP(ctx, hipMalloc(&m->A, (size_t)n * m->ld * sizeof(double)));
+    if ((st = fp_mat_convert(ctx, tmp.as<double>(), n, n, m->A, m->ld, FP_F64, 1))) { (void)hipFree(m->A); return st; }
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *out = m.release();
+    return FP_OK;
+}
+
+int64_t fp_mat_size(const fp_mat *m) { return m ? m->n : -1; }
+
+int fp_mat_download(fp_ctx *ctx, const fp_mat *m, double *Ah)
+{
+    if (!ctx || !m || !Ah || m->ctx != ctx) return FP_ERR_ARG;
+    DBuf tmp;
+    int st = dalloc(ctx, tmp, (size_t)m->n * m->n * sizeof(double));
+    if (st) return st;
+    dim3 grid((unsigned)((m->n + 255) / 256), (unsigned)m->n);
+    if (m->precision == FP_F64)
+        hipLaunchKernelGGL(k_download_cm<double>, grid, dim3(256), 0, ctx->stream, (const double *)m->A, m->n, m->ld,
+                           tmp.as<double>());
+    else
+        hipLaunchKernelGGL(k_download_cm<float>, grid, dim3(256), 0, ctx->stream, (const float *)m->A, m->n, m->ld,
+                           tmp.as<double>());
+    FP_HIP(ctx, hipGetLastError());
+    FP_HIP(ctx, hipMemcpyAsync(Ah, tmp.p, (size_t)m->n * m->n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FP_OK;
+}
+
+void fp_mat_destroy(fp_mat *m)
+{
+    if (!m) return;
+    (void)hipSetDevice(m->ctx->device);
+    (void)hipStreamSynchronize(m->ctx->stream);
+    if (m->A) (void)hipFree(m->A);
+    delete m;
+}
+
+int fp_mat_getrf(fp_ctx *ctx, fp_mat *m, int precision, double *LU, int64_t *ipiv)
+{
+    if (!ctx || !m || m->ctx != ctx || !LU || !ipiv) return FP_ERR_ARG;
+    if (precision != FP_F64 && precision != FP_F32) return fp_fail(ctx, FP_ERR_ARG, "unknown precision");
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t n = m->n, ld = m->ld;
+    DBuf F, p, pm, src64, tmp;
+    int st;
+    const size_t es = precision == FP_F64 ? sizeof(double) : sizeof(float);
+    if ((st = dalloc(ctx, F, (size_t)n * ld * es))) return st;
+    if ((st = dalloc(ctx, p, n * sizeof(int)))) return st;
+    if ((st = dalloc(ctx, pm, n * sizeof(int)))) return st;
+    // copy/convert the matrix into F
+    if (m->precision == FP_F64 && precision == FP_F64) {
+        FP_HIP(ctx, hipMemcpyAsync(F.p, m->A, (size_t)n * ld * es, hipMemcpyDeviceToDevice, ctx->stream));
+    } else if (m->precision == FP_F64) {
+        if ((st = fp_mat_narrow(ctx, (const double *)m->A, F.as<float>(), n, ld))) return st;
+    } else if (precision == FP_F32) {
+        FP_HIP(ctx, hipMemcpyAsync(F.p, m->A, (size_t)n * ld * es, hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+        return fp_fail(ctx, FP_ERR_UNSUPPORTED, "FP64 factors of an FP32 matrix");
+    }
+    int info = 0;
+    st = precision == FP_F64 ? factor<double>(ctx, F.as<double>(), n, ld, p.as<int>(), pm.as<int>(), &info)
+                             : factor<float>(ctx, F.as<float>(), n, ld, p.as<int>(), pm.as<int>(), &info);
+    if (st && st != FP_ERR_SINGULAR) return st;
+    if ((st == FP_OK || st == FP_ERR_SINGULAR)) {
+        int st2 = dalloc(ctx, tmp, (size_t)n * n * sizeof(double));
+        if (st2) return st2;
+        dim3 grid((unsigned)((n + 255) / 256), (unsigned)n);
+        if (precision == FP_F64)
+            hipLaunchKernelGGL(k_download_cm<double>, grid, dim3(256), 0, ctx->stream, F.as<double>(), n, ld,
+                               tmp.as<double>());
+        else
+            hipLaunchKernelGGL(k_download_cm<float>, grid, dim3(256), 0, ctx->stream, F.as<float>(), n, ld,
+                               tmp.as<double>());
+        FP_HIP(ctx, hipMemcpyAsync(LU, tmp.p, (size_t)n * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        std::vector<int> ip(n);
+        FP_HIP(ctx, hipMemcpyAsync(ip.data(), p.p, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (int64_t i = 0; i < n; ++i) ipiv[i] = ip[i] + 1;
+    }
+    return st;
+}
+
+int fp_solve_mat(fp_ctx *ctx, fp_mat *m, const double *bh, int64_t k, double *xh, const fp_solve_opts *opts,
+                 fp_solve_stats *stats)
+{
+    if (!ctx || !m || m->ctx != ctx || !bh || !xh || k <= 0) return FP_ERR_ARG;
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    fp_solve_opts o;
+    if (opts) o = *opts;
+    else fp_solve_opts_default(&o, FP_SOLVER_LU);
+    const int64_t n = m->n, ld = m->ld;
+    int st;
+    DBuf b, x, W;
+    if ((st = dalloc(ctx, b, (size_t)n * k * sizeof(double)))) return st;
+    if ((st = dalloc(ctx, x, (size_t)n * k * sizeof(double)))) return st;
+    FP_HIP(ctx, hipMemcpy(b.p, bh, (size_t)n * k * sizeof(double), hipMemcpyHostToDevice));
+    // FP64 residual operator: stored FP64 matrix, else matrix-free from the bodies
+    std::unique_ptr<Operator> resid;
+    std::unique_ptr<MatrixFreeOp> mf;
+    Targets t;
+    if (m->precision == FP_F64) {
+        resid.reset(new DenseOp<double>(ctx, (const double *)m->A, n, ld));
+    } else {
+        if (m->bodies.empty()) return fp_fail(ctx, FP_ERR_UNSUPPORTED, "FP32 matrix without bodies");
+        if ((st = gather_targets(ctx, m->bodies.data(), (int)m->bodies.size(), t))) return st;
+        mf.reset(new MatrixFreeOp(ctx, m->bodies, t.tv));
+        if ((st = mf->init())) return st;
+    }
+    Operator *R = resid ? resid.get() : (Operator *)mf.get();
+    fp_solve_stats s{};
+    for (int64_t c = 0; c < k; ++c) {
+        SolveArgs a;
+        a.solver = o.solver;
+        a.n = n;
+        a.ld = ld;
+        a.resid = R;
+        fp_solve_stats sc{};
+        if (o.solver == FP_SOLVER_LU) {
+            if (m->precision != FP_F64) return fp_fail(ctx, FP_ERR_UNSUPPORTED, "FP64 LU of an FP32 matrix");
+            if ((st = dalloc(ctx, W, (size_t)n * ld * sizeof(double)))) return st;
+            FP_HIP(ctx, hipMemcpyAsync(W.p, m->A, (size_t)n * ld * sizeof(double), hipMemcpyDeviceToDevice,
+                                       ctx->stream));
+            a.A64 = W.as<double>();
+        } else if (o.solver == FP_SOLVER_GMRES) {
+            if (m->precision == FP_F64) a.A64 = (double *)m->A;
+            else a.A32 = (float *)m->A;
+        } else {
+            if ((st = dalloc(ctx, W, (size_t)n * ld * sizeof(float)))) return st;
+            if (m->precision == FP_F64) {
+                if ((st = fp_mat_narrow(ctx, (const double *)m->A, W.as<float>(), n, ld))) return st;
+            } else {
+                FP_HIP(ctx, hipMemcpyAsync(W.p, m->A, (size_t)n * ld * sizeof(float), hipMemcpyDeviceToDevice,
+                                           ctx->stream));
+            }
+            a.A32 = W.as<float>();
+        }
+        st = solve_core(ctx, a, o, b.as<double>() + c * n, x.as<double>() + c * n, sc);
+        s.ms_factor += sc.ms_factor;
+        s.ms_solve += sc.ms_solve;
+        s.iterations += sc.iterations;
+        s.refinements += sc.refinements;
+        s.residual = std::max(s.residual, sc.residual);
+        s.info = sc.info;
+        if (st) { if (stats) *stats = s; return st; }
+    }
+    FP_HIP(ctx, hipMemcpy(xh, x.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToHost));
+    if (stats) *stats = s;
+    return FP_OK;
+}
+
+static int field_eval(fp_ctx *ctx, fp_body *const *bodies, int nb, const double *targets, int64_t nt, double *out,
+                      int mode)
+{
+    int st = check_bodies(ctx, bodies, nb);
+    if (st) return st;
+    if (nt < 0 || (nt > 0 && (!targets || !out))) return fp_fail(ctx, FP_ERR_ARG, "null targets/out");
+    for (int k = 0; k < nb; ++k)
+        if (!bodies[k]->solved)
+            return fp_fail(ctx, FP_ERR_UNSOLVED, "Source body hasn't been solved yet. Please call `solve(...)` function first.");
+    if (nt == 0) return FP_OK;
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    const int comps = mode == 0 ? 3 : 1;
+    DBuf tg, o, part;
+    if ((st = dalloc(ctx, tg, 3 * nt * sizeof(double)))) return st;
+    if ((st = dalloc(ctx, o, comps * nt * sizeof(double)))) return st;
+    if ((st = upload_soa3(ctx, targets, nt, tg.as<double>()))) return st;
+    FP_HIP(ctx, hipMemsetAsync(o.p, 0, comps * nt * sizeof(double), ctx->stream));
+    TargetView tv{};
+    tv.x = tg.as<double>(); tv.y = tv.x + nt; tv.z = tv.x + 2 * nt; tv.M = nt;
+    size_t pb = 0;
+    for (int k = 0; k < nb; ++k) pb = std::max(pb, fp_field_partial_bytes(nt, bodies[k]->N));
+    if (pb && (st = dalloc(ctx, part, pb))) return st;
+    for (int k = 0; k < nb; ++k)
+        if ((st = fp_launch_field(ctx, bodies[k], bodies[k]->d_strength, tv, mode, o.as<double>(), part.as<double>(), pb)))
+            return st;
+    std::vector<double> h(comps * nt);
+    FP_HIP(ctx, hipMemcpyAsync(h.data(), o.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < h.size(); ++i) out[i] += h[i];
+    return FP_OK;
+}
+
+int fp_uind(fp_ctx *ctx, fp_body *const *bodies, int nb, const double *targets, int64_t nt, double *out)
+{
+    return field_eval(ctx, bodies, nb, targets, nt, out, 0);
+}
+
+int fp_phi(fp_ctx *ctx, fp_body *const *bodies, int nb, const double *targets, int64_t nt, double *out)
+{
+    return field_eval(ctx, bodies, nb, targets, nt, out, 2);
+}
+
+int fp_calcfield_U(fp_ctx *ctx, fp_body *const *bodies, int nb, double *out)
+{
+    int st = check_bodies(ctx, bodies, nb);
+    if (st) return st;
+    if (!out) return fp_fail(ctx, FP_ERR_ARG, "null out");
+    for (int k = 0; k < nb; ++k) {
+        if (!bodies[k]->solved)
+            return fp_fail(ctx, FP_ERR_UNSOLVED, "Source body hasn't been solved yet. Please call `solve(...)` function first.");
+        if (!bodies[k]->has_uinf) return fp_fail(ctx, FP_ERR_ARG, "Target body doesn't have freestream field `Uinf`");
+    }
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    Targets t;
+    if ((st = gather_targets(ctx, bodies, nb, t))) return st;
+    const int64_t M = t.M;
+    DBuf o, part;
+    if ((st = dalloc(ctx, o, 3 * M * sizeof(double)))) return st;
+    FP_HIP(ctx, hipMemsetAsync(o.p, 0, 3 * M * sizeof(double), ctx->stream));
+    int64_t off = 0;
+    for (int k = 0; k < nb; ++k) {
+        hipLaunchKernelGGL(k_aos_add_soa, dim3((unsigned)((bodies[k]->N + 255) / 256)), dim3(256), 0, ctx->stream,
+                           bodies[k]->d_uinf, bodies[k]->N, o.as<double>() + 3 * off);
+        off += bodies[k]->N;
+    }
+    size_t pb = 0;
+    for (int k = 0; k < nb; ++k) pb = std::max(pb, fp_field_partial_bytes(M, bodies[k]->N));
+    if (pb && (st = dalloc(ctx, part, pb))) return st;
+    for (int k = 0; k < nb; ++k)
+        if ((st = fp_launch_field(ctx, bodies[k], bodies[k]->d_strength, t.tv, 0, o.as<double>(), part.as<double>(), pb)))
+            return st;
+    std::vector<double> h(3 * M);
+    FP_HIP(ctx, hipMemcpyAsync(h.data(), o.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < h.size(); ++i) out[i] += h[i];
+    return FP_OK;
+}
+
+int fp_calcfield_cp(fp_ctx *ctx, const double *Us, int64_t n, double uref, const int64_t *shedding, int64_t ns,
+                    int index_base, int kutta_mode, double *out)
+{
+    if (!ctx || !Us || !out || n < 0 || ns < 0) return FP_ERR_ARG;
+    if (kutta_mode < FP_KUTTA_NONE || kutta_mode > FP_KUTTA_PAIR) return fp_fail(ctx, FP_ERR_ARG, "bad kutta mode");
+    if (ns > 0 && !shedding && kutta_mode != FP_KUTTA_NONE) return fp_fail(ctx, FP_ERR_ARG, "null shedding");
+    if (n == 0) return FP_OK;
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<int64_t> sh;
+    if (kutta_mode != FP_KUTTA_NONE) {
+        sh.resize(6 * ns);
+        for (int64_t e = 0; e < ns; ++e)
+            for (int k = 0; k < 6; ++k) {
+                const int64_t v = shedding[6 * e + k];
+                sh[6 * e + k] = (k == 3 && v == -1) ? -1 : v - index_base;
+            }
+        for (int64_t e = 0; e < ns; ++e) {
+            const int64_t pi = sh[6 * e], pj = sh[6 * e + 3];
+            const bool ref = kutta_mode == FP_KUTTA_REFERENCE;
+            if (pi < 0 || pi + (ref ? 1 : 0) >= n || (pj != -1 && (pj - (ref ? 1 : 0) < 0 || pj >= n)))
+                return fp_fail(ctx, FP_ERR_ARG, "Kutta panel index out of range");
+        }
+    }
+    DBuf U, o, S;
+    int st;
+    if ((st = dalloc(ctx, U, 3 * n * sizeof(double)))) return st;
+    if ((st = dalloc(ctx, o, n * sizeof(double)))) return st;
+    FP_HIP(ctx, hipMemcpyAsync(U.p, Us, 3 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    FP_HIP(ctx, hipMemcpyAsync(o.p, out, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_cp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, U.as<double>(), n, uref,
+                       o.as<double>());
+    if (kutta_mode != FP_KUTTA_NONE && ns > 0) {
+        if ((st = dalloc(ctx, S, sh.size() * sizeof(int64_t)))) return st;
+        FP_HIP(ctx, hipMemcpyAsync(S.p, sh.data(), sh.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_kutta, dim3(1), dim3(64), 0, ctx->stream, S.as<int64_t>(), ns, kutta_mode, o.as<double>());
+    }
+    FP_HIP(ctx, hipGetLastError());
+    FP_HIP(ctx, hipMemcpyAsync(out, o.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,974 @@
+// fp_dense.hip -- dense LU with partial (tournament) pivoting on MFMA, triangular
+// solves, GEMV and the block-Jacobi preconditioner (gfx950).
+//
+// Replaces LAPACK getrf/getrs behind `solve_ludiv!` (src/FLOWPanel_linearsolver.jl:57-78,
+// calc_Alu :173-234).  Matrices are row-major (A[i*ld + j]).
+//
+// getrf: right-looking blocked LU, outer blocks of NB = 256 columns (the trailing
+// update is one MFMA GEMM with k = 256), each outer block factored in sub-panels of
+// PW = 32 columns.  A sub-panel (m x 32) is factored with tournament pivoting
+// (TSLU, Grigori-Demmel-Xiang 2011): every workgroup runs partial pivoting on a
+// chunk of <= 256 rows in LDS and proposes 32 candidate rows; candidate sets are
+// merged 8 at a time by partial pivoting on their original values until 32
+// pivot rows remain.  This needs no grid-wide synchronisation: one sub-panel is
+// six short launches instead of one grid barrier per column.
+#include "fp_internal.h"
+
+#include <algorithm>
+#include <climits>
+
+typedef double fp_d4 __attribute__((ext_vector_type(4)));
+typedef float fp_f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct MfmaOp;
+template <> struct MfmaOp<double> {
+    typedef fp_d4 acc;
+    static __device__ __forceinline__ acc run(double a, double b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // accumulator element r of lane l holds D(l/16 + 4r, l%16)
+    static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct MfmaOp<float> {
+    typedef fp_f4 acc;
+    static __device__ __forceinline__ acc run(float a, float b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // accumulator element r of lane l holds D(4*(l/16) + r, l%16)
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+// ----------------------------------------------------------------------------
+// C[m x n] -= A[m x k] * B[k x n], all row-major.  128 x 128 tile per 256-thread
+// workgroup (2 x 2 waves of 64 x 64), K step 16 through LDS, v_mfma 16x16x4.
+// MFMA operand layout (16x16x4): lane l supplies A(l%16, l/16) and B(l/16, l%16);
+// the accumulator rows differ between f32 and f64 (MfmaOp<T>::row, measured on
+// gfx950 with an in-bounds lane-layout probe).
+constexpr int GB_M = 128, GB_N = 128, GB_K = 16, GB_PAD = 4;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t k, const T *__restrict__ A,
+                                                  int64_t lda, const T *__restrict__ B, int64_t ldb,
+                                                  T *__restrict__ C, int64_t ldc)
+{
+    typedef typename MfmaOp<T>::acc acc_t;
+    __shared__ T As[GB_K][GB_M + GB_PAD];
+    __shared__ T Bs[GB_K][GB_N + GB_PAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    // XCD-friendly order: consecutive workgroups (dispatched round-robin over
+    // the 8 XCDs) walk down a column of C tiles so tiles sharing a B panel
+    // land on different XCDs evenly; the grid is 1-D.
+    const int64_t tiles_m = (m + GB_M - 1) / GB_M;
+    const int64_t bid = blockIdx.x;
+    const int64_t tm = bid % tiles_m, tn = bid / tiles_m;
+    const int64_t m0 = tm * GB_M, n0 = tn * GB_N;
+
+    acc_t acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+
+    // global -> register staging: A: row tid/2, k (tid%2)*8..+8 ; B: k tid/16, cols (tid%16)*8..+8
+    const int arow = tid >> 1, ak = (tid & 1) * 8;
+    const int bk = tid >> 4, bc = (tid & 15) * 8;
+    T ra[8], rb[8];
+    auto load = [&](int64_t k0) {
+        const int64_t gr = m0 + arow;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int64_t gk = k0 + ak + e;
+            ra[e] = (gr < m && gk < k) ? A[gr * lda + gk] : T(0);
+        }
+        const int64_t gk = k0 + bk;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int64_t gc = n0 + bc + e;
+            rb[e] = (gk < k && gc < n) ? B[gk * ldb + gc] : T(0);
+        }
+    };
+    load(0);
+    for (int64_t k0 = 0; k0 < k; k0 += GB_K) {
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 8; ++e) As[ak + e][arow] = ra[e];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) Bs[bk][bc + e] = rb[e];
+        __syncthreads();
+        if (k0 + GB_K < k) load(k0 + GB_K);
+#pragma unroll
+        for (int ks = 0; ks < GB_K; ks += 4) {
+            T fa[4], fb[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) fa[a] = As[ks + (lane >> 4)][wm * 64 + a * 16 + (lane & 15)];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) fb[b] = Bs[ks + (lane >> 4)][wn * 64 + b * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = MfmaOp<T>::run(fa[a], fb[b], acc[a][b]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int64_t col = n0 + wn * 64 + b * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = m0 + wm * 64 + a * 16 + MfmaOp<T>::row(lane, r);
+                if (row < m && col < n) C[row * ldc + col] -= acc[a][b][r];
+            }
+        }
+}
+
+template <typename T>
+static int gemm_sub(fp_ctx *ctx, int64_t m, int64_t n, int64_t k, const T *A, int64_t lda, const T *B, int64_t ldb,
+                    T *C, int64_t ldc)
+{
+    if (m <= 0 || n <= 0 || k <= 0) return FP_OK;
+    const int64_t tiles = ((m + GB_M - 1) / GB_M) * ((n + GB_N - 1) / GB_N);
+    ProfScope ps(ctx, sizeof(T) == 8 ? "gemm_f64" : "gemm_f32");
+    hipLaunchKernelGGL(k_gemm_sub<T>, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, m, n, k, A, lda, B, ldb, C,
+                       ldc);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+// ----------------------------------------------------------------------------
+// X (nb x ncols, row-major ld) := L^-1 X with L (nb x nb, ld) unit lower, nb <= 256.
+// One workgroup per 32 columns; X in LDS; diagonal 32 x 32 blocks solved by one
+// lane per column in registers, the rows below updated by the whole workgroup.
+constexpr int TRSM_TC = 32;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm_llu(int nb, int64_t ncols, const T *__restrict__ L, int64_t ldl,
+                                                  T *__restrict__ X, int64_t ldx)
+{
+    __shared__ T Xs[256][TRSM_TC + 1];
+    const int tid = threadIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.x * TRSM_TC;
+    const int tc = (int)min((int64_t)TRSM_TC, ncols - c0);
+    for (int e = tid; e < nb * TRSM_TC; e += 256) {
+        const int r = e / TRSM_TC, c = e % TRSM_TC;
+        Xs[r][c] = c < tc ? X[(int64_t)r * ldx + c0 + c] : T(0);
+    }
+    __syncthreads();
+    for (int b = 0; b < nb; b += 32) {
+        const int bs = min(32, nb - b);
+        if (tid < TRSM_TC) {
+            T x[32];
+#pragma unroll
+            for (int r = 0; r < 32; ++r) x[r] = r < bs ? Xs[b + r][tid] : T(0);
+#pragma unroll
+            for (int r = 1; r < 32; ++r) {
+                if (r < bs) {
+                    T acc = x[r];
+                    const T *lr = L + (int64_t)(b + r) * ldl + b;
+#pragma unroll
+                    for (int q = 0; q < 32; ++q)
+                        if (q < r) acc -= lr[q] * x[q];
+                    x[r] = acc;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 32; ++r)
+                if (r < bs) Xs[b + r][tid] = x[r];
+        }
+        __syncthreads();
+        const int rows = nb - b - bs;
+        for (int e = tid; e < rows * TRSM_TC; e += 256) {
+            const int r = b + bs + e / TRSM_TC, c = e % TRSM_TC;
+            const T *lr = L + (int64_t)r * ldl + b;
+            T acc = Xs[r][c];
+            for (int q = 0; q < bs; ++q) acc -= lr[q] * Xs[b + q][c];
+            Xs[r][c] = acc;
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < nb * TRSM_TC; e += 256) {
+        const int r = e / TRSM_TC, c = e % TRSM_TC;
+        if (c < tc) X[(int64_t)r * ldx + c0 + c] = Xs[r][c];
+    }
+}
+
+template <typename T>
+static int trsm_llu(fp_ctx *ctx, int nb, int64_t ncols, const T *L, int64_t ldl, T *X, int64_t ldx)
+{
+    if (ncols <= 0 || nb <= 0) return FP_OK;
+    ProfScope ps(ctx, "trsm");
+    hipLaunchKernelGGL(k_trsm_llu<T>, dim3((unsigned)((ncols + TRSM_TC - 1) / TRSM_TC)), dim3(256), 0,
+                       ctx->stream, nb, ncols, L, ldl, X, ldx);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+// ----------------------------------------------------------------------------
+// Row interchanges: rows i in [k1, k2) swapped with ipiv[i] (sequentially), for
+// every column; one lane per column (row-major rows are contiguous).
+template <typename T>
+__global__ void k_laswp(T *__restrict__ A, int64_t ld, int64_t ncols, int64_t ld_rows, const int *__restrict__ ipiv,
+                        int k1, int k2)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    for (int i = k1; i < k2; ++i) {
+        const int p = ipiv[i];
+        if (p != i && p >= 0 && (int64_t)p < ld_rows) {
+            const T a = A[(int64_t)i * ld + c];
+            A[(int64_t)i * ld + c] = A[(int64_t)p * ld + c];
+            A[(int64_t)p * ld + c] = a;
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Tournament pivoting.
+constexpr int PW = 32;      // sub-panel width
+constexpr int CH = 256;     // rows per candidate chunk (and per merge)
+constexpr int MERGE_F = CH / PW;
+
+// Partial pivoting of a (rows x cols) tile in LDS, rows <= 256, cols <= PW.
+// key[] breaks ties (smaller wins, as LAPACK's first maximum); rows are
+// permuted in place together with key[] and tag[].  Returns (via *sing) 1 if
+// some pivot was exactly zero.
+template <typename T>
+__device__ void tile_gepp(T (*tile)[PW + 1], int *key, int *tag, int rows, int cols, int *sing)
+{
+    __shared__ double red_v[4];
+    __shared__ int red_k[4], red_r[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nc = min(rows, cols);
+    for (int c = 0; c < nc; ++c) {
+        double bv = -1.0;
+        int bk = INT_MAX, br = -1;
+        for (int r = c + tid; r < rows; r += blockDim.x) {
+            const double v = fabs((double)tile[r][c]);
+            if (v > bv || (v == bv && key[r] < bk)) { bv = v; bk = key[r]; br = r; }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ov = __shfl_xor(bv, o);
+            const int ok = __shfl_xor(bk, o), orr = __shfl_xor(br, o);
+            if (ov > bv || (ov == bv && ok < bk)) { bv = ov; bk = ok; br = orr; }
+        }
+        if (lane == 0) { red_v[wave] = bv; red_k[wave] = bk; red_r[wave] = br; }
+        __syncthreads();
+        bv = red_v[0]; bk = red_k[0]; br = red_r[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+            if (red_v[w] > bv || (red_v[w] == bv && red_k[w] < bk)) { bv = red_v[w]; bk = red_k[w]; br = red_r[w]; }
+        __syncthreads();
+        const int p = (br >= c && br < rows) ? br : c;  // all-NaN column: keep the row
+        if (p != c) {
+            for (int e = tid; e < cols; e += blockDim.x) {
+                const T t = tile[c][e]; tile[c][e] = tile[p][e]; tile[p][e] = t;
+            }
+            if (tid == 0) {
+                int t = key[c]; key[c] = key[p]; key[p] = t;
+                t = tag[c]; tag[c] = tag[p]; tag[p] = t;
+            }
+        }
+        __syncthreads();
+        const T piv = tile[c][c];
+        if (piv == T(0)) {
+            if (tid == 0 && *sing == 0) *sing = c + 1;
+        } else {
+            for (int r = c + 1 + tid; r < rows; r += blockDim.x) {
+                const T l = tile[r][c] / piv;
+                tile[r][c] = l;
+                for (int e = c + 1; e < cols; ++e) tile[r][e] -= l * tile[c][e];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Level 0: each workgroup proposes PW candidate rows out of CH rows of the
+// sub-panel A[s:n, s:s+pw].  Output: original values + global row index.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tslu_local(const T *__restrict__ A, int64_t ld, int n, int s, int pw,
+                                                    T *__restrict__ cval, int *__restrict__ cidx,
+                                                    int *__restrict__ info)
+{
+    __shared__ T tile[CH][PW + 1];
+    __shared__ int key[CH], tag[CH];
+    __shared__ int sing;
+    const int g = blockIdx.x, tid = threadIdx.x;
+    const int r0 = s + g * CH;
+    const int rows = min(CH, n - r0);
+    if (tid == 0) sing = 0;
+    for (int e = tid; e < CH * PW; e += blockDim.x) {
+        const int r = e / PW, c = e % PW;
+        tile[r][c] = (r < rows && c < pw) ? A[(int64_t)(r0 + r) * ld + s + c] : T(0);
+    }
+    for (int r = tid; r < CH; r += blockDim.x) { key[r] = r < rows ? r0 + r : INT_MAX; tag[r] = r; }
+    __syncthreads();
+    tile_gepp<T>(tile, key, tag, rows, pw, &sing);
+    // candidates: the first min(rows, pw) rows, original values re-read
+    for (int e = tid; e < PW * PW; e += blockDim.x) {
+        const int r = e / PW, c = e % PW;
+        T v = T(0);
+        if (r < rows && r < pw && c < pw) v = A[(int64_t)(r0 + tag[r]) * ld + s + c];
+        cval[((int64_t)g * PW + r) * PW + c] = v;
+    }
+    for (int r = tid; r < PW; r += blockDim.x) cidx[g * PW + r] = (r < rows && r < pw) ? key[r] : INT_MAX;
+    (void)info;
+}
+
+// Merge MERGE_F candidate sets per workgroup.  When `final_out` is set (one
+// workgroup left) it also writes the factored pw x pw block and the winners.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tslu_merge(int nsets, int pw, const T *__restrict__ ival,
+                                                    const int *__restrict__ iidx, T *__restrict__ oval,
+                                                    int *__restrict__ oidx, T *__restrict__ final_lu,
+                                                    int *__restrict__ info, int s)
+{
+    __shared__ T tile[CH][PW + 1];
+    __shared__ int key[CH], tag[CH];
+    __shared__ int sing;
+    const int g = blockIdx.x, tid = threadIdx.x;
+    const int set0 = g * MERGE_F;
+    const int nset = min(MERGE_F, nsets - set0);
+    const int rows = nset * PW;
+    if (tid == 0) sing = 0;
+    for (int e = tid; e < CH * PW; e += blockDim.x) {
+        const int r = e / PW, c = e % PW;
+        tile[r][c] = r < rows ? ival[((int64_t)set0 * PW + r) * PW + c] : T(0);
+    }
+    for (int r = tid; r < CH; r += blockDim.x) { key[r] = r < rows ? iidx[set0 * PW + r] : INT_MAX; tag[r] = r; }
+    __syncthreads();
+    tile_gepp<T>(tile, key, tag, rows, pw, &sing);
+    for (int e = tid; e < PW * PW; e += blockDim.x) {
+        const int r = e / PW, c = e % PW;
+        oval[((int64_t)g * PW + r) * PW + c] = ival[((int64_t)set0 * PW + tag[r]) * PW + c];
+        if (final_lu) final_lu[r * PW + c] = tile[r][c];
+    }
+    for (int r = tid; r < PW; r += blockDim.x) oidx[g * PW + r] = key[r];
+    if (final_lu && tid == 0 && sing && *info == 0) *info = s + sing;
+}
+
+// Winners (in pivot order) -> LAPACK swap sequence ipiv[s..s+pw) (one thread;
+// at most 2*pw rows move).
+__global__ void k_tslu_ipiv(const int *__restrict__ win, int s, int pw, int n, int *__restrict__ ipiv)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int mrow[2 * PW], mpos[2 * PW];  // rows that moved and where they are
+    int nm = 0;
+    auto loc = [&](int row) {
+        for (int q = 0; q < nm; ++q)
+            if (mrow[q] == row) return mpos[q];
+        return row;
+    };
+    auto set = [&](int row, int pos) {
+        for (int q = 0; q < nm; ++q)
+            if (mrow[q] == row) { mpos[q] = pos; return; }
+        mrow[nm] = row; mpos[nm] = pos; ++nm;
+    };
+    auto who = [&](int pos) {
+        for (int q = 0; q < nm; ++q)
+            if (mpos[q] == pos) return mrow[q];
+        // a position not in the table still holds its own row unless that row moved
+        for (int q = 0; q < nm; ++q)
+            if (mrow[q] == pos) return -1;
+        return pos;
+    };
+    for (int i = 0; i < pw; ++i) {
+        const int t = s + i;
+        const int w = win[i];
+        if (w < s || w >= n) { ipiv[t] = t; continue; }  // cannot happen for m >= pw; keep rows in place
+        const int p = loc(w);
+        ipiv[t] = p;
+        if (p != t) {
+            const int y = who(t);
+            set(w, t);
+            if (y >= 0) set(y, p);
+        }
+    }
+}
+
+// Rows [s, s+pw) of the sub-panel <- factored block; rows below <- A21 U11^-1.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tslu_l21(T *__restrict__ A, int64_t ld, int n, int s, int pw,
+                                                  const T *__restrict__ lu)
+{
+    __shared__ T U[PW][PW + 1];
+    for (int e = threadIdx.x; e < PW * PW; e += blockDim.x) U[e / PW][e % PW] = lu[e];
+    __syncthreads();
+    const int64_t r = s + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    T *row = A + r * ld + s;
+    if (r < s + pw) {
+        const int i = (int)(r - s);
+        for (int c = 0; c < pw; ++c) row[c] = U[i][c];
+        return;
+    }
+    T x[PW];
+#pragma unroll
+    for (int c = 0; c < PW; ++c) x[c] = c < pw ? row[c] : T(0);
+#pragma unroll
+    for (int c = 0; c < PW; ++c) {
+        if (c < pw) {
+            T acc = x[c];
+#pragma unroll
+            for (int q = 0; q < PW; ++q)
+                if (q < c) acc -= x[q] * U[q][c];
+            x[c] = U[c][c] != T(0) ? acc / U[c][c] : acc;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < PW; ++c)
+        if (c < pw) row[c] = x[c];
+}
+
+__global__ void k_fill_i32(int *p, int64_t n, int v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+size_t fp_getrf_work_bytes(int64_t n, int elem_bytes)
+{
+    const int64_t g = (n + CH - 1) / CH;
+    // two candidate buffers (values + indices) + final block
+    return (size_t)(2 * g * PW * PW) * elem_bytes + (size_t)(2 * g * PW) * sizeof(int) + PW * PW * elem_bytes + 256;
+}
+
+template <typename T>
+int fp_getrf(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *ipiv, int *info_dev, void *work, size_t work_bytes)
+{
+    if (n <= 0) return FP_OK;
+    if (work_bytes < fp_getrf_work_bytes(n, sizeof(T))) return fp_fail(ctx, FP_ERR_ARG, "getrf workspace too small");
+    if (n > INT_MAX / 2) return fp_fail(ctx, FP_ERR_UNSUPPORTED, "matrix too large for 32-bit pivots");
+    const int64_t gmax = (n + CH - 1) / CH;
+    char *w = (char *)work;
+    T *cv0 = (T *)w; w += gmax * PW * PW * sizeof(T);
+    T *cv1 = (T *)w; w += gmax * PW * PW * sizeof(T);
+    int *ci0 = (int *)w; w += gmax * PW * sizeof(int);
+    int *ci1 = (int *)w; w += gmax * PW * sizeof(int);
+    T *flu = (T *)w;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, info_dev, (int64_t)1, 0);
+    const int NB = 256;
+    for (int64_t k = 0; k < n; k += NB) {
+        const int nb = (int)std::min<int64_t>(NB, n - k);
+        for (int64_t s = k; s < k + nb; s += PW) {
+            const int pw = (int)std::min<int64_t>(PW, k + nb - s);
+            const int m = (int)(n - s);
+            {
+            int g = (m + CH - 1) / CH;
+            ProfScope ps(ctx, "panel_tslu");
+            hipLaunchKernelGGL(k_tslu_local<T>, dim3(g), dim3(256), 0, ctx->stream, (const T *)A, ld, (int)n, (int)s,
+                               pw, cv0, ci0, info_dev);
+            T *iv = cv0, *ov = cv1;
+            int *ii = ci0, *oi = ci1;
+            for (;;) {
+                const int go = (g + MERGE_F - 1) / MERGE_F;
+                const bool last = go == 1;
+                hipLaunchKernelGGL(k_tslu_merge<T>, dim3(go), dim3(256), 0, ctx->stream, g, pw, (const T *)iv,
+                                   (const int *)ii, ov, oi, last ? flu : (T *)nullptr, info_dev, (int)s);
+                std::swap(iv, ov);
+                std::swap(ii, oi);
+                g = go;
+                if (last) break;
+            }
+            // ii now holds the pw winners in pivot order
+            hipLaunchKernelGGL(k_tslu_ipiv, dim3(1), dim3(64), 0, ctx->stream, (const int *)ii, (int)s, pw, (int)n, ipiv);
+            hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, A, ld, n, n,
+                               (const int *)ipiv, (int)s, (int)(s + pw));
+            hipLaunchKernelGGL(k_tslu_l21<T>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, ctx->stream, A, ld,
+                               (int)n, (int)s, pw, (const T *)flu);
+            FP_HIP(ctx, hipGetLastError());
+            }
+            // update the rest of the outer block
+            const int64_t rest = k + nb - (s + pw);
+            if (rest > 0) {
+                int st = trsm_llu<T>(ctx, pw, rest, A + s * ld + s, ld, A + s * ld + s + pw, ld);
+                if (st) return st;
+                st = gemm_sub<T>(ctx, n - s - pw, rest, pw, A + (s + pw) * ld + s, ld, A + s * ld + s + pw, ld,
+                                 A + (s + pw) * ld + s + pw, ld);
+                if (st) return st;
+            }
+        }
+        const int64_t right = n - k - nb;
+        if (right > 0) {
+            int st = trsm_llu<T>(ctx, nb, right, A + k * ld + k, ld, A + k * ld + k + nb, ld);
+            if (st) return st;
+            st = gemm_sub<T>(ctx, right, right, nb, A + (k + nb) * ld + k, ld, A + k * ld + k + nb, ld,
+                             A + (k + nb) * ld + k + nb, ld);
+            if (st) return st;
+        }
+    }
+    return FP_OK;
+}
+template int fp_getrf<double>(fp_ctx *, double *, int64_t, int64_t, int *, int *, void *, size_t);
+template int fp_getrf<float>(fp_ctx *, float *, int64_t, int64_t, int *, int *, void *, size_t);
+
+// ----------------------------------------------------------------------------
+// Triangular solves with the LU factors, FP64 right-hand sides (n x k,
+// column-major with ldx, i.e. each RHS contiguous).  Blocked by TB rows: one
+// workgroup solves the TB x TB diagonal block (one wave walks the columns,
+// the block in LDS), then every row below/above is updated by a GEMV.
+constexpr int TB = 128;
+
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsv_diag(const T *__restrict__ LU, int64_t ld, int64_t b0, int bs,
+                                                   double *__restrict__ x, int64_t ldx)
+{
+    __shared__ double Ls[TB][TB + 1];
+    double *xv = x + (int64_t)blockIdx.x * ldx + b0;
+    for (int e = threadIdx.x; e < bs * bs; e += blockDim.x) {
+        const int r = e / bs, c = e % bs;
+        Ls[r][c] = (double)LU[(b0 + r) * ld + b0 + c];
+    }
+    __syncthreads();
+    if (threadIdx.x >= 64) return;
+    const int lane = threadIdx.x;
+    double y0 = lane < bs ? xv[lane] : 0.0, y1 = lane + 64 < bs ? xv[lane + 64] : 0.0;
+    if (LOWER) {
+        for (int q = 0; q < bs; ++q) {
+            const double yq = __shfl(q < 64 ? y0 : y1, q & 63);
+            if (lane > q && lane < bs) y0 -= Ls[lane][q] * yq;
+            if (lane + 64 > q && lane + 64 < bs) y1 -= Ls[lane + 64][q] * yq;
+        }
+    } else {
+        for (int q = bs - 1; q >= 0; --q) {
+            // finalise y_q, then eliminate it from the rows above
+            if (lane == (q & 63)) {
+                if (q < 64) y0 /= Ls[q][q];
+                else y1 /= Ls[q][q];
+            }
+            const double yq = __shfl(q < 64 ? y0 : y1, q & 63);
+            if (lane < q) y0 -= Ls[lane][q] * yq;
+            if (lane + 64 < q) y1 -= Ls[lane + 64][q] * yq;
+        }
+    }
+    if (lane < bs) xv[lane] = y0;
+    if (lane + 64 < bs) xv[lane + 64] = y1;
+}
+
+// x[rows] -= LU[rows, b0:b0+bs] * x[b0:b0+bs]; one wave per row.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv_update(const T *__restrict__ LU, int64_t ld, int64_t r0, int64_t r1,
+                                                     int64_t b0, int bs, double *__restrict__ x, int64_t ldx)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t r = r0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    double *xv = x + (int64_t)blockIdx.y * ldx;
+    if (r >= r1) return;
+    double acc = 0;
+    for (int c = lane; c < bs; c += 64) acc += (double)LU[r * ld + b0 + c] * xv[b0 + c];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) xv[r] -= acc;
+}
+
+__global__ void k_permute(const int *__restrict__ perm, int64_t n, const double *__restrict__ src, double *__restrict__ dst,
+                          int64_t ld)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dst[(int64_t)blockIdx.y * ld + i] = src[(int64_t)blockIdx.y * ld + perm[i]];
+}
+
+size_t fp_getrs_work_bytes(int64_t n, int64_t k) { return (size_t)n * k * sizeof(double) + 256; }
+
+// `ipiv` here is the row permutation perm (row i of P*A is row perm[i] of A).
+template <typename T>
+int fp_getrs(fp_ctx *ctx, const T *LU, int64_t n, int64_t ld, const int *perm, double *x, int64_t k, int64_t ldx,
+             void *work, size_t work_bytes)
+{
+    if (n <= 0 || k <= 0) return FP_OK;
+    if (work_bytes < fp_getrs_work_bytes(n, k)) return fp_fail(ctx, FP_ERR_ARG, "getrs workspace too small");
+    ProfScope ps(ctx, "getrs");
+    double *tmp = (double *)work;
+    hipLaunchKernelGGL(k_permute, dim3((unsigned)((n + 255) / 256), (unsigned)k), dim3(256), 0, ctx->stream, perm, n,
+                       (const double *)x, tmp, n);
+    FP_HIP(ctx, hipMemcpy2DAsync(x, ldx * sizeof(double), tmp, n * sizeof(double), n * sizeof(double), k,
+                                 hipMemcpyDeviceToDevice, ctx->stream));
+    for (int64_t b0 = 0; b0 < n; b0 += TB) {
+        const int bs = (int)std::min<int64_t>(TB, n - b0);
+        hipLaunchKernelGGL((k_trsv_diag<T, true>), dim3((unsigned)k), dim3(256), 0, ctx->stream, LU, ld, b0, bs, x, ldx);
+        const int64_t r0 = b0 + bs;
+        if (r0 < n)
+            hipLaunchKernelGGL(k_trsv_update<T>, dim3((unsigned)((n - r0 + 3) / 4), (unsigned)k), dim3(256), 0,
+                               ctx->stream, LU, ld, r0, n, b0, bs, x, ldx);
+    }
+    const int64_t nblk = (n + TB - 1) / TB;
+    for (int64_t bi = nblk - 1; bi >= 0; --bi) {
+        const int64_t b0 = bi * TB;
+        const int bs = (int)std::min<int64_t>(TB, n - b0);
+        hipLaunchKernelGGL((k_trsv_diag<T, false>), dim3((unsigned)k), dim3(256), 0, ctx->stream, LU, ld, b0, bs, x,
+                           ldx);
+        if (b0 > 0)
+            hipLaunchKernelGGL(k_trsv_update<T>, dim3((unsigned)((b0 + 3) / 4), (unsigned)k), dim3(256), 0,
+                               ctx->stream, LU, ld, (int64_t)0, b0, b0, bs, x, ldx);
+    }
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+template int fp_getrs<double>(fp_ctx *, const double *, int64_t, int64_t, const int *, double *, int64_t, int64_t,
+                              void *, size_t);
+template int fp_getrs<float>(fp_ctx *, const float *, int64_t, int64_t, const int *, double *, int64_t, int64_t,
+                             void *, size_t);
+
+// ----------------------------------------------------------------------------
+// y = alpha A x + beta y (row-major A, FP64 vectors): one wave per row,
+// 4-wide vector loads when the row is 16-byte aligned.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv(const T *__restrict__ A, int64_t n, int64_t ld,
+                                              const double *__restrict__ x, double *__restrict__ y, double alpha,
+                                              double beta)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    const T *row = A + r * ld;
+    double acc = 0;
+    const int64_t n4 = (n / 4) * 4;
+    for (int64_t c = 4 * lane; c < n4; c += 256) {
+        acc += (double)row[c] * x[c] + (double)row[c + 1] * x[c + 1] + (double)row[c + 2] * x[c + 2] +
+               (double)row[c + 3] * x[c + 3];
+    }
+    for (int64_t c = n4 + lane; c < n; c += 64) acc += (double)row[c] * x[c];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) y[r] = alpha * acc + (beta == 0.0 ? 0.0 : beta * y[r]);
+}
+
+template <typename T>
+int fp_gemv(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, const double *x, double *y, double alpha, double beta)
+{
+    if (n <= 0) return FP_OK;
+    ProfScope ps(ctx, sizeof(T) == 8 ? "gemv_f64" : "gemv_f32");
+    hipLaunchKernelGGL(k_gemv<T>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, A, n, ld, x, y, alpha,
+                       beta);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+template int fp_gemv<double>(fp_ctx *, const double *, int64_t, int64_t, const double *, double *, double, double);
+template int fp_gemv<float>(fp_ctx *, const float *, int64_t, int64_t, const double *, double *, double, double);
+
+// ----------------------------------------------------------------------------
+// Block-Jacobi: LU with partial pivoting of each diagonal block (one
+// workgroup per block, the block copied to BJ in FP32 and factored there).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_bj_factor(const T *__restrict__ A, int64_t n, int64_t ld, int block,
+                                                    float *__restrict__ BJ, int *__restrict__ piv,
+                                                    int *__restrict__ info)
+{
+    __shared__ float red_v[16];
+    __shared__ int red_r[16];
+    __shared__ float prow[1024];
+    const int b = blockIdx.x;
+    const int64_t b0 = (int64_t)b * block;
+    const int bs = (int)min((int64_t)block, n - b0);
+    float *M = BJ + (int64_t)b * block * block;
+    int *pv = piv + (int64_t)b * block;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+    for (int64_t e = tid; e < (int64_t)bs * bs; e += blockDim.x) {
+        const int r = (int)(e / bs), c = (int)(e % bs);
+        M[(int64_t)r * block + c] = (float)A[(b0 + r) * ld + b0 + c];
+    }
+    __syncthreads();
+    for (int c = 0; c < bs; ++c) {
+        float bv = -1.f;
+        int br = -1;
+        for (int r = c + tid; r < bs; r += blockDim.x) {
+            const float v = fabsf(M[(int64_t)r * block + c]);
+            if (v > bv) { bv = v; br = r; }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const float ov = __shfl_xor(bv, o);
+            const int orr = __shfl_xor(br, o);
+            if (ov > bv || (ov == bv && orr < br && orr >= 0)) { bv = ov; br = orr; }
+        }
+        if (lane == 0) { red_v[wave] = bv; red_r[wave] = br; }
+        __syncthreads();
+        bv = red_v[0]; br = red_r[0];
+        for (int w = 1; w < nw; ++w)
+            if (red_v[w] > bv || (red_v[w] == bv && red_r[w] < br && red_r[w] >= 0)) { bv = red_v[w]; br = red_r[w]; }
+        if (br < c || br >= bs) br = c;  // all-NaN column: keep the row
+        __syncthreads();
+        if (tid == 0) pv[c] = br;
+        if (br != c)
+            for (int e = tid; e < bs; e += blockDim.x) {
+                const float t = M[(int64_t)c * block + e];
+                M[(int64_t)c * block + e] = M[(int64_t)br * block + e];
+                M[(int64_t)br * block + e] = t;
+            }
+        __syncthreads();
+        for (int e = tid; e < bs; e += blockDim.x) prow[e] = M[(int64_t)c * block + e];
+        __syncthreads();
+        const float p = prow[c];
+        if (p == 0.f) {
+            if (tid == 0) atomicCAS(info, 0, b * block + c + 1);
+        } else {
+            const int w = bs - c - 1;
+            for (int64_t e = tid; e < (int64_t)w * (w + 1); e += blockDim.x) {
+                // (w rows) x (w + 1 columns: multiplier column c then c+1..)
+                const int r = c + 1 + (int)(e / (w + 1)), cc = c + (int)(e % (w + 1));
+                if (cc == c) continue;
+                const float l = M[(int64_t)r * block + c] / p;
+                M[(int64_t)r * block + cc] -= l * prow[cc];
+            }
+            __syncthreads();
+            for (int r = c + 1 + tid; r < bs; r += blockDim.x) M[(int64_t)r * block + c] /= p;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+int fp_bj_factor(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *BJ, int *piv, int *info_dev)
+{
+    if (block > 1024) return fp_fail(ctx, FP_ERR_ARG, "block-Jacobi block larger than 1024");
+    const int nb = (int)((n + block - 1) / block);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, info_dev, (int64_t)1, 0);
+    ProfScope ps(ctx, "bj_factor");
+    hipLaunchKernelGGL(k_bj_factor<T>, dim3(nb), dim3(1024), 0, ctx->stream, A, n, ld, block, BJ, piv, info_dev);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+template int fp_bj_factor<double>(fp_ctx *, const double *, int64_t, int64_t, int, float *, int *, int *);
+template int fp_bj_factor<float>(fp_ctx *, const float *, int64_t, int64_t, int, float *, int *, int *);
+
+// y = M^-1 x with the block LU factors: one workgroup (4 waves) per block.
+// Rows are processed in 64-row slabs: the waves first subtract the already
+// solved part (row dot products), then one wave solves the 64 x 64 diagonal
+// triangle with its rows preloaded into registers.
+__global__ __launch_bounds__(256) void k_bj_apply(const float *__restrict__ BJ, const int *__restrict__ piv,
+                                                  int64_t n, int block, const double *__restrict__ x,
+                                                  double *__restrict__ y)
+{
+    __shared__ double v[1024];
+    const int b = blockIdx.x;
+    const int64_t b0 = (int64_t)b * block;
+    const int bs = (int)min((int64_t)block, n - b0);
+    const float *M = BJ + (int64_t)b * block * block;
+    const int *pv = piv + (int64_t)b * block;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int r = tid; r < bs; r += blockDim.x) v[r] = x[b0 + r];
+    __syncthreads();
+    if (tid == 0)
+        for (int c = 0; c < bs; ++c) {
+            const int p = pv[c];
+            if (p != c) { const double t = v[c]; v[c] = v[p]; v[p] = t; }
+        }
+    __syncthreads();
+    // forward: unit lower
+    for (int s0 = 0; s0 < bs; s0 += 64) {
+        const int sl = min(64, bs - s0);
+        for (int rr = wave; rr < sl; rr += 4) {
+            const int r = s0 + rr;
+            double acc = 0;
+            for (int q = lane; q < s0; q += 64) acc += (double)M[(int64_t)r * block + q] * v[q];
+            for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+            if (lane == 0) v[r] -= acc;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            float lrow[64];
+            const int r = s0 + lane;
+#pragma unroll
+            for (int q = 0; q < 64; ++q) lrow[q] = (lane < sl && q < sl) ? M[(int64_t)r * block + s0 + q] : 0.f;
+            double yv = lane < sl ? v[r] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 64; ++q) {
+                const double yq = __shfl(yv, q);
+                if (lane > q) yv -= (double)lrow[q] * yq;
+            }
+            if (lane < sl) v[r] = yv;
+        }
+        __syncthreads();
+    }
+    // backward: upper with diagonal
+    const int nsl = (bs + 63) / 64;
+    for (int si = nsl - 1; si >= 0; --si) {
+        const int s0 = si * 64;
+        const int sl = min(64, bs - s0);
+        for (int rr = wave; rr < sl; rr += 4) {
+            const int r = s0 + rr;
+            double acc = 0;
+            for (int q = s0 + 64 + lane; q < bs; q += 64) acc += (double)M[(int64_t)r * block + q] * v[q];
+            for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+            if (lane == 0) v[r] -= acc;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            float urow[64];
+            const int r = s0 + lane;
+#pragma unroll
+            for (int q = 0; q < 64; ++q) urow[q] = (lane < sl && q < sl) ? M[(int64_t)r * block + s0 + q] : 0.f;
+            double yv = lane < sl ? v[r] : 0.0;
+#pragma unroll
+            for (int q = 63; q >= 0; --q) {
+                if (lane == q && q < sl) yv /= (double)urow[q];
+                const double yq = __shfl(yv, q);
+                if (lane < q && q < sl) yv -= (double)urow[q] * yq;
+            }
+            if (lane < sl) v[r] = yv;
+        }
+        __syncthreads();
+    }
+    for (int r = tid; r < bs; r += blockDim.x) y[b0 + r] = v[r];
+}
+
+int fp_bj_apply(fp_ctx *ctx, const float *BJ, const int *piv, int64_t n, int block, const double *x, double *y)
+{
+    const int nb = (int)((n + block - 1) / block);
+    ProfScope ps(ctx, "bj_apply");
+    hipLaunchKernelGGL(k_bj_apply, dim3(nb), dim3(256), 0, ctx->stream, BJ, piv, n, block, x, y);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+// ----------------------------------------------------------------------------
+// conversions and BLAS-1
+__global__ void k_convert(const double *__restrict__ src, int64_t n, int64_t lds, void *__restrict__ dst, int64_t ldd,
+                          int prec, int transpose)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = blockIdx.y;
+    if (j >= n) return;
+    // transpose: src column-major (element (i,j) at j*lds + i), dst row-major
+    const double v = transpose ? src[j * lds + i] : src[i * lds + j];
+    if (prec == FP_F64) ((double *)dst)[i * ldd + j] = v;
+    else ((float *)dst)[i * ldd + j] = (float)v;
+}
+
+int fp_mat_convert(fp_ctx *ctx, const double *src, int64_t n, int64_t lds, void *dst, int64_t ldd, int dst_prec,
+                   int transpose)
+{
+    if (n <= 0) return FP_OK;
+    hipLaunchKernelGGL(k_convert, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, ctx->stream, src, n,
+                       lds, dst, ldd, dst_prec, transpose);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+__global__ void k_narrow(const double *__restrict__ src, float *__restrict__ dst, int64_t n, int64_t ld)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = blockIdx.y;
+    if (j < n) dst[i * ld + j] = (float)src[i * ld + j];
+}
+
+int fp_mat_narrow(fp_ctx *ctx, const double *src, float *dst, int64_t n, int64_t ld)
+{
+    if (n <= 0) return FP_OK;
+    hipLaunchKernelGGL(k_narrow, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, ctx->stream, src, dst,
+                       n, ld);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+__global__ void k_resid(const double *__restrict__ b, double *__restrict__ r, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = b[i] - r[i];
+}
+
+int fp_residual_update(fp_ctx *ctx, const double *b, double *r, int64_t n)
+{
+    if (n <= 0) return FP_OK;
+    hipLaunchKernelGGL(k_resid, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, b, r, n);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+// multi-dot: out[c] = sum_i V[c*ldv + i] * w[i], c < nv; two passes (partials
+// per workgroup in a fixed order, then a single-workgroup sum) so the result is
+// deterministic.
+__global__ __launch_bounds__(256) void k_mdot_partial(const double *__restrict__ V, int64_t ldv, int nv,
+                                                      const double *__restrict__ w, int64_t n,
+                                                      double *__restrict__ part)
+{
+    __shared__ double red[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+    for (int c = 0; c < nv; ++c) {
+        double acc = 0;
+        for (int64_t i = i0 + tid; i < i1; i += 256) acc += V[c * ldv + i] * w[i];
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+        if (lane == 0) red[wave] = acc;
+        __syncthreads();
+        if (tid == 0) part[(int64_t)c * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mdot_final(const double *__restrict__ part, int nparts, int nv,
+                                                    double *__restrict__ out)
+{
+    __shared__ double red[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int c = 0; c < nv; ++c) {
+        double acc = 0;
+        for (int k = tid; k < nparts; k += 256) acc += part[(int64_t)c * nparts + k];
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+        if (lane == 0) red[wave] = acc;
+        __syncthreads();
+        if (tid == 0) out[c] = red[0] + red[1] + red[2] + red[3];
+        __syncthreads();
+    }
+}
+
+int fp_mdot(fp_ctx *ctx, const double *V, int64_t ldv, int nv, const double *w, int64_t n, double *part,
+            double *out)
+{
+    const int parts = 256;
+    hipLaunchKernelGGL(k_mdot_partial, dim3(parts), dim3(256), 0, ctx->stream, V, ldv, nv, w, n, part);
+    hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(256), 0, ctx->stream, (const double *)part, parts, nv, out);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+// w -= V h (h on the device, nv entries)
+__global__ void k_vupdate(const double *__restrict__ V, int64_t ldv, int nv, const double *__restrict__ h,
+                          double *__restrict__ w, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = w[i];
+    for (int c = 0; c < nv; ++c) acc -= V[c * ldv + i] * h[c];
+    w[i] = acc;
+}
+
+int fp_vupdate(fp_ctx *ctx, const double *V, int64_t ldv, int nv, const double *h, double *w, int64_t n)
+{
+    hipLaunchKernelGGL(k_vupdate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, V, ldv, nv, h, w, n);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+// y = a*x + b*y
+__global__ void k_axpby(double a, const double *__restrict__ x, double b, double *__restrict__ y, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a * x[i] + (b == 0.0 ? 0.0 : b * y[i]);
+}
+
+int fp_axpby(fp_ctx *ctx, double a, const double *x, double b, double *y, int64_t n)
+{
+    hipLaunchKernelGGL(k_axpby, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, a, x, b, y, n);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+// x += V z (z on the device, nv entries): Krylov solution update
+__global__ void k_vcombine(const double *__restrict__ V, int64_t ldv, int nv, const double *__restrict__ z,
+                           double *__restrict__ x, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0;
+    for (int c = 0; c < nv; ++c) acc += V[c * ldv + i] * z[c];
+    x[i] = acc;
+}
+
+int fp_vcombine(fp_ctx *ctx, const double *V, int64_t ldv, int nv, const double *z, double *x, int64_t n)
+{
+    hipLaunchKernelGGL(k_vcombine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, V, ldv, nv, z, x, n);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}

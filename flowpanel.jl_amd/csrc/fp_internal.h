@@ -1,0 +1,195 @@
+// fp_internal.h -- shared declarations of the gfx950 panel-method library.
+//
+// Device data layout (all structure-of-arrays, FP64 master copies):
+//   panel vertex table  pv[(3*k + d) * N + j]   vertex k (0..NVP-1), coordinate d, panel j.
+//                       Panels with fewer than NVP vertices repeat their first
+//                       vertex; that closing edge has zero length and adds
+//                       exactly nothing to a vortex ring.
+//   panel frame table   hs[c * N + j]: Hess-Smith frame ex(3) ey(3) ez(3) O(3),
+//                       local vertex coords (3*NVP), edge d/S/C (3*NVP).
+//   geometry            geo[c * N + j]: nx ny nz cx cy cz area.
+//   horseshoes          hh[c * nh + h]: pa(3) pb(3) da(3) db(3), hcol[h] = panel.
+//   dense matrices      row-major, A[i * ld + j] = G(i, j) (row i = control point,
+//                       column j = panel): row interchanges of the LU are then
+//                       contiguous and the AIC store of a wavefront (64 panels of
+//                       one target row) is one coalesced segment.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/flowpanel_hip.h"
+
+#define FP_PI 3.14159265358979323846
+#define FP_MAX_NV 8
+#define FP_HS_STRIDE(nvp) (12 + 6 * (nvp))
+#define FP_GEO_STRIDE 7
+
+struct fp_prof_rec {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+struct fp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int num_cu = 256;
+    // cached scratch (grown on demand, freed with the context)
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // per-kernel device timing (fp_ctx_profile)
+    bool prof = false;
+    std::vector<fp_prof_rec> prof_pending;
+    std::map<std::string, std::pair<int64_t, double>> prof_acc;
+};
+
+// Brackets the launches of one kernel with events on the context stream when
+// profiling is on (fp_ctx_profile); the elapsed times are folded in on query.
+struct ProfScope {
+    fp_ctx *c;
+    int idx = -1;
+    ProfScope(fp_ctx *ctx, const char *name) : c(ctx)
+    {
+        if (!c->prof) return;
+        fp_prof_rec r;
+        r.name = name;
+        if (hipEventCreate(&r.a) != hipSuccess) return;
+        if (hipEventCreate(&r.b) != hipSuccess) { (void)hipEventDestroy(r.a); return; }
+        (void)hipEventRecord(r.a, c->stream);
+        c->prof_pending.push_back(r);
+        idx = (int)c->prof_pending.size() - 1;
+    }
+    ~ProfScope()
+    {
+        if (idx >= 0) (void)hipEventRecord(c->prof_pending[idx].b, c->stream);
+    }
+};
+
+struct fp_body {
+    fp_ctx *ctx = nullptr;
+    int element = FP_ELEM_VORTEXRING;
+    int64_t N = 0;
+    int nvp = 4;
+    double kernel_offset = 1e-8, kernel_cutoff = 1e-14, cp_offset = 1e-14;
+    int charlen = FP_CHARLEN_UNITARY;
+    // host copies (0-based)
+    std::vector<double> nodes;
+    std::vector<int64_t> cell_ptr, cell_idx;
+    std::vector<int64_t> shedding;  // 6 per edge, edge-major
+    int64_t ns = 0;
+    std::vector<double> das, dbs;   // 3 per edge
+    // device
+    double *d_pv = nullptr;
+    int *d_nv = nullptr;
+    double *d_hs = nullptr;
+    double *d_geo = nullptr;
+    double *d_uinf = nullptr;       // 3 x N SoA
+    double *d_hh = nullptr;         // horseshoes
+    int *d_hcol = nullptr;
+    int nh = 0;
+    double *d_strength = nullptr;
+    bool has_uinf = false, has_wake = false, solved = false;
+};
+
+struct fp_mat {
+    fp_ctx *ctx = nullptr;
+    int64_t n = 0, ld = 0;
+    int precision = FP_F64;         // storage of A
+    void *A = nullptr;              // working copy (unfactorised until `factored`)
+    double *A64 = nullptr;          // FP64 original kept for residuals (uploaded matrices)
+    int *ipiv = nullptr;            // device, 0-based
+    int factored = -1;              // -1 none, else precision of the factors in `F`
+    void *F = nullptr;              // factors (row-major, same ld)
+    std::vector<fp_body *> bodies;  // bodies of an assembled AIC (matrix-free FP64 residual)
+    // block-Jacobi preconditioner
+    void *BJ = nullptr;
+    int *bj_piv = nullptr;
+    int bj_block = 0, bj_nblocks = 0;
+};
+
+// ---- error helpers --------------------------------------------------------
+int fp_fail(fp_ctx *ctx, int code, const std::string &msg);
+int fp_hip_fail(fp_ctx *ctx, hipError_t e, const char *what);
+#define FP_HIP(ctx, call)                                                  \
+    do {                                                                   \
+        hipError_t _e = (call);                                            \
+        if (_e != hipSuccess) return fp_hip_fail((ctx), _e, #call);        \
+    } while (0)
+
+// ---- views passed to kernels ---------------------------------------------
+struct PanelView {
+    const double *pv;
+    const int *nv;
+    const double *hs;
+    int64_t N;
+    int nvp;
+    int element;
+    double off, cut;
+};
+struct TargetView {
+    const double *x, *y, *z;     // points
+    const double *nx, *ny, *nz;  // normals (may be null when not dotting)
+    int64_t M;
+};
+struct HorseView {
+    const double *hh;
+    const int *col;
+    int nh;
+};
+
+static inline PanelView panel_view(const fp_body *b)
+{
+    PanelView v;
+    v.pv = b->d_pv; v.nv = b->d_nv; v.hs = b->d_hs; v.N = b->N; v.nvp = b->nvp;
+    v.element = b->element; v.off = b->kernel_offset; v.cut = b->kernel_cutoff;
+    return v;
+}
+static inline HorseView horse_view(const fp_body *b)
+{
+    HorseView h; h.hh = b->d_hh; h.col = b->d_hcol; h.nh = b->nh;
+    return h;
+}
+
+// ---- launchers (fp_influence.hip) ---------------------------------------
+// geometry of one body (normals, CPs, areas, Hess-Smith frames)
+int fp_launch_geometry(fp_ctx *ctx, fp_body *b);
+// AIC column block of one body into a row-major matrix (T = float|double)
+template <typename T>
+int fp_launch_assemble(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G, int64_t ld, int64_t col0);
+// Field sums of one body over targets.  mode 0: out3 (3 x M SoA) += U;
+// mode 1: out1[i] += n_i . U (strengths `x`);  mode 2: out1 += phi.
+int fp_launch_field(fp_ctx *ctx, const fp_body *b, const double *strength, const TargetView &tv, int mode,
+                    double *out, double *partial, size_t partial_bytes);
+size_t fp_field_partial_bytes(int64_t M, int64_t N);
+
+// ---- dense linear algebra (fp_dense.hip) --------------------------------
+template <typename T>
+int fp_getrf(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *ipiv, int *info_dev, void *work, size_t work_bytes);
+size_t fp_getrf_work_bytes(int64_t n, int elem_bytes);
+// solve with LU factors: x (FP64, n x k SoA columns, ldx) overwritten
+template <typename T>
+int fp_getrs(fp_ctx *ctx, const T *LU, int64_t n, int64_t ld, const int *ipiv, double *x, int64_t k, int64_t ldx,
+             void *work, size_t work_bytes);
+size_t fp_getrs_work_bytes(int64_t n, int64_t k);
+// y = alpha*A*x + beta*y, A row-major (T), x,y FP64
+template <typename T>
+int fp_gemv(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, const double *x, double *y, double alpha, double beta);
+// batched block-Jacobi LU of the diagonal blocks (FP32 factors)
+template <typename T>
+int fp_bj_factor(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *BJ, int *piv, int *info_dev);
+int fp_bj_apply(fp_ctx *ctx, const float *BJ, const int *piv, int64_t n, int block, const double *x, double *y);
+// small vector kernels
+int fp_vec_copy_convert(fp_ctx *ctx, const double *src, void *dst, int64_t n, int dst_prec);
+int fp_mat_convert(fp_ctx *ctx, const double *src, int64_t n, int64_t lds, void *dst, int64_t ldd, int dst_prec,
+                   int transpose);
+int fp_mat_narrow(fp_ctx *ctx, const double *src, float *dst, int64_t n, int64_t ld);
+int fp_rhs_kernel(fp_ctx *ctx, const fp_body *b, double *rhs);   // rhs_i = -Uinf_i . n_i
+int fp_residual_update(fp_ctx *ctx, const double *b, double *r, int64_t n);   // r = b - r
+int fp_dot(fp_ctx *ctx, const double *x, const double *y, int64_t n, double *d_out);
+int fp_axpy(fp_ctx *ctx, double a, const double *x, double *y, int64_t n);
+
+// ---- Krylov (fp_krylov.hip / fp_abi.hip) ---------------------------------

@@ -1,0 +1,143 @@
+"""Field evaluation and post-processing on the device.
+
+* ``Uind``         -- ``Uind!(body, targets, out)`` (src/FLOWPanel_abstractbody.jl:101-110)
+* ``phi``          -- ``phi!(body, targets, out)`` (src/FLOWPanel_abstractbody.jl:118-127)
+* ``calcfield_U``  -- src/FLOWPanel_postprocess.jl:28-110 (at the control points)
+* ``calcfield_Cp`` -- src/FLOWPanel_postprocess.jl:1059-1176
+* ``calcfield_F``, ``calcfield_LDS`` -- src/FLOWPanel_postprocess.jl:1199-1307, 1615-1689
+"""
+import numpy as np
+
+from . import _native as nat
+from .bodies import calc_areas, calc_normals
+
+
+def _check_targets(targets):
+    t = nat.f64(targets)
+    if t.ndim != 2 or t.shape[0] != 3:
+        raise ValueError("targets must be 3 x nt, got %s" % (t.shape,))
+    return t
+
+
+def Uind(body, targets, out):
+    """out (3 x nt) += velocity induced by the solved body at targets."""
+    t = _check_targets(targets)
+    if out.shape != t.shape:
+        raise ValueError("out must be 3 x %d" % t.shape[1])
+    o = nat.f64(out)
+    comps = body.bodies
+    body_ctx = comps[0].ctx
+    body_ctx.check(nat.lib().fp_uind(body_ctx.handle, nat.handles(comps), len(comps), nat.dptr(t), t.shape[1],
+                                     nat.dptr(o)))
+    if o is not out:
+        out[...] = o
+    return out
+
+
+def phi(body, targets, out):
+    """out (nt) += potential of the solved body at targets."""
+    t = _check_targets(targets)
+    o = np.ascontiguousarray(out, dtype=np.float64)
+    if o.shape != (t.shape[1],):
+        raise ValueError("out must have length %d" % t.shape[1])
+    comps = body.bodies
+    ctx = comps[0].ctx
+    ctx.check(nat.lib().fp_phi(ctx.handle, nat.handles(comps), len(comps), nat.dptr(t), t.shape[1], nat.dptr(o)))
+    if o is not out:
+        out[...] = o
+    return out
+
+
+def calcfield_U(sourcebody, targetbody=None, fieldname="U", addfield=True):
+    """U = Uinf + induced velocity at the control points (the source body is the target body)."""
+    if targetbody is not None and targetbody is not sourcebody:
+        raise NotImplementedError("calcfield_U supports sourcebody == targetbody; use Uind for other targets")
+    comps = sourcebody.bodies
+    ctx = comps[0].ctx
+    n = sourcebody.ncells
+    out = np.zeros((3, n), order="F")
+    ctx.check(nat.lib().fp_calcfield_U(ctx.handle, nat.handles(comps), len(comps), nat.dptr(out)))
+    if addfield:
+        sourcebody.add_field(fieldname, out)
+        col = 0
+        for b in comps:
+            b.add_field(fieldname, out[:, col:col + b.ncells])
+            col += b.ncells
+    return out
+
+
+def calcfield_Cp(body, Uref, Us=None, correct_kuttacondition=True, clip=None, fieldname="Cp", addfield=True,
+                 U_fieldname="U"):
+    """Cp = 1 - (|U|/Uref)^2 with Kutta averaging over each body's shedding edges."""
+    comps = body.bodies
+    if Us is None:
+        Us = body.get_field(U_fieldname)
+    Us = nat.f64(Us)
+    out = np.zeros(body.ncells)
+    col = 0
+    for b in comps:
+        o = np.zeros(b.ncells)
+        U = nat.f64(Us[:, col:col + b.ncells])
+        mode = nat.FP_KUTTA_NONE
+        if correct_kuttacondition and b.nsheddings:
+            mode = {"reference": nat.FP_KUTTA_REFERENCE, "pair": nat.FP_KUTTA_PAIR,
+                    "none": nat.FP_KUTTA_NONE}[b.kutta]
+        sh = np.ascontiguousarray(b.shedding.T) if b.nsheddings else None
+        b.ctx.check(nat.lib().fp_calcfield_cp(b.ctx.handle, nat.dptr(U), b.ncells, float(Uref), nat.iptr(sh),
+                                              b.nsheddings, 0, mode, nat.dptr(o)))
+        if clip is not None:
+            o = np.array([clip(v) for v in o])
+        out[col:col + b.ncells] = o
+        if addfield:
+            b.add_field(fieldname, o)
+        col += b.ncells
+    if addfield:
+        body.add_field(fieldname, out)
+    return out
+
+
+def calcfield_F(body, Uinf, rho, Cps=None, areas=None, normals=None, correct_kuttacondition=True, fieldname="F",
+                addfield=True):
+    """F_i = -Cp_i q A_i n_i with the Kutta override at trailing-edge panels."""
+    if Cps is None:
+        Cps = body.get_field("Cp")
+    if areas is None:
+        areas = calc_areas(body)
+    if normals is None:
+        normals = calc_normals(body)
+    q = 0.5 * rho * Uinf ** 2
+    F = -(np.asarray(Cps) * q * np.asarray(areas))[None, :] * np.asarray(normals)
+    if correct_kuttacondition:
+        col = 0
+        for b in body.bodies:
+            for e in range(b.nsheddings):
+                pi, pj = b.shedding[0, e] + col, b.shedding[3, e]
+                if b.kutta == "reference":
+                    ids = [pi, pi + 1] + ([pj + col, pj + col - 1] if pj != -1 else [])
+                else:
+                    ids = [pi] + ([pj + col] if pj != -1 else [])
+                ave = np.mean([Cps[i] for i in ids])
+                for i in ids:
+                    F[:, i] = -ave * q * areas[i] * normals[:, i]
+            col += b.ncells
+    if addfield:
+        body.add_field(fieldname, F)
+    return F
+
+
+def calcfield_LDS(body, Lhat, Dhat, Shat=None, F=None):
+    """3 x 3 [L D S] projections of the integrated force."""
+    if F is None:
+        F = body.get_field("F")
+    Lhat, Dhat = np.asarray(Lhat, float), np.asarray(Dhat, float)
+    for v, nm in ((Lhat, "Lhat"), (Dhat, "Dhat")):
+        if abs(np.linalg.norm(v) - 1) > 2 * np.finfo(float).eps:
+            raise ValueError("%s=%s is not a unitary vector" % (nm, v))
+    if Shat is None:
+        Shat = np.cross(Lhat, Dhat)
+    Ftot = np.asarray(F).sum(axis=1)
+    out = np.stack([Lhat * Ftot.dot(Lhat), Dhat * Ftot.dot(Dhat), Shat * Ftot.dot(Shat)], axis=1)
+    body.add_field("L", out[:, 0])
+    body.add_field("D", out[:, 1])
+    body.add_field("S", out[:, 2])
+    return out

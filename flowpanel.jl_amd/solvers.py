@@ -1,0 +1,191 @@
+"""``solve`` and the solver-function protocol.
+
+``solve(body, Uinfs, Das, Dbs; solver)`` mirrors src/FLOWPanel_liftingbody.jl:172-216,
+src/FLOWPanel_nonliftingbody.jl:93-130 and src/FLOWPanel_multibody.jl:312-364:
+the AIC is assembled, the no-flow-through RHS formed and the system solved on
+the device; the strengths stay on the device and a host copy is stored in the
+body's fields (``Gamma``/``sigma``, ``Uinf``, ``Da``, ``Db``).
+
+The solver functions keep the reference's protocol
+``solver(y, A, b; optargs...) -> y`` (src/FLOWPanel_linearsolver.jl:24-157) for
+caller matrices, and double as selectors of the device path when passed to
+``solve``:
+
+* ``solve_ludiv`` / ``solve_backslash`` -- FP64 LU (getrf + getrs)
+* ``solve_gmres``                       -- FP64 GMRES (memory 50, atol = rtol = sqrt(eps))
+* ``solve_lu32_ir``                     -- FP32 LU on MFMA + FP64 iterative refinement
+* ``solve_gmres_ir``                    -- FP32 block-Jacobi GMRES inside FP64 refinement
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as nat
+
+last_stats = {}
+
+
+def _solve_mat(code, y, A, b, **optargs):
+    A = np.asarray(A, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    n = A.shape[0]
+    if A.ndim != 2 or A.shape[1] != n:
+        raise ValueError("A must be square, got %s" % (A.shape,))
+    if b.shape[0] != n:
+        raise ValueError("b has %d rows, expected %d" % (b.shape[0], n))
+    ctx = nat.context()
+    Af = nat.f64(A)
+    h = ctypes.c_void_p()
+    ctx.check(nat.lib().fp_mat_upload(ctx.handle, n, nat.dptr(Af), ctypes.byref(h)))
+    try:
+        k = 1 if b.ndim == 1 else b.shape[1]
+        bf = nat.f64(b.reshape(n, k))
+        x = np.zeros((n, k), order="F")
+        o = nat.opts(code, **optargs)
+        s = nat.SolveStats()
+        ctx.check(nat.lib().fp_solve_mat(ctx.handle, h, nat.dptr(bf), k, nat.dptr(x), ctypes.byref(o),
+                                         ctypes.byref(s)))
+        last_stats.clear()
+        last_stats.update(s.as_dict())
+    finally:
+        nat.lib().fp_mat_destroy(h)
+    x = x.reshape(b.shape)
+    if y is None:
+        return x
+    y[...] = x
+    return y
+
+
+def solve_ludiv(y, A, b, **optargs):
+    """solve_ludiv!(y, A, b): LU factorisation + substitution (FP64)."""
+    return _solve_mat(nat.FP_SOLVER_LU, y, A, b, **optargs)
+
+
+def solve_backslash(y, A, b, **optargs):
+    """solve_backslash!(y, A, b): y = A \\ b (dense, square: LU with partial pivoting)."""
+    return _solve_mat(nat.FP_SOLVER_LU, y, A, b, **optargs)
+
+
+def solve_gmres(y, A, b, atol=None, rtol=None, memory=50, **optargs):
+    """solve_gmres!(y, A, b): restarted GMRES in FP64 (defaults of src/FLOWPanel_linearsolver.jl:136-139)."""
+    tol = rtol if rtol is not None else (atol if atol is not None else None)
+    return _solve_mat(nat.FP_SOLVER_GMRES, y, A, b, tol=tol, restart=memory, **optargs)
+
+
+def solve_lu32_ir(y, A, b, **optargs):
+    """FP32 LU (MFMA) with FP64 iterative refinement of r = b - A x."""
+    return _solve_mat(nat.FP_SOLVER_LU32_IR, y, A, b, **optargs)
+
+
+def solve_gmres_ir(y, A, b, **optargs):
+    """Block-Jacobi GMRES on the FP32 matrix inside FP64 iterative refinement."""
+    return _solve_mat(nat.FP_SOLVER_GMRES_IR, y, A, b, **optargs)
+
+
+_CODES = {solve_ludiv: nat.FP_SOLVER_LU, solve_backslash: nat.FP_SOLVER_LU, solve_gmres: nat.FP_SOLVER_GMRES,
+          solve_lu32_ir: nat.FP_SOLVER_LU32_IR, solve_gmres_ir: nat.FP_SOLVER_GMRES_IR}
+
+
+def set_freestream(body, Uinfs, Das=None, Dbs=None):
+    """Upload Uinfs (3 x N) and Das/Dbs (3 x ns) of every component."""
+    Uinfs = np.asarray(Uinfs, dtype=np.float64)
+    if Uinfs.shape != (3, body.ncells):
+        raise nat.FPError(nat.FP_ERR_SHAPE, "Invalid Uinfs; expected size (3, %d), got %s"
+                          % (body.ncells, Uinfs.shape))
+    ns = body.nsheddings
+    if ns:
+        for nm, D in (("Das", Das), ("Dbs", Dbs)):
+            if D is None or np.asarray(D).shape != (3, ns):
+                raise nat.FPError(nat.FP_ERR_SHAPE, "Invalid %s; expected size (3, %d), got %s"
+                                  % (nm, ns, None if D is None else np.asarray(D).shape))
+        Das = np.asarray(Das, dtype=np.float64)
+        Dbs = np.asarray(Dbs, dtype=np.float64)
+    col = sh = 0
+    for b in body.bodies:
+        U = nat.f64(Uinfs[:, col:col + b.ncells])
+        da = nat.f64(Das[:, sh:sh + b.nsheddings]) if b.nsheddings else None
+        db = nat.f64(Dbs[:, sh:sh + b.nsheddings]) if b.nsheddings else None
+        b.ctx.check(nat.lib().fp_body_set_freestream(b.ctx.handle, b.handle, nat.dptr(U), nat.dptr(da),
+                                                     nat.dptr(db)))
+        b.add_field("Uinf", U)
+        if b.nsheddings:
+            b.add_field("Da", da)
+            b.add_field("Db", db)
+        col += b.ncells
+        sh += b.nsheddings
+
+
+def solve(body, Uinfs, Das=None, Dbs=None, solver=solve_ludiv, solver_optargs=None, **unused):
+    """solve(body, Uinfs[, Das, Dbs]; solver, solver_optargs) on the device.
+
+    Returns the solver statistics (refinement steps, GMRES iterations, FP64
+    relative residual, device milliseconds of assembly/factorisation/solve).
+    """
+    if solver not in _CODES:
+        raise ValueError("solver must be one of %s" % [f.__name__ for f in _CODES])
+    set_freestream(body, Uinfs, Das, Dbs)
+    kw = dict(solver_optargs or {})
+    if solver is solve_gmres:
+        if "rtol" in kw or "atol" in kw:
+            kw["tol"] = kw.pop("rtol", kw.pop("atol", None))
+        if "memory" in kw:
+            kw["restart"] = kw.pop("memory")
+    o = nat.opts(_CODES[solver], **kw)
+    s = nat.SolveStats()
+    ctx = body.ctx
+    comps = body.bodies
+    ctx.check(nat.lib().fp_solve_bodies(ctx.handle, nat.handles(comps), len(comps), ctypes.byref(o),
+                                        ctypes.byref(s)))
+    for b in comps:
+        g = np.zeros(b.ncells)
+        ctx.check(nat.lib().fp_body_get_strength(ctx.handle, b.handle, nat.dptr(g)))
+        b.strength = g
+        b._solved = True
+        b.add_field(b.element.field, g)
+    stats = s.as_dict()
+    last_stats.clear()
+    last_stats.update(stats)
+    return stats
+
+
+def assemble(body, precision="f64"):
+    """Device AIC of the body (column-major numpy copy), _G_U! of the reference."""
+    ctx = body.ctx
+    comps = body.bodies
+    h = ctypes.c_void_p()
+    prec = nat.FP_F64 if precision == "f64" else nat.FP_F32
+    ctx.check(nat.lib().fp_assemble(ctx.handle, nat.handles(comps), len(comps), prec, ctypes.byref(h)))
+    try:
+        n = nat.lib().fp_mat_size(h)
+        G = np.zeros((n, n), order="F")
+        ctx.check(nat.lib().fp_mat_download(ctx.handle, h, nat.dptr(G)))
+    finally:
+        nat.lib().fp_mat_destroy(h)
+    return G
+
+
+def calc_bc_noflowthrough(Us, normals):
+    """RHS_i = -U_i . n_i (src/FLOWPanel_linearsolver.jl:321-328)."""
+    Us = np.asarray(Us, dtype=np.float64)
+    normals = np.asarray(normals, dtype=np.float64)
+    if Us.shape != normals.shape:
+        raise ValueError("Us and normals must have the same size")
+    return -np.einsum("ij,ij->j", Us, normals)
+
+
+def getrf(A, precision="f64"):
+    """Device LU factorisation of a caller matrix: (LU, ipiv) in LAPACK form (1-based ipiv)."""
+    A = np.asarray(A, dtype=np.float64)
+    n = A.shape[0]
+    ctx = nat.context()
+    Af = nat.f64(A)
+    h = ctypes.c_void_p()
+    ctx.check(nat.lib().fp_mat_upload(ctx.handle, n, nat.dptr(Af), ctypes.byref(h)))
+    try:
+        LU = np.zeros((n, n), order="F")
+        ipiv = np.zeros(n, dtype=np.int64)
+        ctx.check(nat.lib().fp_mat_getrf(ctx.handle, h, nat.FP_F64 if precision == "f64" else nat.FP_F32,
+                                         nat.dptr(LU), nat.iptr(ipiv)))
+    finally:
+        nat.lib().fp_mat_destroy(h)
+    return LU, ipiv

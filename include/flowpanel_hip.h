@@ -1,0 +1,192 @@
+/*
+ * flowpanel_hip.h -- C ABI of the MI355X (gfx950) panel-method library.
+ *
+ * Drop-in boundary for the hot path of FLOWPanel.jl: body constructors,
+ * `solve`, the solver-function protocol, `Uind!`/`phi!`, `calcfield_U`/
+ * `calcfield_Cp`.  The reference binds these through Julia multiple dispatch;
+ * a Julia host calls this library with `ccall` (see INTEGRATION.md for the
+ * binding), the Python host mirror in flowpanel.jl_amd/ calls it through ctypes.
+ *
+ * Conventions (the reference's):
+ *   - 3-vector arrays are column-major 3 x n float64 (x, y, z contiguous per
+ *     point): nodes, targets, normals, control points, Uinfs, Das, Dbs.
+ *   - Matrices exchanged with the host are column-major (column j = panel j).
+ *   - Cells are CSR (cell_ptr[ncells+1], cell_idx) with `index_base` 0 or 1
+ *     (1 = Julia).  `shedding` is 6 x ns column-major (pi, nia, nib, pj, nja,
+ *     njb): panel index, local vertex indices of its trailing edge, partner
+ *     panel (-1 = none, whatever the base) and its local vertex indices.
+ *   - Outputs documented as "+=" accumulate into caller memory, as the
+ *     reference's kernels do (src/FLOWPanel_postprocess.jl:25-26).
+ *   - Host pointers are borrowed for the duration of a call.  Device state is
+ *     owned by the handles and freed by the matching destroy call.
+ *   - A context owns one HIP stream and is not thread-safe; distinct contexts
+ *     may be used from distinct threads.
+ *   - Every function returns FP_OK or one of the FP_ERR_* codes below, and
+ *     fp_ctx_last_error() describes the last failure.
+ */
+#ifndef FLOWPANEL_HIP_H
+#define FLOWPANEL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define FP_OK 0
+#define FP_ERR_ARG 1               /* null pointer / bad enum / bad index                     */
+#define FP_ERR_SHAPE 2             /* size mismatch: src/FLOWPanel_liftingbody.jl:180-189      */
+#define FP_ERR_NONUNIT_DIRECTION 3 /* non-unit wake direction: src/FLOWPanel_elements.jl:495-498 */
+#define FP_ERR_BAD_DIRECTIONS 4    /* da.db <= 0: src/FLOWPanel_elements.jl:1130-1133          */
+#define FP_ERR_UNSOLVED 5          /* Uind!/phi! before solve: src/FLOWPanel_abstractbody.jl:104-107 */
+#define FP_ERR_SINGULAR 6          /* exactly zero pivot in getrf (LAPACK info > 0)            */
+#define FP_ERR_NOT_CONVERGED 7     /* refinement / GMRES missed the requested tolerance        */
+#define FP_ERR_DEVICE 8            /* HIP runtime error                                        */
+#define FP_ERR_NOMEM 9             /* device allocation failed                                 */
+#define FP_ERR_UNSUPPORTED 10      /* valid request this library does not implement            */
+
+/* ---- element types (src/FLOWPanel_elements.jl:12-17) ------------------- */
+#define FP_ELEM_SOURCE 0     /* ConstantSource                                    */
+#define FP_ELEM_VORTEXRING 1 /* VortexRing == ConstantDoublet (for the velocity)   */
+
+/* ---- arithmetic precision of a dense matrix ----------------------------- */
+#define FP_F64 0
+#define FP_F32 1
+
+/* ---- characteristic length of a panel (src/FLOWPanel_abstractbody.jl:573-633) */
+#define FP_CHARLEN_UNITARY 0
+#define FP_CHARLEN_BBOX 1
+#define FP_CHARLEN_MAXDIST 2
+#define FP_CHARLEN_SQRTAREA 3
+
+/* ---- linear solvers (the solver-function protocol,
+ *      src/FLOWPanel_linearsolver.jl:24-157) -------------------------------- */
+#define FP_SOLVER_LU 0       /* solve_ludiv!/solve_backslash!: FP64 getrf + getrs            */
+#define FP_SOLVER_LU32_IR 1  /* FP32 getrf (MFMA) + FP64 iterative refinement                */
+#define FP_SOLVER_GMRES_IR 2 /* FP32 block-Jacobi GMRES inside FP64 iterative refinement     */
+#define FP_SOLVER_GMRES 3    /* solve_gmres!: FP64 GMRES (memory, atol = rtol = sqrt(eps))   */
+
+/* Kutta averaging of Cp at the trailing edge (src/FLOWPanel_postprocess.jl:1075-1097) */
+#define FP_KUTTA_NONE 0
+#define FP_KUTTA_REFERENCE 1 /* average (pi, pi+1, pj, pj-1): structured triangulated grids */
+#define FP_KUTTA_PAIR 2      /* average (pi, pj): structured quad grids                     */
+
+typedef struct fp_ctx fp_ctx;
+typedef struct fp_body fp_body;
+typedef struct fp_mat fp_mat;
+
+typedef struct fp_solve_opts {
+    int solver;        /* FP_SOLVER_*                                                     */
+    double tol;        /* target ||b - A x|| / ||b|| of the FP64 result                   */
+    double inner_tol;  /* relative tolerance of each inner GMRES solve (GMRES_IR)         */
+    int restart;       /* GMRES memory (Krylov basis size before restart)                 */
+    int max_iters;     /* cap on the total number of GMRES iterations                     */
+    int max_refine;    /* cap on refinement steps                                          */
+    int precond_block; /* block-Jacobi block size (GMRES_IR, GMRES)                       */
+} fp_solve_opts;
+
+typedef struct fp_solve_stats {
+    int refinements;   /* FP64 refinement steps taken                                      */
+    int iterations;    /* GMRES iterations (all inner solves)                              */
+    double residual;   /* final ||b - A x|| / ||b|| in FP64                                */
+    double ms_assemble;/* device time of AIC assembly                                      */
+    double ms_factor;  /* device time of the factorisation / preconditioner set-up         */
+    double ms_solve;   /* device time of substitution / Krylov / refinement                */
+    int info;          /* getrf info (first zero pivot, 1-based) or 0                      */
+} fp_solve_stats;
+
+/* Fill `opts` with the defaults of `solver`. */
+void fp_solve_opts_default(fp_solve_opts *opts, int solver);
+
+/* ---- context ------------------------------------------------------------ */
+int fp_ctx_create(int device, fp_ctx **out);
+void fp_ctx_destroy(fp_ctx *ctx);
+const char *fp_ctx_last_error(const fp_ctx *ctx);
+int fp_ctx_synchronize(fp_ctx *ctx);
+
+/* ---- bodies ------------------------------------------------------------- */
+/* NonLiftingBody{ConstantSource}(grid) (src/FLOWPanel_nonliftingbody.jl:32-72) and
+ * RigidWakeBody{VortexRing}(grid, shedding; CPoffset, kerneloffset, kernelcutoff,
+ * characteristiclength) (src/FLOWPanel_liftingbody.jl:43-134).  Uploads the
+ * geometry and computes normals, control points and areas on the device
+ * (src/FLOWPanel_abstractbody.jl:635-927).  Panels may have 3..8 vertices. */
+int fp_body_create(fp_ctx *ctx, const double *nodes, int64_t nnodes, const int64_t *cell_ptr,
+                   const int64_t *cell_idx, int64_t ncells, int index_base, int element,
+                   const int64_t *shedding, int64_t nsheddings, double cp_offset, int charlen_kind,
+                   double kernel_offset, double kernel_cutoff, fp_body **out);
+void fp_body_destroy(fp_body *body);
+int64_t fp_body_ncells(const fp_body *body);
+int64_t fp_body_nsheddings(const fp_body *body);
+/* calc_normals / calc_controlpoints / calc_areas (src/FLOWPanel_abstractbody.jl:706-927);
+ * any output may be NULL. */
+int fp_body_geometry(fp_ctx *ctx, const fp_body *body, double *normals, double *cps, double *areas);
+/* Freestream at every control point (3 x ncells) and trailing-wake directions
+ * Das/Dbs (3 x nsheddings each; NULL for a non-lifting body), as passed to
+ * solve(body, Uinfs, Das, Dbs) (src/FLOWPanel_liftingbody.jl:172-189). */
+int fp_body_set_freestream(fp_ctx *ctx, fp_body *body, const double *Uinfs, const double *Das,
+                           const double *Dbs);
+int fp_body_set_strength(fp_ctx *ctx, fp_body *body, const double *strength);
+int fp_body_get_strength(fp_ctx *ctx, const fp_body *body, double *strength);
+
+/* solve(body, Uinfs, Das, Dbs; solver) for one body or the components of a
+ * MultiBody (src/FLOWPanel_liftingbody.jl:172-216, src/FLOWPanel_nonliftingbody.jl:93-130,
+ * src/FLOWPanel_multibody.jl:312-364): AIC assembly, no-flow-through RHS,
+ * linear solve; the strengths stay on the device in the bodies.  Uses the
+ * freestream stored by fp_body_set_freestream.  `opts` may be NULL (LU). */
+int fp_solve_bodies(fp_ctx *ctx, fp_body *const *bodies, int nbodies, const fp_solve_opts *opts,
+                    fp_solve_stats *stats);
+
+/* ---- dense AIC and the solver protocol ---------------------------------- */
+/* _G_U!/_G_Uvortexring! (src/FLOWPanel_liftingbody.jl:455-572,
+ * src/FLOWPanel_nonliftingbody.jl:141-186, src/FLOWPanel_multibody.jl:502-555):
+ * device-resident AIC of the bodies with their stored wake directions. */
+int fp_assemble(fp_ctx *ctx, fp_body *const *bodies, int nbodies, int precision, fp_mat **out);
+/* Upload a caller matrix (column-major n x n) for the solver protocol. */
+int fp_mat_upload(fp_ctx *ctx, int64_t n, const double *A, fp_mat **out);
+int64_t fp_mat_size(const fp_mat *mat);
+/* Column-major n x n copy of the (unfactorised) matrix. */
+int fp_mat_download(fp_ctx *ctx, const fp_mat *mat, double *A);
+void fp_mat_destroy(fp_mat *mat);
+/* solver(y, A, b) for k right-hand sides (b, x column-major n x k):
+ * solve_ludiv!/solve_backslash!/solve_gmres! (src/FLOWPanel_linearsolver.jl:24-157).
+ * The factorisation is cached in `mat` and reused by later calls. */
+int fp_solve_mat(fp_ctx *ctx, fp_mat *mat, const double *b, int64_t k, double *x, const fp_solve_opts *opts,
+                 fp_solve_stats *stats);
+/* getrf on the device: LU factors (row interchanges applied, L unit lower and U
+ * packed column-major like LAPACK) and 1-based ipiv (LAPACK convention).
+ * `precision` FP_F64 or FP_F32 (factors returned widened to FP64). */
+int fp_mat_getrf(fp_ctx *ctx, fp_mat *mat, int precision, double *LU, int64_t *ipiv);
+
+/* ---- field evaluation and post-processing ------------------------------- */
+/* Uind!(body, targets, out) (src/FLOWPanel_abstractbody.jl:101-110,
+ * src/FLOWPanel_liftingbody.jl:576-661, src/FLOWPanel_nonliftingbody.jl:188-218):
+ * out (3 x nt) += velocity induced by the solved bodies. */
+int fp_uind(fp_ctx *ctx, fp_body *const *bodies, int nbodies, const double *targets, int64_t nt,
+            double *out);
+/* phi!(body, targets, out) (src/FLOWPanel_abstractbody.jl:118-127,
+ * src/FLOWPanel_liftingbody.jl:664-742): out (nt) += potential. */
+int fp_phi(fp_ctx *ctx, fp_body *const *bodies, int nbodies, const double *targets, int64_t nt,
+           double *out);
+/* calcfield_U(body, body) at the control points (src/FLOWPanel_postprocess.jl:28-110):
+ * out (3 x N, N = sum of ncells) += Uinf + induced velocity. */
+int fp_calcfield_U(fp_ctx *ctx, fp_body *const *bodies, int nbodies, double *out);
+/* calcfield_Cp!(out, body, Us, Uref) (src/FLOWPanel_postprocess.jl:1059-1112):
+ * out (n) += 1 - (|U|/Uref)^2, then Kutta averaging over the shedding edges. */
+int fp_calcfield_cp(fp_ctx *ctx, const double *Us, int64_t n, double uref, const int64_t *shedding,
+                    int64_t nsheddings, int index_base, int kutta_mode, double *out);
+
+/* ---- instrumentation (no counterpart in the reference) ------------------
+ * Per-kernel device time measured with HIP events on the context stream, for
+ * roofline reporting.  fp_ctx_profile(ctx, 1) clears the counters and starts
+ * recording; fp_ctx_profile_get returns the launches and total milliseconds
+ * recorded so far for one kernel class (e.g. "gemv_f32", "gemm_f64",
+ * "assemble_vr_f32", "field_matvec_f64", "panel_tslu", "bj_apply"). */
+int fp_ctx_profile(fp_ctx *ctx, int enable);
+int fp_ctx_profile_get(fp_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLOWPANEL_HIP_H */
