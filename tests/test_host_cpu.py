@@ -1,0 +1,97 @@
+"""CPU-only checks of the host side: the C ABI library loads and exports every
+symbol the header declares, host-only entry points behave, and the synthetic
+geometry generators produce the workloads' shapes and conventions.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "flowpanel_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fp_[A-Za-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol(fp):
+    from flowpanel_amd import _native
+    lib = ctypes.CDLL(_native.LIBPATH)
+    syms = declared_symbols()
+    assert len(syms) >= 25
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    assert sorted(_native.EXPORTS) == syms
+
+
+def test_solve_opts_defaults(fp):
+    from flowpanel_amd import _native as nat
+    o = nat.opts(nat.FP_SOLVER_GMRES)
+    assert abs(o.tol - np.sqrt(np.finfo(float).eps)) < 1e-20 and o.restart == 50
+    o = nat.opts(nat.FP_SOLVER_GMRES_IR, tol=1e-11)
+    assert o.tol == 1e-11 and o.precond_block == 512
+    with pytest.raises(TypeError):
+        nat.opts(nat.FP_SOLVER_LU, bogus=1)
+
+
+def test_sphere_generator(geo):
+    g = geo.sphere()
+    assert g.ncells == 1152
+    nv = g.nvertices()
+    assert (nv == 3).sum() == 96 and (nv == 4).sum() == 1056
+    # outward normals: (p2-p1) x (p3-p1) points away from the centre
+    for c in range(g.ncells):
+        p = g.nodes[:, g.cell(c)]
+        n = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+        assert np.dot(n, p.mean(axis=1)) > 0
+
+
+def test_wing_generator_and_shedding(geo):
+    g, sh = geo.loft_wing(n_chord=60, n_span=80)
+    assert g.ncells == 4800 and sh.shape == (6, 80)
+    for e in range(sh.shape[1]):
+        pu, pl = sh[0, e], sh[3, e]
+        cu, cl = g.cell(pu), g.cell(pl)
+        te_u = g.nodes[:, [cu[sh[1, e]], cu[sh[2, e]]]]
+        te_l = g.nodes[:, [cl[sh[4, e]], cl[sh[5, e]]]]
+        # the lower edge runs the other way along the same trailing edge
+        assert np.abs(te_u[:, 0] - te_l[:, 1]).max() < 1e-12
+        assert np.abs(te_u[:, 1] - te_l[:, 0]).max() < 1e-12
+    # upper-surface normals point up, lower down (mid-span, mid-chord)
+    j = 40
+    up = g.nodes[:, g.cell(45 + 60 * j)]
+    lo = g.nodes[:, g.cell(15 + 60 * j)]
+    assert np.cross(up[:, 1] - up[:, 0], up[:, 2] - up[:, 0])[2] > 0
+    assert np.cross(lo[:, 1] - lo[:, 0], lo[:, 2] - lo[:, 0])[2] < 0
+
+
+def test_triangulate_keeps_te_pattern(geo):
+    g, sh = geo.loft_wing(n_chord=8, n_span=3, span=2.0)
+    gt, st = geo.triangulate(g, sh)
+    assert gt.ncells == 2 * g.ncells
+    for e in range(st.shape[1]):
+        cu, cl = gt.cell(st[0, e]), gt.cell(st[3, e])
+        a = gt.nodes[:, [cu[st[1, e]], cu[st[2, e]]]]
+        b = gt.nodes[:, [cl[st[4, e]], cl[st[5, e]]]]
+        assert np.abs(a[:, 0] - b[:, 1]).max() < 1e-12 and np.abs(a[:, 1] - b[:, 0]).max() < 1e-12
+        # the Kutta neighbours pi+1 and pj-1 are the other halves of the same quads
+        assert st[0, e] % 2 == 0 and st[3, e] % 2 == 1
+
+
+def test_bwb_generator(geo):
+    g, sh = geo.bwb()
+    assert g.ncells == 30000 and sh.shape[1] == 120
+    g2, _ = geo.bwb()
+    assert np.array_equal(g.nodes, g2.nodes)  # deterministic
+
+
+def test_naca_section(geo):
+    c = geo.naca4("0012", 60)
+    assert c.shape == (2, 61)
+    assert np.allclose(c[:, 0], c[:, -1])          # sharp trailing edge
+    assert abs(c[1].max() - 0.06) < 2e-3             # 12 % thick
+    assert np.allclose(geo.direction(0.0), [1, 0, 0])

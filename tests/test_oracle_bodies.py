@@ -1,0 +1,107 @@
+"""Body-level pins of the oracle (CPU only): closed forms and invariants.
+
+* sphere of constant-source panels: Cp -> 1 - 9/4 sin^2(theta) (examples/sphere.jl:86-87)
+  with the error shrinking under refinement, and zero net source strength;
+* a watertight vortex-ring body: a uniform ring strength induces no velocity
+  (the singular mode noted at src/FLOWPanel_abstractliftingbody.jl:113-121);
+* assembly: column j equals n . U_vortexring(panel j) at every control point;
+  the wake only touches the shedding columns;
+* LU solve agrees with numpy; Kutta/force/LDS small cases by hand.
+"""
+import types
+
+import numpy as np
+import pytest
+
+
+def obody(grid, element, shedding=None):
+    sh = np.zeros((6, 0), np.int64) if shedding is None else shedding
+    return types.SimpleNamespace(nodes=grid.nodes, cell_ptr=grid.cell_ptr, cell_idx=grid.cell_idx,
+                                 ncells=grid.ncells, element=element, shedding=sh, nsheddings=sh.shape[1],
+                                 kerneloffset=1e-8, kernelcutoff=1e-14, CPoffset=1e-14, charlen_kind=0)
+
+
+def sphere_cp_error(orc, geo, nm, na):
+    b = obody(geo.sphere(n_meridional=nm, n_azimuthal=na), "source")
+    G, normals, cps = orc.assemble(b)
+    U = np.tile([[1.0], [0.0], [0.0]], (1, b.ncells))
+    sig = orc.solve_lu(G, orc.rhs(U, normals))
+    Us = U + orc.uind(b, sig, cps)
+    cp = orc.cp(Us, 1.0)
+    x = cps[0] / np.linalg.norm(cps, axis=0)
+    ana = 1 - 9 / 4 * (1 - x ** 2)
+    _, _, areas = orc.geometry(b.nodes, b.cell_ptr, b.cell_idx)
+    return np.abs(cp - ana).max(), abs(np.dot(sig, areas)) / np.abs(sig * areas).sum()
+
+
+def test_sphere_closed_form_and_refinement(orc, geo):
+    e1, f1 = sphere_cp_error(orc, geo, 12, 24)
+    e2, f2 = sphere_cp_error(orc, geo, 24, 48)
+    assert e2 < 0.02
+    assert e2 < 0.6 * e1          # converges under refinement
+    assert f1 < 1e-6 and f2 < 1e-6  # closed body: no net source
+
+
+def test_watertight_vortex_ring_null_mode(orc, geo):
+    b = obody(geo.sphere(n_meridional=10, n_azimuthal=16), "vortexring")
+    G, normals, cps = orc.assemble([b], [None], [None])
+    # a uniform ring strength on a closed surface cancels edge by edge
+    assert np.abs(G @ np.ones(b.ncells)).max() < 1e-10 * np.abs(G).max()
+    T = np.asfortranarray(np.array([[3.0, 0.2, 0.1], [0.1, 0.2, 0.3], [-2.0, 1.0, 1.5]]).T)
+    U = orc.uind(b, np.ones(b.ncells), T)
+    assert np.abs(U).max() < 1e-12
+
+
+def test_assembly_columns_and_wake(orc, geo):
+    g, sh = geo.loft_wing(n_chord=10, n_span=4, span=3.0)
+    b = obody(g, "vortexring", sh)
+    V = np.array([1.0, 0.0, 0.0])
+    D = np.tile(V[:, None], (1, b.nsheddings))
+    G, normals, cps = orc.assemble([b], [D], [D])
+    b0 = obody(g, "vortexring")
+    G0, _, _ = orc.assemble([b0], [None], [None])
+    for j in range(b.ncells):
+        col = np.zeros(b.ncells)
+        orc.U_vortexring(g.nodes, g.cell(j), 1.0, cps, col, dot_with=normals)
+        assert np.abs(G0[:, j] - col).max() == 0.0
+    te = set(sh[0]) | set(sh[3])
+    diff = np.abs(G - G0).max(axis=0)
+    assert all((diff[j] > 0) == (j in te) for j in range(b.ncells))
+
+
+def test_solve_matches_numpy(orc, geo):
+    g, sh = geo.loft_wing(n_chord=12, n_span=6, span=4.0)
+    b = obody(g, "vortexring", sh)
+    V = geo.direction(5.0)
+    D = np.tile(V[:, None], (1, b.nsheddings))
+    G, normals, cps = orc.assemble([b], [D], [D])
+    U = np.tile(V[:, None], (1, b.ncells))
+    rhs = orc.rhs(U, normals)
+    assert np.abs(orc.solve_lu(G, rhs) - np.linalg.solve(G, rhs)).max() < 1e-10
+    assert np.abs(rhs + np.einsum("ij,ij->j", U, normals)).max() == 0
+
+
+def test_kutta_cp_reference_indexing(orc):
+    Us = np.zeros((3, 6))
+    Us[0] = [0.0, 1.0, 2.0, 0.5, 0.1, 0.3]
+    sh = np.array([[4], [2], [1], [1], [0], [2]])  # pi=4, pj=1
+    cp = orc.cp(Us, 2.0, sh, kutta=True)
+    raw = 1 - (Us[0] / 2.0) ** 2
+    ave = (raw[4] + raw[5] + raw[1] + raw[0]) / 4
+    assert np.allclose(cp[[0, 1, 4, 5]], ave)
+    assert np.allclose(cp[[2, 3]], raw[[2, 3]])
+    one = orc.cp(Us, 2.0, np.array([[4], [2], [1], [-1], [0], [0]]), kutta=True)
+    assert np.allclose(one[[4, 5]], (raw[4] + raw[5]) / 2)
+
+
+def test_forces_and_lds(orc):
+    areas = np.array([2.0, 3.0])
+    normals = np.asfortranarray(np.array([[0, 0, 1.0], [0, 1.0, 0]]).T)
+    cps = np.array([0.5, -1.0])
+    F = orc.forces(areas, normals, cps, 2.0, 1.2)
+    q = 0.5 * 1.2 * 4.0
+    assert np.allclose(F[:, 0], [0, 0, -0.5 * q * 2])
+    assert np.allclose(F[:, 1], [0, 1.0 * q * 3, 0])
+    L = orc.lds(F, [0, 0, 1.0], [1.0, 0, 0])
+    assert np.allclose(L[:, 0], [0, 0, F[2].sum()])
+    assert np.allclose(L[:, 2], np.cross([0, 0, 1.0], [1.0, 0, 0]) * F[1].sum())
