@@ -6,8 +6,10 @@ shedding edges, rigid wake along the freestream), AoA 5 deg.  One step is one
 `solve(body, Uinfs, Das, Dbs)` from device-resident geometry and freestream to
 FP64 strengths on the device:
 
-  mixed  = FP32 AIC assembly + block-Jacobi GMRES on the FP32 AIC inside FP64
-           iterative refinement (FP64 residual recomputed matrix-free)  -> `value`
+  mixed  = AIC computed in FP64 and stored in FP32 (plus the FP64 copy for the
+           residual); restarted GMRES on the FP32 AIC, preconditioned by block-Jacobi
+           (250-panel blocks) and a Galerkin coarse correction on chordwise-strip
+           indicators, inside FP64 iterative refinement                  -> `value`
   fp64   = FP64 AIC assembly + FP64 LU (getrf/getrs)                   -> reported beside it
 
 The Cp error of the mixed result is measured against the FP64 LU result at the
@@ -78,6 +80,30 @@ def cpu_baseline(fp, body, U, D, seconds_hint=20.0):
             "assembly_ms_est": t_asm * 1e3, "lu_ms_est": t_lu * 1e3}
 
 
+def vendor_lu(fp, body, N):
+    """rocSOLVER FP64 getrf+getrs (through torch) on the same FP64 AIC: an external anchor for the FP64 path."""
+    try:
+        import torch
+        if not torch.cuda.is_available():
+            return None
+        G = fp.assemble(body, "f64")
+        A = torch.from_numpy(G).cuda()
+        del G
+        b = torch.ones(N, 1, dtype=torch.float64, device="cuda")
+        torch.linalg.lu_factor(A[:2048, :2048].contiguous())
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        LU, piv = torch.linalg.lu_factor(A)
+        x = torch.linalg.lu_solve(LU, piv, b)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) * 1e3
+        del A, LU, piv, x
+        torch.cuda.empty_cache()
+        return {"ms": round(ms, 2), "what": "torch.linalg.lu_factor+lu_solve (rocSOLVER) FP64, matrix resident"}
+    except Exception as e:  # the anchor is optional
+        return {"error": str(e)[:200]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,7 +111,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--chord", type=int, default=250)
     ap.add_argument("--span", type=int, default=120)
-    ap.add_argument("--block", type=int, default=500, help="block-Jacobi block size")
+    ap.add_argument("--block", type=int, default=250, help="block-Jacobi block size")
+    ap.add_argument("--coarse", type=int, default=-1, help="coarse chunk (default: one chordwise strip)")
+    ap.add_argument("--tol", type=float, default=1e-10, help="FP64 relative residual target of the mixed solve")
+    ap.add_argument("--inner-tol", type=float, default=1e-5, help="relative tolerance of each inner FP32 GMRES")
+    ap.add_argument("--vendor", action="store_true", help="also time rocSOLVER FP64 getrf through torch")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -101,7 +131,9 @@ def main():
     body, U, D = build_body(fp, args.chord, args.span, alpha)
     N = body.ncells
     fp.solvers.set_freestream(body, U, D, D)
-    mixed_opts = {"precond_block": args.block, "tol": 1e-12, "inner_tol": 1e-4, "restart": 60}
+    coarse = args.chord if args.coarse < 0 else args.coarse
+    mixed_opts = {"precond_block": args.block, "coarse_block": coarse, "tol": args.tol, "inner_tol": args.inner_tol,
+                  "restart": 100, "residual_mode": 1}
 
     def run(solver, opts):
         return fp.solve(body, U, D, D, solver=solver, solver_optargs=opts)
@@ -118,14 +150,15 @@ def main():
         log("bench: mixed step %d: %s" % (k, stats[-1]))
     ctx.synchronize()
     t_mixed = (time.perf_counter() - t0) / args.steps * 1e3
-    prof = {k: ctx.profile_get(k) for k in ("gemv_f32", "assemble_vr_f32", "assemble_wake", "field_matvec_f64",
-                                           "bj_factor", "bj_apply")}
+    prof = {k: ctx.profile_get(k) for k in ("gemv_f32", "gemv_f64", "assemble_vr_f64", "assemble_wake", "bj_factor",
+                                           "bj_invert", "bj_apply", "coarse_setup", "coarse_apply")}
     ctx.profile(False)
     gam_mixed = body.strength.copy()
     Us_m = fp.calcfield_U(body, body)
     cp_m = fp.calcfield_Cp(body, 1.0)
 
     log("bench: fp64 LU")
+    ctx.profile(True)
     t64 = []
     st64 = None
     for k in range(args.fp64_steps + 1):
@@ -136,6 +169,8 @@ def main():
         t64.append((time.perf_counter() - t) * 1e3)
         log("bench: fp64 step %d: %.1f ms %s" % (k, t64[-1], st64))
     t_fp64 = min(t64[1:]) if len(t64) > 1 else t64[0]
+    prof64 = {k: ctx.profile_get(k) for k in ("assemble_vr_f64", "gemm_f64", "panel_tslu", "trsm", "getrs")}
+    ctx.profile(False)
     gam64 = body.strength.copy()
     fp.calcfield_U(body, body)
     cp64 = fp.calcfield_Cp(body, 1.0)
@@ -152,7 +187,11 @@ def main():
             "avg_launch_ms": round(avg_ms, 4), "launches": launches,
             "algorithmic_bytes_per_launch": gemv_bytes}
     kernels = {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in prof.items()}
+    kernels_fp64 = {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in prof64.items()}
 
+    vendor = None
+    if args.vendor:
+        vendor = vendor_lu(fp, body, N)
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")
@@ -169,7 +208,8 @@ def main():
         "fp64_lu_ms": round(t_fp64, 3), "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
         "cp_err_max": cp_err, "gamma_rel_err": gam_err,
         "mixed_stats": stats[-1], "fp64_stats": st64, "adjoint_ms": None,
-        "kernels": kernels, "roofline": roof, "cpu_baseline": cpu,
+        "kernels": kernels, "kernels_fp64": kernels_fp64, "vendor_fp64_getrf": vendor, "roofline": roof,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 

@@ -31,6 +31,8 @@ FP_SOLVER_LU = 0
 FP_SOLVER_LU32_IR = 1
 FP_SOLVER_GMRES_IR = 2
 FP_SOLVER_GMRES = 3
+FP_RESID_MATRIX_FREE = 0
+FP_RESID_STORED = 1
 FP_KUTTA_NONE = 0
 FP_KUTTA_REFERENCE = 1
 FP_KUTTA_PAIR = 2
@@ -49,7 +51,7 @@ EXPORTS = [
 class SolveOpts(ctypes.Structure):
     _fields_ = [("solver", ctypes.c_int), ("tol", ctypes.c_double), ("inner_tol", ctypes.c_double),
                 ("restart", ctypes.c_int), ("max_iters", ctypes.c_int), ("max_refine", ctypes.c_int),
-                ("precond_block", ctypes.c_int)]
+                ("precond_block", ctypes.c_int), ("coarse_block", ctypes.c_int), ("residual_mode", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):

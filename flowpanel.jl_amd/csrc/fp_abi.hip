@@ -167,10 +167,54 @@ struct Precond {
     virtual int apply(const double *x, double *y) = 0;
     virtual ~Precond() {}
 };
+// Block-Jacobi: LU of every diagonal block; for blocks <= 512 the explicit
+// inverses are formed once so that each application is one batched GEMV.
+// With coarse_block > 0 a Galerkin coarse correction on chunk indicators Z is
+// applied first (two-level, multiplicative):  y = Z c + B (v - A Z c),
+// c = (Z^T A Z)^-1 Z^T v.
 struct BJPrecond : Precond {
     fp_ctx *ctx; const float *BJ; const int *piv; int64_t n; int block;
+    DBuf inv, AZ, Einv, work, tc, r2, info;
+    bool has_inv = false;
+    int cs = 0;
     BJPrecond(fp_ctx *c, const float *b, const int *p, int64_t n_, int bl) : ctx(c), BJ(b), piv(p), n(n_), block(bl) {}
-    int apply(const double *x, double *y) override { return fp_bj_apply(ctx, BJ, piv, n, block, x, y); }
+    int setup(const void *A = nullptr, int prec = FP_F32, int64_t ld = 0, int coarse = 0)
+    {
+        int st;
+        if (block <= 512) {
+            const int64_t nb = (n + block - 1) / block;
+            if ((st = dalloc(ctx, inv, (size_t)nb * block * block * sizeof(float)))) return st;
+            if ((st = fp_bj_invert(ctx, BJ, piv, n, block, inv.as<float>()))) return st;
+            has_inv = true;
+        }
+        if (coarse > 0 && A && has_inv) {
+            cs = coarse;
+            const int nc = (int)((n + cs - 1) / cs);
+            if ((st = dalloc(ctx, AZ, (size_t)nc * n * sizeof(double)))) return st;
+            if ((st = dalloc(ctx, Einv, (size_t)nc * nc * sizeof(double)))) return st;
+            if ((st = dalloc(ctx, work, (size_t)2 * nc * nc * sizeof(double)))) return st;
+            if ((st = dalloc(ctx, tc, 2 * nc * sizeof(double)))) return st;
+            if ((st = dalloc(ctx, r2, n * sizeof(double)))) return st;
+            if ((st = dalloc(ctx, info, sizeof(int)))) return st;
+            st = prec == FP_F64 ? fp_coarse_setup<double>(ctx, (const double *)A, n, ld, cs, AZ.as<double>(),
+                                                          Einv.as<double>(), work.as<double>(), info.as<int>())
+                                : fp_coarse_setup<float>(ctx, (const float *)A, n, ld, cs, AZ.as<double>(),
+                                                         Einv.as<double>(), work.as<double>(), info.as<int>());
+            if (st) return st;
+        }
+        return FP_OK;
+    }
+    int apply(const double *x, double *y) override
+    {
+        if (!has_inv) return fp_bj_apply(ctx, BJ, piv, n, block, x, y);
+        if (cs > 0) {
+            int st = fp_coarse_apply(ctx, x, n, cs, AZ.as<double>(), Einv.as<double>(), tc.as<double>(), y,
+                                     r2.as<double>());
+            if (st) return st;
+            return fp_bj_apply_inv(ctx, inv.as<float>(), n, block, r2.as<double>(), y, 1);
+        }
+        return fp_bj_apply_inv(ctx, inv.as<float>(), n, block, x, y, 0);
+    }
 };
 
 double dnorm_host(fp_ctx *ctx, const double *x, int64_t n, double *part, double *dout, int *st)
@@ -289,13 +333,14 @@ struct Gmres {
             }
             FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
             if (*rel <= tol || *its >= maxit) {
-                // report the true residual of the final iterate
+                // the true residual of the final iterate decides; restart if the
+                // Arnoldi estimate was optimistic and iterations are left
                 if ((st = A.apply(x, wp))) return st;
                 if ((st = fp_residual_update(ctx, b, wp, n))) return st;
                 const double rn = dnorm_host(ctx, wp, n, pp, hp, &st);
                 if (st) return st;
                 *rel = rn / bnorm;
-                return FP_OK;
+                if (*rel <= tol || *its >= maxit) return FP_OK;
             }
         }
     }
@@ -396,6 +441,9 @@ int solve_core(fp_ctx *ctx, const SolveArgs &a, const fp_solve_opts &o, const do
             else st = fp_bj_factor<float>(ctx, a.A32, n, a.ld, blk, bj.as<float>(), bjp.as<int>(), info.as<int>());
             if (st) return st;
             M.reset(new BJPrecond(ctx, bj.as<float>(), bjp.as<int>(), n, blk));
+            if (a.A64) st = M->setup(a.A64, FP_F64, a.ld, o.coarse_block);
+            else st = M->setup(a.A32, FP_F32, a.ld, o.coarse_block);
+            if (st) return st;
         }
         stats.ms_factor = tm.stop();
         tm.restart();
@@ -428,6 +476,7 @@ int solve_core(fp_ctx *ctx, const SolveArgs &a, const fp_solve_opts &o, const do
         if ((st = fp_bj_factor<float>(ctx, a.A32, n, a.ld, blk, bj.as<float>(), bjp.as<int>(), info.as<int>())))
             return st;
         M.reset(new BJPrecond(ctx, bj.as<float>(), bjp.as<int>(), n, blk));
+        if ((st = M->setup(a.A32, FP_F32, a.ld, o.coarse_block))) return st;
         A32op.reset(new DenseOp<float>(ctx, a.A32, n, a.ld));
         g.reset(new Gmres());
         if ((st = g->init(ctx, n, std::max(1, o.restart)))) return st;
@@ -457,7 +506,10 @@ int solve_core(fp_ctx *ctx, const SolveArgs &a, const fp_solve_opts &o, const do
             int its = 0;
             double irel = 0;
             const int left = std::max(1, o.max_iters - stats.iterations);
-            if ((st = g->solve(*A32op, M.get(), rp, dp, o.inner_tol, left, &its, &irel))) return st;
+            // the correction only has to bring the residual below tol: once
+            // rel is small the inner solve may stop earlier
+            const double itol = std::max(o.inner_tol, std::min(0.5, 0.3 * o.tol / std::max(rel, 1e-300)));
+            if ((st = g->solve(*A32op, M.get(), rp, dp, itol, left, &its, &irel))) return st;
             stats.iterations += its;
         }
         if ((st = fp_axpby(ctx, 1.0, dp, 1.0, x, n))) return st;
@@ -586,6 +638,8 @@ void fp_solve_opts_default(fp_solve_opts *o, int solver)
     o->max_iters = 2000;
     o->max_refine = 20;
     o->precond_block = 0;
+    o->coarse_block = 0;
+    o->residual_mode = FP_RESID_STORED;
     o->inner_tol = 1e-5;
     o->tol = 1e-12;
     if (solver == FP_SOLVER_GMRES) {
@@ -625,6 +679,7 @@ void fp_ctx_destroy(fp_ctx *ctx)
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
     }
+    for (auto e : ctx->prof_pool) (void)hipEventDestroy(e);
     delete ctx;
 }
 
@@ -641,8 +696,8 @@ static void prof_fold(fp_ctx *ctx)
             acc.first += 1;
             acc.second += ms;
         }
-        (void)hipEventDestroy(r.a);
-        (void)hipEventDestroy(r.b);
+        ctx->prof_pool.push_back(r.a);
+        ctx->prof_pool.push_back(r.b);
     }
     ctx->prof_pending.clear();
 }
@@ -881,18 +936,30 @@ int fp_solve_bodies(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_solve_
     a.resid = &mf;
     Timer tm(ctx->stream);
     const bool f64 = o.solver == FP_SOLVER_LU || o.solver == FP_SOLVER_GMRES;
-    if ((st = dalloc(ctx, A, (size_t)n * ld * (f64 ? sizeof(double) : sizeof(float))))) return st;
-    if ((st = assemble_into(ctx, bodies, nb, t.tv, f64 ? FP_F64 : FP_F32, A.p, ld))) return st;
-    s.ms_assemble = tm.stop();
-    if (f64) a.A64 = A.as<double>();
-    else a.A32 = A.as<float>();
-    if (o.solver == FP_SOLVER_GMRES) {
-        DenseOp<double> op(ctx, a.A64, n, ld);
-        a.resid = &op;
-        st = solve_core(ctx, a, o, rhs.as<double>(), x.as<double>(), s);
+    // refinement with a stored FP64 residual: assemble in FP64 once, keep it,
+    // and narrow a copy to FP32 for the low-precision solver
+    const bool dual = !f64 && o.residual_mode == FP_RESID_STORED;
+    DBuf A32buf;
+    std::unique_ptr<DenseOp<double>> op64;
+    if ((st = dalloc(ctx, A, (size_t)n * ld * ((f64 || dual) ? sizeof(double) : sizeof(float))))) return st;
+    if (dual && (st = dalloc(ctx, A32buf, (size_t)n * ld * sizeof(float)))) return st;
+    if ((st = assemble_into(ctx, bodies, nb, t.tv, (f64 || dual) ? FP_F64 : FP_F32, A.p, ld))) return st;
+    if (dual) {
+        if ((st = fp_mat_narrow(ctx, A.as<double>(), A32buf.as<float>(), n, ld))) return st;
+        a.A32 = A32buf.as<float>();
+        op64.reset(new DenseOp<double>(ctx, A.as<double>(), n, ld));
+        a.resid = op64.get();
+    } else if (f64) {
+        a.A64 = A.as<double>();
     } else {
-        st = solve_core(ctx, a, o, rhs.as<double>(), x.as<double>(), s);
+        a.A32 = A.as<float>();
     }
+    s.ms_assemble = tm.stop();
+    if (o.solver == FP_SOLVER_GMRES) {
+        op64.reset(new DenseOp<double>(ctx, a.A64, n, ld));
+        a.resid = op64.get();
+    }
+    st = solve_core(ctx, a, o, rhs.as<double>(), x.as<double>(), s);
     if (stats) *stats = s;
     if (st) return st;
     col = 0;

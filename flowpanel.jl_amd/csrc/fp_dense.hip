@@ -42,28 +42,58 @@ template <> struct MfmaOp<float> {
 
 // ----------------------------------------------------------------------------
 // C[m x n] -= A[m x k] * B[k x n], all row-major.  128 x 128 tile per 256-thread
-// workgroup (2 x 2 waves of 64 x 64), K step 16 through LDS, v_mfma 16x16x4.
+// workgroup (2 x 2 waves of 64 x 64), K step 32 through LDS with register
+// prefetch of the next step, 16-byte global loads, v_mfma 16x16x4.
 // MFMA operand layout (16x16x4): lane l supplies A(l%16, l/16) and B(l/16, l%16);
 // the accumulator rows differ between f32 and f64 (MfmaOp<T>::row, measured on
 // gfx950 with an in-bounds lane-layout probe).
-constexpr int GB_M = 128, GB_N = 128, GB_K = 16, GB_PAD = 4;
+// Tile order: workgroup b runs on XCD b % 8; every XCD is given a contiguous
+// range of a grouped order (8 tile rows walked together), so the A rows of a
+// group stay in that XCD's L2 while it sweeps across the B columns.
+constexpr int GB_M = 128, GB_N = 128, GB_K = 32, GB_PAD = 4, GB_GROUP = 8, GB_XCD = 8;
+
+template <typename T>
+__device__ __forceinline__ void ld16(const T *__restrict__ p, bool full, bool vec, int valid, T *out)
+{
+    if (full && vec) {
+        if (sizeof(T) == 8) {
+#pragma unroll
+            for (int e = 0; e < 16; e += 2) {
+                const double2 v = *reinterpret_cast<const double2 *>(p + e);
+                out[e] = (T)v.x; out[e + 1] = (T)v.y;
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 16; e += 4) {
+                const float4 v = *reinterpret_cast<const float4 *>(p + e);
+                out[e] = (T)v.x; out[e + 1] = (T)v.y; out[e + 2] = (T)v.z; out[e + 3] = (T)v.w;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) out[e] = e < valid ? p[e] : T(0);
+    }
+}
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t k, const T *__restrict__ A,
                                                   int64_t lda, const T *__restrict__ B, int64_t ldb,
-                                                  T *__restrict__ C, int64_t ldc)
+                                                  T *__restrict__ C, int64_t ldc, int vec)
 {
     typedef typename MfmaOp<T>::acc acc_t;
     __shared__ T As[GB_K][GB_M + GB_PAD];
     __shared__ T Bs[GB_K][GB_N + GB_PAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    // XCD-friendly order: consecutive workgroups (dispatched round-robin over
-    // the 8 XCDs) walk down a column of C tiles so tiles sharing a B panel
-    // land on different XCDs evenly; the grid is 1-D.
-    const int64_t tiles_m = (m + GB_M - 1) / GB_M;
-    const int64_t bid = blockIdx.x;
-    const int64_t tm = bid % tiles_m, tn = bid / tiles_m;
+    const int64_t tiles_m = (m + GB_M - 1) / GB_M, tiles_n = (n + GB_N - 1) / GB_N;
+    const int64_t ntiles = tiles_m * tiles_n;
+    const int64_t per = (ntiles + GB_XCD - 1) / GB_XCD;
+    const int64_t t = (int64_t)(blockIdx.x % GB_XCD) * per + blockIdx.x / GB_XCD;
+    if (t >= ntiles) return;
+    const int64_t in_group = (int64_t)GB_GROUP * tiles_n;
+    const int64_t first_m = (t / in_group) * GB_GROUP;
+    const int64_t gsize = min((int64_t)GB_GROUP, tiles_m - first_m);
+    const int64_t tm = first_m + (t % in_group) % gsize, tn = (t % in_group) / gsize;
     const int64_t m0 = tm * GB_M, n0 = tn * GB_N;
 
     acc_t acc[4][4];
@@ -72,31 +102,25 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t 
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
 
-    // global -> register staging: A: row tid/2, k (tid%2)*8..+8 ; B: k tid/16, cols (tid%16)*8..+8
-    const int arow = tid >> 1, ak = (tid & 1) * 8;
-    const int bk = tid >> 4, bc = (tid & 15) * 8;
-    T ra[8], rb[8];
+    // A: row tid/2, k (tid%2)*16..+16 ; B: k row tid/8, cols (tid%8)*16..+16
+    const int arow = tid >> 1, ak = (tid & 1) * 16;
+    const int bk = tid >> 3, bc = (tid & 7) * 16;
+    T ra[16], rb[16];
     auto load = [&](int64_t k0) {
-        const int64_t gr = m0 + arow;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int64_t gk = k0 + ak + e;
-            ra[e] = (gr < m && gk < k) ? A[gr * lda + gk] : T(0);
-        }
-        const int64_t gk = k0 + bk;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int64_t gc = n0 + bc + e;
-            rb[e] = (gk < k && gc < n) ? B[gk * ldb + gc] : T(0);
-        }
+        const int64_t gr = m0 + arow, gk = k0 + ak;
+        const int av = gr < m ? (int)max((int64_t)0, min((int64_t)16, k - gk)) : 0;
+        ld16<T>(A + gr * lda + gk, av == 16, vec, av, ra);
+        const int64_t gkb = k0 + bk, gc = n0 + bc;
+        const int bv = gkb < k ? (int)max((int64_t)0, min((int64_t)16, n - gc)) : 0;
+        ld16<T>(B + gkb * ldb + gc, bv == 16, vec, bv, rb);
     };
     load(0);
     for (int64_t k0 = 0; k0 < k; k0 += GB_K) {
         __syncthreads();
 #pragma unroll
-        for (int e = 0; e < 8; ++e) As[ak + e][arow] = ra[e];
+        for (int e = 0; e < 16; ++e) As[ak + e][arow] = ra[e];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) Bs[bk][bc + e] = rb[e];
+        for (int e = 0; e < 16; ++e) Bs[bk][bc + e] = rb[e];
         __syncthreads();
         if (k0 + GB_K < k) load(k0 + GB_K);
 #pragma unroll
@@ -130,10 +154,14 @@ static int gemm_sub(fp_ctx *ctx, int64_t m, int64_t n, int64_t k, const T *A, in
                     T *C, int64_t ldc)
 {
     if (m <= 0 || n <= 0 || k <= 0) return FP_OK;
-    const int64_t tiles = ((m + GB_M - 1) / GB_M) * ((n + GB_N - 1) / GB_N);
+    const int64_t ntiles = ((m + GB_M - 1) / GB_M) * ((n + GB_N - 1) / GB_N);
+    const int64_t grid = (ntiles + GB_XCD - 1) / GB_XCD * GB_XCD;
+    // 16-byte loads need 16-byte aligned rows and offsets
+    const int vec = (((uintptr_t)A | (uintptr_t)B) & 15) == 0 && (lda * sizeof(T)) % 16 == 0 &&
+                    (ldb * sizeof(T)) % 16 == 0;
     ProfScope ps(ctx, sizeof(T) == 8 ? "gemm_f64" : "gemm_f32");
-    hipLaunchKernelGGL(k_gemm_sub<T>, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, m, n, k, A, lda, B, ldb, C,
-                       ldc);
+    hipLaunchKernelGGL(k_gemm_sub<T>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, m, n, k, A, lda, B, ldb, C,
+                       ldc, vec);
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }
@@ -349,9 +377,11 @@ __global__ __launch_bounds__(256) void k_tslu_merge(int nsets, int pw, const T *
     if (final_lu && tid == 0 && sing && *info == 0) *info = s + sing;
 }
 
-// Winners (in pivot order) -> LAPACK swap sequence ipiv[s..s+pw) (one thread;
-// at most 2*pw rows move).
-__global__ void k_tslu_ipiv(const int *__restrict__ win, int s, int pw, int n, int *__restrict__ ipiv)
+// Winners (in pivot order) -> LAPACK swap sequence ipiv[s..s+pw) and the
+// equivalent row map (final position <- original row, at most 2*pw rows move),
+// computed by one thread.  map[0] = count, then (dst, src) pairs.
+__global__ void k_tslu_ipiv(const int *__restrict__ win, int s, int pw, int n, int *__restrict__ ipiv,
+                            int *__restrict__ map)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     int mrow[2 * PW], mpos[2 * PW];  // rows that moved and where they are
@@ -386,6 +416,31 @@ __global__ void k_tslu_ipiv(const int *__restrict__ win, int s, int pw, int n, i
             if (y >= 0) set(y, p);
         }
     }
+    int cnt = 0;
+    for (int q = 0; q < nm; ++q)
+        if (mpos[q] != mrow[q]) {
+            map[1 + 2 * cnt] = mpos[q];
+            map[2 + 2 * cnt] = mrow[q];
+            ++cnt;
+        }
+    map[0] = cnt;
+}
+
+// Apply a row map (A[dst] <- A[src] for every pair) to every column: one lane
+// per column, all loads issued before the stores.
+template <typename T>
+__global__ void k_rowmap(T *__restrict__ A, int64_t ld, int64_t ncols, const int *__restrict__ map)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int cnt = map[0];
+    if (c >= ncols) return;
+    T v[2 * PW];
+#pragma unroll
+    for (int e = 0; e < 2 * PW; ++e)
+        if (e < cnt) v[e] = A[(int64_t)map[2 + 2 * e] * ld + c];
+#pragma unroll
+    for (int e = 0; e < 2 * PW; ++e)
+        if (e < cnt) A[(int64_t)map[1 + 2 * e] * ld + c] = v[e];
 }
 
 // Rows [s, s+pw) of the sub-panel <- factored block; rows below <- A21 U11^-1.
@@ -432,7 +487,8 @@ size_t fp_getrf_work_bytes(int64_t n, int elem_bytes)
 {
     const int64_t g = (n + CH - 1) / CH;
     // two candidate buffers (values + indices) + final block
-    return (size_t)(2 * g * PW * PW) * elem_bytes + (size_t)(2 * g * PW) * sizeof(int) + PW * PW * elem_bytes + 256;
+    return (size_t)(2 * g * PW * PW) * elem_bytes + (size_t)(2 * g * PW) * sizeof(int) + PW * PW * elem_bytes +
+           (4 * PW + 16) * sizeof(int) + 256;
 }
 
 template <typename T>
@@ -447,7 +503,8 @@ int fp_getrf(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *ipiv, int *info_dev,
     T *cv1 = (T *)w; w += gmax * PW * PW * sizeof(T);
     int *ci0 = (int *)w; w += gmax * PW * sizeof(int);
     int *ci1 = (int *)w; w += gmax * PW * sizeof(int);
-    T *flu = (T *)w;
+    T *flu = (T *)w; w += PW * PW * sizeof(T);
+    int *rmap = (int *)w;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, info_dev, (int64_t)1, 0);
     const int NB = 256;
     for (int64_t k = 0; k < n; k += NB) {
@@ -473,9 +530,10 @@ int fp_getrf(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *ipiv, int *info_dev,
                 if (last) break;
             }
             // ii now holds the pw winners in pivot order
-            hipLaunchKernelGGL(k_tslu_ipiv, dim3(1), dim3(64), 0, ctx->stream, (const int *)ii, (int)s, pw, (int)n, ipiv);
-            hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, A, ld, n, n,
-                               (const int *)ipiv, (int)s, (int)(s + pw));
+            hipLaunchKernelGGL(k_tslu_ipiv, dim3(1), dim3(64), 0, ctx->stream, (const int *)ii, (int)s, pw, (int)n, ipiv,
+                               rmap);
+            hipLaunchKernelGGL(k_rowmap<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, A, ld, n,
+                               (const int *)rmap);
             hipLaunchKernelGGL(k_tslu_l21<T>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, ctx->stream, A, ld,
                                (int)n, (int)s, pw, (const T *)flu);
             FP_HIP(ctx, hipGetLastError());
@@ -612,35 +670,79 @@ template int fp_getrs<float>(fp_ctx *, const float *, int64_t, int64_t, const in
                              void *, size_t);
 
 // ----------------------------------------------------------------------------
-// y = alpha A x + beta y (row-major A, FP64 vectors): one wave per row,
-// 4-wide vector loads when the row is 16-byte aligned.
-template <typename T>
+// y = alpha A x + beta y (row-major A, FP64 vectors).  One wave per 4 rows:
+// every lane loads 4 consecutive x values once and reuses them for the 4 rows,
+// A is read with 16-byte (float4) or 32-byte (2 x double2) vector loads.
+template <typename T> struct Vec4;
+template <> struct Vec4<float> {
+    static __device__ __forceinline__ void load(const float *p, double *v)
+    {
+        const float4 q = *reinterpret_cast<const float4 *>(p);
+        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    }
+};
+template <> struct Vec4<double> {
+    static __device__ __forceinline__ void load(const double *p, double *v)
+    {
+        const double2 a = *reinterpret_cast<const double2 *>(p);
+        const double2 b = *reinterpret_cast<const double2 *>(p + 2);
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    }
+};
+
+constexpr int GEMV_R = 4;
+
+template <typename T, bool XVEC>
 __global__ __launch_bounds__(256) void k_gemv(const T *__restrict__ A, int64_t n, int64_t ld,
                                               const double *__restrict__ x, double *__restrict__ y, double alpha,
                                               double beta)
 {
     const int lane = threadIdx.x & 63;
-    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (r >= n) return;
-    const T *row = A + r * ld;
-    double acc = 0;
-    const int64_t n4 = (n / 4) * 4;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * GEMV_R;
+    if (r0 >= n) return;
+    const int nr = (int)min((int64_t)GEMV_R, n - r0);
+    double acc[GEMV_R] = {0, 0, 0, 0};
+    const int64_t n4 = (n / 4) * 4;  // ld is a multiple of 64, rows are 256-byte aligned
     for (int64_t c = 4 * lane; c < n4; c += 256) {
-        acc += (double)row[c] * x[c] + (double)row[c + 1] * x[c + 1] + (double)row[c + 2] * x[c + 2] +
-               (double)row[c + 3] * x[c + 3];
+        double2 xa, xb;
+        if (XVEC) {
+            xa = *reinterpret_cast<const double2 *>(x + c);
+            xb = *reinterpret_cast<const double2 *>(x + c + 2);
+        } else {
+            xa.x = x[c]; xa.y = x[c + 1]; xb.x = x[c + 2]; xb.y = x[c + 3];
+        }
+#pragma unroll
+        for (int r = 0; r < GEMV_R; ++r) {
+            if (r < nr) {
+                double v[4];
+                Vec4<T>::load(A + (r0 + r) * ld + c, v);
+                acc[r] += v[0] * xa.x + v[1] * xa.y + v[2] * xb.x + v[3] * xb.y;
+            }
+        }
     }
-    for (int64_t c = n4 + lane; c < n; c += 64) acc += (double)row[c] * x[c];
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if (lane == 0) y[r] = alpha * acc + (beta == 0.0 ? 0.0 : beta * y[r]);
+    for (int64_t c = n4 + lane; c < n; c += 64)
+        for (int r = 0; r < nr; ++r) acc[r] += (double)A[(r0 + r) * ld + c] * x[c];
+#pragma unroll
+    for (int r = 0; r < GEMV_R; ++r) {
+        double a = acc[r];
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+        if (lane == 0 && r < nr) y[r0 + r] = alpha * a + (beta == 0.0 ? 0.0 : beta * y[r0 + r]);
+    }
 }
 
 template <typename T>
 int fp_gemv(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, const double *x, double *y, double alpha, double beta)
 {
     if (n <= 0) return FP_OK;
+    if (ld % 4) return fp_fail(ctx, FP_ERR_ARG, "gemv needs ld % 4 == 0");
     ProfScope ps(ctx, sizeof(T) == 8 ? "gemv_f64" : "gemv_f32");
-    hipLaunchKernelGGL(k_gemv<T>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, A, n, ld, x, y, alpha,
-                       beta);
+    const dim3 grid((unsigned)((n + 4 * GEMV_R - 1) / (4 * GEMV_R)));
+    if (((uintptr_t)x & 15) == 0 && ((uintptr_t)A & 15) == 0)
+        hipLaunchKernelGGL((k_gemv<T, true>), grid, dim3(256), 0, ctx->stream, A, n, ld, x, y, alpha, beta);
+    else if (((uintptr_t)A & 15) == 0)
+        hipLaunchKernelGGL((k_gemv<T, false>), grid, dim3(256), 0, ctx->stream, A, n, ld, x, y, alpha, beta);
+    else
+        return fp_fail(ctx, FP_ERR_ARG, "gemv needs a 16-byte aligned matrix");
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }
@@ -817,6 +919,253 @@ int fp_bj_apply(fp_ctx *ctx, const float *BJ, const int *piv, int64_t n, int blo
     const int nb = (int)((n + block - 1) / block);
     ProfScope ps(ctx, "bj_apply");
     hipLaunchKernelGGL(k_bj_apply, dim3(nb), dim3(256), 0, ctx->stream, BJ, piv, n, block, x, y);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+// Explicit inverse of every diagonal block from its LU factors:
+// inv = U^-1 L^-1 P, 64 columns per workgroup, the columns in LDS.
+constexpr int BJI_MAX = 512;
+__global__ __launch_bounds__(256) void k_bj_invert(const float *__restrict__ BJ, const int *__restrict__ piv,
+                                                   int64_t n, int block, float *__restrict__ inv)
+{
+    __shared__ float X[BJI_MAX][65];
+    __shared__ int perm[BJI_MAX];
+    const int b = blockIdx.x;
+    const int c0 = blockIdx.y * 64;
+    const int64_t b0 = (int64_t)b * block;
+    const int bs = (int)min((int64_t)block, n - b0);
+    if (c0 >= bs) return;
+    const float *M = BJ + (int64_t)b * block * block;
+    const int *pv = piv + (int64_t)b * block;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        for (int r = 0; r < bs; ++r) perm[r] = r;
+        for (int c = 0; c < bs; ++c) {
+            const int p = pv[c];
+            if (p != c && p >= 0 && p < bs) { const int t = perm[c]; perm[c] = perm[p]; perm[p] = t; }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < bs * 64; e += blockDim.x) {
+        const int r = e / 64, c = e % 64;
+        X[r][c] = (perm[r] == c0 + c) ? 1.f : 0.f;
+    }
+    // forward, unit lower
+    for (int q = 0; q < bs - 1; ++q) {
+        __syncthreads();
+        const int rows = bs - q - 1;
+        for (int e = tid; e < rows * 64; e += blockDim.x) {
+            const int r = q + 1 + e / 64, c = e % 64;
+            X[r][c] -= M[(int64_t)r * block + q] * X[q][c];
+        }
+    }
+    // backward, upper
+    for (int q = bs - 1; q >= 0; --q) {
+        __syncthreads();
+        if (tid < 64) X[q][tid] /= M[(int64_t)q * block + q];
+        __syncthreads();
+        for (int e = tid; e < q * 64; e += blockDim.x) {
+            const int r = e / 64, c = e % 64;
+            X[r][c] -= M[(int64_t)r * block + q] * X[q][c];
+        }
+    }
+    __syncthreads();
+    float *I = inv + (int64_t)b * block * block;
+    for (int e = tid; e < bs * 64; e += blockDim.x) {
+        const int r = e / 64, c = e % 64;
+        if (c0 + c < bs) I[(int64_t)r * block + c0 + c] = X[r][c];
+    }
+}
+
+int fp_bj_invert(fp_ctx *ctx, const float *BJ, const int *piv, int64_t n, int block, float *inv)
+{
+    if (block > BJI_MAX) return fp_fail(ctx, FP_ERR_ARG, "block-Jacobi inverse needs block <= 512");
+    const int nb = (int)((n + block - 1) / block);
+    ProfScope ps(ctx, "bj_invert");
+    hipLaunchKernelGGL(k_bj_invert, dim3(nb, (block + 63) / 64), dim3(256), 0, ctx->stream, BJ, piv, n, block, inv);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+// y_b = inv_b x_b for every diagonal block: one wave per row.
+__global__ __launch_bounds__(256) void k_bj_apply_inv(const float *__restrict__ inv, int64_t n, int block,
+                                                      const double *__restrict__ x, double *__restrict__ y,
+                                                      int accumulate)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    const int64_t b = r / block, b0 = b * block;
+    const int bs = (int)min((int64_t)block, n - b0);
+    const float *row = inv + b * block * block + (r - b0) * block;
+    double acc = 0;
+    for (int c = lane; c < bs; c += 64) acc += (double)row[c] * x[b0 + c];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) y[r] = accumulate ? y[r] + acc : acc;
+}
+
+int fp_bj_apply_inv(fp_ctx *ctx, const float *inv, int64_t n, int block, const double *x, double *y, int accumulate)
+{
+    ProfScope ps(ctx, "bj_apply");
+    hipLaunchKernelGGL(k_bj_apply_inv, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, inv, n, block, x, y,
+                       accumulate);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+// ----------------------------------------------------------------------------
+// Coarse space of chunk indicators Z (column k = 1 on rows [k*cs, (k+1)*cs)).
+// AZ = A Z: segmented row sums of A, one wave per row.
+template <typename T>
+__global__ __launch_bounds__(256) void k_chunk_rowsum(const T *__restrict__ A, int64_t n, int64_t ld, int cs, int nc,
+                                                      double *__restrict__ AZ)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    const T *row = A + r * ld;
+    for (int k = 0; k < nc; ++k) {
+        const int64_t c0 = (int64_t)k * cs, c1 = min(n, c0 + cs);
+        double acc = 0;
+        for (int64_t c = c0 + lane; c < c1; c += 64) acc += (double)row[c];
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+        if (lane == 0) AZ[(int64_t)k * n + r] = acc;
+    }
+}
+
+// E = Z^T AZ (nc x nc, row-major): E[k'][k] = sum over chunk k' of AZ[:, k]
+__global__ void k_coarse_galerkin(const double *__restrict__ AZ, int64_t n, int cs, int nc, double *__restrict__ E)
+{
+    const int kp = blockIdx.x, k = blockIdx.y;
+    __shared__ double red[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t r0 = (int64_t)kp * cs, r1 = min(n, r0 + cs);
+    double acc = 0;
+    for (int64_t r = r0 + tid; r < r1; r += blockDim.x) acc += AZ[(int64_t)k * n + r];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) red[wave] = acc;
+    __syncthreads();
+    if (tid == 0) E[(int64_t)kp * nc + k] = red[0] + red[1] + red[2] + red[3];
+}
+
+// In-place inverse of a small dense matrix (m <= 1024, row-major) by
+// Gauss-Jordan elimination with partial pivoting; one workgroup.
+__global__ __launch_bounds__(1024) void k_small_inverse(double *__restrict__ E, int m, double *__restrict__ work,
+                                                        int *__restrict__ info)
+{
+    // work: m x 2m augmented matrix [E | I]
+    const int tid = threadIdx.x;
+    const int w2 = 2 * m;
+    __shared__ int prow;
+    __shared__ double pval;
+    for (int e = tid; e < m * w2; e += blockDim.x) {
+        const int r = e / w2, c = e % w2;
+        work[e] = c < m ? E[(int64_t)r * m + c] : (c - m == r ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    for (int c = 0; c < m; ++c) {
+        if (tid == 0) {
+            int p = c;
+            double best = fabs(work[(int64_t)c * w2 + c]);
+            for (int r = c + 1; r < m; ++r) {
+                const double v = fabs(work[(int64_t)r * w2 + c]);
+                if (v > best) { best = v; p = r; }
+            }
+            prow = p;
+            pval = work[(int64_t)p * w2 + c];
+            if (pval == 0.0 && *info == 0) *info = c + 1;
+        }
+        __syncthreads();
+        const int p = prow;
+        if (p != c)
+            for (int e = tid; e < w2; e += blockDim.x) {
+                const double t = work[(int64_t)c * w2 + e];
+                work[(int64_t)c * w2 + e] = work[(int64_t)p * w2 + e];
+                work[(int64_t)p * w2 + e] = t;
+            }
+        __syncthreads();
+        const double inv = pval != 0.0 ? 1.0 / pval : 0.0;
+        for (int e = tid; e < w2; e += blockDim.x) work[(int64_t)c * w2 + e] *= inv;
+        __syncthreads();
+        for (int e = tid; e < m * w2; e += blockDim.x) {
+            const int r = e / w2, cc = e % w2;
+            if (r != c) {
+                const double f = work[(int64_t)r * w2 + c];
+                if (cc != c) work[e] -= f * work[(int64_t)c * w2 + cc];
+            }
+        }
+        __syncthreads();
+        for (int r = tid; r < m; r += blockDim.x)
+            if (r != c) work[(int64_t)r * w2 + c] = 0.0;
+        __syncthreads();
+    }
+    for (int e = tid; e < m * m; e += blockDim.x) {
+        const int r = e / m, c = e % m;
+        E[e] = work[(int64_t)r * w2 + m + c];
+    }
+}
+
+template <typename T>
+int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, double *AZ, double *Einv, double *work,
+                    int *info_dev)
+{
+    const int nc = (int)((n + cs - 1) / cs);
+    if (nc > 1024) return fp_fail(ctx, FP_ERR_ARG, "coarse space larger than 1024 vectors");
+    ProfScope ps(ctx, "coarse_setup");
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, info_dev, (int64_t)1, 0);
+    hipLaunchKernelGGL(k_chunk_rowsum<T>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, A, n, ld, cs, nc,
+                       AZ);
+    hipLaunchKernelGGL(k_coarse_galerkin, dim3(nc, nc), dim3(256), 0, ctx->stream, (const double *)AZ, n, cs, nc, Einv);
+    hipLaunchKernelGGL(k_small_inverse, dim3(1), dim3(1024), 0, ctx->stream, Einv, nc, work, info_dev);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+template int fp_coarse_setup<double>(fp_ctx *, const double *, int64_t, int64_t, int, double *, double *, double *, int *);
+template int fp_coarse_setup<float>(fp_ctx *, const float *, int64_t, int64_t, int, double *, double *, double *, int *);
+
+// c = Einv (Z^T v): one workgroup (restriction + coarse solve)
+__global__ __launch_bounds__(1024) void k_coarse_solve(const double *__restrict__ v, int64_t n, int cs, int nc,
+                                                       const double *__restrict__ Einv, double *__restrict__ t,
+                                                       double *__restrict__ c)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+    for (int k = wave; k < nc; k += nw) {
+        const int64_t r0 = (int64_t)k * cs, r1 = min(n, r0 + cs);
+        double acc = 0;
+        for (int64_t r = r0 + lane; r < r1; r += 64) acc += v[r];
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+        if (lane == 0) t[k] = acc;
+    }
+    __syncthreads();
+    for (int k = wave; k < nc; k += nw) {
+        double acc = 0;
+        for (int q = lane; q < nc; q += 64) acc += Einv[(int64_t)k * nc + q] * t[q];
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+        if (lane == 0) c[k] = acc;
+    }
+}
+
+// y = Z c, r2 = v - AZ c
+__global__ void k_coarse_resid(const double *__restrict__ v, const double *__restrict__ AZ, int64_t n, int cs, int nc,
+                               const double *__restrict__ c, double *__restrict__ y, double *__restrict__ r2)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0;
+    for (int k = 0; k < nc; ++k) acc += AZ[(int64_t)k * n + i] * c[k];
+    r2[i] = v[i] - acc;
+    y[i] = c[i / cs];
+}
+
+int fp_coarse_apply(fp_ctx *ctx, const double *v, int64_t n, int cs, const double *AZ, const double *Einv, double *tc,
+                    double *y, double *r2)
+{
+    const int nc = (int)((n + cs - 1) / cs);
+    ProfScope ps(ctx, "coarse_apply");
+    hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(1024), 0, ctx->stream, v, n, cs, nc, Einv, tc, tc + nc);
+    hipLaunchKernelGGL(k_coarse_resid, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, v, AZ, n, cs, nc,
+                       (const double *)(tc + nc), y, r2);
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }

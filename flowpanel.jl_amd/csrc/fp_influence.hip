@@ -246,6 +246,10 @@ __global__ __launch_bounds__(256) void k_assemble_wake(HorseView hv, double off_
     }
 }
 
+// The entries are always computed in FP64 arithmetic and rounded once to the
+// storage type T: an FP32 kernel loses about log10(R/h) digits in the far
+// field (r_i/|r_i| - r_j/|r_j| and r_i x r_j cancel), which would limit the
+// refinement of the mixed-precision solve.
 template <typename T>
 int fp_launch_assemble(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G, int64_t ld, int64_t col0)
 {
@@ -256,24 +260,24 @@ int fp_launch_assemble(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G
     if (b->element == FP_ELEM_VORTEXRING) {
         ProfScope ps(ctx, sizeof(T) == 8 ? "assemble_vr_f64" : "assemble_vr_f32");
         if (b->nvp == 4)
-            hipLaunchKernelGGL((k_assemble_vr<T, T, 4, TM>), grid, dim3(256), 0, ctx->stream, pan, tv, G, ld, col0);
+            hipLaunchKernelGGL((k_assemble_vr<double, T, 4, TM>), grid, dim3(256), 0, ctx->stream, pan, tv, G, ld, col0);
         else
-            hipLaunchKernelGGL((k_assemble_vr<T, T, 8, TM>), grid, dim3(256), 0, ctx->stream, pan, tv, G, ld, col0);
+            hipLaunchKernelGGL((k_assemble_vr<double, T, 8, TM>), grid, dim3(256), 0, ctx->stream, pan, tv, G, ld, col0);
         FP_HIP(ctx, hipGetLastError());
         if (b->nh > 0) {
             ProfScope pw(ctx, "assemble_wake");
             dim3 g2((unsigned)((tv.M + 255) / 256));
-            hipLaunchKernelGGL((k_assemble_wake<T, T>), g2, dim3(256), 0, ctx->stream, horse_view(b),
+            hipLaunchKernelGGL((k_assemble_wake<double, T>), g2, dim3(256), 0, ctx->stream, horse_view(b),
                                b->kernel_offset, b->kernel_cutoff, tv, G, ld, col0);
             FP_HIP(ctx, hipGetLastError());
         }
     } else {
         ProfScope ps(ctx, sizeof(T) == 8 ? "assemble_source_f64" : "assemble_source_f32");
         if (b->nvp == 4)
-            hipLaunchKernelGGL((k_assemble_source<T, T, 4, TM>), grid, dim3(256), 0, ctx->stream, pan, tv, G, ld,
+            hipLaunchKernelGGL((k_assemble_source<double, T, 4, TM>), grid, dim3(256), 0, ctx->stream, pan, tv, G, ld,
                                col0);
         else
-            hipLaunchKernelGGL((k_assemble_source<T, T, 8, TM>), grid, dim3(256), 0, ctx->stream, pan, tv, G, ld,
+            hipLaunchKernelGGL((k_assemble_source<double, T, 8, TM>), grid, dim3(256), 0, ctx->stream, pan, tv, G, ld,
                                col0);
         FP_HIP(ctx, hipGetLastError());
     }

@@ -44,6 +44,7 @@ struct fp_ctx {
     // per-kernel device timing (fp_ctx_profile)
     bool prof = false;
     std::vector<fp_prof_rec> prof_pending;
+    std::vector<hipEvent_t> prof_pool;  // recycled events
     std::map<std::string, std::pair<int64_t, double>> prof_acc;
 };
 
@@ -57,8 +58,8 @@ struct ProfScope {
         if (!c->prof) return;
         fp_prof_rec r;
         r.name = name;
-        if (hipEventCreate(&r.a) != hipSuccess) return;
-        if (hipEventCreate(&r.b) != hipSuccess) { (void)hipEventDestroy(r.a); return; }
+        if (!take(r.a)) return;
+        if (!take(r.b)) { c->prof_pool.push_back(r.a); return; }
         (void)hipEventRecord(r.a, c->stream);
         c->prof_pending.push_back(r);
         idx = (int)c->prof_pending.size() - 1;
@@ -66,6 +67,11 @@ struct ProfScope {
     ~ProfScope()
     {
         if (idx >= 0) (void)hipEventRecord(c->prof_pending[idx].b, c->stream);
+    }
+    bool take(hipEvent_t &e)
+    {
+        if (!c->prof_pool.empty()) { e = c->prof_pool.back(); c->prof_pool.pop_back(); return true; }
+        return hipEventCreate(&e) == hipSuccess;
     }
 };
 
@@ -182,6 +188,16 @@ int fp_gemv(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, const double *x, dou
 template <typename T>
 int fp_bj_factor(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *BJ, int *piv, int *info_dev);
 int fp_bj_apply(fp_ctx *ctx, const float *BJ, const int *piv, int64_t n, int block, const double *x, double *y);
+// explicit block inverses (block <= 512) and their application
+int fp_bj_invert(fp_ctx *ctx, const float *BJ, const int *piv, int64_t n, int block, float *inv);
+int fp_bj_apply_inv(fp_ctx *ctx, const float *inv, int64_t n, int block, const double *x, double *y, int accumulate);
+// Galerkin coarse space of chunk indicators: AZ = A Z (n x nc), Einv = (Z^T A Z)^-1
+template <typename T>
+int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, double *AZ, double *Einv, double *work,
+                    int *info_dev);
+// y = Z Einv Z^T v and r2 = v - AZ Einv Z^T v (tc: 2*nc scratch)
+int fp_coarse_apply(fp_ctx *ctx, const double *v, int64_t n, int cs, const double *AZ, const double *Einv, double *tc,
+                    double *y, double *r2);
 // small vector kernels
 int fp_vec_copy_convert(fp_ctx *ctx, const double *src, void *dst, int64_t n, int dst_prec);
 int fp_mat_convert(fp_ctx *ctx, const double *src, int64_t n, int64_t lds, void *dst, int64_t ldd, int dst_prec,

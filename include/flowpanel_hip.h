@@ -84,7 +84,15 @@ typedef struct fp_solve_opts {
     int max_iters;     /* cap on the total number of GMRES iterations                     */
     int max_refine;    /* cap on refinement steps                                          */
     int precond_block; /* block-Jacobi block size (GMRES_IR, GMRES)                       */
+    int coarse_block;  /* > 0: add a Galerkin coarse correction on the indicator vectors of
+                          consecutive chunks of this many panels (e.g. one chordwise strip)   */
+    int residual_mode; /* refinement residual r = b - A x in FP64 from: FP_RESID_STORED (the
+                          FP64 AIC kept beside the FP32 copy) or FP_RESID_MATRIX_FREE
+                          (influences recomputed in FP64, no FP64 matrix in memory)          */
 } fp_solve_opts;
+
+#define FP_RESID_MATRIX_FREE 0
+#define FP_RESID_STORED 1
 
 typedef struct fp_solve_stats {
     int refinements;   /* FP64 refinement steps taken                                      */

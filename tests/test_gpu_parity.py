@@ -135,6 +135,19 @@ def test_solve_wing_matches_oracle(fp, orc, wing, solver):
     assert stats["residual"] < 1e-11
 
 
+@pytest.mark.parametrize("solver", ["solve_gmres_ir", "solve_gmres"])
+def test_two_level_preconditioner_matches_oracle(fp, orc, wing, solver):
+    """Block-Jacobi + Galerkin coarse correction on one-strip chunks (24 panels)."""
+    U, D = _freestream(wing)
+    opt = {"precond_block": 48, "coarse_block": 24, "tol": 1e-12}
+    stats = fp.solve(wing, U, D, D, solver=getattr(fp, solver), solver_optargs=opt)
+    Go, normals, cps = orc.assemble([_obody(wing)], [D], [D])
+    gam = orc.solve_lu(Go, orc.rhs(U, normals))
+    assert rel(wing.strength, gam) < TOL64, stats
+    plain = fp.solve(wing, U, D, D, solver=getattr(fp, solver), solver_optargs={"precond_block": 48, "tol": 1e-12})
+    assert stats["iterations"] <= plain["iterations"]
+
+
 def test_solve_sphere_and_cp(fp, orc, sphere):
     U, _ = _freestream(sphere, alpha=0.0)
     fp.solve(sphere, U)
