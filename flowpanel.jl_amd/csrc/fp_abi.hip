@@ -346,34 +346,35 @@ struct Gmres {
     }
 };
 
-// LAPACK ipiv (0-based swap sequence) -> row permutation, on the host.
-int ipiv_to_perm(fp_ctx *ctx, const int *d_ipiv, int64_t n, int *d_perm)
+// Row permutation (perm[i] = original row at row i) -> LAPACK swap sequence
+// (0-based), on the host: only needed to report factors in LAPACK form.
+static std::vector<int> perm_to_ipiv(const std::vector<int> &perm)
 {
-    std::vector<int> ipiv(n), perm(n);
-    FP_HIP(ctx, hipMemcpyAsync(ipiv.data(), d_ipiv, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    for (int64_t i = 0; i < n; ++i) perm[i] = (int)i;
+    const int64_t n = (int64_t)perm.size();
+    std::vector<int> cur(n), where(n), ipiv(n);
+    for (int64_t i = 0; i < n; ++i) { cur[i] = (int)i; where[i] = (int)i; }
     for (int64_t i = 0; i < n; ++i) {
-        const int p = ipiv[i];
-        if (p < 0 || p >= n) return fp_fail(ctx, FP_ERR_DEVICE, "corrupt pivot from getrf");
-        if (p != i) std::swap(perm[i], perm[p]);
+        const int want = perm[i];
+        const int p = where[want];
+        ipiv[i] = p;
+        const int other = cur[i];
+        cur[p] = other; where[other] = p;
+        cur[i] = want; where[want] = (int)i;
     }
-    FP_HIP(ctx, hipMemcpyAsync(d_perm, perm.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-    return FP_OK;
+    return ipiv;
 }
 
 template <typename T>
-int factor(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *d_ipiv, int *d_perm, int *info_out)
+int factor(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *d_perm, int *info_out)
 {
     DBuf work, info;
     int st;
     if ((st = dalloc(ctx, work, fp_getrf_work_bytes(n, sizeof(T))))) return st;
     if ((st = dalloc(ctx, info, sizeof(int)))) return st;
-    if ((st = fp_getrf<T>(ctx, A, n, ld, d_ipiv, info.as<int>(), work.p, fp_getrf_work_bytes(n, sizeof(T)))))
+    if ((st = fp_getrf<T>(ctx, A, n, ld, d_perm, info.as<int>(), work.p, fp_getrf_work_bytes(n, sizeof(T)))))
         return st;
     int hinfo = 0;
     FP_HIP(ctx, hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    if ((st = ipiv_to_perm(ctx, d_ipiv, n, d_perm))) return st;
     FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (info_out) *info_out = hinfo;
     if (hinfo) {
@@ -399,14 +400,13 @@ int solve_core(fp_ctx *ctx, const SolveArgs &a, const fp_solve_opts &o, const do
 {
     const int64_t n = a.n;
     int st;
-    DBuf ipiv, perm, work, r, d, part, dout;
+    DBuf perm, work, r, d, part, dout;
     if ((st = dalloc(ctx, part, 256 * 4 * sizeof(double)))) return st;
     if ((st = dalloc(ctx, dout, 4 * sizeof(double)))) return st;
     Timer tm(ctx->stream);
     if (a.solver == FP_SOLVER_LU) {
-        if ((st = dalloc(ctx, ipiv, n * sizeof(int)))) return st;
         if ((st = dalloc(ctx, perm, n * sizeof(int)))) return st;
-        if ((st = factor<double>(ctx, a.A64, n, a.ld, ipiv.as<int>(), perm.as<int>(), &stats.info))) return st;
+        if ((st = factor<double>(ctx, a.A64, n, a.ld, perm.as<int>(), &stats.info))) return st;
         stats.ms_factor = tm.stop();
         tm.restart();
         if ((st = dalloc(ctx, work, fp_getrs_work_bytes(n, 1)))) return st;
@@ -463,9 +463,8 @@ int solve_core(fp_ctx *ctx, const SolveArgs &a, const fp_solve_opts &o, const do
     std::unique_ptr<DenseOp<float>> A32op;
     DBuf bj, bjp, info;
     if (a.solver == FP_SOLVER_LU32_IR) {
-        if ((st = dalloc(ctx, ipiv, n * sizeof(int)))) return st;
         if ((st = dalloc(ctx, perm, n * sizeof(int)))) return st;
-        if ((st = factor<float>(ctx, a.A32, n, a.ld, ipiv.as<int>(), perm.as<int>(), &stats.info))) return st;
+        if ((st = factor<float>(ctx, a.A32, n, a.ld, perm.as<int>(), &stats.info))) return st;
         if ((st = dalloc(ctx, work, fp_getrs_work_bytes(n, 1)))) return st;
     } else if (a.solver == FP_SOLVER_GMRES_IR) {
         const int blk = (int)std::min<int64_t>(o.precond_block > 0 ? o.precond_block : 512, n);
@@ -1069,8 +1068,8 @@ int fp_mat_getrf(fp_ctx *ctx, fp_mat *m, int precision, double *LU, int64_t *ipi
         return fp_fail(ctx, FP_ERR_UNSUPPORTED, "FP64 factors of an FP32 matrix");
     }
     int info = 0;
-    st = precision == FP_F64 ? factor<double>(ctx, F.as<double>(), n, ld, p.as<int>(), pm.as<int>(), &info)
-                             : factor<float>(ctx, F.as<float>(), n, ld, p.as<int>(), pm.as<int>(), &info);
+    st = precision == FP_F64 ? factor<double>(ctx, F.as<double>(), n, ld, pm.as<int>(), &info)
+                             : factor<float>(ctx, F.as<float>(), n, ld, pm.as<int>(), &info);
     if (st && st != FP_ERR_SINGULAR) return st;
     if ((st == FP_OK || st == FP_ERR_SINGULAR)) {
         int st2 = dalloc(ctx, tmp, (size_t)n * n * sizeof(double));
@@ -1083,9 +1082,10 @@ int fp_mat_getrf(fp_ctx *ctx, fp_mat *m, int precision, double *LU, int64_t *ipi
             hipLaunchKernelGGL(k_download_cm<float>, grid, dim3(256), 0, ctx->stream, F.as<float>(), n, ld,
                                tmp.as<double>());
         FP_HIP(ctx, hipMemcpyAsync(LU, tmp.p, (size_t)n * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        std::vector<int> ip(n);
-        FP_HIP(ctx, hipMemcpyAsync(ip.data(), p.p, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        std::vector<int> pr(n);
+        FP_HIP(ctx, hipMemcpyAsync(pr.data(), pm.p, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        const std::vector<int> ip = perm_to_ipiv(pr);
         for (int64_t i = 0; i < n; ++i) ipiv[i] = ip[i] + 1;
     }
     return st;

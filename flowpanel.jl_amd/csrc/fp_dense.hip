@@ -256,39 +256,38 @@ __global__ void k_laswp(T *__restrict__ A, int64_t ld, int64_t ncols, int64_t ld
 // ----------------------------------------------------------------------------
 // Tournament pivoting.
 constexpr int PW = 32;      // sub-panel width
-constexpr int CH = 256;     // rows per candidate chunk (and per merge)
-constexpr int MERGE_F = CH / PW;
+constexpr int CH = 256;     // rows per candidate chunk
+constexpr int MR = 512;     // rows per merge (16 candidate sets)
+constexpr int MERGE_F = MR / PW;
 
-// Partial pivoting of a (rows x cols) tile in LDS, rows <= 256, cols <= PW.
-// key[] breaks ties (smaller wins, as LAPACK's first maximum); rows are
-// permuted in place together with key[] and tag[].  Returns (via *sing) 1 if
-// some pivot was exactly zero.
+// Partial pivoting of a (rows x cols) tile in LDS (cols <= PW).  The pivot
+// of each column is found with one 64-bit LDS atomicMax per row on the packed
+// key (float(|v|) bits, ~row): largest magnitude first, the lower tile row on
+// ties (a NaN counts as zero).  Rows are permuted in place together with key[]
+// and tag[].  *sing receives 1 + the first column whose pivot was exactly zero.
+__device__ __forceinline__ unsigned long long pivot_key(double v, int r)
+{
+    float a = (float)fabs(v);
+    if (!(a == a)) a = 0.f;
+    return ((unsigned long long)__float_as_uint(a) << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)r);
+}
+
 template <typename T>
 __device__ void tile_gepp(T (*tile)[PW + 1], int *key, int *tag, int rows, int cols, int *sing)
 {
-    __shared__ double red_v[4];
-    __shared__ int red_k[4], red_r[4];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ unsigned long long amax[2];
+    const int tid = threadIdx.x;
+    if (tid == 0) { amax[0] = 0ull; amax[1] = 0ull; }
+    __syncthreads();
     const int nc = min(rows, cols);
     for (int c = 0; c < nc; ++c) {
-        double bv = -1.0;
-        int bk = INT_MAX, br = -1;
-        for (int r = c + tid; r < rows; r += blockDim.x) {
-            const double v = fabs((double)tile[r][c]);
-            if (v > bv || (v == bv && key[r] < bk)) { bv = v; bk = key[r]; br = r; }
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            const double ov = __shfl_xor(bv, o);
-            const int ok = __shfl_xor(bk, o), orr = __shfl_xor(br, o);
-            if (ov > bv || (ov == bv && ok < bk)) { bv = ov; bk = ok; br = orr; }
-        }
-        if (lane == 0) { red_v[wave] = bv; red_k[wave] = bk; red_r[wave] = br; }
+        const int slot = c & 1;
+        for (int r = c + tid; r < rows; r += blockDim.x) atomicMax(&amax[slot], pivot_key((double)tile[r][c], r));
         __syncthreads();
-        bv = red_v[0]; bk = red_k[0]; br = red_r[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
-            if (red_v[w] > bv || (red_v[w] == bv && red_k[w] < bk)) { bv = red_v[w]; bk = red_k[w]; br = red_r[w]; }
-        __syncthreads();
-        const int p = (br >= c && br < rows) ? br : c;  // all-NaN column: keep the row
+        const unsigned long long k = amax[slot];
+        int p = (int)(0xFFFFFFFFu - (unsigned)(k & 0xFFFFFFFFull));
+        if (k == 0ull || p < c || p >= rows) p = c;
+        if (tid == 0) amax[slot ^ 1] = 0ull;
         if (p != c) {
             for (int e = tid; e < cols; e += blockDim.x) {
                 const T t = tile[c][e]; tile[c][e] = tile[p][e]; tile[p][e] = t;
@@ -303,10 +302,13 @@ __device__ void tile_gepp(T (*tile)[PW + 1], int *key, int *tag, int rows, int c
         if (piv == T(0)) {
             if (tid == 0 && *sing == 0) *sing = c + 1;
         } else {
+            const T rp = T(1) / piv;
+            const int w = cols - c - 1;
+            // (rows-c-1) x w updates plus the multipliers, spread over all threads
             for (int r = c + 1 + tid; r < rows; r += blockDim.x) {
-                const T l = tile[r][c] / piv;
+                const T l = tile[r][c] * rp;
                 tile[r][c] = l;
-                for (int e = c + 1; e < cols; ++e) tile[r][e] -= l * tile[c][e];
+                for (int e = 0; e < w; ++e) tile[r][c + 1 + e] -= l * tile[c][c + 1 + e];
             }
         }
         __syncthreads();
@@ -334,6 +336,7 @@ __global__ __launch_bounds__(256) void k_tslu_local(const T *__restrict__ A, int
     for (int r = tid; r < CH; r += blockDim.x) { key[r] = r < rows ? r0 + r : INT_MAX; tag[r] = r; }
     __syncthreads();
     tile_gepp<T>(tile, key, tag, rows, pw, &sing);
+    __syncthreads();
     // candidates: the first min(rows, pw) rows, original values re-read
     for (int e = tid; e < PW * PW; e += blockDim.x) {
         const int r = e / PW, c = e % PW;
@@ -348,26 +351,27 @@ __global__ __launch_bounds__(256) void k_tslu_local(const T *__restrict__ A, int
 // Merge MERGE_F candidate sets per workgroup.  When `final_out` is set (one
 // workgroup left) it also writes the factored pw x pw block and the winners.
 template <typename T>
-__global__ __launch_bounds__(256) void k_tslu_merge(int nsets, int pw, const T *__restrict__ ival,
+__global__ __launch_bounds__(512) void k_tslu_merge(int nsets, int pw, const T *__restrict__ ival,
                                                     const int *__restrict__ iidx, T *__restrict__ oval,
                                                     int *__restrict__ oidx, T *__restrict__ final_lu,
                                                     int *__restrict__ info, int s)
 {
-    __shared__ T tile[CH][PW + 1];
-    __shared__ int key[CH], tag[CH];
+    __shared__ T tile[MR][PW + 1];
+    __shared__ int key[MR], tag[MR];
     __shared__ int sing;
     const int g = blockIdx.x, tid = threadIdx.x;
     const int set0 = g * MERGE_F;
     const int nset = min(MERGE_F, nsets - set0);
     const int rows = nset * PW;
     if (tid == 0) sing = 0;
-    for (int e = tid; e < CH * PW; e += blockDim.x) {
+    for (int e = tid; e < MR * PW; e += blockDim.x) {
         const int r = e / PW, c = e % PW;
         tile[r][c] = r < rows ? ival[((int64_t)set0 * PW + r) * PW + c] : T(0);
     }
-    for (int r = tid; r < CH; r += blockDim.x) { key[r] = r < rows ? iidx[set0 * PW + r] : INT_MAX; tag[r] = r; }
+    for (int r = tid; r < MR; r += blockDim.x) { key[r] = r < rows ? iidx[set0 * PW + r] : INT_MAX; tag[r] = r; }
     __syncthreads();
     tile_gepp<T>(tile, key, tag, rows, pw, &sing);
+    __syncthreads();
     for (int e = tid; e < PW * PW; e += blockDim.x) {
         const int r = e / PW, c = e % PW;
         oval[((int64_t)g * PW + r) * PW + c] = ival[((int64_t)set0 * PW + tag[r]) * PW + c];
@@ -377,53 +381,60 @@ __global__ __launch_bounds__(256) void k_tslu_merge(int nsets, int pw, const T *
     if (final_lu && tid == 0 && sing && *info == 0) *info = s + sing;
 }
 
-// Winners (in pivot order) -> LAPACK swap sequence ipiv[s..s+pw) and the
-// equivalent row map (final position <- original row, at most 2*pw rows move),
-// computed by one thread.  map[0] = count, then (dst, src) pairs.
-__global__ void k_tslu_ipiv(const int *__restrict__ win, int s, int pw, int n, int *__restrict__ ipiv,
-                            int *__restrict__ map)
+// Winners (in pivot order) -> row map: position s+i <- row w_i, and the top
+// rows [s, s+pw) that are not winners move to the positions the winners left
+// outside the top block (paired in order).  One wave; map[0] = count, then
+// (dst, src) pairs.  The same map is applied to the running permutation
+// perm (perm[pos] = original row now at pos).
+__global__ __launch_bounds__(64) void k_tslu_rowmap(const int *__restrict__ win, int s, int pw, int n,
+                                                    int *__restrict__ map, int *__restrict__ perm)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int mrow[2 * PW], mpos[2 * PW];  // rows that moved and where they are
-    int nm = 0;
-    auto loc = [&](int row) {
-        for (int q = 0; q < nm; ++q)
-            if (mrow[q] == row) return mpos[q];
-        return row;
-    };
-    auto set = [&](int row, int pos) {
-        for (int q = 0; q < nm; ++q)
-            if (mrow[q] == row) { mpos[q] = pos; return; }
-        mrow[nm] = row; mpos[nm] = pos; ++nm;
-    };
-    auto who = [&](int pos) {
-        for (int q = 0; q < nm; ++q)
-            if (mpos[q] == pos) return mrow[q];
-        // a position not in the table still holds its own row unless that row moved
-        for (int q = 0; q < nm; ++q)
-            if (mrow[q] == pos) return -1;
-        return pos;
-    };
-    for (int i = 0; i < pw; ++i) {
-        const int t = s + i;
-        const int w = win[i];
-        if (w < s || w >= n) { ipiv[t] = t; continue; }  // cannot happen for m >= pw; keep rows in place
-        const int p = loc(w);
-        ipiv[t] = p;
-        if (p != t) {
-            const int y = who(t);
-            set(w, t);
-            if (y >= 0) set(y, p);
-        }
+    __shared__ int w[PW], intop[PW], vac[PW], dis[PW];
+    __shared__ int nv, nd;
+    const int lane = threadIdx.x;
+    if (lane < PW) intop[lane] = 0;
+    if (lane == 0) { nv = 0; nd = 0; }
+    __syncthreads();
+    if (lane < pw) {
+        int wi = win[lane];
+        if (wi < s || wi >= n) wi = s + lane;  // cannot happen for m >= pw
+        w[lane] = wi;
+        if (wi < s + pw) intop[wi - s] = 1;
     }
-    int cnt = 0;
-    for (int q = 0; q < nm; ++q)
-        if (mpos[q] != mrow[q]) {
-            map[1 + 2 * cnt] = mpos[q];
-            map[2 + 2 * cnt] = mrow[q];
-            ++cnt;
-        }
-    map[0] = cnt;
+    __syncthreads();
+    if (lane == 0) {
+        for (int i = 0; i < pw; ++i)
+            if (w[i] >= s + pw) vac[nv++] = w[i];
+        for (int t = 0; t < pw; ++t)
+            if (!intop[t]) dis[nd++] = s + t;
+    }
+    __syncthreads();
+    // pairs: (s+i <- w_i) for w_i != s+i, then (vac_k <- dis_k)
+    __shared__ int dst[2 * PW], src[2 * PW];
+    __shared__ int cnt;
+    if (lane == 0) {
+        int c = 0;
+        for (int i = 0; i < pw; ++i)
+            if (w[i] != s + i) { dst[c] = s + i; src[c] = w[i]; ++c; }
+        for (int k = 0; k < nv && k < nd; ++k) { dst[c] = vac[k]; src[c] = dis[k]; ++c; }
+        cnt = c;
+        map[0] = c;
+    }
+    __syncthreads();
+    int pv = 0;
+    if (lane < cnt) {
+        map[1 + 2 * lane] = dst[lane];
+        map[2 + 2 * lane] = src[lane];
+        pv = perm[src[lane]];
+    }
+    __syncthreads();
+    if (lane < cnt) perm[dst[lane]] = pv;
+}
+
+__global__ void k_iota(int *p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (int)i;
 }
 
 // Apply a row map (A[dst] <- A[src] for every pair) to every column: one lane
@@ -491,8 +502,9 @@ size_t fp_getrf_work_bytes(int64_t n, int elem_bytes)
            (4 * PW + 16) * sizeof(int) + 256;
 }
 
+// Factors P A = L U in place; perm[i] is the original row now at row i.
 template <typename T>
-int fp_getrf(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *ipiv, int *info_dev, void *work, size_t work_bytes)
+int fp_getrf(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *perm, int *info_dev, void *work, size_t work_bytes)
 {
     if (n <= 0) return FP_OK;
     if (work_bytes < fp_getrf_work_bytes(n, sizeof(T))) return fp_fail(ctx, FP_ERR_ARG, "getrf workspace too small");
@@ -506,6 +518,7 @@ int fp_getrf(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *ipiv, int *info_dev,
     T *flu = (T *)w; w += PW * PW * sizeof(T);
     int *rmap = (int *)w;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, info_dev, (int64_t)1, 0);
+    hipLaunchKernelGGL(k_iota, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, perm, n);
     const int NB = 256;
     for (int64_t k = 0; k < n; k += NB) {
         const int nb = (int)std::min<int64_t>(NB, n - k);
@@ -522,7 +535,7 @@ int fp_getrf(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *ipiv, int *info_dev,
             for (;;) {
                 const int go = (g + MERGE_F - 1) / MERGE_F;
                 const bool last = go == 1;
-                hipLaunchKernelGGL(k_tslu_merge<T>, dim3(go), dim3(256), 0, ctx->stream, g, pw, (const T *)iv,
+                hipLaunchKernelGGL(k_tslu_merge<T>, dim3(go), dim3(512), 0, ctx->stream, g, pw, (const T *)iv,
                                    (const int *)ii, ov, oi, last ? flu : (T *)nullptr, info_dev, (int)s);
                 std::swap(iv, ov);
                 std::swap(ii, oi);
@@ -530,8 +543,8 @@ int fp_getrf(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *ipiv, int *info_dev,
                 if (last) break;
             }
             // ii now holds the pw winners in pivot order
-            hipLaunchKernelGGL(k_tslu_ipiv, dim3(1), dim3(64), 0, ctx->stream, (const int *)ii, (int)s, pw, (int)n, ipiv,
-                               rmap);
+            hipLaunchKernelGGL(k_tslu_rowmap, dim3(1), dim3(64), 0, ctx->stream, (const int *)ii, (int)s, pw, (int)n,
+                               rmap, perm);
             hipLaunchKernelGGL(k_rowmap<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, A, ld, n,
                                (const int *)rmap);
             hipLaunchKernelGGL(k_tslu_l21<T>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, ctx->stream, A, ld,

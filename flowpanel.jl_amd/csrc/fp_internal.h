@@ -173,8 +173,9 @@ int fp_launch_field(fp_ctx *ctx, const fp_body *b, const double *strength, const
 size_t fp_field_partial_bytes(int64_t M, int64_t N);
 
 // ---- dense linear algebra (fp_dense.hip) --------------------------------
+// P A = L U in place (row-major); perm[i] = original row now at row i.
 template <typename T>
-int fp_getrf(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *ipiv, int *info_dev, void *work, size_t work_bytes);
+int fp_getrf(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *perm, int *info_dev, void *work, size_t work_bytes);
 size_t fp_getrf_work_bytes(int64_t n, int elem_bytes);
 // solve with LU factors: x (FP64, n x k SoA columns, ldx) overwritten
 template <typename T>
