@@ -28,6 +28,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per k_gemv<float> launch at 30,000 panels from rocprofv3 FETCH_SIZE/WRITE_SIZE, scaled by the
+# same counters on a 1 GiB plain copy (FETCH x2.0, WRITE x1.0): profiles/gemv_f32_bwb30k_hbm_counters.md
+GEMV_TRAFFIC_30K = 3.6052e9
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 
@@ -85,7 +88,7 @@ def vendor_lu(fp, body, N):
     try:
         import torch
         if not torch.cuda.is_available():
-            return None
+            return {"error": "torch sees no GPU (device_count %d)" % torch.cuda.device_count()}
         G = fp.assemble(body, "f64")
         A = torch.from_numpy(G).cuda()
         del G
@@ -116,6 +119,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10, help="FP64 relative residual target of the mixed solve")
     ap.add_argument("--inner-tol", type=float, default=1e-5, help="relative tolerance of each inner FP32 GMRES")
     ap.add_argument("--vendor", action="store_true", help="also time rocSOLVER FP64 getrf through torch")
+    ap.add_argument("--mixed-only", action="store_true", help="profiling runs: mixed steps only (no FP64 LU, no CPU)")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="profiling runs: also run a plain 1 GiB device copy (torch) to calibrate HBM counters")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -156,6 +162,14 @@ def main():
     gam_mixed = body.strength.copy()
     Us_m = fp.calcfield_U(body, body)
     cp_m = fp.calcfield_Cp(body, 1.0)
+    if args.copy_calib:
+        ctx.check(fp._native.lib().fp_ctx_copy_calibrate(ctx.handle, 1 << 30))
+        log("bench: copy calibration: 1 GiB read + 1 GiB written (__amd_rocclr_copyBuffer)")
+    if args.mixed_only:
+        launches, ms = prof["gemv_f32"]
+        log("bench: gemv_f32 %d launches avg %.4f ms" % (launches, ms / max(launches, 1)))
+        print(json.dumps({"mixed_ms": round(t_mixed, 3), "stats": stats[-1]}), flush=True)
+        return
 
     log("bench: fp64 LU")
     ctx.profile(True)
@@ -183,7 +197,8 @@ def main():
     gemv_bytes = 4.0 * N * N + 16.0 * N
     achieved = gemv_bytes / (avg_ms * 1e-3) / 1e9 if launches else 0.0
     roof = {"bound": "hbm", "kernel": "k_gemv<float>", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-            "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": GEMV_TRAFFIC_30K if N == 30000 else None,
             "avg_launch_ms": round(avg_ms, 4), "launches": launches,
             "algorithmic_bytes_per_launch": gemv_bytes}
     kernels = {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in prof.items()}

@@ -44,7 +44,7 @@ EXPORTS = [
     "fp_body_set_freestream", "fp_body_set_strength", "fp_body_get_strength", "fp_solve_bodies",
     "fp_assemble", "fp_mat_upload", "fp_mat_size", "fp_mat_download", "fp_mat_destroy", "fp_solve_mat",
     "fp_mat_getrf", "fp_uind", "fp_phi", "fp_calcfield_U", "fp_calcfield_cp", "fp_ctx_profile",
-    "fp_ctx_profile_get",
+    "fp_ctx_profile_get", "fp_ctx_copy_calibrate",
 ]
 
 
@@ -124,6 +124,7 @@ def lib():
     L.fp_calcfield_cp.argtypes = [_vp, _d, i64, dbl, _i64, i64, cint, cint, _d]
     L.fp_ctx_profile.argtypes = [_vp, cint]
     L.fp_ctx_profile_get.argtypes = [_vp, ctypes.c_char_p, _i64, _d]
+    L.fp_ctx_copy_calibrate.argtypes = [_vp, i64]
     _LIB = L
     return L
 

@@ -710,6 +710,19 @@ int fp_ctx_profile(fp_ctx *ctx, int enable)
     return FP_OK;
 }
 
+int fp_ctx_copy_calibrate(fp_ctx *ctx, int64_t bytes)
+{
+    if (!ctx || bytes <= 0) return FP_ERR_ARG;
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    DBuf a, b;
+    int st;
+    if ((st = dalloc(ctx, a, bytes)) || (st = dalloc(ctx, b, bytes))) return st;
+    FP_HIP(ctx, hipMemsetAsync(a.p, 0, bytes, ctx->stream));
+    FP_HIP(ctx, hipMemcpyAsync(b.p, a.p, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FP_OK;
+}
+
 int fp_ctx_profile_get(fp_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms)
 {
     if (!ctx || !kernel || !launches || !total_ms) return FP_ERR_ARG;

@@ -192,6 +192,9 @@ int fp_calcfield_cp(fp_ctx *ctx, const double *Us, int64_t n, double uref, const
  * "assemble_vr_f32", "field_matvec_f64", "panel_tslu", "bj_apply"). */
 int fp_ctx_profile(fp_ctx *ctx, int enable);
 int fp_ctx_profile_get(fp_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms);
+/* One device-to-device copy of `bytes` through the HIP runtime's copy kernel: a
+ * plain copy of known size to calibrate HBM byte counters (FETCH_SIZE/WRITE_SIZE). */
+int fp_ctx_copy_calibrate(fp_ctx *ctx, int64_t bytes);
 
 #ifdef __cplusplus
 }
