@@ -14,7 +14,15 @@ FP64 strengths on the device:
 
 The Cp error of the mixed result is measured against the FP64 LU result at the
 control points (the FP64 path itself is checked against the CPU oracle by the
-parity tests at smaller sizes).  One JSON line is printed.
+parity tests at smaller sizes).
+
+Adjoint leg: dCL/dnodes, dCDi/dnodes and dCL/dUinf, dCDi/dUinf of a
+10,000-panel wing (NACA 2412, AR 8, linear twist 0 -> -3 deg, 100 x 100 quads,
+AoA 4 deg) by the offset-probe lift/drag objective: FP32 forward solve with FP64
+refinement, then the hand-written reverse pass (transpose solve in the same
+mixed solver).  Its gradient is checked against central finite differences
+(h = 1e-6) of the FP64-LU forward on 64 seeded node coordinates and all three
+Uinf components.  One JSON line is printed.
 """
 import argparse
 import json
@@ -107,6 +115,88 @@ def vendor_lu(fp, body, N):
         return {"error": str(e)[:200]}
 
 
+def adjoint_leg(fp, ctx, fd_coords=64, reps=3):
+    """Lift/drag adjoint of the 10,000-panel wing; returns the JSON record."""
+    grid, sh = fp.geometry.loft_wing(airfoil="2412", n_chord=100, n_span=100, span=8.0, root_chord=1.0, taper=1.0,
+                                     sweep_deg=0.0, twist_root_deg=0.0, twist_tip_deg=-3.0)
+    body = fp.RigidWakeBody(grid, sh)
+    U = fp.direction(4.0)
+    kw = dict(probe_offset=0.02, probe_length=1.0, span_dir=(0, 1, 0), ref_area=8.0)
+    mixed = dict(solver=fp.solve_gmres_ir, solver_optargs={"precond_block": 100, "coarse_block": 100, "tol": 1e-12,
+                                                           "inner_tol": 1e-5, "restart": 100})
+    log("bench: adjoint warmup (%d panels)" % body.ncells)
+    fp.lift_drag_gradient(body, U, **kw, **mixed)
+    ctx.profile(True)
+    walls, recs = [], []
+    for _ in range(reps):
+        ctx.synchronize()
+        t = time.perf_counter()
+        r = fp.lift_drag_gradient(body, U, **kw, **mixed)
+        ctx.synchronize()
+        walls.append((time.perf_counter() - t) * 1e3)
+        recs.append(r)
+        log("bench: adjoint %.1f ms (forward %.1f, reverse %.1f, transpose solve %.1f)"
+            % (walls[-1], r["stats"]["ms_forward"], r["stats"]["ms_adjoint"], r["stats"]["ms_transpose_solve"]))
+    prof = {k: ctx.profile_get(k) for k in ("adjoint_pairs_probe_f64", "adjoint_pairs_cp_f64", "adjoint_wake_f64",
+                                           "transpose", "gemv_f32", "gemv_f64", "assemble_vr_f64",
+                                           "field_velocity_f64")}
+    ctx.profile(False)
+    best = int(np.argmin(walls))
+    r = recs[best]
+    # FP64-LU adjoint of the same objective (one run) for the mixed/FP64 gradient difference
+    r64 = fp.lift_drag_gradient(body, U, **kw)
+    g_m = np.concatenate([r["dCL_dnodes"][0].ravel(), r["dCDi_dnodes"][0].ravel()])
+    g_64 = np.concatenate([r64["dCL_dnodes"][0].ravel(), r64["dCDi_dnodes"][0].ravel()])
+    # central finite differences of the FP64-LU forward
+    log("bench: adjoint FD check (%d node coordinates + Uinf)" % fd_coords)
+    rng = np.random.Generator(np.random.PCG64(4))
+    idx = rng.choice(3 * body.grid.nnodes, fd_coords, replace=False)
+    X0 = np.array(body.grid.nodes)
+    h = 1e-6
+    fd = np.zeros((2, fd_coords))
+    for c, k in enumerate(idx):
+        v, d = divmod(int(k), 3)
+        vals = []
+        for sgn in (1, -1):
+            X = X0.copy()
+            X[d, v] += sgn * h
+            fp.set_nodes(body, X)
+            cl, cd, _ = fp.lift_drag(body, U, **kw)
+            vals.append((cl, cd))
+        fd[:, c] = [(vals[0][0] - vals[1][0]) / (2 * h), (vals[0][1] - vals[1][1]) / (2 * h)]
+    fp.set_nodes(body, X0)
+    fdu = np.zeros((2, 3))
+    for d in range(3):
+        vals = []
+        for sgn in (1, -1):
+            Up = U.copy()
+            Up[d] += sgn * h
+            cl, cd, _ = fp.lift_drag(body, Up, **kw)
+            vals.append((cl, cd))
+        fdu[:, d] = [(vals[0][0] - vals[1][0]) / (2 * h), (vals[0][1] - vals[1][1]) / (2 * h)]
+    adj = np.stack([r["dCL_dnodes"][0].T.ravel()[idx], r["dCDi_dnodes"][0].T.ravel()[idx]])
+    adju = np.stack([r["dCL_dUinf"], r["dCDi_dUinf"]])
+
+    def nrel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    kernels = {k: {"launches": v[0], "total_ms": round(v[1] / reps, 3)} for k, v in prof.items()}
+    return {
+        "workload": "wing NACA 2412, AR 8, twist 0 -> -3 deg, 100 x 100 vortex-ring quads + rigid wake, AoA 4.0",
+        "panels": body.ncells, "nodes": body.grid.nnodes,
+        "adjoint_total_ms": round(walls[best], 3),
+        "forward_ms": round(r["stats"]["ms_forward"], 3), "reverse_ms": round(r["stats"]["ms_adjoint"], 3),
+        "transpose_solve_ms": round(r["stats"]["ms_transpose_solve"], 3),
+        "forward_solve": r["stats"]["forward"], "adjoint_iterations": r["stats"]["iterations_adjoint"],
+        "CL": r["CL"], "CDi": r["CDi"],
+        "fd_rel_err_nodes": {"CL": nrel(adj[0], fd[0]), "CDi": nrel(adj[1], fd[1])},
+        "fd_rel_err_uinf": {"CL": nrel(adju[0], fdu[0]), "CDi": nrel(adju[1], fdu[1])},
+        "mixed_vs_fp64_grad_rel": nrel(g_m, g_64),
+        "fp64_lu_adjoint_total_ms": round(r64["stats"]["ms_forward"] + r64["stats"]["ms_adjoint"], 3),
+        "kernels_per_run": kernels,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +214,8 @@ def main():
                     help="profiling runs: also run a plain 1 GiB device copy (torch) to calibrate HBM counters")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the 10,000-panel adjoint leg")
+    ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg (prints its record)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
@@ -132,6 +224,9 @@ def main():
     import __graft_entry__
     fp = __graft_entry__.load_package()
     ctx = fp.context()
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_leg(fp, ctx)}), flush=True)
+        return
     alpha = 5.0
     log("bench: building BWB %d x %d" % (args.chord, args.span))
     body, U, D = build_body(fp, args.chord, args.span, alpha)
@@ -207,6 +302,9 @@ def main():
     vendor = None
     if args.vendor:
         vendor = vendor_lu(fp, body, N)
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_leg(fp, ctx)
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")
@@ -222,7 +320,8 @@ def main():
                    "panels": N, "shedding_edges": body.nsheddings},
         "fp64_lu_ms": round(t_fp64, 3), "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
         "cp_err_max": cp_err, "gamma_rel_err": gam_err,
-        "mixed_stats": stats[-1], "fp64_stats": st64, "adjoint_ms": None,
+        "mixed_stats": stats[-1], "fp64_stats": st64,
+        "adjoint_ms": adj["adjoint_total_ms"] if adj else None, "adjoint": adj,
         "kernels": kernels, "kernels_fp64": kernels_fp64, "vendor_fp64_getrf": vendor, "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -44,7 +44,8 @@ EXPORTS = [
     "fp_body_set_freestream", "fp_body_set_strength", "fp_body_get_strength", "fp_solve_bodies",
     "fp_assemble", "fp_mat_upload", "fp_mat_size", "fp_mat_download", "fp_mat_destroy", "fp_solve_mat",
     "fp_mat_getrf", "fp_uind", "fp_phi", "fp_calcfield_U", "fp_calcfield_cp", "fp_ctx_profile",
-    "fp_ctx_profile_get", "fp_ctx_copy_calibrate",
+    "fp_ctx_profile_get", "fp_ctx_copy_calibrate", "fp_body_nnodes", "fp_body_set_nodes",
+    "fp_liftdrag_opts_default", "fp_lift_drag", "fp_lift_drag_adjoint", "fp_solve_bodies_multi",
 ]
 
 
@@ -61,6 +62,23 @@ class SolveStats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class LiftDragOpts(ctypes.Structure):
+    _fields_ = [("probe_offset", ctypes.c_double), ("probe_length", ctypes.c_double),
+                ("probe_charlen", ctypes.c_int), ("kutta_mode", ctypes.c_int),
+                ("span_dir", ctypes.c_double * 3), ("ref_area", ctypes.c_double), ("solve", SolveOpts)]
+
+
+class LiftDragStats(ctypes.Structure):
+    _fields_ = [("CL", ctypes.c_double), ("CDi", ctypes.c_double), ("ms_forward", ctypes.c_double),
+                ("ms_adjoint", ctypes.c_double), ("ms_transpose_solve", ctypes.c_double),
+                ("iterations_adjoint", ctypes.c_int), ("forward", SolveStats)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "forward"}
+        d["forward"] = self.forward.as_dict()
+        return d
 
 
 class FPError(RuntimeError):
@@ -125,6 +143,17 @@ def lib():
     L.fp_ctx_profile.argtypes = [_vp, cint]
     L.fp_ctx_profile_get.argtypes = [_vp, ctypes.c_char_p, _i64, _d]
     L.fp_ctx_copy_calibrate.argtypes = [_vp, i64]
+    L.fp_solve_bodies_multi.argtypes = [_vp, pp, cint, _d, i64, ctypes.POINTER(SolveOpts), _d,
+                                        ctypes.POINTER(SolveStats)]
+    L.fp_body_nnodes.argtypes = [_vp]
+    L.fp_body_nnodes.restype = i64
+    L.fp_body_set_nodes.argtypes = [_vp, _vp, _d]
+    L.fp_liftdrag_opts_default.argtypes = [ctypes.POINTER(LiftDragOpts), cint]
+    L.fp_liftdrag_opts_default.restype = None
+    L.fp_lift_drag.argtypes = [_vp, pp, cint, _d, ctypes.POINTER(LiftDragOpts), _d, _d,
+                               ctypes.POINTER(LiftDragStats)]
+    L.fp_lift_drag_adjoint.argtypes = [_vp, pp, cint, _d, ctypes.POINTER(LiftDragOpts), _d, _d, _d, _d,
+                                       ctypes.POINTER(LiftDragStats)]
     _LIB = L
     return L
 

@@ -38,7 +38,16 @@ namespace {
 // device buffer owned by a host scope
 struct DBuf {
     void *p = nullptr;
-    ~DBuf() { if (p) (void)hipFree(p); }
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p) { o.p = nullptr; }
+    ~DBuf() { reset(); }
+    void reset()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
     template <typename T> T *as() const { return (T *)p; }
 };
 
@@ -393,139 +402,215 @@ struct SolveArgs {
     double *A64 = nullptr;   // FP64 matrix (LU path, or residual for uploaded matrices)
     float *A32 = nullptr;    // FP32 matrix (LU32_IR / GMRES_IR)
     Operator *resid = nullptr;  // FP64 residual operator
+    const double *R64 = nullptr;  // the stored FP64 matrix behind `resid`, if any (batched residuals)
 };
 
-int solve_core(fp_ctx *ctx, const SolveArgs &a, const fp_solve_opts &o, const double *b, double *x,
-               fp_solve_stats &stats)
-{
-    const int64_t n = a.n;
-    int st;
-    DBuf perm, work, r, d, part, dout;
-    if ((st = dalloc(ctx, part, 256 * 4 * sizeof(double)))) return st;
-    if ((st = dalloc(ctx, dout, 4 * sizeof(double)))) return st;
-    Timer tm(ctx->stream);
-    if (a.solver == FP_SOLVER_LU) {
-        if ((st = dalloc(ctx, perm, n * sizeof(int)))) return st;
-        if ((st = factor<double>(ctx, a.A64, n, a.ld, perm.as<int>(), &stats.info))) return st;
-        stats.ms_factor = tm.stop();
-        tm.restart();
-        if ((st = dalloc(ctx, work, fp_getrs_work_bytes(n, 1)))) return st;
-        FP_HIP(ctx, hipMemcpyAsync(x, b, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-        if ((st = fp_getrs<double>(ctx, a.A64, n, a.ld, perm.as<int>(), x, 1, n, work.p, fp_getrs_work_bytes(n, 1))))
-            return st;
-        stats.ms_solve = tm.stop();
-        stats.refinements = 0;
-        if (a.resid) {  // report the FP64 residual
-            if ((st = dalloc(ctx, r, n * sizeof(double)))) return st;
-            if ((st = a.resid->apply(x, r.as<double>()))) return st;
-            if ((st = fp_residual_update(ctx, b, r.as<double>(), n))) return st;
-            const double bn = dnorm_host(ctx, b, n, part.as<double>(), dout.as<double>(), &st);
-            const double rn = dnorm_host(ctx, r.as<double>(), n, part.as<double>(), dout.as<double>(), &st);
-            stats.residual = bn > 0 ? rn / bn : 0;
-        }
-        return FP_OK;
-    }
-    if (a.solver == FP_SOLVER_GMRES) {
-        // FP64 GMRES on the FP64 operator (solve_gmres!), optional block-Jacobi
-        Gmres g;
-        if ((st = g.init(ctx, n, std::max(1, o.restart)))) return st;
-        std::unique_ptr<BJPrecond> M;
-        DBuf bj, bjp, info;
-        if (o.precond_block > 0) {
-            const int blk = std::min<int64_t>(o.precond_block, n);
-            const int64_t nbk = (n + blk - 1) / blk;
-            if ((st = dalloc(ctx, bj, (size_t)nbk * blk * blk * sizeof(float)))) return st;
-            if ((st = dalloc(ctx, bjp, (size_t)nbk * blk * sizeof(int)))) return st;
-            if ((st = dalloc(ctx, info, sizeof(int)))) return st;
-            if (a.A64) st = fp_bj_factor<double>(ctx, a.A64, n, a.ld, blk, bj.as<float>(), bjp.as<int>(), info.as<int>());
-            else st = fp_bj_factor<float>(ctx, a.A32, n, a.ld, blk, bj.as<float>(), bjp.as<int>(), info.as<int>());
-            if (st) return st;
-            M.reset(new BJPrecond(ctx, bj.as<float>(), bjp.as<int>(), n, blk));
-            if (a.A64) st = M->setup(a.A64, FP_F64, a.ld, o.coarse_block);
-            else st = M->setup(a.A32, FP_F32, a.ld, o.coarse_block);
-            if (st) return st;
-        }
-        stats.ms_factor = tm.stop();
-        tm.restart();
-        int its = 0;
-        double rel = 0;
-        if ((st = g.solve(*a.resid, M.get(), b, x, o.tol, o.max_iters, &its, &rel))) return st;
-        stats.ms_solve = tm.stop();
-        stats.iterations = its;
-        stats.residual = rel;
-        if (rel > o.tol) return fp_fail(ctx, FP_ERR_NOT_CONVERGED, "GMRES did not reach the tolerance");
-        return FP_OK;
-    }
-    // refinement paths: FP32 inner solver, FP64 residual and update
-    if (!a.A32) return fp_fail(ctx, FP_ERR_ARG, "internal: no FP32 matrix");
+// A solver prepared once for a matrix (factorisation or preconditioner) and
+// then applied to any number of right-hand sides.
+struct Solver {
+    fp_ctx *ctx;
+    SolveArgs a;
+    fp_solve_opts o;
+    DBuf perm, work, bj, bjp, info, r, d, part, dout;
     std::unique_ptr<Gmres> g;
     std::unique_ptr<BJPrecond> M;
     std::unique_ptr<DenseOp<float>> A32op;
-    DBuf bj, bjp, info;
-    if (a.solver == FP_SOLVER_LU32_IR) {
-        if ((st = dalloc(ctx, perm, n * sizeof(int)))) return st;
-        if ((st = factor<float>(ctx, a.A32, n, a.ld, perm.as<int>(), &stats.info))) return st;
-        if ((st = dalloc(ctx, work, fp_getrs_work_bytes(n, 1)))) return st;
-    } else if (a.solver == FP_SOLVER_GMRES_IR) {
-        const int blk = (int)std::min<int64_t>(o.precond_block > 0 ? o.precond_block : 512, n);
+    Solver(fp_ctx *c, const SolveArgs &args, const fp_solve_opts &opts) : ctx(c), a(args), o(opts) {}
+
+    int bj_setup(const void *A, int prec, int blk)
+    {
+        const int64_t n = a.n;
         const int64_t nbk = (n + blk - 1) / blk;
+        int st;
         if ((st = dalloc(ctx, bj, (size_t)nbk * blk * blk * sizeof(float)))) return st;
         if ((st = dalloc(ctx, bjp, (size_t)nbk * blk * sizeof(int)))) return st;
         if ((st = dalloc(ctx, info, sizeof(int)))) return st;
-        if ((st = fp_bj_factor<float>(ctx, a.A32, n, a.ld, blk, bj.as<float>(), bjp.as<int>(), info.as<int>())))
-            return st;
-        M.reset(new BJPrecond(ctx, bj.as<float>(), bjp.as<int>(), n, blk));
-        if ((st = M->setup(a.A32, FP_F32, a.ld, o.coarse_block))) return st;
-        A32op.reset(new DenseOp<float>(ctx, a.A32, n, a.ld));
-        g.reset(new Gmres());
-        if ((st = g->init(ctx, n, std::max(1, o.restart)))) return st;
-    } else {
-        return fp_fail(ctx, FP_ERR_ARG, "unknown solver");
-    }
-    stats.ms_factor = tm.stop();
-    tm.restart();
-    if ((st = dalloc(ctx, r, n * sizeof(double)))) return st;
-    if ((st = dalloc(ctx, d, n * sizeof(double)))) return st;
-    double *rp = r.as<double>(), *dp = d.as<double>();
-    const double bn = dnorm_host(ctx, b, n, part.as<double>(), dout.as<double>(), &st);
-    if (st) return st;
-    FP_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
-    FP_HIP(ctx, hipMemcpyAsync(rp, b, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-    double rel = bn > 0 ? 1.0 : 0.0;
-    int k = 0;
-    stats.iterations = 0;
-    while (rel > o.tol && k < o.max_refine) {
-        // correction d ~ A^-1 r in the low-precision solver
-        if (a.solver == FP_SOLVER_LU32_IR) {
-            FP_HIP(ctx, hipMemcpyAsync(dp, rp, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-            if ((st = fp_getrs<float>(ctx, a.A32, n, a.ld, perm.as<int>(), dp, 1, n, work.p,
-                                      fp_getrs_work_bytes(n, 1))))
-                return st;
-        } else {
-            int its = 0;
-            double irel = 0;
-            const int left = std::max(1, o.max_iters - stats.iterations);
-            // the correction only has to bring the residual below tol: once
-            // rel is small the inner solve may stop earlier
-            const double itol = std::max(o.inner_tol, std::min(0.5, 0.3 * o.tol / std::max(rel, 1e-300)));
-            if ((st = g->solve(*A32op, M.get(), rp, dp, itol, left, &its, &irel))) return st;
-            stats.iterations += its;
-        }
-        if ((st = fp_axpby(ctx, 1.0, dp, 1.0, x, n))) return st;
-        // FP64 residual
-        if ((st = a.resid->apply(x, rp))) return st;
-        if ((st = fp_residual_update(ctx, b, rp, n))) return st;
-        const double rn = dnorm_host(ctx, rp, n, part.as<double>(), dout.as<double>(), &st);
+        st = prec == FP_F64
+                 ? fp_bj_factor<double>(ctx, (const double *)A, n, a.ld, blk, bj.as<float>(), bjp.as<int>(), info.as<int>())
+                 : fp_bj_factor<float>(ctx, (const float *)A, n, a.ld, blk, bj.as<float>(), bjp.as<int>(), info.as<int>());
         if (st) return st;
-        rel = bn > 0 ? rn / bn : 0.0;
-        ++k;
+        M.reset(new BJPrecond(ctx, bj.as<float>(), bjp.as<int>(), n, blk));
+        return M->setup(A, prec, a.ld, o.coarse_block);
     }
-    stats.ms_solve = tm.stop();
-    stats.refinements = k;
-    stats.residual = rel;
-    if (rel > o.tol) return fp_fail(ctx, FP_ERR_NOT_CONVERGED, "iterative refinement did not reach the tolerance");
-    return FP_OK;
-}
+
+    // factorisation / preconditioner set-up (stats.ms_factor, stats.info)
+    int setup(fp_solve_stats &stats)
+    {
+        const int64_t n = a.n;
+        int st;
+        if ((st = dalloc(ctx, part, 256 * 4 * sizeof(double)))) return st;
+        if ((st = dalloc(ctx, dout, 4 * sizeof(double)))) return st;
+        Timer tm(ctx->stream);
+        if (a.solver == FP_SOLVER_LU) {
+            if (!a.A64) return fp_fail(ctx, FP_ERR_ARG, "internal: no FP64 matrix");
+            if ((st = dalloc(ctx, perm, n * sizeof(int)))) return st;
+            if ((st = factor<double>(ctx, a.A64, n, a.ld, perm.as<int>(), &stats.info))) return st;
+            if ((st = dalloc(ctx, work, fp_getrs_work_bytes(n, 1)))) return st;
+        } else if (a.solver == FP_SOLVER_GMRES) {
+            // FP64 GMRES on the FP64 operator (solve_gmres!), optional block-Jacobi
+            g.reset(new Gmres());
+            if ((st = g->init(ctx, n, std::max(1, o.restart)))) return st;
+            if (o.precond_block > 0) {
+                const int blk = (int)std::min<int64_t>(o.precond_block, n);
+                st = a.A64 ? bj_setup(a.A64, FP_F64, blk) : bj_setup(a.A32, FP_F32, blk);
+                if (st) return st;
+            }
+        } else if (a.solver == FP_SOLVER_LU32_IR || a.solver == FP_SOLVER_GMRES_IR) {
+            if (!a.A32) return fp_fail(ctx, FP_ERR_ARG, "internal: no FP32 matrix");
+            if (a.solver == FP_SOLVER_LU32_IR) {
+                if ((st = dalloc(ctx, perm, n * sizeof(int)))) return st;
+                if ((st = factor<float>(ctx, a.A32, n, a.ld, perm.as<int>(), &stats.info))) return st;
+                if ((st = dalloc(ctx, work, fp_getrs_work_bytes(n, 1)))) return st;
+            } else {
+                const int blk = (int)std::min<int64_t>(o.precond_block > 0 ? o.precond_block : 512, n);
+                if ((st = bj_setup(a.A32, FP_F32, blk))) return st;
+                A32op.reset(new DenseOp<float>(ctx, a.A32, n, a.ld));
+                g.reset(new Gmres());
+                if ((st = g->init(ctx, n, std::max(1, o.restart)))) return st;
+            }
+            if ((st = dalloc(ctx, r, n * sizeof(double)))) return st;
+            if ((st = dalloc(ctx, d, n * sizeof(double)))) return st;
+        } else {
+            return fp_fail(ctx, FP_ERR_ARG, "unknown solver");
+        }
+        stats.ms_factor += tm.stop();
+        return FP_OK;
+    }
+
+    // x = A^-1 b for one right-hand side (stats: ms_solve, iterations and
+    // refinements accumulate; residual is the worst over the calls)
+    int solve(const double *b, double *x, fp_solve_stats &stats)
+    {
+        const int64_t n = a.n;
+        int st;
+        Timer tm(ctx->stream);
+        if (a.solver == FP_SOLVER_LU) {
+            FP_HIP(ctx, hipMemcpyAsync(x, b, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            if ((st = fp_getrs<double>(ctx, a.A64, n, a.ld, perm.as<int>(), x, 1, n, work.p,
+                                       fp_getrs_work_bytes(n, 1))))
+                return st;
+            stats.ms_solve += tm.stop();
+            if (a.resid) {  // report the FP64 residual
+                DBuf rr;
+                if ((st = dalloc(ctx, rr, n * sizeof(double)))) return st;
+                if ((st = a.resid->apply(x, rr.as<double>()))) return st;
+                if ((st = fp_residual_update(ctx, b, rr.as<double>(), n))) return st;
+                const double bn = dnorm_host(ctx, b, n, part.as<double>(), dout.as<double>(), &st);
+                const double rn = dnorm_host(ctx, rr.as<double>(), n, part.as<double>(), dout.as<double>(), &st);
+                stats.residual = std::max(stats.residual, bn > 0 ? rn / bn : 0);
+            }
+            return FP_OK;
+        }
+        if (a.solver == FP_SOLVER_GMRES) {
+            int its = 0;
+            double rel = 0;
+            if ((st = g->solve(*a.resid, M.get(), b, x, o.tol, o.max_iters, &its, &rel))) return st;
+            stats.ms_solve += tm.stop();
+            stats.iterations += its;
+            stats.residual = std::max(stats.residual, rel);
+            if (rel > o.tol) return fp_fail(ctx, FP_ERR_NOT_CONVERGED, "GMRES did not reach the tolerance");
+            return FP_OK;
+        }
+        // refinement paths: FP32 inner solver, FP64 residual and update
+        double *rp = r.as<double>(), *dp = d.as<double>();
+        const double bn = dnorm_host(ctx, b, n, part.as<double>(), dout.as<double>(), &st);
+        if (st) return st;
+        FP_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
+        FP_HIP(ctx, hipMemcpyAsync(rp, b, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        double rel = bn > 0 ? 1.0 : 0.0;
+        int k = 0, its_total = 0;
+        while (rel > o.tol && k < o.max_refine) {
+            // correction d ~ A^-1 r in the low-precision solver
+            if (a.solver == FP_SOLVER_LU32_IR) {
+                FP_HIP(ctx, hipMemcpyAsync(dp, rp, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+                if ((st = fp_getrs<float>(ctx, a.A32, n, a.ld, perm.as<int>(), dp, 1, n, work.p,
+                                          fp_getrs_work_bytes(n, 1))))
+                    return st;
+            } else {
+                int its = 0;
+                double irel = 0;
+                const int left = std::max(1, o.max_iters - its_total);
+                // the correction only has to bring the residual below tol: once
+                // rel is small the inner solve may stop earlier
+                const double itol = std::max(o.inner_tol, std::min(0.5, 0.3 * o.tol / std::max(rel, 1e-300)));
+                if ((st = g->solve(*A32op, M.get(), rp, dp, itol, left, &its, &irel))) return st;
+                its_total += its;
+            }
+            if ((st = fp_axpby(ctx, 1.0, dp, 1.0, x, n))) return st;
+            // FP64 residual
+            if ((st = a.resid->apply(x, rp))) return st;
+            if ((st = fp_residual_update(ctx, b, rp, n))) return st;
+            const double rn = dnorm_host(ctx, rp, n, part.as<double>(), dout.as<double>(), &st);
+            if (st) return st;
+            rel = bn > 0 ? rn / bn : 0.0;
+            ++k;
+        }
+        stats.ms_solve += tm.stop();
+        stats.refinements += k;
+        stats.iterations += its_total;
+        stats.residual = std::max(stats.residual, rel);
+        if (rel > o.tol) return fp_fail(ctx, FP_ERR_NOT_CONVERGED, "iterative refinement did not reach the tolerance");
+        return FP_OK;
+    }
+    // X = A^-1 B for k right-hand sides (B, X column-major n x k).  LU paths
+    // substitute all columns per sweep and refine them together (residuals by
+    // a multi-vector GEMV of the stored FP64 matrix); Krylov paths solve the
+    // columns one after the other with the same preconditioner.
+    int solve_multi(const double *B, double *X, int64_t k, fp_solve_stats &stats)
+    {
+        const int64_t n = a.n;
+        int st;
+        if (k == 1 || a.solver == FP_SOLVER_GMRES || a.solver == FP_SOLVER_GMRES_IR ||
+            (a.solver == FP_SOLVER_LU32_IR && !a.R64)) {
+            for (int64_t c = 0; c < k; ++c)
+                if ((st = solve(B + c * n, X + c * n, stats))) return st;
+            return FP_OK;
+        }
+        Timer tm(ctx->stream);
+        DBuf wk;
+        const size_t wbytes = fp_getrs_work_bytes(n, k);
+        if ((st = dalloc(ctx, wk, wbytes))) return st;
+        FP_HIP(ctx, hipMemcpyAsync(X, B, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        if (a.solver == FP_SOLVER_LU) {
+            if ((st = fp_getrs<double>(ctx, a.A64, n, a.ld, perm.as<int>(), X, k, n, wk.p, wbytes))) return st;
+            stats.ms_solve += tm.stop();
+            return FP_OK;
+        }
+        // FP32 LU + FP64 refinement of all columns together
+        DBuf R, D, nr;
+        if ((st = dalloc(ctx, R, (size_t)n * k * sizeof(double))) || (st = dalloc(ctx, D, (size_t)n * k * sizeof(double))) ||
+            (st = dalloc(ctx, nr, k * sizeof(double))))
+            return st;
+        std::vector<double> bn(k), rn(k);
+        if ((st = fp_colnorm2(ctx, B, n, n, k, nr.as<double>()))) return st;
+        FP_HIP(ctx, hipMemcpyAsync(bn.data(), nr.p, k * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        FP_HIP(ctx, hipMemsetAsync(X, 0, (size_t)n * k * sizeof(double), ctx->stream));
+        FP_HIP(ctx, hipMemcpyAsync(R.p, B, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        double rel = 1.0;
+        int it = 0;
+        while (rel > o.tol && it < o.max_refine) {
+            FP_HIP(ctx, hipMemcpyAsync(D.p, R.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            if ((st = fp_getrs<float>(ctx, a.A32, n, a.ld, perm.as<int>(), D.as<double>(), k, n, wk.p, wbytes)))
+                return st;
+            if ((st = fp_axpby(ctx, 1.0, D.as<double>(), 1.0, X, n * k))) return st;
+            if ((st = fp_gemv_multi<double>(ctx, a.R64, n, a.ld, X, n, k, R.as<double>(), n))) return st;
+            if ((st = fp_residual_update(ctx, B, R.as<double>(), n * k))) return st;
+            if ((st = fp_colnorm2(ctx, R.as<double>(), n, n, k, nr.as<double>()))) return st;
+            FP_HIP(ctx, hipMemcpyAsync(rn.data(), nr.p, k * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            rel = 0;
+            for (int64_t c = 0; c < k; ++c) rel = std::max(rel, bn[c] > 0 ? std::sqrt(rn[c] / bn[c]) : 0.0);
+            ++it;
+        }
+        stats.ms_solve += tm.stop();
+        stats.refinements += it;
+        stats.residual = std::max(stats.residual, rel);
+        if (rel > o.tol) return fp_fail(ctx, FP_ERR_NOT_CONVERGED, "iterative refinement did not reach the tolerance");
+        return FP_OK;
+    }
+};
+
 
 // host AoS 3 x n -> device SoA
 int upload_soa3(fp_ctx *ctx, const double *h, int64_t n, double *d)
@@ -915,29 +1000,17 @@ static int assemble_into(fp_ctx *ctx, fp_body *const *bodies, int nb, const Targ
     return FP_OK;
 }
 
-int fp_solve_bodies(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_solve_opts *opts, fp_solve_stats *stats)
+// Assembly of the bodies' AIC and solution for k right-hand sides (device,
+// column-major n x k).  The matrix storage follows the solver: FP64 for LU and
+// GMRES, FP64 + its FP32 narrowing for the refinement paths with a stored
+// residual, FP32 alone with a matrix-free FP64 residual.
+static int solve_system(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_solve_opts &o, const double *rhs,
+                        int64_t k, double *x, fp_solve_stats &s)
 {
-    int st = check_bodies(ctx, bodies, nb);
-    if (st) return st;
-    for (int k = 0; k < nb; ++k)
-        if (!bodies[k]->has_uinf)
-            return fp_fail(ctx, FP_ERR_SHAPE, "Invalid Uinfs; call fp_body_set_freestream before solve");
-    FP_HIP(ctx, hipSetDevice(ctx->device));
-    fp_solve_opts o;
-    if (opts) o = *opts;
-    else fp_solve_opts_default(&o, FP_SOLVER_LU);
-    fp_solve_stats s{};
+    int st;
     Targets t;
     if ((st = gather_targets(ctx, bodies, nb, t))) return st;
     const int64_t n = t.M, ld = round_ld(n);
-    DBuf A, rhs, x;
-    if ((st = dalloc(ctx, rhs, n * sizeof(double)))) return st;
-    if ((st = dalloc(ctx, x, n * sizeof(double)))) return st;
-    int64_t col = 0;
-    for (int k = 0; k < nb; ++k) {
-        if ((st = fp_rhs_kernel(ctx, bodies[k], rhs.as<double>() + col))) return st;
-        col += bodies[k]->N;
-    }
     std::vector<fp_body *> bl(bodies, bodies + nb);
     MatrixFreeOp mf(ctx, bl, t.tv);
     if ((st = mf.init())) return st;
@@ -951,7 +1024,7 @@ int fp_solve_bodies(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_solve_
     // refinement with a stored FP64 residual: assemble in FP64 once, keep it,
     // and narrow a copy to FP32 for the low-precision solver
     const bool dual = !f64 && o.residual_mode == FP_RESID_STORED;
-    DBuf A32buf;
+    DBuf A, A32buf;
     std::unique_ptr<DenseOp<double>> op64;
     if ((st = dalloc(ctx, A, (size_t)n * ld * ((f64 || dual) ? sizeof(double) : sizeof(float))))) return st;
     if (dual && (st = dalloc(ctx, A32buf, (size_t)n * ld * sizeof(float)))) return st;
@@ -961,6 +1034,7 @@ int fp_solve_bodies(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_solve_
         a.A32 = A32buf.as<float>();
         op64.reset(new DenseOp<double>(ctx, A.as<double>(), n, ld));
         a.resid = op64.get();
+        a.R64 = A.as<double>();
     } else if (f64) {
         a.A64 = A.as<double>();
     } else {
@@ -971,7 +1045,42 @@ int fp_solve_bodies(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_solve_
         op64.reset(new DenseOp<double>(ctx, a.A64, n, ld));
         a.resid = op64.get();
     }
-    st = solve_core(ctx, a, o, rhs.as<double>(), x.as<double>(), s);
+    Solver solver(ctx, a, o);
+    if ((st = solver.setup(s))) return st;
+    return solver.solve_multi(rhs, x, k, s);
+}
+
+static int check_solve_inputs(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_solve_opts *opts,
+                              fp_solve_opts &o)
+{
+    int st = check_bodies(ctx, bodies, nb);
+    if (st) return st;
+    for (int k = 0; k < nb; ++k)
+        if (!bodies[k]->has_uinf)
+            return fp_fail(ctx, FP_ERR_SHAPE, "Invalid Uinfs; call fp_body_set_freestream before solve");
+    if (opts) o = *opts;
+    else fp_solve_opts_default(&o, FP_SOLVER_LU);
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    return FP_OK;
+}
+
+int fp_solve_bodies(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_solve_opts *opts, fp_solve_stats *stats)
+{
+    fp_solve_opts o;
+    int st = check_solve_inputs(ctx, bodies, nb, opts, o);
+    if (st) return st;
+    fp_solve_stats s{};
+    int64_t n = 0;
+    for (int k = 0; k < nb; ++k) n += bodies[k]->N;
+    DBuf rhs, x;
+    if ((st = dalloc(ctx, rhs, n * sizeof(double)))) return st;
+    if ((st = dalloc(ctx, x, n * sizeof(double)))) return st;
+    int64_t col = 0;
+    for (int k = 0; k < nb; ++k) {
+        if ((st = fp_rhs_kernel(ctx, bodies[k], rhs.as<double>() + col))) return st;
+        col += bodies[k]->N;
+    }
+    st = solve_system(ctx, bodies, nb, o, rhs.as<double>(), 1, x.as<double>(), s);
     if (stats) *stats = s;
     if (st) return st;
     col = 0;
@@ -981,6 +1090,51 @@ int fp_solve_bodies(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_solve_
         bodies[k]->solved = true;
         col += bodies[k]->N;
     }
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FP_OK;
+}
+
+int fp_solve_bodies_multi(fp_ctx *ctx, fp_body *const *bodies, int nb, const double *Uinfs, int64_t k,
+                          const fp_solve_opts *opts, double *strengths, fp_solve_stats *stats)
+{
+    fp_solve_opts o;
+    int st = check_solve_inputs(ctx, bodies, nb, opts, o);
+    if (st) return st;
+    if (k <= 0 || !Uinfs || !strengths) return fp_fail(ctx, FP_ERR_ARG, "need k >= 1 freestreams and outputs");
+    fp_solve_stats s{};
+    int64_t n = 0;
+    for (int b = 0; b < nb; ++b) n += bodies[b]->N;
+    for (int64_t q = 0; q < 3 * n * k; ++q)
+        if (!std::isfinite(Uinfs[q])) return fp_fail(ctx, FP_ERR_ARG, "non-finite freestream");
+    DBuf rhs, x, U;
+    if ((st = dalloc(ctx, rhs, (size_t)n * k * sizeof(double))) || (st = dalloc(ctx, x, (size_t)n * k * sizeof(double))) ||
+        (st = dalloc(ctx, U, (size_t)3 * n * k * sizeof(double))))
+        return st;
+    // freestream c of body b as SoA 3 x N_b at U + 3*(c*n + col_b)
+    std::vector<double> soa((size_t)3 * n * k);
+    for (int64_t c = 0; c < k; ++c) {
+        int64_t col = 0;
+        for (int b = 0; b < nb; ++b) {
+            const int64_t N = bodies[b]->N;
+            double *dst = &soa[(size_t)3 * (c * n + col)];
+            for (int64_t i = 0; i < N; ++i)
+                for (int d = 0; d < 3; ++d) dst[d * N + i] = Uinfs[3 * (c * n + col + i) + d];
+            col += N;
+        }
+    }
+    FP_HIP(ctx, hipMemcpyAsync(U.p, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    for (int64_t c = 0; c < k; ++c) {
+        int64_t col = 0;
+        for (int b = 0; b < nb; ++b) {
+            if ((st = fp_rhs_from(ctx, bodies[b], U.as<double>() + 3 * (c * n + col), rhs.as<double>() + c * n + col)))
+                return st;
+            col += bodies[b]->N;
+        }
+    }
+    st = solve_system(ctx, bodies, nb, o, rhs.as<double>(), k, x.as<double>(), s);
+    if (stats) *stats = s;
+    if (st) return st;
+    FP_HIP(ctx, hipMemcpyAsync(strengths, x.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return FP_OK;
 }
@@ -1131,42 +1285,37 @@ int fp_solve_mat(fp_ctx *ctx, fp_mat *m, const double *bh, int64_t k, double *xh
         if ((st = mf->init())) return st;
     }
     Operator *R = resid ? resid.get() : (Operator *)mf.get();
+    // one factorisation / preconditioner for all k right-hand sides
     fp_solve_stats s{};
-    for (int64_t c = 0; c < k; ++c) {
-        SolveArgs a;
-        a.solver = o.solver;
-        a.n = n;
-        a.ld = ld;
-        a.resid = R;
-        fp_solve_stats sc{};
-        if (o.solver == FP_SOLVER_LU) {
-            if (m->precision != FP_F64) return fp_fail(ctx, FP_ERR_UNSUPPORTED, "FP64 LU of an FP32 matrix");
-            if ((st = dalloc(ctx, W, (size_t)n * ld * sizeof(double)))) return st;
-            FP_HIP(ctx, hipMemcpyAsync(W.p, m->A, (size_t)n * ld * sizeof(double), hipMemcpyDeviceToDevice,
-                                       ctx->stream));
-            a.A64 = W.as<double>();
-        } else if (o.solver == FP_SOLVER_GMRES) {
-            if (m->precision == FP_F64) a.A64 = (double *)m->A;
-            else a.A32 = (float *)m->A;
+    SolveArgs a;
+    a.solver = o.solver;
+    a.n = n;
+    a.ld = ld;
+    a.resid = R;
+    if (o.solver == FP_SOLVER_LU) {
+        if (m->precision != FP_F64) return fp_fail(ctx, FP_ERR_UNSUPPORTED, "FP64 LU of an FP32 matrix");
+        if ((st = dalloc(ctx, W, (size_t)n * ld * sizeof(double)))) return st;
+        FP_HIP(ctx, hipMemcpyAsync(W.p, m->A, (size_t)n * ld * sizeof(double), hipMemcpyDeviceToDevice,
+                                   ctx->stream));
+        a.A64 = W.as<double>();
+    } else if (o.solver == FP_SOLVER_GMRES) {
+        if (m->precision == FP_F64) a.A64 = (double *)m->A;
+        else a.A32 = (float *)m->A;
+    } else {
+        if ((st = dalloc(ctx, W, (size_t)n * ld * sizeof(float)))) return st;
+        if (m->precision == FP_F64) {
+            if ((st = fp_mat_narrow(ctx, (const double *)m->A, W.as<float>(), n, ld))) return st;
         } else {
-            if ((st = dalloc(ctx, W, (size_t)n * ld * sizeof(float)))) return st;
-            if (m->precision == FP_F64) {
-                if ((st = fp_mat_narrow(ctx, (const double *)m->A, W.as<float>(), n, ld))) return st;
-            } else {
-                FP_HIP(ctx, hipMemcpyAsync(W.p, m->A, (size_t)n * ld * sizeof(float), hipMemcpyDeviceToDevice,
-                                           ctx->stream));
-            }
-            a.A32 = W.as<float>();
+            FP_HIP(ctx, hipMemcpyAsync(W.p, m->A, (size_t)n * ld * sizeof(float), hipMemcpyDeviceToDevice,
+                                       ctx->stream));
         }
-        st = solve_core(ctx, a, o, b.as<double>() + c * n, x.as<double>() + c * n, sc);
-        s.ms_factor += sc.ms_factor;
-        s.ms_solve += sc.ms_solve;
-        s.iterations += sc.iterations;
-        s.refinements += sc.refinements;
-        s.residual = std::max(s.residual, sc.residual);
-        s.info = sc.info;
-        if (st) { if (stats) *stats = s; return st; }
+        a.A32 = W.as<float>();
     }
+    if (m->precision == FP_F64) a.R64 = (const double *)m->A;
+    Solver solver(ctx, a, o);
+    st = solver.setup(s);
+    if (!st) st = solver.solve_multi(b.as<double>(), x.as<double>(), k, s);
+    if (st) { if (stats) *stats = s; return st; }
     FP_HIP(ctx, hipMemcpy(xh, x.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToHost));
     if (stats) *stats = s;
     return FP_OK;
@@ -1290,6 +1439,440 @@ int fp_calcfield_cp(fp_ctx *ctx, const double *Us, int64_t n, double uref, const
     FP_HIP(ctx, hipMemcpyAsync(out, o.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return FP_OK;
+}
+
+int64_t fp_body_nnodes(const fp_body *b) { return b ? (int64_t)(b->nodes.size() / 3) : -1; }
+
+int fp_body_set_nodes(fp_ctx *ctx, fp_body *b, const double *nodes)
+{
+    if (!ctx || !b || b->ctx != ctx || !nodes) return FP_ERR_ARG;
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t nn = (int64_t)b->nodes.size() / 3;
+    for (int64_t k = 0; k < 3 * nn; ++k)
+        if (!std::isfinite(nodes[k])) return fp_fail(ctx, FP_ERR_ARG, "non-finite node coordinate");
+    b->nodes.assign(nodes, nodes + 3 * nn);
+    const int nvp = b->nvp;
+    const int64_t N = b->N;
+    std::vector<double> pv((size_t)3 * nvp * N);
+    for (int64_t j = 0; j < N; ++j) {
+        const int64_t *pc = &b->cell_idx[b->cell_ptr[j]];
+        const int nv = (int)(b->cell_ptr[j + 1] - b->cell_ptr[j]);
+        for (int k = 0; k < nvp; ++k) {
+            const int64_t v = pc[k < nv ? k : 0];
+            for (int d = 0; d < 3; ++d) pv[(size_t)(3 * k + d) * N + j] = b->nodes[3 * v + d];
+        }
+    }
+    FP_HIP(ctx, hipMemcpyAsync(b->d_pv, pv.data(), pv.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    int st = fp_launch_geometry(ctx, b);
+    if (st) return st;
+    if (b->has_wake && (st = build_horseshoes(ctx, b))) return st;
+    b->solved = false;
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FP_OK;
+}
+
+void fp_liftdrag_opts_default(fp_liftdrag_opts *o, int solver)
+{
+    if (!o) return;
+    o->probe_offset = 0.02;
+    o->probe_length = 1.0;
+    o->probe_charlen = FP_CHARLEN_UNITARY;
+    o->kutta_mode = FP_KUTTA_PAIR;
+    o->span_dir[0] = 0; o->span_dir[1] = 1; o->span_dir[2] = 0;
+    o->ref_area = 1.0;
+    fp_solve_opts_default(&o->solve, solver);
+}
+
+// Forward (and optionally reverse) pass of the lift/drag objective.
+static int liftdrag_run(fp_ctx *ctx, fp_body *const *bodies, int nb, const double *U, const fp_liftdrag_opts *lo,
+                        bool adjoint, double *CLo, double *CDo, double *dCL_dn, double *dCD_dn, double *dCL_dU,
+                        double *dCD_dU, fp_liftdrag_stats *stats_out)
+{
+    int st = check_bodies(ctx, bodies, nb);
+    if (st) return st;
+    if (!U) return fp_fail(ctx, FP_ERR_ARG, "null Uinf");
+    fp_liftdrag_opts o;
+    if (lo) o = *lo;
+    else fp_liftdrag_opts_default(&o, FP_SOLVER_LU);
+    const double un = std::sqrt(U[0] * U[0] + U[1] * U[1] + U[2] * U[2]);
+    if (!(un > 0) || !std::isfinite(un)) return fp_fail(ctx, FP_ERR_ARG, "Uinf must be finite and non-zero");
+    if (!(o.ref_area > 0)) return fp_fail(ctx, FP_ERR_ARG, "ref_area must be positive");
+    if (!std::isfinite(o.probe_offset) || !std::isfinite(o.probe_length))
+        return fp_fail(ctx, FP_ERR_ARG, "probe offset/length must be finite");
+    if (o.probe_charlen < 0 || o.probe_charlen > 3) return fp_fail(ctx, FP_ERR_ARG, "unknown probe_charlen");
+    if (o.kutta_mode < FP_KUTTA_NONE || o.kutta_mode > FP_KUTTA_PAIR) return fp_fail(ctx, FP_ERR_ARG, "bad kutta mode");
+    const double dh[3] = {U[0] / un, U[1] / un, U[2] / un};
+    const double *sd = o.span_dir;
+    const double wv[3] = {dh[1] * sd[2] - dh[2] * sd[1], dh[2] * sd[0] - dh[0] * sd[2], dh[0] * sd[1] - dh[1] * sd[0]};
+    const double wn = std::sqrt(wv[0] * wv[0] + wv[1] * wv[1] + wv[2] * wv[2]);
+    if (!(wn > 1e-12)) return fp_fail(ctx, FP_ERR_ARG, "span_dir is parallel to Uinf");
+    const double Lh[3] = {wv[0] / wn, wv[1] / wn, wv[2] / wn};
+    int64_t M = 0;
+    for (int k = 0; k < nb; ++k) {
+        const fp_body *b = bodies[k];
+        if (b->element != FP_ELEM_VORTEXRING)
+            return fp_fail(ctx, FP_ERR_UNSUPPORTED, "lift/drag objective needs vortex-ring bodies");
+        for (int64_t e = 0; e < b->ns && o.kutta_mode != FP_KUTTA_NONE; ++e) {
+            const int64_t pi = b->shedding[6 * e], pj = b->shedding[6 * e + 3];
+            const bool ref = o.kutta_mode == FP_KUTTA_REFERENCE;
+            if (pi + (ref ? 1 : 0) >= b->N || (pj != -1 && pj - (ref ? 1 : 0) < 0))
+                return fp_fail(ctx, FP_ERR_ARG, "Kutta panel index out of range");
+        }
+        M += b->N;
+    }
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    fp_liftdrag_stats S{};
+    Timer tm(ctx->stream);
+    // 1) freestream at every control point, wake along Uinf
+    for (int k = 0; k < nb; ++k) {
+        fp_body *b = bodies[k];
+        std::vector<double> ui(3 * b->N);
+        for (int64_t i = 0; i < b->N; ++i)
+            for (int c = 0; c < 3; ++c) ui[3 * i + c] = U[c];
+        if ((st = upload_soa3(ctx, ui.data(), b->N, b->d_uinf))) return st;
+        b->das.assign(3 * b->ns, 0.0);
+        for (int64_t e = 0; e < b->ns; ++e)
+            for (int c = 0; c < 3; ++c) b->das[3 * e + c] = dh[c];
+        b->dbs = b->das;
+        if ((st = build_horseshoes(ctx, b))) return st;
+        b->has_uinf = b->has_wake = true;
+    }
+    // 2) assembly and solve (FP64 stored matrix; transposed copy for the adjoint)
+    fp_solve_opts so = o.solve;
+    const bool f64 = so.solver == FP_SOLVER_LU || so.solver == FP_SOLVER_GMRES;
+    if (adjoint) so.residual_mode = FP_RESID_STORED;
+    const bool stored = f64 || so.residual_mode == FP_RESID_STORED;
+    Targets t;
+    if ((st = gather_targets(ctx, bodies, nb, t))) return st;
+    const int64_t n = t.M, ld = round_ld(n);
+    DBuf A, A32, AT, AT32, rhs, x;
+    if ((st = dalloc(ctx, rhs, n * sizeof(double))) || (st = dalloc(ctx, x, n * sizeof(double)))) return st;
+    int64_t col = 0;
+    for (int k = 0; k < nb; ++k) {
+        if ((st = fp_rhs_kernel(ctx, bodies[k], rhs.as<double>() + col))) return st;
+        col += bodies[k]->N;
+    }
+    std::vector<fp_body *> bl(bodies, bodies + nb);
+    MatrixFreeOp mf(ctx, bl, t.tv);
+    if ((st = mf.init())) return st;
+    std::unique_ptr<DenseOp<double>> op64, opT;
+    SolveArgs a;
+    a.solver = so.solver;
+    a.n = n;
+    a.ld = ld;
+    a.resid = &mf;
+    if ((st = dalloc(ctx, A, (size_t)n * ld * (stored ? sizeof(double) : sizeof(float))))) return st;
+    if ((st = assemble_into(ctx, bodies, nb, t.tv, stored ? FP_F64 : FP_F32, A.p, ld))) return st;
+    if (adjoint) {  // before any in-place factorisation of A
+        if ((st = dalloc(ctx, AT, (size_t)n * ld * sizeof(double)))) return st;
+        if ((st = fp_mat_transpose(ctx, A.as<double>(), n, ld, AT.p, FP_F64))) return st;
+    }
+    if (f64) {
+        a.A64 = A.as<double>();
+        if (so.solver == FP_SOLVER_GMRES) {
+            op64.reset(new DenseOp<double>(ctx, a.A64, n, ld));
+            a.resid = op64.get();
+        }
+    } else if (stored) {
+        if ((st = dalloc(ctx, A32, (size_t)n * ld * sizeof(float)))) return st;
+        if ((st = fp_mat_narrow(ctx, A.as<double>(), A32.as<float>(), n, ld))) return st;
+        a.A32 = A32.as<float>();
+        op64.reset(new DenseOp<double>(ctx, A.as<double>(), n, ld));
+        a.resid = op64.get();
+    } else {
+        a.A32 = A.as<float>();
+    }
+    S.forward.ms_assemble = tm.stop();
+    {
+        Solver fwd(ctx, a, so);
+        st = fwd.setup(S.forward);
+        if (!st) st = fwd.solve(rhs.as<double>(), x.as<double>(), S.forward);
+        if (st) { if (stats_out) *stats_out = S; return st; }
+    }
+    col = 0;
+    for (int k = 0; k < nb; ++k) {
+        FP_HIP(ctx, hipMemcpyAsync(bodies[k]->d_strength, x.as<double>() + col, bodies[k]->N * sizeof(double),
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+        bodies[k]->solved = true;
+        col += bodies[k]->N;
+    }
+    // 3) probes, velocities, Cp, forces
+    DBuf geo, V, Cp, T, ones, part, fpart, out8, shs;
+    if ((st = dalloc(ctx, geo, 10 * M * sizeof(double)))) return st;
+    double *nrm = geo.as<double>(), *area = nrm + 3 * M, *cpt = area + M, *prb = cpt + 3 * M;
+    if ((st = dalloc(ctx, V, 3 * M * sizeof(double))) || (st = dalloc(ctx, Cp, M * sizeof(double))) ||
+        (st = dalloc(ctx, T, 8 * M * sizeof(double))) || (st = dalloc(ctx, ones, M * sizeof(double))) ||
+        (st = dalloc(ctx, part, 256 * 8 * sizeof(double))) || (st = dalloc(ctx, out8, 8 * sizeof(double))))
+        return st;
+    {
+        std::vector<double> one(M, 1.0);
+        FP_HIP(ctx, hipMemcpyAsync(ones.p, one.data(), M * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    col = 0;
+    for (int k = 0; k < nb; ++k) {
+        if ((st = fp_adj_geometry(ctx, bodies[k], o.probe_offset, o.probe_charlen, o.probe_length, M, col, nrm, area,
+                                  cpt, prb)))
+            return st;
+        col += bodies[k]->N;
+    }
+    TargetView tp;
+    tp.x = prb; tp.y = prb + M; tp.z = prb + 2 * M;
+    tp.nx = nrm; tp.ny = nrm + M; tp.nz = nrm + 2 * M;
+    tp.M = M;
+    size_t fpb = 0;
+    for (int k = 0; k < nb; ++k) fpb = std::max(fpb, fp_field_partial_bytes(M, bodies[k]->N));
+    if (fpb && (st = dalloc(ctx, fpart, fpb))) return st;
+    FP_HIP(ctx, hipMemsetAsync(V.p, 0, 3 * M * sizeof(double), ctx->stream));
+    for (int k = 0; k < nb; ++k)
+        if ((st = fp_launch_field(ctx, bodies[k], bodies[k]->d_strength, tp, 0, V.as<double>(), fpart.as<double>(), fpb)))
+            return st;
+    if ((st = fp_adj_cp(ctx, V.as<double>(), M, U, Cp.as<double>()))) return st;
+    // per-body shedding tables on the device (Kutta)
+    std::vector<DBuf> dsh(nb);
+    col = 0;
+    for (int k = 0; k < nb; ++k) {
+        const fp_body *b = bodies[k];
+        if (b->ns > 0) {
+            if ((st = dalloc(ctx, dsh[k], 6 * b->ns * sizeof(int64_t)))) return st;
+            FP_HIP(ctx, hipMemcpyAsync(dsh[k].p, b->shedding.data(), 6 * b->ns * sizeof(int64_t),
+                                       hipMemcpyHostToDevice, ctx->stream));
+            if ((st = fp_adj_kutta(ctx, dsh[k].as<int64_t>(), b->ns, o.kutta_mode, Cp.as<double>() + col, nullptr, 0)))
+                return st;
+        }
+        col += b->N;
+    }
+    if ((st = fp_adj_force_terms(ctx, Cp.as<double>(), area, nrm, M, T.as<double>()))) return st;
+    if ((st = fp_mdot(ctx, T.as<double>(), M, 3, ones.as<double>(), M, part.as<double>(), out8.as<double>()))) return st;
+    double fs[8];
+    FP_HIP(ctx, hipMemcpyAsync(fs, out8.p, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const double Fn[3] = {-fs[0] / o.ref_area, -fs[1] / o.ref_area, -fs[2] / o.ref_area};
+    S.CL = Fn[0] * Lh[0] + Fn[1] * Lh[1] + Fn[2] * Lh[2];
+    S.CDi = Fn[0] * dh[0] + Fn[1] * dh[1] + Fn[2] * dh[2];
+    if (CLo) *CLo = S.CL;
+    if (CDo) *CDo = S.CDi;
+    S.ms_forward = tm.stop();
+    if (!adjoint) {
+        if (stats_out) *stats_out = S;
+        return FP_OK;
+    }
+    // 4) reverse pass
+    tm.restart();
+    DBuf Cpb, Ab, nbar, W1, W2, Xbp, Xbc, g, lam, Uc, work;
+    if ((st = dalloc(ctx, Cpb, 2 * M * sizeof(double))) || (st = dalloc(ctx, Ab, 2 * M * sizeof(double))) ||
+        (st = dalloc(ctx, nbar, 6 * M * sizeof(double))) || (st = dalloc(ctx, W1, 8 * M * sizeof(double))) ||
+        (st = dalloc(ctx, W2, 6 * M * sizeof(double))) || (st = dalloc(ctx, Xbp, 6 * M * sizeof(double))) ||
+        (st = dalloc(ctx, Xbc, 6 * M * sizeof(double))) || (st = dalloc(ctx, g, 2 * M * sizeof(double))) ||
+        (st = dalloc(ctx, lam, 2 * M * sizeof(double))) || (st = dalloc(ctx, Uc, 3 * M * sizeof(double))))
+        return st;
+    FP_HIP(ctx, hipMemsetAsync(Xbp.p, 0, 6 * M * sizeof(double), ctx->stream));
+    FP_HIP(ctx, hipMemsetAsync(Xbc.p, 0, 6 * M * sizeof(double), ctx->stream));
+    FP_HIP(ctx, hipMemsetAsync(g.p, 0, 2 * M * sizeof(double), ctx->stream));
+    if ((st = fp_adj_objective(ctx, Cp.as<double>(), area, nrm, M, Lh, dh, 1.0 / o.ref_area, Cpb.as<double>(),
+                               Ab.as<double>(), nbar.as<double>())))
+        return st;
+    col = 0;
+    for (int k = 0; k < nb; ++k) {
+        const fp_body *b = bodies[k];
+        if (b->ns > 0 && (st = fp_adj_kutta(ctx, dsh[k].as<int64_t>(), b->ns, o.kutta_mode, Cpb.as<double>() + col,
+                                            Cpb.as<double>() + M + col, 1)))
+            return st;
+        col += b->N;
+    }
+    if ((st = fp_adj_vbar(ctx, Cpb.as<double>(), V.as<double>(), M, 1.0 / (un * un), W1.as<double>()))) return st;
+    double s1[8];
+    if ((st = fp_mdot(ctx, W1.as<double>(), M, 8, ones.as<double>(), M, part.as<double>(), out8.as<double>()))) return st;
+    FP_HIP(ctx, hipMemcpyAsync(s1, out8.p, 8 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    // probe pass: strength adjoints, probe-target and vertex adjoints
+    size_t wb = 0;
+    for (int k = 0; k < nb; ++k) wb = std::max(wb, fp_adj_pairs_work_bytes(ctx, bodies[k]->N, bodies[k]->nvp, M));
+    if ((st = dalloc(ctx, work, wb))) return st;
+    std::vector<DBuf> vbar(nb), hpart(nb);
+    col = 0;
+    for (int k = 0; k < nb; ++k) {
+        fp_body *b = bodies[k];
+        const size_t vb = (size_t)6 * b->nvp * b->N * sizeof(double);
+        if ((st = dalloc(ctx, vbar[k], vb)) || (st = dalloc(ctx, hpart[k], (size_t)2 * 26 * std::max(1, b->nh) * sizeof(double))))
+            return st;
+        FP_HIP(ctx, hipMemsetAsync(vbar[k].p, 0, vb, ctx->stream));
+        if ((st = fp_adj_pairs(ctx, b, prb, M, W1.as<double>(), 1, Xbp.as<double>(), vbar[k].as<double>(),
+                               g.as<double>() + col, M, work.p, wb)))
+            return st;
+        if ((st = fp_adj_wake(ctx, b, prb, M, W1.as<double>(), 1, Xbp.as<double>(), hpart[k].as<double>(),
+                              g.as<double>(), col)))
+            return st;
+        col += b->N;
+    }
+    // transpose solve A^T lambda_m = g_m
+    {
+        Timer tt(ctx->stream);
+        fp_solve_stats ts{};
+        SolveArgs at;
+        at.solver = so.solver;
+        at.n = n;
+        at.ld = ld;
+        at.A64 = f64 ? AT.as<double>() : nullptr;
+        if (!f64) {
+            if ((st = dalloc(ctx, AT32, (size_t)n * ld * sizeof(float)))) return st;
+            if ((st = fp_mat_narrow(ctx, AT.as<double>(), AT32.as<float>(), n, ld))) return st;
+            at.A32 = AT32.as<float>();
+        }
+        // the FP64 residual of the transposed system needs an unfactorised copy
+        DBuf ATr;
+        const double *Ares = AT.as<double>();
+        if (so.solver == FP_SOLVER_LU) {
+            if ((st = dalloc(ctx, ATr, (size_t)n * ld * sizeof(double)))) return st;
+            FP_HIP(ctx, hipMemcpyAsync(ATr.p, AT.p, (size_t)n * ld * sizeof(double), hipMemcpyDeviceToDevice,
+                                       ctx->stream));
+            Ares = ATr.as<double>();
+        }
+        opT.reset(new DenseOp<double>(ctx, Ares, n, ld));
+        at.resid = opT.get();
+        Solver ts_solver(ctx, at, so);
+        st = ts_solver.setup(ts);
+        for (int m = 0; m < 2 && !st; ++m)
+            st = ts_solver.solve(g.as<double>() + m * M, lam.as<double>() + m * M, ts);
+        if (st) { if (stats_out) *stats_out = S; return st; }
+        S.iterations_adjoint = ts.iterations;
+        S.ms_transpose_solve = tt.stop();
+    }
+    AT.reset();
+    AT32.reset();
+    A.reset();
+    A32.reset();
+    // control-point pass: induced velocity at the control points, weights, pairs
+    TargetView tc;
+    tc.x = cpt; tc.y = cpt + M; tc.z = cpt + 2 * M;
+    tc.nx = nrm; tc.ny = nrm + M; tc.nz = nrm + 2 * M;
+    tc.M = M;
+    FP_HIP(ctx, hipMemsetAsync(Uc.p, 0, 3 * M * sizeof(double), ctx->stream));
+    for (int k = 0; k < nb; ++k)
+        if ((st = fp_launch_field(ctx, bodies[k], bodies[k]->d_strength, tc, 0, Uc.as<double>(), fpart.as<double>(), fpb)))
+            return st;
+    if ((st = fp_adj_cpw(ctx, lam.as<double>(), nrm, Uc.as<double>(), M, U, W2.as<double>(), nbar.as<double>())))
+        return st;
+    double s2[8];
+    if ((st = fp_mdot(ctx, W2.as<double>(), M, 6, ones.as<double>(), M, part.as<double>(), out8.as<double>()))) return st;
+    FP_HIP(ctx, hipMemcpyAsync(s2, out8.p, 6 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    col = 0;
+    for (int k = 0; k < nb; ++k) {
+        fp_body *b = bodies[k];
+        if ((st = fp_adj_pairs(ctx, b, cpt, M, W2.as<double>(), 0, Xbc.as<double>(), vbar[k].as<double>(), nullptr,
+                               0, work.p, wb)))
+            return st;
+        if ((st = fp_adj_wake(ctx, b, cpt, M, W2.as<double>(), 0, Xbc.as<double>(),
+                              hpart[k].as<double>() + (size_t)26 * std::max(1, b->nh), nullptr, col)))
+            return st;
+        col += b->N;
+    }
+    // geometry reverse and node gather; horseshoe end points and directions on the host
+    double dbar[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    col = 0;
+    int64_t noff = 0;
+    for (int k = 0; k < nb; ++k) {
+        fp_body *b = bodies[k];
+        const int64_t nn = (int64_t)b->nodes.size() / 3, N = b->N;
+        if ((st = fp_adj_geom_rev(ctx, b, o.probe_offset, o.probe_charlen, o.probe_length, M, col, nbar.as<double>(),
+                                  Xbp.as<double>(), Xbc.as<double>(), Ab.as<double>(), vbar[k].as<double>())))
+            return st;
+        // node -> panel-vertex slots (k * N + j)
+        std::vector<int64_t> ptr(nn + 1, 0), ent;
+        for (int64_t j = 0; j < N; ++j)
+            for (int64_t q = b->cell_ptr[j]; q < b->cell_ptr[j + 1]; ++q) ptr[b->cell_idx[q] + 1]++;
+        for (int64_t v = 0; v < nn; ++v) ptr[v + 1] += ptr[v];
+        ent.resize(ptr[nn]);
+        std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+        for (int64_t j = 0; j < N; ++j)
+            for (int64_t q = b->cell_ptr[j]; q < b->cell_ptr[j + 1]; ++q)
+                ent[fill[b->cell_idx[q]]++] = (q - b->cell_ptr[j]) * N + j;
+        DBuf dptr, dent, dout;
+        if ((st = dalloc(ctx, dptr, (nn + 1) * sizeof(int64_t))) ||
+            (st = dalloc(ctx, dent, std::max<size_t>(1, ent.size()) * sizeof(int64_t))) ||
+            (st = dalloc(ctx, dout, 6 * nn * sizeof(double))))
+            return st;
+        FP_HIP(ctx, hipMemcpyAsync(dptr.p, ptr.data(), ptr.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+        FP_HIP(ctx, hipMemcpyAsync(dent.p, ent.data(), ent.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+        if ((st = fp_adj_nodes(ctx, b, dptr.as<int64_t>(), dent.as<int64_t>(), nn, vbar[k].as<double>(),
+                               dout.as<double>())))
+            return st;
+        std::vector<double> nodeb(6 * nn), hp((size_t)2 * 26 * std::max(1, b->nh)), gam(N);
+        FP_HIP(ctx, hipMemcpyAsync(nodeb.data(), dout.p, nodeb.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        FP_HIP(ctx, hipMemcpyAsync(hp.data(), hpart[k].p, hp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        FP_HIP(ctx, hipMemcpyAsync(gam.data(), b->d_strength, N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        // horseshoes in the order build_horseshoes made them
+        int h = 0;
+        const size_t second = (size_t)26 * std::max(1, b->nh);
+        for (int64_t e = 0; e < b->ns; ++e) {
+            const int64_t *s = &b->shedding[6 * e];
+            for (int side = 0; side < 2; ++side) {
+                const int64_t pnl = side == 0 ? s[0] : s[3];
+                if (pnl == -1) continue;
+                const int64_t ia = b->cell_idx[b->cell_ptr[pnl] + s[3 * side + 1]];
+                const int64_t ib = b->cell_idx[b->cell_ptr[pnl] + s[3 * side + 2]];
+                const double G = gam[pnl];
+                for (int m = 0; m < 2; ++m)
+                    for (int d = 0; d < 3; ++d) {
+                        const double *h0 = &hp[(size_t)h * 26 + 12 * m], *h1 = &hp[second + (size_t)h * 26 + 12 * m];
+                        nodeb[(3 * m + d) * nn + ia] += G * (h0[d] + h1[d]);
+                        nodeb[(3 * m + d) * nn + ib] += G * (h0[3 + d] + h1[3 + d]);
+                        dbar[m][d] += G * (h0[6 + d] + h1[6 + d] + h0[9 + d] + h1[9 + d]);
+                    }
+                ++h;
+            }
+        }
+        for (int64_t v = 0; v < nn; ++v)
+            for (int d = 0; d < 3; ++d) {
+                if (dCL_dn) dCL_dn[3 * (noff + v) + d] = nodeb[d * nn + v];
+                if (dCD_dn) dCD_dn[3 * (noff + v) + d] = nodeb[(3 + d) * nn + v];
+            }
+        noff += nn;
+        col += N;
+    }
+    // freestream adjoint: probe velocities, Cp normalisation, no-flow-through
+    // RHS, and the direction Dhat = Uinf/|Uinf| (wake, Lhat, Dhat)
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const double u4 = un * un * un * un;
+    // objective directions: CL = Fn.Lhat, Lhat = w/|w|, w = Dhat x span; CDi = Fn.Dhat
+    {
+        const double lf = Lh[0] * Fn[0] + Lh[1] * Fn[1] + Lh[2] * Fn[2];
+        const double wb3[3] = {(Fn[0] - Lh[0] * lf) / wn, (Fn[1] - Lh[1] * lf) / wn, (Fn[2] - Lh[2] * lf) / wn};
+        // dbar += span x wbar
+        dbar[0][0] += sd[1] * wb3[2] - sd[2] * wb3[1];
+        dbar[0][1] += sd[2] * wb3[0] - sd[0] * wb3[2];
+        dbar[0][2] += sd[0] * wb3[1] - sd[1] * wb3[0];
+        for (int d = 0; d < 3; ++d) dbar[1][d] += Fn[d];
+    }
+    double Ub[2][3];
+    for (int m = 0; m < 2; ++m) {
+        const double dd = dh[0] * dbar[m][0] + dh[1] * dbar[m][1] + dh[2] * dbar[m][2];
+        for (int d = 0; d < 3; ++d)
+            Ub[m][d] = s1[3 * m + d] + 2 * s1[6 + m] * U[d] / u4 + s2[3 * m + d] + (dbar[m][d] - dh[d] * dd) / un;
+    }
+    if (dCL_dU)
+        for (int d = 0; d < 3; ++d) dCL_dU[d] = Ub[0][d];
+    if (dCD_dU)
+        for (int d = 0; d < 3; ++d) dCD_dU[d] = Ub[1][d];
+    S.ms_adjoint = tm.stop();
+    if (stats_out) *stats_out = S;
+    return FP_OK;
+}
+
+int fp_lift_drag(fp_ctx *ctx, fp_body *const *bodies, int nb, const double *Uinf, const fp_liftdrag_opts *opts,
+                 double *CL, double *CDi, fp_liftdrag_stats *stats)
+{
+    return liftdrag_run(ctx, bodies, nb, Uinf, opts, false, CL, CDi, nullptr, nullptr, nullptr, nullptr, stats);
+}
+
+int fp_lift_drag_adjoint(fp_ctx *ctx, fp_body *const *bodies, int nb, const double *Uinf, const fp_liftdrag_opts *opts,
+                         double *dCL_dnodes, double *dCDi_dnodes, double *dCL_dUinf, double *dCDi_dUinf,
+                         fp_liftdrag_stats *stats)
+{
+    return liftdrag_run(ctx, bodies, nb, Uinf, opts, true, nullptr, nullptr, dCL_dnodes, dCDi_dnodes, dCL_dUinf,
+                        dCDi_dUinf, stats);
 }
 
 }  // extern "C"

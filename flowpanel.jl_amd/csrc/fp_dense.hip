@@ -762,6 +762,100 @@ int fp_gemv(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, const double *x, dou
 template int fp_gemv<double>(fp_ctx *, const double *, int64_t, int64_t, const double *, double *, double, double);
 template int fp_gemv<float>(fp_ctx *, const float *, int64_t, int64_t, const double *, double *, double, double);
 
+// Y[:, c] = A X[:, c] for c < k (X, Y column-major with leading dimensions
+// ldx, ldy): GEMVM_NV vectors per grid row, so A is streamed ceil(k/GEMVM_NV)
+// times instead of k times.
+constexpr int GEMVM_NV = 8;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_multi(const T *__restrict__ A, int64_t n, int64_t ld,
+                                                    const double *__restrict__ X, int64_t ldx, int k,
+                                                    double *__restrict__ Y, int64_t ldy)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * GEMV_R;
+    if (r0 >= n) return;
+    const int nr = (int)min((int64_t)GEMV_R, n - r0);
+    const int c0 = blockIdx.y * GEMVM_NV;
+    const int nv = min(GEMVM_NV, k - c0);
+    double acc[GEMV_R][GEMVM_NV];
+#pragma unroll
+    for (int r = 0; r < GEMV_R; ++r)
+#pragma unroll
+        for (int q = 0; q < GEMVM_NV; ++q) acc[r][q] = 0;
+    const int64_t n4 = (n / 4) * 4;
+    for (int64_t c = 4 * lane; c < n4; c += 256) {
+        double v[GEMV_R][4];
+#pragma unroll
+        for (int r = 0; r < GEMV_R; ++r) {
+            if (r < nr) Vec4<T>::load(A + (r0 + r) * ld + c, v[r]);
+            else v[r][0] = v[r][1] = v[r][2] = v[r][3] = 0;
+        }
+#pragma unroll
+        for (int q = 0; q < GEMVM_NV; ++q) {
+            if (q < nv) {
+                const double *x = X + (int64_t)(c0 + q) * ldx + c;
+                const double x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3];
+#pragma unroll
+                for (int r = 0; r < GEMV_R; ++r) acc[r][q] += v[r][0] * x0 + v[r][1] * x1 + v[r][2] * x2 + v[r][3] * x3;
+            }
+        }
+    }
+    for (int64_t c = n4 + lane; c < n; c += 64)
+        for (int r = 0; r < nr; ++r)
+            for (int q = 0; q < nv; ++q) acc[r][q] += (double)A[(r0 + r) * ld + c] * X[(int64_t)(c0 + q) * ldx + c];
+#pragma unroll
+    for (int r = 0; r < GEMV_R; ++r)
+#pragma unroll
+        for (int q = 0; q < GEMVM_NV; ++q) {
+            double a = acc[r][q];
+            for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+            if (lane == 0 && r < nr && q < nv) Y[(int64_t)(c0 + q) * ldy + r0 + r] = a;
+        }
+}
+
+template <typename T>
+int fp_gemv_multi(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, const double *X, int64_t ldx, int64_t k, double *Y,
+                  int64_t ldy)
+{
+    if (n <= 0 || k <= 0) return FP_OK;
+    if (ld % 4 || ((uintptr_t)A & 15)) return fp_fail(ctx, FP_ERR_ARG, "gemv needs an aligned matrix");
+    ProfScope ps(ctx, sizeof(T) == 8 ? "gemv_multi_f64" : "gemv_multi_f32");
+    const dim3 grid((unsigned)((n + 4 * GEMV_R - 1) / (4 * GEMV_R)), (unsigned)((k + GEMVM_NV - 1) / GEMVM_NV));
+    hipLaunchKernelGGL(k_gemv_multi<T>, grid, dim3(256), 0, ctx->stream, A, n, ld, X, ldx, (int)k, Y, ldy);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+template int fp_gemv_multi<double>(fp_ctx *, const double *, int64_t, int64_t, const double *, int64_t, int64_t,
+                                   double *, int64_t);
+template int fp_gemv_multi<float>(fp_ctx *, const float *, int64_t, int64_t, const double *, int64_t, int64_t,
+                                  double *, int64_t);
+
+// column norms^2 of a column-major n x k block: out[c] = sum_i R[c*ld + i]^2
+__global__ __launch_bounds__(256) void k_colnorm2(const double *__restrict__ R, int64_t n, int64_t ld,
+                                                  double *__restrict__ out)
+{
+    __shared__ double red[4];
+    const int c = blockIdx.x;
+    double s = 0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        const double v = R[(int64_t)c * ld + i];
+        s += v * v;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[c] = red[0] + red[1] + red[2] + red[3];
+}
+
+int fp_colnorm2(fp_ctx *ctx, const double *R, int64_t n, int64_t ld, int64_t k, double *out)
+{
+    if (k <= 0) return FP_OK;
+    hipLaunchKernelGGL(k_colnorm2, dim3((unsigned)k), dim3(256), 0, ctx->stream, R, n, ld, out);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
 // ----------------------------------------------------------------------------
 // Block-Jacobi: LU with partial pivoting of each diagonal block (one
 // workgroup per block, the block copied to BJ in FP32 and factored there).

@@ -12,8 +12,13 @@
 //   there are few targets and the splits are summed in a fixed order.
 #include "fp_internal.h"
 #include "fp_elements.h"
+#include "fp_geometry.h"
 
 #include <algorithm>
+
+static_assert(FPG_CHARLEN_UNITARY == FP_CHARLEN_UNITARY && FPG_CHARLEN_BBOX == FP_CHARLEN_BBOX &&
+                  FPG_CHARLEN_MAXDIST == FP_CHARLEN_MAXDIST && FPG_CHARLEN_SQRTAREA == FP_CHARLEN_SQRTAREA,
+              "characteristic-length kinds");
 
 // ----------------------------------------------------------------------------
 // geometry: one thread per panel, FP64 (src/FLOWPanel_abstractbody.jl:635-927)
@@ -30,55 +35,8 @@ __global__ __launch_bounds__(256) void k_geometry(const double *__restrict__ pv,
 #pragma unroll
         for (int d = 0; d < 3; ++d) p[k][d] = pv[(3 * k + d) * N + j];
     const int nv = nvv[j];
-    // frame t, n, o from the first three vertices
-    double a[3] = {p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]};
-    double b[3] = {p[2][0] - p[0][0], p[2][1] - p[0][1], p[2][2] - p[0][2]};
-    const double na = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
-    double t[3] = {a[0] / na, a[1] / na, a[2] / na};
-    double c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
-    const double nc = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
-    double n[3] = {c[0] / nc, c[1] / nc, c[2] / nc};
-    double o[3] = {n[1] * t[2] - n[2] * t[1], n[2] * t[0] - n[0] * t[2], n[0] * t[1] - n[1] * t[0]};
-    // control point: vertex mean + in-plane centroid correction + normal offset
-    double m[3] = {0, 0, 0};
-    for (int k = 0; k < NVP; ++k)
-        if (k < nv) { m[0] += p[k][0]; m[1] += p[k][1]; m[2] += p[k][2]; }
-    m[0] /= nv; m[1] /= nv; m[2] /= nv;
-    double xs = 0, ys = 0;
-    for (int k = 0; k < NVP; ++k)
-        if (k < nv) {
-            const double dx = p[k][0] - m[0], dy = p[k][1] - m[1], dz = p[k][2] - m[2];
-            xs += dx * t[0] + dy * t[1] + dz * t[2];
-            ys += dx * o[0] + dy * o[1] + dz * o[2];
-        }
-    xs /= nv; ys /= nv;
-    for (int d = 0; d < 3; ++d) m[d] += xs * t[d] + ys * o[d];
-    // area: fan triangulation from the first vertex
-    double ac[3] = {0, 0, 0};
-    for (int k = 1; k < NVP - 1; ++k)
-        if (k + 1 < nv) {
-            const double u0 = p[k][0] - p[0][0], u1 = p[k][1] - p[0][1], u2 = p[k][2] - p[0][2];
-            const double w0 = p[k + 1][0] - p[0][0], w1 = p[k + 1][1] - p[0][1], w2 = p[k + 1][2] - p[0][2];
-            ac[0] += u1 * w2 - u2 * w1; ac[1] += u2 * w0 - u0 * w2; ac[2] += u0 * w1 - u1 * w0;
-        }
-    const double area = 0.5 * sqrt(ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2]);
-    double l = 1.0;
-    if (charlen == FP_CHARLEN_BBOX) {
-        double lo[3] = {p[0][0], p[0][1], p[0][2]}, hi[3] = {p[0][0], p[0][1], p[0][2]};
-        for (int k = 0; k < NVP; ++k)
-            if (k < nv)
-                for (int d = 0; d < 3; ++d) { lo[d] = fmin(lo[d], p[k][d]); hi[d] = fmax(hi[d], p[k][d]); }
-        l = sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) + (hi[2] - lo[2]) * (hi[2] - lo[2]));
-    } else if (charlen == FP_CHARLEN_MAXDIST) {
-        l = 0;
-        for (int k = 0; k < NVP; ++k)
-            if (k < nv) {
-                const double dx = p[0][0] - p[k][0], dy = p[0][1] - p[k][1], dz = p[0][2] - p[k][2];
-                l = fmax(l, sqrt(dx * dx + dy * dy + dz * dz));
-            }
-    } else if (charlen == FP_CHARLEN_SQRTAREA) {
-        l = sqrt(area);
-    }
+    double t[3], n[3], o[3], m[3], area, l;
+    panel_geom<double, NVP>(p, nv, charlen, t, n, o, m, area, l);
     for (int d = 0; d < 3; ++d) m[d] += cpoff * l * n[d];
     geo[0 * N + j] = n[0]; geo[1 * N + j] = n[1]; geo[2 * N + j] = n[2];
     geo[3 * N + j] = m[0]; geo[4 * N + j] = m[1]; geo[5 * N + j] = m[2];
@@ -506,6 +464,15 @@ __global__ void k_rhs(const double *__restrict__ uinf, const double *__restrict_
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= N) return;
     rhs[j] = -(uinf[j] * geo[j] + uinf[N + j] * geo[N + j] + uinf[2 * N + j] * geo[2 * N + j]);
+}
+
+int fp_rhs_from(fp_ctx *ctx, const fp_body *b, const double *uinf_soa, double *rhs)
+{
+    if (b->N == 0) return FP_OK;
+    hipLaunchKernelGGL(k_rhs, dim3((unsigned)((b->N + 255) / 256)), dim3(256), 0, ctx->stream, uinf_soa, b->d_geo,
+                       b->N, rhs);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
 }
 
 int fp_rhs_kernel(fp_ctx *ctx, const fp_body *b, double *rhs)

@@ -185,6 +185,12 @@ size_t fp_getrs_work_bytes(int64_t n, int64_t k);
 // y = alpha*A*x + beta*y, A row-major (T), x,y FP64
 template <typename T>
 int fp_gemv(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, const double *x, double *y, double alpha, double beta);
+// Y[:, c] = A X[:, c], c < k (X, Y column-major)
+template <typename T>
+int fp_gemv_multi(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, const double *X, int64_t ldx, int64_t k, double *Y,
+                  int64_t ldy);
+// out[c] = |R[:, c]|^2 (device)
+int fp_colnorm2(fp_ctx *ctx, const double *R, int64_t n, int64_t ld, int64_t k, double *out);
 // batched block-Jacobi LU of the diagonal blocks (FP32 factors)
 template <typename T>
 int fp_bj_factor(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *BJ, int *piv, int *info_dev);
@@ -205,8 +211,45 @@ int fp_mat_convert(fp_ctx *ctx, const double *src, int64_t n, int64_t lds, void 
                    int transpose);
 int fp_mat_narrow(fp_ctx *ctx, const double *src, float *dst, int64_t n, int64_t ld);
 int fp_rhs_kernel(fp_ctx *ctx, const fp_body *b, double *rhs);   // rhs_i = -Uinf_i . n_i
+int fp_rhs_from(fp_ctx *ctx, const fp_body *b, const double *uinf_soa, double *rhs);  // the same for a 3 x N SoA Uinf
 int fp_residual_update(fp_ctx *ctx, const double *b, double *r, int64_t n);   // r = b - r
 int fp_dot(fp_ctx *ctx, const double *x, const double *y, int64_t n, double *d_out);
 int fp_axpy(fp_ctx *ctx, double a, const double *x, double *y, int64_t n);
 
-// ---- Krylov (fp_krylov.hip / fp_abi.hip) ---------------------------------
+// multi-dot out[c] = sum_i V[c*ldv + i] w[i] (part: 256*nv doubles)
+int fp_mdot(fp_ctx *ctx, const double *V, int64_t ldv, int nv, const double *w, int64_t n, double *part,
+            double *out);
+
+// ---- lift/drag adjoint (fp_adjoint.hip) ----------------------------------
+// Per-panel arrays are SoA over all M panels of the bodies (panel i = col0 + j
+// of its body); the two objectives (CL, CDi) are stacked, component 3m + d.
+// normals, areas, control points and probes of one body
+int fp_adj_geometry(fp_ctx *ctx, const fp_body *b, double poff, int pkind, double pscale, int64_t M, int64_t col0,
+                    double *nrm, double *area, double *cpt, double *prb);
+// V (3 x M, induced velocity) += Uinf; Cp = 1 - |V|^2/|Uinf|^2
+int fp_adj_cp(fp_ctx *ctx, double *V, int64_t M, const double *Uinf, double *Cp);
+// Kutta averaging of c0 (reverse = 0) or its reverse on c0 and c1
+int fp_adj_kutta(fp_ctx *ctx, const int64_t *d_sh, int64_t ns, int mode, double *c0, double *c1, int reverse);
+int fp_adj_force_terms(fp_ctx *ctx, const double *Cp, const double *area, const double *nrm, int64_t M, double *T);
+int fp_adj_objective(fp_ctx *ctx, const double *Cp, const double *area, const double *nrm, int64_t M,
+                     const double *e0, const double *e1, double inv_s, double *Cpb, double *Ab, double *nb);
+int fp_adj_vbar(fp_ctx *ctx, const double *Cpb, const double *V, int64_t M, double inv_u2, double *W);
+size_t fp_adj_pairs_work_bytes(fp_ctx *ctx, int64_t N, int nvp, int64_t MT);
+// reverse of the rings of one body at MT targets with weights W (6 x MT):
+// Xb (6 x MT) +=, vbar (6*nvp x N) +=, and (gamma) g[m*ldg + j] += u_j . w
+int fp_adj_pairs(fp_ctx *ctx, const fp_body *b, const double *tx, int64_t MT, const double *W, int gamma,
+                 double *Xb, double *vbar, double *g, int64_t ldg, void *work, size_t work_bytes);
+// the same for the body's wake horseshoes; hpart (nh x 26) receives the
+// unit-strength horseshoe parameter adjoints (and strength adjoints)
+int fp_adj_wake(fp_ctx *ctx, const fp_body *b, const double *tx, int64_t MT, const double *W, int gamma, double *Xb,
+                double *hpart, double *g, int64_t col0);
+int fp_adj_cpw(fp_ctx *ctx, const double *lam, const double *nrm, const double *Uc, int64_t M, const double *Uinf,
+               double *W2, double *nb);
+int fp_adj_geom_rev(fp_ctx *ctx, const fp_body *b, double poff, int pkind, double pscale, int64_t M, int64_t col0,
+                    const double *nb, const double *Xbp, const double *Xbc, const double *Ab, double *vbar);
+int fp_adj_nodes(fp_ctx *ctx, const fp_body *b, const int64_t *ptr, const int64_t *ent, int64_t nn,
+                 const double *vbar, double *out);
+// B = A^T (row-major n x n, leading dimension ld for both), FP64 or narrowed to FP32
+int fp_mat_transpose(fp_ctx *ctx, const double *A, int64_t n, int64_t ld, void *B, int dst_prec);
+
+// ---- Krylov (fp_abi.hip) ---------------------------------------------------

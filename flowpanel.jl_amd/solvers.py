@@ -148,6 +148,39 @@ def solve(body, Uinfs, Das=None, Dbs=None, solver=solve_ludiv, solver_optargs=No
     return stats
 
 
+def solve_freestreams(body, Uinfs, Das=None, Dbs=None, solver=solve_ludiv, solver_optargs=None):
+    """Strengths for k freestreams under the fixed-wake convention.
+
+    ``Uinfs`` is 3 x N x k (or a list of k arrays 3 x N); the wake directions
+    ``Das``/``Dbs`` (3 x ns) are shared by every case, so the AIC is assembled
+    and factorised (or preconditioned) once and the k freestreams are solved as
+    k right-hand sides.  Returns (strengths N x k, stats).  The body keeps the
+    wake directions; its stored strengths are left unchanged.
+    """
+    if solver not in _CODES:
+        raise ValueError("solver must be one of %s" % [f.__name__ for f in _CODES])
+    U = np.asarray(Uinfs, dtype=np.float64)
+    if U.ndim == 3 and U.shape[0] != 3 and U.shape[1] == 3:
+        U = np.transpose(U, (1, 2, 0))  # k x 3 x N -> 3 x N x k
+    if U.ndim != 3 or U.shape[:2] != (3, body.ncells):
+        raise nat.FPError(nat.FP_ERR_SHAPE, "Invalid Uinfs; expected size (3, %d, k), got %s"
+                          % (body.ncells, U.shape))
+    k = U.shape[2]
+    set_freestream(body, U[:, :, 0], Das, Dbs)
+    Uk = np.ascontiguousarray(np.transpose(U, (2, 1, 0)))  # k x N x 3: column-major 3 x N x k
+    G = np.zeros((body.ncells, k), order="F")
+    o = nat.opts(_CODES[solver], **(solver_optargs or {}))
+    s = nat.SolveStats()
+    ctx = body.ctx
+    comps = body.bodies
+    ctx.check(nat.lib().fp_solve_bodies_multi(ctx.handle, nat.handles(comps), len(comps), nat.dptr(Uk), k,
+                                              ctypes.byref(o), nat.dptr(G), ctypes.byref(s)))
+    stats = s.as_dict()
+    last_stats.clear()
+    last_stats.update(stats)
+    return G, stats
+
+
 def assemble(body, precision="f64"):
     """Device AIC of the body (column-major numpy copy), _G_U! of the reference."""
     ctx = body.ctx

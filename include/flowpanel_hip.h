@@ -145,6 +145,17 @@ int fp_body_get_strength(fp_ctx *ctx, const fp_body *body, double *strength);
 int fp_solve_bodies(fp_ctx *ctx, fp_body *const *bodies, int nbodies, const fp_solve_opts *opts,
                     fp_solve_stats *stats);
 
+/* The same for k freestreams at once under the fixed-wake convention: the wake
+ * directions stored by fp_body_set_freestream are shared by all k cases, so the
+ * AIC is one matrix and the freestreams are k right-hand sides (one assembly, one
+ * factorisation or preconditioner; the refinement paths refine all columns
+ * together).  Uinfs: 3 x N x k (N = sum of ncells, column-major per freestream);
+ * strengths: N x k column-major.  The bodies' stored strengths are not changed.
+ * The reference solves one freestream per call (src/FLOWPanel_liftingbody.jl:172-216)
+ * and takes several RHS columns in its solver protocol (src/FLOWPanel_linearsolver.jl:149-157). */
+int fp_solve_bodies_multi(fp_ctx *ctx, fp_body *const *bodies, int nbodies, const double *Uinfs, int64_t k,
+                          const fp_solve_opts *opts, double *strengths, fp_solve_stats *stats);
+
 /* ---- dense AIC and the solver protocol ---------------------------------- */
 /* _G_U!/_G_Uvortexring! (src/FLOWPanel_liftingbody.jl:455-572,
  * src/FLOWPanel_nonliftingbody.jl:141-186, src/FLOWPanel_multibody.jl:502-555):
@@ -183,6 +194,62 @@ int fp_calcfield_U(fp_ctx *ctx, fp_body *const *bodies, int nbodies, double *out
  * out (n) += 1 - (|U|/Uref)^2, then Kutta averaging over the shedding edges. */
 int fp_calcfield_cp(fp_ctx *ctx, const double *Us, int64_t n, double uref, const int64_t *shedding,
                     int64_t nsheddings, int index_base, int kutta_mode, double *out);
+
+/* ---- shape updates ------------------------------------------------------- */
+int64_t fp_body_nnodes(const fp_body *body);
+/* Move the mesh nodes (3 x nnodes, same connectivity) as the reference does by
+ * editing grid._nodes in place: the device geometry and wake horseshoes are
+ * recomputed and the body is marked unsolved. */
+int fp_body_set_nodes(fp_ctx *ctx, fp_body *body, const double *nodes);
+
+/* ---- lift/drag coefficients and their adjoint ----------------------------
+ * The reference's offset-probe force workflow (test/runtests_solvers.jl:136-166)
+ * on vortex-ring bodies, as one function of the mesh nodes and Uinf:
+ *   solve with Uinfs = Uinf at every control point and wake directions
+ *     Das = Dbs = Uinf/|Uinf| (src/FLOWPanel_liftingbody.jl:172-216);
+ *   velocity at the probes centroid + probe_offset * l * n, l = probe_length x
+ *     the characteristic length of kind probe_charlen (calcfield_U with offset and
+ *     characteristiclength, src/FLOWPanel_postprocess.jl:81-99);
+ *   Cp = 1 - |U|^2/|Uinf|^2 with Kutta averaging (:1059-1112);
+ *   F_i = -Cp_i q A_i n_i (:1199-1307);
+ *   CL = F.Lhat/(q S), CDi = F.Dhat/(q S), Dhat = Uinf/|Uinf|,
+ *     Lhat = normalize(Dhat x span_dir) (calcfield_LDS, :1615-1652).
+ * The Kutta override of calcfield_F re-averages values that are already
+ * averaged, so it is the identity here and is not applied twice. */
+typedef struct fp_liftdrag_opts {
+    double probe_offset;  /* reference test: 0.02                           */
+    double probe_length;  /* reference test: b/ar (with FP_CHARLEN_UNITARY) */
+    int probe_charlen;    /* FP_CHARLEN_*                                    */
+    int kutta_mode;       /* FP_KUTTA_*                                      */
+    double span_dir[3];   /* Shat, reference test: (0, 1, 0)                 */
+    double ref_area;      /* S, reference test: b^2/ar                       */
+    fp_solve_opts solve;  /* forward solve; the adjoint's transpose solve uses the same solver */
+} fp_liftdrag_opts;
+
+typedef struct fp_liftdrag_stats {
+    double CL, CDi;
+    double ms_forward;          /* freestream, assembly, solve, probes, coefficients      */
+    double ms_adjoint;          /* reverse kernels, transpose solve, gathers               */
+    double ms_transpose_solve;  /* transposition, set-up and solve of A^T lambda = dJ/dG  */
+    int iterations_adjoint;
+    fp_solve_stats forward;     /* the forward solve                                       */
+} fp_liftdrag_stats;
+
+void fp_liftdrag_opts_default(fp_liftdrag_opts *opts, int solver);
+/* CL and CDi of the bodies at Uinf (3); stats may be NULL.  The strengths of the
+ * solve stay in the bodies, as after fp_solve_bodies. */
+int fp_lift_drag(fp_ctx *ctx, fp_body *const *bodies, int nbodies, const double *Uinf,
+                 const fp_liftdrag_opts *opts, double *CL, double *CDi, fp_liftdrag_stats *stats);
+/* Forward as fp_lift_drag, then the hand-written reverse pass (transpose solve
+ * plus the reverse of the geometry, influence and post-processing kernels):
+ * dCL/dnodes and dCDi/dnodes (3 x nnodes per body, bodies concatenated) and
+ * dCL/dUinf, dCDi/dUinf (3).  Any gradient output may be NULL.  The ImplicitAD
+ * rule of the reference's linear solve (src/FLOWPanel_linearsolver.jl:73,128)
+ * is the transpose solve here; the solver is opts->solve.solver with the FP64
+ * residual stored (mixed precision: FP32 inner solves, FP64 refinement). */
+int fp_lift_drag_adjoint(fp_ctx *ctx, fp_body *const *bodies, int nbodies, const double *Uinf,
+                         const fp_liftdrag_opts *opts, double *dCL_dnodes, double *dCDi_dnodes,
+                         double *dCL_dUinf, double *dCDi_dUinf, fp_liftdrag_stats *stats);
 
 /* ---- instrumentation (no counterpart in the reference) ------------------
  * Per-kernel device time measured with HIP events on the context stream, for
