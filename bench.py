@@ -115,6 +115,92 @@ def vendor_lu(fp, body, N):
         return {"error": str(e)[:200]}
 
 
+def batch_legs(fp, ctx, reps=3):
+    """Batched workloads: 64 freestreams as 64 RHS, and 16 uneven wings in one captured graph."""
+    out = {}
+    # (i) the 4,800-panel swept wing at 64 freestreams (alpha U[-5,10], beta U[-4,4], seed 3),
+    # fixed-wake convention (wake along +x for every case): one AIC, 64 right-hand sides
+    g, sh = fp.geometry.loft_wing(airfoil="0012", n_chord=60, n_span=80, span=10.0, root_chord=1.0, taper=0.5,
+                                  sweep_deg=30.0)
+    wing = fp.RigidWakeBody(g, sh)
+    rng = np.random.Generator(np.random.PCG64(3))
+    al, be = rng.uniform(-5, 10, 64), rng.uniform(-4, 4, 64)
+    V = np.stack([fp.direction(a, b) for a, b in zip(al, be)], axis=1)
+    U = np.repeat(V[:, None, :], wing.ncells, axis=1)
+    D = np.tile(np.array([[1.0], [0.0], [0.0]]), (1, wing.nsheddings))
+    rec = {}
+    for name, solver in (("lu32_ir", fp.solve_lu32_ir), ("lu64", fp.solve_ludiv)):
+        fp.solve_freestreams(wing, U, D, D, solver=solver)
+        walls = []
+        for _ in range(reps):
+            ctx.synchronize()
+            t = time.perf_counter()
+            G, st = fp.solve_freestreams(wing, U, D, D, solver=solver)
+            walls.append((time.perf_counter() - t) * 1e3)
+        rec[name] = {"ms": round(min(walls), 3), "stats": st}
+        rec[name + "_G"] = G
+    # 64 independent single solves for comparison (the same mixed solver)
+    ctx.synchronize()
+    t = time.perf_counter()
+    for c in range(64):
+        fp.solve(wing, U[:, :, c], D, D, solver=fp.solve_lu32_ir)
+    ctx.synchronize()
+    seq_ms = (time.perf_counter() - t) * 1e3
+    err = float(np.abs(rec["lu32_ir_G"] - rec["lu64_G"]).max() / np.abs(rec["lu64_G"]).max())
+    out["freestreams"] = {
+        "workload": "wing NACA 0012, AR 10, sweep 30, taper 0.5, 60 x 80 quads (4,800 panels), 64 freestreams "
+                    "(alpha U[-5,10], beta U[-4,4], seed 3), wake along +x for all (fixed-wake convention)",
+        "lu32_ir_ms": rec["lu32_ir"]["ms"], "lu64_ms": rec["lu64"]["ms"], "sequential_64_solves_ms": round(seq_ms, 3),
+        "gamma_rel_err_mixed_vs_fp64": err, "lu32_ir_stats": rec["lu32_ir"]["stats"],
+    }
+    # (ii) 16 lofted wings, AR U[6,12], sweep U[0,35] deg, N U[1k,12k] (seed 3), one captured graph
+    rng = np.random.Generator(np.random.PCG64(3))
+    bodies = []
+    for _ in range(16):
+        ar, sweep, n = rng.uniform(6, 12), rng.uniform(0, 35), int(rng.uniform(1000, 12000))
+        nc = max(10, int(round(np.sqrt(n / ar) * 2.0)))
+        ns = max(4, int(round(n / nc)))
+        g, sh = fp.geometry.loft_wing(airfoil="0012", n_chord=nc, n_span=ns, span=ar * 0.75, root_chord=1.0,
+                                      taper=0.5, sweep_deg=sweep)
+        b = fp.RigidWakeBody(g, sh)
+        Vb = fp.direction(4.0)
+        fp.solvers.set_freestream(b, np.tile(Vb[:, None], (1, b.ncells)), np.tile(Vb[:, None], (1, b.nsheddings)),
+                                  np.tile(Vb[:, None], (1, b.nsheddings)))
+        bodies.append(b)
+    t = time.perf_counter()
+    batch = fp.BodyBatch(bodies, solver=fp.solve_lu32_ir, solver_optargs={"residual_mode": 0, "tol": 1e-10})
+    ctx.synchronize()
+    create_ms = (time.perf_counter() - t) * 1e3
+    batch.solve()
+    walls, stats = [], None
+    for _ in range(reps):
+        ctx.synchronize()
+        t = time.perf_counter()
+        stats = batch.solve()
+        walls.append((time.perf_counter() - t) * 1e3)
+    gam = [b.strength.copy() for b in bodies]
+    # per-body eager solves with the same solver for comparison
+    ctx.synchronize()
+    t = time.perf_counter()
+    for b in bodies:
+        Vb = fp.direction(4.0)
+        fp.solve(b, np.tile(Vb[:, None], (1, b.ncells)), np.tile(Vb[:, None], (1, b.nsheddings)),
+                 np.tile(Vb[:, None], (1, b.nsheddings)), solver=fp.solve_lu32_ir, solver_optargs={"residual_mode": 0,
+                                                                                                   "tol": 1e-10})
+    ctx.synchronize()
+    seq_ms = (time.perf_counter() - t) * 1e3
+    err = max(float(np.abs(g0 - b.strength).max() / np.abs(b.strength).max()) for g0, b in zip(gam, bodies))
+    out["bodies"] = {
+        "workload": "16 lofted NACA 0012 wings, AR U[6,12], sweep U[0,35] deg, N U[1k,12k] (seed 3), "
+                    "FP32 AIC + FP32 LU, matrix-free FP64 residuals",
+        "panels": [b.ncells for b in bodies], "total_panels": int(sum(b.ncells for b in bodies)),
+        "graph_replay_ms": round(min(walls), 3), "graph_device_ms": round(stats["ms_graph"], 3),
+        "create_capture_autotune_ms": round(create_ms, 1), "sequential_solves_ms": round(seq_ms, 3),
+        "stats": stats, "autotune": batch.autotune_table(), "gamma_rel_err_vs_sequential": err,
+    }
+    return out
+
+
 def adjoint_leg(fp, ctx, fd_coords=64, reps=3):
     """Lift/drag adjoint of the 10,000-panel wing; returns the JSON record."""
     grid, sh = fp.geometry.loft_wing(airfoil="2412", n_chord=100, n_span=100, span=8.0, root_chord=1.0, taper=1.0,
@@ -216,6 +302,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 10,000-panel adjoint leg")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg (prints its record)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched legs (64 freestreams, 16 wings)")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched legs (prints their record)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
@@ -226,6 +314,9 @@ def main():
     ctx = fp.context()
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_leg(fp, ctx)}), flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps({"batch": batch_legs(fp, ctx)}), flush=True)
         return
     alpha = 5.0
     log("bench: building BWB %d x %d" % (args.chord, args.span))
@@ -305,6 +396,9 @@ def main():
     adj = None
     if not args.no_adjoint:
         adj = adjoint_leg(fp, ctx)
+    bat = None
+    if not args.no_batch:
+        bat = batch_legs(fp, ctx)
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")
@@ -321,7 +415,7 @@ def main():
         "fp64_lu_ms": round(t_fp64, 3), "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
         "cp_err_max": cp_err, "gamma_rel_err": gam_err,
         "mixed_stats": stats[-1], "fp64_stats": st64,
-        "adjoint_ms": adj["adjoint_total_ms"] if adj else None, "adjoint": adj,
+        "adjoint_ms": adj["adjoint_total_ms"] if adj else None, "adjoint": adj, "batch": bat,
         "kernels": kernels, "kernels_fp64": kernels_fp64, "vendor_fp64_getrf": vendor, "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -46,6 +46,7 @@ EXPORTS = [
     "fp_mat_getrf", "fp_uind", "fp_phi", "fp_calcfield_U", "fp_calcfield_cp", "fp_ctx_profile",
     "fp_ctx_profile_get", "fp_ctx_copy_calibrate", "fp_body_nnodes", "fp_body_set_nodes",
     "fp_liftdrag_opts_default", "fp_lift_drag", "fp_lift_drag_adjoint", "fp_solve_bodies_multi",
+    "fp_batch_create", "fp_batch_solve", "fp_batch_destroy", "fp_batch_autotune_get",
 ]
 
 
@@ -79,6 +80,16 @@ class LiftDragStats(ctypes.Structure):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "forward"}
         d["forward"] = self.forward.as_dict()
         return d
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("ms_graph", ctypes.c_double), ("ms_capture", ctypes.c_double), ("ms_extra", ctypes.c_double),
+                ("residual", ctypes.c_double), ("panels", ctypes.c_int64), ("nbodies", ctypes.c_int),
+                ("sweeps_in_graph", ctypes.c_int), ("extra_sweeps", ctypes.c_int), ("graph_nodes", ctypes.c_int),
+                ("recaptured", ctypes.c_int), ("tile_targets", ctypes.c_int), ("queue_workgroups", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class FPError(RuntimeError):
@@ -145,6 +156,11 @@ def lib():
     L.fp_ctx_copy_calibrate.argtypes = [_vp, i64]
     L.fp_solve_bodies_multi.argtypes = [_vp, pp, cint, _d, i64, ctypes.POINTER(SolveOpts), _d,
                                         ctypes.POINTER(SolveStats)]
+    L.fp_batch_create.argtypes = [_vp, pp, cint, ctypes.POINTER(SolveOpts), pp]
+    L.fp_batch_solve.argtypes = [_vp, _vp, ctypes.POINTER(BatchStats)]
+    L.fp_batch_destroy.argtypes = [_vp]
+    L.fp_batch_destroy.restype = None
+    L.fp_batch_autotune_get.argtypes = [_vp, cint, ctypes.POINTER(cint), ctypes.POINTER(cint), _d]
     L.fp_body_nnodes.argtypes = [_vp]
     L.fp_body_nnodes.restype = i64
     L.fp_body_set_nodes.argtypes = [_vp, _vp, _d]

@@ -161,7 +161,8 @@ struct MatrixFreeOp : Operator {
     }
     int apply(const double *x, double *y) override
     {
-        FP_HIP(ctx, hipMemsetAsync(y, 0, n * sizeof(double), ctx->stream));
+        int st0 = fp_fill(ctx, y, n, 0.0);
+        if (st0) return st0;
         int64_t col = 0;
         for (auto *b : bodies) {
             int st = fp_launch_field(ctx, b, x + col, tv, 1, y, part.as<double>(), part_bytes);
@@ -624,6 +625,7 @@ int upload_soa3(fp_ctx *ctx, const double *h, int64_t n, double *d)
 
 int build_horseshoes(fp_ctx *ctx, fp_body *b)
 {
+    ++b->version;
     if (b->d_hh) { (void)hipFree(b->d_hh); b->d_hh = nullptr; }
     if (b->d_hcol) { (void)hipFree(b->d_hcol); b->d_hcol = nullptr; }
     b->nh = 0;
@@ -708,6 +710,247 @@ __global__ void k_kutta(const int64_t *__restrict__ sh, int64_t ns, int mode, do
         }
     }
 }
+
+
+// ----------------------------------------------------------------------------
+// Batch of independent bodies (each its own system, no mutual influence).
+// One persistent queue launch assembles every body's vortex rings; each body's
+// wake columns, factorisation and refinement sweeps then run on one of
+// BATCH_STREAMS parallel streams.  The whole sequence is captured once into a
+// hipGraph and replayed; bodies are re-captured only when their device
+// buffers change.
+constexpr int BATCH_STREAMS = 4;
+
+struct BatchBody {
+    fp_body *b = nullptr;
+    uint64_t version = 0;
+    int64_t n = 0, ld = 0;
+    int stream = 0;
+    Targets t;
+    DBuf A64, A32, rhs, x, r, d, perm, info, fwork, swork;
+    std::unique_ptr<MatrixFreeOp> mf;
+};
+
+struct BatchImpl {
+    fp_ctx *ctx = nullptr;
+    fp_solve_opts o{};
+    std::vector<std::unique_ptr<BatchBody>> bodies;
+    int nvp = 4, tm = 64, blocks = 0, sweeps = 3;
+    double tune_ms[9] = {0};
+    int tune_tm[9] = {0}, tune_blocks[9] = {0}, ntune = 0;
+    int64_t total = 0;
+    DBuf items, counter, norms;
+    hipStream_t streams[BATCH_STREAMS] = {};
+    hipEvent_t fork = nullptr, join[BATCH_STREAMS] = {};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_nodes = 0;
+    bool lu32() const { return o.solver == FP_SOLVER_LU32_IR; }
+    bool stored() const { return o.solver == FP_SOLVER_LU || o.residual_mode == FP_RESID_STORED; }
+
+    ~BatchImpl()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto &s : streams)
+            if (s) (void)hipStreamDestroy(s);
+        if (fork) (void)hipEventDestroy(fork);
+        for (auto &e : join)
+            if (e) (void)hipEventDestroy(e);
+    }
+
+    int alloc()
+    {
+        int st;
+        for (int q = 0; q < BATCH_STREAMS; ++q) {
+            FP_HIP(ctx, hipStreamCreateWithFlags(&streams[q], hipStreamNonBlocking));
+            FP_HIP(ctx, hipEventCreateWithFlags(&join[q], hipEventDisableTiming));
+        }
+        FP_HIP(ctx, hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        if ((st = dalloc(ctx, counter, 16)) || (st = dalloc(ctx, norms, 2 * bodies.size() * sizeof(double))))
+            return st;
+        // balance the factorisations (n^3) over the streams, largest first
+        std::vector<int> order(bodies.size());
+        for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
+        std::sort(order.begin(), order.end(), [&](int a, int b) { return bodies[a]->n > bodies[b]->n; });
+        double load[BATCH_STREAMS] = {0};
+        for (int k : order) {
+            int q = (int)(std::min_element(load, load + BATCH_STREAMS) - load);
+            bodies[k]->stream = q;
+            load[q] += std::pow((double)bodies[k]->n, 3);
+        }
+        for (auto &bb : bodies) {
+            const int64_t n = bb->n, ld = bb->ld;
+            if (stored() && (st = dalloc(ctx, bb->A64, (size_t)n * ld * sizeof(double)))) return st;
+            if (lu32() && (st = dalloc(ctx, bb->A32, (size_t)n * ld * sizeof(float)))) return st;
+            if ((st = dalloc(ctx, bb->rhs, n * sizeof(double))) || (st = dalloc(ctx, bb->x, n * sizeof(double))) ||
+                (st = dalloc(ctx, bb->r, n * sizeof(double))) || (st = dalloc(ctx, bb->d, n * sizeof(double))) ||
+                (st = dalloc(ctx, bb->perm, n * sizeof(int))) || (st = dalloc(ctx, bb->info, sizeof(int))) ||
+                (st = dalloc(ctx, bb->fwork, fp_getrf_work_bytes(n, lu32() ? sizeof(float) : sizeof(double)))) ||
+                (st = dalloc(ctx, bb->swork, fp_getrs_work_bytes(n, 1))))
+                return st;
+        }
+        return FP_OK;
+    }
+
+    // (re)build the device table of the assembly queue
+    int build_items()
+    {
+        std::vector<AssemblyItem> it;
+        total = 0;
+        for (auto &bb : bodies) {
+            AssemblyItem a;
+            a.pan = panel_view(bb->b);
+            a.tv = bb->t.tv;
+            a.G = stored() ? bb->A64.p : bb->A32.p;
+            a.ld = bb->ld;
+            a.tile0 = total;
+            int nbx;
+            int64_t nt;
+            fp_assembly_queue_tiles(bb->n, bb->n, tm, &nbx, &nt);
+            a.nbx = nbx;
+            total += nt;
+            it.push_back(a);
+        }
+        int st = dalloc(ctx, items, it.size() * sizeof(AssemblyItem));
+        if (st) return st;
+        FP_HIP(ctx, hipMemcpy(items.p, it.data(), it.size() * sizeof(AssemblyItem), hipMemcpyHostToDevice));
+        return FP_OK;
+    }
+
+    int assemble_queue()
+    {
+        return fp_launch_assemble_queue(ctx, items.as<AssemblyItem>(), (int)bodies.size(), total, nvp, tm,
+                                        stored() ? FP_F64 : FP_F32, counter.as<int>(), blocks);
+    }
+
+    // time the queue assembly for each tile shape and workgroup count; keep the fastest
+    int autotune()
+    {
+        const int tms[3] = {32, 64, 128}, per_cu[3] = {2, 4, 8};
+        double best = 1e300;
+        int btm = 64, bbl = 4 * ctx->num_cu;
+        ntune = 0;
+        for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c) {
+                tm = tms[a];
+                blocks = per_cu[c] * ctx->num_cu;
+                int st = build_items();
+                if (st) return st;
+                if ((st = assemble_queue())) return st;  // warm
+                Timer t(ctx->stream);
+                if ((st = assemble_queue())) return st;
+                const double ms = t.stop();
+                tune_ms[ntune] = ms;
+                tune_tm[ntune] = tm;
+                tune_blocks[ntune] = blocks;
+                ++ntune;
+                if (ms < best) { best = ms; btm = tm; bbl = blocks; }
+            }
+        tm = btm;
+        blocks = bbl;
+        return build_items();
+    }
+
+    // the work of one body after the queue assembly, on the current ctx->stream
+    int issue_body(BatchBody &bb)
+    {
+        fp_body *b = bb.b;
+        const int64_t n = bb.n, ld = bb.ld;
+        int st;
+        double *x = bb.x.as<double>(), *r = bb.r.as<double>(), *rhs = bb.rhs.as<double>();
+        double *nb = norms.as<double>();
+        size_t k = 0;
+        while (bodies[k].get() != &bb) ++k;
+        if (stored()) {
+            if ((st = fp_launch_assemble_wake<double>(ctx, b, bb.t.tv, bb.A64.as<double>(), ld, 0))) return st;
+        } else {
+            if ((st = fp_launch_assemble_wake<float>(ctx, b, bb.t.tv, bb.A32.as<float>(), ld, 0))) return st;
+        }
+        if (lu32() && stored() && (st = fp_mat_narrow(ctx, bb.A64.as<double>(), bb.A32.as<float>(), n, ld))) return st;
+        if ((st = fp_rhs_kernel(ctx, b, rhs))) return st;
+        if ((st = fp_colnorm2(ctx, rhs, n, n, 1, nb + k))) return st;
+        if (!lu32()) {
+            // FP64 LU + one substitution; the FP64 residual is reported
+            if ((st = fp_getrf<double>(ctx, bb.A64.as<double>(), n, ld, bb.perm.as<int>(), bb.info.as<int>(), bb.fwork.p,
+                                       fp_getrf_work_bytes(n, sizeof(double)))))
+                return st;
+            FP_HIP(ctx, hipMemcpyAsync(x, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            if ((st = fp_getrs<double>(ctx, bb.A64.as<double>(), n, ld, bb.perm.as<int>(), x, 1, n, bb.swork.p,
+                                       fp_getrs_work_bytes(n, 1))))
+                return st;
+            if ((st = bb.mf->apply(x, r))) return st;
+            if ((st = fp_residual_update(ctx, rhs, r, n))) return st;
+        } else {
+            if ((st = fp_getrf<float>(ctx, bb.A32.as<float>(), n, ld, bb.perm.as<int>(), bb.info.as<int>(), bb.fwork.p,
+                                      fp_getrf_work_bytes(n, sizeof(float)))))
+                return st;
+            if ((st = fp_fill(ctx, x, n, 0.0))) return st;
+            FP_HIP(ctx, hipMemcpyAsync(r, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            for (int s = 0; s < sweeps; ++s)
+                if ((st = sweep(bb))) return st;
+        }
+        if ((st = fp_colnorm2(ctx, r, n, n, 1, nb + bodies.size() + k))) return st;
+        FP_HIP(ctx, hipMemcpyAsync(b->d_strength, x, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        return FP_OK;
+    }
+
+    // one refinement sweep: d = A32^-1 r, x += d, r = b - A x (FP64)
+    int sweep(BatchBody &bb)
+    {
+        const int64_t n = bb.n, ld = bb.ld;
+        double *x = bb.x.as<double>(), *r = bb.r.as<double>(), *dd = bb.d.as<double>();
+        int st;
+        FP_HIP(ctx, hipMemcpyAsync(dd, r, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        if ((st = fp_getrs<float>(ctx, bb.A32.as<float>(), n, ld, bb.perm.as<int>(), dd, 1, n, bb.swork.p,
+                                  fp_getrs_work_bytes(n, 1))))
+            return st;
+        if ((st = fp_axpby(ctx, 1.0, dd, 1.0, x, n))) return st;
+        if (stored()) st = fp_gemv<double>(ctx, bb.A64.as<double>(), n, ld, x, r, 1.0, 0.0);
+        else st = bb.mf->apply(x, r);
+        if (st) return st;
+        return fp_residual_update(ctx, bb.rhs.as<double>(), r, n);
+    }
+
+    int capture()
+    {
+        if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
+        if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+        const bool prof = ctx->prof;
+        ctx->prof = false;  // no profiling events inside the graph
+        hipStream_t main = ctx->stream;
+        int st = FP_OK;
+        hipError_t e = hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) { ctx->prof = prof; return fp_hip_fail(ctx, e, "hipStreamBeginCapture"); }
+        st = assemble_queue();
+        if (!st && (e = hipEventRecord(fork, main)) != hipSuccess) st = fp_hip_fail(ctx, e, "hipEventRecord");
+        for (int q = 0; q < BATCH_STREAMS && !st; ++q)
+            if ((e = hipStreamWaitEvent(streams[q], fork, 0)) != hipSuccess) st = fp_hip_fail(ctx, e, "hipStreamWaitEvent");
+        for (auto &bb : bodies) {
+            if (st) break;
+            ctx->stream = streams[bb->stream];
+            st = issue_body(*bb);
+            ctx->stream = main;
+        }
+        for (int q = 0; q < BATCH_STREAMS && !st; ++q) {
+            if ((e = hipEventRecord(join[q], streams[q])) != hipSuccess) st = fp_hip_fail(ctx, e, "hipEventRecord");
+            else if ((e = hipStreamWaitEvent(main, join[q], 0)) != hipSuccess)
+                st = fp_hip_fail(ctx, e, "hipStreamWaitEvent");
+        }
+        ctx->stream = main;
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(main, &g);
+        ctx->prof = prof;
+        if (st) { if (g) (void)hipGraphDestroy(g); return st; }
+        if (e != hipSuccess) return fp_hip_fail(ctx, e, "hipStreamEndCapture");
+        graph = g;
+        size_t nn = 0;
+        if (hipGraphGetNodes(graph, nullptr, &nn) == hipSuccess) graph_nodes = (int)nn;
+        FP_HIP(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        for (auto &bb : bodies) bb->version = bb->b->version;
+        return FP_OK;
+    }
+};
 
 }  // namespace
 
@@ -1451,6 +1694,7 @@ int fp_body_set_nodes(fp_ctx *ctx, fp_body *b, const double *nodes)
     for (int64_t k = 0; k < 3 * nn; ++k)
         if (!std::isfinite(nodes[k])) return fp_fail(ctx, FP_ERR_ARG, "non-finite node coordinate");
     b->nodes.assign(nodes, nodes + 3 * nn);
+    ++b->version;
     const int nvp = b->nvp;
     const int64_t N = b->N;
     std::vector<double> pv((size_t)3 * nvp * N);
@@ -1873,6 +2117,152 @@ int fp_lift_drag_adjoint(fp_ctx *ctx, fp_body *const *bodies, int nb, const doub
 {
     return liftdrag_run(ctx, bodies, nb, Uinf, opts, true, nullptr, nullptr, dCL_dnodes, dCDi_dnodes, dCL_dUinf,
                         dCDi_dUinf, stats);
+}
+
+
+struct fp_batch {
+    fp_ctx *ctx = nullptr;
+    BatchImpl *impl = nullptr;
+};
+
+int fp_batch_create(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_solve_opts *opts, fp_batch **out)
+{
+    int st = check_bodies(ctx, bodies, nb);
+    if (st) return st;
+    if (!out) return fp_fail(ctx, FP_ERR_ARG, "null output");
+    *out = nullptr;
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    std::unique_ptr<BatchImpl> B(new BatchImpl());
+    B->ctx = ctx;
+    if (opts) B->o = *opts;
+    else fp_solve_opts_default(&B->o, FP_SOLVER_LU32_IR);
+    if (B->o.solver != FP_SOLVER_LU && B->o.solver != FP_SOLVER_LU32_IR)
+        return fp_fail(ctx, FP_ERR_UNSUPPORTED, "a body batch runs FP_SOLVER_LU or FP_SOLVER_LU32_IR");
+    B->sweeps = std::max(1, std::min(B->o.max_refine, 3));
+    for (int k = 0; k < nb; ++k) {
+        fp_body *b = bodies[k];
+        if (b->element != FP_ELEM_VORTEXRING) return fp_fail(ctx, FP_ERR_UNSUPPORTED, "a body batch takes vortex-ring bodies");
+        if (!b->has_uinf) return fp_fail(ctx, FP_ERR_SHAPE, "Invalid Uinfs; call fp_body_set_freestream before solve");
+        if (b->nvp != bodies[0]->nvp) return fp_fail(ctx, FP_ERR_UNSUPPORTED, "bodies of a batch share the panel size class");
+        for (int q = 0; q < k; ++q)
+            if (bodies[q] == b) return fp_fail(ctx, FP_ERR_ARG, "a body appears twice in the batch");
+        std::unique_ptr<BatchBody> bb(new BatchBody());
+        bb->b = b;
+        bb->n = b->N;
+        bb->ld = round_ld(b->N);
+        if ((st = gather_targets(ctx, &bodies[k], 1, bb->t))) return st;
+        std::vector<fp_body *> one(1, b);
+        bb->mf.reset(new MatrixFreeOp(ctx, one, bb->t.tv));
+        if ((st = bb->mf->init())) return st;
+        B->bodies.push_back(std::move(bb));
+    }
+    B->nvp = bodies[0]->nvp;
+    if ((st = B->alloc())) return st;
+    if ((st = B->autotune())) return st;
+    if ((st = B->capture())) return st;
+    fp_batch *h = new fp_batch();
+    h->ctx = ctx;
+    h->impl = B.release();
+    *out = h;
+    return FP_OK;
+}
+
+void fp_batch_destroy(fp_batch *h)
+{
+    if (!h) return;
+    if (h->ctx) {
+        (void)hipSetDevice(h->ctx->device);
+        (void)hipStreamSynchronize(h->ctx->stream);
+    }
+    delete h->impl;
+    delete h;
+}
+
+int fp_batch_solve(fp_ctx *ctx, fp_batch *h, fp_batch_stats *stats)
+{
+    if (!ctx || !h || h->ctx != ctx) return FP_ERR_ARG;
+    BatchImpl &B = *h->impl;
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    fp_batch_stats S{};
+    int st;
+    bool stale = false;
+    for (auto &bb : B.bodies) stale = stale || bb->version != bb->b->version || !bb->b->has_uinf;
+    if (stale) {
+        for (auto &bb : B.bodies) {
+            if (!bb->b->has_uinf) return fp_fail(ctx, FP_ERR_SHAPE, "Invalid Uinfs; call fp_body_set_freestream before solve");
+            if ((st = gather_targets(ctx, &bb->b, 1, bb->t))) return st;
+            std::vector<fp_body *> one(1, bb->b);
+            bb->mf.reset(new MatrixFreeOp(ctx, one, bb->t.tv));
+            if ((st = bb->mf->init())) return st;
+        }
+        if ((st = B.build_items())) return st;
+        Timer tc(ctx->stream);
+        if ((st = B.capture())) return st;
+        S.ms_capture = tc.stop();
+        S.recaptured = 1;
+    }
+    Timer t(ctx->stream);
+    FP_HIP(ctx, hipGraphLaunch(B.exec, ctx->stream));
+    S.ms_graph = t.stop();
+    const size_t nb = B.bodies.size();
+    std::vector<double> nr(2 * nb);
+    FP_HIP(ctx, hipMemcpyAsync(nr.data(), B.norms.p, nr.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto &bb : B.bodies) {
+        int info = 0;
+        FP_HIP(ctx, hipMemcpy(&info, bb->info.p, sizeof(int), hipMemcpyDeviceToHost));
+        if (info) {
+            char buf[128];
+            snprintf(buf, sizeof buf, "singular matrix: zero pivot at column %d", info);
+            return fp_fail(ctx, FP_ERR_SINGULAR, buf);
+        }
+    }
+    // bodies that need more refinement continue eagerly on the context stream
+    Timer te(ctx->stream);
+    int extra = 0;
+    double worst = 0;
+    for (size_t k = 0; k < nb; ++k) {
+        BatchBody &bb = *B.bodies[k];
+        double rel = nr[k] > 0 ? std::sqrt(nr[nb + k] / nr[k]) : 0.0;
+        int steps = B.lu32() ? B.sweeps : 0;
+        while (B.lu32() && rel > B.o.tol && steps < B.o.max_refine) {
+            if ((st = B.sweep(bb))) return st;
+            if ((st = fp_colnorm2(ctx, bb.r.as<double>(), bb.n, bb.n, 1, B.norms.as<double>() + nb + k))) return st;
+            double v = 0;
+            FP_HIP(ctx, hipMemcpyAsync(&v, B.norms.as<double>() + nb + k, sizeof(double), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+            FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            rel = nr[k] > 0 ? std::sqrt(v / nr[k]) : 0.0;
+            ++steps;
+            ++extra;
+            FP_HIP(ctx, hipMemcpyAsync(bb.b->d_strength, bb.x.p, bb.n * sizeof(double), hipMemcpyDeviceToDevice,
+                                       ctx->stream));
+        }
+        worst = std::max(worst, rel);
+        S.panels += bb.n;
+    }
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    S.ms_extra = te.stop();
+    S.extra_sweeps = extra;
+    S.sweeps_in_graph = B.lu32() ? B.sweeps : 0;
+    S.residual = worst;
+    S.graph_nodes = B.graph_nodes;
+    S.tile_targets = B.tm;
+    S.queue_workgroups = B.blocks;
+    S.nbodies = (int)nb;
+    for (auto &bb : B.bodies) bb->b->solved = true;
+    if (stats) *stats = S;
+    if (worst > B.o.tol) return fp_fail(ctx, FP_ERR_NOT_CONVERGED, "batch refinement did not reach the tolerance");
+    return FP_OK;
+}
+
+int fp_batch_autotune_get(const fp_batch *h, int k, int *tile_targets, int *workgroups, double *ms)
+{
+    if (!h || k < 0 || k >= h->impl->ntune) return FP_ERR_ARG;
+    if (tile_targets) *tile_targets = h->impl->tune_tm[k];
+    if (workgroups) *workgroups = h->impl->tune_blocks[k];
+    if (ms) *ms = h->impl->tune_ms[k];
+    return FP_OK;
 }
 
 }  // extern "C"

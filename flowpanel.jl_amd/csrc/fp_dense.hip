@@ -831,6 +831,20 @@ template int fp_gemv_multi<double>(fp_ctx *, const double *, int64_t, int64_t, c
 template int fp_gemv_multi<float>(fp_ctx *, const float *, int64_t, int64_t, const double *, int64_t, int64_t,
                                   double *, int64_t);
 
+__global__ void k_fill(double *__restrict__ x, int64_t n, double v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+int fp_fill(fp_ctx *ctx, double *x, int64_t n, double v)
+{
+    if (n <= 0) return FP_OK;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, x, n, v);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
 // column norms^2 of a column-major n x k block: out[c] = sum_i R[c*ld + i]^2
 __global__ __launch_bounds__(256) void k_colnorm2(const double *__restrict__ R, int64_t n, int64_t ld,
                                                   double *__restrict__ out)

@@ -84,12 +84,11 @@ int fp_launch_geometry(fp_ctx *ctx, fp_body *b)
 
 // ----------------------------------------------------------------------------
 // AIC assembly, vortex ring: one lane per panel, TM targets per block in LDS.
+// One tile = 256 panels (block bx) x TM targets (tile by); the caller owns the
+// LDS tile s and the barriers around a tile.
 template <typename T, typename TO, int NVP, int TM>
-__global__ __launch_bounds__(256) void k_assemble_vr(PanelView pan, TargetView tv, TO *__restrict__ G, int64_t ld,
-                                                     int64_t col0)
+__device__ __forceinline__ void vr_load_targets(const TargetView &tv, int64_t i0, T (&s)[6][TM])
 {
-    __shared__ T s[6][TM];
-    const int64_t i0 = (int64_t)blockIdx.y * TM;
     for (int t = threadIdx.x; t < TM; t += blockDim.x) {
         const int64_t i = i0 + t;
         const bool ok = i < tv.M;
@@ -100,8 +99,13 @@ __global__ __launch_bounds__(256) void k_assemble_vr(PanelView pan, TargetView t
         s[4][t] = ok ? T(tv.ny[i]) : T(0);
         s[5][t] = ok ? T(tv.nz[i]) : T(0);
     }
-    __syncthreads();
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+}
+
+template <typename T, typename TO, int NVP, int TM>
+__device__ __forceinline__ void vr_tile(const PanelView &pan, const TargetView &tv, TO *__restrict__ G, int64_t ld,
+                                        int64_t col0, int64_t bx, int64_t i0, const T (&s)[6][TM])
+{
+    const int64_t j = bx * blockDim.x + threadIdx.x;
     if (j >= pan.N) return;
     const int64_t N = pan.N;
     T px[NVP], py[NVP], pz[NVP], ex[NVP], ey[NVP], ez[NVP];
@@ -125,6 +129,46 @@ __global__ __launch_bounds__(256) void k_assemble_vr(PanelView pan, TargetView t
         g[(int64_t)t * ld] = TO(ux * s[3][t] + uy * s[4][t] + uz * s[5][t]);
     }
 }
+
+template <typename T, typename TO, int NVP, int TM>
+__global__ __launch_bounds__(256) void k_assemble_vr(PanelView pan, TargetView tv, TO *__restrict__ G, int64_t ld,
+                                                     int64_t col0)
+{
+    __shared__ T s[6][TM];
+    const int64_t i0 = (int64_t)blockIdx.y * TM;
+    vr_load_targets<T, TO, NVP, TM>(tv, i0, s);
+    __syncthreads();
+    vr_tile<T, TO, NVP, TM>(pan, tv, G, ld, col0, blockIdx.x, i0, s);
+}
+
+// Batched assembly of independent bodies (each its own matrix): persistent
+// workgroups pull (body, panel block, target tile) items from a device queue
+// (one atomic counter) until the queue is empty.  Every body's items are the
+// same size, so uneven bodies share the chip without per-body launches.
+template <typename T, typename TO, int NVP, int TM>
+__global__ __launch_bounds__(256) void k_assemble_vr_queue(const AssemblyItem *__restrict__ items, int nitems,
+                                                           int64_t total, int *__restrict__ counter)
+{
+    __shared__ T s[6][TM];
+    __shared__ int64_t cur;
+    for (;;) {
+        if (threadIdx.x == 0) cur = atomicAdd((unsigned long long *)counter, 1ull);
+        __syncthreads();
+        const int64_t it = cur;
+        if (it >= total) break;
+        int b = 0;
+        while (b + 1 < nitems && items[b + 1].tile0 <= it) ++b;
+        const AssemblyItem &a = items[b];
+        const int64_t local = it - a.tile0;
+        const int64_t by = local / a.nbx, bx = local % a.nbx;
+        vr_load_targets<T, TO, NVP, TM>(a.tv, by * TM, s);
+        __syncthreads();
+        vr_tile<T, TO, NVP, TM>(a.pan, a.tv, (TO *)a.G, a.ld, 0, bx, by * TM, s);
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_counter(int *counter) { *(unsigned long long *)counter = 0ull; }
 
 // AIC assembly, constant source (Hess-Smith), one lane per panel.
 template <typename T, typename TO, int NVP, int TM>
@@ -242,6 +286,52 @@ int fp_launch_assemble(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G
     return FP_OK;
 }
 template int fp_launch_assemble<double>(fp_ctx *, const fp_body *, const TargetView &, double *, int64_t, int64_t);
+
+template <typename T>
+int fp_launch_assemble_wake(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G, int64_t ld, int64_t col0)
+{
+    if (b->element != FP_ELEM_VORTEXRING || b->nh == 0 || tv.M == 0) return FP_OK;
+    ProfScope pw(ctx, "assemble_wake");
+    dim3 g2((unsigned)((tv.M + 255) / 256));
+    hipLaunchKernelGGL((k_assemble_wake<double, T>), g2, dim3(256), 0, ctx->stream, horse_view(b), b->kernel_offset,
+                       b->kernel_cutoff, tv, G, ld, col0);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+template int fp_launch_assemble_wake<double>(fp_ctx *, const fp_body *, const TargetView &, double *, int64_t, int64_t);
+template int fp_launch_assemble_wake<float>(fp_ctx *, const fp_body *, const TargetView &, float *, int64_t, int64_t);
+
+int fp_assembly_queue_tiles(int64_t N, int64_t M, int tm, int *nbx, int64_t *ntiles)
+{
+    *nbx = (int)((N + 255) / 256);
+    *ntiles = (int64_t)(*nbx) * ((M + tm - 1) / tm);
+    return FP_OK;
+}
+
+template <int NVP, int TM, typename TO>
+static void launch_queue(fp_ctx *ctx, const AssemblyItem *items, int nitems, int64_t total, int *counter, int blocks)
+{
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, ctx->stream, counter);
+    hipLaunchKernelGGL((k_assemble_vr_queue<double, TO, NVP, TM>), dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                       items, nitems, total, counter);
+}
+
+int fp_launch_assemble_queue(fp_ctx *ctx, const AssemblyItem *d_items, int nitems, int64_t total, int nvp, int tm,
+                             int prec, int *counter, int blocks)
+{
+    if (total <= 0) return FP_OK;
+    ProfScope ps(ctx, prec == FP_F64 ? "assemble_vr_queue_f64" : "assemble_vr_queue_f32");
+#define FP_Q(NV, TMV)                                                                             \
+    if (nvp == NV && tm == TMV) {                                                                 \
+        if (prec == FP_F64) launch_queue<NV, TMV, double>(ctx, d_items, nitems, total, counter, blocks); \
+        else launch_queue<NV, TMV, float>(ctx, d_items, nitems, total, counter, blocks);          \
+        FP_HIP(ctx, hipGetLastError());                                                           \
+        return FP_OK;                                                                             \
+    }
+    FP_Q(4, 32) FP_Q(4, 64) FP_Q(4, 128) FP_Q(8, 32) FP_Q(8, 64) FP_Q(8, 128)
+#undef FP_Q
+    return fp_fail(ctx, FP_ERR_ARG, "internal: unsupported assembly tile");
+}
 template int fp_launch_assemble<float>(fp_ctx *, const fp_body *, const TargetView &, float *, int64_t, int64_t);
 
 // ----------------------------------------------------------------------------

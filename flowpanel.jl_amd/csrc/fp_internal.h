@@ -99,6 +99,7 @@ struct fp_body {
     int nh = 0;
     double *d_strength = nullptr;
     bool has_uinf = false, has_wake = false, solved = false;
+    uint64_t version = 0;  // bumped whenever device geometry or wake buffers change
 };
 
 struct fp_mat {
@@ -160,12 +161,32 @@ static inline HorseView horse_view(const fp_body *b)
     return h;
 }
 
+// one body of a batched assembly: its panels, its control points and its own
+// matrix (row-major, ld); items [tile0, tile0 + nbx * nby) of the work queue
+struct AssemblyItem {
+    PanelView pan;
+    TargetView tv;
+    void *G;
+    int64_t ld;
+    int64_t tile0;
+    int64_t nbx;
+};
+
 // ---- launchers (fp_influence.hip) ---------------------------------------
 // geometry of one body (normals, CPs, areas, Hess-Smith frames)
 int fp_launch_geometry(fp_ctx *ctx, fp_body *b);
 // AIC column block of one body into a row-major matrix (T = float|double)
 template <typename T>
 int fp_launch_assemble(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G, int64_t ld, int64_t col0);
+// wake horseshoes of one body only (added into its columns)
+template <typename T>
+int fp_launch_assemble_wake(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G, int64_t ld, int64_t col0);
+// tiles of one body in the assembly queue (256 panels x tm targets)
+int fp_assembly_queue_tiles(int64_t N, int64_t M, int tm, int *nbx, int64_t *ntiles);
+// persistent queue assembly of the vortex rings of several bodies (no wake);
+// `counter` is 8 bytes of device memory, reset by the launch itself
+int fp_launch_assemble_queue(fp_ctx *ctx, const AssemblyItem *d_items, int nitems, int64_t total, int nvp, int tm,
+                             int prec, int *counter, int blocks);
 // Field sums of one body over targets.  mode 0: out3 (3 x M SoA) += U;
 // mode 1: out1[i] += n_i . U (strengths `x`);  mode 2: out1 += phi.
 int fp_launch_field(fp_ctx *ctx, const fp_body *b, const double *strength, const TargetView &tv, int mode,
@@ -215,6 +236,7 @@ int fp_rhs_from(fp_ctx *ctx, const fp_body *b, const double *uinf_soa, double *r
 int fp_residual_update(fp_ctx *ctx, const double *b, double *r, int64_t n);   // r = b - r
 int fp_dot(fp_ctx *ctx, const double *x, const double *y, int64_t n, double *d_out);
 int fp_axpy(fp_ctx *ctx, double a, const double *x, double *y, int64_t n);
+int fp_fill(fp_ctx *ctx, double *x, int64_t n, double v);   // x[:] = v (a kernel: safe inside graphs)
 
 // multi-dot out[c] = sum_i V[c*ldv + i] w[i] (part: 256*nv doubles)
 int fp_mdot(fp_ctx *ctx, const double *V, int64_t ldv, int nv, const double *w, int64_t n, double *part,

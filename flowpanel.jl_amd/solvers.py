@@ -181,6 +181,58 @@ def solve_freestreams(body, Uinfs, Das=None, Dbs=None, solver=solve_ludiv, solve
     return G, stats
 
 
+class BodyBatch:
+    """Independent bodies (each its own system) solved together.
+
+    Construction sizes the device buffers, autotunes the batched assembly and
+    captures one hipGraph (persistent work-queue assembly of every body, then
+    per body its wake columns, LU and refinement sweeps on parallel streams).
+    ``solve()`` replays it; set each body's freestream first (``set_freestream``
+    or ``solve``).  Solvers: ``solve_lu32_ir`` (FP32 AIC + LU, FP64 residuals;
+    ``residual_mode=0`` recomputes them matrix-free) or ``solve_ludiv``.
+    """
+
+    def __init__(self, bodies, solver=solve_lu32_ir, solver_optargs=None):
+        if solver not in (solve_lu32_ir, solve_ludiv, solve_backslash):
+            raise ValueError("a body batch runs solve_lu32_ir or solve_ludiv")
+        self.bodies = list(bodies)
+        self.ctx = self.bodies[0].ctx
+        o = nat.opts(_CODES[solver], **(solver_optargs or {}))
+        h = ctypes.c_void_p()
+        self.ctx.check(nat.lib().fp_batch_create(self.ctx.handle, nat.handles(self.bodies), len(self.bodies),
+                                                 ctypes.byref(o), ctypes.byref(h)))
+        self.handle = h
+
+    def solve(self):
+        s = nat.BatchStats()
+        self.ctx.check(nat.lib().fp_batch_solve(self.ctx.handle, self.handle, ctypes.byref(s)))
+        for b in self.bodies:
+            g = np.zeros(b.ncells)
+            self.ctx.check(nat.lib().fp_body_get_strength(self.ctx.handle, b.handle, nat.dptr(g)))
+            b.strength = g
+            b._solved = True
+            b.add_field(b.element.field, g)
+        return s.as_dict()
+
+    def autotune_table(self):
+        """[(targets per tile, workgroups, ms)] of the assembly candidates timed at construction."""
+        out = []
+        for k in range(9):
+            tm, wg, ms = ctypes.c_int(), ctypes.c_int(), np.zeros(1)
+            if nat.lib().fp_batch_autotune_get(self.handle, k, ctypes.byref(tm), ctypes.byref(wg), nat.dptr(ms)):
+                break
+            out.append((tm.value, wg.value, float(ms[0])))
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None) and nat._LIB is not None:
+                nat._LIB.fp_batch_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
 def assemble(body, precision="f64"):
     """Device AIC of the body (column-major numpy copy), _G_U! of the reference."""
     ctx = body.ctx

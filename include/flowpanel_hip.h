@@ -156,6 +156,35 @@ int fp_solve_bodies(fp_ctx *ctx, fp_body *const *bodies, int nbodies, const fp_s
 int fp_solve_bodies_multi(fp_ctx *ctx, fp_body *const *bodies, int nbodies, const double *Uinfs, int64_t k,
                           const fp_solve_opts *opts, double *strengths, fp_solve_stats *stats);
 
+/* A batch of independent bodies (each its own linear system, no mutual
+ * influence), e.g. many wings of uneven panel counts.  fp_batch_create sizes
+ * the device buffers, times the batched assembly for each tile shape (targets
+ * per LDS tile 32/64/128 x persistent workgroups 2/4/8 per CU) and keeps the
+ * fastest, then captures one hipGraph of: a persistent work-queue launch that
+ * assembles every body's AIC, and per body (on parallel streams) its wake
+ * columns, LU factorisation and refinement sweeps.  fp_batch_solve replays the
+ * graph (re-capturing first if a body's geometry or wake changed), finishes any
+ * body that still needs refinement, and leaves the strengths in the bodies.
+ * opts->solver: FP_SOLVER_LU32_IR (FP32 AIC and LU; FP64 residuals, matrix-free
+ * with FP_RESID_MATRIX_FREE or from a stored FP64 AIC) or FP_SOLVER_LU.
+ * Each body is solve(body, Uinfs, Das, Dbs) (src/FLOWPanel_liftingbody.jl:172-216);
+ * the reference has no batch of bodies. */
+typedef struct fp_batch fp_batch;
+typedef struct fp_batch_stats {
+    double ms_graph;       /* device time of one graph replay (assembly + factorisations + sweeps) */
+    double ms_capture;     /* re-capture and instantiation before this replay (0 if none)        */
+    double ms_extra;       /* eager refinement after the replay                                  */
+    double residual;       /* worst FP64 relative residual over the bodies                       */
+    int64_t panels;        /* total panels                                                       */
+    int nbodies, sweeps_in_graph, extra_sweeps, graph_nodes, recaptured;
+    int tile_targets, queue_workgroups;  /* the autotuned assembly shape                          */
+} fp_batch_stats;
+int fp_batch_create(fp_ctx *ctx, fp_body *const *bodies, int nbodies, const fp_solve_opts *opts, fp_batch **out);
+int fp_batch_solve(fp_ctx *ctx, fp_batch *batch, fp_batch_stats *stats);
+void fp_batch_destroy(fp_batch *batch);
+/* The k-th autotuning candidate (k = 0..8): tile shape and its measured milliseconds. */
+int fp_batch_autotune_get(const fp_batch *batch, int k, int *tile_targets, int *workgroups, double *ms);
+
 /* ---- dense AIC and the solver protocol ---------------------------------- */
 /* _G_U!/_G_Uvortexring! (src/FLOWPanel_liftingbody.jl:455-572,
  * src/FLOWPanel_nonliftingbody.jl:141-186, src/FLOWPanel_multibody.jl:502-555):

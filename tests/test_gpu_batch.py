@@ -62,3 +62,48 @@ def test_solver_protocol_multi_rhs_factor_once(fp):
         assert rel(X, np.linalg.solve(A, B)) < 1e-11, solver
         if solver == "solve_lu32_ir":
             assert fp.last_stats["refinements"] >= 2   # FP32 factors: refinement did the work
+
+
+def _wings(fp, geo, sizes, seed=3):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for nc, ns in sizes:
+        ar = rng.uniform(6, 12)
+        g, sh = geo.loft_wing(airfoil="0012", n_chord=nc, n_span=ns, span=ar * 0.75, root_chord=1.0, taper=0.75,
+                              sweep_deg=rng.uniform(0, 35))
+        out.append(fp.RigidWakeBody(g, sh))
+    return out
+
+
+@pytest.mark.parametrize("solver,resid", [("solve_lu32_ir", 0), ("solve_lu32_ir", 1), ("solve_ludiv", 1)])
+def test_body_batch_matches_single_solves(fp, geo, solver, resid):
+    bodies = _wings(fp, geo, [(20, 10), (34, 18), (12, 6), (26, 30), (40, 8)])
+    refs = []
+    for k, b in enumerate(bodies):
+        V = fp.direction(2.0 + k, 0.5 * k)
+        U = np.tile(V[:, None], (1, b.ncells))
+        D = np.tile(V[:, None], (1, b.nsheddings))
+        fp.solve(b, U, D, D)                 # FP64 LU reference (also stores the freestream)
+        refs.append(b.strength.copy())
+    batch = fp.BodyBatch(bodies, solver=getattr(fp, solver), solver_optargs={"residual_mode": resid, "tol": 1e-12})
+    assert len(batch.autotune_table()) == 9
+    from flowpanel_amd import _native as nat
+    for rep in range(3):
+        for b in bodies:                     # eager work between replays: clear the device strengths
+            z = np.zeros(b.ncells)
+            b.ctx.check(nat.lib().fp_body_set_strength(b.ctx.handle, b.handle, nat.dptr(z)))
+        st = batch.solve()
+        assert st["nbodies"] == 5 and st["residual"] <= 1e-12
+        for b, ref in zip(bodies, refs):
+            assert rel(b.strength, ref) < 1e-10, (rep, rel(b.strength, ref))
+        fp.calcfield_U(bodies[0], bodies[0])  # an unrelated eager kernel between replays
+    # a new freestream on one body re-captures the graph and is honoured
+    b = bodies[2]
+    V = fp.direction(7.0)
+    U = np.tile(V[:, None], (1, b.ncells))
+    D = np.tile(V[:, None], (1, b.nsheddings))
+    fp.solve(b, U, D, D)
+    ref = b.strength.copy()
+    st = batch.solve()
+    assert st["recaptured"] == 1
+    assert rel(b.strength, ref) < 1e-10
