@@ -233,6 +233,21 @@ def adjoint_leg(fp, ctx, fd_coords=64, reps=3):
     r64 = fp.lift_drag_gradient(body, U, **kw)
     g_m = np.concatenate([r["dCL_dnodes"][0].ravel(), r["dCDi_dnodes"][0].ravel()])
     g_64 = np.concatenate([r64["dCL_dnodes"][0].ravel(), r64["dCDi_dnodes"][0].ravel()])
+    kernels = {k: {"launches": v[0], "total_ms": round(v[1] / reps, 3)} for k, v in prof.items()}
+    rec = {
+        "workload": "wing NACA 2412, AR 8, twist 0 -> -3 deg, 100 x 100 vortex-ring quads + rigid wake, AoA 4.0",
+        "panels": body.ncells, "nodes": body.grid.nnodes,
+        "adjoint_total_ms": round(walls[best], 3),
+        "forward_ms": round(r["stats"]["ms_forward"], 3), "reverse_ms": round(r["stats"]["ms_adjoint"], 3),
+        "transpose_solve_ms": round(r["stats"]["ms_transpose_solve"], 3),
+        "forward_solve": r["stats"]["forward"], "adjoint_iterations": r["stats"]["iterations_adjoint"],
+        "CL": r["CL"], "CDi": r["CDi"],
+        "mixed_vs_fp64_grad_rel": float(np.linalg.norm(g_m - g_64) / np.linalg.norm(g_64)),
+        "fp64_lu_adjoint_total_ms": round(r64["stats"]["ms_forward"] + r64["stats"]["ms_adjoint"], 3),
+        "kernels_per_run": kernels,
+    }
+    if fd_coords <= 0:
+        return rec
     # central finite differences of the FP64-LU forward
     log("bench: adjoint FD check (%d node coordinates + Uinf)" % fd_coords)
     rng = np.random.Generator(np.random.PCG64(4))
@@ -266,21 +281,10 @@ def adjoint_leg(fp, ctx, fd_coords=64, reps=3):
     def nrel(a, b):
         return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
-    kernels = {k: {"launches": v[0], "total_ms": round(v[1] / reps, 3)} for k, v in prof.items()}
-    return {
-        "workload": "wing NACA 2412, AR 8, twist 0 -> -3 deg, 100 x 100 vortex-ring quads + rigid wake, AoA 4.0",
-        "panels": body.ncells, "nodes": body.grid.nnodes,
-        "adjoint_total_ms": round(walls[best], 3),
-        "forward_ms": round(r["stats"]["ms_forward"], 3), "reverse_ms": round(r["stats"]["ms_adjoint"], 3),
-        "transpose_solve_ms": round(r["stats"]["ms_transpose_solve"], 3),
-        "forward_solve": r["stats"]["forward"], "adjoint_iterations": r["stats"]["iterations_adjoint"],
-        "CL": r["CL"], "CDi": r["CDi"],
-        "fd_rel_err_nodes": {"CL": nrel(adj[0], fd[0]), "CDi": nrel(adj[1], fd[1])},
-        "fd_rel_err_uinf": {"CL": nrel(adju[0], fdu[0]), "CDi": nrel(adju[1], fdu[1])},
-        "mixed_vs_fp64_grad_rel": nrel(g_m, g_64),
-        "fp64_lu_adjoint_total_ms": round(r64["stats"]["ms_forward"] + r64["stats"]["ms_adjoint"], 3),
-        "kernels_per_run": kernels,
-    }
+    rec["fd_coords"] = int(fd_coords)
+    rec["fd_rel_err_nodes"] = {"CL": nrel(adj[0], fd[0]), "CDi": nrel(adj[1], fd[1])}
+    rec["fd_rel_err_uinf"] = {"CL": nrel(adju[0], fdu[0]), "CDi": nrel(adju[1], fdu[1])}
+    return rec
 
 
 def main():
@@ -303,6 +307,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 10,000-panel adjoint leg")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg (prints its record)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched legs (64 freestreams, 16 wings)")
+    ap.add_argument("--adjoint-fd", type=int, default=64, help="node coordinates checked by finite differences")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched legs (prints their record)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -313,7 +318,7 @@ def main():
     fp = __graft_entry__.load_package()
     ctx = fp.context()
     if args.adjoint_only:
-        print(json.dumps({"adjoint": adjoint_leg(fp, ctx)}), flush=True)
+        print(json.dumps({"adjoint": adjoint_leg(fp, ctx, fd_coords=args.adjoint_fd)}), flush=True)
         return
     if args.batch_only:
         print(json.dumps({"batch": batch_legs(fp, ctx)}), flush=True)
@@ -395,7 +400,7 @@ def main():
         vendor = vendor_lu(fp, body, N)
     adj = None
     if not args.no_adjoint:
-        adj = adjoint_leg(fp, ctx)
+        adj = adjoint_leg(fp, ctx, fd_coords=args.adjoint_fd)
     bat = None
     if not args.no_batch:
         bat = batch_legs(fp, ctx)

@@ -14,7 +14,8 @@ from .geometry import Grid, direction
 from .bodies import (ConstantDoublet, ConstantSource, MultiBody, NonLiftingBody, RigidWakeBody, VortexRing,
                      calc_areas, calc_controlpoints, calc_normals)
 from .solvers import (assemble, calc_bc_noflowthrough, getrf, solve, solve_backslash, solve_gmres,
-                      solve_gmres_ir, solve_lu32_ir, solve_ludiv, solve_freestreams, BodyBatch, last_stats)
+                      solve_gmres_ir, solve_lu32_ir, solve_ludiv, solve_freestreams, BodyBatch, calc_elprescribe,
+                      last_stats)
 from .postprocess import Uind, phi, calcfield_U, calcfield_Cp, calcfield_F, calcfield_LDS
 from .adjoint import lift_drag, lift_drag_gradient, set_nodes
 from ._native import FPError, context
@@ -24,5 +25,5 @@ __all__ = [
     "RigidWakeBody", "MultiBody", "calc_normals", "calc_controlpoints", "calc_areas", "assemble",
     "calc_bc_noflowthrough", "getrf", "solve", "solve_ludiv", "solve_backslash", "solve_gmres", "solve_lu32_ir",
     "solve_gmres_ir", "last_stats", "Uind", "phi", "calcfield_U", "calcfield_Cp", "calcfield_F", "calcfield_LDS",
-    "FPError", "context", "solve_freestreams", "BodyBatch", "lift_drag", "lift_drag_gradient", "set_nodes",
+    "FPError", "context", "solve_freestreams", "BodyBatch", "calc_elprescribe", "lift_drag", "lift_drag_gradient", "set_nodes",
 ]

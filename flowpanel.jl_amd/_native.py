@@ -46,7 +46,7 @@ EXPORTS = [
     "fp_mat_getrf", "fp_uind", "fp_phi", "fp_calcfield_U", "fp_calcfield_cp", "fp_ctx_profile",
     "fp_ctx_profile_get", "fp_ctx_copy_calibrate", "fp_body_nnodes", "fp_body_set_nodes",
     "fp_liftdrag_opts_default", "fp_lift_drag", "fp_lift_drag_adjoint", "fp_solve_bodies_multi",
-    "fp_batch_create", "fp_batch_solve", "fp_batch_destroy", "fp_batch_autotune_get",
+    "fp_batch_create", "fp_batch_solve", "fp_batch_destroy", "fp_batch_autotune_get", "fp_solve_bodies_lsq",
 ]
 
 
@@ -161,6 +161,8 @@ def lib():
     L.fp_batch_destroy.argtypes = [_vp]
     L.fp_batch_destroy.restype = None
     L.fp_batch_autotune_get.argtypes = [_vp, cint, ctypes.POINTER(cint), ctypes.POINTER(cint), _d]
+    L.fp_solve_bodies_lsq.argtypes = [_vp, pp, cint, _i64, _d, i64, cint, ctypes.POINTER(SolveOpts),
+                                      ctypes.POINTER(SolveStats)]
     L.fp_body_nnodes.argtypes = [_vp]
     L.fp_body_nnodes.restype = i64
     L.fp_body_set_nodes.argtypes = [_vp, _vp, _d]

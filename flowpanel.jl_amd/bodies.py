@@ -111,11 +111,20 @@ class NonLiftingBody(AbstractBody):
 
 
 class RigidWakeBody(AbstractBody):
-    """RigidWakeBody{VortexRing}(grid, shedding)."""
+    """RigidWakeBody{VortexRing, N}(grid, shedding).
 
-    def __init__(self, grid, shedding=None, element=VortexRing, **kw):
+    ``formulation=1`` is the square no-flow-through system; ``formulation=2`` the
+    watertight least-squares one (``RigidWakeBody{VortexRing, 2}``,
+    src/FLOWPanel_liftingbody.jl:236-316): ``solve`` then prescribes the strengths
+    ``elprescribe`` (default ``[(0, 0.0)]``, 0-based) and solves the normal equations.
+    """
+
+    def __init__(self, grid, shedding=None, element=VortexRing, formulation=1, **kw):
         if element is not VortexRing:
             raise NotImplementedError("RigidWakeBody supports VortexRing elements")
+        if formulation not in (1, 2):
+            raise ValueError("formulation must be 1 or 2")
+        self.formulation = formulation
         super().__init__(grid, element, shedding, **kw)
 
 

@@ -1337,6 +1337,104 @@ int fp_solve_bodies(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_solve_
     return FP_OK;
 }
 
+// solve(RigidWakeBody{VortexRing, 2}, Uinfs, Das, Dbs; elprescribe): the
+// watertight least-squares formulation (src/FLOWPanel_liftingbody.jl:236-286,
+// 357-447).  The prescribed strengths move to the right-hand side, and the
+// normal equations G_red^T G_red x = G_red^T b are formed on MFMA as the full
+// G^T G (one FP64 GEMM of the transposed AIC with the AIC) with the prescribed
+// rows and columns replaced by the identity, which gives the same x and the
+// prescribed values in place (set_solution, :290-316).
+int fp_solve_bodies_lsq(fp_ctx *ctx, fp_body *const *bodies, int nb, const int64_t *pres_idx, const double *pres_val,
+                        int64_t npres, int index_base, const fp_solve_opts *opts, fp_solve_stats *stats)
+{
+    fp_solve_opts o;
+    int st = check_solve_inputs(ctx, bodies, nb, opts, o);
+    if (st) return st;
+    int64_t n = 0;
+    for (int b = 0; b < nb; ++b) {
+        if (bodies[b]->element != FP_ELEM_VORTEXRING)
+            return fp_fail(ctx, FP_ERR_UNSUPPORTED, "the least-squares formulation takes vortex-ring bodies");
+        n += bodies[b]->N;
+    }
+    if (npres < 0 || (npres > 0 && (!pres_idx || !pres_val))) return fp_fail(ctx, FP_ERR_ARG, "null elprescribe");
+    if (npres >= n) return fp_fail(ctx, FP_ERR_ARG, "every strength is prescribed");
+    std::vector<int64_t> pidx(npres);
+    for (int64_t q = 0; q < npres; ++q) {
+        pidx[q] = pres_idx[q] - index_base;
+        if (pidx[q] < 0 || pidx[q] >= n) return fp_fail(ctx, FP_ERR_ARG, "prescribed element index out of range");
+        for (int64_t r = 0; r < q; ++r)
+            if (pidx[r] == pidx[q]) return fp_fail(ctx, FP_ERR_ARG, "element prescribed twice");
+    }
+    fp_solve_stats s{};
+    Timer tm(ctx->stream);
+    Targets t;
+    if ((st = gather_targets(ctx, bodies, nb, t))) return st;
+    const int64_t ld = round_ld(n);
+    DBuf G, GT, C, rhs, bls, x, dp, dv;
+    if ((st = dalloc(ctx, G, (size_t)n * ld * sizeof(double))) || (st = dalloc(ctx, GT, (size_t)n * ld * sizeof(double))) ||
+        (st = dalloc(ctx, C, (size_t)n * ld * sizeof(double))) || (st = dalloc(ctx, rhs, n * sizeof(double))) ||
+        (st = dalloc(ctx, bls, n * sizeof(double))) || (st = dalloc(ctx, x, n * sizeof(double))) ||
+        (st = dalloc(ctx, dp, std::max<int64_t>(1, npres) * sizeof(int64_t))) ||
+        (st = dalloc(ctx, dv, std::max<int64_t>(1, npres) * sizeof(double))))
+        return st;
+    if (npres) {
+        FP_HIP(ctx, hipMemcpyAsync(dp.p, pidx.data(), npres * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+        FP_HIP(ctx, hipMemcpyAsync(dv.p, pres_val, npres * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    }
+    int64_t col = 0;
+    for (int b = 0; b < nb; ++b) {
+        if ((st = fp_rhs_kernel(ctx, bodies[b], rhs.as<double>() + col))) return st;
+        col += bodies[b]->N;
+    }
+    if ((st = assemble_into(ctx, bodies, nb, t.tv, FP_F64, G.p, ld))) return st;
+    if ((st = fp_lsq_rhs(ctx, G.as<double>(), n, ld, dp.as<int64_t>(), dv.as<double>(), (int)npres, rhs.as<double>())))
+        return st;
+    if ((st = fp_mat_transpose(ctx, G.as<double>(), n, ld, GT.p, FP_F64))) return st;
+    if ((st = fp_fill(ctx, C.as<double>(), n * ld, 0.0))) return st;
+    if ((st = fp_gemm_sub<double>(ctx, n, n, n, GT.as<double>(), ld, G.as<double>(), ld, C.as<double>(), ld))) return st;
+    if ((st = fp_gemv<double>(ctx, GT.as<double>(), n, ld, rhs.as<double>(), bls.as<double>(), 1.0, 0.0))) return st;
+    if ((st = fp_lsq_fix(ctx, C.as<double>(), n, ld, dp.as<int64_t>(), (int)npres, bls.as<double>(), dv.as<double>())))
+        return st;
+    s.ms_assemble = tm.stop();
+    G.reset();
+    GT.reset();
+    // normal equations with the chosen solver (FP64 copy kept for the residual)
+    SolveArgs a;
+    a.solver = o.solver;
+    a.n = n;
+    a.ld = ld;
+    DBuf W, W32;
+    std::unique_ptr<DenseOp<double>> op;
+    op.reset(new DenseOp<double>(ctx, C.as<double>(), n, ld));
+    a.resid = op.get();
+    a.R64 = C.as<double>();
+    if (o.solver == FP_SOLVER_LU) {
+        if ((st = dalloc(ctx, W, (size_t)n * ld * sizeof(double)))) return st;
+        FP_HIP(ctx, hipMemcpyAsync(W.p, C.p, (size_t)n * ld * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        a.A64 = W.as<double>();
+    } else if (o.solver == FP_SOLVER_GMRES) {
+        a.A64 = C.as<double>();
+    } else {
+        if ((st = dalloc(ctx, W32, (size_t)n * ld * sizeof(float)))) return st;
+        if ((st = fp_mat_narrow(ctx, C.as<double>(), W32.as<float>(), n, ld))) return st;
+        a.A32 = W32.as<float>();
+    }
+    Solver solver(ctx, a, o);
+    st = solver.setup(s);
+    if (!st) st = solver.solve(bls.as<double>(), x.as<double>(), s);
+    if (stats) *stats = s;
+    if (st) return st;
+    col = 0;
+    for (int b = 0; b < nb; ++b) {
+        FP_HIP(ctx, hipMemcpyAsync(bodies[b]->d_strength, x.as<double>() + col, bodies[b]->N * sizeof(double),
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+        bodies[b]->solved = true;
+        col += bodies[b]->N;
+    }
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FP_OK;
+}
+
 int fp_solve_bodies_multi(fp_ctx *ctx, fp_body *const *bodies, int nb, const double *Uinfs, int64_t k,
                           const fp_solve_opts *opts, double *strengths, fp_solve_stats *stats)
 {

@@ -66,14 +66,15 @@ __global__ __launch_bounds__(256) void k_adj_geometry(const double *__restrict__
     area[i] = a;
 }
 
-// V += Uinf; Cp = 1 - |V|^2/|Uinf|^2 (src/FLOWPanel_postprocess.jl:1059-1074)
+// V += Uinf; Cp = 1 - |V|^2/|Uinf|^2 (src/FLOWPanel_postprocess.jl:1059-1074).
+// Velocities are point-major (V[3i + d]), as the field kernel writes them.
 __global__ void k_adj_cp(double *__restrict__ V, int64_t M, double ux, double uy, double uz, double uref,
                          double *__restrict__ Cp)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
-    const double vx = V[i] + ux, vy = V[M + i] + uy, vz = V[2 * M + i] + uz;
-    V[i] = vx; V[M + i] = vy; V[2 * M + i] = vz;
+    const double vx = V[3 * i] + ux, vy = V[3 * i + 1] + uy, vz = V[3 * i + 2] + uz;
+    V[3 * i] = vx; V[3 * i + 1] = vy; V[3 * i + 2] = vz;
     const double u = sqrt(vx * vx + vy * vy + vz * vz) / uref;
     Cp[i] = 1 - u * u;
 }
@@ -166,7 +167,7 @@ __global__ void k_adj_vbar(const double *__restrict__ Cpb, const double *__restr
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
-    const double vx = V[i], vy = V[M + i], vz = V[2 * M + i];
+    const double vx = V[3 * i], vy = V[3 * i + 1], vz = V[3 * i + 2];
     const double v2 = vx * vx + vy * vy + vz * vz;
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
@@ -351,6 +352,7 @@ __global__ void k_adj_wake_g(const double *__restrict__ hp, const int *__restric
 
 // control-point weights W2 = -lambda n, normal adjoints of the no-flow-through
 // rows (b_i = -Uinf.n_i and A_ij = n_i.u_j(C_i)):  nb += -lambda (Uinf + Uind(C))
+// (Uc point-major, Uc[3i + d])
 __global__ void k_adj_cpw(const double *__restrict__ lam, const double *__restrict__ nrm,
                           const double *__restrict__ Uc, int64_t M, double ux, double uy, double uz,
                           double *__restrict__ W2, double *__restrict__ nb)
@@ -364,7 +366,7 @@ __global__ void k_adj_cpw(const double *__restrict__ lam, const double *__restri
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             W2[(3 * m + d) * M + i] = -l * nrm[d * M + i];
-            nb[(3 * m + d) * M + i] -= l * (U[d] + Uc[d * M + i]);
+            nb[(3 * m + d) * M + i] -= l * (U[d] + Uc[3 * i + d]);
         }
     }
 }

@@ -166,6 +166,70 @@ static int gemm_sub(fp_ctx *ctx, int64_t m, int64_t n, int64_t k, const T *A, in
     return FP_OK;
 }
 
+template <typename T>
+int fp_gemm_sub(fp_ctx *ctx, int64_t m, int64_t n, int64_t k, const T *A, int64_t lda, const T *B, int64_t ldb, T *C,
+                int64_t ldc)
+{
+    return gemm_sub<T>(ctx, m, n, k, A, lda, B, ldb, C, ldc);
+}
+template int fp_gemm_sub<double>(fp_ctx *, int64_t, int64_t, int64_t, const double *, int64_t, const double *, int64_t,
+                                 double *, int64_t);
+template int fp_gemm_sub<float>(fp_ctx *, int64_t, int64_t, int64_t, const float *, int64_t, const float *, int64_t,
+                                float *, int64_t);
+
+// Least-squares system of the watertight formulation from C = -(G^T G) (as left
+// by fp_gemm_sub on a zero C): C := G^T G, then every prescribed strength p gets
+// the row and column of the identity and the right-hand side entry val_p, which
+// leaves the other unknowns' normal equations (G_red^T G_red) untouched.
+__global__ void k_lsq_fix(double *__restrict__ C, int64_t n, int64_t ld, const int64_t *__restrict__ pidx, int np,
+                          double *__restrict__ b, const double *__restrict__ pval)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = blockIdx.y;
+    if (j >= n) return;
+    double v = -C[i * ld + j];
+    bool hit = false;
+    for (int q = 0; q < np; ++q) {
+        if (pidx[q] == i || pidx[q] == j) hit = true;
+    }
+    if (hit) v = (i == j) ? 1.0 : 0.0;
+    C[i * ld + j] = v;
+    if (j == 0)
+        for (int q = 0; q < np; ++q)
+            if (pidx[q] == i) b[i] = pval[q];
+}
+
+int fp_lsq_fix(fp_ctx *ctx, double *C, int64_t n, int64_t ld, const int64_t *pidx, int np, double *b,
+               const double *pval)
+{
+    if (n <= 0) return FP_OK;
+    hipLaunchKernelGGL(k_lsq_fix, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, ctx->stream, C, n, ld,
+                       pidx, np, b, pval);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+// rhs[i] -= sum_q val_q G[i, p_q]: the prescribed strengths moved to the right-hand side
+__global__ void k_lsq_rhs(const double *__restrict__ G, int64_t n, int64_t ld, const int64_t *__restrict__ pidx,
+                          const double *__restrict__ pval, int np, double *__restrict__ rhs)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = rhs[i];
+    for (int q = 0; q < np; ++q) s -= pval[q] * G[i * ld + pidx[q]];
+    rhs[i] = s;
+}
+
+int fp_lsq_rhs(fp_ctx *ctx, const double *G, int64_t n, int64_t ld, const int64_t *pidx, const double *pval, int np,
+               double *rhs)
+{
+    if (n <= 0) return FP_OK;
+    hipLaunchKernelGGL(k_lsq_rhs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, G, n, ld, pidx, pval,
+                       np, rhs);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
 // ----------------------------------------------------------------------------
 // X (nb x ncols, row-major ld) := L^-1 X with L (nb x nb, ld) unit lower, nb <= 256.
 // One workgroup per 32 columns; X in LDS; diagonal 32 x 32 blocks solved by one

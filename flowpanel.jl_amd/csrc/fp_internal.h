@@ -187,7 +187,7 @@ int fp_assembly_queue_tiles(int64_t N, int64_t M, int tm, int *nbx, int64_t *nti
 // `counter` is 8 bytes of device memory, reset by the launch itself
 int fp_launch_assemble_queue(fp_ctx *ctx, const AssemblyItem *d_items, int nitems, int64_t total, int nvp, int tm,
                              int prec, int *counter, int blocks);
-// Field sums of one body over targets.  mode 0: out3 (3 x M SoA) += U;
+// Field sums of one body over targets.  mode 0: out3 (3 x M point-major, out3[3i + d]) += U;
 // mode 1: out1[i] += n_i . U (strengths `x`);  mode 2: out1 += phi.
 int fp_launch_field(fp_ctx *ctx, const fp_body *b, const double *strength, const TargetView &tv, int mode,
                     double *out, double *partial, size_t partial_bytes);
@@ -206,6 +206,16 @@ size_t fp_getrs_work_bytes(int64_t n, int64_t k);
 // y = alpha*A*x + beta*y, A row-major (T), x,y FP64
 template <typename T>
 int fp_gemv(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, const double *x, double *y, double alpha, double beta);
+// C -= A B (row-major; MFMA tiles)
+template <typename T>
+int fp_gemm_sub(fp_ctx *ctx, int64_t m, int64_t n, int64_t k, const T *A, int64_t lda, const T *B, int64_t ldb, T *C,
+                int64_t ldc);
+// least squares of the watertight formulation: C (= -G^T G) -> normal matrix with the
+// prescribed rows/columns replaced by the identity, b[p] = val_p
+int fp_lsq_fix(fp_ctx *ctx, double *C, int64_t n, int64_t ld, const int64_t *pidx, int np, double *b,
+               const double *pval);
+int fp_lsq_rhs(fp_ctx *ctx, const double *G, int64_t n, int64_t ld, const int64_t *pidx, const double *pval, int np,
+               double *rhs);
 // Y[:, c] = A X[:, c], c < k (X, Y column-major)
 template <typename T>
 int fp_gemv_multi(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, const double *X, int64_t ldx, int64_t k, double *Y,
@@ -248,7 +258,7 @@ int fp_mdot(fp_ctx *ctx, const double *V, int64_t ldv, int nv, const double *w, 
 // normals, areas, control points and probes of one body
 int fp_adj_geometry(fp_ctx *ctx, const fp_body *b, double poff, int pkind, double pscale, int64_t M, int64_t col0,
                     double *nrm, double *area, double *cpt, double *prb);
-// V (3 x M, induced velocity) += Uinf; Cp = 1 - |V|^2/|Uinf|^2
+// V (3 x M point-major, induced velocity) += Uinf; Cp = 1 - |V|^2/|Uinf|^2
 int fp_adj_cp(fp_ctx *ctx, double *V, int64_t M, const double *Uinf, double *Cp);
 // Kutta averaging of c0 (reverse = 0) or its reverse on c0 and c1
 int fp_adj_kutta(fp_ctx *ctx, const int64_t *d_sh, int64_t ns, int mode, double *c0, double *c1, int reverse);

