@@ -158,7 +158,7 @@ def batch_legs(fp, ctx, reps=3):
     bodies = []
     for _ in range(16):
         ar, sweep, n = rng.uniform(6, 12), rng.uniform(0, 35), int(rng.uniform(1000, 12000))
-        nc = max(10, int(round(np.sqrt(n / ar) * 2.0)))
+        nc = max(10, 2 * int(round(np.sqrt(n / ar))))   # panels around the section: even
         ns = max(4, int(round(n / nc)))
         g, sh = fp.geometry.loft_wing(airfoil="0012", n_chord=nc, n_span=ns, span=ar * 0.75, root_chord=1.0,
                                       taper=0.5, sweep_deg=sweep)
