@@ -19,6 +19,10 @@ int fp_mdot(fp_ctx *ctx, const double *V, int64_t ldv, int nv, const double *w, 
 int fp_vupdate(fp_ctx *ctx, const double *V, int64_t ldv, int nv, const double *h, double *w, int64_t n);
 int fp_axpby(fp_ctx *ctx, double a, const double *x, double b, double *y, int64_t n);
 int fp_vcombine(fp_ctx *ctx, const double *V, int64_t ldv, int nv, const double *z, double *x, int64_t n);
+int fp_arnoldi_start(fp_ctx *ctx, double *g, int m, double beta);
+int fp_arnoldi_step(fp_ctx *ctx, const double *hbuf, int m, int j, double bnorm, double *H, double *cs, double *sn,
+                    double *g, double *relh, double *hnv, const double *w, double *vnext, int64_t n);
+int fp_arnoldi_solve(fp_ctx *ctx, const double *H, int m, int jc, const double *g, double *y);
 
 int fp_fail(fp_ctx *ctx, int code, const std::string &msg)
 {
@@ -238,12 +242,17 @@ double dnorm_host(fp_ctx *ctx, const double *x, int64_t n, double *part, double 
     return std::sqrt(v);
 }
 
-// Restarted GMRES with right preconditioning (x0 = 0 on entry when
-// `zero_start`).  Stops when ||b - A x|| <= tol * ||b|| (Arnoldi estimate)
-// or after maxit iterations.  CGS with one re-orthogonalisation.
+// Restarted GMRES with right preconditioning (x = 0 on entry).  Stops when
+// ||b - A x|| <= tol * ||b|| (Arnoldi estimate, confirmed by the true residual
+// at the end of a cycle) or after maxit iterations.  CGS with one
+// re-orthogonalisation.  The Hessenberg column, the Givens rotations and the
+// residual estimate are updated on the device; the host reads the estimates
+// only every few iterations, at a spacing predicted from their decay, so the
+// iterations queue back to back.  Iterations run past the converged one are
+// discarded (their columns are never used).
 struct Gmres {
     fp_ctx *ctx; int64_t n; int m;
-    DBuf V, Z, w, h, part, tmp;
+    DBuf V, Z, w, h, part, tmp, arn;
     int init(fp_ctx *c, int64_t n_, int m_)
     {
         ctx = c; n = n_; m = m_;
@@ -254,6 +263,8 @@ struct Gmres {
         if ((st = dalloc(ctx, Z, n * sizeof(double)))) return st;
         if ((st = dalloc(ctx, h, (m + 2) * 2 * sizeof(double)))) return st;
         if ((st = dalloc(ctx, part, (size_t)256 * (m + 2) * sizeof(double)))) return st;
+        // H (m+1) x m, cs, sn, g (m+1), relh, hn, y
+        if ((st = dalloc(ctx, arn, ((size_t)(m + 1) * m + 6 * (m + 1)) * sizeof(double)))) return st;
         return FP_OK;
     }
     // solve A x = b; x overwritten. Returns iterations via *its, final relative residual estimate via *rel.
@@ -262,13 +273,15 @@ struct Gmres {
         int st = FP_OK;
         double *Vp = V.as<double>(), *wp = w.as<double>(), *hp = h.as<double>(), *pp = part.as<double>();
         double *tp = tmp.as<double>(), *zp = Z.as<double>();
+        double *dH = arn.as<double>(), *dcs = dH + (size_t)(m + 1) * m, *dsn = dcs + (m + 1), *dg = dsn + (m + 1);
+        double *drel = dg + (m + 1), *dhn = drel + (m + 1), *dy = dhn + (m + 1);
         const double bnorm = dnorm_host(ctx, b, n, pp, hp, &st);
         if (st) return st;
         *its = 0;
         *rel = 0;
         FP_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
         if (bnorm == 0) return FP_OK;
-        std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hv(m + 2);
+        std::vector<double> relh(m), hnh(m);
         bool first = true;
         for (;;) {
             // r = b - A x
@@ -284,10 +297,11 @@ struct Gmres {
             *rel = beta / bnorm;
             if (*rel <= tol || *its >= maxit) return FP_OK;
             if ((st = fp_axpby(ctx, 1.0 / beta, wp, 0.0, Vp, n))) return st;
-            std::fill(g.begin(), g.end(), 0.0);
-            g[0] = beta;
-            int j = 0;
-            for (; j < m && *its < maxit; ++j) {
+            if ((st = fp_arnoldi_start(ctx, dg, m, beta))) return st;
+            int j = 0, jc = -1, checked = 0, next_check = 4;
+            double last_rel = *rel;
+            int last_j = 0;
+            while (j < m && *its < maxit) {
                 const double *vj = Vp + (int64_t)j * n;
                 const double *zin = vj;
                 if (M) {
@@ -295,53 +309,54 @@ struct Gmres {
                     zin = zp;
                 }
                 if ((st = A.apply(zin, wp))) return st;
-                // CGS2
+                // CGS2 and the norm of what is left
                 if ((st = fp_mdot(ctx, Vp, n, j + 1, wp, n, pp, hp))) return st;
                 if ((st = fp_vupdate(ctx, Vp, n, j + 1, hp, wp, n))) return st;
                 if ((st = fp_mdot(ctx, Vp, n, j + 1, wp, n, pp, hp + (m + 2)))) return st;
                 if ((st = fp_vupdate(ctx, Vp, n, j + 1, hp + (m + 2), wp, n))) return st;
                 if ((st = fp_mdot(ctx, wp, n, 1, wp, n, pp, hp + 2 * (m + 2) - 1))) return st;
-                std::vector<double> hh(2 * (m + 2));
-                FP_HIP(ctx, hipMemcpyAsync(hh.data(), hp, 2 * (m + 2) * sizeof(double), hipMemcpyDeviceToHost,
-                                           ctx->stream));
-                FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-                for (int i = 0; i <= j; ++i) hv[i] = hh[i] + hh[(m + 2) + i];
-                const double hn = std::sqrt(std::max(0.0, hh[2 * (m + 2) - 1]));
-                // apply previous rotations
-                for (int i = 0; i < j; ++i) {
-                    const double t = cs[i] * hv[i] + sn[i] * hv[i + 1];
-                    hv[i + 1] = -sn[i] * hv[i] + cs[i] * hv[i + 1];
-                    hv[i] = t;
-                }
-                const double den = std::hypot(hv[j], hn);
-                cs[j] = den > 0 ? hv[j] / den : 1.0;
-                sn[j] = den > 0 ? hn / den : 0.0;
-                hv[j] = den;
-                g[j + 1] = -sn[j] * g[j];
-                g[j] = cs[j] * g[j];
-                for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = hv[i];
+                if ((st = fp_arnoldi_step(ctx, hp, m, j, bnorm, dH, dcs, dsn, dg, drel, dhn, wp,
+                                          Vp + (int64_t)(j + 1) * n, n)))
+                    return st;
+                ++j;
                 ++*its;
-                if (hn > 0) {
-                    if ((st = fp_axpby(ctx, 1.0 / hn, wp, 0.0, Vp + (int64_t)(j + 1) * n, n))) return st;
+                if (j < next_check && j < m && *its < maxit) continue;
+                const int cnt = j - checked;
+                FP_HIP(ctx, hipMemcpyAsync(relh.data() + checked, drel + checked, cnt * sizeof(double),
+                                           hipMemcpyDeviceToHost, ctx->stream));
+                FP_HIP(ctx, hipMemcpyAsync(hnh.data() + checked, dhn + checked, cnt * sizeof(double),
+                                           hipMemcpyDeviceToHost, ctx->stream));
+                FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                for (int q = checked; q < j; ++q)
+                    if (relh[q] <= tol || hnh[q] == 0 || !std::isfinite(relh[q])) { jc = q + 1; break; }
+                if (jc > 0) {
+                    *its -= j - jc;
+                    break;
                 }
-                *rel = std::fabs(g[j + 1]) / bnorm;
-                if (*rel <= tol || hn == 0) { ++j; break; }
+                // next check where the decay of the estimate says tol is reached
+                const double r = relh[j - 1];
+                int K = 8;
+                if (j > last_j && r < last_rel && r > 0) {
+                    const double rate = std::log(r / last_rel) / (j - last_j);
+                    const double remaining = std::log(tol / r) / rate;
+                    K = (int)std::max(1.0, std::min(16.0, std::floor(0.8 * remaining)));
+                }
+                last_rel = r;
+                last_j = j;
+                checked = j;
+                next_check = j + K;
             }
-            // y = H^-1 g
-            for (int i = j - 1; i >= 0; --i) {
-                double acc = g[i];
-                for (int q = i + 1; q < j; ++q) acc -= H[(size_t)i * m + q] * y[q];
-                y[i] = acc / H[(size_t)i * m + i];
-            }
-            FP_HIP(ctx, hipMemcpyAsync(hp, y.data(), j * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-            if ((st = fp_vcombine(ctx, Vp, n, j, hp, tp, n))) return st;
+            if (jc < 0) jc = j;
+            *rel = relh[jc - 1];
+            if (!std::isfinite(*rel)) return fp_fail(ctx, FP_ERR_NOT_CONVERGED, "GMRES produced a non-finite residual");
+            if ((st = fp_arnoldi_solve(ctx, dH, m, jc, dg, dy))) return st;
+            if ((st = fp_vcombine(ctx, Vp, n, jc, dy, tp, n))) return st;
             if (M) {
                 if ((st = M->apply(tp, zp))) return st;
                 if ((st = fp_axpby(ctx, 1.0, zp, 1.0, x, n))) return st;
             } else {
                 if ((st = fp_axpby(ctx, 1.0, tp, 1.0, x, n))) return st;
             }
-            FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
             if (*rel <= tol || *its >= maxit) {
                 // the true residual of the final iterate decides; restart if the
                 // Arnoldi estimate was optimistic and iterations are left

@@ -767,9 +767,13 @@ template <> struct Vec4<double> {
     }
 };
 
-constexpr int GEMV_R = 4;
+// Rows per wave R of the Krylov GEMV: each x load is reused by R rows.  At
+// small n two rows keep twice as many waves in flight (measured on gfx950 at
+// n = 10,000: 0.076 ms with R = 2 against 0.110 ms with R = 4); at n = 30,000
+// there are waves enough and R = 4 is faster (0.64 against 0.68 ms).
+constexpr int GEMVM_R = 4;  // rows per wave of the multi-vector GEMV
 
-template <typename T, bool XVEC>
+template <typename T, bool XVEC, int GEMV_R>
 __global__ __launch_bounds__(256) void k_gemv(const T *__restrict__ A, int64_t n, int64_t ld,
                                               const double *__restrict__ x, double *__restrict__ y, double alpha,
                                               double beta)
@@ -778,7 +782,9 @@ __global__ __launch_bounds__(256) void k_gemv(const T *__restrict__ A, int64_t n
     const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * GEMV_R;
     if (r0 >= n) return;
     const int nr = (int)min((int64_t)GEMV_R, n - r0);
-    double acc[GEMV_R] = {0, 0, 0, 0};
+    double acc[GEMV_R];
+#pragma unroll
+    for (int r = 0; r < GEMV_R; ++r) acc[r] = 0;
     const int64_t n4 = (n / 4) * 4;  // ld is a multiple of 64, rows are 256-byte aligned
     for (int64_t c = 4 * lane; c < n4; c += 256) {
         double2 xa, xb;
@@ -807,21 +813,29 @@ __global__ __launch_bounds__(256) void k_gemv(const T *__restrict__ A, int64_t n
     }
 }
 
+template <typename T, int R>
+static int gemv_launch(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, const double *x, double *y, double alpha,
+                       double beta)
+{
+    const dim3 grid((unsigned)((n + 4 * R - 1) / (4 * R)));
+    if (((uintptr_t)x & 15) == 0 && ((uintptr_t)A & 15) == 0)
+        hipLaunchKernelGGL((k_gemv<T, true, R>), grid, dim3(256), 0, ctx->stream, A, n, ld, x, y, alpha, beta);
+    else if (((uintptr_t)A & 15) == 0)
+        hipLaunchKernelGGL((k_gemv<T, false, R>), grid, dim3(256), 0, ctx->stream, A, n, ld, x, y, alpha, beta);
+    else
+        return fp_fail(ctx, FP_ERR_ARG, "gemv needs a 16-byte aligned matrix");
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
 template <typename T>
 int fp_gemv(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, const double *x, double *y, double alpha, double beta)
 {
     if (n <= 0) return FP_OK;
     if (ld % 4) return fp_fail(ctx, FP_ERR_ARG, "gemv needs ld % 4 == 0");
     ProfScope ps(ctx, sizeof(T) == 8 ? "gemv_f64" : "gemv_f32");
-    const dim3 grid((unsigned)((n + 4 * GEMV_R - 1) / (4 * GEMV_R)));
-    if (((uintptr_t)x & 15) == 0 && ((uintptr_t)A & 15) == 0)
-        hipLaunchKernelGGL((k_gemv<T, true>), grid, dim3(256), 0, ctx->stream, A, n, ld, x, y, alpha, beta);
-    else if (((uintptr_t)A & 15) == 0)
-        hipLaunchKernelGGL((k_gemv<T, false>), grid, dim3(256), 0, ctx->stream, A, n, ld, x, y, alpha, beta);
-    else
-        return fp_fail(ctx, FP_ERR_ARG, "gemv needs a 16-byte aligned matrix");
-    FP_HIP(ctx, hipGetLastError());
-    return FP_OK;
+    return n <= 20000 ? gemv_launch<T, 2>(ctx, A, n, ld, x, y, alpha, beta)
+                      : gemv_launch<T, 4>(ctx, A, n, ld, x, y, alpha, beta);
 }
 template int fp_gemv<double>(fp_ctx *, const double *, int64_t, int64_t, const double *, double *, double, double);
 template int fp_gemv<float>(fp_ctx *, const float *, int64_t, int64_t, const double *, double *, double, double);
@@ -837,21 +851,21 @@ __global__ __launch_bounds__(256) void k_gemv_multi(const T *__restrict__ A, int
                                                     double *__restrict__ Y, int64_t ldy)
 {
     const int lane = threadIdx.x & 63;
-    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * GEMV_R;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * GEMVM_R;
     if (r0 >= n) return;
-    const int nr = (int)min((int64_t)GEMV_R, n - r0);
+    const int nr = (int)min((int64_t)GEMVM_R, n - r0);
     const int c0 = blockIdx.y * GEMVM_NV;
     const int nv = min(GEMVM_NV, k - c0);
-    double acc[GEMV_R][GEMVM_NV];
+    double acc[GEMVM_R][GEMVM_NV];
 #pragma unroll
-    for (int r = 0; r < GEMV_R; ++r)
+    for (int r = 0; r < GEMVM_R; ++r)
 #pragma unroll
         for (int q = 0; q < GEMVM_NV; ++q) acc[r][q] = 0;
     const int64_t n4 = (n / 4) * 4;
     for (int64_t c = 4 * lane; c < n4; c += 256) {
-        double v[GEMV_R][4];
+        double v[GEMVM_R][4];
 #pragma unroll
-        for (int r = 0; r < GEMV_R; ++r) {
+        for (int r = 0; r < GEMVM_R; ++r) {
             if (r < nr) Vec4<T>::load(A + (r0 + r) * ld + c, v[r]);
             else v[r][0] = v[r][1] = v[r][2] = v[r][3] = 0;
         }
@@ -861,7 +875,7 @@ __global__ __launch_bounds__(256) void k_gemv_multi(const T *__restrict__ A, int
                 const double *x = X + (int64_t)(c0 + q) * ldx + c;
                 const double x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3];
 #pragma unroll
-                for (int r = 0; r < GEMV_R; ++r) acc[r][q] += v[r][0] * x0 + v[r][1] * x1 + v[r][2] * x2 + v[r][3] * x3;
+                for (int r = 0; r < GEMVM_R; ++r) acc[r][q] += v[r][0] * x0 + v[r][1] * x1 + v[r][2] * x2 + v[r][3] * x3;
             }
         }
     }
@@ -869,7 +883,7 @@ __global__ __launch_bounds__(256) void k_gemv_multi(const T *__restrict__ A, int
         for (int r = 0; r < nr; ++r)
             for (int q = 0; q < nv; ++q) acc[r][q] += (double)A[(r0 + r) * ld + c] * X[(int64_t)(c0 + q) * ldx + c];
 #pragma unroll
-    for (int r = 0; r < GEMV_R; ++r)
+    for (int r = 0; r < GEMVM_R; ++r)
 #pragma unroll
         for (int q = 0; q < GEMVM_NV; ++q) {
             double a = acc[r][q];
@@ -885,7 +899,7 @@ int fp_gemv_multi(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, const double *
     if (n <= 0 || k <= 0) return FP_OK;
     if (ld % 4 || ((uintptr_t)A & 15)) return fp_fail(ctx, FP_ERR_ARG, "gemv needs an aligned matrix");
     ProfScope ps(ctx, sizeof(T) == 8 ? "gemv_multi_f64" : "gemv_multi_f32");
-    const dim3 grid((unsigned)((n + 4 * GEMV_R - 1) / (4 * GEMV_R)), (unsigned)((k + GEMVM_NV - 1) / GEMVM_NV));
+    const dim3 grid((unsigned)((n + 4 * GEMVM_R - 1) / (4 * GEMVM_R)), (unsigned)((k + GEMVM_NV - 1) / GEMVM_NV));
     hipLaunchKernelGGL(k_gemv_multi<T>, grid, dim3(256), 0, ctx->stream, A, n, ld, X, ldx, (int)k, Y, ldy);
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
@@ -1309,38 +1323,51 @@ int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, doub
 template int fp_coarse_setup<double>(fp_ctx *, const double *, int64_t, int64_t, int, double *, double *, double *, int *);
 template int fp_coarse_setup<float>(fp_ctx *, const float *, int64_t, int64_t, int, double *, double *, double *, int *);
 
-// c = Einv (Z^T v): one workgroup (restriction + coarse solve)
-__global__ __launch_bounds__(1024) void k_coarse_solve(const double *__restrict__ v, int64_t n, int cs, int nc,
-                                                       const double *__restrict__ Einv, double *__restrict__ t,
-                                                       double *__restrict__ c)
+// t = Z^T v: one wave per chunk (chunk sums of v)
+__global__ __launch_bounds__(256) void k_coarse_restrict(const double *__restrict__ v, int64_t n, int cs, int nc,
+                                                         double *__restrict__ t)
 {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
-    for (int k = wave; k < nc; k += nw) {
-        const int64_t r0 = (int64_t)k * cs, r1 = min(n, r0 + cs);
-        double acc = 0;
-        for (int64_t r = r0 + lane; r < r1; r += 64) acc += v[r];
-        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-        if (lane == 0) t[k] = acc;
-    }
-    __syncthreads();
-    for (int k = wave; k < nc; k += nw) {
-        double acc = 0;
-        for (int q = lane; q < nc; q += 64) acc += Einv[(int64_t)k * nc + q] * t[q];
-        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-        if (lane == 0) c[k] = acc;
-    }
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (k >= nc) return;
+    const int64_t r0 = (int64_t)k * cs, r1 = min(n, r0 + cs);
+    double acc = 0;
+    for (int64_t r = r0 + lane; r < r1; r += 64) acc += v[r];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) t[k] = acc;
 }
 
-// y = Z c, r2 = v - AZ c
-__global__ void k_coarse_resid(const double *__restrict__ v, const double *__restrict__ AZ, int64_t n, int cs, int nc,
-                               const double *__restrict__ c, double *__restrict__ y, double *__restrict__ r2)
+// c = Einv t: one wave per coarse row
+__global__ __launch_bounds__(256) void k_coarse_solve(const double *__restrict__ Einv, int nc,
+                                                      const double *__restrict__ t, double *__restrict__ c)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (k >= nc) return;
     double acc = 0;
-    for (int k = 0; k < nc; ++k) acc += AZ[(int64_t)k * n + i] * c[k];
-    r2[i] = v[i] - acc;
-    y[i] = c[i / cs];
+    for (int q = lane; q < nc; q += 64) acc += Einv[(int64_t)k * nc + q] * t[q];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) c[k] = acc;
+}
+
+// y = Z c, r2 = v - AZ c: 64 rows per workgroup, the nc columns of AZ split
+// over the 4 waves (coalesced rows of AZ, summed through LDS)
+__global__ __launch_bounds__(256) void k_coarse_resid(const double *__restrict__ v, const double *__restrict__ AZ,
+                                                      int64_t n, int cs, int nc, const double *__restrict__ c,
+                                                      double *__restrict__ y, double *__restrict__ r2)
+{
+    __shared__ double part[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+    double acc = 0;
+    if (i < n)
+        for (int k = wave; k < nc; k += 4) acc += AZ[(int64_t)k * n + i] * c[k];
+    part[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && i < n) {
+        r2[i] = v[i] - (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
+        y[i] = c[i / cs];
+    }
 }
 
 int fp_coarse_apply(fp_ctx *ctx, const double *v, int64_t n, int cs, const double *AZ, const double *Einv, double *tc,
@@ -1348,8 +1375,10 @@ int fp_coarse_apply(fp_ctx *ctx, const double *v, int64_t n, int cs, const doubl
 {
     const int nc = (int)((n + cs - 1) / cs);
     ProfScope ps(ctx, "coarse_apply");
-    hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(1024), 0, ctx->stream, v, n, cs, nc, Einv, tc, tc + nc);
-    hipLaunchKernelGGL(k_coarse_resid, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, v, AZ, n, cs, nc,
+    const unsigned g4 = (unsigned)((nc + 3) / 4);
+    hipLaunchKernelGGL(k_coarse_restrict, dim3(g4), dim3(256), 0, ctx->stream, v, n, cs, nc, tc);
+    hipLaunchKernelGGL(k_coarse_solve, dim3(g4), dim3(256), 0, ctx->stream, Einv, nc, (const double *)tc, tc + nc);
+    hipLaunchKernelGGL(k_coarse_resid, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, ctx->stream, v, AZ, n, cs, nc,
                        (const double *)(tc + nc), y, r2);
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
@@ -1412,47 +1441,60 @@ int fp_residual_update(fp_ctx *ctx, const double *b, double *r, int64_t n)
 // multi-dot: out[c] = sum_i V[c*ldv + i] * w[i], c < nv; two passes (partials
 // per workgroup in a fixed order, then a single-workgroup sum) so the result is
 // deterministic.
+// out[c] = V[:, c] . w for c < nv: MDOT_G columns per grid row accumulated in
+// registers, MDOT_PARTS row chunks; a second kernel sums the chunks (one
+// thread per column, fixed order).
+constexpr int MDOT_G = 8;
+constexpr int MDOT_PARTS = 128;
+
 __global__ __launch_bounds__(256) void k_mdot_partial(const double *__restrict__ V, int64_t ldv, int nv,
                                                       const double *__restrict__ w, int64_t n,
                                                       double *__restrict__ part)
 {
-    __shared__ double red[4];
+    __shared__ double red[4][MDOT_G];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c0 = blockIdx.y * MDOT_G;
+    const int nc = min(MDOT_G, nv - c0);
     const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
     const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
-    for (int c = 0; c < nv; ++c) {
-        double acc = 0;
-        for (int64_t i = i0 + tid; i < i1; i += 256) acc += V[c * ldv + i] * w[i];
-        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-        if (lane == 0) red[wave] = acc;
-        __syncthreads();
-        if (tid == 0) part[(int64_t)c * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
-        __syncthreads();
+    double acc[MDOT_G];
+#pragma unroll
+    for (int q = 0; q < MDOT_G; ++q) acc[q] = 0;
+    for (int64_t i = i0 + tid; i < i1; i += 256) {
+        const double wi = w[i];
+#pragma unroll
+        for (int q = 0; q < MDOT_G; ++q)
+            if (q < nc) acc[q] += V[(int64_t)(c0 + q) * ldv + i] * wi;
     }
+#pragma unroll
+    for (int q = 0; q < MDOT_G; ++q) {
+        double a = acc[q];
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+        if (lane == 0) red[wave][q] = a;
+    }
+    __syncthreads();
+    if (tid < nc) part[(int64_t)(c0 + tid) * gridDim.x + blockIdx.x] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
 }
 
 __global__ __launch_bounds__(256) void k_mdot_final(const double *__restrict__ part, int nparts, int nv,
                                                     double *__restrict__ out)
 {
-    __shared__ double red[4];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int c = 0; c < nv; ++c) {
-        double acc = 0;
-        for (int k = tid; k < nparts; k += 256) acc += part[(int64_t)c * nparts + k];
-        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-        if (lane == 0) red[wave] = acc;
-        __syncthreads();
-        if (tid == 0) out[c] = red[0] + red[1] + red[2] + red[3];
-        __syncthreads();
-    }
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nv) return;
+    double acc = 0;
+    for (int k = 0; k < nparts; ++k) acc += part[(int64_t)c * nparts + k];
+    out[c] = acc;
 }
 
 int fp_mdot(fp_ctx *ctx, const double *V, int64_t ldv, int nv, const double *w, int64_t n, double *part,
             double *out)
 {
-    const int parts = 256;
-    hipLaunchKernelGGL(k_mdot_partial, dim3(parts), dim3(256), 0, ctx->stream, V, ldv, nv, w, n, part);
-    hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(256), 0, ctx->stream, (const double *)part, parts, nv, out);
+    if (nv <= 0) return FP_OK;
+    const int parts = (int)std::max<int64_t>(1, std::min<int64_t>(MDOT_PARTS, (n + 255) / 256));
+    hipLaunchKernelGGL(k_mdot_partial, dim3(parts, (nv + MDOT_G - 1) / MDOT_G), dim3(256), 0, ctx->stream, V, ldv, nv,
+                       w, n, part);
+    hipLaunchKernelGGL(k_mdot_final, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, (const double *)part, parts, nv,
+                       out);
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }
@@ -1485,6 +1527,85 @@ __global__ void k_axpby(double a, const double *__restrict__ x, double b, double
 int fp_axpby(fp_ctx *ctx, double a, const double *x, double b, double *y, int64_t n)
 {
     hipLaunchKernelGGL(k_axpby, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, a, x, b, y, n);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+// ----------------------------------------------------------------------------
+// Arnoldi bookkeeping of restarted GMRES kept on the device, so that the
+// iterations are queued back to back without a host round trip; the host only
+// reads the residual estimates every few iterations.  One thread: the work is
+// O(j) per iteration.  H row-major (m+1) x m.
+__global__ void k_arnoldi_start(double *__restrict__ g, int m, double beta)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= m) g[i] = i == 0 ? beta : 0.0;
+}
+
+__global__ void k_arnoldi_step(const double *__restrict__ hbuf, int m, int j, double bnorm, double *__restrict__ H,
+                               double *__restrict__ cs, double *__restrict__ sn, double *__restrict__ g,
+                               double *__restrict__ relh, double *__restrict__ hnv)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double *h1 = hbuf, *h2 = hbuf + (m + 2);
+    const double hn = sqrt(fmax(0.0, hbuf[2 * (m + 2) - 1]));
+    for (int i = 0; i <= j; ++i) H[(int64_t)i * m + j] = h1[i] + h2[i];
+    for (int i = 0; i < j; ++i) {
+        const double a = H[(int64_t)i * m + j], b = H[(int64_t)(i + 1) * m + j];
+        H[(int64_t)i * m + j] = cs[i] * a + sn[i] * b;
+        H[(int64_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
+    }
+    const double hj = H[(int64_t)j * m + j];
+    const double den = hypot(hj, hn);
+    cs[j] = den > 0 ? hj / den : 1.0;
+    sn[j] = den > 0 ? hn / den : 0.0;
+    H[(int64_t)j * m + j] = den;
+    g[j + 1] = -sn[j] * g[j];
+    g[j] = cs[j] * g[j];
+    relh[j] = fabs(g[j + 1]) / bnorm;
+    hnv[j] = hn;
+}
+
+__global__ void k_arnoldi_normalize(const double *__restrict__ w, const double *__restrict__ hnv, int j,
+                                    double *__restrict__ v, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double hn = hnv[j];
+    if (i < n && hn > 0) v[i] = w[i] / hn;
+}
+
+// y = H(0:jc, 0:jc)^-1 g(0:jc), upper triangular
+__global__ void k_arnoldi_solve(const double *__restrict__ H, int m, int jc, const double *__restrict__ g,
+                                double *__restrict__ y)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int i = jc - 1; i >= 0; --i) {
+        double acc = g[i];
+        for (int q = i + 1; q < jc; ++q) acc -= H[(int64_t)i * m + q] * y[q];
+        y[i] = acc / H[(int64_t)i * m + i];
+    }
+}
+
+int fp_arnoldi_start(fp_ctx *ctx, double *g, int m, double beta)
+{
+    hipLaunchKernelGGL(k_arnoldi_start, dim3((m + 256) / 256), dim3(256), 0, ctx->stream, g, m, beta);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+int fp_arnoldi_step(fp_ctx *ctx, const double *hbuf, int m, int j, double bnorm, double *H, double *cs, double *sn,
+                    double *g, double *relh, double *hnv, const double *w, double *vnext, int64_t n)
+{
+    hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(64), 0, ctx->stream, hbuf, m, j, bnorm, H, cs, sn, g, relh, hnv);
+    hipLaunchKernelGGL(k_arnoldi_normalize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, w,
+                       (const double *)hnv, j, vnext, n);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+int fp_arnoldi_solve(fp_ctx *ctx, const double *H, int m, int jc, const double *g, double *y)
+{
+    hipLaunchKernelGGL(k_arnoldi_solve, dim3(1), dim3(64), 0, ctx->stream, H, m, jc, g, y);
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }

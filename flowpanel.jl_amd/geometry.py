@@ -277,6 +277,71 @@ def triangulate(grid, shedding=None):
     return g, sh
 
 
+def calc_shedding(grid, trailingedge, tolerance=None, periodic=False):
+    """Shedding matrix (6 x ne, 0-based, pj = -1 when single-sided) of the mesh
+    edges lying on a trailing-edge polyline (src/FLOWPanel_liftingbody.jl:942-1043).
+
+    ``trailingedge`` is 3 x k points ordered along the edge.  Nodes within
+    ``tolerance`` of the polyline are ordered by their position along it;
+    consecutive pairs that are mesh edges become shedding edges.  The panel in
+    which the pair appears in the order (first, second) is ``pi`` (nia = local
+    index of the second node, nib of the first); the panel on the other side is
+    ``pj`` (nja = first, njb = second).
+    """
+    X = np.asarray(grid.nodes, dtype=np.float64)
+    T = np.asarray(trailingedge, dtype=np.float64)
+    if T.ndim != 2 or T.shape[0] != 3 or T.shape[1] < 2:
+        raise ValueError("trailingedge must be 3 x k (k >= 2)")
+    if tolerance is None:
+        tolerance = 1e2 * np.finfo(float).eps
+    # distance of every node to the polyline and its arc-length position
+    seg0, seg1 = T[:, :-1], T[:, 1:]
+    d = seg1 - seg0
+    L = np.linalg.norm(d, axis=0)
+    s0 = np.concatenate([[0.0], np.cumsum(L)[:-1]])
+    best = np.full(X.shape[1], np.inf)
+    pos = np.zeros(X.shape[1])
+    for k in range(seg0.shape[1]):
+        if L[k] == 0:
+            continue
+        t = np.clip(((X - seg0[:, k:k + 1]) * d[:, k:k + 1]).sum(axis=0) / L[k] ** 2, 0.0, 1.0)
+        dist = np.linalg.norm(X - (seg0[:, k:k + 1] + t * d[:, k:k + 1]), axis=0)
+        better = dist < best
+        best[better] = dist[better]
+        pos[better] = s0[k] + t[better] * L[k]
+    te = np.nonzero(best <= tolerance)[0]
+    te = te[np.argsort(pos[te], kind="stable")]
+    if len(te) == 0:
+        return np.zeros((6, 0), dtype=np.int64)
+    if periodic:
+        te = np.append(te, te[0])
+    # directed edges of the cells
+    owner = {}
+    for c in range(grid.ncells):
+        cell = grid.cell(c)
+        for k in range(len(cell)):
+            owner.setdefault((int(cell[k]), int(cell[(k + 1) % len(cell)])), []).append(c)
+    cols = []
+    for a, b in zip(te[:-1], te[1:]):
+        a, b = int(a), int(b)
+        fwd, bwd = owner.get((a, b), []), owner.get((b, a), [])
+        if not fwd and not bwd:
+            continue
+        if fwd and bwd:
+            pi, pj = fwd[0], bwd[0]
+        else:
+            pi, pj = (fwd or bwd)[0], -1
+        ci = list(grid.cell(pi))
+        nia, nib = ci.index(b), ci.index(a)
+        if pj != -1:
+            cj = list(grid.cell(pj))
+            nja, njb = cj.index(a), cj.index(b)
+        else:
+            nja = njb = -1
+        cols.append((pi, nia, nib, pj, nja, njb))
+    return np.array(cols, dtype=np.int64).T.reshape(6, -1)
+
+
 def direction(alpha_deg, beta_deg=0.0):
     """Freestream unit vector (cos a cos b, -sin b, sin a cos b)
     (src/FLOWPanel_utils.jl:34-37)."""

@@ -95,3 +95,89 @@ def test_naca_section(geo):
     assert np.allclose(c[:, 0], c[:, -1])          # sharp trailing edge
     assert abs(c[1].max() - 0.06) < 2e-3             # 12 % thick
     assert np.allclose(geo.direction(0.0), [1, 0, 0])
+
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _sweptwing_te(side):
+    """The trailing-edge lines of the reference's mesh test (test/runtests_meshes.jl:215-245)."""
+    y = -1.2446 if side == "left" else 1.2446
+    X1 = np.array([1.74244, y, 1.38094e-07])
+    X2 = np.array([0.49784, -1.38178e-16 if side == "left" else 1.38178e-16, 1.38094e-07])
+    te = np.stack([X1 + v * (X2 - X1) for v in np.linspace(0, 1, 1000)], axis=1)
+    key = te[1] if side == "left" else -te[1]
+    return te[:, np.argsort(key, kind="stable")]
+
+
+def test_read_vtu_reference_meshes(fp):
+    for side in ("left", "right"):
+        g = fp.read_vtu(os.path.join(GOLDEN, "sweptwing-%s000.vtu" % side))
+        assert g.nnodes == 768 and g.ncells == 1440
+        assert (g.nvertices() == 3).all()
+        # the tip trailing-edge node and the span of each half
+        assert np.isclose(np.abs(g.nodes[1]).max(), 1.2446)
+        assert np.allclose(g.nodes[0].max(), 1.74244)
+        # a closed surface: every edge is shared by exactly two triangles (open only at the root and tip)
+        edges = {}
+        for c in range(g.ncells):
+            cell = g.cell(c)
+            for k in range(3):
+                e = tuple(sorted((int(cell[k]), int(cell[(k + 1) % 3]))))
+                edges[e] = edges.get(e, 0) + 1
+        assert max(edges.values()) == 2
+
+
+def test_calc_shedding_reference_meshes(fp):
+    for side in ("left", "right"):
+        g = fp.read_vtu(os.path.join(GOLDEN, "sweptwing-%s000.vtu" % side))
+        te = _sweptwing_te(side)
+        span = np.ptp(te[1])
+        sh = fp.calc_shedding(g, te, tolerance=0.001 * span)
+        assert sh.shape == (6, 15)
+        for e in range(sh.shape[1]):
+            pi, nia, nib, pj, nja, njb = sh[:, e]
+            ci, cj = g.cell(pi), g.cell(pj)
+            # the two panels share the edge with opposite orientation
+            assert ci[nia] == cj[njb] and ci[nib] == cj[nja]
+            assert (list(ci).index(ci[nib]) + 1) % 3 == nia
+            # both end points lie on the trailing edge line
+            for v in (ci[nia], ci[nib]):
+                p = g.nodes[:, v]
+                d = np.cross(te[:, -1] - te[:, 0], p - te[:, 0])
+                assert np.linalg.norm(d) / np.linalg.norm(te[:, -1] - te[:, 0]) < 1e-6
+        # consecutive shedding edges are chained along the edge
+        for e in range(sh.shape[1] - 1):
+            assert g.cell(sh[0, e])[sh[1, e]] == g.cell(sh[0, e + 1])[sh[2, e + 1]]
+
+
+def test_read_msh_synthetic(fp, tmp_path):
+    txt = """$MeshFormat
+4.1 0 8
+$EndMeshFormat
+$Nodes
+1 4 1 4
+2 1 0 4
+10
+11
+12
+13
+0 0 0
+1 0 0
+1 1 0
+0 1 0
+$EndNodes
+$Elements
+2 3 1 3
+1 1 1 1
+7 10 11
+2 1 2 2
+5 10 11 12
+6 10 12 13
+$EndElements
+"""
+    p = tmp_path / "two.msh"
+    p.write_text(txt)
+    g = fp.read_msh(str(p))
+    assert g.nnodes == 4 and g.ncells == 2
+    assert list(g.cell(0)) == [0, 1, 2] and list(g.cell(1)) == [0, 2, 3]
