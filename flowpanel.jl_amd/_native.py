@@ -47,6 +47,7 @@ EXPORTS = [
     "fp_ctx_profile_get", "fp_ctx_copy_calibrate", "fp_body_nnodes", "fp_body_set_nodes",
     "fp_liftdrag_opts_default", "fp_lift_drag", "fp_lift_drag_adjoint", "fp_solve_bodies_multi",
     "fp_batch_create", "fp_batch_solve", "fp_batch_destroy", "fp_batch_autotune_get", "fp_solve_bodies_lsq",
+    "fp_calcfield_ugradmu",
 ]
 
 
@@ -163,6 +164,7 @@ def lib():
     L.fp_batch_autotune_get.argtypes = [_vp, cint, ctypes.POINTER(cint), ctypes.POINTER(cint), _d]
     L.fp_solve_bodies_lsq.argtypes = [_vp, pp, cint, _i64, _d, i64, cint, ctypes.POINTER(SolveOpts),
                                       ctypes.POINTER(SolveStats)]
+    L.fp_calcfield_ugradmu.argtypes = [_vp, pp, cint, cint, _d]
     L.fp_body_nnodes.argtypes = [_vp]
     L.fp_body_nnodes.restype = i64
     L.fp_body_set_nodes.argtypes = [_vp, _vp, _d]

@@ -1178,7 +1178,7 @@ void fp_body_destroy(fp_body *b)
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
     for (void *p : {(void *)b->d_pv, (void *)b->d_nv, (void *)b->d_hs, (void *)b->d_geo, (void *)b->d_uinf,
-                    (void *)b->d_hh, (void *)b->d_hcol, (void *)b->d_strength})
+                    (void *)b->d_hh, (void *)b->d_hcol, (void *)b->d_strength, (void *)b->d_nbr, (void *)b->d_sh})
         if (p) (void)hipFree(p);
     delete b;
 }
@@ -2375,6 +2375,69 @@ int fp_batch_autotune_get(const fp_batch *h, int k, int *tile_targets, int *work
     if (tile_targets) *tile_targets = h->impl->tune_tm[k];
     if (workgroups) *workgroups = h->impl->tune_blocks[k];
     if (ms) *ms = h->impl->tune_ms[k];
+    return FP_OK;
+}
+
+// neighbour across edge k (vertices k, k+1) of every cell, the cell sharing
+// both end points (gt.neighbor with preserveEdge), uploaded once per body
+static int body_neighbors(fp_ctx *ctx, fp_body *b)
+{
+    if (b->d_nbr) return FP_OK;
+    const int64_t N = b->N, nvp = b->nvp;
+    std::map<std::pair<int64_t, int64_t>, std::vector<int64_t>> owners;
+    for (int64_t j = 0; j < N; ++j) {
+        const int64_t p0 = b->cell_ptr[j], nv = b->cell_ptr[j + 1] - p0;
+        for (int64_t k = 0; k < nv; ++k) {
+            int64_t a = b->cell_idx[p0 + k], c = b->cell_idx[p0 + (k + 1) % nv];
+            if (a > c) std::swap(a, c);
+            owners[{a, c}].push_back(j);
+        }
+    }
+    std::vector<int> nb((size_t)nvp * N, -1);
+    for (int64_t j = 0; j < N; ++j) {
+        const int64_t p0 = b->cell_ptr[j], nv = b->cell_ptr[j + 1] - p0;
+        for (int64_t k = 0; k < nv; ++k) {
+            int64_t a = b->cell_idx[p0 + k], c = b->cell_idx[p0 + (k + 1) % nv];
+            if (a > c) std::swap(a, c);
+            for (int64_t o : owners[{a, c}])
+                if (o != j) { nb[(size_t)k * N + j] = (int)o; break; }
+        }
+    }
+    FP_HIP(ctx, hipMalloc((void **)&b->d_nbr, nb.size() * sizeof(int)));
+    FP_HIP(ctx, hipMemcpy(b->d_nbr, nb.data(), nb.size() * sizeof(int), hipMemcpyHostToDevice));
+    return FP_OK;
+}
+
+int fp_calcfield_ugradmu(fp_ctx *ctx, fp_body *const *bodies, int nb, int te_correction, double *out)
+{
+    int st = check_bodies(ctx, bodies, nb);
+    if (st) return st;
+    if (!out) return fp_fail(ctx, FP_ERR_ARG, "null out");
+    int64_t M = 0;
+    for (int k = 0; k < nb; ++k) {
+        if (!bodies[k]->solved)
+            return fp_fail(ctx, FP_ERR_UNSOLVED, "Source body hasn't been solved yet. Please call `solve(...)` function first.");
+        if (bodies[k]->element != FP_ELEM_VORTEXRING)
+            return fp_fail(ctx, FP_ERR_UNSUPPORTED, "the doublet-gradient velocity needs vortex-ring bodies");
+        M += bodies[k]->N;
+    }
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    DBuf o;
+    if ((st = dalloc(ctx, o, 3 * M * sizeof(double)))) return st;
+    FP_HIP(ctx, hipMemcpyAsync(o.p, out, 3 * M * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    int64_t off = 0;
+    for (int k = 0; k < nb; ++k) {
+        fp_body *b = bodies[k];
+        if ((st = body_neighbors(ctx, b))) return st;
+        if (b->ns > 0 && !b->d_sh) {
+            FP_HIP(ctx, hipMalloc((void **)&b->d_sh, 6 * b->ns * sizeof(int64_t)));
+            FP_HIP(ctx, hipMemcpy(b->d_sh, b->shedding.data(), 6 * b->ns * sizeof(int64_t), hipMemcpyHostToDevice));
+        }
+        if ((st = fp_launch_ugradmu(ctx, b, b->d_nbr, b->d_sh, te_correction, o.as<double>() + 3 * off))) return st;
+        off += b->N;
+    }
+    FP_HIP(ctx, hipMemcpyAsync(out, o.p, 3 * M * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return FP_OK;
 }
 

@@ -100,6 +100,8 @@ struct fp_body {
     double *d_strength = nullptr;
     bool has_uinf = false, has_wake = false, solved = false;
     uint64_t version = 0;  // bumped whenever device geometry or wake buffers change
+    int *d_nbr = nullptr;         // cell neighbours across each edge (built on first use)
+    int64_t *d_sh = nullptr;      // shedding table on the device (built on first use)
 };
 
 struct fp_mat {
@@ -251,6 +253,11 @@ int fp_fill(fp_ctx *ctx, double *x, int64_t n, double v);   // x[:] = v (a kerne
 // multi-dot out[c] = sum_i V[c*ldv + i] w[i] (part: 256*nv doubles)
 int fp_mdot(fp_ctx *ctx, const double *V, int64_t ldv, int nv, const double *w, int64_t n, double *part,
             double *out);
+
+// ---- surface post-processing (fp_post.hip) --------------------------------
+// out (3 x N point-major) += -1/2 grad(mu) of one solved body, cell-centred
+// Green-Gauss; d_nbr: neighbour across edge k of cell j at [k*N + j]
+int fp_launch_ugradmu(fp_ctx *ctx, const fp_body *b, const int *d_nbr, const int64_t *d_sh, int te, double *out);
 
 // ---- lift/drag adjoint (fp_adjoint.hip) ----------------------------------
 // Per-panel arrays are SoA over all M panels of the bodies (panel i = col0 + j

@@ -1,0 +1,107 @@
+// fp_post.hip -- surface post-processing kernels (gfx950, FP64).
+//
+// Surface velocity from the gradient of the doublet strength, cell-centred
+// Green-Gauss scheme (src/FLOWPanel_postprocess.jl:139-244, with the trailing-
+// edge treatment of RigidWakeBody at :245-497).  One lane per cell walks its
+// edges; the neighbour across edge k of cell j is nbr[k*N + j] (-1 if none),
+// found once on the host from the connectivity.  For triangles this is the
+// reference's formula; other polygons use all their edges.
+#include "fp_internal.h"
+
+namespace {
+
+template <int NVP>
+__device__ __forceinline__ void ugm_cell(const double *__restrict__ pv, const int *__restrict__ nvv, int64_t N,
+                                         const double *__restrict__ geo, const int *__restrict__ nbr,
+                                         const double *__restrict__ G, double sgn, int64_t j, double *acc)
+{
+    const int nv = nvv[j];
+    const double nx = geo[j], ny = geo[N + j], nz = geo[2 * N + j];
+    const double cx = geo[3 * N + j], cy = geo[4 * N + j], cz = geo[5 * N + j];
+    const double area = geo[6 * N + j];
+    acc[0] = acc[1] = acc[2] = 0;
+    for (int k = 0; k < NVP; ++k) {
+        if (k >= nv) break;
+        const int m = nbr[(int64_t)k * N + j];
+        if (m < 0) continue;
+        const int q = (k + 1) % nv;
+        const double ax = pv[(3 * k) * N + j], ay = pv[(3 * k + 1) * N + j], az = pv[(3 * k + 2) * N + j];
+        const double rx = pv[(3 * q) * N + j] - ax, ry = pv[(3 * q + 1) * N + j] - ay, rz = pv[(3 * q + 2) * N + j] - az;
+        double dx = ry * nz - rz * ny, dy = rz * nx - rx * nz, dz = rx * ny - ry * nx;
+        const double dm = sqrt(dx * dx + dy * dy + dz * dz);
+        dx /= dm; dy /= dm; dz /= dm;
+        const double mx = ax - cx, my = ay - cy, mz = az - cz;
+        const double qx = ax - geo[3 * N + m], qy = ay - geo[4 * N + m], qz = az - geo[5 * N + m];
+        const double c1 = my * rz - mz * ry, c2 = mz * rx - mx * rz, c3 = mx * ry - my * rx;
+        const double e1 = qy * rz - qz * ry, e2 = qz * rx - qx * rz, e3 = qx * ry - qy * rx;
+        const double dmain = sqrt(c1 * c1 + c2 * c2 + c3 * c3), dnear = sqrt(e1 * e1 + e2 * e2 + e3 * e3);
+        const double f = dnear / (dmain + dnear);
+        const double face = f * G[j] + (1.0 - f) * G[m];
+        const double mag = face * sqrt(rx * rx + ry * ry + rz * rz) / area;
+        acc[0] += 0.5 * sgn * dx * mag;
+        acc[1] += 0.5 * sgn * dy * mag;
+        acc[2] += 0.5 * sgn * dz * mag;
+    }
+}
+
+// out (3 x N point-major) -= sum of the cell's edge terms
+template <int NVP>
+__global__ __launch_bounds__(256) void k_ugradmu_cell(const double *__restrict__ pv, const int *__restrict__ nvv,
+                                                      int64_t N, const double *__restrict__ geo,
+                                                      const int *__restrict__ nbr, const double *__restrict__ G,
+                                                      double sgn, double *__restrict__ out)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    double acc[3];
+    ugm_cell<NVP>(pv, nvv, N, geo, nbr, G, sgn, j, acc);
+    out[3 * j] -= acc[0];
+    out[3 * j + 1] -= acc[1];
+    out[3 * j + 2] -= acc[2];
+}
+
+// trailing-edge panels: their edge terms added back, once per appearance in
+// the shedding list, in list order
+template <int NVP>
+__global__ void k_ugradmu_te(const double *__restrict__ pv, const int *__restrict__ nvv, int64_t N,
+                             const double *__restrict__ geo, const int *__restrict__ nbr, const double *__restrict__ G,
+                             double sgn, const int64_t *__restrict__ sh, int64_t ns, double *__restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int64_t e = 0; e < ns; ++e)
+        for (int side = 0; side < 2; ++side) {
+            const int64_t c = sh[6 * e + 3 * side];
+            if (c < 0) continue;
+            double acc[3];
+            ugm_cell<NVP>(pv, nvv, N, geo, nbr, G, sgn, c, acc);
+            out[3 * c] += acc[0];
+            out[3 * c + 1] += acc[1];
+            out[3 * c + 2] += acc[2];
+        }
+}
+
+}  // namespace
+
+int fp_launch_ugradmu(fp_ctx *ctx, const fp_body *b, const int *d_nbr, const int64_t *d_sh, int te, double *out)
+{
+    const int64_t N = b->N;
+    if (N == 0) return FP_OK;
+    const double sgn = b->cp_offset == 0 ? 1.0 : (b->cp_offset > 0 ? 1.0 : -1.0);
+    const dim3 grid((unsigned)((N + 255) / 256));
+    ProfScope ps(ctx, "ugradmu_cell");
+    if (b->nvp == 4) {
+        hipLaunchKernelGGL(k_ugradmu_cell<4>, grid, dim3(256), 0, ctx->stream, b->d_pv, b->d_nv, N, b->d_geo, d_nbr,
+                           b->d_strength, sgn, out);
+        if (te && b->ns > 0)
+            hipLaunchKernelGGL(k_ugradmu_te<4>, dim3(1), dim3(64), 0, ctx->stream, b->d_pv, b->d_nv, N, b->d_geo,
+                               d_nbr, b->d_strength, sgn, d_sh, b->ns, out);
+    } else {
+        hipLaunchKernelGGL(k_ugradmu_cell<8>, grid, dim3(256), 0, ctx->stream, b->d_pv, b->d_nv, N, b->d_geo, d_nbr,
+                           b->d_strength, sgn, out);
+        if (te && b->ns > 0)
+            hipLaunchKernelGGL(k_ugradmu_te<8>, dim3(1), dim3(64), 0, ctx->stream, b->d_pv, b->d_nv, N, b->d_geo,
+                               d_nbr, b->d_strength, sgn, d_sh, b->ns, out);
+    }
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}

@@ -3,6 +3,7 @@
 * ``Uind``         -- ``Uind!(body, targets, out)`` (src/FLOWPanel_abstractbody.jl:101-110)
 * ``phi``          -- ``phi!(body, targets, out)`` (src/FLOWPanel_abstractbody.jl:118-127)
 * ``calcfield_U``  -- src/FLOWPanel_postprocess.jl:28-110 (at the control points)
+* ``calcfield_Ugradmu_cell`` -- src/FLOWPanel_postprocess.jl:139-497 (Green-Gauss)
 * ``calcfield_Cp`` -- src/FLOWPanel_postprocess.jl:1059-1176
 * ``calcfield_F``, ``calcfield_LDS`` -- src/FLOWPanel_postprocess.jl:1199-1307, 1615-1689
 """
@@ -64,6 +65,34 @@ def calcfield_U(sourcebody, targetbody=None, fieldname="U", addfield=True):
             b.add_field(fieldname, out[:, col:col + b.ncells])
             col += b.ncells
     return out
+
+
+def calcfield_Ugradmu_cell(body, fieldname="Ugradmu", addfield=True, te_correction=True):
+    """-1/2 grad(mu) at the cells, Green-Gauss (src/FLOWPanel_postprocess.jl:139-497), 3 x N."""
+    comps = body.bodies
+    ctx = comps[0].ctx
+    out = np.zeros((3, body.ncells), order="F")
+    ctx.check(nat.lib().fp_calcfield_ugradmu(ctx.handle, nat.handles(comps), len(comps), int(bool(te_correction)),
+                                             nat.dptr(out)))
+    if addfield:
+        body.add_field(fieldname, out)
+        col = 0
+        for b in comps:
+            b.add_field(fieldname, out[:, col:col + b.ncells])
+            col += b.ncells
+    return out
+
+
+def addfields(body, name1, name2):
+    """field name2 += field name1 (src/FLOWPanel_abstractbody.jl add_field helpers)."""
+    f = np.asarray(body.get_field(name2)) + np.asarray(body.get_field(name1))
+    body.add_field(name2, f)
+    col = 0
+    for b in body.bodies:
+        if b is not body:
+            b.add_field(name2, f[..., col:col + b.ncells])
+        col += b.ncells
+    return f
 
 
 def calcfield_Cp(body, Uref, Us=None, correct_kuttacondition=True, clip=None, fieldname="Cp", addfield=True,

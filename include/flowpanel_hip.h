@@ -234,6 +234,14 @@ int fp_calcfield_U(fp_ctx *ctx, fp_body *const *bodies, int nbodies, double *out
 int fp_calcfield_cp(fp_ctx *ctx, const double *Us, int64_t n, double uref, const int64_t *shedding,
                     int64_t nsheddings, int index_base, int kutta_mode, double *out);
 
+/* calcfield_Ugradmu_cell!(out, body) (src/FLOWPanel_postprocess.jl:139-497):
+ * out (3 x N, += ) -= 1/2 grad(mu), the surface velocity due to the gradient of
+ * the doublet strength, by the cell-centred Green-Gauss scheme with inverse-
+ * distance face values; with te_correction the trailing-edge panels drop their
+ * neighbour terms (the RigidWakeBody variant).  Cells are neighbours when they
+ * share an edge.  The nodal and combined schemes (:622-1042) are not provided. */
+int fp_calcfield_ugradmu(fp_ctx *ctx, fp_body *const *bodies, int nbodies, int te_correction, double *out);
+
 /* ---- shape updates ------------------------------------------------------- */
 int64_t fp_body_nnodes(const fp_body *body);
 /* Move the mesh nodes (3 x nnodes, same connectivity) as the reference does by

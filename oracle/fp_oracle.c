@@ -624,3 +624,65 @@ void orc_calcfield_F(const double *areas, const double *normals, const double *c
         }
     }
 }
+
+/* Surface velocity from the gradient of the doublet strength, cell-centred
+ * Green-Gauss scheme (src/FLOWPanel_postprocess.jl:139-244; RigidWakeBody
+ * trailing-edge treatment :245-497).  For every edge k -> k+1 of cell c with a
+ * neighbour m across it (neighbor[c*maxnv + k], -1 if none):
+ *   r = p_{k+1} - p_k, d = (r x n_c)/|r x n_c|,
+ *   f = |(p_k - cp_m) x r| / (|(p_k - cp_c) x r| + |(p_k - cp_m) x r|),
+ *   out_c -= 0.5 sgn d (f G_c + (1-f) G_m) |r| / A_c,   sgn = sign(CPoffset) (1 if 0).
+ * With te != 0 every neighbour term of the shedding panels pi, pj is added
+ * back, once per appearance in the shedding list.  Polygons use all nv edges
+ * (the reference's grids are triangles, for which this is its formula). */
+static void ugm_edge(const double *nodes, const int64_t *cell_ptr, const int64_t *cell_idx, const int64_t *neighbor,
+                     int maxnv, const double *G, const double *normals, const double *cps, const double *areas,
+                     double sgn, int64_t c, int k, double *acc)
+{
+    const int64_t *p = cell_idx + cell_ptr[c];
+    const int nv = (int)(cell_ptr[c + 1] - cell_ptr[c]);
+    const int64_t m = neighbor[c * maxnv + k];
+    if (m < 0) return;
+    const double *a = nodes + 3 * p[k], *b = nodes + 3 * p[(k + 1) % nv];
+    const double *n = normals + 3 * c;
+    double r[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    double d[3] = {r[1] * n[2] - r[2] * n[1], r[2] * n[0] - r[0] * n[2], r[0] * n[1] - r[1] * n[0]};
+    double dm = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    double vm[3], vn[3], cm[3], cn[3];
+    for (int i = 0; i < 3; ++i) { vm[i] = a[i] - cps[3 * c + i]; vn[i] = a[i] - cps[3 * m + i]; }
+    cm[0] = vm[1] * r[2] - vm[2] * r[1]; cm[1] = vm[2] * r[0] - vm[0] * r[2]; cm[2] = vm[0] * r[1] - vm[1] * r[0];
+    cn[0] = vn[1] * r[2] - vn[2] * r[1]; cn[1] = vn[2] * r[0] - vn[0] * r[2]; cn[2] = vn[0] * r[1] - vn[1] * r[0];
+    double dmain = sqrt(cm[0] * cm[0] + cm[1] * cm[1] + cm[2] * cm[2]);
+    double dnear = sqrt(cn[0] * cn[0] + cn[1] * cn[1] + cn[2] * cn[2]);
+    double f = dnear / (dmain + dnear);
+    double face = f * G[c] + (1.0 - f) * G[m];
+    double mag = face * sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) / areas[c];
+    for (int i = 0; i < 3; ++i) acc[i] += 0.5 * sgn * d[i] / dm * mag;
+}
+
+void orc_ugradmu_cell(const double *nodes, const int64_t *cell_ptr, const int64_t *cell_idx, int64_t ncells,
+                      const int64_t *neighbor, int maxnv, const double *G, const double *normals, const double *cps,
+                      const double *areas, double cpoffset, const int64_t *shedding, int64_t ns, int te,
+                      double *out)
+{
+    const double sgn = cpoffset == 0 ? 1.0 : (cpoffset > 0 ? 1.0 : -1.0);
+    for (int64_t c = 0; c < ncells; ++c) {
+        const int nv = (int)(cell_ptr[c + 1] - cell_ptr[c]);
+        double acc[3] = {0, 0, 0};
+        for (int k = 0; k < nv; ++k)
+            ugm_edge(nodes, cell_ptr, cell_idx, neighbor, maxnv, G, normals, cps, areas, sgn, c, k, acc);
+        for (int i = 0; i < 3; ++i) out[3 * c + i] -= acc[i];
+    }
+    if (!te || !shedding) return;
+    for (int64_t e = 0; e < ns; ++e) {
+        for (int side = 0; side < 2; ++side) {
+            const int64_t c = shedding[6 * e + 3 * side];
+            if (c < 0) continue;
+            const int nv = (int)(cell_ptr[c + 1] - cell_ptr[c]);
+            double acc[3] = {0, 0, 0};
+            for (int k = 0; k < nv; ++k)
+                ugm_edge(nodes, cell_ptr, cell_idx, neighbor, maxnv, G, normals, cps, areas, sgn, c, k, acc);
+            for (int i = 0; i < 3; ++i) out[3 * c + i] += acc[i];
+        }
+    }
+}

@@ -71,6 +71,7 @@ def lib():
         L.orc_phi_source.argtypes = [_d, _i, _i, i64, _d, _d, i64, _d, dbl]
         L.orc_calcfield_cp.argtypes = [_d, i64, dbl, _i, i64, cint, _d]
         L.orc_calcfield_F.argtypes = [_d, _d, _d, i64, dbl, dbl, _i, i64, cint, _d]
+        L.orc_ugradmu_cell.argtypes = [_d, _i, _i, i64, _i, cint, _d, _d, _d, _d, dbl, _i, i64, cint, _d]
     return _LIB
 
 
@@ -339,3 +340,37 @@ def lds(F, Lhat, Dhat, Shat=None):
         Shat = np.cross(Lhat, Dhat)
     Ftot = np.asarray(F).sum(axis=1)
     return np.stack([Lhat * Ftot.dot(Lhat), Dhat * Ftot.dot(Dhat), Shat * Ftot.dot(Shat)], axis=1)
+
+
+def cell_neighbors(cell_ptr, cell_idx):
+    """neighbor[c, k] = the other cell holding edge (k, k+1) of cell c, -1 if none."""
+    cell_ptr, cell_idx = np.asarray(cell_ptr), np.asarray(cell_idx)
+    n = len(cell_ptr) - 1
+    maxnv = int(np.diff(cell_ptr).max())
+    owners = {}
+    for c in range(n):
+        p = cell_idx[cell_ptr[c]:cell_ptr[c + 1]]
+        for k in range(len(p)):
+            owners.setdefault(frozenset((int(p[k]), int(p[(k + 1) % len(p)]))), []).append(c)
+    nb = -np.ones((n, maxnv), dtype=np.int64)
+    for c in range(n):
+        p = cell_idx[cell_ptr[c]:cell_ptr[c + 1]]
+        for k in range(len(p)):
+            other = [o for o in owners[frozenset((int(p[k]), int(p[(k + 1) % len(p)])))] if o != c]
+            if other:
+                nb[c, k] = other[0]
+    return nb
+
+
+def ugradmu_cell(body, strength, shedding=None, te=True):
+    """Green-Gauss cell-centred -1/2 grad(mu) (src/FLOWPanel_postprocess.jl:139-497), 3 x n."""
+    normals, cps, areas = _body_geometry(body)
+    nb = cell_neighbors(body.cell_ptr, body.cell_idx)
+    out = np.zeros((3, body.ncells), order="F")
+    sh = _shed(shedding) if shedding is not None and np.asarray(shedding).size else None
+    ns = 0 if sh is None else len(sh) // 6
+    lib().orc_ugradmu_cell(_pd(_f64(body.nodes)), _pi(_i64(body.cell_ptr)), _pi(_i64(body.cell_idx)), body.ncells,
+                           _pi(np.ascontiguousarray(nb)), nb.shape[1],
+                           _pd(np.ascontiguousarray(strength, dtype=np.float64)), _pd(normals), _pd(cps), _pd(areas),
+                           float(body.CPoffset), _pi(sh), ns, int(bool(te and sh is not None)), _pd(out))
+    return out
