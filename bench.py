@@ -115,6 +115,37 @@ def vendor_lu(fp, body, N):
         return {"error": str(e)[:200]}
 
 
+def field_leg(fp, ctx, body, U, n=256):
+    """Velocity and Cp of the solved body on an n^3 off-body box grid (Uind!, calcfield_Cp formula)."""
+    X = body.grid.nodes
+    lo, hi = X.min(axis=1), X.max(axis=1)
+    pad = 0.25 * (hi - lo)
+    axes = [np.linspace(lo[d] - pad[d], hi[d] + pad[d], n) for d in range(3)]
+    g = np.meshgrid(*axes, indexing="ij")
+    targets = np.asfortranarray(np.stack([a.ravel() for a in g]))
+    del g
+    out = np.zeros_like(targets)
+    log("bench: field on a %d^3 grid (%d targets)" % (n, targets.shape[1]))
+    ctx.synchronize()
+    ctx.profile(True)
+    t = time.perf_counter()
+    fp.Uind(body, targets, out)
+    ctx.synchronize()
+    wall = (time.perf_counter() - t) * 1e3
+    launches, dev_ms = ctx.profile_get("field_velocity_f64")
+    ctx.profile(False)
+    Uinf = U[:, 0]
+    Vt = out + Uinf[:, None]
+    cp = 1.0 - (Vt * Vt).sum(axis=0) / Uinf.dot(Uinf)
+    pairs = float(targets.shape[1]) * body.ncells
+    flop = pairs * 180.0 + float(targets.shape[1]) * 2 * body.nsheddings * 244.0
+    return {"targets": int(targets.shape[1]), "grid": "%d^3 box around the body, 25%% margin" % n,
+            "device_ms": round(dev_ms, 1), "wall_ms_incl_transfers": round(wall, 1), "launches": launches,
+            "pairs": pairs, "flop": flop, "tflops": round(flop / (dev_ms * 1e-3) / 1e12, 2),
+            "fp64_frac": round(flop / (dev_ms * 1e-3) / (PEAK_FP64_TFLOPS * 1e12), 3),
+            "cp_min": float(cp.min()), "cp_max": float(cp.max())}
+
+
 def batch_legs(fp, ctx, reps=3):
     """Batched workloads: 64 freestreams as 64 RHS, and 16 uneven wings in one captured graph."""
     out = {}
@@ -308,6 +339,7 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg (prints its record)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched legs (64 freestreams, 16 wings)")
     ap.add_argument("--adjoint-fd", type=int, default=64, help="node coordinates checked by finite differences")
+    ap.add_argument("--field", type=int, default=256, help="off-body field grid points per axis (0: skip)")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched legs (prints their record)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -381,6 +413,7 @@ def main():
     cp64 = fp.calcfield_Cp(body, 1.0)
     cp_err = float(np.abs(cp_m - cp64).max())
     gam_err = float(np.abs(gam_mixed - gam64).max() / np.abs(gam64).max())
+    field = field_leg(fp, ctx, body, U, args.field) if args.field > 0 else None
 
     # roofline of the dominant kernel of the mixed step (FP32 GEMV of the Krylov matvec)
     launches, ms = prof["gemv_f32"]
@@ -420,7 +453,7 @@ def main():
         "fp64_lu_ms": round(t_fp64, 3), "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
         "cp_err_max": cp_err, "gamma_rel_err": gam_err,
         "mixed_stats": stats[-1], "fp64_stats": st64,
-        "adjoint_ms": adj["adjoint_total_ms"] if adj else None, "adjoint": adj, "batch": bat,
+        "adjoint_ms": adj["adjoint_total_ms"] if adj else None, "adjoint": adj, "batch": bat, "field": field,
         "kernels": kernels, "kernels_fp64": kernels_fp64, "vendor_fp64_getrf": vendor, "roofline": roof,
         "cpu_baseline": cpu,
     }
