@@ -17,19 +17,9 @@
 #include <algorithm>
 #include <climits>
 
-typedef double fp_d4 __attribute__((ext_vector_type(4)));
 typedef float fp_f4 __attribute__((ext_vector_type(4)));
 
 template <typename T> struct MfmaOp;
-template <> struct MfmaOp<double> {
-    typedef fp_d4 acc;
-    static __device__ __forceinline__ acc run(double a, double b, acc c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    // accumulator element r of lane l holds D(l/16 + 4r, l%16)
-    static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
-};
 template <> struct MfmaOp<float> {
     typedef fp_f4 acc;
     static __device__ __forceinline__ acc run(float a, float b, acc c)
@@ -43,10 +33,13 @@ template <> struct MfmaOp<float> {
 // ----------------------------------------------------------------------------
 // C[m x n] -= A[m x k] * B[k x n], all row-major.  128 x 128 tile per 256-thread
 // workgroup (2 x 2 waves of 64 x 64), K step 32 through LDS with register
-// prefetch of the next step, 16-byte global loads, v_mfma 16x16x4.
-// MFMA operand layout (16x16x4): lane l supplies A(l%16, l/16) and B(l/16, l%16);
-// the accumulator rows differ between f32 and f64 (MfmaOp<T>::row, measured on
-// gfx950 with an in-bounds lane-layout probe).
+// prefetch of the next step, 16-byte global loads.  FP32 runs v_mfma_f32_16x16x4;
+// FP64 runs v_mfma_f64_4x4x4_4b (k_gemm_sub_f64 below): on gfx950 the FP64
+// 16x16x4 form issues at half the FP64 matrix rate (measured 36 of 78.6 TFLOP/s
+// with every SIMD busy, register-only), the four-block 4x4x4 form at 67-73.
+// FP32 operand layout (16x16x4): lane l supplies A(l%16, l/16) and B(l/16, l%16);
+// accumulator rows as in MfmaOp<float>::row (measured on gfx950 with an
+// in-bounds lane-layout probe).
 // Tile order: workgroup b runs on XCD b % 8; every XCD is given a contiguous
 // range of a grouped order (8 tile rows walked together), so the A rows of a
 // group stay in that XCD's L2 while it sweeps across the B columns.
@@ -149,6 +142,103 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t 
         }
 }
 
+// FP64 tile kernel on v_mfma_f64_4x4x4_4b.  Operand layout of that instruction
+// (in-bounds lane probe on gfx950): lane L supplies A(4*blk + i, k) and B(k, j)
+// of block blk = (L>>2)&3 with k = L>>4, i = j = L&3, and receives
+// D(4*blk + (L>>4), L&3).  Every block is given the same B columns, so one
+// instruction is a 16 x 4 x 4 product; a wave's 64 x 64 tile is 4 row fragments
+// x 16 column fragments, 64 independent accumulators per lane.
+// LDS placement: slot a*16 + p of the wave's A rows holds tile row p*4 + a and
+// slot b*4 + j of its B columns holds tile column j*16 + b.  Fragment reads are
+// then 16 consecutive doubles (A) and 4 consecutive doubles per k row (B) --
+// no bank conflicts -- and lane L owns the 4 x 16 block of C at rows P*4..P*4+3,
+// P = 4*((L>>2)&3) + (L>>4), columns (L&3)*16..+15 of its wave tile, written
+// back with 16-byte accesses.
+__global__ __launch_bounds__(256) void k_gemm_sub_f64(int64_t m, int64_t n, int64_t k, const double *__restrict__ A,
+                                                      int64_t lda, const double *__restrict__ B, int64_t ldb,
+                                                      double *__restrict__ C, int64_t ldc, int vec)
+{
+    __shared__ double As[GB_K][GB_M + GB_PAD];
+    __shared__ double Bs[GB_K][GB_N + GB_PAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int64_t tiles_m = (m + GB_M - 1) / GB_M, tiles_n = (n + GB_N - 1) / GB_N;
+    const int64_t ntiles = tiles_m * tiles_n;
+    const int64_t per = (ntiles + GB_XCD - 1) / GB_XCD;
+    const int64_t t = (int64_t)(blockIdx.x % GB_XCD) * per + blockIdx.x / GB_XCD;
+    if (t >= ntiles) return;
+    const int64_t in_group = (int64_t)GB_GROUP * tiles_n;
+    const int64_t first_m = (t / in_group) * GB_GROUP;
+    const int64_t gsize = min((int64_t)GB_GROUP, tiles_m - first_m);
+    const int64_t tm = first_m + (t % in_group) % gsize, tn = (t % in_group) / gsize;
+    const int64_t m0 = tm * GB_M, n0 = tn * GB_N;
+
+    double acc[4][16];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 16; ++b) acc[a][b] = 0.0;
+
+    // A: thread loads 16 k of tile row arow into LDS slot aslot; B: 16 columns of k row bk
+    const int aslot = tid >> 1, ak = (tid & 1) * 16;
+    const int arow = (aslot & 64) + (aslot & 15) * 4 + ((aslot & 63) >> 4);
+    const int bk = tid >> 3, bc = (tid & 7) * 16, bslot = (tid & 4) * 16 + (tid & 3);
+    double ra[16], rb[16];
+    auto load = [&](int64_t k0) {
+        const int64_t gr = m0 + arow, gk = k0 + ak;
+        const int av = gr < m ? (int)max((int64_t)0, min((int64_t)16, k - gk)) : 0;
+        ld16<double>(A + gr * lda + gk, av == 16, vec, av, ra);
+        const int64_t gkb = k0 + bk, gc = n0 + bc;
+        const int bv = gkb < k ? (int)max((int64_t)0, min((int64_t)16, n - gc)) : 0;
+        ld16<double>(B + gkb * ldb + gc, bv == 16, vec, bv, rb);
+    };
+    load(0);
+    for (int64_t k0 = 0; k0 < k; k0 += GB_K) {
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 16; ++e) As[ak + e][aslot] = ra[e];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) Bs[bk][bslot + e * 4] = rb[e];
+        __syncthreads();
+        if (k0 + GB_K < k) load(k0 + GB_K);
+#pragma unroll
+        for (int ks = 0; ks < GB_K; ks += 4) {
+            double fa[4], fb[16];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) fa[a] = As[ks + (lane >> 4)][wm * 64 + a * 16 + (lane & 15)];
+#pragma unroll
+            for (int b = 0; b < 16; ++b) fb[b] = Bs[ks + (lane >> 4)][wn * 64 + b * 4 + (lane & 3)];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 16; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_4x4x4f64(fa[a], fb[b], acc[a][b], 0, 0, 0);
+        }
+    }
+    const int P = 4 * ((lane >> 2) & 3) + (lane >> 4);
+    const int64_t col0 = n0 + wn * 64 + (lane & 3) * 16;
+    const bool cvec = vec && (ldc & 1) == 0 && ((uintptr_t)C & 15) == 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int64_t row = m0 + wm * 64 + P * 4 + a;
+        if (row >= m) continue;
+        double *c = C + row * ldc + col0;
+        if (cvec && col0 + 16 <= n) {
+#pragma unroll
+            for (int b = 0; b < 16; b += 2) {
+                double2 v = *reinterpret_cast<double2 *>(c + b);
+                v.x -= acc[a][b];
+                v.y -= acc[a][b + 1];
+                *reinterpret_cast<double2 *>(c + b) = v;
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < 16; ++b)
+                if (col0 + b < n) c[b] -= acc[a][b];
+        }
+    }
+}
+
 template <typename T>
 static int gemm_sub(fp_ctx *ctx, int64_t m, int64_t n, int64_t k, const T *A, int64_t lda, const T *B, int64_t ldb,
                     T *C, int64_t ldc)
@@ -160,8 +250,12 @@ static int gemm_sub(fp_ctx *ctx, int64_t m, int64_t n, int64_t k, const T *A, in
     const int vec = (((uintptr_t)A | (uintptr_t)B) & 15) == 0 && (lda * sizeof(T)) % 16 == 0 &&
                     (ldb * sizeof(T)) % 16 == 0;
     ProfScope ps(ctx, sizeof(T) == 8 ? "gemm_f64" : "gemm_f32");
-    hipLaunchKernelGGL(k_gemm_sub<T>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, m, n, k, A, lda, B, ldb, C,
-                       ldc, vec);
+    if constexpr (sizeof(T) == 8)
+        hipLaunchKernelGGL(k_gemm_sub_f64, dim3((unsigned)grid), dim3(256), 0, ctx->stream, m, n, k,
+                           (const double *)A, lda, (const double *)B, ldb, (double *)C, ldc, vec);
+    else
+        hipLaunchKernelGGL(k_gemm_sub<T>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, m, n, k, A, lda, B, ldb,
+                           C, ldc, vec);
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }
@@ -324,124 +418,147 @@ constexpr int CH = 256;     // rows per candidate chunk
 constexpr int MR = 512;     // rows per merge (16 candidate sets)
 constexpr int MERGE_F = MR / PW;
 
-// Partial pivoting of a (rows x cols) tile in LDS (cols <= PW).  The pivot
-// of each column is found with one 64-bit LDS atomicMax per row on the packed
-// key (float(|v|) bits, ~row): largest magnitude first, the lower tile row on
-// ties (a NaN counts as zero).  Rows are permuted in place together with key[]
-// and tag[].  *sing receives 1 + the first column whose pivot was exactly zero.
-__device__ __forceinline__ unsigned long long pivot_key(double v, int r)
+// Partial pivoting of a tile of up to NT rows x cols (cols <= PW) with one row
+// per thread held in registers (v[], plus its key and tag).  The pivot of each
+// column is the largest packed key (float(|v|) bits, ~row): largest magnitude
+// first, the lower tile row on ties (a NaN counts as zero).  Each wave finds its
+// largest float(|v|) with the DPP wave reduction and its lowest row holding it
+// with a ballot; one pass over the per-wave keys picks the pivot.  The pivot
+// row and the row it displaces are exchanged through LDS; the elimination
+// itself touches registers only.  Two barriers per column.  *sing receives
+// 1 + the first column whose pivot was exactly zero.
+template <typename T, int NT>
+__device__ __forceinline__ void gepp_rows(T (&v)[PW], int &key, int &tag, int rows, int cols, int *sing)
 {
-    float a = (float)fabs(v);
-    if (!(a == a)) a = 0.f;
-    return ((unsigned long long)__float_as_uint(a) << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)r);
-}
-
-template <typename T>
-__device__ void tile_gepp(T (*tile)[PW + 1], int *key, int *tag, int rows, int cols, int *sing)
-{
-    __shared__ unsigned long long amax[2];
-    const int tid = threadIdx.x;
-    if (tid == 0) { amax[0] = 0ull; amax[1] = 0ull; }
-    __syncthreads();
+    __shared__ unsigned long long wmax[2][NT / 64];   // per-wave maxima, alternating by column parity
+    __shared__ T xrow[2][PW];                          // [0] pivot row, [1] the row it displaces
+    __shared__ int xkt[2][2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nc = min(rows, cols);
-    for (int c = 0; c < nc; ++c) {
-        const int slot = c & 1;
-        for (int r = c + tid; r < rows; r += blockDim.x) atomicMax(&amax[slot], pivot_key((double)tile[r][c], r));
-        __syncthreads();
-        const unsigned long long k = amax[slot];
-        int p = (int)(0xFFFFFFFFu - (unsigned)(k & 0xFFFFFFFFull));
-        if (k == 0ull || p < c || p >= rows) p = c;
-        if (tid == 0) amax[slot ^ 1] = 0ull;
-        if (p != c) {
-            for (int e = tid; e < cols; e += blockDim.x) {
-                const T t = tile[c][e]; tile[c][e] = tile[p][e]; tile[p][e] = t;
+#pragma unroll
+    for (int c = 0; c < PW; ++c) {
+        if (c < nc) {
+            const bool valid = tid >= c && tid < rows;
+            unsigned a = 0u;
+            if (valid) {
+                float f = (float)fabs((double)v[c]);
+                if (!(f == f)) f = 0.f;
+                a = __float_as_uint(f);
             }
-            if (tid == 0) {
-                int t = key[c]; key[c] = key[p]; key[p] = t;
-                t = tag[c]; tag[c] = tag[p]; tag[p] = t;
+            const unsigned amax = __ockl_wfred_max_u32(a);
+            const unsigned long long ball = __ballot(valid && a == amax);
+            if (lane == 0) {
+                unsigned long long kk = 0ull;
+                if (ball) {
+                    const unsigned row = (unsigned)(wave * 64 + __ffsll((long long)ball) - 1);
+                    kk = ((unsigned long long)amax << 32) | (unsigned long long)(0xFFFFFFFFu - row);
+                }
+                wmax[c & 1][wave] = kk;
+            }
+            __syncthreads();
+            unsigned long long best = 0ull;
+#pragma unroll
+            for (int w = 0; w < NT / 64; ++w) best = wmax[c & 1][w] > best ? wmax[c & 1][w] : best;
+            int p = (int)(0xFFFFFFFFu - (unsigned)(best & 0xFFFFFFFFull));
+            if (best == 0ull || p < c || p >= rows) p = c;
+            if (tid == p) {
+#pragma unroll
+                for (int e = 0; e < PW; ++e) xrow[0][e] = v[e];
+                xkt[0][0] = key; xkt[0][1] = tag;
+            }
+            if (tid == c && p != c) {
+#pragma unroll
+                for (int e = 0; e < PW; ++e) xrow[1][e] = v[e];
+                xkt[1][0] = key; xkt[1][1] = tag;
+            }
+            __syncthreads();
+            if (p != c && (tid == c || tid == p)) {
+                const int from = tid == c ? 0 : 1;
+#pragma unroll
+                for (int e = 0; e < PW; ++e) v[e] = xrow[from][e];
+                key = xkt[from][0]; tag = xkt[from][1];
+            }
+            const T piv = xrow[0][c];
+            if (piv == T(0)) {
+                if (tid == 0 && *sing == 0) *sing = c + 1;
+            } else if (tid > c && tid < rows) {
+                const T l = v[c] * (T(1) / piv);
+                v[c] = l;
+#pragma unroll
+                for (int e = c + 1; e < PW; ++e)
+                    if (e < cols) v[e] -= l * xrow[0][e];
             }
         }
-        __syncthreads();
-        const T piv = tile[c][c];
-        if (piv == T(0)) {
-            if (tid == 0 && *sing == 0) *sing = c + 1;
-        } else {
-            const T rp = T(1) / piv;
-            const int w = cols - c - 1;
-            // (rows-c-1) x w updates plus the multipliers, spread over all threads
-            for (int r = c + 1 + tid; r < rows; r += blockDim.x) {
-                const T l = tile[r][c] * rp;
-                tile[r][c] = l;
-                for (int e = 0; e < w; ++e) tile[r][c + 1 + e] -= l * tile[c][c + 1 + e];
-            }
-        }
-        __syncthreads();
     }
 }
 
 // Level 0: each workgroup proposes PW candidate rows out of CH rows of the
-// sub-panel A[s:n, s:s+pw].  Output: original values + global row index.
+// sub-panel A[s:n, s:s+pw] (thread t holds tile row t).  Output: original
+// values + global row index.
 template <typename T>
-__global__ __launch_bounds__(256) void k_tslu_local(const T *__restrict__ A, int64_t ld, int n, int s, int pw,
-                                                    T *__restrict__ cval, int *__restrict__ cidx,
-                                                    int *__restrict__ info)
+__global__ __launch_bounds__(CH) void k_tslu_local(const T *__restrict__ A, int64_t ld, int n, int s, int pw,
+                                                   T *__restrict__ cval, int *__restrict__ cidx,
+                                                   int *__restrict__ info)
 {
-    __shared__ T tile[CH][PW + 1];
-    __shared__ int key[CH], tag[CH];
     __shared__ int sing;
     const int g = blockIdx.x, tid = threadIdx.x;
     const int r0 = s + g * CH;
     const int rows = min(CH, n - r0);
     if (tid == 0) sing = 0;
-    for (int e = tid; e < CH * PW; e += blockDim.x) {
-        const int r = e / PW, c = e % PW;
-        tile[r][c] = (r < rows && c < pw) ? A[(int64_t)(r0 + r) * ld + s + c] : T(0);
-    }
-    for (int r = tid; r < CH; r += blockDim.x) { key[r] = r < rows ? r0 + r : INT_MAX; tag[r] = r; }
+    const bool has = tid < rows;
+    T v[PW];
+#pragma unroll
+    for (int c = 0; c < PW; ++c) v[c] = (has && c < pw) ? A[(int64_t)(r0 + tid) * ld + s + c] : T(0);
+    int key = has ? r0 + tid : INT_MAX, tag = tid;
     __syncthreads();
-    tile_gepp<T>(tile, key, tag, rows, pw, &sing);
-    __syncthreads();
+    gepp_rows<T, CH>(v, key, tag, rows, pw, &sing);
     // candidates: the first min(rows, pw) rows, original values re-read
-    for (int e = tid; e < PW * PW; e += blockDim.x) {
-        const int r = e / PW, c = e % PW;
-        T v = T(0);
-        if (r < rows && r < pw && c < pw) v = A[(int64_t)(r0 + tag[r]) * ld + s + c];
-        cval[((int64_t)g * PW + r) * PW + c] = v;
+    if (tid < PW) {
+        const bool ok = tid < rows && tid < pw;
+        T o[PW];
+#pragma unroll
+        for (int c = 0; c < PW; ++c) o[c] = (ok && c < pw) ? A[(int64_t)(r0 + tag) * ld + s + c] : T(0);
+#pragma unroll
+        for (int c = 0; c < PW; ++c) cval[((int64_t)g * PW + tid) * PW + c] = o[c];
+        cidx[g * PW + tid] = ok ? key : INT_MAX;
     }
-    for (int r = tid; r < PW; r += blockDim.x) cidx[g * PW + r] = (r < rows && r < pw) ? key[r] : INT_MAX;
     (void)info;
 }
 
-// Merge MERGE_F candidate sets per workgroup.  When `final_out` is set (one
-// workgroup left) it also writes the factored pw x pw block and the winners.
+// Merge MERGE_F candidate sets per workgroup (thread t holds candidate row t).
+// When `final_lu` is set (one workgroup left) it also writes the factored
+// pw x pw block and the winners.
 template <typename T>
-__global__ __launch_bounds__(512) void k_tslu_merge(int nsets, int pw, const T *__restrict__ ival,
-                                                    const int *__restrict__ iidx, T *__restrict__ oval,
-                                                    int *__restrict__ oidx, T *__restrict__ final_lu,
-                                                    int *__restrict__ info, int s)
+__global__ __launch_bounds__(MR) void k_tslu_merge(int nsets, int pw, const T *__restrict__ ival,
+                                                   const int *__restrict__ iidx, T *__restrict__ oval,
+                                                   int *__restrict__ oidx, T *__restrict__ final_lu,
+                                                   int *__restrict__ info, int s)
 {
-    __shared__ T tile[MR][PW + 1];
-    __shared__ int key[MR], tag[MR];
     __shared__ int sing;
     const int g = blockIdx.x, tid = threadIdx.x;
     const int set0 = g * MERGE_F;
     const int nset = min(MERGE_F, nsets - set0);
     const int rows = nset * PW;
     if (tid == 0) sing = 0;
-    for (int e = tid; e < MR * PW; e += blockDim.x) {
-        const int r = e / PW, c = e % PW;
-        tile[r][c] = r < rows ? ival[((int64_t)set0 * PW + r) * PW + c] : T(0);
-    }
-    for (int r = tid; r < MR; r += blockDim.x) { key[r] = r < rows ? iidx[set0 * PW + r] : INT_MAX; tag[r] = r; }
+    const bool has = tid < rows;
+    T v[PW];
+#pragma unroll
+    for (int c = 0; c < PW; ++c) v[c] = has ? ival[((int64_t)set0 * PW + tid) * PW + c] : T(0);
+    int key = has ? iidx[set0 * PW + tid] : INT_MAX, tag = tid;
     __syncthreads();
-    tile_gepp<T>(tile, key, tag, rows, pw, &sing);
-    __syncthreads();
-    for (int e = tid; e < PW * PW; e += blockDim.x) {
-        const int r = e / PW, c = e % PW;
-        oval[((int64_t)g * PW + r) * PW + c] = ival[((int64_t)set0 * PW + tag[r]) * PW + c];
-        if (final_lu) final_lu[r * PW + c] = tile[r][c];
+    gepp_rows<T, MR>(v, key, tag, rows, pw, &sing);
+    if (tid < PW) {
+        T o[PW];
+#pragma unroll
+        for (int c = 0; c < PW; ++c) o[c] = ival[((int64_t)set0 * PW + tag) * PW + c];
+#pragma unroll
+        for (int c = 0; c < PW; ++c) {
+            oval[((int64_t)g * PW + tid) * PW + c] = o[c];
+            if (final_lu) final_lu[tid * PW + c] = v[c];
+        }
+        oidx[g * PW + tid] = key;
     }
-    for (int r = tid; r < PW; r += blockDim.x) oidx[g * PW + r] = key[r];
+    __syncthreads();
     if (final_lu && tid == 0 && sing && *info == 0) *info = s + sing;
 }
 
@@ -504,7 +621,7 @@ __global__ void k_iota(int *p, int64_t n)
 // Apply a row map (A[dst] <- A[src] for every pair) to every column: one lane
 // per column, all loads issued before the stores.
 template <typename T>
-__global__ void k_rowmap(T *__restrict__ A, int64_t ld, int64_t ncols, const int *__restrict__ map)
+__global__ __launch_bounds__(256) void k_rowmap(T *__restrict__ A, int64_t ld, int64_t ncols, const int *__restrict__ map)
 {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int cnt = map[0];
@@ -520,8 +637,8 @@ __global__ void k_rowmap(T *__restrict__ A, int64_t ld, int64_t ncols, const int
 
 // Rows [s, s+pw) of the sub-panel <- factored block; rows below <- A21 U11^-1.
 template <typename T>
-__global__ __launch_bounds__(256) void k_tslu_l21(T *__restrict__ A, int64_t ld, int n, int s, int pw,
-                                                  const T *__restrict__ lu)
+__global__ __launch_bounds__(64) void k_tslu_l21(T *__restrict__ A, int64_t ld, int n, int s, int pw,
+                                                 const T *__restrict__ lu)
 {
     __shared__ T U[PW][PW + 1];
     for (int e = threadIdx.x; e < PW * PW; e += blockDim.x) U[e / PW][e % PW] = lu[e];
@@ -558,12 +675,78 @@ __global__ void k_fill_i32(int *p, int64_t n, int v)
     if (i < n) p[i] = v;
 }
 
+// Outer-block triangular update U12 := L11^-1 A12 (nb <= 256 rows) as one MFMA
+// GEMM: M = I - L11^-1 (strictly lower, written by k_trtri_llu), W = a copy of
+// A12, then A12 -= M W.  Inverting the unit lower L11 explicitly is the
+// inverted-diagonal-block trsm used by GPU LAPACK codes; with pivoted
+// multipliers |l| <= 1 its entries stay modest.
+constexpr int TRI_NB = 256;
+
+// Column j = blockIdx.x of L^-1 by forward substitution: thread r owns x_r and
+// keeps 32 entries of its own row of L in registers at a time; x_q is published
+// in LDS the step it becomes final.  Computed in FP64 for both storage types.
+template <typename T>
+__global__ __launch_bounds__(TRI_NB) void k_trtri_llu(int nb, const T *__restrict__ L, int64_t ldl,
+                                                      T *__restrict__ M, int64_t ldm)
+{
+    __shared__ double xs[TRI_NB];
+    const int j = blockIdx.x, r = threadIdx.x;
+    double x = r == j ? 1.0 : 0.0;
+    for (int q0 = j & ~31; q0 < nb; q0 += 32) {
+        double lr[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) {
+            const int q = q0 + u;
+            lr[u] = (r < nb && q < r && q >= j) ? (double)L[(int64_t)r * ldl + q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 32; ++u) {
+            const int q = q0 + u;
+            if (q >= j && q < nb) {
+                if (r == q) xs[q] = x;
+                __syncthreads();
+                if (r > q) x -= lr[u] * xs[q];
+            }
+        }
+    }
+    if (r < nb) M[(int64_t)r * ldm + j] = r > j ? (T)(-x) : T(0);
+}
+
+template <typename T>
+__global__ void k_copy_rows(const T *__restrict__ src, int64_t lds, T *__restrict__ dst, int64_t ldd, int rows,
+                            int64_t cols)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (c < cols && r < rows) dst[(int64_t)r * ldd + c] = src[(int64_t)r * lds + c];
+}
+
+static int64_t tri_ldw(int64_t n) { return (n + 3) & ~(int64_t)3; }
+
 size_t fp_getrf_work_bytes(int64_t n, int elem_bytes)
 {
     const int64_t g = (n + CH - 1) / CH;
-    // two candidate buffers (values + indices) + final block
-    return (size_t)(2 * g * PW * PW) * elem_bytes + (size_t)(2 * g * PW) * sizeof(int) + PW * PW * elem_bytes +
-           (4 * PW + 16) * sizeof(int) + 256;
+    // two candidate buffers (values + indices) + final block + row map, then
+    // the outer-block inverse M (256 x 256) and the A12 copy W (256 x ldw)
+    const size_t tslu = (size_t)(2 * g * PW * PW) * elem_bytes + (size_t)(2 * g * PW) * sizeof(int) +
+                        PW * PW * elem_bytes + (4 * PW + 16) * sizeof(int);
+    const size_t head = (tslu + 255) & ~(size_t)255;
+    return head + (size_t)TRI_NB * TRI_NB * elem_bytes + (size_t)TRI_NB * tri_ldw(n) * elem_bytes + 256;
+}
+
+template <typename T>
+static int trsm_outer(fp_ctx *ctx, int nb, int64_t ncols, const T *L, int64_t ld, T *X, T *M, T *W, int64_t ldw)
+{
+    if (ncols <= 0 || nb <= 0) return FP_OK;
+    {
+        ProfScope ps(ctx, "trsm");
+        hipLaunchKernelGGL(k_trtri_llu<T>, dim3((unsigned)nb), dim3(TRI_NB), 0, ctx->stream, nb, L, ld, M,
+                           (int64_t)TRI_NB);
+        hipLaunchKernelGGL(k_copy_rows<T>, dim3((unsigned)((ncols + 255) / 256), (unsigned)nb), dim3(256), 0,
+                           ctx->stream, (const T *)X, ld, W, ldw, nb, ncols);
+        FP_HIP(ctx, hipGetLastError());
+    }
+    return gemm_sub<T>(ctx, nb, ncols, nb, M, (int64_t)TRI_NB, W, ldw, X, ld);
 }
 
 // Factors P A = L U in place; perm[i] is the original row now at row i.
@@ -581,6 +764,10 @@ int fp_getrf(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *perm, int *info_dev,
     int *ci1 = (int *)w; w += gmax * PW * sizeof(int);
     T *flu = (T *)w; w += PW * PW * sizeof(T);
     int *rmap = (int *)w;
+    const size_t tslu = (size_t)(w - (char *)work) + (4 * PW + 16) * sizeof(int);
+    T *tri_m = (T *)((char *)work + ((tslu + 255) & ~(size_t)255));
+    T *tri_w = tri_m + TRI_NB * TRI_NB;
+    const int64_t ldw = tri_ldw(n);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, info_dev, (int64_t)1, 0);
     hipLaunchKernelGGL(k_iota, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, perm, n);
     const int NB = 256;
@@ -611,7 +798,7 @@ int fp_getrf(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *perm, int *info_dev,
                                rmap, perm);
             hipLaunchKernelGGL(k_rowmap<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, A, ld, n,
                                (const int *)rmap);
-            hipLaunchKernelGGL(k_tslu_l21<T>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, ctx->stream, A, ld,
+            hipLaunchKernelGGL(k_tslu_l21<T>, dim3((unsigned)((m + 63) / 64)), dim3(64), 0, ctx->stream, A, ld,
                                (int)n, (int)s, pw, (const T *)flu);
             FP_HIP(ctx, hipGetLastError());
             }
@@ -627,7 +814,7 @@ int fp_getrf(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *perm, int *info_dev,
         }
         const int64_t right = n - k - nb;
         if (right > 0) {
-            int st = trsm_llu<T>(ctx, nb, right, A + k * ld + k, ld, A + k * ld + k + nb, ld);
+            int st = trsm_outer<T>(ctx, nb, right, A + k * ld + k, ld, A + k * ld + k + nb, tri_m, tri_w, ldw);
             if (st) return st;
             st = gemm_sub<T>(ctx, right, right, nb, A + (k + nb) * ld + k, ld, A + k * ld + k + nb, ld,
                              A + (k + nb) * ld + k + nb, ld);
@@ -879,9 +1066,17 @@ __global__ __launch_bounds__(256) void k_gemv_multi(const T *__restrict__ A, int
             }
         }
     }
-    for (int64_t c = n4 + lane; c < n; c += 64)
-        for (int r = 0; r < nr; ++r)
-            for (int q = 0; q < nv; ++q) acc[r][q] += (double)A[(r0 + r) * ld + c] * X[(int64_t)(c0 + q) * ldx + c];
+    for (int64_t c = n4 + lane; c < n; c += 64) {
+#pragma unroll
+        for (int r = 0; r < GEMVM_R; ++r) {
+            if (r < nr) {
+                const double a = (double)A[(r0 + r) * ld + c];
+#pragma unroll
+                for (int q = 0; q < GEMVM_NV; ++q)
+                    if (q < nv) acc[r][q] += a * X[(int64_t)(c0 + q) * ldx + c];
+            }
+        }
+    }
 #pragma unroll
     for (int r = 0; r < GEMVM_R; ++r)
 #pragma unroll
