@@ -828,71 +828,194 @@ template int fp_getrf<float>(fp_ctx *, float *, int64_t, int64_t, int *, int *, 
 
 // ----------------------------------------------------------------------------
 // Triangular solves with the LU factors, FP64 right-hand sides (n x k,
-// column-major with ldx, i.e. each RHS contiguous).  Blocked by TB rows: one
-// workgroup solves the TB x TB diagonal block (one wave walks the columns,
-// the block in LDS), then every row below/above is updated by a GEMV.
+// column-major with ldx, i.e. each RHS contiguous).  Blocked by TB rows with
+// explicit inverses of the TB x TB diagonal blocks (k_trtri_diag, once per
+// call).  One launch per block step: every workgroup subtracts the solved
+// block's contribution from the TB rows it owns, and the workgroup owning the
+// next block -- final after that subtraction -- solves it with one small GEMV
+// (y = D^-1 x) for the next launch.  No grid-wide synchronisation; the running
+// right-hand side and the solution live in separate arrays.
 constexpr int TB = 128;
+constexpr int TRSV_ROWS = TB;   // rows of the update per workgroup: workgroup 0 owns the next block
 
+// Inverse of diagonal block bi = blockIdx.y, column j = blockIdx.x: thread r
+// owns x_r, 32 entries of its own row are staged in registers at a time.
+// LOWER: unit lower L (the strictly lower part of LU).  Upper: U incl. diagonal.
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void k_trsv_diag(const T *__restrict__ LU, int64_t ld, int64_t b0, int bs,
-                                                   double *__restrict__ x, int64_t ldx)
+__global__ __launch_bounds__(TB) void k_trtri_diag(const T *__restrict__ LU, int64_t ld, int64_t n,
+                                                   double *__restrict__ D)
 {
-    __shared__ double Ls[TB][TB + 1];
-    double *xv = x + (int64_t)blockIdx.x * ldx + b0;
-    for (int e = threadIdx.x; e < bs * bs; e += blockDim.x) {
-        const int r = e / bs, c = e % bs;
-        Ls[r][c] = (double)LU[(b0 + r) * ld + b0 + c];
-    }
-    __syncthreads();
-    if (threadIdx.x >= 64) return;
-    const int lane = threadIdx.x;
-    double y0 = lane < bs ? xv[lane] : 0.0, y1 = lane + 64 < bs ? xv[lane + 64] : 0.0;
+    __shared__ double xs[TB];
+    const int j = blockIdx.x, r = threadIdx.x;
+    const int64_t b0 = (int64_t)blockIdx.y * TB;
+    const int bs = (int)min((int64_t)TB, n - b0);
+    if (j >= bs) return;   // uniform per workgroup
+    const T *row = LU + (b0 + r) * ld + b0;
+    double x = r == j ? 1.0 : 0.0;
     if (LOWER) {
-        for (int q = 0; q < bs; ++q) {
-            const double yq = __shfl(q < 64 ? y0 : y1, q & 63);
-            if (lane > q && lane < bs) y0 -= Ls[lane][q] * yq;
-            if (lane + 64 > q && lane + 64 < bs) y1 -= Ls[lane + 64][q] * yq;
+        for (int q0 = j & ~31; q0 < bs; q0 += 32) {
+            double lr[32];
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                const int q = q0 + u;
+                lr[u] = (r < bs && q < r && q >= j) ? (double)row[q] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                const int q = q0 + u;
+                if (q >= j && q < bs) {
+                    if (r == q) xs[q] = x;
+                    __syncthreads();
+                    if (r > q) x -= lr[u] * xs[q];
+                }
+            }
         }
     } else {
-        for (int q = bs - 1; q >= 0; --q) {
-            // finalise y_q, then eliminate it from the rows above
-            if (lane == (q & 63)) {
-                if (q < 64) y0 /= Ls[q][q];
-                else y1 /= Ls[q][q];
+        // x = U^-1 e_j: rows q = j, j-1, ..., 0 become final in turn
+        for (int q1 = j | 31; q1 >= 0; q1 -= 32) {
+            double ur[32];
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                const int q = q1 - u;
+                ur[u] = (r < bs && q >= r && q <= j && q >= 0) ? (double)row[q] : 0.0;
             }
-            const double yq = __shfl(q < 64 ? y0 : y1, q & 63);
-            if (lane < q) y0 -= Ls[lane][q] * yq;
-            if (lane + 64 < q) y1 -= Ls[lane + 64][q] * yq;
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                const int q = q1 - u;
+                if (q <= j && q >= 0) {
+                    if (r == q) {
+                        x /= ur[u];
+                        xs[q] = x;
+                    }
+                    __syncthreads();
+                    if (r < q) x -= ur[u] * xs[q];
+                }
+            }
         }
     }
-    if (lane < bs) xv[lane] = y0;
-    if (lane + 64 < bs) xv[lane + 64] = y1;
+    if (r < bs) D[((int64_t)blockIdx.y * TB + r) * TB + j] = x;   // row-major block
 }
 
-// x[rows] -= LU[rows, b0:b0+bs] * x[b0:b0+bs]; one wave per row.
-template <typename T>
-__global__ __launch_bounds__(256) void k_trsv_update(const T *__restrict__ LU, int64_t ld, int64_t r0, int64_t r1,
-                                                     int64_t b0, int bs, double *__restrict__ x, int64_t ldx)
+// Row t of D in[0:bs] for a row-major TB x TB block inverse D; 256 threads,
+// threads t and t+128 each take half of the columns.  dv[] holds this thread's
+// half of row t, loaded ahead by load_block_row as 16-byte loads from one base
+// (entries at or beyond bs are never written by k_trtri_diag and are skipped by
+// block_gemv_row); the sum is meaningful for threads t < bs of the first half.
+__device__ __forceinline__ void load_block_row(const double *__restrict__ D, int tid, double (&dv)[TB / 2])
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t r = r0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    double *xv = x + (int64_t)blockIdx.y * ldx;
-    if (r >= r1) return;
-    double acc = 0;
-    for (int c = lane; c < bs; c += 64) acc += (double)LU[r * ld + b0 + c] * xv[b0 + c];
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if (lane == 0) xv[r] -= acc;
+    const int t = tid & (TB - 1), cb = (tid >> 7) * (TB / 2);
+    const double2 *p = reinterpret_cast<const double2 *>(D + (int64_t)t * TB + cb);
+#pragma unroll
+    for (int c = 0; c < TB / 4; ++c) {
+        const double2 w = p[c];
+        dv[2 * c] = w.x;
+        dv[2 * c + 1] = w.y;
+    }
 }
 
-__global__ void k_permute(const int *__restrict__ perm, int64_t n, const double *__restrict__ src, double *__restrict__ dst,
-                          int64_t ld)
+__device__ __forceinline__ double block_gemv_row(const double (&dv)[TB / 2], const double *in, int bs, int tid,
+                                                 double *part)
+{
+    const int t = tid & (TB - 1), h = tid >> 7, cb = h * (TB / 2);
+    double acc = 0.0;
+#pragma unroll
+    for (int c = 0; c < TB / 2; ++c)
+        if (cb + c < bs) acc += dv[c] * in[cb + c];
+    if (h == 1) part[t] = acc;
+    __syncthreads();
+    return acc + part[t];
+}
+
+// First block of a sweep: out[b0:b0+bs] = D x[b0:b0+bs] (one workgroup per RHS).
+__global__ __launch_bounds__(256) void k_trsv_first(const double *__restrict__ D, int64_t b0, int bs,
+                                                    const double *__restrict__ x, int64_t ldx,
+                                                    double *__restrict__ out, int64_t ldo)
+{
+    __shared__ double xb[TB], part[TB];
+    const int tid = threadIdx.x;
+    double dv[TB / 2];
+    load_block_row(D, tid, dv);
+    const double *xv = x + (int64_t)blockIdx.y * ldx + b0;
+    if (tid < bs) xb[tid] = xv[tid];
+    __syncthreads();
+    const double v = block_gemv_row(dv, xb, bs, tid, part);
+    if (tid < bs) out[(int64_t)blockIdx.y * ldo + b0 + tid] = v;
+}
+
+// Step for the solved block [b0, b0+bs) of right-hand side blockIdx.y (its
+// solution sol[b0:b0+bs] written by the previous launch): x[r] -= LU[r, b0:b0+bs]
+// . sol[b0:b0+bs] for the TB rows this workgroup owns -- rows [rlo + w*TB, ...)
+// going down (L), rows [.., rhi - w*TB) going up (U).  Workgroup 0 owns exactly
+// the next block [nb0, nb0+nbs) and, when Dn is given, writes its solution
+// sol[nb0:nb0+nbs] = Dn x[nb0:nb0+nbs].  Every global load (LU rows, x rows,
+// the solved block, workgroup 0's rows of Dn) is issued before the first use.
+template <typename T, bool DOWN>
+__global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ LU, int64_t ld, int64_t b0, int bs,
+                                                   double *__restrict__ x, int64_t ldx, double *__restrict__ sol,
+                                                   int64_t lds, int64_t rlo, int64_t rhi,
+                                                   const double *__restrict__ Dn, int64_t nb0, int nbs)
+{
+    __shared__ double yb[TB], xn[TB], part[TB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double *xv = x + (int64_t)blockIdx.y * ldx;
+    double *sv = sol + (int64_t)blockIdx.y * lds;
+    int64_t r0, r1;
+    if (DOWN) {
+        r0 = rlo + (int64_t)blockIdx.x * TRSV_ROWS;
+        r1 = min(rhi, r0 + TRSV_ROWS);
+    } else {
+        r1 = rhi - (int64_t)blockIdx.x * TRSV_ROWS;
+        r0 = max(rlo, r1 - TRSV_ROWS);
+    }
+    const bool lead = blockIdx.x == 0 && Dn != nullptr;   // uniform per workgroup
+    const int sub = lane & 15, c0 = sub * 8;
+    constexpr int NIT = TRSV_ROWS / 16;
+    double v[NIT][8], xr[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int64_t r = r0 + it * 16 + wave * 4 + (lane >> 4);
+        const T *lr = LU + r * ld + b0 + c0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[it][e] = (r < r1 && c0 + e < bs) ? (double)lr[e] : 0.0;
+        xr[it] = (sub == 0 && r < r1) ? xv[r] : 0.0;
+    }
+    double dv[TB / 2];
+    if (lead) load_block_row(Dn, tid, dv);
+    if (tid < bs) yb[tid] = sv[b0 + tid];
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int64_t r = r0 + it * 16 + wave * 4 + (lane >> 4);
+        double acc = 0.0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc += v[it][e] * (c0 + e < bs ? yb[c0 + e] : 0.0);
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+        if (sub == 0 && r < r1) {
+            const double nv = xr[it] - acc;
+            xv[r] = nv;
+            if (lead) xn[r - r0] = nv;
+        }
+    }
+    if (!lead) return;
+    __syncthreads();
+    const double vn = block_gemv_row(dv, xn, nbs, tid, part);
+    if (tid < nbs) sv[nb0 + tid] = vn;
+}
+
+__global__ void k_permute(const int *__restrict__ perm, int64_t n, const double *__restrict__ src, int64_t lds,
+                          double *__restrict__ dst, int64_t ldd)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    dst[(int64_t)blockIdx.y * ld + i] = src[(int64_t)blockIdx.y * ld + perm[i]];
+    dst[(int64_t)blockIdx.y * ldd + i] = src[(int64_t)blockIdx.y * lds + perm[i]];
 }
 
-size_t fp_getrs_work_bytes(int64_t n, int64_t k) { return (size_t)n * k * sizeof(double) + 256; }
+size_t fp_getrs_work_bytes(int64_t n, int64_t k)
+{
+    const int64_t nblk = (n + TB - 1) / TB;
+    return (size_t)2 * n * k * sizeof(double) + (size_t)2 * nblk * TB * TB * sizeof(double) + 256;
+}
 
 // `ipiv` here is the row permutation perm (row i of P*A is row perm[i] of A).
 template <typename T>
@@ -901,29 +1024,36 @@ int fp_getrs(fp_ctx *ctx, const T *LU, int64_t n, int64_t ld, const int *perm, d
 {
     if (n <= 0 || k <= 0) return FP_OK;
     if (work_bytes < fp_getrs_work_bytes(n, k)) return fp_fail(ctx, FP_ERR_ARG, "getrs workspace too small");
+    if (k > 65535) return fp_fail(ctx, FP_ERR_UNSUPPORTED, "getrs: too many right-hand sides in one call");
     ProfScope ps(ctx, "getrs");
-    double *tmp = (double *)work;
-    hipLaunchKernelGGL(k_permute, dim3((unsigned)((n + 255) / 256), (unsigned)k), dim3(256), 0, ctx->stream, perm, n,
-                       (const double *)x, tmp, n);
-    FP_HIP(ctx, hipMemcpy2DAsync(x, ldx * sizeof(double), tmp, n * sizeof(double), n * sizeof(double), k,
-                                 hipMemcpyDeviceToDevice, ctx->stream));
-    for (int64_t b0 = 0; b0 < n; b0 += TB) {
-        const int bs = (int)std::min<int64_t>(TB, n - b0);
-        hipLaunchKernelGGL((k_trsv_diag<T, true>), dim3((unsigned)k), dim3(256), 0, ctx->stream, LU, ld, b0, bs, x, ldx);
-        const int64_t r0 = b0 + bs;
-        if (r0 < n)
-            hipLaunchKernelGGL(k_trsv_update<T>, dim3((unsigned)((n - r0 + 3) / 4), (unsigned)k), dim3(256), 0,
-                               ctx->stream, LU, ld, r0, n, b0, bs, x, ldx);
-    }
     const int64_t nblk = (n + TB - 1) / TB;
-    for (int64_t bi = nblk - 1; bi >= 0; --bi) {
+    auto bsz = [&](int64_t bi) { return (int)std::min<int64_t>(TB, n - bi * TB); };
+    double *b = (double *)work;          // P rhs: running right-hand side of the forward sweep
+    double *y = b + n * k;               // L^-1 P rhs: running right-hand side of the backward sweep
+    double *Dl = y + n * k, *Du = Dl + nblk * TB * TB;
+    hipLaunchKernelGGL(k_permute, dim3((unsigned)((n + 255) / 256), (unsigned)k), dim3(256), 0, ctx->stream, perm, n,
+                       (const double *)x, ldx, b, n);
+    hipLaunchKernelGGL((k_trtri_diag<T, true>), dim3(TB, (unsigned)nblk), dim3(TB), 0, ctx->stream, LU, ld, n, Dl);
+    hipLaunchKernelGGL((k_trtri_diag<T, false>), dim3(TB, (unsigned)nblk), dim3(TB), 0, ctx->stream, LU, ld, n, Du);
+    // forward: y = L^-1 b
+    hipLaunchKernelGGL(k_trsv_first, dim3(1, (unsigned)k), dim3(256), 0, ctx->stream, (const double *)Dl, (int64_t)0,
+                       bsz(0), (const double *)b, n, y, n);
+    for (int64_t bi = 0; bi + 1 < nblk; ++bi) {
+        const int64_t b0 = bi * TB, rlo = b0 + TB;
+        const int64_t g = (n - rlo + TRSV_ROWS - 1) / TRSV_ROWS;
+        hipLaunchKernelGGL((k_trsv_step<T, true>), dim3((unsigned)g, (unsigned)k), dim3(256), 0, ctx->stream, LU, ld,
+                           b0, TB, b, n, y, n, rlo, n, (const double *)(Dl + (bi + 1) * TB * TB), rlo, bsz(bi + 1));
+    }
+    // backward: x = U^-1 y
+    hipLaunchKernelGGL(k_trsv_first, dim3(1, (unsigned)k), dim3(256), 0, ctx->stream,
+                       (const double *)(Du + (nblk - 1) * TB * TB), (nblk - 1) * TB, bsz(nblk - 1), (const double *)y,
+                       n, x, ldx);
+    for (int64_t bi = nblk - 1; bi >= 1; --bi) {
         const int64_t b0 = bi * TB;
-        const int bs = (int)std::min<int64_t>(TB, n - b0);
-        hipLaunchKernelGGL((k_trsv_diag<T, false>), dim3((unsigned)k), dim3(256), 0, ctx->stream, LU, ld, b0, bs, x,
-                           ldx);
-        if (b0 > 0)
-            hipLaunchKernelGGL(k_trsv_update<T>, dim3((unsigned)((b0 + 3) / 4), (unsigned)k), dim3(256), 0,
-                               ctx->stream, LU, ld, (int64_t)0, b0, b0, bs, x, ldx);
+        const int64_t g = (b0 + TRSV_ROWS - 1) / TRSV_ROWS;
+        hipLaunchKernelGGL((k_trsv_step<T, false>), dim3((unsigned)g, (unsigned)k), dim3(256), 0, ctx->stream, LU,
+                           ld, b0, bsz(bi), y, n, x, ldx, (int64_t)0, b0, (const double *)(Du + (bi - 1) * TB * TB),
+                           b0 - TB, TB);
     }
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
