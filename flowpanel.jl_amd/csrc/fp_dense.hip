@@ -4,14 +4,15 @@
 // Replaces LAPACK getrf/getrs behind `solve_ludiv!` (src/FLOWPanel_linearsolver.jl:57-78,
 // calc_Alu :173-234).  Matrices are row-major (A[i*ld + j]).
 //
-// getrf: right-looking blocked LU, outer blocks of NB = 256 columns (the trailing
-// update is one MFMA GEMM with k = 256), each outer block factored in sub-panels of
-// PW = 32 columns.  A sub-panel (m x 32) is factored with tournament pivoting
-// (TSLU, Grigori-Demmel-Xiang 2011): every workgroup runs partial pivoting on a
-// chunk of <= 256 rows in LDS and proposes 32 candidate rows; candidate sets are
-// merged 8 at a time by partial pivoting on their original values until 32
-// pivot rows remain.  This needs no grid-wide synchronisation: one sub-panel is
-// six short launches instead of one grid barrier per column.
+// getrf: right-looking blocked LU, outer blocks of NB = 512 columns (the block
+// row U12 = L11^-1 A12 and the trailing update are MFMA GEMMs with k = 512),
+// each outer block factored in sub-panels of PW = 32 columns.  A sub-panel
+// (m x 32) is factored with tournament pivoting (TSLU, Grigori-Demmel-Xiang
+// 2011): every workgroup runs partial pivoting on a chunk of <= 256 rows (one
+// row per thread, in registers) and proposes 32 candidate rows; candidate sets
+// are merged 16 at a time by partial pivoting on their original values until
+// 32 pivot rows remain.  This needs no grid-wide synchronisation: one
+// sub-panel is a few short launches instead of one grid barrier per column.
 #include "fp_internal.h"
 
 #include <algorithm>
@@ -215,26 +216,41 @@ __global__ __launch_bounds__(256) void k_gemm_sub_f64(int64_t m, int64_t n, int6
                     acc[a][b] = __builtin_amdgcn_mfma_f64_4x4x4f64(fa[a], fb[b], acc[a][b], 0, 0, 0);
         }
     }
+    // epilogue: every load of the lane's 4 x 16 block of C is issued before the
+    // first store (rows of C could alias as far as the compiler knows, which
+    // would otherwise serialise one memory round trip per row)
     const int P = 4 * ((lane >> 2) & 3) + (lane >> 4);
     const int64_t col0 = n0 + wn * 64 + (lane & 3) * 16;
-    const bool cvec = vec && (ldc & 1) == 0 && ((uintptr_t)C & 15) == 0;
+    const int64_t row0 = m0 + wm * 64 + P * 4;
+    const bool cvec = vec && (ldc & 1) == 0 && ((uintptr_t)C & 15) == 0 && col0 + 16 <= n;
+    double cv[4][16];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-        const int64_t row = m0 + wm * 64 + P * 4 + a;
-        if (row >= m) continue;
-        double *c = C + row * ldc + col0;
-        if (cvec && col0 + 16 <= n) {
+        const double *c = C + (row0 + a) * ldc + col0;
+        if (row0 + a < m && cvec) {
 #pragma unroll
             for (int b = 0; b < 16; b += 2) {
-                double2 v = *reinterpret_cast<double2 *>(c + b);
-                v.x -= acc[a][b];
-                v.y -= acc[a][b + 1];
-                *reinterpret_cast<double2 *>(c + b) = v;
+                const double2 v = *reinterpret_cast<const double2 *>(c + b);
+                cv[a][b] = v.x;
+                cv[a][b + 1] = v.y;
             }
         } else {
 #pragma unroll
+            for (int b = 0; b < 16; ++b) cv[a][b] = (row0 + a < m && col0 + b < n) ? c[b] : 0.0;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        if (row0 + a >= m) continue;
+        double *c = C + (row0 + a) * ldc + col0;
+        if (cvec) {
+#pragma unroll
+            for (int b = 0; b < 16; b += 2)
+                *reinterpret_cast<double2 *>(c + b) = double2{cv[a][b] - acc[a][b], cv[a][b + 1] - acc[a][b + 1]};
+        } else {
+#pragma unroll
             for (int b = 0; b < 16; ++b)
-                if (col0 + b < n) c[b] -= acc[a][b];
+                if (col0 + b < n) c[b] = cv[a][b] - acc[a][b];
         }
     }
 }
@@ -675,12 +691,12 @@ __global__ void k_fill_i32(int *p, int64_t n, int v)
     if (i < n) p[i] = v;
 }
 
-// Outer-block triangular update U12 := L11^-1 A12 (nb <= 256 rows) as one MFMA
+// Outer-block triangular update U12 := L11^-1 A12 (nb <= TRI_NB rows) as one MFMA
 // GEMM: M = I - L11^-1 (strictly lower, written by k_trtri_llu), W = a copy of
 // A12, then A12 -= M W.  Inverting the unit lower L11 explicitly is the
 // inverted-diagonal-block trsm used by GPU LAPACK codes; with pivoted
 // multipliers |l| <= 1 its entries stay modest.
-constexpr int TRI_NB = 256;
+constexpr int TRI_NB = 512;   // outer block of the LU (columns per trailing update)
 
 // Column j = blockIdx.x of L^-1 by forward substitution: thread r owns x_r and
 // keeps 32 entries of its own row of L in registers at a time; x_q is published
@@ -727,7 +743,7 @@ size_t fp_getrf_work_bytes(int64_t n, int elem_bytes)
 {
     const int64_t g = (n + CH - 1) / CH;
     // two candidate buffers (values + indices) + final block + row map, then
-    // the outer-block inverse M (256 x 256) and the A12 copy W (256 x ldw)
+    // the outer-block inverse M (TRI_NB x TRI_NB) and the A12 copy W (TRI_NB x ldw)
     const size_t tslu = (size_t)(2 * g * PW * PW) * elem_bytes + (size_t)(2 * g * PW) * sizeof(int) +
                         PW * PW * elem_bytes + (4 * PW + 16) * sizeof(int);
     const size_t head = (tslu + 255) & ~(size_t)255;
@@ -770,7 +786,7 @@ int fp_getrf(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *perm, int *info_dev,
     const int64_t ldw = tri_ldw(n);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, info_dev, (int64_t)1, 0);
     hipLaunchKernelGGL(k_iota, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, perm, n);
-    const int NB = 256;
+    const int NB = TRI_NB;
     for (int64_t k = 0; k < n; k += NB) {
         const int nb = (int)std::min<int64_t>(NB, n - k);
         for (int64_t s = k; s < k + nb; s += PW) {
@@ -1643,6 +1659,75 @@ __global__ __launch_bounds__(1024) void k_small_inverse(double *__restrict__ E, 
     }
 }
 
+// m <= SMALL_INV_LDS: the same inverse in place in LDS (Gauss-Jordan with row
+// interchanges, the pivot row scaled by the reciprocal pivot, the columns
+// swapped back in reverse order at the end); ties go to the lowest row.
+constexpr int SMALL_INV_LDS = 128;
+
+__global__ __launch_bounds__(1024) void k_small_inverse_lds(double *__restrict__ E, int m, int *__restrict__ info)
+{
+    __shared__ double a[SMALL_INV_LDS][SMALL_INV_LDS + 1];
+    __shared__ int piv[SMALL_INV_LDS];
+    __shared__ double col[SMALL_INV_LDS];
+    __shared__ double wv[16];
+    __shared__ int wr[16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nt = blockDim.x;
+    for (int e = tid; e < m * m; e += nt) a[e / m][e % m] = E[e];
+    __syncthreads();
+    for (int k = 0; k < m; ++k) {
+        double v = -1.0;
+        int r = m;
+        for (int rr = k + tid; rr < m; rr += nt) argmax_pair(v, r, fabs(a[rr][k]), rr);
+        for (int o = 32; o > 0; o >>= 1) {
+            const double v2 = __shfl_xor(v, o);
+            const int r2 = __shfl_xor(r, o);
+            argmax_pair(v, r, v2, r2);
+        }
+        if (lane == 0) { wv[wave] = v; wr[wave] = r; }
+        __syncthreads();
+        double bv = wv[0];
+        int p = wr[0];
+        for (int q = 1; q < (nt >> 6); ++q) argmax_pair(bv, p, wv[q], wr[q]);
+        if (p >= m) p = k;
+        if (tid == 0) piv[k] = p;
+        if (p != k)
+            for (int e = tid; e < m; e += nt) {
+                const double t = a[k][e];
+                a[k][e] = a[p][e];
+                a[p][e] = t;
+            }
+        __syncthreads();
+        const double pv = a[k][k];
+        if (pv == 0.0) {
+            if (tid == 0 && *info == 0) *info = k + 1;
+            __syncthreads();
+            continue;
+        }
+        const double inv = 1.0 / pv;
+        // the multipliers (column k of every row) are saved before any entry changes
+        for (int i = tid; i < m; i += nt) col[i] = a[i][k];
+        __syncthreads();
+        for (int e = tid; e < m; e += nt) a[k][e] = e == k ? inv : a[k][e] * inv;
+        __syncthreads();
+        for (int e = tid; e < m * m; e += nt) {
+            const int i = e / m, j = e % m;
+            if (i != k) a[i][j] = j == k ? -col[i] * inv : a[i][j] - col[i] * a[k][j];
+        }
+        __syncthreads();
+    }
+    for (int k = m - 1; k >= 0; --k) {
+        const int p = piv[k];
+        if (p != k)
+            for (int i = tid; i < m; i += nt) {
+                const double t = a[i][k];
+                a[i][k] = a[i][p];
+                a[i][p] = t;
+            }
+        __syncthreads();
+    }
+    for (int e = tid; e < m * m; e += nt) E[e] = a[e / m][e % m];
+}
+
 template <typename T>
 int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, double *AZ, double *Einv, double *work,
                     int *info_dev)
@@ -1654,7 +1739,10 @@ int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, doub
     hipLaunchKernelGGL(k_chunk_rowsum<T>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, A, n, ld, cs, nc,
                        AZ);
     hipLaunchKernelGGL(k_coarse_galerkin, dim3(nc, nc), dim3(256), 0, ctx->stream, (const double *)AZ, n, cs, nc, Einv);
-    hipLaunchKernelGGL(k_small_inverse, dim3(1), dim3(1024), 0, ctx->stream, Einv, nc, work, info_dev);
+    if (nc <= SMALL_INV_LDS)
+        hipLaunchKernelGGL(k_small_inverse_lds, dim3(1), dim3(1024), 0, ctx->stream, Einv, nc, info_dev);
+    else
+        hipLaunchKernelGGL(k_small_inverse, dim3(1), dim3(1024), 0, ctx->stream, Einv, nc, work, info_dev);
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }
@@ -1814,14 +1902,20 @@ __global__ __launch_bounds__(256) void k_mdot_partial(const double *__restrict__
     if (tid < nc) part[(int64_t)(c0 + tid) * gridDim.x + blockIdx.x] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
 }
 
+// one wave per column: every partial loaded at once, then a shuffle sum
 __global__ __launch_bounds__(256) void k_mdot_final(const double *__restrict__ part, int nparts, int nv,
                                                     double *__restrict__ out)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (c >= nv) return;
+    const double *pc = part + (int64_t)c * nparts;
     double acc = 0;
-    for (int k = 0; k < nparts; ++k) acc += part[(int64_t)c * nparts + k];
-    out[c] = acc;
+#pragma unroll
+    for (int k = 0; k < MDOT_PARTS; k += 64)
+        if (k + lane < nparts) acc += pc[k + lane];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) out[c] = acc;
 }
 
 int fp_mdot(fp_ctx *ctx, const double *V, int64_t ldv, int nv, const double *w, int64_t n, double *part,
@@ -1831,7 +1925,7 @@ int fp_mdot(fp_ctx *ctx, const double *V, int64_t ldv, int nv, const double *w, 
     const int parts = (int)std::max<int64_t>(1, std::min<int64_t>(MDOT_PARTS, (n + 255) / 256));
     hipLaunchKernelGGL(k_mdot_partial, dim3(parts, (nv + MDOT_G - 1) / MDOT_G), dim3(256), 0, ctx->stream, V, ldv, nv,
                        w, n, part);
-    hipLaunchKernelGGL(k_mdot_final, dim3((nv + 255) / 256), dim3(256), 0, ctx->stream, (const double *)part, parts, nv,
+    hipLaunchKernelGGL(k_mdot_final, dim3((nv + 3) / 4), dim3(256), 0, ctx->stream, (const double *)part, parts, nv,
                        out);
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;

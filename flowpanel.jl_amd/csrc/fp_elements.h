@@ -9,6 +9,33 @@
 template <typename T>
 __device__ __forceinline__ T fp_sign(T z) { return z > T(0) ? T(1) : (z < T(0) ? T(-1) : z); }
 
+// 1/x: the hardware reciprocal refined by two Newton steps, r <- r + r (1 - x r)
+// (a few ulp at most, where the IEEE division sequence is correctly rounded but
+// about twice the instructions).  0, infinities and NaN keep the raw reciprocal.
+__device__ __forceinline__ double fp_recip(double x)
+{
+    const double r0 = __builtin_amdgcn_rcp(x);
+    double r = fma(r0, fma(-x, r0, 1.0), r0);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return __builtin_isfinite(r) ? r : r0;
+}
+__device__ __forceinline__ float fp_recip(float x) { return 1.0f / x; }
+
+// sqrt(x) for x >= 0 as x * rsqrt(x), the hardware reciprocal square root
+// refined by two Newton steps y <- y (3 - x y^2) / 2 (a few ulp at most, against
+// the longer correctly rounded sequence); x below 2^-1000 (including 0) and
+// non-finite x take the library sqrt.
+__device__ __forceinline__ double fp_sqrt(double x)
+{
+    if (!(x > 0x1p-1000 && x < 0x1p1000)) return sqrt(x);
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return x * y;
+}
+__device__ __forceinline__ float fp_sqrt(float x) { return sqrtf(x); }
+
 // Regularised Biot-Savart segment a -> b (src/FLOWPanel_elements.jl:345-428),
 // velocity per unit strength added to u.  e = b - a.
 template <typename T>
@@ -21,11 +48,11 @@ __device__ __forceinline__ void seg_u(T ax, T ay, T az, T ex, T ey, T ez, T bx, 
     const T cy = riz * rjx - rix * rjz;
     const T cz = rix * rjy - riy * rjx;
     const T c2 = cx * cx + cy * cy + cz * cz;
-    const T ni = sqrt(rix * rix + riy * riy + riz * riz) + off;
-    const T nj = sqrt(rjx * rjx + rjy * rjy + rjz * rjz) + off;
-    const T proj = ex * (rix / ni - rjx / nj) + ey * (riy / ni - rjy / nj) + ez * (riz / ni - rjz / nj);
+    const T ii = fp_recip(fp_sqrt(rix * rix + riy * riy + riz * riz) + off);
+    const T ij = fp_recip(fp_sqrt(rjx * rjx + rjy * rjy + rjz * rjz) + off);
+    const T proj = ex * (rix * ii - rjx * ij) + ey * (riy * ii - rjy * ij) + ez * (riz * ii - rjz * ij);
     if (c2 > cut2) {
-        const T aux = s * proj / (T(4 * FP_PI) * (c2 + off2));
+        const T aux = s * proj * fp_recip(T(4 * FP_PI) * (c2 + off2));
         ux += aux * cx; uy += aux * cy; uz += aux * cz;
     }
 }
@@ -42,7 +69,7 @@ __device__ __forceinline__ void ring_u(const T *px, const T *py, const T *pz, co
 #pragma unroll
     for (int k = 0; k < NVP; ++k) {
         rx[k] = X - px[k]; ry[k] = Y - py[k]; rz[k] = Z - pz[k];
-        iv[k] = T(1) / (sqrt(rx[k] * rx[k] + ry[k] * ry[k] + rz[k] * rz[k]) + off);
+        iv[k] = fp_recip(fp_sqrt(rx[k] * rx[k] + ry[k] * ry[k] + rz[k] * rz[k]) + off);
     }
 #pragma unroll
     for (int k = 0; k < NVP; ++k) {
@@ -54,7 +81,7 @@ __device__ __forceinline__ void ring_u(const T *px, const T *py, const T *pz, co
         const T proj = ex[k] * (rx[k] * iv[k] - rx[q] * iv[q]) + ey[k] * (ry[k] * iv[k] - ry[q] * iv[q]) +
                        ez[k] * (rz[k] * iv[k] - rz[q] * iv[q]);
         if (c2 > cut2) {
-            const T aux = proj / (T(4 * FP_PI) * (c2 + off2));
+            const T aux = proj * fp_recip(T(4 * FP_PI) * (c2 + off2));
             ux += aux * cx; uy += aux * cy; uz += aux * cz;
         }
     }
