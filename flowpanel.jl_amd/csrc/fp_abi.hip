@@ -192,10 +192,18 @@ struct BJPrecond : Precond {
     bool has_inv = false;
     int cs = 0;
     BJPrecond(fp_ctx *c, const float *b, const int *p, int64_t n_, int bl) : ctx(c), BJ(b), piv(p), n(n_), block(bl) {}
-    int setup(const void *A = nullptr, int prec = FP_F32, int64_t ld = 0, int coarse = 0)
+    int setup(const void *A = nullptr, int prec = FP_F32, int64_t ld = 0, int coarse = 0, int *info_dev = nullptr)
     {
         int st;
-        if (block <= 512) {
+        if (!BJ) {
+            // no LU factors: inverses straight from A by Gauss-Jordan (block <= 256)
+            const int64_t nb = (n + block - 1) / block;
+            if ((st = dalloc(ctx, inv, (size_t)nb * block * block * sizeof(float)))) return st;
+            st = prec == FP_F64 ? fp_bj_gj_inverse<double>(ctx, (const double *)A, n, ld, block, inv.as<float>(), info_dev)
+                                : fp_bj_gj_inverse<float>(ctx, (const float *)A, n, ld, block, inv.as<float>(), info_dev);
+            if (st) return st;
+            has_inv = true;
+        } else if (block <= 512) {
             const int64_t nb = (n + block - 1) / block;
             if ((st = dalloc(ctx, inv, (size_t)nb * block * block * sizeof(float)))) return st;
             if ((st = fp_bj_invert(ctx, BJ, piv, n, block, inv.as<float>()))) return st;
@@ -438,9 +446,13 @@ struct Solver {
         const int64_t n = a.n;
         const int64_t nbk = (n + blk - 1) / blk;
         int st;
+        if ((st = dalloc(ctx, info, sizeof(int)))) return st;
+        if (blk <= 256) {
+            M.reset(new BJPrecond(ctx, nullptr, nullptr, n, blk));
+            return M->setup(A, prec, a.ld, o.coarse_block, info.as<int>());
+        }
         if ((st = dalloc(ctx, bj, (size_t)nbk * blk * blk * sizeof(float)))) return st;
         if ((st = dalloc(ctx, bjp, (size_t)nbk * blk * sizeof(int)))) return st;
-        if ((st = dalloc(ctx, info, sizeof(int)))) return st;
         st = prec == FP_F64
                  ? fp_bj_factor<double>(ctx, (const double *)A, n, a.ld, blk, bj.as<float>(), bjp.as<int>(), info.as<int>())
                  : fp_bj_factor<float>(ctx, (const float *)A, n, a.ld, blk, bj.as<float>(), bjp.as<int>(), info.as<int>());

@@ -1359,6 +1359,130 @@ __global__ __launch_bounds__(1024) void k_bj_factor(const T *__restrict__ A, int
     }
 }
 
+// Block-Jacobi inverses in one pass for blocks <= GJ_MAX: Gauss-Jordan with
+// partial pivoting, the block held in registers (1024 threads as a 32 x 32
+// grid, thread (tr, tc) holding rows tr + 32 i and columns tc + 32 j, 8 x 8
+// FP32 values).  Rows are never swapped: the pivot of step k is the unused
+// row p_k with the largest |a(p, k)| (lowest row on ties, NaN counted as 0),
+// and the in-place result W satisfies inverse(pinv(r), p_c) = W(r, c), so the
+// final store scatters through the two index maps.  Per step: column k and
+// the scaled pivot row go through LDS, everything else stays in registers.
+constexpr int GJ_MAX = 256;
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_bj_gj(const T *__restrict__ A, int64_t n, int64_t ld, int block,
+                                                float *__restrict__ inv, int *__restrict__ info)
+{
+    __shared__ float colk[GJ_MAX], prow[GJ_MAX];
+    __shared__ int used[GJ_MAX], prow_of[GJ_MAX], step_of[GJ_MAX];
+    __shared__ float wv[16];
+    __shared__ int wr[16];
+    const int b = blockIdx.x;
+    const int64_t b0 = (int64_t)b * block;
+    const int bs = (int)min((int64_t)block, n - b0);
+    const int tid = threadIdx.x, tr = tid >> 5, tc = tid & 31, lane = tid & 63, wave = tid >> 6;
+    float w[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int r = tr + 32 * i, c = tc + 32 * j;
+            w[i][j] = (r < bs && c < bs) ? (float)A[(b0 + r) * ld + b0 + c] : 0.f;
+        }
+    if (tid < GJ_MAX) used[tid] = 0;
+    __syncthreads();
+    for (int kb = 0; kb < 8; ++kb) {
+        for (int kk = 0; kk < 32; ++kk) {
+            const int k = kb * 32 + kk;
+            if (k >= bs) break;
+            if (tc == kk) {
+                // column k is register column kb of these threads (selected, not indexed)
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    float val = 0.f;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        if (j == kb) val = w[i][j];
+                    if (tr + 32 * i < bs) colk[tr + 32 * i] = val;
+                }
+            }
+            __syncthreads();
+            float v = -1.f;
+            int rr = GJ_MAX;
+            if (tid < bs && !used[tid]) {
+                v = fabsf(colk[tid]);
+                if (!(v == v)) v = 0.f;
+                rr = tid;
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                const float v2 = __shfl_xor(v, o);
+                const int r2 = __shfl_xor(rr, o);
+                if (v2 > v || (v2 == v && r2 < rr)) { v = v2; rr = r2; }
+            }
+            if (lane == 0) { wv[wave] = v; wr[wave] = rr; }
+            __syncthreads();
+            float bv = wv[0];
+            int p = wr[0];
+            for (int q = 1; q < 16; ++q)
+                if (wv[q] > bv || (wv[q] == bv && wr[q] < p)) { bv = wv[q]; p = wr[q]; }
+            const float piv = colk[p];
+            if (piv == 0.f && tid == 0) atomicCAS(info, 0, b * block + k + 1);
+            const float ip = 1.f / piv;
+            if (tr == (p & 31)) {
+                const int ip5 = p >> 5;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    float val = 0.f;
+#pragma unroll
+                    for (int i = 0; i < 8; ++i)
+                        if (i == ip5) val = w[i][j];
+                    const int c = tc + 32 * j;
+                    if (c < bs) prow[c] = c == k ? ip : val * ip;
+                }
+            }
+            if (tid == 0) { used[p] = 1; prow_of[k] = p; step_of[p] = k; }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int r = tr + 32 * i;
+                if (r >= bs) continue;
+                const float f = colk[r];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int c = tc + 32 * j;
+                    if (c >= bs) continue;
+                    if (r == p) w[i][j] = prow[c];
+                    else if (c == k) w[i][j] = -f * ip;
+                    else w[i][j] -= f * prow[c];
+                }
+            }
+            __syncthreads();
+        }
+    }
+    float *I = inv + (int64_t)b * block * block;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int r = tr + 32 * i, c = tc + 32 * j;
+            if (r < bs && c < bs) I[(int64_t)step_of[r] * block + prow_of[c]] = w[i][j];
+        }
+}
+
+template <typename T>
+int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *inv, int *info_dev)
+{
+    if (block > GJ_MAX) return fp_fail(ctx, FP_ERR_ARG, "Gauss-Jordan block inverse: block larger than 256");
+    const int nb = (int)((n + block - 1) / block);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, info_dev, (int64_t)1, 0);
+    ProfScope ps(ctx, "bj_invert");
+    hipLaunchKernelGGL(k_bj_gj<T>, dim3(nb), dim3(1024), 0, ctx->stream, A, n, ld, block, inv, info_dev);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+template int fp_bj_gj_inverse<double>(fp_ctx *, const double *, int64_t, int64_t, int, float *, int *);
+template int fp_bj_gj_inverse<float>(fp_ctx *, const float *, int64_t, int64_t, int, float *, int *);
+
 template <typename T>
 int fp_bj_factor(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *BJ, int *piv, int *info_dev)
 {

@@ -230,6 +230,9 @@ int fp_bj_factor(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, floa
 int fp_bj_apply(fp_ctx *ctx, const float *BJ, const int *piv, int64_t n, int block, const double *x, double *y);
 // explicit block inverses (block <= 512) and their application
 int fp_bj_invert(fp_ctx *ctx, const float *BJ, const int *piv, int64_t n, int block, float *inv);
+// Block-Jacobi inverses straight from A (Gauss-Jordan in registers), block <= 256
+template <typename T>
+int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *inv, int *info_dev);
 int fp_bj_apply_inv(fp_ctx *ctx, const float *inv, int64_t n, int block, const double *x, double *y, int accumulate);
 // Galerkin coarse space of chunk indicators: AZ = A Z (n x nc), Einv = (Z^T A Z)^-1
 template <typename T>
