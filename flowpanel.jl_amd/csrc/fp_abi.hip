@@ -1252,7 +1252,7 @@ int fp_body_get_strength(fp_ctx *ctx, const fp_body *b, double *s)
 }
 
 static int assemble_into(fp_ctx *ctx, fp_body *const *bodies, int nb, const TargetView &tv, int precision, void *A,
-                         int64_t ld)
+                         int64_t ld, float *A32 = nullptr)
 {
     int64_t col = 0;
     for (int k = 0; k < nb; ++k) {
@@ -1262,7 +1262,7 @@ static int assemble_into(fp_ctx *ctx, fp_body *const *bodies, int nb, const Targ
         if (b->kernel_offset != bodies[0]->kernel_offset || b->kernel_cutoff != bodies[0]->kernel_cutoff) {
             // each body assembles with its own scalars; nothing to check
         }
-        int st = precision == FP_F64 ? fp_launch_assemble<double>(ctx, b, tv, (double *)A, ld, col)
+        int st = precision == FP_F64 ? fp_launch_assemble<double>(ctx, b, tv, (double *)A, ld, col, A32)
                                      : fp_launch_assemble<float>(ctx, b, tv, (float *)A, ld, col);
         if (st) return st;
         col += b->N;
@@ -1292,15 +1292,16 @@ static int solve_system(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_so
     Timer tm(ctx->stream);
     const bool f64 = o.solver == FP_SOLVER_LU || o.solver == FP_SOLVER_GMRES;
     // refinement with a stored FP64 residual: assemble in FP64 once, keep it,
-    // and narrow a copy to FP32 for the low-precision solver
+    // and store its FP32 rounding in the same pass for the low-precision solver
     const bool dual = !f64 && o.residual_mode == FP_RESID_STORED;
     DBuf A, A32buf;
     std::unique_ptr<DenseOp<double>> op64;
     if ((st = dalloc(ctx, A, (size_t)n * ld * ((f64 || dual) ? sizeof(double) : sizeof(float))))) return st;
     if (dual && (st = dalloc(ctx, A32buf, (size_t)n * ld * sizeof(float)))) return st;
-    if ((st = assemble_into(ctx, bodies, nb, t.tv, (f64 || dual) ? FP_F64 : FP_F32, A.p, ld))) return st;
+    if ((st = assemble_into(ctx, bodies, nb, t.tv, (f64 || dual) ? FP_F64 : FP_F32, A.p, ld,
+                            dual ? A32buf.as<float>() : nullptr)))
+        return st;
     if (dual) {
-        if ((st = fp_mat_narrow(ctx, A.as<double>(), A32buf.as<float>(), n, ld))) return st;
         a.A32 = A32buf.as<float>();
         op64.reset(new DenseOp<double>(ctx, A.as<double>(), n, ld));
         a.resid = op64.get();
@@ -1931,7 +1932,10 @@ static int liftdrag_run(fp_ctx *ctx, fp_body *const *bodies, int nb, const doubl
     a.ld = ld;
     a.resid = &mf;
     if ((st = dalloc(ctx, A, (size_t)n * ld * (stored ? sizeof(double) : sizeof(float))))) return st;
-    if ((st = assemble_into(ctx, bodies, nb, t.tv, stored ? FP_F64 : FP_F32, A.p, ld))) return st;
+    if (!f64 && stored && (st = dalloc(ctx, A32, (size_t)n * ld * sizeof(float)))) return st;
+    if ((st = assemble_into(ctx, bodies, nb, t.tv, stored ? FP_F64 : FP_F32, A.p, ld,
+                            (!f64 && stored) ? A32.as<float>() : nullptr)))
+        return st;
     if (adjoint) {  // before any in-place factorisation of A
         if ((st = dalloc(ctx, AT, (size_t)n * ld * sizeof(double)))) return st;
         if ((st = fp_mat_transpose(ctx, A.as<double>(), n, ld, AT.p, FP_F64))) return st;
@@ -1943,8 +1947,6 @@ static int liftdrag_run(fp_ctx *ctx, fp_body *const *bodies, int nb, const doubl
             a.resid = op64.get();
         }
     } else if (stored) {
-        if ((st = dalloc(ctx, A32, (size_t)n * ld * sizeof(float)))) return st;
-        if ((st = fp_mat_narrow(ctx, A.as<double>(), A32.as<float>(), n, ld))) return st;
         a.A32 = A32.as<float>();
         op64.reset(new DenseOp<double>(ctx, A.as<double>(), n, ld));
         a.resid = op64.get();

@@ -103,7 +103,8 @@ __device__ __forceinline__ void vr_load_targets(const TargetView &tv, int64_t i0
 
 template <typename T, typename TO, int NVP, int TM>
 __device__ __forceinline__ void vr_tile(const PanelView &pan, const TargetView &tv, TO *__restrict__ G, int64_t ld,
-                                        int64_t col0, int64_t bx, int64_t i0, const T (&s)[6][TM])
+                                        int64_t col0, int64_t bx, int64_t i0, const T (&s)[6][TM],
+                                        float *__restrict__ G2 = nullptr)
 {
     const int64_t j = bx * blockDim.x + threadIdx.x;
     if (j >= pan.N) return;
@@ -123,22 +124,25 @@ __device__ __forceinline__ void vr_tile(const PanelView &pan, const TargetView &
     const T off = T(pan.off), off2 = T(pan.off * pan.off), cut2 = T(pan.cut * pan.cut);
     const int iend = (int)min((int64_t)TM, tv.M - i0);
     TO *g = G + i0 * ld + col0 + j;
+    float *g2 = G2 ? G2 + i0 * ld + col0 + j : nullptr;
     for (int t = 0; t < iend; ++t) {
         T ux = 0, uy = 0, uz = 0;
         ring_u<T, NVP>(px, py, pz, ex, ey, ez, s[0][t], s[1][t], s[2][t], off, off2, cut2, ux, uy, uz);
-        g[(int64_t)t * ld] = TO(ux * s[3][t] + uy * s[4][t] + uz * s[5][t]);
+        const T v = ux * s[3][t] + uy * s[4][t] + uz * s[5][t];
+        g[(int64_t)t * ld] = TO(v);
+        if (g2) g2[(int64_t)t * ld] = float(v);
     }
 }
 
 template <typename T, typename TO, int NVP, int TM>
 __global__ __launch_bounds__(256) void k_assemble_vr(PanelView pan, TargetView tv, TO *__restrict__ G, int64_t ld,
-                                                     int64_t col0)
+                                                     int64_t col0, float *__restrict__ G2)
 {
     __shared__ T s[6][TM];
     const int64_t i0 = (int64_t)blockIdx.y * TM;
     vr_load_targets<T, TO, NVP, TM>(tv, i0, s);
     __syncthreads();
-    vr_tile<T, TO, NVP, TM>(pan, tv, G, ld, col0, blockIdx.x, i0, s);
+    vr_tile<T, TO, NVP, TM>(pan, tv, G, ld, col0, blockIdx.x, i0, s, G2);
 }
 
 // Batched assembly of independent bodies (each its own matrix): persistent
@@ -173,7 +177,7 @@ __global__ void k_zero_counter(int *counter) { *(unsigned long long *)counter = 
 // AIC assembly, constant source (Hess-Smith), one lane per panel.
 template <typename T, typename TO, int NVP, int TM>
 __global__ __launch_bounds__(256) void k_assemble_source(PanelView pan, TargetView tv, TO *__restrict__ G,
-                                                         int64_t ld, int64_t col0)
+                                                         int64_t ld, int64_t col0, float *__restrict__ G2)
 {
     __shared__ T s[6][TM];
     const int64_t i0 = (int64_t)blockIdx.y * TM;
@@ -200,6 +204,7 @@ __global__ __launch_bounds__(256) void k_assemble_source(PanelView pan, TargetVi
     const T k4 = T(1.0 / (4 * FP_PI));
     const int iend = (int)min((int64_t)TM, tv.M - i0);
     TO *g = G + i0 * ld + col0 + j;
+    float *g2 = G2 ? G2 + i0 * ld + col0 + j : nullptr;
     for (int t = 0; t < iend; ++t) {
         const T dx = s[0][t] - h[9], dy = s[1][t] - h[10], dz = s[2][t] - h[11];
         const T x = h[0] * dx + h[1] * dy + h[2] * dz;
@@ -212,14 +217,17 @@ __global__ __launch_bounds__(256) void k_assemble_source(PanelView pan, TargetVi
         const T ux = V1 * h[0] + V2 * h[3] + V3 * h[6];
         const T uy = V1 * h[1] + V2 * h[4] + V3 * h[7];
         const T uz = V1 * h[2] + V2 * h[5] + V3 * h[8];
-        g[(int64_t)t * ld] = TO(ux * s[3][t] + uy * s[4][t] + uz * s[5][t]);
+        const T v = ux * s[3][t] + uy * s[4][t] + uz * s[5][t];
+        g[(int64_t)t * ld] = TO(v);
+        if (g2) g2[(int64_t)t * ld] = float(v);
     }
 }
 
 // Wake horseshoes into their columns: one lane per target, shedding list in order.
 template <typename T, typename TO>
 __global__ __launch_bounds__(256) void k_assemble_wake(HorseView hv, double off_, double cut_, TargetView tv,
-                                                       TO *__restrict__ G, int64_t ld, int64_t col0)
+                                                       TO *__restrict__ G, int64_t ld, int64_t col0,
+                                                       float *__restrict__ G2)
 {
     __shared__ T sh[64][12];
     __shared__ int sc[64];
@@ -242,7 +250,9 @@ __global__ __launch_bounds__(256) void k_assemble_wake(HorseView hv, double off_
                 T ux = 0, uy = 0, uz = 0;
                 horseshoe_u<T>(sh[hh], X, Y, Z, T(1), off, off2, cut2, ux, uy, uz);
                 TO *g = G + i * ld + col0 + sc[hh];
-                *g = TO(T(*g) + (ux * nx + uy * ny + uz * nz));
+                const TO v = TO(T(*g) + (ux * nx + uy * ny + uz * nz));
+                *g = v;
+                if (G2) G2[i * ld + col0 + sc[hh]] = float(v);
             }
         }
     }
@@ -253,7 +263,8 @@ __global__ __launch_bounds__(256) void k_assemble_wake(HorseView hv, double off_
 // field (r_i/|r_i| - r_j/|r_j| and r_i x r_j cancel), which would limit the
 // refinement of the mixed-precision solve.
 template <typename T>
-int fp_launch_assemble(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G, int64_t ld, int64_t col0)
+int fp_launch_assemble(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G, int64_t ld, int64_t col0,
+                       float *G32)
 {
     constexpr int TM = 64;
     const PanelView pan = panel_view(b);
@@ -262,30 +273,33 @@ int fp_launch_assemble(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G
     if (b->element == FP_ELEM_VORTEXRING) {
         ProfScope ps(ctx, sizeof(T) == 8 ? "assemble_vr_f64" : "assemble_vr_f32");
         if (b->nvp == 4)
-            hipLaunchKernelGGL((k_assemble_vr<double, T, 4, TM>), grid, dim3(256), 0, ctx->stream, pan, tv, G, ld, col0);
+            hipLaunchKernelGGL((k_assemble_vr<double, T, 4, TM>), grid, dim3(256), 0, ctx->stream, pan, tv, G, ld, col0,
+                               G32);
         else
-            hipLaunchKernelGGL((k_assemble_vr<double, T, 8, TM>), grid, dim3(256), 0, ctx->stream, pan, tv, G, ld, col0);
+            hipLaunchKernelGGL((k_assemble_vr<double, T, 8, TM>), grid, dim3(256), 0, ctx->stream, pan, tv, G, ld, col0,
+                               G32);
         FP_HIP(ctx, hipGetLastError());
         if (b->nh > 0) {
             ProfScope pw(ctx, "assemble_wake");
             dim3 g2((unsigned)((tv.M + 255) / 256));
             hipLaunchKernelGGL((k_assemble_wake<double, T>), g2, dim3(256), 0, ctx->stream, horse_view(b),
-                               b->kernel_offset, b->kernel_cutoff, tv, G, ld, col0);
+                               b->kernel_offset, b->kernel_cutoff, tv, G, ld, col0, G32);
             FP_HIP(ctx, hipGetLastError());
         }
     } else {
         ProfScope ps(ctx, sizeof(T) == 8 ? "assemble_source_f64" : "assemble_source_f32");
         if (b->nvp == 4)
             hipLaunchKernelGGL((k_assemble_source<double, T, 4, TM>), grid, dim3(256), 0, ctx->stream, pan, tv, G, ld,
-                               col0);
+                               col0, G32);
         else
             hipLaunchKernelGGL((k_assemble_source<double, T, 8, TM>), grid, dim3(256), 0, ctx->stream, pan, tv, G, ld,
-                               col0);
+                               col0, G32);
         FP_HIP(ctx, hipGetLastError());
     }
     return FP_OK;
 }
-template int fp_launch_assemble<double>(fp_ctx *, const fp_body *, const TargetView &, double *, int64_t, int64_t);
+template int fp_launch_assemble<double>(fp_ctx *, const fp_body *, const TargetView &, double *, int64_t, int64_t,
+                                        float *);
 
 template <typename T>
 int fp_launch_assemble_wake(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G, int64_t ld, int64_t col0)
@@ -294,7 +308,7 @@ int fp_launch_assemble_wake(fp_ctx *ctx, const fp_body *b, const TargetView &tv,
     ProfScope pw(ctx, "assemble_wake");
     dim3 g2((unsigned)((tv.M + 255) / 256));
     hipLaunchKernelGGL((k_assemble_wake<double, T>), g2, dim3(256), 0, ctx->stream, horse_view(b), b->kernel_offset,
-                       b->kernel_cutoff, tv, G, ld, col0);
+                       b->kernel_cutoff, tv, G, ld, col0, (float *)nullptr);
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }
@@ -332,7 +346,8 @@ int fp_launch_assemble_queue(fp_ctx *ctx, const AssemblyItem *d_items, int nitem
 #undef FP_Q
     return fp_fail(ctx, FP_ERR_ARG, "internal: unsupported assembly tile");
 }
-template int fp_launch_assemble<float>(fp_ctx *, const fp_body *, const TargetView &, float *, int64_t, int64_t);
+template int fp_launch_assemble<float>(fp_ctx *, const fp_body *, const TargetView &, float *, int64_t, int64_t,
+                                       float *);
 
 // ----------------------------------------------------------------------------
 // Field sums: one lane per target, panels in LDS.

@@ -177,9 +177,11 @@ struct AssemblyItem {
 // ---- launchers (fp_influence.hip) ---------------------------------------
 // geometry of one body (normals, CPs, areas, Hess-Smith frames)
 int fp_launch_geometry(fp_ctx *ctx, fp_body *b);
-// AIC column block of one body into a row-major matrix (T = float|double)
+// AIC column block of one body into a row-major matrix (T = float|double); with
+// G32 set, the same entries rounded to FP32 go to G32 (same ld) as well
 template <typename T>
-int fp_launch_assemble(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G, int64_t ld, int64_t col0);
+int fp_launch_assemble(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G, int64_t ld, int64_t col0,
+                       float *G32 = nullptr);
 // wake horseshoes of one body only (added into its columns)
 template <typename T>
 int fp_launch_assemble_wake(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G, int64_t ld, int64_t col0);
