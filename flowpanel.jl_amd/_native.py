@@ -44,7 +44,7 @@ EXPORTS = [
     "fp_body_set_freestream", "fp_body_set_strength", "fp_body_get_strength", "fp_solve_bodies",
     "fp_assemble", "fp_mat_upload", "fp_mat_size", "fp_mat_download", "fp_mat_destroy", "fp_solve_mat",
     "fp_mat_getrf", "fp_uind", "fp_phi", "fp_calcfield_U", "fp_calcfield_cp", "fp_ctx_profile",
-    "fp_ctx_profile_get", "fp_ctx_copy_calibrate", "fp_body_nnodes", "fp_body_set_nodes",
+    "fp_ctx_profile_only", "fp_ctx_profile_get", "fp_ctx_copy_calibrate", "fp_body_nnodes", "fp_body_set_nodes",
     "fp_liftdrag_opts_default", "fp_lift_drag", "fp_lift_drag_adjoint", "fp_solve_bodies_multi",
     "fp_batch_create", "fp_batch_solve", "fp_batch_destroy", "fp_batch_autotune_get", "fp_solve_bodies_lsq",
     "fp_calcfield_ugradmu",
@@ -153,6 +153,7 @@ def lib():
     L.fp_calcfield_U.argtypes = [_vp, pp, cint, _d]
     L.fp_calcfield_cp.argtypes = [_vp, _d, i64, dbl, _i64, i64, cint, cint, _d]
     L.fp_ctx_profile.argtypes = [_vp, cint]
+    L.fp_ctx_profile_only.argtypes = [_vp, ctypes.c_char_p]
     L.fp_ctx_profile_get.argtypes = [_vp, ctypes.c_char_p, _i64, _d]
     L.fp_ctx_copy_calibrate.argtypes = [_vp, i64]
     L.fp_solve_bodies_multi.argtypes = [_vp, pp, cint, _d, i64, ctypes.POINTER(SolveOpts), _d,
@@ -196,9 +197,12 @@ class Context:
     def synchronize(self):
         self.check(lib().fp_ctx_synchronize(self.handle))
 
-    def profile(self, enable=True):
-        """Start (clearing) or stop per-kernel device timing."""
-        self.check(lib().fp_ctx_profile(self.handle, int(bool(enable))))
+    def profile(self, enable=True, only=None):
+        """Start (clearing) or stop per-kernel device timing; `only` restricts it to one kernel class."""
+        if enable and only:
+            self.check(lib().fp_ctx_profile_only(self.handle, only.encode()))
+        else:
+            self.check(lib().fp_ctx_profile(self.handle, int(bool(enable))))
 
     def profile_get(self, kernel):
         """(launches, total device ms) recorded for one kernel class."""

@@ -1062,6 +1062,17 @@ int fp_ctx_profile(fp_ctx *ctx, int enable)
     prof_fold(ctx);
     if (enable) ctx->prof_acc.clear();
     ctx->prof = enable != 0;
+    ctx->prof_only.clear();
+    return FP_OK;
+}
+
+int fp_ctx_profile_only(fp_ctx *ctx, const char *kernel)
+{
+    if (!ctx) return FP_ERR_ARG;
+    prof_fold(ctx);
+    ctx->prof_acc.clear();
+    ctx->prof = true;
+    ctx->prof_only = kernel ? kernel : "";
     return FP_OK;
 }
 

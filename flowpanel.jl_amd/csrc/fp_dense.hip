@@ -1083,12 +1083,27 @@ template int fp_getrs<float>(fp_ctx *, const float *, int64_t, int64_t, const in
 // y = alpha A x + beta y (row-major A, FP64 vectors).  One wave per 4 rows:
 // every lane loads 4 consecutive x values once and reuses them for the 4 rows,
 // A is read with 16-byte (float4) or 32-byte (2 x double2) vector loads.
+// 16-byte loads of 4 matrix entries; `stream` marks them non-temporal (each
+// entry of a Krylov matvec is read once per launch and the matrix is far
+// larger than the caches: measured 0.558 against 0.616 ms for the 30k FP32 AIC).
+typedef float fp_f4v __attribute__((ext_vector_type(4)));
+typedef double fp_d2v __attribute__((ext_vector_type(2)));
 template <typename T> struct Vec4;
 template <> struct Vec4<float> {
     static __device__ __forceinline__ void load(const float *p, double *v)
     {
         const float4 q = *reinterpret_cast<const float4 *>(p);
         v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    }
+    // raw non-temporal load kept in the storage type until the dot product
+    typedef fp_f4v raw;
+    static __device__ __forceinline__ raw stream(const float *p)
+    {
+        return __builtin_nontemporal_load(reinterpret_cast<const fp_f4v *>(p));
+    }
+    static __device__ __forceinline__ double dot(const raw &q, const double2 &xa, const double2 &xb)
+    {
+        return (double)q.x * xa.x + (double)q.y * xa.y + (double)q.z * xb.x + (double)q.w * xb.y;
     }
 };
 template <> struct Vec4<double> {
@@ -1098,12 +1113,27 @@ template <> struct Vec4<double> {
         const double2 b = *reinterpret_cast<const double2 *>(p + 2);
         v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
     }
+    struct raw { fp_d2v a, b; };
+    static __device__ __forceinline__ raw stream(const double *p)
+    {
+        raw q;
+        q.a = __builtin_nontemporal_load(reinterpret_cast<const fp_d2v *>(p));
+        q.b = __builtin_nontemporal_load(reinterpret_cast<const fp_d2v *>(p + 2));
+        return q;
+    }
+    static __device__ __forceinline__ double dot(const raw &q, const double2 &xa, const double2 &xb)
+    {
+        return q.a.x * xa.x + q.a.y * xa.y + q.b.x * xb.x + q.b.y * xb.y;
+    }
 };
 
 // Rows per wave R of the Krylov GEMV: each x load is reused by R rows.  At
-// small n two rows keep twice as many waves in flight (measured on gfx950 at
-// n = 10,000: 0.076 ms with R = 2 against 0.110 ms with R = 4); at n = 30,000
-// there are waves enough and R = 4 is faster (0.64 against 0.68 ms).
+// small n two rows keep more waves in flight (measured on gfx950 at n = 10,000:
+// 0.076 ms with R = 2 against 0.110 ms with R = 4); at n = 30,000 there are
+// waves enough and, with non-temporal loads, R = 8 is fastest (back-to-back
+// launches: 0.532 ms against 0.540 for R = 6, 0.561 for R = 4, 0.609 for R = 2;
+// inside the Krylov loop 0.59 ms, 0.71 -> 0.77 of 8 TB/s with the 0.636 ms of
+// plain loads).
 constexpr int GEMVM_R = 4;  // rows per wave of the multi-vector GEMV
 
 template <typename T, bool XVEC, int GEMV_R>
@@ -1127,14 +1157,12 @@ __global__ __launch_bounds__(256) void k_gemv(const T *__restrict__ A, int64_t n
         } else {
             xa.x = x[c]; xa.y = x[c + 1]; xb.x = x[c + 2]; xb.y = x[c + 3];
         }
+        // rows past the end re-read row r0 (in bounds, result discarded): no branches around the loads
+        typename Vec4<T>::raw v[GEMV_R];
 #pragma unroll
-        for (int r = 0; r < GEMV_R; ++r) {
-            if (r < nr) {
-                double v[4];
-                Vec4<T>::load(A + (r0 + r) * ld + c, v);
-                acc[r] += v[0] * xa.x + v[1] * xa.y + v[2] * xb.x + v[3] * xb.y;
-            }
-        }
+        for (int r = 0; r < GEMV_R; ++r) v[r] = Vec4<T>::stream(A + (r0 + (r < nr ? r : 0)) * ld + c);
+#pragma unroll
+        for (int r = 0; r < GEMV_R; ++r) acc[r] += Vec4<T>::dot(v[r], xa, xb);
     }
     for (int64_t c = n4 + lane; c < n; c += 64)
         for (int r = 0; r < nr; ++r) acc[r] += (double)A[(r0 + r) * ld + c] * x[c];
@@ -1168,7 +1196,7 @@ int fp_gemv(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, const double *x, dou
     if (ld % 4) return fp_fail(ctx, FP_ERR_ARG, "gemv needs ld % 4 == 0");
     ProfScope ps(ctx, sizeof(T) == 8 ? "gemv_f64" : "gemv_f32");
     return n <= 20000 ? gemv_launch<T, 2>(ctx, A, n, ld, x, y, alpha, beta)
-                      : gemv_launch<T, 4>(ctx, A, n, ld, x, y, alpha, beta);
+                      : gemv_launch<T, 8>(ctx, A, n, ld, x, y, alpha, beta);
 }
 template int fp_gemv<double>(fp_ctx *, const double *, int64_t, int64_t, const double *, double *, double, double);
 template int fp_gemv<float>(fp_ctx *, const float *, int64_t, int64_t, const double *, double *, double, double);

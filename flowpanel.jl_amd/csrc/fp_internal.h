@@ -43,6 +43,7 @@ struct fp_ctx {
     size_t scratch_bytes = 0;
     // per-kernel device timing (fp_ctx_profile)
     bool prof = false;
+    std::string prof_only;  // when set, only scopes of this name record events
     std::vector<fp_prof_rec> prof_pending;
     std::vector<hipEvent_t> prof_pool;  // recycled events
     std::map<std::string, std::pair<int64_t, double>> prof_acc;
@@ -55,7 +56,7 @@ struct ProfScope {
     int idx = -1;
     ProfScope(fp_ctx *ctx, const char *name) : c(ctx)
     {
-        if (!c->prof) return;
+        if (!c->prof || (!c->prof_only.empty() && c->prof_only != name)) return;
         fp_prof_rec r;
         r.name = name;
         if (!take(r.a)) return;

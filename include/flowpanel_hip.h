@@ -305,6 +305,9 @@ int fp_lift_drag_adjoint(fp_ctx *ctx, fp_body *const *bodies, int nbodies, const
  * recorded so far for one kernel class (e.g. "gemv_f32", "gemm_f64",
  * "assemble_vr_f32", "field_matvec_f64", "panel_tslu", "bj_apply"). */
 int fp_ctx_profile(fp_ctx *ctx, int enable);
+/* As fp_ctx_profile(ctx, 1), but only launches of one kernel class record
+ * events (two per launch), so the timing adds little host work to a timed run. */
+int fp_ctx_profile_only(fp_ctx *ctx, const char *kernel);
 int fp_ctx_profile_get(fp_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms);
 /* One device-to-device copy of `bytes` through the HIP runtime's copy kernel: a
  * plain copy of known size to calibrate HBM byte counters (FETCH_SIZE/WRITE_SIZE). */
