@@ -1,0 +1,61 @@
+"""Dense kernels at the block boundaries of the device LU and triangular solves.
+
+getrf works in 512-column outer blocks of 32-column sub-panels (the outer
+U12 = L11^-1 A12 update through an explicit inverse), getrs in 128-row
+blocks with 128-row update slices, the block-Jacobi inverses by Gauss-Jordan
+in registers up to 256 rows (LU + inversion above).  Sizes straddle each of
+those boundaries.  Bars: P A = L U to 1e-12 of max|A| (FP64), solutions to
+1e-11 relative against numpy (FP64 LU and FP32 LU + FP64 refinement), GMRES-IR
+to its 1e-12 target.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
+
+
+@pytest.mark.parametrize("n", [1, 2, 31, 32, 33, 128, 129, 511, 512, 513, 1025])
+def test_getrf_block_boundaries(fp, n):
+    rng = np.random.default_rng(1000 + n)
+    A = rng.standard_normal((n, n))
+    LU, ipiv = fp.getrf(A)
+    L = np.tril(LU, -1) + np.eye(n)
+    Uu = np.triu(LU)
+    PA = A.copy()
+    for i, p in enumerate(ipiv - 1):
+        PA[[i, p]] = PA[[p, i]]
+    assert np.abs(PA - L @ Uu).max() / np.abs(A).max() < 1e-12
+    # (tournament pivoting picks each pivot from the winners of a reduction tree, so
+    # |l_ij| <= 1 -- the GEPP bound -- need not hold; P A = L U and the solves are the bar)
+
+
+@pytest.mark.parametrize("n", [1, 33, 128, 129, 257, 640])
+@pytest.mark.parametrize("k", [1, 3])
+@pytest.mark.parametrize("solver", ["solve_ludiv", "solve_lu32_ir"])
+def test_getrs_block_boundaries(fp, n, k, solver):
+    rng = np.random.default_rng(2000 + 7 * n + k)
+    A = rng.standard_normal((n, n)) + 2.0 * np.sqrt(n) * np.eye(n)
+    B = rng.standard_normal((n, k))
+    X = np.zeros((n, k))
+    getattr(fp, solver)(X, A, B)
+    assert rel(X, np.linalg.solve(A, B)) < 1e-11, (n, k, solver)
+
+
+@pytest.mark.parametrize("block", [1, 64, 250, 256, 300])
+def test_block_jacobi_paths(fp, block):
+    # blocks <= 256: Gauss-Jordan inverses from A; above: LU per block, then inverses
+    rng = np.random.default_rng(3000 + block)
+    n = 777
+    A = rng.standard_normal((n, n)) * 0.3 / np.sqrt(n) + 3.0 * np.eye(n)
+    for s in range(0, n, 50):                       # diagonal blocks for the preconditioner to capture
+        e = min(n, s + 50)
+        A[s:e, s:e] += rng.standard_normal((e - s, e - s)) * 0.2
+    b = rng.standard_normal(n)
+    y = np.zeros(n)
+    fp.solve_gmres_ir(y, A, b, precond_block=block, tol=1e-12)
+    assert np.linalg.norm(A @ y - b) / np.linalg.norm(b) < 1e-11
+    assert rel(y, np.linalg.solve(A, b)) < 1e-9
