@@ -52,7 +52,7 @@ def build(verbose=False, jobs=None):
         if failed:
             raise RuntimeError("HIP compilation failed:\n" + "\n".join(failed))
         tmp_lib = LIB + ".tmp"
-        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp_lib] + objs, verbose)
+        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp_lib] + objs, verbose)
         os.replace(tmp_lib, LIB)
     return LIB
 

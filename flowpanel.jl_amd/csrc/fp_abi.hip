@@ -6,6 +6,8 @@
 // All arithmetic runs in the kernels of fp_influence.hip and fp_dense.hip.
 #include "fp_internal.h"
 
+#include <thread>
+
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -1865,6 +1867,65 @@ void fp_liftdrag_opts_default(fp_liftdrag_opts *o, int solver)
 }
 
 // Forward (and optionally reverse) pass of the lift/drag objective.
+// The two adjoint systems A^T lambda_m = g_m (m = CL, CDi) with a Krylov solver:
+// system 0 on the context stream, system 1 on a second stream (default
+// priority) owned by a child context and driven from a host thread.  Each has
+// its own solver workspace and preconditioner; the matrices are shared
+// read-only.  The child stream first waits for everything already queued on
+// the context stream.
+static int transpose_solves_concurrent(fp_ctx *ctx, const SolveArgs &at, const fp_solve_opts &so,
+                                       const double *Ares, const double *g, double *lam, int64_t M,
+                                       fp_solve_stats &ts)
+{
+    fp_ctx child;
+    child.device = ctx->device;
+    child.num_cu = ctx->num_cu;
+    FP_HIP(ctx, hipStreamCreateWithFlags(&child.stream, hipStreamNonBlocking));
+    hipEvent_t ready = nullptr;
+    hipError_t e = hipEventCreateWithFlags(&ready, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(ready, ctx->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(child.stream, ready, 0);
+    if (e != hipSuccess) {
+        if (ready) (void)hipEventDestroy(ready);
+        (void)hipStreamDestroy(child.stream);
+        FP_HIP(ctx, e);
+    }
+    int st1 = FP_OK;
+    fp_solve_stats ts1{};
+    std::thread worker([&]() {
+        if (hipSetDevice(child.device) != hipSuccess) {
+            st1 = fp_fail(&child, FP_ERR_DEVICE, "hipSetDevice failed in the adjoint worker thread");
+            return;
+        }
+        SolveArgs a1 = at;
+        DenseOp<double> op1(&child, Ares, at.n, at.ld);
+        a1.resid = &op1;
+        {
+            Solver s1(&child, a1, so);
+            st1 = s1.setup(ts1);
+            if (!st1) st1 = s1.solve(g + M, lam + M, ts1);
+        }
+        if (hipStreamSynchronize(child.stream) != hipSuccess && !st1)
+            st1 = fp_fail(&child, FP_ERR_DEVICE, "adjoint worker stream synchronisation failed");
+    });
+    int st0;
+    {
+        Solver s0(ctx, at, so);
+        st0 = s0.setup(ts);
+        if (!st0) st0 = s0.solve(g, lam, ts);
+    }
+    worker.join();
+    (void)hipEventDestroy(ready);
+    (void)hipStreamDestroy(child.stream);
+    child.stream = nullptr;
+    if (st0) return st0;
+    if (st1) return fp_fail(ctx, st1, child.err.c_str());
+    ts.iterations += ts1.iterations;
+    ts.refinements += ts1.refinements;
+    ts.residual = std::max(ts.residual, ts1.residual);
+    return FP_OK;
+}
+
 static int liftdrag_run(fp_ctx *ctx, fp_body *const *bodies, int nb, const double *U, const fp_liftdrag_opts *lo,
                         bool adjoint, double *CLo, double *CDo, double *dCL_dn, double *dCD_dn, double *dCL_dU,
                         double *dCD_dU, fp_liftdrag_stats *stats_out)
@@ -2111,10 +2172,17 @@ static int liftdrag_run(fp_ctx *ctx, fp_body *const *bodies, int nb, const doubl
         }
         opT.reset(new DenseOp<double>(ctx, Ares, n, ld));
         at.resid = opT.get();
-        Solver ts_solver(ctx, at, so);
-        st = ts_solver.setup(ts);
-        for (int m = 0; m < 2 && !st; ++m)
-            st = ts_solver.solve(g.as<double>() + m * M, lam.as<double>() + m * M, ts);
+        if (so.solver == FP_SOLVER_GMRES || so.solver == FP_SOLVER_GMRES_IR) {
+            // Krylov solves touch the matrices read-only: the two coefficients' systems run at the
+            // same time, the second on a stream of its own driven by a host thread (each is bound by
+            // kernel latency at these sizes, so the two fill each other's gaps)
+            st = transpose_solves_concurrent(ctx, at, so, Ares, g.as<double>(), lam.as<double>(), M, ts);
+        } else {
+            Solver ts_solver(ctx, at, so);
+            st = ts_solver.setup(ts);
+            for (int m = 0; m < 2 && !st; ++m)
+                st = ts_solver.solve(g.as<double>() + m * M, lam.as<double>() + m * M, ts);
+        }
         if (st) { if (stats_out) *stats_out = S; return st; }
         S.iterations_adjoint = ts.iterations;
         S.ms_transpose_solve = tt.stop();
