@@ -262,11 +262,12 @@ double dnorm_host(fp_ctx *ctx, const double *x, int64_t n, double *part, double 
 // discarded (their columns are never used).
 struct Gmres {
     fp_ctx *ctx; int64_t n; int m;
-    DBuf V, Z, w, h, part, tmp, arn;
+    DBuf V, Z, w, h, part, tmp, arn, cgs;
     int init(fp_ctx *c, int64_t n_, int m_)
     {
         ctx = c; n = n_; m = m_;
         int st;
+        if ((st = dalloc(ctx, cgs, fp_cgs_work_doubles(n, m) * sizeof(double)))) return st;
         if ((st = dalloc(ctx, V, (size_t)n * (m + 1) * sizeof(double)))) return st;
         if ((st = dalloc(ctx, w, n * sizeof(double)))) return st;
         if ((st = dalloc(ctx, tmp, n * sizeof(double)))) return st;
@@ -319,15 +320,26 @@ struct Gmres {
                     zin = zp;
                 }
                 if ((st = A.apply(zin, wp))) return st;
-                // CGS2 and the norm of what is left
-                if ((st = fp_mdot(ctx, Vp, n, j + 1, wp, n, pp, hp))) return st;
-                if ((st = fp_vupdate(ctx, Vp, n, j + 1, hp, wp, n))) return st;
-                if ((st = fp_mdot(ctx, Vp, n, j + 1, wp, n, pp, hp + (m + 2)))) return st;
-                if ((st = fp_vupdate(ctx, Vp, n, j + 1, hp + (m + 2), wp, n))) return st;
-                if ((st = fp_mdot(ctx, wp, n, 1, wp, n, pp, hp + 2 * (m + 2) - 1))) return st;
-                if ((st = fp_arnoldi_step(ctx, hp, m, j, bnorm, dH, dcs, dsn, dg, drel, dhn, wp,
-                                          Vp + (int64_t)(j + 1) * n, n)))
-                    return st;
+                // CGS2 and the norm of what is left: two fused passes (restart <= FP_CGS_MAXV - 1), or
+                // separate dot, update and norm launches
+                if (m + 1 <= FP_CGS_MAXV) {
+                    double *c1 = cgs.as<double>(), *c2 = c1 + (size_t)FP_CGS_PARTS * (m + 2);
+                    double *cn = c2 + (size_t)FP_CGS_PARTS * (m + 2);
+                    if ((st = fp_cgs_pass(ctx, Vp, n, j + 1, wp, n, c1, nullptr))) return st;
+                    if ((st = fp_cgs_pass(ctx, Vp, n, j + 1, wp, n, c2, cn))) return st;
+                    if ((st = fp_arnoldi_step2(ctx, c1, c2, cn, m, j, bnorm, dH, dcs, dsn, dg, drel, dhn, wp,
+                                               Vp + (int64_t)(j + 1) * n, n)))
+                        return st;
+                } else {
+                    if ((st = fp_mdot(ctx, Vp, n, j + 1, wp, n, pp, hp))) return st;
+                    if ((st = fp_vupdate(ctx, Vp, n, j + 1, hp, wp, n))) return st;
+                    if ((st = fp_mdot(ctx, Vp, n, j + 1, wp, n, pp, hp + (m + 2)))) return st;
+                    if ((st = fp_vupdate(ctx, Vp, n, j + 1, hp + (m + 2), wp, n))) return st;
+                    if ((st = fp_mdot(ctx, wp, n, 1, wp, n, pp, hp + 2 * (m + 2) - 1))) return st;
+                    if ((st = fp_arnoldi_step(ctx, hp, m, j, bnorm, dH, dcs, dsn, dg, drel, dhn, wp,
+                                              Vp + (int64_t)(j + 1) * n, n)))
+                        return st;
+                }
                 ++j;
                 ++*its;
                 if (j < next_check && j < m && *its < maxit) continue;

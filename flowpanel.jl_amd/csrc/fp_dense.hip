@@ -2236,6 +2236,142 @@ int fp_arnoldi_step(fp_ctx *ctx, const double *hbuf, int m, int j, double bnorm,
     return FP_OK;
 }
 
+// ----------------------------------------------------------------------------
+// Fused classical Gram-Schmidt passes for the Arnoldi step (fewer launches per
+// Krylov iteration).  A pass is k_mdot_partial with CGS_PARTS row chunks, then
+// k_cgs_update: every workgroup sums the CGS_PARTS partials of each coefficient
+// itself (independent loads), subtracts V h from its rows of w and, on the second
+// pass, leaves the sum of squares of its rows of the new w.  k_arnoldi_step2
+// finishes both coefficient vectors and the norm in one workgroup and applies the
+// Givens rotations.  Requires nv <= CGS_MAXV.
+constexpr int CGS_PARTS = FP_CGS_PARTS, CGS_MAXV = FP_CGS_MAXV;
+
+size_t fp_cgs_work_doubles(int64_t n, int m) { return (size_t)2 * CGS_PARTS * (m + 2) + (n + 255) / 256 + 64; }
+
+template <bool NORM>
+__global__ __launch_bounds__(256) void k_cgs_update(const double *__restrict__ V, int64_t ldv, int nv,
+                                                    const double *__restrict__ part, int P,
+                                                    double *__restrict__ w, int64_t n, double *__restrict__ normpart)
+{
+    __shared__ double hl[CGS_MAXV];
+    __shared__ double red[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int c = tid; c < nv; c += 256) {
+        const double *pc = part + (int64_t)c * P;
+        double a = 0.0;
+#pragma unroll
+        for (int q = 0; q < CGS_PARTS; ++q)
+            if (q < P) a += pc[q];
+        hl[c] = a;
+    }
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * 256 + tid;
+    double acc = 0.0;
+    if (i < n) {
+        acc = w[i];
+#pragma unroll 8
+        for (int c = 0; c < nv; ++c) acc -= V[(int64_t)c * ldv + i] * hl[c];
+        w[i] = acc;
+    }
+    if (NORM) {
+        double sq = acc * acc;
+        for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
+        if (lane == 0) red[wave] = sq;
+        __syncthreads();
+        if (tid == 0) normpart[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    }
+}
+
+int fp_cgs_pass(fp_ctx *ctx, const double *V, int64_t ldv, int nv, double *w, int64_t n, double *part,
+                double *normpart)
+{
+    if (nv > CGS_MAXV) return fp_fail(ctx, FP_ERR_ARG, "Gram-Schmidt pass: too many basis vectors");
+    const int P = (int)std::max<int64_t>(1, std::min<int64_t>(CGS_PARTS, (n + 255) / 256));
+    hipLaunchKernelGGL(k_mdot_partial, dim3(P, (nv + MDOT_G - 1) / MDOT_G), dim3(256), 0, ctx->stream, V, ldv, nv,
+                       (const double *)w, n, part);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (normpart)
+        hipLaunchKernelGGL(k_cgs_update<true>, grid, dim3(256), 0, ctx->stream, V, ldv, nv, (const double *)part, P, w,
+                           n, normpart);
+    else
+        hipLaunchKernelGGL(k_cgs_update<false>, grid, dim3(256), 0, ctx->stream, V, ldv, nv, (const double *)part, P,
+                           w, n, normpart);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+// Hessenberg column j from the two passes' partials (h1 + h2) and the norm of
+// what is left, then the Givens rotations as in k_arnoldi_step.
+__global__ __launch_bounds__(256) void k_arnoldi_step2(const double *__restrict__ part1,
+                                                       const double *__restrict__ part2, int P,
+                                                       const double *__restrict__ normpart, int nblk, int m, int j,
+                                                       double bnorm, double *__restrict__ H, double *__restrict__ cs,
+                                                       double *__restrict__ sn, double *__restrict__ g,
+                                                       double *__restrict__ relh, double *__restrict__ hnv)
+{
+    __shared__ double hc[CGS_MAXV], lcs[CGS_MAXV], lsn[CGS_MAXV];
+    __shared__ double red[256];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < j; i += 256) {
+        lcs[i] = cs[i];
+        lsn[i] = sn[i];
+    }
+    for (int i = tid; i <= j; i += 256) {
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int q = 0; q < CGS_PARTS; ++q)
+            if (q < P) {
+                a += part1[(int64_t)i * P + q];
+                b += part2[(int64_t)i * P + q];
+            }
+        hc[i] = a + b;
+    }
+    double s = 0.0;
+    for (int q = tid; q < nblk; q += 256) s += normpart[q];
+    red[tid] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    // the earlier rotations applied to the column in LDS (one thread: they are sequential)
+    if (tid == 0) {
+        const double hn = sqrt(fmax(0.0, red[0]));
+        for (int i = 0; i < j; ++i) {
+            const double a = hc[i], b = hc[i + 1];
+            hc[i] = lcs[i] * a + lsn[i] * b;
+            hc[i + 1] = -lsn[i] * a + lcs[i] * b;
+        }
+        const double hj = hc[j];
+        const double den = hypot(hj, hn);
+        const double c = den > 0 ? hj / den : 1.0, sj = den > 0 ? hn / den : 0.0;
+        cs[j] = c;
+        sn[j] = sj;
+        hc[j] = den;
+        const double gj = g[j];
+        g[j + 1] = -sj * gj;
+        g[j] = c * gj;
+        relh[j] = fabs(sj * gj) / bnorm;
+        hnv[j] = hn;
+    }
+    __syncthreads();
+    for (int i = tid; i <= j; i += 256) H[(int64_t)i * m + j] = hc[i];
+}
+
+int fp_arnoldi_step2(fp_ctx *ctx, const double *part1, const double *part2, const double *normpart, int m, int j,
+                     double bnorm, double *H, double *cs, double *sn, double *g, double *relh, double *hnv,
+                     const double *w, double *vnext, int64_t n)
+{
+    const int P = (int)std::max<int64_t>(1, std::min<int64_t>(CGS_PARTS, (n + 255) / 256));
+    const int nblk = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(k_arnoldi_step2, dim3(1), dim3(256), 0, ctx->stream, part1, part2, P, normpart, nblk, m, j,
+                       bnorm, H, cs, sn, g, relh, hnv);
+    hipLaunchKernelGGL(k_arnoldi_normalize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, w,
+                       (const double *)hnv, j, vnext, n);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
 int fp_arnoldi_solve(fp_ctx *ctx, const double *H, int m, int jc, const double *g, double *y)
 {
     hipLaunchKernelGGL(k_arnoldi_solve, dim3(1), dim3(64), 0, ctx->stream, H, m, jc, g, y);

@@ -259,6 +259,17 @@ int fp_fill(fp_ctx *ctx, double *x, int64_t n, double v);   // x[:] = v (a kerne
 // multi-dot out[c] = sum_i V[c*ldv + i] w[i] (part: 256*nv doubles)
 int fp_mdot(fp_ctx *ctx, const double *V, int64_t ldv, int nv, const double *w, int64_t n, double *part,
             double *out);
+// fused classical Gram-Schmidt pass: w -= V (V^T w) through per-chunk partials
+// (FP_CGS_PARTS chunks; at most FP_CGS_MAXV basis vectors);
+// with normpart set, per-256-row sums of squares of the new w (Arnoldi norm)
+constexpr int FP_CGS_PARTS = 32, FP_CGS_MAXV = 256;
+size_t fp_cgs_work_doubles(int64_t n, int m);
+int fp_cgs_pass(fp_ctx *ctx, const double *V, int64_t ldv, int nv, double *w, int64_t n, double *part,
+                double *normpart);
+// Hessenberg column j from the two passes' partials and the norm; Givens update; v_{j+1} = w / h_{j+1,j}
+int fp_arnoldi_step2(fp_ctx *ctx, const double *part1, const double *part2, const double *normpart, int m, int j,
+                     double bnorm, double *H, double *cs, double *sn, double *g, double *relh, double *hnv,
+                     const double *w, double *vnext, int64_t n);
 
 // ---- surface post-processing (fp_post.hip) --------------------------------
 // out (3 x N point-major) += -1/2 grad(mu) of one solved body, cell-centred
