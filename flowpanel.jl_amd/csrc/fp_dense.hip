@@ -1697,28 +1697,44 @@ __global__ __launch_bounds__(256) void k_bj_apply_inv(const float *__restrict__ 
                                                       const double *__restrict__ x, double *__restrict__ y,
                                                       int accumulate)
 {
+    // four rows per wave (each row in its own diagonal block), up to 8 entries per lane
+    // and row (block <= 512): 32 independent loads per lane in flight
     const int lane = threadIdx.x & 63;
-    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (r >= n) return;
-    const int64_t b = r / block, b0 = b * block;
-    const int bs = (int)min((int64_t)block, n - b0);
-    const float *row = inv + b * block * block + (r - b0) * block;
-    double acc = 0;
-    // block <= 512: at most 8 entries per lane, all loads issued together
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 4;
+    if (r0 >= n) return;
+    double acc[4];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int c = lane + 64 * q;
-        if (c < bs) acc += (double)row[c] * x[b0 + c];
+    for (int q = 0; q < 4; ++q) {
+        const int64_t r = min(r0 + q, n - 1);
+        const int64_t b = r / block, b0 = b * block;
+        const int bs = (int)min((int64_t)block, n - b0);
+        const float *row = inv + b * block * block + (r - b0) * block;
+        double a = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int c = lane + 64 * k;
+            if (c < bs) a += (double)row[c] * x[b0 + c];
+        }
+        acc[q] = a;
     }
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if (lane == 0) y[r] = accumulate ? y[r] + acc : acc;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o);
+    if (lane < 4 && r0 + lane < n) {
+        double a = acc[0];
+#pragma unroll
+        for (int q = 1; q < 4; ++q)
+            if (lane == q) a = acc[q];
+        const int64_t r = r0 + lane;
+        y[r] = accumulate ? y[r] + a : a;
+    }
 }
 
 int fp_bj_apply_inv(fp_ctx *ctx, const float *inv, int64_t n, int block, const double *x, double *y, int accumulate)
 {
     ProfScope ps(ctx, "bj_apply");
-    hipLaunchKernelGGL(k_bj_apply_inv, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, inv, n, block, x, y,
-                       accumulate);
+    hipLaunchKernelGGL(k_bj_apply_inv, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, ctx->stream, inv, n, block, x,
+                       y, accumulate);
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }
