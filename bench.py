@@ -37,8 +37,8 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per k_gemv<float> launch at 30,000 panels from rocprofv3 FETCH_SIZE/WRITE_SIZE, scaled by the
-# same counters on a 1 GiB plain copy (FETCH x2.0, WRITE x1.0): profiles/gemv_f32_bwb30k_hbm_counters.md
-GEMV_TRAFFIC_30K = 3.6040e9   # HBM bytes per launch of the 30k FP32 GEMV (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
+# same counters on a 1 GiB plain copy (FETCH x2.0, WRITE x1.0)
+GEMV_TRAFFIC_30K = 3.6040e9
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 

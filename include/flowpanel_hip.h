@@ -4,8 +4,9 @@
  * Drop-in boundary for the hot path of FLOWPanel.jl: body constructors,
  * `solve`, the solver-function protocol, `Uind!`/`phi!`, `calcfield_U`/
  * `calcfield_Cp`.  The reference binds these through Julia multiple dispatch;
- * a Julia host calls this library with `ccall` (see INTEGRATION.md for the
- * binding), the Python host mirror in flowpanel.jl_amd/ calls it through ctypes.
+ * a Julia host calls this library with `ccall` (one `ccall` per entry point,
+ * plain pointers and sizes), the Python host mirror in flowpanel.jl_amd/ calls
+ * it through ctypes.
  *
  * Conventions (the reference's):
  *   - 3-vector arrays are column-major 3 x n float64 (x, y, z contiguous per

@@ -1,7 +1,7 @@
 """Parity of the HIP path against the CPU oracle (FP64 restatement of the reference).
 
-Tolerances: FP64 paths 1e-10 relative (BASELINE north-star bound for the FP64
-path, measured against max |.| of the quantity); FP32 assembly 2e-5 relative.
+Tolerances: FP64 paths 1e-10 relative (the agreement asked of the FP64 path,
+measured against max |.| of the quantity); FP32 assembly 2e-5 relative.
 """
 import numpy as np
 import pytest
