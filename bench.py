@@ -345,7 +345,7 @@ def main():
     ap.add_argument("--batch-only", action="store_true", help="run only the batched legs (prints their record)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
 
     import __graft_entry__
