@@ -232,14 +232,14 @@ def batch_legs(fp, ctx, reps=3):
     return out
 
 
-def adjoint_leg(fp, ctx, fd_coords=64, reps=3):
+def adjoint_leg(fp, ctx, fd_coords=64, reps=3, block=200):
     """Lift/drag adjoint of the 10,000-panel wing; returns the JSON record."""
     grid, sh = fp.geometry.loft_wing(airfoil="2412", n_chord=100, n_span=100, span=8.0, root_chord=1.0, taper=1.0,
                                      sweep_deg=0.0, twist_root_deg=0.0, twist_tip_deg=-3.0)
     body = fp.RigidWakeBody(grid, sh)
     U = fp.direction(4.0)
     kw = dict(probe_offset=0.02, probe_length=1.0, span_dir=(0, 1, 0), ref_area=8.0)
-    mixed = dict(solver=fp.solve_gmres_ir, solver_optargs={"precond_block": 100, "coarse_block": 100, "tol": 1e-12,
+    mixed = dict(solver=fp.solve_gmres_ir, solver_optargs={"precond_block": block, "coarse_block": 100, "tol": 1e-12,
                                                            "inner_tol": 1e-5, "restart": 100})
     log("bench: adjoint warmup (%d panels)" % body.ncells)
     fp.lift_drag_gradient(body, U, **kw, **mixed)
@@ -325,8 +325,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--chord", type=int, default=250)
     ap.add_argument("--span", type=int, default=120)
-    ap.add_argument("--block", type=int, default=250, help="block-Jacobi block size")
-    ap.add_argument("--coarse", type=int, default=-1, help="coarse chunk (default: one chordwise strip)")
+    ap.add_argument("--block", type=int, default=-1, help="block-Jacobi block size (default: four chordwise strips)")
+    ap.add_argument("--coarse", type=int, default=-1, help="coarse chunk (default: the block-Jacobi block)")
     ap.add_argument("--tol", type=float, default=1e-8,
                     help="FP64 relative residual target of the mixed solve (max |dCp| ~1e-8 against FP64 LU, "
                          "100x inside the 1e-6 bound)")
@@ -362,8 +362,9 @@ def main():
     body, U, D = build_body(fp, args.chord, args.span, alpha)
     N = body.ncells
     fp.solvers.set_freestream(body, U, D, D)
-    coarse = args.chord if args.coarse < 0 else args.coarse
-    mixed_opts = {"precond_block": args.block, "coarse_block": coarse, "tol": args.tol, "inner_tol": args.inner_tol,
+    block = 4 * args.chord if args.block < 0 else args.block
+    coarse = block if args.coarse < 0 else args.coarse
+    mixed_opts = {"precond_block": block, "coarse_block": coarse, "tol": args.tol, "inner_tol": args.inner_tol,
                   "restart": 100, "residual_mode": 1}
 
     def run(solver, opts):
@@ -469,7 +470,7 @@ def main():
         "ms_per_step": round(t_mixed, 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "f32 AIC + f64 refinement", "data": "synthetic",
         "config": {"workload": "bwb-%dx%d vortex-ring + rigid wake, AoA %.1f, GMRES-IR (FP32 AIC, block-Jacobi %d)"
-                               % (args.chord, args.span, alpha, args.block),
+                               % (args.chord, args.span, alpha, block),
                    "panels": N, "shedding_edges": body.nsheddings},
         "fp64_lu_ms": round(t_fp64, 3), "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
         "cp_err_max": cp_err, "gamma_rel_err": gam_err, "tol": args.tol, "inner_tol": args.inner_tol,

@@ -183,35 +183,29 @@ struct Precond {
     virtual int apply(const double *x, double *y) = 0;
     virtual ~Precond() {}
 };
-// Block-Jacobi: LU of every diagonal block; for blocks <= 512 the explicit
-// inverses are formed once so that each application is one batched GEMV.
+// Block-Jacobi: the explicit inverses of the diagonal blocks (block <= 1024)
+// are formed once, so that each application is one batched GEMV.
 // With coarse_block > 0 a Galerkin coarse correction on chunk indicators Z is
 // applied first (two-level, multiplicative):  y = Z c + B (v - A Z c),
 // c = (Z^T A Z)^-1 Z^T v.
 struct BJPrecond : Precond {
-    fp_ctx *ctx; const float *BJ; const int *piv; int64_t n; int block;
-    DBuf inv, AZ, Einv, work, tc, r2, info;
-    bool has_inv = false;
+    fp_ctx *ctx; int64_t n; int block;
+    DBuf inv, gjs, AZ, Einv, work, tc, r2, cinfo;
     int cs = 0;
-    BJPrecond(fp_ctx *c, const float *b, const int *p, int64_t n_, int bl) : ctx(c), BJ(b), piv(p), n(n_), block(bl) {}
-    int setup(const void *A = nullptr, int prec = FP_F32, int64_t ld = 0, int coarse = 0, int *info_dev = nullptr)
+    BJPrecond(fp_ctx *c, int64_t n_, int bl) : ctx(c), n(n_), block(bl) {}
+    int setup(const void *A, int prec, int64_t ld, int coarse, int *info_dev)
     {
         int st;
-        if (!BJ) {
-            // no LU factors: inverses straight from A by Gauss-Jordan (block <= 256)
-            const int64_t nb = (n + block - 1) / block;
-            if ((st = dalloc(ctx, inv, (size_t)nb * block * block * sizeof(float)))) return st;
-            st = prec == FP_F64 ? fp_bj_gj_inverse<double>(ctx, (const double *)A, n, ld, block, inv.as<float>(), info_dev)
-                                : fp_bj_gj_inverse<float>(ctx, (const float *)A, n, ld, block, inv.as<float>(), info_dev);
-            if (st) return st;
-            has_inv = true;
-        } else if (block <= 512) {
-            const int64_t nb = (n + block - 1) / block;
-            if ((st = dalloc(ctx, inv, (size_t)nb * block * block * sizeof(float)))) return st;
-            if ((st = fp_bj_invert(ctx, BJ, piv, n, block, inv.as<float>()))) return st;
-            has_inv = true;
-        }
-        if (coarse > 0 && A && has_inv) {
+        const int64_t nb = (n + block - 1) / block;
+        if ((st = dalloc(ctx, inv, (size_t)nb * block * block * sizeof(float)))) return st;
+        const size_t sf = fp_bj_gj_scratch_floats(n, block);
+        if (sf && (st = dalloc(ctx, gjs, sf * sizeof(float)))) return st;
+        float *sp = sf ? gjs.as<float>() : nullptr;
+        st = prec == FP_F64
+                 ? fp_bj_gj_inverse<double>(ctx, (const double *)A, n, ld, block, inv.as<float>(), info_dev, sp)
+                 : fp_bj_gj_inverse<float>(ctx, (const float *)A, n, ld, block, inv.as<float>(), info_dev, sp);
+        if (st) return st;
+        if (coarse > 0) {
             cs = coarse;
             const int nc = (int)((n + cs - 1) / cs);
             if ((st = dalloc(ctx, AZ, (size_t)nc * n * sizeof(double)))) return st;
@@ -219,18 +213,17 @@ struct BJPrecond : Precond {
             if ((st = dalloc(ctx, work, (size_t)2 * nc * nc * sizeof(double)))) return st;
             if ((st = dalloc(ctx, tc, 2 * nc * sizeof(double)))) return st;
             if ((st = dalloc(ctx, r2, n * sizeof(double)))) return st;
-            if ((st = dalloc(ctx, info, sizeof(int)))) return st;
+            if ((st = dalloc(ctx, cinfo, sizeof(int)))) return st;
             st = prec == FP_F64 ? fp_coarse_setup<double>(ctx, (const double *)A, n, ld, cs, AZ.as<double>(),
-                                                          Einv.as<double>(), work.as<double>(), info.as<int>())
+                                                          Einv.as<double>(), work.as<double>(), cinfo.as<int>())
                                 : fp_coarse_setup<float>(ctx, (const float *)A, n, ld, cs, AZ.as<double>(),
-                                                         Einv.as<double>(), work.as<double>(), info.as<int>());
+                                                         Einv.as<double>(), work.as<double>(), cinfo.as<int>());
             if (st) return st;
         }
         return FP_OK;
     }
     int apply(const double *x, double *y) override
     {
-        if (!has_inv) return fp_bj_apply(ctx, BJ, piv, n, block, x, y);
         if (cs > 0) {
             int st = fp_coarse_apply(ctx, x, n, cs, AZ.as<double>(), Einv.as<double>(), tc.as<double>(), y,
                                      r2.as<double>());
@@ -449,7 +442,7 @@ struct Solver {
     fp_ctx *ctx;
     SolveArgs a;
     fp_solve_opts o;
-    DBuf perm, work, bj, bjp, info, r, d, part, dout;
+    DBuf perm, work, info, r, d, part, dout;
     std::unique_ptr<Gmres> g;
     std::unique_ptr<BJPrecond> M;
     std::unique_ptr<DenseOp<float>> A32op;
@@ -457,22 +450,10 @@ struct Solver {
 
     int bj_setup(const void *A, int prec, int blk)
     {
-        const int64_t n = a.n;
-        const int64_t nbk = (n + blk - 1) / blk;
         int st;
         if ((st = dalloc(ctx, info, sizeof(int)))) return st;
-        if (blk <= 256) {
-            M.reset(new BJPrecond(ctx, nullptr, nullptr, n, blk));
-            return M->setup(A, prec, a.ld, o.coarse_block, info.as<int>());
-        }
-        if ((st = dalloc(ctx, bj, (size_t)nbk * blk * blk * sizeof(float)))) return st;
-        if ((st = dalloc(ctx, bjp, (size_t)nbk * blk * sizeof(int)))) return st;
-        st = prec == FP_F64
-                 ? fp_bj_factor<double>(ctx, (const double *)A, n, a.ld, blk, bj.as<float>(), bjp.as<int>(), info.as<int>())
-                 : fp_bj_factor<float>(ctx, (const float *)A, n, a.ld, blk, bj.as<float>(), bjp.as<int>(), info.as<int>());
-        if (st) return st;
-        M.reset(new BJPrecond(ctx, bj.as<float>(), bjp.as<int>(), n, blk));
-        return M->setup(A, prec, a.ld, o.coarse_block);
+        M.reset(new BJPrecond(ctx, a.n, blk));
+        return M->setup(A, prec, a.ld, o.coarse_block, info.as<int>());
     }
 
     // factorisation / preconditioner set-up (stats.ms_factor, stats.info)

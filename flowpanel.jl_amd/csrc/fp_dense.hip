@@ -1322,115 +1322,62 @@ int fp_colnorm2(fp_ctx *ctx, const double *R, int64_t n, int64_t ld, int64_t k, 
 }
 
 // ----------------------------------------------------------------------------
-// Block-Jacobi: LU with partial pivoting of each diagonal block (one
-// workgroup per block, the block copied to BJ in FP32 and factored there).
-template <typename T>
-__global__ __launch_bounds__(1024) void k_bj_factor(const T *__restrict__ A, int64_t n, int64_t ld, int block,
-                                                    float *__restrict__ BJ, int *__restrict__ piv,
-                                                    int *__restrict__ info)
-{
-    __shared__ float red_v[16];
-    __shared__ int red_r[16];
-    __shared__ float prow[1024];
-    const int b = blockIdx.x;
-    const int64_t b0 = (int64_t)b * block;
-    const int bs = (int)min((int64_t)block, n - b0);
-    float *M = BJ + (int64_t)b * block * block;
-    int *pv = piv + (int64_t)b * block;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
-    for (int64_t e = tid; e < (int64_t)bs * bs; e += blockDim.x) {
-        const int r = (int)(e / bs), c = (int)(e % bs);
-        M[(int64_t)r * block + c] = (float)A[(b0 + r) * ld + b0 + c];
-    }
-    __syncthreads();
-    for (int c = 0; c < bs; ++c) {
-        float bv = -1.f;
-        int br = -1;
-        for (int r = c + tid; r < bs; r += blockDim.x) {
-            const float v = fabsf(M[(int64_t)r * block + c]);
-            if (v > bv) { bv = v; br = r; }
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            const float ov = __shfl_xor(bv, o);
-            const int orr = __shfl_xor(br, o);
-            if (ov > bv || (ov == bv && orr < br && orr >= 0)) { bv = ov; br = orr; }
-        }
-        if (lane == 0) { red_v[wave] = bv; red_r[wave] = br; }
-        __syncthreads();
-        bv = red_v[0]; br = red_r[0];
-        for (int w = 1; w < nw; ++w)
-            if (red_v[w] > bv || (red_v[w] == bv && red_r[w] < br && red_r[w] >= 0)) { bv = red_v[w]; br = red_r[w]; }
-        if (br < c || br >= bs) br = c;  // all-NaN column: keep the row
-        __syncthreads();
-        if (tid == 0) pv[c] = br;
-        if (br != c)
-            for (int e = tid; e < bs; e += blockDim.x) {
-                const float t = M[(int64_t)c * block + e];
-                M[(int64_t)c * block + e] = M[(int64_t)br * block + e];
-                M[(int64_t)br * block + e] = t;
-            }
-        __syncthreads();
-        for (int e = tid; e < bs; e += blockDim.x) prow[e] = M[(int64_t)c * block + e];
-        __syncthreads();
-        const float p = prow[c];
-        if (p == 0.f) {
-            if (tid == 0) atomicCAS(info, 0, b * block + c + 1);
-        } else {
-            const int w = bs - c - 1;
-            for (int64_t e = tid; e < (int64_t)w * (w + 1); e += blockDim.x) {
-                // (w rows) x (w + 1 columns: multiplier column c then c+1..)
-                const int r = c + 1 + (int)(e / (w + 1)), cc = c + (int)(e % (w + 1));
-                if (cc == c) continue;
-                const float l = M[(int64_t)r * block + c] / p;
-                M[(int64_t)r * block + cc] -= l * prow[cc];
-            }
-            __syncthreads();
-            for (int r = c + 1 + tid; r < bs; r += blockDim.x) M[(int64_t)r * block + c] /= p;
-        }
-        __syncthreads();
-    }
-}
-
-// Block-Jacobi inverses in one pass for blocks <= GJ_MAX: Gauss-Jordan with
-// partial pivoting, the block held in registers (1024 threads as a 32 x 32
-// grid, thread (tr, tc) holding rows tr + 32 i and columns tc + 32 j, 8 x 8
-// FP32 values).  Rows are never swapped: the pivot of step k is the unused
+// Block-Jacobi inverses by Gauss-Jordan with partial pivoting, the block held
+// in registers (TG x TG threads, thread (tr, tc) holding rows tr + TG i and
+// columns tc + TG j, 8 x 8 FP32 values: blocks up to 8 TG rows).  Rows are never swapped: the pivot of step k is the unused
 // row p_k with the largest |a(p, k)| (lowest row on ties, NaN counted as 0),
 // and the in-place result W satisfies inverse(pinv(r), p_c) = W(r, c), so the
 // final store scatters through the two index maps.  Per step: column k and
 // the scaled pivot row go through LDS, everything else stays in registers.
-constexpr int GJ_MAX = 256;
 
-template <typename T>
-__global__ __launch_bounds__(1024) void k_bj_gj(const T *__restrict__ A, int64_t n, int64_t ld, int block,
-                                                float *__restrict__ inv, int *__restrict__ info)
+//
+// Block b of the grid inverts the square sub-block of hs rows at offset
+// (off, off) of its diagonal block (hs < 0: the whole block, bs = min(block,
+// n - b * block) rows).  Source element (r, c) of block b is
+// A[b * a_bstride + (off + r) * a_ld + off + c]; the result goes to the same
+// place of inv (i_bstride, i_ld).  A and inv may be the same buffer: every
+// element is read into registers before the first barrier and written after
+// the last one.
+template <typename T, int TG>
+__global__ __launch_bounds__(TG * TG) void k_bj_gj(const T *A, int64_t a_bstride, int64_t a_ld, float *inv,
+                                                int64_t i_bstride, int64_t i_ld, int64_t n, int block, int off,
+                                                int hs, int *__restrict__ info)
 {
-    __shared__ float colk[GJ_MAX], prow[GJ_MAX];
-    __shared__ int used[GJ_MAX], prow_of[GJ_MAX], step_of[GJ_MAX];
-    __shared__ float wv[16];
-    __shared__ int wr[16];
+    constexpr int NR = 8 * TG, NW = TG * TG / 64;
+    __shared__ float colk[2][NR], prow[NR];
+    __shared__ int used[NR], prow_of[NR], step_of[NR];
+    __shared__ float wv[NW];
+    __shared__ int wr[NW];
     const int b = blockIdx.x;
     const int64_t b0 = (int64_t)b * block;
-    const int bs = (int)min((int64_t)block, n - b0);
-    const int tid = threadIdx.x, tr = tid >> 5, tc = tid & 31, lane = tid & 63, wave = tid >> 6;
+    const int bs = hs < 0 ? (int)min((int64_t)block, n - b0) : hs;
+    const T *src = A + (int64_t)b * a_bstride + off * a_ld + off;
+    const int tid = threadIdx.x, tr = tid / TG, tc = tid % TG, lane = tid & 63, wave = tid >> 6;
     float w[8][8];
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const int r = tr + 32 * i, c = tc + 32 * j;
-            w[i][j] = (r < bs && c < bs) ? (float)A[(b0 + r) * ld + b0 + c] : 0.f;
+            const int r = tr + TG * i, c = tc + TG * j;
+            w[i][j] = (r < bs && c < bs) ? (float)src[r * a_ld + c] : 0.f;
         }
-    if (tid < GJ_MAX) {
-        used[tid] = 0;
-        colk[tid] = 0.f;
-        prow[tid] = 0.f;
+    for (int r = tid; r < NR; r += TG * TG) {
+        used[r] = 0;
+        colk[0][r] = 0.f;
+        colk[1][r] = 0.f;
+        prow[r] = 0.f;
     }
     __syncthreads();
+    // Two barriers per step.  Column k goes to colk[k & 1] together with the
+    // pivot candidates of the TG threads holding it, so that writing column
+    // k + 1 never races with the update of step k still reading column k.
     for (int kb = 0; kb < 8; ++kb) {
-        for (int kk = 0; kk < 32; ++kk) {
-            const int k = kb * 32 + kk;
+        for (int kk = 0; kk < TG; ++kk) {
+            const int k = kb * TG + kk;
             if (k >= bs) break;
+            float *ck = colk[k & 1];
+            float v = -1.f;
+            int rr = NR;
             if (tc == kk) {
                 // column k is register column kb of these threads (selected, not indexed)
 #pragma unroll
@@ -1439,40 +1386,40 @@ __global__ __launch_bounds__(1024) void k_bj_gj(const T *__restrict__ A, int64_t
 #pragma unroll
                     for (int j = 0; j < 8; ++j)
                         if (j == kb) val = w[i][j];
-                    if (tr + 32 * i < bs) colk[tr + 32 * i] = val;
+                    const int r = tr + TG * i;
+                    if (r < bs) {
+                        ck[r] = val;
+                        float a = fabsf(val);
+                        if (!(a == a)) a = 0.f;
+                        if (!used[r] && (a > v || (a == v && r < rr))) { v = a; rr = r; }
+                    }
                 }
             }
-            __syncthreads();
-            float v = -1.f;
-            int rr = GJ_MAX;
-            if (tid < bs && !used[tid]) {
-                v = fabsf(colk[tid]);
-                if (!(v == v)) v = 0.f;
-                rr = tid;
-            }
-            for (int o = 32; o > 0; o >>= 1) {
+            // the holders of column k in this wave are lanes kk, kk + TG, ...
+#pragma unroll
+            for (int o = TG; o < 64; o <<= 1) {
                 const float v2 = __shfl_xor(v, o);
                 const int r2 = __shfl_xor(rr, o);
                 if (v2 > v || (v2 == v && r2 < rr)) { v = v2; rr = r2; }
             }
-            if (lane == 0) { wv[wave] = v; wr[wave] = rr; }
+            if (lane == kk) { wv[wave] = v; wr[wave] = rr; }
             __syncthreads();
             float bv = wv[0];
             int p = wr[0];
-            for (int q = 1; q < 16; ++q)
+            for (int q = 1; q < NW; ++q)
                 if (wv[q] > bv || (wv[q] == bv && wr[q] < p)) { bv = wv[q]; p = wr[q]; }
-            const float piv = colk[p];
-            if (piv == 0.f && tid == 0) atomicCAS(info, 0, b * block + k + 1);
+            const float piv = ck[p];
+            if (piv == 0.f && tid == 0) atomicCAS(info, 0, (int)(b0 + off + k + 1));
             const float ip = 1.f / piv;
-            if (tr == (p & 31)) {
-                const int ip5 = p >> 5;
+            if (tr == p % TG) {
+                const int ip5 = p / TG;
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     float val = 0.f;
 #pragma unroll
                     for (int i = 0; i < 8; ++i)
                         if (i == ip5) val = w[i][j];
-                    const int c = tc + 32 * j;
+                    const int c = tc + TG * j;
                     if (c < bs) prow[c] = c == k ? ip : val * ip;
                 }
             }
@@ -1482,223 +1429,203 @@ __global__ __launch_bounds__(1024) void k_bj_gj(const T *__restrict__ A, int64_t
             // (entries at or beyond bs hold zeros of colk/prow and are never stored)
             float f[8], pr[8];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) f[i] = colk[tr + 32 * i];
+            for (int i = 0; i < 8; ++i) f[i] = ck[tr + TG * i];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) pr[j] = prow[tc + 32 * j];
+            for (int j = 0; j < 8; ++j) pr[j] = prow[tc + TG * j];
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
-                const bool isp = tr + 32 * i == p;
+                const bool isp = tr + TG * i == p;
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
-                    const bool isk = tc + 32 * j == k;
+                    const bool isk = tc + TG * j == k;
                     float v = w[i][j] - f[i] * pr[j];
                     v = isk ? -f[i] * ip : v;
                     w[i][j] = isp ? pr[j] : v;
                 }
             }
-            __syncthreads();
         }
     }
-    float *I = inv + (int64_t)b * block * block;
+    float *I = inv + (int64_t)b * i_bstride + off * i_ld + off;
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const int r = tr + 32 * i, c = tc + 32 * j;
-            if (r < bs && c < bs) I[(int64_t)step_of[r] * block + prow_of[c]] = w[i][j];
+            const int r = tr + TG * i, c = tc + TG * j;
+            if (r < bs && c < bs) I[(int64_t)step_of[r] * i_ld + prow_of[c]] = w[i][j];
         }
 }
 
+// FP32 copy of every diagonal block, padded to block x block with the
+// identity (rows and columns at or beyond bs of a short last block), so that
+// every block has the same shape: the padded inverse is diag(inverse, I).
+// Workgroup (r, b) writes row r of block b.
 template <typename T>
-int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *inv, int *info_dev)
+__global__ __launch_bounds__(256) void k_bj_copy(const T *__restrict__ A, int64_t n, int64_t ld, int block,
+                                                 float *__restrict__ X)
 {
-    if (block > GJ_MAX) return fp_fail(ctx, FP_ERR_ARG, "Gauss-Jordan block inverse: block larger than 256");
-    const int nb = (int)((n + block - 1) / block);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, info_dev, (int64_t)1, 0);
-    ProfScope ps(ctx, "bj_invert");
-    hipLaunchKernelGGL(k_bj_gj<T>, dim3(nb), dim3(1024), 0, ctx->stream, A, n, ld, block, inv, info_dev);
-    FP_HIP(ctx, hipGetLastError());
-    return FP_OK;
+    const int b = blockIdx.y, r = blockIdx.x;
+    const int64_t b0 = (int64_t)b * block;
+    const int bs = (int)min((int64_t)block, n - b0);
+    float *x = X + (int64_t)b * block * block + (int64_t)r * block;
+    if (r < bs) {
+        const T *a = A + (b0 + r) * ld + b0;
+        for (int c = threadIdx.x; c < block; c += blockDim.x) x[c] = c < bs ? (float)a[c] : 0.f;
+    } else {
+        for (int c = threadIdx.x; c < block; c += blockDim.x) x[c] = c == r ? 1.f : 0.f;
+    }
 }
-template int fp_bj_gj_inverse<double>(fp_ctx *, const double *, int64_t, int64_t, int, float *, int *);
-template int fp_bj_gj_inverse<float>(fp_ctx *, const float *, int64_t, int64_t, int, float *, int *);
 
+// Operand of the batched block products: block b's matrix starts at
+// p + b * bstride + r0 * ld + c0.
+struct BjOp {
+    float *p;
+    int64_t bstride;
+    int ld, r0, c0;
+};
+
+// C_b = alpha A_b B_b (+ C_b when beta) for every diagonal block b (m x n,
+// inner dimension k), leaving the C entries whose row or column lies in
+// [skip0, skip1) untouched.  64 x 64 tiles of C per workgroup, 4 x 4 per
+// thread, K in LDS steps of 16.  C never overlaps A or B.
+__global__ __launch_bounds__(256) void k_bj_gemm(BjOp Ao, BjOp Bo, BjOp Co, int m, int nn, int kk, float alpha,
+                                                 int beta, int skip0, int skip1)
+{
+    __shared__ float As[16][65], Bs[16][65];
+    const int b = blockIdx.z;
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const float *Ap = Ao.p + b * Ao.bstride + (int64_t)Ao.r0 * Ao.ld + Ao.c0;
+    const float *Bp = Bo.p + b * Bo.bstride + (int64_t)Bo.r0 * Bo.ld + Bo.c0;
+    float *Cp = Co.p + b * Co.bstride + (int64_t)Co.r0 * Co.ld + Co.c0;
+    const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+    float acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+    const int ar = tid >> 2, ak = (tid & 3) * 4, bk = tid >> 4, bc = (tid & 15) * 4;
+    for (int k0 = 0; k0 < kk; k0 += 16) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = m0 + ar, kx = k0 + ak + e;
+            As[ak + e][ar] = (r < m && kx < kk) ? Ap[(int64_t)r * Ao.ld + kx] : 0.f;
+            const int kb = k0 + bk, c = n0 + bc + e;
+            Bs[bk][bc + e] = (kb < kk && c < nn) ? Bp[(int64_t)kb * Bo.ld + c] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            float a[4], bb[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[q][ty + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bb[j] = Bs[q][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * bb[j];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = m0 + ty + 16 * i, c = n0 + tx + 16 * j;
+            const bool skip = (r >= skip0 && r < skip1) || (c >= skip0 && c < skip1);
+            if (r < m && c < nn && !skip) {
+                float *cp = Cp + (int64_t)r * Co.ld + c;
+                *cp = alpha * acc[i][j] + (beta ? *cp : 0.f);
+            }
+        }
+}
+
+// Row chunk J = [j0, j0 + hj) of block b takes W (hj x block) outside the
+// chunk's own columns, the other rows take V (block x hj) in J's columns.
+// Workgroup (r, b) handles row r.
+__global__ __launch_bounds__(256) void k_bj_panels(float *__restrict__ X, int block, int j0, int hj,
+                                                   const float *__restrict__ W, const float *__restrict__ V)
+{
+    const int b = blockIdx.y, r = blockIdx.x;
+    float *x = X + (int64_t)b * block * block + (int64_t)r * block;
+    if (r >= j0 && r < j0 + hj) {
+        const float *w = W + (int64_t)b * hj * block + (int64_t)(r - j0) * block;
+        for (int c = threadIdx.x; c < block; c += blockDim.x)
+            if (c < j0 || c >= j0 + hj) x[c] = w[c];
+    } else {
+        const float *v = V + (int64_t)b * block * hj + (int64_t)r * hj;
+        for (int c = threadIdx.x; c < hj; c += blockDim.x) x[j0 + c] = v[c];
+    }
+}
+
+constexpr int GJ_CHUNK = 128;   // rows per Gauss-Jordan chunk (TG = 16 kernel, 256 threads)
+static int bj_chunks(int block) { return (block + GJ_CHUNK - 1) / GJ_CHUNK; }
+
+size_t fp_bj_gj_scratch_floats(int64_t n, int block)
+{
+    if (block <= GJ_CHUNK) return 0;
+    const int64_t nb = (n + block - 1) / block, h = (block + bj_chunks(block) - 1) / bj_chunks(block);
+    return (size_t)(2 * nb * h * block);
+}
+
+// Inverses of the diagonal blocks straight from A, block <= 1024.  Up to 128
+// rows: one Gauss-Jordan pass per block.  Larger blocks: blocked Gauss-Jordan
+// over q = ceil(block / 128) chunks J of h = ceil(block / q) rows (pivoting
+// within each chunk), on the identity-padded FP32 copy X:
+//   X_JJ = X_JJ^-1;  W = X_JJ X_J*;  X_RC -= X_RJ W_C (R, C outside J);
+//   V = -X_*J X_JJ;  X_J* = W, X_*J = V (outside X_JJ).
+// scratch holds W and V (fp_bj_gj_scratch_floats).
 template <typename T>
-int fp_bj_factor(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *BJ, int *piv, int *info_dev)
+int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *inv, int *info_dev,
+                     float *scratch)
 {
     if (block > 1024) return fp_fail(ctx, FP_ERR_ARG, "block-Jacobi block larger than 1024");
     const int nb = (int)((n + block - 1) / block);
+    const int64_t bb = (int64_t)block * block;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, info_dev, (int64_t)1, 0);
-    ProfScope ps(ctx, "bj_factor");
-    hipLaunchKernelGGL(k_bj_factor<T>, dim3(nb), dim3(1024), 0, ctx->stream, A, n, ld, block, BJ, piv, info_dev);
-    FP_HIP(ctx, hipGetLastError());
-    return FP_OK;
-}
-template int fp_bj_factor<double>(fp_ctx *, const double *, int64_t, int64_t, int, float *, int *, int *);
-template int fp_bj_factor<float>(fp_ctx *, const float *, int64_t, int64_t, int, float *, int *, int *);
-
-// y = M^-1 x with the block LU factors: one workgroup (4 waves) per block.
-// Rows are processed in 64-row slabs: the waves first subtract the already
-// solved part (row dot products), then one wave solves the 64 x 64 diagonal
-// triangle with its rows preloaded into registers.
-__global__ __launch_bounds__(256) void k_bj_apply(const float *__restrict__ BJ, const int *__restrict__ piv,
-                                                  int64_t n, int block, const double *__restrict__ x,
-                                                  double *__restrict__ y)
-{
-    __shared__ double v[1024];
-    const int b = blockIdx.x;
-    const int64_t b0 = (int64_t)b * block;
-    const int bs = (int)min((int64_t)block, n - b0);
-    const float *M = BJ + (int64_t)b * block * block;
-    const int *pv = piv + (int64_t)b * block;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int r = tid; r < bs; r += blockDim.x) v[r] = x[b0 + r];
-    __syncthreads();
-    if (tid == 0)
-        for (int c = 0; c < bs; ++c) {
-            const int p = pv[c];
-            if (p != c) { const double t = v[c]; v[c] = v[p]; v[p] = t; }
-        }
-    __syncthreads();
-    // forward: unit lower
-    for (int s0 = 0; s0 < bs; s0 += 64) {
-        const int sl = min(64, bs - s0);
-        for (int rr = wave; rr < sl; rr += 4) {
-            const int r = s0 + rr;
-            double acc = 0;
-            for (int q = lane; q < s0; q += 64) acc += (double)M[(int64_t)r * block + q] * v[q];
-            for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-            if (lane == 0) v[r] -= acc;
-        }
-        __syncthreads();
-        if (wave == 0) {
-            float lrow[64];
-            const int r = s0 + lane;
-#pragma unroll
-            for (int q = 0; q < 64; ++q) lrow[q] = (lane < sl && q < sl) ? M[(int64_t)r * block + s0 + q] : 0.f;
-            double yv = lane < sl ? v[r] : 0.0;
-#pragma unroll
-            for (int q = 0; q < 64; ++q) {
-                const double yq = __shfl(yv, q);
-                if (lane > q) yv -= (double)lrow[q] * yq;
-            }
-            if (lane < sl) v[r] = yv;
-        }
-        __syncthreads();
-    }
-    // backward: upper with diagonal
-    const int nsl = (bs + 63) / 64;
-    for (int si = nsl - 1; si >= 0; --si) {
-        const int s0 = si * 64;
-        const int sl = min(64, bs - s0);
-        for (int rr = wave; rr < sl; rr += 4) {
-            const int r = s0 + rr;
-            double acc = 0;
-            for (int q = s0 + 64 + lane; q < bs; q += 64) acc += (double)M[(int64_t)r * block + q] * v[q];
-            for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-            if (lane == 0) v[r] -= acc;
-        }
-        __syncthreads();
-        if (wave == 0) {
-            float urow[64];
-            const int r = s0 + lane;
-#pragma unroll
-            for (int q = 0; q < 64; ++q) urow[q] = (lane < sl && q < sl) ? M[(int64_t)r * block + s0 + q] : 0.f;
-            double yv = lane < sl ? v[r] : 0.0;
-#pragma unroll
-            for (int q = 63; q >= 0; --q) {
-                if (lane == q && q < sl) yv /= (double)urow[q];
-                const double yq = __shfl(yv, q);
-                if (lane < q && q < sl) yv -= (double)urow[q] * yq;
-            }
-            if (lane < sl) v[r] = yv;
-        }
-        __syncthreads();
-    }
-    for (int r = tid; r < bs; r += blockDim.x) y[b0 + r] = v[r];
-}
-
-int fp_bj_apply(fp_ctx *ctx, const float *BJ, const int *piv, int64_t n, int block, const double *x, double *y)
-{
-    const int nb = (int)((n + block - 1) / block);
-    ProfScope ps(ctx, "bj_apply");
-    hipLaunchKernelGGL(k_bj_apply, dim3(nb), dim3(256), 0, ctx->stream, BJ, piv, n, block, x, y);
-    FP_HIP(ctx, hipGetLastError());
-    return FP_OK;
-}
-
-// Explicit inverse of every diagonal block from its LU factors:
-// inv = U^-1 L^-1 P, 64 columns per workgroup, the columns in LDS.
-constexpr int BJI_MAX = 512;
-__global__ __launch_bounds__(256) void k_bj_invert(const float *__restrict__ BJ, const int *__restrict__ piv,
-                                                   int64_t n, int block, float *__restrict__ inv)
-{
-    __shared__ float X[BJI_MAX][65];
-    __shared__ int perm[BJI_MAX];
-    const int b = blockIdx.x;
-    const int c0 = blockIdx.y * 64;
-    const int64_t b0 = (int64_t)b * block;
-    const int bs = (int)min((int64_t)block, n - b0);
-    if (c0 >= bs) return;
-    const float *M = BJ + (int64_t)b * block * block;
-    const int *pv = piv + (int64_t)b * block;
-    const int tid = threadIdx.x;
-    if (tid == 0) {
-        for (int r = 0; r < bs; ++r) perm[r] = r;
-        for (int c = 0; c < bs; ++c) {
-            const int p = pv[c];
-            if (p != c && p >= 0 && p < bs) { const int t = perm[c]; perm[c] = perm[p]; perm[p] = t; }
-        }
-    }
-    __syncthreads();
-    for (int e = tid; e < bs * 64; e += blockDim.x) {
-        const int r = e / 64, c = e % 64;
-        X[r][c] = (perm[r] == c0 + c) ? 1.f : 0.f;
-    }
-    // forward, unit lower
-    for (int q = 0; q < bs - 1; ++q) {
-        __syncthreads();
-        const int rows = bs - q - 1;
-        for (int e = tid; e < rows * 64; e += blockDim.x) {
-            const int r = q + 1 + e / 64, c = e % 64;
-            X[r][c] -= M[(int64_t)r * block + q] * X[q][c];
-        }
-    }
-    // backward, upper
-    for (int q = bs - 1; q >= 0; --q) {
-        __syncthreads();
-        if (tid < 64) X[q][tid] /= M[(int64_t)q * block + q];
-        __syncthreads();
-        for (int e = tid; e < q * 64; e += blockDim.x) {
-            const int r = e / 64, c = e % 64;
-            X[r][c] -= M[(int64_t)r * block + q] * X[q][c];
-        }
-    }
-    __syncthreads();
-    float *I = inv + (int64_t)b * block * block;
-    for (int e = tid; e < bs * 64; e += blockDim.x) {
-        const int r = e / 64, c = e % 64;
-        if (c0 + c < bs) I[(int64_t)r * block + c0 + c] = X[r][c];
-    }
-}
-
-int fp_bj_invert(fp_ctx *ctx, const float *BJ, const int *piv, int64_t n, int block, float *inv)
-{
-    if (block > BJI_MAX) return fp_fail(ctx, FP_ERR_ARG, "block-Jacobi inverse needs block <= 512");
-    const int nb = (int)((n + block - 1) / block);
     ProfScope ps(ctx, "bj_invert");
-    hipLaunchKernelGGL(k_bj_invert, dim3(nb, (block + 63) / 64), dim3(256), 0, ctx->stream, BJ, piv, n, block, inv);
+    if (block <= GJ_CHUNK) {
+        hipLaunchKernelGGL((k_bj_gj<T, 16>), dim3(nb), dim3(256), 0, ctx->stream, A, (int64_t)block * (ld + 1), ld,
+                           inv, bb, (int64_t)block, n, block, 0, -1, info_dev);
+        FP_HIP(ctx, hipGetLastError());
+        return FP_OK;
+    }
+    if (!scratch) return fp_fail(ctx, FP_ERR_ARG, "internal: block inverse above 128 rows needs scratch");
+    const int q = bj_chunks(block), h = (block + q - 1) / q;
+    float *W = scratch, *V = scratch + (int64_t)nb * h * block;
+    hipLaunchKernelGGL(k_bj_copy<T>, dim3(block, nb), dim3(256), 0, ctx->stream, A, n, ld, block, inv);
+    const int tb = (block + 63) / 64;
+    for (int j = 0; j < q; ++j) {
+        const int j0 = j * h, hj = min(h, block - j0), th = (hj + 63) / 64;
+        hipLaunchKernelGGL((k_bj_gj<float, 16>), dim3(nb), dim3(256), 0, ctx->stream, (const float *)inv, bb,
+                           (int64_t)block, inv, bb, (int64_t)block, n, block, j0, hj, info_dev);
+        const BjOp XJJ{inv, bb, block, j0, j0}, XJrow{inv, bb, block, j0, 0}, XcolJ{inv, bb, block, 0, j0},
+            Xall{inv, bb, block, 0, 0}, Wo{W, (int64_t)hj * block, block, 0, 0}, Vo{V, (int64_t)block * hj, hj, 0, 0};
+        // W = X_JJ X_J*  (hj x block)
+        hipLaunchKernelGGL(k_bj_gemm, dim3(tb, th, nb), dim3(256), 0, ctx->stream, XJJ, XJrow, Wo, hj, block, hj,
+                           1.f, 0, 0, 0);
+        // X_RC -= X_RJ W_C outside row and column chunk J
+        hipLaunchKernelGGL(k_bj_gemm, dim3(tb, tb, nb), dim3(256), 0, ctx->stream, XcolJ, Wo, Xall, block, block, hj,
+                           -1.f, 1, j0, j0 + hj);
+        // V = -X_*J X_JJ  (block x hj)
+        hipLaunchKernelGGL(k_bj_gemm, dim3(th, tb, nb), dim3(256), 0, ctx->stream, XcolJ, XJJ, Vo, block, hj, hj,
+                           -1.f, 0, 0, 0);
+        hipLaunchKernelGGL(k_bj_panels, dim3(block, nb), dim3(256), 0, ctx->stream, inv, block, j0, hj,
+                           (const float *)W, (const float *)V);
+    }
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }
+template int fp_bj_gj_inverse<double>(fp_ctx *, const double *, int64_t, int64_t, int, float *, int *, float *);
+template int fp_bj_gj_inverse<float>(fp_ctx *, const float *, int64_t, int64_t, int, float *, int *, float *);
 
-// y_b = inv_b x_b for every diagonal block: one wave per row.
+// y_b = inv_b x_b for every diagonal block.
+template <int KM>
 __global__ __launch_bounds__(256) void k_bj_apply_inv(const float *__restrict__ inv, int64_t n, int block,
                                                       const double *__restrict__ x, double *__restrict__ y,
                                                       int accumulate)
 {
-    // four rows per wave (each row in its own diagonal block), up to 8 entries per lane
-    // and row (block <= 512): 32 independent loads per lane in flight
+    // four rows per wave (each row in its own diagonal block), up to KM entries per lane
+    // and row (block <= 64 KM): 4 KM independent loads per lane in flight
     const int lane = threadIdx.x & 63;
     const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 4;
     if (r0 >= n) return;
@@ -1711,7 +1638,7 @@ __global__ __launch_bounds__(256) void k_bj_apply_inv(const float *__restrict__ 
         const float *row = inv + b * block * block + (r - b0) * block;
         double a = 0;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < KM; ++k) {
             const int c = lane + 64 * k;
             if (c < bs) a += (double)row[c] * x[b0 + c];
         }
@@ -1732,9 +1659,13 @@ __global__ __launch_bounds__(256) void k_bj_apply_inv(const float *__restrict__ 
 
 int fp_bj_apply_inv(fp_ctx *ctx, const float *inv, int64_t n, int block, const double *x, double *y, int accumulate)
 {
+    if (block > 1024) return fp_fail(ctx, FP_ERR_ARG, "block-Jacobi block larger than 1024");
     ProfScope ps(ctx, "bj_apply");
-    hipLaunchKernelGGL(k_bj_apply_inv, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, ctx->stream, inv, n, block, x,
-                       y, accumulate);
+    const dim3 g((unsigned)((n + 15) / 16));
+    if (block <= 512)
+        hipLaunchKernelGGL(k_bj_apply_inv<8>, g, dim3(256), 0, ctx->stream, inv, n, block, x, y, accumulate);
+    else
+        hipLaunchKernelGGL(k_bj_apply_inv<16>, g, dim3(256), 0, ctx->stream, inv, n, block, x, y, accumulate);
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }

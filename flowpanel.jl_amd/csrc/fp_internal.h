@@ -227,15 +227,13 @@ int fp_gemv_multi(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, const double *
                   int64_t ldy);
 // out[c] = |R[:, c]|^2 (device)
 int fp_colnorm2(fp_ctx *ctx, const double *R, int64_t n, int64_t ld, int64_t k, double *out);
-// batched block-Jacobi LU of the diagonal blocks (FP32 factors)
+// Block-Jacobi inverses straight from A, block <= 1024: Gauss-Jordan in registers
+// up to 256 rows, above that blocked Gauss-Jordan over chunks of at most 256
+// rows (scratch of fp_bj_gj_scratch_floats floats), and their application
+size_t fp_bj_gj_scratch_floats(int64_t n, int block);
 template <typename T>
-int fp_bj_factor(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *BJ, int *piv, int *info_dev);
-int fp_bj_apply(fp_ctx *ctx, const float *BJ, const int *piv, int64_t n, int block, const double *x, double *y);
-// explicit block inverses (block <= 512) and their application
-int fp_bj_invert(fp_ctx *ctx, const float *BJ, const int *piv, int64_t n, int block, float *inv);
-// Block-Jacobi inverses straight from A (Gauss-Jordan in registers), block <= 256
-template <typename T>
-int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *inv, int *info_dev);
+int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *inv, int *info_dev,
+                     float *scratch);
 int fp_bj_apply_inv(fp_ctx *ctx, const float *inv, int64_t n, int block, const double *x, double *y, int accumulate);
 // Galerkin coarse space of chunk indicators: AZ = A Z (n x nc), Einv = (Z^T A Z)^-1
 template <typename T>

@@ -3,8 +3,8 @@
 getrf works in 512-column outer blocks of 32-column sub-panels (the outer
 U12 = L11^-1 A12 update through an explicit inverse), getrs in 128-row
 blocks with 128-row update slices, the block-Jacobi inverses by Gauss-Jordan
-in registers up to 256 rows (LU + inversion above).  Sizes straddle each of
-those boundaries.  Bars: P A = L U to 1e-12 of max|A| (FP64), solutions to
+in registers up to 128 rows and by blocked Gauss-Jordan over chunks of at most
+128 rows up to 1024.  Sizes straddle each of those boundaries.  Bars: P A = L U to 1e-12 of max|A| (FP64), solutions to
 1e-11 relative against numpy (FP64 LU and FP32 LU + FP64 refinement), GMRES-IR
 to its 1e-12 target.
 """
@@ -45,9 +45,10 @@ def test_getrs_block_boundaries(fp, n, k, solver):
     assert rel(X, np.linalg.solve(A, B)) < 1e-11, (n, k, solver)
 
 
-@pytest.mark.parametrize("block", [1, 64, 250, 256, 300])
+@pytest.mark.parametrize("block", [1, 64, 128, 129, 250, 256, 257, 300, 500, 512, 513, 777])
 def test_block_jacobi_paths(fp, block):
-    # blocks <= 256: Gauss-Jordan inverses from A; above: LU per block, then inverses
+    # blocks <= 128: Gauss-Jordan inverses from A; above: blocked Gauss-Jordan over
+    # ceil(block / 128) chunks (777: one block covering the whole matrix)
     rng = np.random.default_rng(3000 + block)
     n = 777
     A = rng.standard_normal((n, n)) * 0.3 / np.sqrt(n) + 3.0 * np.eye(n)
@@ -59,3 +60,29 @@ def test_block_jacobi_paths(fp, block):
     fp.solve_gmres_ir(y, A, b, precond_block=block, tol=1e-12)
     assert np.linalg.norm(A @ y - b) / np.linalg.norm(b) < 1e-11
     assert rel(y, np.linalg.solve(A, b)) < 1e-9
+
+
+@pytest.mark.parametrize("n,block", [(1001, 128), (1001, 250), (1001, 300), (1001, 500), (1130, 500), (1100, 512),
+                                     (777, 257), (1600, 750), (2100, 1000), (1500, 1024), (900, 777)])
+def test_block_jacobi_exact_on_block_diagonal(fp, n, block):
+    """On a matrix that is block diagonal with the preconditioner's own blocks,
+    M^-1 A = I up to the FP32 inverse: every inner solve ends after one
+    iteration.  A wrong chunk, panel or offset of the blocked inverse (blocks
+    above 256, identity-padded ragged last blocks) would show as many more."""
+    rng = np.random.default_rng(4000 + n + block)
+    A = np.zeros((n, n))
+    for s in range(0, n, block):
+        e = min(n, s + block)
+        A[s:e, s:e] = rng.standard_normal((e - s, e - s)) / np.sqrt(e - s) + 2.0 * np.eye(e - s)
+    b = rng.standard_normal(n)
+    y = np.zeros(n)
+    fp.solve_gmres_ir(y, A, b, precond_block=block, tol=1e-12)
+    st = dict(fp.solvers.last_stats)
+    assert rel(y, np.linalg.solve(A, b)) < 1e-10, st
+    assert st["iterations"] <= st["refinements"], st
+
+
+def test_block_jacobi_block_limit(fp):
+    A = np.eye(1100) * 2.0
+    with pytest.raises(Exception, match="1024"):
+        fp.solve_gmres_ir(np.zeros(1100), A, np.ones(1100), precond_block=1025)
