@@ -191,8 +191,14 @@ struct Precond {
 struct BJPrecond : Precond {
     fp_ctx *ctx; int64_t n; int block;
     DBuf inv, gjs, AZ, Einv, work, tc, r2, cinfo;
+    hipEvent_t fork = nullptr, join = nullptr;
     int cs = 0;
     BJPrecond(fp_ctx *c, int64_t n_, int bl) : ctx(c), n(n_), block(bl) {}
+    ~BJPrecond()
+    {
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+    }
     int setup(const void *A, int prec, int64_t ld, int coarse, int *info_dev)
     {
         int st;
@@ -201,11 +207,9 @@ struct BJPrecond : Precond {
         const size_t sf = fp_bj_gj_scratch_floats(n, block);
         if (sf && (st = dalloc(ctx, gjs, sf * sizeof(float)))) return st;
         float *sp = sf ? gjs.as<float>() : nullptr;
-        st = prec == FP_F64
-                 ? fp_bj_gj_inverse<double>(ctx, (const double *)A, n, ld, block, inv.as<float>(), info_dev, sp)
-                 : fp_bj_gj_inverse<float>(ctx, (const float *)A, n, ld, block, inv.as<float>(), info_dev, sp);
-        if (st) return st;
         if (coarse > 0) {
+            // the coarse space does not depend on the block inverses: it is built
+            // on the side stream while the main stream runs the Gauss-Jordan steps
             cs = coarse;
             const int nc = (int)((n + cs - 1) / cs);
             if ((st = dalloc(ctx, AZ, (size_t)nc * n * sizeof(double)))) return st;
@@ -214,12 +218,27 @@ struct BJPrecond : Precond {
             if ((st = dalloc(ctx, tc, 2 * nc * sizeof(double)))) return st;
             if ((st = dalloc(ctx, r2, n * sizeof(double)))) return st;
             if ((st = dalloc(ctx, cinfo, sizeof(int)))) return st;
-            st = prec == FP_F64 ? fp_coarse_setup<double>(ctx, (const double *)A, n, ld, cs, AZ.as<double>(),
+            if (!ctx->side) FP_HIP(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+            FP_HIP(ctx, hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+            FP_HIP(ctx, hipEventCreateWithFlags(&join, hipEventDisableTiming));
+            FP_HIP(ctx, hipEventRecord(fork, ctx->stream));
+            FP_HIP(ctx, hipStreamWaitEvent(ctx->side, fork, 0));
+            fp_ctx sc;
+            sc.device = ctx->device;
+            sc.num_cu = ctx->num_cu;
+            sc.stream = ctx->side;
+            st = prec == FP_F64 ? fp_coarse_setup<double>(&sc, (const double *)A, n, ld, cs, AZ.as<double>(),
                                                           Einv.as<double>(), work.as<double>(), cinfo.as<int>())
-                                : fp_coarse_setup<float>(ctx, (const float *)A, n, ld, cs, AZ.as<double>(),
+                                : fp_coarse_setup<float>(&sc, (const float *)A, n, ld, cs, AZ.as<double>(),
                                                          Einv.as<double>(), work.as<double>(), cinfo.as<int>());
-            if (st) return st;
+            if (st) return fp_fail(ctx, st, sc.err.c_str());
+            FP_HIP(ctx, hipEventRecord(join, ctx->side));
         }
+        st = prec == FP_F64
+                 ? fp_bj_gj_inverse<double>(ctx, (const double *)A, n, ld, block, inv.as<float>(), info_dev, sp)
+                 : fp_bj_gj_inverse<float>(ctx, (const float *)A, n, ld, block, inv.as<float>(), info_dev, sp);
+        if (st) return st;
+        if (join) FP_HIP(ctx, hipStreamWaitEvent(ctx->stream, join, 0));
         return FP_OK;
     }
     int apply(const double *x, double *y) override
@@ -1022,6 +1041,10 @@ void fp_ctx_destroy(fp_ctx *ctx)
     if (ctx->stream) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);
+    }
+    if (ctx->side) {
+        (void)hipStreamSynchronize(ctx->side);
+        (void)hipStreamDestroy(ctx->side);
     }
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     for (auto &r : ctx->prof_pending) {
@@ -1909,6 +1932,7 @@ static int transpose_solves_concurrent(fp_ctx *ctx, const SolveArgs &at, const f
     }
     worker.join();
     (void)hipEventDestroy(ready);
+    if (child.side) (void)hipStreamDestroy(child.side);
     (void)hipStreamDestroy(child.stream);
     child.stream = nullptr;
     if (st0) return st0;

@@ -1485,23 +1485,24 @@ struct BjOp {
 
 // C_b = alpha A_b B_b (+ C_b when beta) for every diagonal block b (m x n,
 // inner dimension k), leaving the C entries whose row or column lies in
-// [skip0, skip1) untouched.  64 x 64 tiles of C per workgroup, 4 x 4 per
-// thread, K in LDS steps of 16.  C never overlaps A or B.
+// [skip0, skip1) untouched.  64 x 64 tiles of C per workgroup on
+// v_mfma_f32_16x16x4f32 (each wave a 32 x 32 quarter, 2 x 2 instruction
+// tiles), K in LDS steps of 16.  C never overlaps A or B.
 __global__ __launch_bounds__(256) void k_bj_gemm(BjOp Ao, BjOp Bo, BjOp Co, int m, int nn, int kk, float alpha,
                                                  int beta, int skip0, int skip1)
 {
-    __shared__ float As[16][65], Bs[16][65];
+    __shared__ float As[16][68], Bs[16][68];
     const int b = blockIdx.z;
     const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
     const float *Ap = Ao.p + b * Ao.bstride + (int64_t)Ao.r0 * Ao.ld + Ao.c0;
     const float *Bp = Bo.p + b * Bo.bstride + (int64_t)Bo.r0 * Bo.ld + Bo.c0;
     float *Cp = Co.p + b * Co.bstride + (int64_t)Co.r0 * Co.ld + Co.c0;
-    const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
-    float acc[4][4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wm = wave >> 1, wn = wave & 1;
+    fp_f4 acc[2][2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+        for (int j = 0; j < 2; ++j) acc[i][j] = fp_f4{0.f, 0.f, 0.f, 0.f};
     const int ar = tid >> 2, ak = (tid & 3) * 4, bk = tid >> 4, bc = (tid & 15) * 4;
     for (int k0 = 0; k0 < kk; k0 += 16) {
 #pragma unroll
@@ -1513,28 +1514,32 @@ __global__ __launch_bounds__(256) void k_bj_gemm(BjOp Ao, BjOp Bo, BjOp Co, int 
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            float a[4], bb[4];
+        for (int ks = 0; ks < 16; ks += 4) {
+            float fa[2], fb[2];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[q][ty + 16 * i];
+            for (int i = 0; i < 2; ++i) fa[i] = As[ks + (lane >> 4)][wm * 32 + i * 16 + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) bb[j] = Bs[q][tx + 16 * j];
+            for (int j = 0; j < 2; ++j) fb[j] = Bs[ks + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * bb[j];
+                for (int j = 0; j < 2; ++j) acc[i][j] = MfmaOp<float>::run(fa[i], fb[j], acc[i][j]);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int r = m0 + ty + 16 * i, c = n0 + tx + 16 * j;
-            const bool skip = (r >= skip0 && r < skip1) || (c >= skip0 && c < skip1);
-            if (r < m && c < nn && !skip) {
-                float *cp = Cp + (int64_t)r * Co.ld + c;
-                *cp = alpha * acc[i][j] + (beta ? *cp : 0.f);
+        for (int j = 0; j < 2; ++j) {
+            const int c = n0 + wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = m0 + wm * 32 + i * 16 + MfmaOp<float>::row(lane, q);
+                const bool skip = (r >= skip0 && r < skip1) || (c >= skip0 && c < skip1);
+                if (r < m && c < nn && !skip) {
+                    float *cp = Cp + (int64_t)r * Co.ld + c;
+                    *cp = alpha * acc[i][j][q] + (beta ? *cp : 0.f);
+                }
             }
         }
 }

@@ -38,6 +38,9 @@ struct fp_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     int num_cu = 256;
+    // second stream for set-up work that runs beside the main stream (created on
+    // first use, destroyed with the context)
+    hipStream_t side = nullptr;
     // cached scratch (grown on demand, freed with the context)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
