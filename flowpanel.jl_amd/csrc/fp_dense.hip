@@ -1371,7 +1371,11 @@ __global__ __launch_bounds__(TG * TG) void k_bj_gj(const T *A, int64_t a_bstride
     // Two barriers per step.  Column k goes to colk[k & 1] together with the
     // pivot candidates of the TG threads holding it, so that writing column
     // k + 1 never races with the update of step k still reading column k.
+    // The register column kb of column k is a compile-time index (unrolled kb
+    // loop); only the pivot row p / TG is selected at run time.
+#pragma unroll
     for (int kb = 0; kb < 8; ++kb) {
+        if (kb * TG >= bs) break;
         for (int kk = 0; kk < TG; ++kk) {
             const int k = kb * TG + kk;
             if (k >= bs) break;
@@ -1379,13 +1383,9 @@ __global__ __launch_bounds__(TG * TG) void k_bj_gj(const T *A, int64_t a_bstride
             float v = -1.f;
             int rr = NR;
             if (tc == kk) {
-                // column k is register column kb of these threads (selected, not indexed)
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
-                    float val = 0.f;
-#pragma unroll
-                    for (int j = 0; j < 8; ++j)
-                        if (j == kb) val = w[i][j];
+                    const float val = w[i][kb];
                     const int r = tr + TG * i;
                     if (r < bs) {
                         ck[r] = val;
@@ -1411,8 +1411,9 @@ __global__ __launch_bounds__(TG * TG) void k_bj_gj(const T *A, int64_t a_bstride
             const float piv = ck[p];
             if (piv == 0.f && tid == 0) atomicCAS(info, 0, (int)(b0 + off + k + 1));
             const float ip = 1.f / piv;
-            if (tr == p % TG) {
-                const int ip5 = p / TG;
+            const bool prow_thread = tr == p % TG;
+            const int ip5 = p / TG;
+            if (prow_thread) {
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     float val = 0.f;
@@ -1425,23 +1426,27 @@ __global__ __launch_bounds__(TG * TG) void k_bj_gj(const T *A, int64_t a_bstride
             }
             if (tid == 0) { used[p] = 1; prow_of[k] = p; step_of[p] = k; }
             __syncthreads();
-            // branch-free update: the LDS operands are read first, then selects
-            // (entries at or beyond bs hold zeros of colk/prow and are never stored)
+            // w -= f pr everywhere, then column k = -f ip and row p = the scaled
+            // pivot row (entries at or beyond bs hold zeros of colk/prow and are
+            // never stored)
             float f[8], pr[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i) f[i] = ck[tr + TG * i];
 #pragma unroll
             for (int j = 0; j < 8; ++j) pr[j] = prow[tc + TG * j];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const bool isp = tr + TG * i == p;
+            for (int i = 0; i < 8; ++i)
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const bool isk = tc + TG * j == k;
-                    float v = w[i][j] - f[i] * pr[j];
-                    v = isk ? -f[i] * ip : v;
-                    w[i][j] = isp ? pr[j] : v;
-                }
+                for (int j = 0; j < 8; ++j) w[i][j] -= f[i] * pr[j];
+            if (tc == kk) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) w[i][kb] = -f[i] * ip;
+            }
+            if (prow_thread) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) w[i][j] = i == ip5 ? pr[j] : w[i][j];
             }
         }
     }
