@@ -388,7 +388,7 @@ def main():
     ctx.profile(True)
     run(fp.solve_gmres_ir, mixed_opts)
     ctx.synchronize()
-    prof = {k: ctx.profile_get(k) for k in ("gemv_f32", "gemv_f64", "assemble_vr_f64", "assemble_wake", "bj_factor",
+    prof = {k: ctx.profile_get(k) for k in ("gemv_f32", "gemv_f64", "assemble_vr_f64", "assemble_wake",
                                            "bj_invert", "bj_apply", "coarse_apply")}
     ctx.profile(False)
     gam_mixed = body.strength.copy()

@@ -84,7 +84,7 @@ typedef struct fp_solve_opts {
     int restart;       /* GMRES memory (Krylov basis size before restart)                 */
     int max_iters;     /* cap on the total number of GMRES iterations                     */
     int max_refine;    /* cap on refinement steps                                          */
-    int precond_block; /* block-Jacobi block size (GMRES_IR, GMRES)                       */
+    int precond_block; /* block-Jacobi block size, at most 1024 (GMRES_IR, GMRES)         */
     int coarse_block;  /* > 0: add a Galerkin coarse correction on the indicator vectors of
                           consecutive chunks of this many panels (e.g. one chordwise strip)   */
     int residual_mode; /* refinement residual r = b - A x in FP64 from: FP_RESID_STORED (the
