@@ -39,7 +39,7 @@ FP_KUTTA_PAIR = 2
 CHARLEN = {"unitary": 0, "bbox": 1, "maxdist": 2, "sqrtarea": 3}
 
 EXPORTS = [
-    "fp_solve_opts_default", "fp_ctx_create", "fp_ctx_destroy", "fp_ctx_last_error", "fp_ctx_synchronize",
+    "fp_solve_opts_default", "fp_ctx_create", "fp_ctx_destroy", "fp_ctx_last_error", "fp_ctx_synchronize", "fp_ctx_trim",
     "fp_body_create", "fp_body_destroy", "fp_body_ncells", "fp_body_nsheddings", "fp_body_geometry",
     "fp_body_set_freestream", "fp_body_set_strength", "fp_body_get_strength", "fp_solve_bodies",
     "fp_assemble", "fp_mat_upload", "fp_mat_size", "fp_mat_download", "fp_mat_destroy", "fp_solve_mat",
@@ -127,6 +127,7 @@ def lib():
     L.fp_ctx_last_error.argtypes = [_vp]
     L.fp_ctx_last_error.restype = ctypes.c_char_p
     L.fp_ctx_synchronize.argtypes = [_vp]
+    L.fp_ctx_trim.argtypes = [_vp]
     L.fp_body_create.argtypes = [_vp, _d, i64, _i64, _i64, i64, cint, cint, _i64, i64, dbl, cint, dbl, dbl, pp]
     L.fp_body_destroy.argtypes = [_vp]
     L.fp_body_destroy.restype = None
@@ -196,6 +197,10 @@ class Context:
 
     def synchronize(self):
         self.check(lib().fp_ctx_synchronize(self.handle))
+
+    def trim(self):
+        """Free the large device buffers the context keeps for re-use by the next call of the same size."""
+        self.check(lib().fp_ctx_trim(self.handle))
 
     def profile(self, enable=True, only=None):
         """Start (clearing) or stop per-kernel device timing; `only` restricts it to one kernel class."""

@@ -41,17 +41,23 @@ int fp_hip_fail(fp_ctx *ctx, hipError_t e, const char *what)
 
 namespace {
 
-// device buffer owned by a host scope
+// device buffer owned by a host scope; large ones go back to their context's
+// pool when released
 struct DBuf {
     void *p = nullptr;
+    fp_ctx *owner = nullptr;
+    size_t bytes = 0;
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
-    DBuf(DBuf &&o) noexcept : p(o.p) { o.p = nullptr; }
+    DBuf(DBuf &&o) noexcept : p(o.p), owner(o.owner), bytes(o.bytes) { o.p = nullptr; }
     ~DBuf() { reset(); }
     void reset()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (owner && bytes >= FP_POOL_MIN) fp_pool_put(owner, p, bytes);
+            else (void)hipFree(p);
+        }
         p = nullptr;
     }
     template <typename T> T *as() const { return (T *)p; }
@@ -59,10 +65,29 @@ struct DBuf {
 
 int dalloc(fp_ctx *ctx, DBuf &b, size_t bytes)
 {
-    if (b.p) { (void)hipFree(b.p); b.p = nullptr; }
+    b.reset();
     if (bytes == 0) bytes = 16;
+    if (bytes >= FP_POOL_MIN) {
+        for (size_t i = 0; i < ctx->pool.size(); ++i)
+            if (ctx->pool[i].bytes == bytes) {
+                b.p = ctx->pool[i].p;
+                ctx->pool.erase(ctx->pool.begin() + (std::ptrdiff_t)i);
+                ctx->pool_bytes -= bytes;
+                b.owner = ctx;
+                b.bytes = bytes;
+                return FP_OK;
+            }
+    }
     hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess && !ctx->pool.empty()) {
+        // out of memory with buffers held for re-use: release them and try once more
+        (void)hipGetLastError();
+        fp_pool_trim(ctx);
+        e = hipMalloc(&b.p, bytes);
+    }
     if (e != hipSuccess) { b.p = nullptr; return fp_hip_fail(ctx, e, "hipMalloc"); }
+    b.owner = ctx;
+    b.bytes = bytes;
     return FP_OK;
 }
 
@@ -196,6 +221,8 @@ struct BJPrecond : Precond {
     BJPrecond(fp_ctx *c, int64_t n_, int bl) : ctx(c), n(n_), block(bl) {}
     ~BJPrecond()
     {
+        // side-stream work must be over before its buffers can be handed out again
+        if (fork && ctx->side) (void)hipStreamSynchronize(ctx->side);
         if (fork) (void)hipEventDestroy(fork);
         if (join) (void)hipEventDestroy(join);
     }
@@ -996,6 +1023,26 @@ struct BatchImpl {
 }  // namespace
 
 // ============================================================================
+void fp_pool_put(fp_ctx *ctx, void *p, size_t bytes)
+{
+    ctx->pool.push_back({p, bytes});
+    ctx->pool_bytes += bytes;
+    while (ctx->pool_bytes > FP_POOL_MAX && !ctx->pool.empty()) {
+        (void)hipFree(ctx->pool.front().p);
+        ctx->pool_bytes -= ctx->pool.front().bytes;
+        ctx->pool.erase(ctx->pool.begin());
+    }
+}
+
+void fp_pool_trim(fp_ctx *ctx)
+{
+    for (auto &b : ctx->pool) (void)hipFree(b.p);
+    ctx->pool.clear();
+    ctx->pool_bytes = 0;
+}
+
+fp_ctx::~fp_ctx() { fp_pool_trim(this); }
+
 extern "C" {
 
 void fp_solve_opts_default(fp_solve_opts *o, int solver)
@@ -1031,6 +1078,16 @@ int fp_ctx_create(int device, fp_ctx **out)
     hipDeviceProp_t p;
     if (hipGetDeviceProperties(&p, device) == hipSuccess) c->num_cu = p.multiProcessorCount;
     *out = c.release();
+    return FP_OK;
+}
+
+int fp_ctx_trim(fp_ctx *ctx)
+{
+    if (!ctx) return FP_ERR_ARG;
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->side) FP_HIP(ctx, hipStreamSynchronize(ctx->side));
+    fp_pool_trim(ctx);
     return FP_OK;
 }
 

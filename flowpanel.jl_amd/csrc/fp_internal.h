@@ -33,6 +33,17 @@ struct fp_prof_rec {
     hipEvent_t a, b;
 };
 
+struct fp_pool_blk {
+    void *p;
+    size_t bytes;
+};
+// device buffers of at least FP_POOL_MIN bytes are kept by the context when the
+// call that used them ends, for the next request of the same size (repeated
+// solves of one body re-use their AIC storage without hipMalloc/hipFree); at
+// most FP_POOL_MAX bytes are kept, oldest released first
+constexpr size_t FP_POOL_MIN = (size_t)16 << 20;
+constexpr size_t FP_POOL_MAX = (size_t)64 << 30;
+
 struct fp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -50,7 +61,13 @@ struct fp_ctx {
     std::vector<fp_prof_rec> prof_pending;
     std::vector<hipEvent_t> prof_pool;  // recycled events
     std::map<std::string, std::pair<int64_t, double>> prof_acc;
+    // released large device buffers (fp_pool_put / fp_pool_trim)
+    std::vector<fp_pool_blk> pool;
+    size_t pool_bytes = 0;
+    ~fp_ctx();
 };
+void fp_pool_put(fp_ctx *ctx, void *p, size_t bytes);
+void fp_pool_trim(fp_ctx *ctx);
 
 // Brackets the launches of one kernel with events on the context stream when
 // profiling is on (fp_ctx_profile); the elapsed times are folded in on query.

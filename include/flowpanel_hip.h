@@ -113,6 +113,11 @@ int fp_ctx_create(int device, fp_ctx **out);
 void fp_ctx_destroy(fp_ctx *ctx);
 const char *fp_ctx_last_error(const fp_ctx *ctx);
 int fp_ctx_synchronize(fp_ctx *ctx);
+/* Free the device buffers (16 MiB and larger) that the context keeps after a call for re-use by the next
+ * call of the same size; the context keeps at most 64 GiB of them and also frees them when an allocation
+ * fails or the context is destroyed.  No counterpart in the reference (memory management of this
+ * library). */
+int fp_ctx_trim(fp_ctx *ctx);
 
 /* ---- bodies ------------------------------------------------------------- */
 /* NonLiftingBody{ConstantSource}(grid) (src/FLOWPanel_nonliftingbody.jl:32-72) and

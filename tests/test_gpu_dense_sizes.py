@@ -86,3 +86,23 @@ def test_block_jacobi_block_limit(fp):
     A = np.eye(1100) * 2.0
     with pytest.raises(Exception, match="1024"):
         fp.solve_gmres_ir(np.zeros(1100), A, np.ones(1100), precond_block=1025)
+
+
+def test_buffer_pool_reuse_and_trim(fp):
+    """Large work buffers (16 MiB and up) go back to the context after a call and
+    are handed to the next call of the same size: repeated solves must give the
+    same bits, before and after the context frees them (trim)."""
+    rng = np.random.default_rng(5000)
+    n = 1600                                         # FP64 matrix copies of 20 MB: pooled sizes
+    A = rng.standard_normal((n, n)) / np.sqrt(n) + 3.0 * np.eye(n)
+    b = rng.standard_normal(n)
+    y1 = fp.solve_gmres_ir(None, A, b, precond_block=400, coarse_block=400, tol=1e-12)
+    y2 = fp.solve_gmres_ir(None, A, b, precond_block=400, coarse_block=400, tol=1e-12)
+    assert np.array_equal(y1, y2)
+    fp.context().trim()
+    y3 = fp.solve_gmres_ir(None, A, b, precond_block=400, coarse_block=400, tol=1e-12)
+    assert np.array_equal(y1, y3)
+    assert rel(y1, np.linalg.solve(A, b)) < 1e-10
+    x1 = fp.solve_ludiv(None, A, b)
+    x2 = fp.solve_ludiv(None, A, b)
+    assert np.array_equal(x1, x2)
