@@ -318,14 +318,16 @@ struct Gmres {
         return FP_OK;
     }
     // solve A x = b; x overwritten. Returns iterations via *its, final relative residual estimate via *rel.
-    int solve(Operator &A, Precond *M, const double *b, double *x, double tol, int maxit, int *its, double *rel)
+    // bnorm_known >= 0: ||b|| as the caller already computed it (same reduction)
+    int solve(Operator &A, Precond *M, const double *b, double *x, double tol, int maxit, int *its, double *rel,
+              double bnorm_known = -1.0)
     {
         int st = FP_OK;
         double *Vp = V.as<double>(), *wp = w.as<double>(), *hp = h.as<double>(), *pp = part.as<double>();
         double *tp = tmp.as<double>(), *zp = Z.as<double>();
         double *dH = arn.as<double>(), *dcs = dH + (size_t)(m + 1) * m, *dsn = dcs + (m + 1), *dg = dsn + (m + 1);
         double *drel = dg + (m + 1), *dhn = drel + (m + 1), *dy = dhn + (m + 1);
-        const double bnorm = dnorm_host(ctx, b, n, pp, hp, &st);
+        const double bnorm = bnorm_known >= 0 ? bnorm_known : dnorm_host(ctx, b, n, pp, hp, &st);
         if (st) return st;
         *its = 0;
         *rel = 0;
@@ -341,8 +343,9 @@ struct Gmres {
                 if ((st = A.apply(x, wp))) return st;
                 if ((st = fp_residual_update(ctx, b, wp, n))) return st;
             }
+            // (first cycle: w = b, whose norm is bnorm)
+            const double beta = first ? bnorm : dnorm_host(ctx, wp, n, pp, hp, &st);
             first = false;
-            const double beta = dnorm_host(ctx, wp, n, pp, hp, &st);
             if (st) return st;
             *rel = beta / bnorm;
             if (*rel <= tol || *its >= maxit) return FP_OK;
@@ -351,13 +354,15 @@ struct Gmres {
             int j = 0, jc = -1, checked = 0, next_check = 4;
             double last_rel = *rel;
             int last_j = 0;
+            bool zready = false;  // M^-1 v_j already queued (before a host check)
             while (j < m && *its < maxit) {
                 const double *vj = Vp + (int64_t)j * n;
                 const double *zin = vj;
                 if (M) {
-                    if ((st = M->apply(vj, zp))) return st;
+                    if (!zready && (st = M->apply(vj, zp))) return st;
                     zin = zp;
                 }
+                zready = false;
                 if ((st = A.apply(zin, wp))) return st;
                 // CGS2 and the norm of what is left: two fused passes (restart <= FP_CGS_MAXV - 1), or
                 // separate dot, update and norm launches
@@ -382,6 +387,12 @@ struct Gmres {
                 ++j;
                 ++*its;
                 if (j < next_check && j < m && *its < maxit) continue;
+                // the next basis vector's preconditioner runs on the device while the
+                // host waits for the estimates (discarded if this check ends the cycle)
+                if (M && j < m && *its < maxit) {
+                    if ((st = M->apply(Vp + (int64_t)j * n, zp))) return st;
+                    zready = true;
+                }
                 const int cnt = j - checked;
                 FP_HIP(ctx, hipMemcpyAsync(relh.data() + checked, drel + checked, cnt * sizeof(double),
                                            hipMemcpyDeviceToHost, ctx->stream));
@@ -586,7 +597,7 @@ struct Solver {
         if (st) return st;
         FP_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
         FP_HIP(ctx, hipMemcpyAsync(rp, b, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-        double rel = bn > 0 ? 1.0 : 0.0;
+        double rel = bn > 0 ? 1.0 : 0.0, rnorm = bn;   // ||r|| of the current residual (r = b at first)
         int k = 0, its_total = 0;
         while (rel > o.tol && k < o.max_refine) {
             // correction d ~ A^-1 r in the low-precision solver
@@ -602,7 +613,7 @@ struct Solver {
                 // the correction only has to bring the residual below tol: once
                 // rel is small the inner solve may stop earlier
                 const double itol = std::max(o.inner_tol, std::min(0.5, 0.3 * o.tol / std::max(rel, 1e-300)));
-                if ((st = g->solve(*A32op, M.get(), rp, dp, itol, left, &its, &irel))) return st;
+                if ((st = g->solve(*A32op, M.get(), rp, dp, itol, left, &its, &irel, rnorm))) return st;
                 its_total += its;
             }
             if ((st = fp_axpby(ctx, 1.0, dp, 1.0, x, n))) return st;
@@ -612,6 +623,7 @@ struct Solver {
             const double rn = dnorm_host(ctx, rp, n, part.as<double>(), dout.as<double>(), &st);
             if (st) return st;
             rel = bn > 0 ? rn / bn : 0.0;
+            rnorm = rn;
             ++k;
         }
         stats.ms_solve += tm.stop();
