@@ -1099,6 +1099,7 @@ int fp_ctx_trim(fp_ctx *ctx)
     FP_HIP(ctx, hipSetDevice(ctx->device));
     FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->side) FP_HIP(ctx, hipStreamSynchronize(ctx->side));
+    if (ctx->side2) FP_HIP(ctx, hipStreamSynchronize(ctx->side2));
     fp_pool_trim(ctx);
     return FP_OK;
 }
@@ -1111,10 +1112,11 @@ void fp_ctx_destroy(fp_ctx *ctx)
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);
     }
-    if (ctx->side) {
-        (void)hipStreamSynchronize(ctx->side);
-        (void)hipStreamDestroy(ctx->side);
-    }
+    for (hipStream_t sd : {ctx->side, ctx->side2})
+        if (sd) {
+            (void)hipStreamSynchronize(sd);
+            (void)hipStreamDestroy(sd);
+        }
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     for (auto &r : ctx->prof_pending) {
         (void)hipEventDestroy(r.a);
@@ -2002,6 +2004,7 @@ static int transpose_solves_concurrent(fp_ctx *ctx, const SolveArgs &at, const f
     worker.join();
     (void)hipEventDestroy(ready);
     if (child.side) (void)hipStreamDestroy(child.side);
+    if (child.side2) (void)hipStreamDestroy(child.side2);
     (void)hipStreamDestroy(child.stream);
     child.stream = nullptr;
     if (st0) return st0;

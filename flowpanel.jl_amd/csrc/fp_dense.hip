@@ -1489,12 +1489,13 @@ struct BjOp {
 };
 
 // C_b = alpha A_b B_b (+ C_b when beta) for every diagonal block b (m x n,
-// inner dimension k), leaving the C entries whose row or column lies in
-// [skip0, skip1) untouched.  64 x 64 tiles of C per workgroup on
-// v_mfma_f32_16x16x4f32 (each wave a 32 x 32 quarter, 2 x 2 instruction
-// tiles), K in LDS steps of 16.  C never overlaps A or B.
+// inner dimension k), leaving untouched the C entries whose row or column lies
+// in [skip0, skip1) and those whose row and column both lie in [sq0, sq1).
+// 64 x 64 tiles of C per workgroup on v_mfma_f32_16x16x4f32 (each wave a
+// 32 x 32 quarter, 2 x 2 instruction tiles), K in LDS steps of 16.  C never
+// overlaps A or B.
 __global__ __launch_bounds__(256) void k_bj_gemm(BjOp Ao, BjOp Bo, BjOp Co, int m, int nn, int kk, float alpha,
-                                                 int beta, int skip0, int skip1)
+                                                 int beta, int skip0, int skip1, int sq0, int sq1)
 {
     __shared__ float As[16][68], Bs[16][68];
     const int b = blockIdx.z;
@@ -1540,7 +1541,8 @@ __global__ __launch_bounds__(256) void k_bj_gemm(BjOp Ao, BjOp Bo, BjOp Co, int 
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int r = m0 + wm * 32 + i * 16 + MfmaOp<float>::row(lane, q);
-                const bool skip = (r >= skip0 && r < skip1) || (c >= skip0 && c < skip1);
+                const bool skip = (r >= skip0 && r < skip1) || (c >= skip0 && c < skip1) ||
+                                  (r >= sq0 && r < sq1 && c >= sq0 && c < sq1);
                 if (r < m && c < nn && !skip) {
                     float *cp = Cp + (int64_t)r * Co.ld + c;
                     *cp = alpha * acc[i][j][q] + (beta ? *cp : 0.f);
@@ -1602,25 +1604,56 @@ int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, 
     if (!scratch) return fp_fail(ctx, FP_ERR_ARG, "internal: block inverse above 128 rows needs scratch");
     const int q = bj_chunks(block), h = (block + q - 1) / q;
     float *W = scratch, *V = scratch + (int64_t)nb * h * block;
+    // look-ahead: once X_{J+1,J+1} has its update from chunk J, its Gauss-Jordan
+    // pass runs on a second stream beside the rest of chunk J's products
+    if (!ctx->side2) FP_HIP(ctx, hipStreamCreateWithFlags(&ctx->side2, hipStreamNonBlocking));
+    struct Events {
+        hipEvent_t fork = nullptr, join = nullptr;
+        ~Events()
+        {
+            if (fork) (void)hipEventDestroy(fork);
+            if (join) (void)hipEventDestroy(join);
+        }
+    } ev;
+    FP_HIP(ctx, hipEventCreateWithFlags(&ev.fork, hipEventDisableTiming));
+    FP_HIP(ctx, hipEventCreateWithFlags(&ev.join, hipEventDisableTiming));
     hipLaunchKernelGGL(k_bj_copy<T>, dim3(block, nb), dim3(256), 0, ctx->stream, A, n, ld, block, inv);
+    auto gj = [&](int j, hipStream_t s) {
+        const int j0 = j * h, hj = min(h, block - j0);
+        hipLaunchKernelGGL((k_bj_gj<float, 16>), dim3(nb), dim3(256), 0, s, (const float *)inv, bb, (int64_t)block, inv,
+                           bb, (int64_t)block, n, block, j0, hj, info_dev);
+    };
     const int tb = (block + 63) / 64;
+    gj(0, ctx->stream);
     for (int j = 0; j < q; ++j) {
         const int j0 = j * h, hj = min(h, block - j0), th = (hj + 63) / 64;
-        hipLaunchKernelGGL((k_bj_gj<float, 16>), dim3(nb), dim3(256), 0, ctx->stream, (const float *)inv, bb,
-                           (int64_t)block, inv, bb, (int64_t)block, n, block, j0, hj, info_dev);
+        const bool ahead = j + 1 < q;
+        const int n0 = ahead ? j0 + hj : 0, hn = ahead ? min(h, block - n0) : 0;
         const BjOp XJJ{inv, bb, block, j0, j0}, XJrow{inv, bb, block, j0, 0}, XcolJ{inv, bb, block, 0, j0},
             Xall{inv, bb, block, 0, 0}, Wo{W, (int64_t)hj * block, block, 0, 0}, Vo{V, (int64_t)block * hj, hj, 0, 0};
         // W = X_JJ X_J*  (hj x block)
         hipLaunchKernelGGL(k_bj_gemm, dim3(tb, th, nb), dim3(256), 0, ctx->stream, XJJ, XJrow, Wo, hj, block, hj,
-                           1.f, 0, 0, 0);
-        // X_RC -= X_RJ W_C outside row and column chunk J
+                           1.f, 0, 0, 0, 0, 0);
+        if (ahead) {
+            // X_{J+1,J+1} -= X_{J+1,J} W_{J+1}, then its inverse on the second stream
+            const BjOp XnJ{inv, bb, block, n0, j0}, Wn{W, (int64_t)hj * block, block, 0, n0}, Xnn{inv, bb, block, n0, n0};
+            const int tn = (hn + 63) / 64;
+            hipLaunchKernelGGL(k_bj_gemm, dim3(tn, tn, nb), dim3(256), 0, ctx->stream, XnJ, Wn, Xnn, hn, hn, hj, -1.f,
+                               1, 0, 0, 0, 0);
+            FP_HIP(ctx, hipEventRecord(ev.fork, ctx->stream));
+            FP_HIP(ctx, hipStreamWaitEvent(ctx->side2, ev.fork, 0));
+            gj(j + 1, ctx->side2);
+            FP_HIP(ctx, hipEventRecord(ev.join, ctx->side2));
+        }
+        // X_RC -= X_RJ W_C outside row and column chunk J (and outside X_{J+1,J+1}, done above)
         hipLaunchKernelGGL(k_bj_gemm, dim3(tb, tb, nb), dim3(256), 0, ctx->stream, XcolJ, Wo, Xall, block, block, hj,
-                           -1.f, 1, j0, j0 + hj);
+                           -1.f, 1, j0, j0 + hj, n0, n0 + hn);
         // V = -X_*J X_JJ  (block x hj)
         hipLaunchKernelGGL(k_bj_gemm, dim3(th, tb, nb), dim3(256), 0, ctx->stream, XcolJ, XJJ, Vo, block, hj, hj,
-                           -1.f, 0, 0, 0);
+                           -1.f, 0, 0, 0, 0, 0);
         hipLaunchKernelGGL(k_bj_panels, dim3(block, nb), dim3(256), 0, ctx->stream, inv, block, j0, hj,
                            (const float *)W, (const float *)V);
+        if (ahead) FP_HIP(ctx, hipStreamWaitEvent(ctx->stream, ev.join, 0));
     }
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;

@@ -52,6 +52,8 @@ struct fp_ctx {
     // second stream for set-up work that runs beside the main stream (created on
     // first use, destroyed with the context)
     hipStream_t side = nullptr;
+    // third stream: look-ahead Gauss-Jordan chunks of the block-Jacobi inverses
+    hipStream_t side2 = nullptr;
     // cached scratch (grown on demand, freed with the context)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
