@@ -234,11 +234,10 @@ struct BJPrecond : Precond {
         const size_t sf = fp_bj_gj_scratch_floats(n, block);
         if (sf && (st = dalloc(ctx, gjs, sf * sizeof(float)))) return st;
         float *sp = sf ? gjs.as<float>() : nullptr;
+        int nc = 0;
         if (coarse > 0) {
-            // the coarse space does not depend on the block inverses: it is built
-            // on the side stream while the main stream runs the Gauss-Jordan steps
             cs = coarse;
-            const int nc = (int)((n + cs - 1) / cs);
+            nc = (int)((n + cs - 1) / cs);
             if ((st = dalloc(ctx, AZ, (size_t)nc * n * sizeof(double)))) return st;
             if ((st = dalloc(ctx, Einv, (size_t)nc * nc * sizeof(double)))) return st;
             if ((st = dalloc(ctx, work, (size_t)2 * nc * nc * sizeof(double)))) return st;
@@ -248,7 +247,16 @@ struct BJPrecond : Precond {
             if (!ctx->side) FP_HIP(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
             FP_HIP(ctx, hipEventCreateWithFlags(&fork, hipEventDisableTiming));
             FP_HIP(ctx, hipEventCreateWithFlags(&join, hipEventDisableTiming));
-            FP_HIP(ctx, hipEventRecord(fork, ctx->stream));
+        }
+        st = prec == FP_F64 ? fp_bj_gj_inverse<double>(ctx, (const double *)A, n, ld, block, inv.as<float>(), info_dev,
+                                                       sp, fork)
+                            : fp_bj_gj_inverse<float>(ctx, (const float *)A, n, ld, block, inv.as<float>(), info_dev,
+                                                      sp, fork);
+        if (st) return st;
+        if (coarse > 0) {
+            // the coarse space does not depend on the block inverses: it is built on
+            // the side stream beside the Gauss-Jordan steps, once the block copy has
+            // read its share of A
             FP_HIP(ctx, hipStreamWaitEvent(ctx->side, fork, 0));
             fp_ctx sc;
             sc.device = ctx->device;
@@ -260,12 +268,8 @@ struct BJPrecond : Precond {
                                                          Einv.as<double>(), work.as<double>(), cinfo.as<int>());
             if (st) return fp_fail(ctx, st, sc.err.c_str());
             FP_HIP(ctx, hipEventRecord(join, ctx->side));
+            FP_HIP(ctx, hipStreamWaitEvent(ctx->stream, join, 0));
         }
-        st = prec == FP_F64
-                 ? fp_bj_gj_inverse<double>(ctx, (const double *)A, n, ld, block, inv.as<float>(), info_dev, sp)
-                 : fp_bj_gj_inverse<float>(ctx, (const float *)A, n, ld, block, inv.as<float>(), info_dev, sp);
-        if (st) return st;
-        if (join) FP_HIP(ctx, hipStreamWaitEvent(ctx->stream, join, 0));
         return FP_OK;
     }
     int apply(const double *x, double *y) override

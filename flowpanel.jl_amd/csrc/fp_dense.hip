@@ -1586,9 +1586,11 @@ size_t fp_bj_gj_scratch_floats(int64_t n, int block)
 //   X_JJ = X_JJ^-1;  W = X_JJ X_J*;  X_RC -= X_RJ W_C (R, C outside J);
 //   V = -X_*J X_JJ;  X_J* = W, X_*J = V (outside X_JJ).
 // scratch holds W and V (fp_bj_gj_scratch_floats).
+// copied (optional): recorded on the context stream once A's diagonal blocks
+// have been read (other work that reads A can start then without slowing the copy).
 template <typename T>
 int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *inv, int *info_dev,
-                     float *scratch)
+                     float *scratch, hipEvent_t copied)
 {
     if (block > 1024) return fp_fail(ctx, FP_ERR_ARG, "block-Jacobi block larger than 1024");
     const int nb = (int)((n + block - 1) / block);
@@ -1596,6 +1598,7 @@ int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, 
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, info_dev, (int64_t)1, 0);
     ProfScope ps(ctx, "bj_invert");
     if (block <= GJ_CHUNK) {
+        if (copied) FP_HIP(ctx, hipEventRecord(copied, ctx->stream));
         hipLaunchKernelGGL((k_bj_gj<T, 16>), dim3(nb), dim3(256), 0, ctx->stream, A, (int64_t)block * (ld + 1), ld,
                            inv, bb, (int64_t)block, n, block, 0, -1, info_dev);
         FP_HIP(ctx, hipGetLastError());
@@ -1618,6 +1621,7 @@ int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, 
     FP_HIP(ctx, hipEventCreateWithFlags(&ev.fork, hipEventDisableTiming));
     FP_HIP(ctx, hipEventCreateWithFlags(&ev.join, hipEventDisableTiming));
     hipLaunchKernelGGL(k_bj_copy<T>, dim3(block, nb), dim3(256), 0, ctx->stream, A, n, ld, block, inv);
+    if (copied) FP_HIP(ctx, hipEventRecord(copied, ctx->stream));
     auto gj = [&](int j, hipStream_t s) {
         const int j0 = j * h, hj = min(h, block - j0);
         hipLaunchKernelGGL((k_bj_gj<float, 16>), dim3(nb), dim3(256), 0, s, (const float *)inv, bb, (int64_t)block, inv,
@@ -1658,8 +1662,10 @@ int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, 
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }
-template int fp_bj_gj_inverse<double>(fp_ctx *, const double *, int64_t, int64_t, int, float *, int *, float *);
-template int fp_bj_gj_inverse<float>(fp_ctx *, const float *, int64_t, int64_t, int, float *, int *, float *);
+template int fp_bj_gj_inverse<double>(fp_ctx *, const double *, int64_t, int64_t, int, float *, int *, float *,
+                                      hipEvent_t);
+template int fp_bj_gj_inverse<float>(fp_ctx *, const float *, int64_t, int64_t, int, float *, int *, float *,
+                                     hipEvent_t);
 
 // y_b = inv_b x_b for every diagonal block.
 template <int KM>
