@@ -17,8 +17,8 @@ from .bodies import (ConstantDoublet, ConstantSource, MultiBody, NonLiftingBody,
 from .solvers import (assemble, calc_bc_noflowthrough, getrf, solve, solve_backslash, solve_gmres,
                       solve_gmres_ir, solve_lu32_ir, solve_ludiv, solve_freestreams, BodyBatch, calc_elprescribe,
                       last_stats)
-from .postprocess import (Uind, phi, calcfield_U, calcfield_Ugradmu_cell, addfields, calcfield_Cp, calcfield_F,
-                          calcfield_LDS)
+from .postprocess import (Uind, phi, calcfield_U, calcfield_Ugradmu_cell, calcfield_Ugradmu_node, calcfield_Ugradmu,
+                          addfields, calcfield_Cp, calcfield_F, calcfield_LDS)
 from .adjoint import lift_drag, lift_drag_gradient, set_nodes
 from ._native import FPError, context
 
@@ -26,6 +26,6 @@ __all__ = [
     "geometry", "meshio", "Grid", "calc_shedding", "read_msh", "read_vtu", "direction", "ConstantSource", "ConstantDoublet", "VortexRing", "NonLiftingBody",
     "RigidWakeBody", "MultiBody", "calc_normals", "calc_controlpoints", "calc_areas", "assemble",
     "calc_bc_noflowthrough", "getrf", "solve", "solve_ludiv", "solve_backslash", "solve_gmres", "solve_lu32_ir",
-    "solve_gmres_ir", "last_stats", "Uind", "phi", "calcfield_U", "calcfield_Ugradmu_cell", "addfields", "calcfield_Cp", "calcfield_F", "calcfield_LDS",
+    "solve_gmres_ir", "last_stats", "Uind", "phi", "calcfield_U", "calcfield_Ugradmu_cell", "calcfield_Ugradmu_node", "calcfield_Ugradmu", "addfields", "calcfield_Cp", "calcfield_F", "calcfield_LDS",
     "FPError", "context", "solve_freestreams", "BodyBatch", "calc_elprescribe", "lift_drag", "lift_drag_gradient", "set_nodes",
 ]

@@ -2628,3 +2628,134 @@ int fp_calcfield_ugradmu(fp_ctx *ctx, fp_body *const *bodies, int nb, int te_cor
 }
 
 }  // extern "C"
+
+namespace {
+
+int ugradmu_inputs(fp_ctx *ctx, fp_body *const *bodies, int nb, int64_t *M)
+{
+    int st = check_bodies(ctx, bodies, nb);
+    if (st) return st;
+    *M = 0;
+    for (int k = 0; k < nb; ++k) {
+        if (!bodies[k]->solved)
+            return fp_fail(ctx, FP_ERR_UNSOLVED, "Source body hasn't been solved yet. Please call `solve(...)` function first.");
+        if (bodies[k]->element != FP_ELEM_VORTEXRING)
+            return fp_fail(ctx, FP_ERR_UNSUPPORTED, "the doublet-gradient velocity needs vortex-ring bodies");
+        *M += bodies[k]->N;
+    }
+    return FP_OK;
+}
+
+// node -> cells lists (CSR over every vertex of every cell) and the first three
+// nodes of each cell, uploaded for one call
+struct NodeData {
+    DBuf nptr, ncell, cn, nodal;
+    int64_t nn = 0;
+    int build(fp_ctx *ctx, const fp_body *b)
+    {
+        int st;
+        nn = (int64_t)b->nodes.size() / 3;
+        const int64_t N = b->N;
+        std::vector<int64_t> ptr(nn + 1, 0);
+        for (int64_t c = 0; c < N; ++c)
+            for (int64_t q = b->cell_ptr[c]; q < b->cell_ptr[c + 1]; ++q) ++ptr[b->cell_idx[q] + 1];
+        for (int64_t v = 0; v < nn; ++v) ptr[v + 1] += ptr[v];
+        std::vector<int> cells(ptr[nn]), cn3(3 * N);
+        std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+        for (int64_t c = 0; c < N; ++c) {
+            for (int64_t q = b->cell_ptr[c]; q < b->cell_ptr[c + 1]; ++q) cells[fill[b->cell_idx[q]]++] = (int)c;
+            for (int k = 0; k < 3; ++k) cn3[3 * c + k] = (int)b->cell_idx[b->cell_ptr[c] + k];
+        }
+        if ((st = dalloc(ctx, nptr, (nn + 1) * sizeof(int64_t)))) return st;
+        if ((st = dalloc(ctx, ncell, std::max<size_t>(1, cells.size()) * sizeof(int)))) return st;
+        if ((st = dalloc(ctx, cn, 3 * N * sizeof(int)))) return st;
+        if ((st = dalloc(ctx, nodal, std::max<int64_t>(1, nn) * sizeof(double)))) return st;
+        FP_HIP(ctx, hipMemcpyAsync(nptr.p, ptr.data(), (nn + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+        if (!cells.empty())
+            FP_HIP(ctx, hipMemcpyAsync(ncell.p, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice,
+                                       ctx->stream));
+        FP_HIP(ctx, hipMemcpyAsync(cn.p, cn3.data(), 3 * N * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        // the host vectors go out of scope: wait for the copies
+        FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return FP_OK;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int fp_calcfield_ugradmu_node(fp_ctx *ctx, fp_body *const *bodies, int nb, double *out, double *nodal_out)
+{
+    int64_t M = 0;
+    int st = ugradmu_inputs(ctx, bodies, nb, &M);
+    if (st) return st;
+    if (!out) return fp_fail(ctx, FP_ERR_ARG, "null out");
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    DBuf o;
+    if ((st = dalloc(ctx, o, 3 * M * sizeof(double)))) return st;
+    FP_HIP(ctx, hipMemcpyAsync(o.p, out, 3 * M * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    int64_t off = 0, noff = 0;
+    for (int k = 0; k < nb; ++k) {
+        fp_body *b = bodies[k];
+        NodeData nd;
+        if ((st = nd.build(ctx, b))) return st;
+        if ((st = fp_launch_ugradmu_node(ctx, b, nd.nptr.as<int64_t>(), nd.ncell.as<int>(), nd.nn, nd.cn.as<int>(),
+                                         nd.nodal.as<double>(), o.as<double>() + 3 * off)))
+            return st;
+        if (nodal_out && nd.nn > 0)
+            FP_HIP(ctx, hipMemcpyAsync(nodal_out + noff, nd.nodal.p, nd.nn * sizeof(double), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+        FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        off += b->N;
+        noff += nd.nn;
+    }
+    FP_HIP(ctx, hipMemcpyAsync(out, o.p, 3 * M * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FP_OK;
+}
+
+int fp_calcfield_ugradmu_combined(fp_ctx *ctx, fp_body *const *bodies, int nb, int force_cell_te,
+                                  double anglecrit_deg, double *out)
+{
+    int64_t M = 0;
+    int st = ugradmu_inputs(ctx, bodies, nb, &M);
+    if (st) return st;
+    if (!out) return fp_fail(ctx, FP_ERR_ARG, "null out");
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    DBuf o, cellv, nodev;
+    if ((st = dalloc(ctx, o, 3 * M * sizeof(double)))) return st;
+    FP_HIP(ctx, hipMemcpyAsync(o.p, out, 3 * M * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    const double cosc = std::cos(anglecrit_deg * 3.14159265358979323846 / 180.0);
+    int64_t off = 0;
+    for (int k = 0; k < nb; ++k) {
+        fp_body *b = bodies[k];
+        const int64_t N = b->N;
+        if ((st = body_neighbors(ctx, b))) return st;
+        if (b->ns > 0 && !b->d_sh) {
+            FP_HIP(ctx, hipMalloc((void **)&b->d_sh, 6 * b->ns * sizeof(int64_t)));
+            FP_HIP(ctx, hipMemcpy(b->d_sh, b->shedding.data(), 6 * b->ns * sizeof(int64_t), hipMemcpyHostToDevice));
+        }
+        if ((st = dalloc(ctx, cellv, 3 * std::max<int64_t>(1, N) * sizeof(double)))) return st;
+        if ((st = dalloc(ctx, nodev, 3 * std::max<int64_t>(1, N) * sizeof(double)))) return st;
+        FP_HIP(ctx, hipMemsetAsync(cellv.p, 0, 3 * N * sizeof(double), ctx->stream));
+        FP_HIP(ctx, hipMemsetAsync(nodev.p, 0, 3 * N * sizeof(double), ctx->stream));
+        NodeData nd;
+        if ((st = nd.build(ctx, b))) return st;
+        // cell-centred values with the trailing-edge treatment, node-based values, then the choice per cell
+        if ((st = fp_launch_ugradmu(ctx, b, b->d_nbr, b->d_sh, 1, cellv.as<double>()))) return st;
+        if ((st = fp_launch_ugradmu_node(ctx, b, nd.nptr.as<int64_t>(), nd.ncell.as<int>(), nd.nn, nd.cn.as<int>(),
+                                         nd.nodal.as<double>(), nodev.as<double>())))
+            return st;
+        if ((st = fp_launch_ugradmu_combine(ctx, b, b->d_nbr, b->d_sh, force_cell_te, cosc, cellv.as<double>(),
+                                            nodev.as<double>(), o.as<double>() + 3 * off)))
+            return st;
+        FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        off += N;
+    }
+    FP_HIP(ctx, hipMemcpyAsync(out, o.p, 3 * M * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FP_OK;
+}
+
+}  // extern "C"

@@ -80,7 +80,130 @@ __global__ void k_ugradmu_te(const double *__restrict__ pv, const int *__restric
         }
 }
 
+// Node-based scheme (src/FLOWPanel_postprocess.jl:622-748): nodal values of
+// Gamma as the area-weighted mean over the cells holding each node (node -> cell
+// lists in CSR form), then per cell the in-plane gradient of the plane through
+// its first three vertices carrying those values.
+__global__ __launch_bounds__(256) void k_nodal_mean(const int64_t *__restrict__ nptr, const int *__restrict__ ncell,
+                                                    int64_t nn, const double *__restrict__ geo, int64_t N,
+                                                    const double *__restrict__ G, double *__restrict__ nodal)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nn) return;
+    double s = 0, w = 0;
+    for (int64_t q = nptr[v]; q < nptr[v + 1]; ++q) {
+        const int c = ncell[q];
+        const double a = geo[6 * N + c];
+        s += a * G[c];
+        w += a;
+    }
+    nodal[v] = w > 0 ? s / w : 0.0;
+}
+
+// out (3 x N point-major) += -1/2 grad: e1 along the first edge, e2 completing
+// the plane of the first three vertices, the plane f = f1 + gx x + gy y through
+// (0, 0), (L, 0), (x3, y3).  cn holds the first three node indices of each cell.
+__global__ __launch_bounds__(256) void k_ugradmu_node(const double *__restrict__ pv, int64_t N,
+                                                      const int *__restrict__ cn, const double *__restrict__ nodal,
+                                                      double sgn, double *__restrict__ out)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    double v[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[k][c] = pv[(3 * k + c) * N + j];
+    const double f1 = sgn * nodal[cn[3 * j]], f2 = sgn * nodal[cn[3 * j + 1]], f3 = sgn * nodal[cn[3 * j + 2]];
+    double e1[3], w[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { e1[c] = v[1][c] - v[0][c]; w[c] = v[2][c] - v[0][c]; }
+    const double L = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) e1[c] /= L;
+    const double x3 = w[0] * e1[0] + w[1] * e1[1] + w[2] * e1[2];
+    double e2[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) e2[c] = w[c] - x3 * e1[c];
+    const double y3 = sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) e2[c] /= y3;
+    const double gx = (f2 - f1) / L, gy = (f3 - f1 - gx * x3) / y3;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[3 * j + c] += -0.5 * (gx * e1[c] + gy * e2[c]);
+}
+
+// trailing-edge override of the node-based values (force_cellTE): cells pi and
+// pi + 1, and pj and pj - 1 when pj exists, take the cell-centred values
+__global__ void k_ugradmu_force_te(const int64_t *__restrict__ sh, int64_t ns, int64_t N,
+                                   const double *__restrict__ cellv, double *__restrict__ nodev)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ns) return;
+    const int64_t pi = sh[6 * e], pj = sh[6 * e + 3];
+    const int64_t cs[4] = {pi, pi + 1, pj, pj - 1};
+    for (int q = 0; q < 4; ++q) {
+        if (q >= 2 && pj < 0) break;
+        const int64_t c = cs[q];
+        if (c < 0 || c >= N) continue;
+        for (int d = 0; d < 3; ++d) nodev[3 * c + d] = cellv[3 * c + d];
+    }
+}
+
+// combined scheme (:756-921): a cell uses the node-based value when one of its
+// edges has no neighbour or the angle between its normal and a neighbour's
+// exceeds the critical angle, the cell-centred value otherwise; out += choice
+__global__ __launch_bounds__(256) void k_ugradmu_combine(const int *__restrict__ nvv, int64_t N,
+                                                         const int *__restrict__ nbr, const double *__restrict__ geo,
+                                                         double cosc, const double *__restrict__ cellv,
+                                                         const double *__restrict__ nodev, double *__restrict__ out)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    const int nv = nvv[j];
+    const double nx = geo[j], ny = geo[N + j], nz = geo[2 * N + j];
+    bool use_node = false;
+    for (int k = 0; k < nv; ++k) {
+        const int m = nbr[(int64_t)k * N + j];
+        if (m < 0) { use_node = true; break; }
+        if (nx * geo[m] + ny * geo[N + m] + nz * geo[2 * N + m] < cosc) { use_node = true; break; }
+    }
+    const double *src = use_node ? nodev : cellv;
+    for (int d = 0; d < 3; ++d) out[3 * j + d] += src[3 * j + d];
+}
+
 }  // namespace
+
+int fp_launch_ugradmu_node(fp_ctx *ctx, const fp_body *b, const int64_t *d_nptr, const int *d_ncell, int64_t nn,
+                           const int *d_cn, double *d_nodal, double *out)
+{
+    const int64_t N = b->N;
+    if (N == 0) return FP_OK;
+    const double sgn = b->cp_offset < 0 ? -1.0 : 1.0;
+    ProfScope ps(ctx, "ugradmu_node");
+    if (nn > 0)
+        hipLaunchKernelGGL(k_nodal_mean, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, ctx->stream, d_nptr,
+                           d_ncell, nn, b->d_geo, N, b->d_strength, d_nodal);
+    hipLaunchKernelGGL(k_ugradmu_node, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream, b->d_pv, N, d_cn,
+                       d_nodal, sgn, out);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+int fp_launch_ugradmu_combine(fp_ctx *ctx, const fp_body *b, const int *d_nbr, const int64_t *d_sh, int force_cell_te,
+                              double cos_crit, double *cellv, double *nodev, double *out)
+{
+    const int64_t N = b->N;
+    if (N == 0) return FP_OK;
+    ProfScope ps(ctx, "ugradmu_combine");
+    if (force_cell_te && b->ns > 0)
+        hipLaunchKernelGGL(k_ugradmu_force_te, dim3((unsigned)((b->ns + 63) / 64)), dim3(64), 0, ctx->stream, d_sh,
+                           b->ns, N, cellv, nodev);
+    hipLaunchKernelGGL(k_ugradmu_combine, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream, b->d_nv, N,
+                       d_nbr, b->d_geo, cos_crit, cellv, nodev, out);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
 
 int fp_launch_ugradmu(fp_ctx *ctx, const fp_body *b, const int *d_nbr, const int64_t *d_sh, int te, double *out)
 {

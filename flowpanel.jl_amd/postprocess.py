@@ -83,6 +83,53 @@ def calcfield_Ugradmu_cell(body, fieldname="Ugradmu", addfield=True, te_correcti
     return out
 
 
+def _store(body, fieldname, out):
+    body.add_field(fieldname, out)
+    col = 0
+    for b in body.bodies:
+        b.add_field(fieldname, out[:, col:col + b.ncells])
+        col += b.ncells
+
+
+def calcfield_Ugradmu_node(body, fieldname="Ugradmu", addfield=True, sharpTE=False):
+    """-1/2 grad(mu) from nodal values of Gamma (src/FLOWPanel_postprocess.jl:622-748), 3 x N.
+
+    The nodal values (area-weighted means over the cells holding each node) are
+    stored as the field "gamma_node".  The sharp trailing-edge variant
+    (``sharpTE=True``, structured grids only) is not provided.
+    """
+    if sharpTE:
+        raise NotImplementedError("calcfield_Ugradmu_node: sharpTE=True is not provided")
+    comps = body.bodies
+    ctx = comps[0].ctx
+    out = np.zeros((3, body.ncells), order="F")
+    nn = sum(int(b.nnodes) for b in comps)
+    nodal = np.zeros(max(nn, 1))
+    ctx.check(nat.lib().fp_calcfield_ugradmu_node(ctx.handle, nat.handles(comps), len(comps), nat.dptr(out),
+                                                  nat.dptr(nodal)))
+    if addfield:
+        _store(body, fieldname, out)
+        body.add_field("gamma_node", nodal[:nn])
+    return out
+
+
+def calcfield_Ugradmu(body, fieldname="Ugradmu", addfield=True, sharpTE=False, force_cellTE=True, anglecrit=30):
+    """Combined -1/2 grad(mu) (src/FLOWPanel_postprocess.jl:756-1000), 3 x N: the
+    cell-centred value, except on cells at an open edge or a sharp edge (normals
+    more than ``anglecrit`` degrees apart), which take the node-based value;
+    with ``force_cellTE`` the trailing-edge panels keep the cell-centred value."""
+    if sharpTE:
+        raise NotImplementedError("calcfield_Ugradmu: sharpTE=True is not provided")
+    comps = body.bodies
+    ctx = comps[0].ctx
+    out = np.zeros((3, body.ncells), order="F")
+    ctx.check(nat.lib().fp_calcfield_ugradmu_combined(ctx.handle, nat.handles(comps), len(comps),
+                                                      int(bool(force_cellTE)), float(anglecrit), nat.dptr(out)))
+    if addfield:
+        _store(body, fieldname, out)
+    return out
+
+
 def addfields(body, name1, name2):
     """field name2 += field name1 (src/FLOWPanel_abstractbody.jl add_field helpers)."""
     f = np.asarray(body.get_field(name2)) + np.asarray(body.get_field(name1))

@@ -245,8 +245,21 @@ int fp_calcfield_cp(fp_ctx *ctx, const double *Us, int64_t n, double uref, const
  * the doublet strength, by the cell-centred Green-Gauss scheme with inverse-
  * distance face values; with te_correction the trailing-edge panels drop their
  * neighbour terms (the RigidWakeBody variant).  Cells are neighbours when they
- * share an edge.  The nodal and combined schemes (:622-1042) are not provided. */
+ * share an edge. */
 int fp_calcfield_ugradmu(fp_ctx *ctx, fp_body *const *bodies, int nbodies, int te_correction, double *out);
+/* calcfield_Ugradmu_node!(out, body) (src/FLOWPanel_postprocess.jl:622-748, sharpTE = false):
+ * out (3 x N, +=) -1/2 grad(mu) from nodal values of Gamma, each the area-weighted mean over the cells
+ * holding the node, and per cell the in-plane gradient of the plane through its first three vertices
+ * (sign flipped with CPoffset < 0).  nodal_out (optional): the nodal values ("gamma_node"), the bodies'
+ * nodes one after the other. */
+int fp_calcfield_ugradmu_node(fp_ctx *ctx, fp_body *const *bodies, int nbodies, double *out, double *nodal_out);
+/* calcfield_Ugradmu!(out, out_cell, out_node, body) (src/FLOWPanel_postprocess.jl:756-1000, sharpTE = false):
+ * out (3 x N, +=) the cell-centred value (with the trailing-edge treatment) of every cell except those with
+ * an edge without neighbour or a neighbour whose normal is more than anglecrit_deg away, which take the
+ * node-based value; with force_cell_te the shedding panels pi, pi+1, pj, pj-1 keep cell-centred values.
+ * All edges of a cell are examined (the reference loops over three, every edge of a triangle). */
+int fp_calcfield_ugradmu_combined(fp_ctx *ctx, fp_body *const *bodies, int nbodies, int force_cell_te,
+                                  double anglecrit_deg, double *out);
 
 /* ---- shape updates ------------------------------------------------------- */
 int64_t fp_body_nnodes(const fp_body *body);

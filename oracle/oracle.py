@@ -374,3 +374,80 @@ def ugradmu_cell(body, strength, shedding=None, te=True):
                            _pd(np.ascontiguousarray(strength, dtype=np.float64)), _pd(normals), _pd(cps), _pd(areas),
                            float(body.CPoffset), _pi(sh), ns, int(bool(te and sh is not None)), _pd(out))
     return out
+
+
+def nodal_mean(body, strength, areas=None):
+    """Nodal values of a cell field: the area-weighted mean over the cells holding
+    each node (the ``gt.get_nodal_data(grid, Gammas; areas)`` call of
+    src/FLOWPanel_postprocess.jl:651; GeometricTools is not vendored, so this
+    restates that call as an area-weighted average)."""
+    if areas is None:
+        areas = _body_geometry(body)[2]
+    nodes = np.asarray(body.nodes)
+    cell_ptr, cell_idx = np.asarray(body.cell_ptr), np.asarray(body.cell_idx)
+    s = np.zeros(nodes.shape[1])
+    w = np.zeros(nodes.shape[1])
+    G = np.asarray(strength, dtype=np.float64)
+    for c in range(len(cell_ptr) - 1):
+        for p in cell_idx[cell_ptr[c]:cell_ptr[c + 1]]:
+            s[p] += areas[c] * G[c]
+            w[p] += areas[c]
+    return np.where(w > 0, s / np.where(w > 0, w, 1.0), 0.0)
+
+
+def ugradmu_node(body, strength):
+    """Node-based -1/2 grad(mu) (src/FLOWPanel_postprocess.jl:622-748, sharpTE = false), 3 x n,
+    and the nodal values.  Per cell: the plane through the first three vertices
+    carrying the (CPoffset-signed) nodal values, solved as a 3 x 3 system in an
+    in-plane frame (np.linalg.solve), its gradient mapped back to 3-D."""
+    normals, cps, areas = _body_geometry(body)
+    nodal = nodal_mean(body, strength, areas)
+    sgn = -1.0 if body.CPoffset < 0 else 1.0
+    nodes = np.asarray(body.nodes, dtype=np.float64)
+    cell_ptr, cell_idx = np.asarray(body.cell_ptr), np.asarray(body.cell_idx)
+    n = len(cell_ptr) - 1
+    out = np.zeros((3, n), order="F")
+    for c in range(n):
+        v = cell_idx[cell_ptr[c]:cell_ptr[c] + 3]
+        X = nodes[:, v]
+        e1 = X[:, 1] - X[:, 0]
+        e1 /= np.linalg.norm(e1)
+        w = X[:, 2] - X[:, 0]
+        e2 = w - w.dot(e1) * e1
+        e2 /= np.linalg.norm(e2)
+        t = np.array([[0.0, 0.0], [(X[:, 1] - X[:, 0]).dot(e1), 0.0], [w.dot(e1), w.dot(e2)]])
+        t0 = t.mean(axis=0)
+        M = np.column_stack([np.ones(3), t - t0])
+        a = np.linalg.solve(M, sgn * nodal[v])
+        out[:, c] = -0.5 * (a[1] * e1 + a[2] * e2)
+    return out, nodal
+
+
+def ugradmu_combined(body, strength, shedding=None, force_cell_te=True, anglecrit=30.0):
+    """Combined -1/2 grad(mu) (src/FLOWPanel_postprocess.jl:756-921, sharpTE = false), 3 x n:
+    node-based values on cells with an edge without neighbour or a neighbour normal
+    more than ``anglecrit`` degrees away, cell-centred values (trailing-edge variant)
+    elsewhere; with ``force_cell_te`` the shedding panels pi, pi+1, pj, pj-1 keep the
+    cell-centred values."""
+    cell = ugradmu_cell(body, strength, shedding, te=True)
+    node, _ = ugradmu_node(body, strength)
+    n = cell.shape[1]
+    if force_cell_te and shedding is not None and np.asarray(shedding).size:
+        for pi, nia, nib, pj, nja, njb in np.asarray(shedding, dtype=np.int64).T:
+            for c in (pi, pi + 1) + ((pj, pj - 1) if pj >= 0 else ()):
+                if 0 <= c < n:
+                    node[:, c] = cell[:, c]
+    normals = _body_geometry(body)[0]
+    nb = cell_neighbors(body.cell_ptr, body.cell_idx)
+    cosc = np.cos(np.deg2rad(anglecrit))
+    cell_ptr = np.asarray(body.cell_ptr)
+    out = np.zeros((3, n), order="F")
+    for c in range(n):
+        use_node = False
+        for k in range(cell_ptr[c + 1] - cell_ptr[c]):
+            m = nb[c, k]
+            if m < 0 or normals[:, c].dot(normals[:, m]) < cosc:
+                use_node = True
+                break
+        out[:, c] = node[:, c] if use_node else cell[:, c]
+    return out

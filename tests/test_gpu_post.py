@@ -38,3 +38,29 @@ def test_ugradmu_requires_solve(fp, geo):
     body = fp.RigidWakeBody(g, sh)
     with pytest.raises(fp.FPError):
         fp.calcfield_Ugradmu_cell(body)
+
+
+@pytest.mark.parametrize("tri", [False, True])
+def test_ugradmu_node_and_combined_match_oracle(fp, geo, orc, tri):
+    """Node-based (src/FLOWPanel_postprocess.jl:622-748) and combined (:756-1000)
+    schemes against the oracle's restatement, 1e-12."""
+    g, sh = geo.loft_wing(airfoil="2412", n_chord=20, n_span=10, span=5.0, sweep_deg=20.0)
+    if tri:
+        g, sh = geo.triangulate(g, sh)
+    body = fp.RigidWakeBody(g, sh, CPoffset=-1e-14 if tri else 1e-14)
+    V = fp.direction(5.0)
+    fp.solve(body, np.tile(V[:, None], (1, body.ncells)), np.tile(V[:, None], (1, body.nsheddings)),
+             np.tile(V[:, None], (1, body.nsheddings)))
+    ob = types.SimpleNamespace(nodes=g.nodes, cell_ptr=g.cell_ptr, cell_idx=g.cell_idx, ncells=g.ncells,
+                               CPoffset=body.CPoffset, charlen_kind=0)
+    out = fp.calcfield_Ugradmu_node(body)
+    ref, nodal = orc.ugradmu_node(ob, body.strength)
+    assert rel(out, ref) < 1e-12
+    assert rel(body.get_field("gamma_node"), nodal) < 1e-13
+    for force in (True, False):
+        for ang in (30.0, 10.0):
+            outc = fp.calcfield_Ugradmu(body, force_cellTE=force, anglecrit=ang)
+            refc = orc.ugradmu_combined(ob, body.strength, sh, force_cell_te=force, anglecrit=ang)
+            assert rel(outc, refc) < 1e-12, (force, ang)
+    with pytest.raises(NotImplementedError):
+        fp.calcfield_Ugradmu(body, sharpTE=True)
