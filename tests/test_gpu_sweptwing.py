@@ -70,3 +70,37 @@ def test_sweptwing_reference_meshes_lift(fp):
     # symmetric geometry and flow: the two halves carry mirrored lift
     nL = body.bodies[0].ncells
     assert abs(body.strength[:nL].sum() - body.strength[nL:].sum()) < 1e-6 * np.abs(body.strength).sum()
+
+
+def test_sweptwing_surface_velocity_workflow(fp):
+    """The reference example's post-processing chain on the same wing
+    (examples/sweptwing.jl:120-150: calcfield_U, the combined calcfield_Ugradmu,
+    addfields, calcfield_Cp, calcfield_F, then L and D), whose published result is
+    CL 0.272, CD 0.0066 (docs/resources/data/sweptwing000-CLCD.md).  On the stored
+    meshes this gives CL 0.2760 (+1.5 %) and CD 0.0084; the nodal averaging and
+    neighbour search of GeometricTools are restated, not run, so the pins are loose:
+    CL within 3 % and CD within 35 % of the published values.  The cell-centred
+    and node-based schemes alone are printed beside it."""
+    body = _wing(fp)
+    aoa, mag, rho = 4.2, 30.0, 1.225
+    U = mag * np.array([np.cos(np.deg2rad(aoa)), 0.0, np.sin(np.deg2rad(aoa))])
+    D = np.tile((U / mag)[:, None], (1, body.nsheddings))
+    fp.solve(body, np.tile(U[:, None], (1, body.ncells)), D, D)
+    Dhat = U / mag
+    Lhat = np.cross(Dhat, np.array([0.0, 1.0, 0.0]))
+    qS = 0.5 * rho * mag ** 2 * B ** 2 / AR
+    res = {}
+    for name, fn in (("combined", fp.calcfield_Ugradmu), ("cell", fp.calcfield_Ugradmu_cell),
+                     ("node", fp.calcfield_Ugradmu_node)):
+        fp.calcfield_U(body, body)
+        fn(body)
+        fp.addfields(body, "Ugradmu", "U")
+        fp.calcfield_Cp(body, mag)
+        fp.calcfield_F(body, mag, rho)
+        LDS = fp.calcfield_LDS(body, Lhat, Dhat)
+        res[name] = (LDS[:, 0].dot(Lhat) / qS, LDS[:, 1].dot(Dhat) / qS)
+    print("surface-velocity workflow: " + ", ".join("%s CL %.4f CD %.5f" % (k, *v) for k, v in res.items()),
+          flush=True)
+    cl, cd = res["combined"]
+    assert abs(cl - 0.272) / 0.272 <= 0.03
+    assert abs(cd - 0.0066) / 0.0066 <= 0.35
