@@ -221,8 +221,10 @@ struct BJPrecond : Precond {
     BJPrecond(fp_ctx *c, int64_t n_, int bl) : ctx(c), n(n_), block(bl) {}
     ~BJPrecond()
     {
-        // side-stream work must be over before its buffers can be handed out again
+        // side-stream work (coarse set-up, look-ahead Gauss-Jordan passes) must be
+        // over before its buffers can be handed out again
         if (fork && ctx->side) (void)hipStreamSynchronize(ctx->side);
+        if (ctx->side2) (void)hipStreamSynchronize(ctx->side2);
         if (fork) (void)hipEventDestroy(fork);
         if (join) (void)hipEventDestroy(join);
     }
