@@ -1569,7 +1569,8 @@ __global__ __launch_bounds__(256) void k_bj_panels(float *__restrict__ X, int bl
     }
 }
 
-constexpr int GJ_CHUNK = 128;   // rows per Gauss-Jordan chunk (TG = 16 kernel, 256 threads)
+constexpr int GJ_TG = 16;               // Gauss-Jordan thread grid: 16 x 16 threads, 8 x 8 values each
+constexpr int GJ_CHUNK = 8 * GJ_TG;     // rows per Gauss-Jordan chunk
 static int bj_chunks(int block) { return (block + GJ_CHUNK - 1) / GJ_CHUNK; }
 
 size_t fp_bj_gj_scratch_floats(int64_t n, int block)
@@ -1579,10 +1580,10 @@ size_t fp_bj_gj_scratch_floats(int64_t n, int block)
     return (size_t)(2 * nb * h * block);
 }
 
-// Inverses of the diagonal blocks straight from A, block <= 1024.  Up to 128
-// rows: one Gauss-Jordan pass per block.  Larger blocks: blocked Gauss-Jordan
-// over q = ceil(block / 128) chunks J of h = ceil(block / q) rows (pivoting
-// within each chunk), on the identity-padded FP32 copy X:
+// Inverses of the diagonal blocks straight from A, block <= 1024.  Up to
+// GJ_CHUNK rows: one Gauss-Jordan pass per block.  Larger blocks: blocked
+// Gauss-Jordan over q = ceil(block / GJ_CHUNK) chunks J of h = ceil(block / q)
+// rows (pivoting within each chunk), on the identity-padded FP32 copy X:
 //   X_JJ = X_JJ^-1;  W = X_JJ X_J*;  X_RC -= X_RJ W_C (R, C outside J);
 //   V = -X_*J X_JJ;  X_J* = W, X_*J = V (outside X_JJ).
 // scratch holds W and V (fp_bj_gj_scratch_floats).
@@ -1599,12 +1600,12 @@ int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, 
     ProfScope ps(ctx, "bj_invert");
     if (block <= GJ_CHUNK) {
         if (copied) FP_HIP(ctx, hipEventRecord(copied, ctx->stream));
-        hipLaunchKernelGGL((k_bj_gj<T, 16>), dim3(nb), dim3(256), 0, ctx->stream, A, (int64_t)block * (ld + 1), ld,
+        hipLaunchKernelGGL((k_bj_gj<T, GJ_TG>), dim3(nb), dim3(GJ_TG * GJ_TG), 0, ctx->stream, A, (int64_t)block * (ld + 1), ld,
                            inv, bb, (int64_t)block, n, block, 0, -1, info_dev);
         FP_HIP(ctx, hipGetLastError());
         return FP_OK;
     }
-    if (!scratch) return fp_fail(ctx, FP_ERR_ARG, "internal: block inverse above 128 rows needs scratch");
+    if (!scratch) return fp_fail(ctx, FP_ERR_ARG, "internal: blocked Gauss-Jordan inverse needs scratch");
     const int q = bj_chunks(block), h = (block + q - 1) / q;
     float *W = scratch, *V = scratch + (int64_t)nb * h * block;
     // look-ahead: once X_{J+1,J+1} has its update from chunk J, its Gauss-Jordan
@@ -1624,7 +1625,7 @@ int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, 
     if (copied) FP_HIP(ctx, hipEventRecord(copied, ctx->stream));
     auto gj = [&](int j, hipStream_t s) {
         const int j0 = j * h, hj = min(h, block - j0);
-        hipLaunchKernelGGL((k_bj_gj<float, 16>), dim3(nb), dim3(256), 0, s, (const float *)inv, bb, (int64_t)block, inv,
+        hipLaunchKernelGGL((k_bj_gj<float, GJ_TG>), dim3(nb), dim3(GJ_TG * GJ_TG), 0, s, (const float *)inv, bb, (int64_t)block, inv,
                            bb, (int64_t)block, n, block, j0, hj, info_dev);
     };
     const int tb = (block + 63) / 64;
