@@ -125,6 +125,10 @@ __device__ __forceinline__ void vr_tile(const PanelView &pan, const TargetView &
     const int iend = (int)min((int64_t)TM, tv.M - i0);
     TO *g = G + i0 * ld + col0 + j;
     float *g2 = G2 ? G2 + i0 * ld + col0 + j : nullptr;
+    // two targets per iteration for quads (more independent work per wave; octagons
+    // would drop to two waves per SIMD)
+    constexpr int TU = NVP == 4 ? 2 : 1;
+#pragma unroll TU
     for (int t = 0; t < iend; ++t) {
         T ux = 0, uy = 0, uz = 0;
         ring_u<T, NVP>(px, py, pz, ex, ey, ez, s[0][t], s[1][t], s[2][t], off, off2, cut2, ux, uy, uz);
