@@ -1917,6 +1917,12 @@ __global__ __launch_bounds__(1024) void k_small_inverse_lds(double *__restrict__
     for (int e = tid; e < m * m; e += nt) E[e] = a[e / m][e % m];
 }
 
+__global__ __launch_bounds__(256) void k_widen_f32(const float *__restrict__ x, int64_t m, double *__restrict__ y)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) y[i] = (double)x[i];
+}
+
 template <typename T>
 int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, double *AZ, double *Einv, double *work,
                     int *info_dev)
@@ -1928,10 +1934,20 @@ int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, doub
     hipLaunchKernelGGL(k_chunk_rowsum<T>, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, ctx->stream, A, n, ld, cs, nc,
                        AZ);
     hipLaunchKernelGGL(k_coarse_galerkin, dim3(nc, nc), dim3(256), 0, ctx->stream, (const double *)AZ, n, cs, nc, Einv);
-    if (nc <= SMALL_INV_LDS)
+    if (nc <= SMALL_INV_LDS) {
         hipLaunchKernelGGL(k_small_inverse_lds, dim3(1), dim3(1024), 0, ctx->stream, Einv, nc, info_dev);
-    else
-        hipLaunchKernelGGL(k_small_inverse, dim3(1), dim3(1024), 0, ctx->stream, Einv, nc, work, info_dev);
+    } else {
+        // larger coarse spaces: FP32 blocked Gauss-Jordan of E (it only feeds the
+        // preconditioner), widened back in place; work (2 nc^2 doubles) holds the
+        // FP32 inverse and the Gauss-Jordan scratch
+        float *inv32 = reinterpret_cast<float *>(work), *scr = inv32 + (int64_t)nc * nc;
+        if (fp_bj_gj_scratch_floats(nc, nc) > (size_t)3 * nc * nc)
+            return fp_fail(ctx, FP_ERR_ARG, "internal: coarse inverse scratch too small");
+        int st = fp_bj_gj_inverse<double>(ctx, Einv, nc, nc, nc, inv32, info_dev, scr);
+        if (st) return st;
+        hipLaunchKernelGGL(k_widen_f32, dim3((unsigned)(((int64_t)nc * nc + 255) / 256)), dim3(256), 0, ctx->stream,
+                           (const float *)inv32, (int64_t)nc * nc, Einv);
+    }
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }

@@ -106,3 +106,21 @@ def test_buffer_pool_reuse_and_trim(fp):
     x1 = fp.solve_ludiv(None, A, b)
     x2 = fp.solve_ludiv(None, A, b)
     assert np.array_equal(x1, x2)
+
+
+@pytest.mark.parametrize("coarse", [8, 5, 3])
+def test_coarse_space_above_128_vectors(fp, coarse):
+    """Coarse spaces of 126, 201 and 334 chunk vectors (1001 rows): above 128 the
+    Galerkin matrix is inverted by the blocked Gauss-Jordan of the block-Jacobi
+    path; the two-level solve must still reach its FP64 target and take no more
+    iterations than block-Jacobi alone."""
+    rng = np.random.default_rng(6000 + coarse)
+    n = 1001
+    A = rng.standard_normal((n, n)) * 0.3 / np.sqrt(n) + 3.0 * np.eye(n) + 0.5 / n
+    b = rng.standard_normal(n)
+    y = fp.solve_gmres_ir(None, A, b, precond_block=100, coarse_block=coarse, tol=1e-12)
+    it2 = fp.solvers.last_stats["iterations"]
+    fp.solve_gmres_ir(None, A, b, precond_block=100, tol=1e-12)
+    it1 = fp.solvers.last_stats["iterations"]
+    assert rel(y, np.linalg.solve(A, b)) < 1e-10
+    assert it2 <= it1, (it2, it1)
