@@ -469,8 +469,8 @@ def main():
         "value": round(t_mixed, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(t_mixed, 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "f32 AIC + f64 refinement", "data": "synthetic",
-        "config": {"workload": "bwb-%dx%d vortex-ring + rigid wake, AoA %.1f, GMRES-IR (FP32 AIC, block-Jacobi %d)"
-                               % (args.chord, args.span, alpha, block),
+        "config": {"workload": "bwb-%dx%d vortex-ring + rigid wake, AoA %.1f, GMRES-IR (FP32 AIC, block-Jacobi %d, "
+                               "coarse chunks %d)" % (args.chord, args.span, alpha, block, coarse),
                    "panels": N, "shedding_edges": body.nsheddings},
         "fp64_lu_ms": round(t_fp64, 3), "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
         "cp_err_max": cp_err, "gamma_rel_err": gam_err, "tol": args.tol, "inner_tol": args.inner_tol,
