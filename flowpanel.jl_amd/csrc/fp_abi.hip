@@ -260,14 +260,10 @@ struct BJPrecond : Precond {
             // the side stream beside the Gauss-Jordan steps, once the block copy has
             // read its share of A
             FP_HIP(ctx, hipStreamWaitEvent(ctx->side, fork, 0));
-            // (a coarse space above 128 vectors is inverted by blocked Gauss-Jordan,
-            // whose look-ahead passes use the context's third stream)
-            if (!ctx->side2) FP_HIP(ctx, hipStreamCreateWithFlags(&ctx->side2, hipStreamNonBlocking));
             fp_ctx sc;
             sc.device = ctx->device;
             sc.num_cu = ctx->num_cu;
             sc.stream = ctx->side;
-            sc.side2 = ctx->side2;
             st = prec == FP_F64 ? fp_coarse_setup<double>(&sc, (const double *)A, n, ld, cs, AZ.as<double>(),
                                                           Einv.as<double>(), work.as<double>(), cinfo.as<int>())
                                 : fp_coarse_setup<float>(&sc, (const float *)A, n, ld, cs, AZ.as<double>(),

@@ -1589,9 +1589,11 @@ size_t fp_bj_gj_scratch_floats(int64_t n, int block)
 // scratch holds W and V (fp_bj_gj_scratch_floats).
 // copied (optional): recorded on the context stream once A's diagonal blocks
 // have been read (other work that reads A can start then without slowing the copy).
+// lookahead: run each next chunk's pass on the context's third stream (false:
+// everything in order on the context stream).
 template <typename T>
 int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *inv, int *info_dev,
-                     float *scratch, hipEvent_t copied)
+                     float *scratch, hipEvent_t copied, bool lookahead)
 {
     if (block > 1024) return fp_fail(ctx, FP_ERR_ARG, "block-Jacobi block larger than 1024");
     const int nb = (int)((n + block - 1) / block);
@@ -1610,7 +1612,7 @@ int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, 
     float *W = scratch, *V = scratch + (int64_t)nb * h * block;
     // look-ahead: once X_{J+1,J+1} has its update from chunk J, its Gauss-Jordan
     // pass runs on a second stream beside the rest of chunk J's products
-    if (!ctx->side2) FP_HIP(ctx, hipStreamCreateWithFlags(&ctx->side2, hipStreamNonBlocking));
+    if (lookahead && !ctx->side2) FP_HIP(ctx, hipStreamCreateWithFlags(&ctx->side2, hipStreamNonBlocking));
     struct Events {
         hipEvent_t fork = nullptr, join = nullptr;
         ~Events()
@@ -1619,8 +1621,10 @@ int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, 
             if (join) (void)hipEventDestroy(join);
         }
     } ev;
-    FP_HIP(ctx, hipEventCreateWithFlags(&ev.fork, hipEventDisableTiming));
-    FP_HIP(ctx, hipEventCreateWithFlags(&ev.join, hipEventDisableTiming));
+    if (lookahead) {
+        FP_HIP(ctx, hipEventCreateWithFlags(&ev.fork, hipEventDisableTiming));
+        FP_HIP(ctx, hipEventCreateWithFlags(&ev.join, hipEventDisableTiming));
+    }
     hipLaunchKernelGGL(k_bj_copy<T>, dim3(block, nb), dim3(256), 0, ctx->stream, A, n, ld, block, inv);
     if (copied) FP_HIP(ctx, hipEventRecord(copied, ctx->stream));
     auto gj = [&](int j, hipStream_t s) {
@@ -1645,10 +1649,14 @@ int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, 
             const int tn = (hn + 63) / 64;
             hipLaunchKernelGGL(k_bj_gemm, dim3(tn, tn, nb), dim3(256), 0, ctx->stream, XnJ, Wn, Xnn, hn, hn, hj, -1.f,
                                1, 0, 0, 0, 0);
-            FP_HIP(ctx, hipEventRecord(ev.fork, ctx->stream));
-            FP_HIP(ctx, hipStreamWaitEvent(ctx->side2, ev.fork, 0));
-            gj(j + 1, ctx->side2);
-            FP_HIP(ctx, hipEventRecord(ev.join, ctx->side2));
+            if (lookahead) {
+                FP_HIP(ctx, hipEventRecord(ev.fork, ctx->stream));
+                FP_HIP(ctx, hipStreamWaitEvent(ctx->side2, ev.fork, 0));
+                gj(j + 1, ctx->side2);
+                FP_HIP(ctx, hipEventRecord(ev.join, ctx->side2));
+            } else {
+                gj(j + 1, ctx->stream);
+            }
         }
         // X_RC -= X_RJ W_C outside row and column chunk J (and outside X_{J+1,J+1}, done above)
         hipLaunchKernelGGL(k_bj_gemm, dim3(tb, tb, nb), dim3(256), 0, ctx->stream, XcolJ, Wo, Xall, block, block, hj,
@@ -1658,15 +1666,15 @@ int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, 
                            -1.f, 0, 0, 0, 0, 0);
         hipLaunchKernelGGL(k_bj_panels, dim3(block, nb), dim3(256), 0, ctx->stream, inv, block, j0, hj,
                            (const float *)W, (const float *)V);
-        if (ahead) FP_HIP(ctx, hipStreamWaitEvent(ctx->stream, ev.join, 0));
+        if (ahead && lookahead) FP_HIP(ctx, hipStreamWaitEvent(ctx->stream, ev.join, 0));
     }
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }
 template int fp_bj_gj_inverse<double>(fp_ctx *, const double *, int64_t, int64_t, int, float *, int *, float *,
-                                      hipEvent_t);
+                                      hipEvent_t, bool);
 template int fp_bj_gj_inverse<float>(fp_ctx *, const float *, int64_t, int64_t, int, float *, int *, float *,
-                                     hipEvent_t);
+                                     hipEvent_t, bool);
 
 // y_b = inv_b x_b for every diagonal block.
 template <int KM>
@@ -1943,7 +1951,7 @@ int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, doub
         float *inv32 = reinterpret_cast<float *>(work), *scr = inv32 + (int64_t)nc * nc;
         if (fp_bj_gj_scratch_floats(nc, nc) > (size_t)3 * nc * nc)
             return fp_fail(ctx, FP_ERR_ARG, "internal: coarse inverse scratch too small");
-        int st = fp_bj_gj_inverse<double>(ctx, Einv, nc, nc, nc, inv32, info_dev, scr);
+        int st = fp_bj_gj_inverse<double>(ctx, Einv, nc, nc, nc, inv32, info_dev, scr, nullptr, false);
         if (st) return st;
         hipLaunchKernelGGL(k_widen_f32, dim3((unsigned)(((int64_t)nc * nc + 255) / 256)), dim3(256), 0, ctx->stream,
                            (const float *)inv32, (int64_t)nc * nc, Einv);

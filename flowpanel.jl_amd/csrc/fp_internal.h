@@ -255,7 +255,7 @@ int fp_colnorm2(fp_ctx *ctx, const double *R, int64_t n, int64_t ld, int64_t k, 
 size_t fp_bj_gj_scratch_floats(int64_t n, int block);
 template <typename T>
 int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *inv, int *info_dev,
-                     float *scratch, hipEvent_t copied = nullptr);
+                     float *scratch, hipEvent_t copied = nullptr, bool lookahead = true);
 int fp_bj_apply_inv(fp_ctx *ctx, const float *inv, int64_t n, int block, const double *x, double *y, int accumulate);
 // Galerkin coarse space of chunk indicators: AZ = A Z (n x nc), Einv = (Z^T A Z)^-1
 template <typename T>
