@@ -1763,6 +1763,28 @@ __global__ __launch_bounds__(256) void k_chunk_rowsum(const T *__restrict__ A, i
     }
 }
 
+// The same for chunks shorter than a wave: one lane per chunk, four rows per
+// workgroup (a wave-wide reduction per chunk would cost more than the chunk's
+// loads).
+template <typename T>
+__global__ __launch_bounds__(256) void k_chunk_rowsum_lane(const T *__restrict__ A, int64_t n, int64_t ld, int cs,
+                                                           int nc, double *__restrict__ AZ)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    const T *a = A + r * ld;
+    for (int k0 = 0; k0 < nc; k0 += 64) {
+        const int k = k0 + lane;
+        if (k >= nc) break;
+        const int64_t c0 = (int64_t)k * cs, c1 = min(n, c0 + cs);
+        double acc = 0;
+#pragma unroll 8
+        for (int64_t c = c0; c < c1; ++c) acc += (double)a[c];
+        AZ[(int64_t)k * n + r] = acc;
+    }
+}
+
 // E = Z^T AZ (nc x nc, row-major): E[k'][k] = sum over chunk k' of AZ[:, k]
 __global__ void k_coarse_galerkin(const double *__restrict__ AZ, int64_t n, int cs, int nc, double *__restrict__ E)
 {
@@ -1939,8 +1961,12 @@ int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, doub
     if (nc > 1024) return fp_fail(ctx, FP_ERR_ARG, "coarse space larger than 1024 vectors");
     ProfScope ps(ctx, "coarse_setup");
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, info_dev, (int64_t)1, 0);
-    hipLaunchKernelGGL(k_chunk_rowsum<T>, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, ctx->stream, A, n, ld, cs, nc,
-                       AZ);
+    if (cs < 64)
+        hipLaunchKernelGGL(k_chunk_rowsum_lane<T>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, A, n, ld,
+                           cs, nc, AZ);
+    else
+        hipLaunchKernelGGL(k_chunk_rowsum<T>, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, ctx->stream, A, n, ld, cs,
+                           nc, AZ);
     hipLaunchKernelGGL(k_coarse_galerkin, dim3(nc, nc), dim3(256), 0, ctx->stream, (const double *)AZ, n, cs, nc, Einv);
     if (nc <= SMALL_INV_LDS) {
         hipLaunchKernelGGL(k_small_inverse_lds, dim3(1), dim3(1024), 0, ctx->stream, Einv, nc, info_dev);
