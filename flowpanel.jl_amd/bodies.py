@@ -33,6 +33,26 @@ class VortexRing:
 ConstantDoublet = VortexRing
 
 
+def shedding_zero_based(shedding, index_base=0):
+    """The 6 x ns shedding matrix (pi, nia, nib, pj, nja, njb) with 0-based entries.
+
+    ``index_base=1`` takes a Julia (1-based) matrix as the reference builds it
+    (src/FLOWPanel_liftingbody.jl:24-31): every entry moves down by one except the
+    partner side of a one-sided edge, whose ``pj = -1`` sentinel and unused vertex
+    slots keep their values.
+    """
+    sh = np.zeros((6, 0), dtype=np.int64) if shedding is None else np.array(shedding, dtype=np.int64)
+    if sh.ndim != 2 or sh.shape[0] != 6:
+        raise ValueError("shedding must be a 6 x ns matrix, got %s" % (sh.shape,))
+    if index_base not in (0, 1):
+        raise ValueError("index_base must be 0 or 1")
+    if index_base:
+        one_sided = sh[3] == -1
+        sh[:3] -= index_base
+        sh[3:, ~one_sided] -= index_base
+    return sh
+
+
 class AbstractBody:
     def __init__(self, grid, element, shedding=None, CPoffset=1e-14, kerneloffset=1e-8, kernelcutoff=1e-14,
                  characteristiclength="unitary", index_base=0, kutta=None, ctx=None):
@@ -45,10 +65,8 @@ class AbstractBody:
         if characteristiclength not in nat.CHARLEN:
             raise ValueError("characteristiclength must be one of %s" % list(nat.CHARLEN))
         self.characteristiclength = characteristiclength
-        sh = np.zeros((6, 0), dtype=np.int64) if shedding is None else np.asarray(shedding, dtype=np.int64)
-        if sh.ndim != 2 or sh.shape[0] != 6:
-            raise ValueError("shedding must be a 6 x ns matrix, got %s" % (sh.shape,))
-        self.shedding = sh
+        sh = shedding_zero_based(shedding, index_base)
+        self.shedding = sh      # 0-based from here on (device tables, Kutta, forces)
         self.index_base = index_base
         self.fields = {}
         self.strength = np.zeros(grid.ncells)

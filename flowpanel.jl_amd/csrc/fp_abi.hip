@@ -120,6 +120,19 @@ struct Timer {
 
 bool check_unit(const double *d) { return std::fabs(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] - 1.0) <= 2 * DBL_EPSILON; }
 
+// Residual bookkeeping that never hides a NaN: the worst of two residuals is
+// NaN if either is, and a residual counts as converged only when it compares
+// <= tol (false for NaN).
+inline double worst_residual(double a, double b) { return (std::isnan(a) || std::isnan(b)) ? NAN : std::max(a, b); }
+inline bool converged(double rel, double tol) { return rel <= tol; }
+int not_converged(fp_ctx *ctx, double rel, const char *what)
+{
+    char buf[160];
+    if (!std::isfinite(rel)) snprintf(buf, sizeof buf, "%s produced a non-finite residual", what);
+    else snprintf(buf, sizeof buf, "%s did not reach the tolerance (relative residual %.3e)", what, rel);
+    return fp_fail(ctx, FP_ERR_NOT_CONVERGED, buf);
+}
+
 // Combined control points + normals of several bodies (SoA, M = sum N).
 struct Targets {
     DBuf buf;
@@ -338,6 +351,7 @@ struct Gmres {
         *its = 0;
         *rel = 0;
         FP_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
+        if (!std::isfinite(bnorm)) { *rel = NAN; return not_converged(ctx, NAN, "GMRES (right-hand side)"); }
         if (bnorm == 0) return FP_OK;
         std::vector<double> relh(m), hnh(m);
         bool first = true;
@@ -354,6 +368,7 @@ struct Gmres {
             first = false;
             if (st) return st;
             *rel = beta / bnorm;
+            if (!std::isfinite(*rel)) return not_converged(ctx, *rel, "GMRES");
             if (*rel <= tol || *its >= maxit) return FP_OK;
             if ((st = fp_axpby(ctx, 1.0 / beta, wp, 0.0, Vp, n))) return st;
             if ((st = fp_arnoldi_start(ctx, dg, m, beta))) return st;
@@ -426,7 +441,7 @@ struct Gmres {
             }
             if (jc < 0) jc = j;
             *rel = relh[jc - 1];
-            if (!std::isfinite(*rel)) return fp_fail(ctx, FP_ERR_NOT_CONVERGED, "GMRES produced a non-finite residual");
+            if (!std::isfinite(*rel)) return not_converged(ctx, *rel, "GMRES");
             if ((st = fp_arnoldi_solve(ctx, dH, m, jc, dg, dy))) return st;
             if ((st = fp_vcombine(ctx, Vp, n, jc, dy, tp, n))) return st;
             if (M) {
@@ -443,6 +458,7 @@ struct Gmres {
                 const double rn = dnorm_host(ctx, wp, n, pp, hp, &st);
                 if (st) return st;
                 *rel = rn / bnorm;
+                if (!std::isfinite(*rel)) return not_converged(ctx, *rel, "GMRES");
                 if (*rel <= tol || *its >= maxit) return FP_OK;
             }
         }
@@ -516,7 +532,28 @@ struct Solver {
         int st;
         if ((st = dalloc(ctx, info, sizeof(int)))) return st;
         M.reset(new BJPrecond(ctx, a.n, blk));
-        return M->setup(A, prec, a.ld, o.coarse_block, info.as<int>());
+        if ((st = M->setup(A, prec, a.ld, o.coarse_block, info.as<int>()))) return st;
+        // zero pivots of the block inverses (main stream) and of the coarse Galerkin
+        // matrix (side stream, joined into the main stream by setup)
+        int hinfo[2] = {0, 0};
+        FP_HIP(ctx, hipMemcpyAsync(&hinfo[0], info.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        if (M->cinfo.p)
+            FP_HIP(ctx, hipMemcpyAsync(&hinfo[1], M->cinfo.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        char buf[192];
+        if (hinfo[0]) {
+            const int64_t row = hinfo[0] - 1;
+            snprintf(buf, sizeof buf,
+                     "block-Jacobi preconditioner: zero pivot in diagonal block %lld (global row %lld)",
+                     (long long)(row / blk), (long long)row);
+            return fp_fail(ctx, FP_ERR_SINGULAR, buf);
+        }
+        if (hinfo[1]) {
+            snprintf(buf, sizeof buf, "coarse correction: zero pivot in the Galerkin matrix at row %d of %lld",
+                     hinfo[1] - 1, (long long)((a.n + o.coarse_block - 1) / o.coarse_block));
+            return fp_fail(ctx, FP_ERR_SINGULAR, buf);
+        }
+        return FP_OK;
     }
 
     // factorisation / preconditioner set-up (stats.ms_factor, stats.info)
@@ -583,7 +620,11 @@ struct Solver {
                 if ((st = fp_residual_update(ctx, b, rr.as<double>(), n))) return st;
                 const double bn = dnorm_host(ctx, b, n, part.as<double>(), dout.as<double>(), &st);
                 const double rn = dnorm_host(ctx, rr.as<double>(), n, part.as<double>(), dout.as<double>(), &st);
-                stats.residual = std::max(stats.residual, bn > 0 ? rn / bn : 0);
+                if (st) return st;
+                const double rel = bn > 0 ? rn / bn : (bn == 0 ? 0.0 : NAN);
+                stats.residual = worst_residual(stats.residual, rel);
+                // a direct solve has no tolerance; a NaN in its result is still an error
+                if (!std::isfinite(rel)) return not_converged(ctx, rel, "LU solve");
             }
             return FP_OK;
         }
@@ -593,8 +634,8 @@ struct Solver {
             if ((st = g->solve(*a.resid, M.get(), b, x, o.tol, o.max_iters, &its, &rel))) return st;
             stats.ms_solve += tm.stop();
             stats.iterations += its;
-            stats.residual = std::max(stats.residual, rel);
-            if (rel > o.tol) return fp_fail(ctx, FP_ERR_NOT_CONVERGED, "GMRES did not reach the tolerance");
+            stats.residual = worst_residual(stats.residual, rel);
+            if (!converged(rel, o.tol)) return not_converged(ctx, rel, "GMRES");
             return FP_OK;
         }
         // refinement paths: FP32 inner solver, FP64 residual and update
@@ -603,9 +644,9 @@ struct Solver {
         if (st) return st;
         FP_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
         FP_HIP(ctx, hipMemcpyAsync(rp, b, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-        double rel = bn > 0 ? 1.0 : 0.0, rnorm = bn;   // ||r|| of the current residual (r = b at first)
+        double rel = bn > 0 ? 1.0 : (bn == 0 ? 0.0 : NAN), rnorm = bn;   // ||r|| of the current residual (r = b at first)
         int k = 0, its_total = 0;
-        while (rel > o.tol && k < o.max_refine) {
+        while (!converged(rel, o.tol) && std::isfinite(rel) && k < o.max_refine) {
             // correction d ~ A^-1 r in the low-precision solver
             if (a.solver == FP_SOLVER_LU32_IR) {
                 FP_HIP(ctx, hipMemcpyAsync(dp, rp, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
@@ -628,15 +669,15 @@ struct Solver {
             if ((st = fp_residual_update(ctx, b, rp, n))) return st;
             const double rn = dnorm_host(ctx, rp, n, part.as<double>(), dout.as<double>(), &st);
             if (st) return st;
-            rel = bn > 0 ? rn / bn : 0.0;
+            rel = rn / bn;
             rnorm = rn;
             ++k;
         }
         stats.ms_solve += tm.stop();
         stats.refinements += k;
         stats.iterations += its_total;
-        stats.residual = std::max(stats.residual, rel);
-        if (rel > o.tol) return fp_fail(ctx, FP_ERR_NOT_CONVERGED, "iterative refinement did not reach the tolerance");
+        stats.residual = worst_residual(stats.residual, rel);
+        if (!converged(rel, o.tol)) return not_converged(ctx, rel, "iterative refinement");
         return FP_OK;
     }
     // X = A^-1 B for k right-hand sides (B, X column-major n x k).  LU paths
@@ -675,8 +716,10 @@ struct Solver {
         FP_HIP(ctx, hipMemcpyAsync(R.p, B, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
         double rel = 1.0;
+        for (int64_t c = 0; c < k; ++c)
+            if (!std::isfinite(bn[c])) rel = NAN;
         int it = 0;
-        while (rel > o.tol && it < o.max_refine) {
+        while (!converged(rel, o.tol) && std::isfinite(rel) && it < o.max_refine) {
             FP_HIP(ctx, hipMemcpyAsync(D.p, R.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
             if ((st = fp_getrs<float>(ctx, a.A32, n, a.ld, perm.as<int>(), D.as<double>(), k, n, wk.p, wbytes)))
                 return st;
@@ -687,13 +730,14 @@ struct Solver {
             FP_HIP(ctx, hipMemcpyAsync(rn.data(), nr.p, k * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
             FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
             rel = 0;
-            for (int64_t c = 0; c < k; ++c) rel = std::max(rel, bn[c] > 0 ? std::sqrt(rn[c] / bn[c]) : 0.0);
+            for (int64_t c = 0; c < k; ++c)
+                rel = worst_residual(rel, bn[c] > 0 ? std::sqrt(rn[c] / bn[c]) : (rn[c] == 0 ? 0.0 : NAN));
             ++it;
         }
         stats.ms_solve += tm.stop();
         stats.refinements += it;
-        stats.residual = std::max(stats.residual, rel);
-        if (rel > o.tol) return fp_fail(ctx, FP_ERR_NOT_CONVERGED, "iterative refinement did not reach the tolerance");
+        stats.residual = worst_residual(stats.residual, rel);
+        if (!converged(rel, o.tol)) return not_converged(ctx, rel, "iterative refinement");
         return FP_OK;
     }
 };
@@ -2017,7 +2061,7 @@ static int transpose_solves_concurrent(fp_ctx *ctx, const SolveArgs &at, const f
     if (st1) return fp_fail(ctx, st1, child.err.c_str());
     ts.iterations += ts1.iterations;
     ts.refinements += ts1.refinements;
-    ts.residual = std::max(ts.residual, ts1.residual);
+    ts.residual = worst_residual(ts.residual, ts1.residual);
     return FP_OK;
 }
 
@@ -2524,22 +2568,22 @@ int fp_batch_solve(fp_ctx *ctx, fp_batch *h, fp_batch_stats *stats)
     double worst = 0;
     for (size_t k = 0; k < nb; ++k) {
         BatchBody &bb = *B.bodies[k];
-        double rel = nr[k] > 0 ? std::sqrt(nr[nb + k] / nr[k]) : 0.0;
+        double rel = nr[k] > 0 ? std::sqrt(nr[nb + k] / nr[k]) : (nr[nb + k] == 0 ? 0.0 : NAN);
         int steps = B.lu32() ? B.sweeps : 0;
-        while (B.lu32() && rel > B.o.tol && steps < B.o.max_refine) {
+        while (B.lu32() && !converged(rel, B.o.tol) && std::isfinite(rel) && steps < B.o.max_refine) {
             if ((st = B.sweep(bb))) return st;
             if ((st = fp_colnorm2(ctx, bb.r.as<double>(), bb.n, bb.n, 1, B.norms.as<double>() + nb + k))) return st;
             double v = 0;
             FP_HIP(ctx, hipMemcpyAsync(&v, B.norms.as<double>() + nb + k, sizeof(double), hipMemcpyDeviceToHost,
                                        ctx->stream));
             FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            rel = nr[k] > 0 ? std::sqrt(v / nr[k]) : 0.0;
+            rel = nr[k] > 0 ? std::sqrt(v / nr[k]) : (v == 0 ? 0.0 : NAN);
             ++steps;
             ++extra;
             FP_HIP(ctx, hipMemcpyAsync(bb.b->d_strength, bb.x.p, bb.n * sizeof(double), hipMemcpyDeviceToDevice,
                                        ctx->stream));
         }
-        worst = std::max(worst, rel);
+        worst = worst_residual(worst, rel);
         S.panels += bb.n;
     }
     FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -2553,7 +2597,7 @@ int fp_batch_solve(fp_ctx *ctx, fp_batch *h, fp_batch_stats *stats)
     S.nbodies = (int)nb;
     for (auto &bb : B.bodies) bb->b->solved = true;
     if (stats) *stats = S;
-    if (worst > B.o.tol) return fp_fail(ctx, FP_ERR_NOT_CONVERGED, "batch refinement did not reach the tolerance");
+    if (!converged(worst, B.o.tol)) return not_converged(ctx, worst, "batch refinement");
     return FP_OK;
 }
 

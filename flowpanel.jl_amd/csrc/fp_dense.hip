@@ -1800,79 +1800,18 @@ __global__ void k_coarse_galerkin(const double *__restrict__ AZ, int64_t n, int 
     if (tid == 0) E[(int64_t)kp * nc + k] = red[0] + red[1] + red[2] + red[3];
 }
 
-// In-place inverse of a small dense matrix (m <= 1024, row-major) by
-// Gauss-Jordan elimination with partial pivoting; one workgroup.  The pivot of
-// each column is found by all lanes at once (wave shuffles, then the 16 wave
-// winners in LDS); ties go to the lowest row.
+// Larger of two (|value|, row) candidates, the lower row on ties; a NaN never wins.
 __device__ __forceinline__ void argmax_pair(double &v, int &r, double v2, int r2)
 {
     if (v2 > v || (v2 == v && r2 < r)) { v = v2; r = r2; }
 }
 
-__global__ __launch_bounds__(1024) void k_small_inverse(double *__restrict__ E, int m, double *__restrict__ work,
-                                                        int *__restrict__ info)
-{
-    // work: m x 2m augmented matrix [E | I]
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int w2 = 2 * m;
-    __shared__ double wv[16];
-    __shared__ int wr[16];
-    __shared__ int prow;
-    __shared__ double pval;
-    for (int e = tid; e < m * w2; e += blockDim.x) {
-        const int r = e / w2, c = e % w2;
-        work[e] = c < m ? E[(int64_t)r * m + c] : (c - m == r ? 1.0 : 0.0);
-    }
-    __syncthreads();
-    for (int c = 0; c < m; ++c) {
-        double v = -1.0;
-        int r = m;
-        for (int rr = c + tid; rr < m; rr += blockDim.x) argmax_pair(v, r, fabs(work[(int64_t)rr * w2 + c]), rr);
-        for (int o = 32; o > 0; o >>= 1) {
-            const double v2 = __shfl_xor(v, o);
-            const int r2 = __shfl_xor(r, o);
-            argmax_pair(v, r, v2, r2);
-        }
-        if (lane == 0) { wv[wave] = v; wr[wave] = r; }
-        __syncthreads();
-        if (tid == 0) {
-            double bv = wv[0];
-            int br = wr[0];
-            for (int q = 1; q < (int)(blockDim.x >> 6); ++q) argmax_pair(bv, br, wv[q], wr[q]);
-            prow = br;
-            pval = work[(int64_t)br * w2 + c];
-            if (pval == 0.0 && *info == 0) *info = c + 1;
-        }
-        __syncthreads();
-        const int p = prow;
-        if (p != c)
-            for (int e = tid; e < w2; e += blockDim.x) {
-                const double t = work[(int64_t)c * w2 + e];
-                work[(int64_t)c * w2 + e] = work[(int64_t)p * w2 + e];
-                work[(int64_t)p * w2 + e] = t;
-            }
-        __syncthreads();
-        const double inv = pval != 0.0 ? 1.0 / pval : 0.0;
-        for (int e = tid; e < w2; e += blockDim.x) work[(int64_t)c * w2 + e] *= inv;
-        __syncthreads();
-        for (int e = tid; e < m * w2; e += blockDim.x) {
-            const int r = e / w2, cc = e % w2;
-            if (r != c && cc != c) work[e] -= work[(int64_t)r * w2 + c] * work[(int64_t)c * w2 + cc];
-        }
-        __syncthreads();
-        for (int r = tid; r < m; r += blockDim.x)
-            if (r != c) work[(int64_t)r * w2 + c] = 0.0;
-        __syncthreads();
-    }
-    for (int e = tid; e < m * m; e += blockDim.x) {
-        const int r = e / m, c = e % m;
-        E[e] = work[(int64_t)r * w2 + m + c];
-    }
-}
-
-// m <= SMALL_INV_LDS: the same inverse in place in LDS (Gauss-Jordan with row
-// interchanges, the pivot row scaled by the reciprocal pivot, the columns
-// swapped back in reverse order at the end); ties go to the lowest row.
+// In-place inverse of a small dense matrix (m <= SMALL_INV_LDS, row-major) in
+// LDS by Gauss-Jordan elimination with partial pivoting (row interchanges, the
+// pivot row scaled by the reciprocal pivot, the columns swapped back in reverse
+// order at the end); one workgroup.  The pivot of each column is found by all
+// lanes at once (wave shuffles, then the 16 wave winners in LDS); ties go to the
+// lowest row, and a column without a finite candidate keeps its diagonal row.
 constexpr int SMALL_INV_LDS = 128;
 
 __global__ __launch_bounds__(1024) void k_small_inverse_lds(double *__restrict__ E, int m, int *__restrict__ info)

@@ -1,0 +1,204 @@
+"""Parity of the benchmark workloads and of the remaining element paths against the oracle.
+
+* The large-body workload (blended-wing-body loft, seed 5) at a reduced size with
+  the benchmark's own preconditioner layout (blocks of four chordwise strips,
+  coarse chunks of a fifth of a strip: chunks shorter than a wave and more than
+  128 coarse vectors) against the oracle's LAPACK solve; and at its full size
+  (250 x 120 = 30,000 panels) against the committed oracle fixture
+  (tests/golden/make_bwb_fixture.py).
+* Every body of an uneven body batch against its own oracle LAPACK solve.
+* The non-planar semi-infinite doublet potential (Das != Dbs) and the constant
+  source potential on the device.
+* A 1-based (Julia) shedding matrix gives the same strengths as the 0-based one.
+* A non-finite matrix is reported as an error, never as a converged solve.
+
+Tolerances: strengths 1e-10 relative to max |Gamma| (the FP64-path bound);
+Cp 1e-6 absolute at the control points for the mixed solve at the benchmark's
+1e-8 residual target (the bound of the headline metric).
+"""
+import os
+import types
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
+
+
+def _obody(b):
+    return types.SimpleNamespace(nodes=b.grid.nodes, cell_ptr=b.grid.cell_ptr, cell_idx=b.grid.cell_idx,
+                                 ncells=b.ncells, element="vortexring" if b.element.code else "source",
+                                 shedding=b.shedding, nsheddings=b.nsheddings, kerneloffset=b.kerneloffset,
+                                 kernelcutoff=b.kernelcutoff, CPoffset=b.CPoffset, charlen_kind=0)
+
+
+def _bwb(fp, nc, ns, alpha=5.0):
+    grid, sh = fp.geometry.bwb(seed=5, n_chord=nc, n_span=ns)
+    body = fp.RigidWakeBody(grid, sh)
+    V = fp.direction(alpha)
+    U = np.tile(V[:, None], (1, body.ncells))
+    D = np.tile(V[:, None], (1, body.nsheddings))
+    return body, U, D
+
+
+def _bench_opts(nc, tol, inner_tol=1e-6):
+    # the benchmark's layout: block-Jacobi blocks of four chordwise strips,
+    # Galerkin coarse chunks of a fifth of a strip
+    return {"precond_block": 4 * nc, "coarse_block": nc // 5, "tol": tol, "inner_tol": inner_tol,
+            "restart": 100, "residual_mode": 1}
+
+
+def _cp_nokutta(fp, body):
+    fp.calcfield_U(body, body)
+    return fp.calcfield_Cp(body, 1.0, correct_kuttacondition=False)
+
+
+def test_bwb_reduced_bench_layout_matches_oracle(fp, orc):
+    nc, ns = 50, 40                     # 2,000 panels; chunks of 10 -> 200 coarse vectors
+    body, U, D = _bwb(fp, nc, ns)
+    Go, normals, cps = orc.assemble([_obody(body)], [D], [D])
+    gam = orc.solve_lu(Go, orc.rhs(U, normals))
+    cp_o = orc.cp(U + orc.uind(_obody(body), gam, cps, D, D), 1.0)
+    # FP64 LU path (the comparator of the headline metric)
+    fp.solve(body, U, D, D, solver=fp.solve_ludiv)
+    assert rel(body.strength, gam) < 1e-10
+    # the mixed path driven to the FP64-path tolerance
+    st = fp.solve(body, U, D, D, solver=fp.solve_gmres_ir, solver_optargs=_bench_opts(nc, 1e-12, 1e-6))
+    assert rel(body.strength, gam) < 1e-10, st
+    assert np.abs(_cp_nokutta(fp, body) - cp_o).max() < 1e-9
+    # the mixed path at the benchmark's residual target: Cp inside the metric's bound
+    st = fp.solve(body, U, D, D, solver=fp.solve_gmres_ir, solver_optargs=_bench_opts(nc, 1e-8))
+    assert st["residual"] <= 1e-8
+    assert np.abs(_cp_nokutta(fp, body) - cp_o).max() < 1e-6, st
+    # the coarse correction earns its keep on this layout
+    plain = fp.solve(body, U, D, D, solver=fp.solve_gmres_ir,
+                     solver_optargs=dict(_bench_opts(nc, 1e-8), coarse_block=0))
+    assert st["iterations"] < plain["iterations"], (st, plain)
+
+
+def test_bwb_full_size_matches_oracle_fixture(fp, orc):
+    path = os.path.join(GOLDEN, "bwb30k_oracle.npz")
+    ref = np.load(path)
+    nc, ns = int(ref["n_chord"]), int(ref["n_span"])
+    body, U, D = _bwb(fp, nc, ns)
+    import hashlib
+    h = hashlib.sha256(np.ascontiguousarray(body.grid.nodes, dtype=np.float64).tobytes()).hexdigest()
+    assert h == str(ref["node_sha256"]), "the BWB generator no longer builds the fixture's body"
+    assert float(ref["residual"]) < 1e-12
+    # the benchmark's mixed solve at its residual target
+    st = fp.solve(body, U, D, D, solver=fp.solve_gmres_ir, solver_optargs=_bench_opts(nc, 1e-8))
+    cp = _cp_nokutta(fp, body)
+    assert np.abs(cp - ref["cp"]).max() < 1e-6, st
+    assert rel(body.strength, ref["gamma"]) < 1e-7, st
+    # driven to 1e-12: the FP64-path bound on the strengths
+    st = fp.solve(body, U, D, D, solver=fp.solve_gmres_ir, solver_optargs=_bench_opts(nc, 1e-12, 1e-5))
+    assert rel(body.strength, ref["gamma"]) < 1e-10, st
+
+
+def _wings(fp, geo, sizes, seed=3):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for nc, ns in sizes:
+        ar = rng.uniform(6, 12)
+        g, sh = geo.loft_wing(airfoil="0012", n_chord=nc, n_span=ns, span=ar * 0.75, root_chord=1.0, taper=0.75,
+                              sweep_deg=rng.uniform(0, 35))
+        out.append(fp.RigidWakeBody(g, sh))
+    return out
+
+
+@pytest.mark.parametrize("solver,resid", [("solve_lu32_ir", 0), ("solve_lu32_ir", 1), ("solve_ludiv", 1)])
+def test_body_batch_matches_oracle(fp, geo, orc, solver, resid):
+    bodies = _wings(fp, geo, [(20, 10), (34, 18), (12, 6), (26, 30), (40, 8)])
+    refs = []
+    for k, b in enumerate(bodies):
+        V = fp.direction(2.0 + k, 0.5 * k)
+        U = np.tile(V[:, None], (1, b.ncells))
+        D = np.tile(V[:, None], (1, b.nsheddings))
+        fp.solvers.set_freestream(b, U, D, D)
+        Go, normals, _ = orc.assemble([_obody(b)], [D], [D])
+        refs.append(orc.solve_lu(Go, orc.rhs(U, normals)))
+    batch = fp.BodyBatch(bodies, solver=getattr(fp, solver), solver_optargs={"residual_mode": resid, "tol": 1e-12})
+    for rep in range(2):
+        st = batch.solve()
+        assert st["nbodies"] == 5 and st["residual"] <= 1e-12
+        for k, (b, ref) in enumerate(zip(bodies, refs)):
+            assert rel(b.strength, ref) < 1e-10, (rep, k, rel(b.strength, ref))
+
+
+def test_nonplanar_wake_potential_matches_oracle(fp, geo, orc):
+    # Das != Dbs: every shedding edge carries the non-planar semi-infinite sheet
+    # (src/FLOWPanel_elements.jl:1110-1244) in phi!, and horseshoes with two
+    # directions in the AIC
+    g, sh = geo.loft_wing(n_chord=16, n_span=10, span=5.0, sweep_deg=15.0)
+    body = fp.RigidWakeBody(g, sh)
+    V = fp.direction(4.0)
+    U = np.tile(V[:, None], (1, body.ncells))
+    ns = body.nsheddings
+    ang = np.linspace(-0.25, 0.25, ns + 1)
+    Da = np.stack([np.cos(ang[:-1]), np.zeros(ns), np.sin(ang[:-1])])
+    Db = np.stack([np.cos(ang[1:]), np.zeros(ns), np.sin(ang[1:])])
+    for D in (Da, Db):
+        D /= np.linalg.norm(D, axis=0)
+    fp.solve(body, U, Da, Db)
+    Go, normals, cps = orc.assemble([_obody(body)], [Da], [Db])
+    gam = orc.solve_lu(Go, orc.rhs(U, normals))
+    assert rel(body.strength, gam) < 1e-10
+    rng = np.random.default_rng(11)
+    T = np.asfortranarray(rng.uniform([-1, -3, -1], [4, 3, 1], size=(400, 3)).T)
+    ph = np.zeros(400)
+    fp.phi(body, T, ph)
+    pr = orc.phi(_obody(body), body.strength, T, Da, Db)
+    # the non-planar triangle reaches 1e12 chords: the sums cancel to ~1e-12 of max |phi|
+    assert rel(ph, pr) < 1e-9
+    out = np.zeros((3, 400), order="F")
+    fp.Uind(body, T, out)
+    assert rel(out, orc.uind(_obody(body), body.strength, T, Da, Db)) < 1e-10
+
+
+def test_source_potential_matches_oracle(fp, geo, orc):
+    body = fp.NonLiftingBody(geo.sphere(n_meridional=10, n_azimuthal=20))
+    U = np.tile([[1.0], [0.2], [-0.1]], (1, body.ncells))
+    fp.solve(body, U)
+    rng = np.random.default_rng(4)
+    d = rng.standard_normal((3, 300))
+    d /= np.linalg.norm(d, axis=0)
+    T = np.asfortranarray(d * rng.uniform(0.5, 3.0, 300))     # inside and outside the sphere
+    ph = np.zeros(300)
+    fp.phi(body, T, ph)
+    pr = orc.phi(_obody(body), body.strength, T)
+    assert rel(ph, pr) < 1e-10
+
+
+def test_one_based_shedding_gives_the_same_strengths(fp, geo):
+    g, sh = geo.loft_wing(n_chord=12, n_span=6, span=3.0)
+    a = fp.RigidWakeBody(g, sh)
+    sh1 = sh.copy()
+    one = sh1[3] == -1
+    sh1[:3] += 1
+    sh1[3:, ~one] += 1
+    b = fp.RigidWakeBody(g, sh1, index_base=1)
+    assert np.array_equal(b.shedding, sh)
+    V = fp.direction(3.0)
+    U = np.tile(V[:, None], (1, a.ncells))
+    D = np.tile(V[:, None], (1, a.nsheddings))
+    fp.solve(a, U, D, D)
+    fp.solve(b, U, D, D)
+    assert np.array_equal(a.strength, b.strength)
+
+
+@pytest.mark.parametrize("solver", ["solve_ludiv", "solve_lu32_ir", "solve_gmres_ir", "solve_gmres"])
+def test_nan_matrix_is_an_error(fp, solver):
+    rng = np.random.default_rng(9)
+    n = 200
+    A = rng.standard_normal((n, n)) + 3 * n ** 0.5 * np.eye(n)
+    A[17, 40] = np.nan
+    x = np.zeros(n)
+    with pytest.raises(fp.FPError) as e:
+        getattr(fp, solver)(x, A, rng.standard_normal(n), **({"precond_block": 50} if "gmres" in solver else {}))
+    assert e.value.code in (6, 7), e.value

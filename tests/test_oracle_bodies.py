@@ -105,3 +105,42 @@ def test_forces_and_lds(orc):
     L = orc.lds(F, [0, 0, 1.0], [1.0, 0, 0])
     assert np.allclose(L[:, 0], [0, 0, F[2].sum()])
     assert np.allclose(L[:, 2], np.cross([0, 0, 1.0], [1.0, 0, 0]) * F[1].sum())
+
+
+def _cells(*cells):
+    ptr = np.cumsum([0] + [len(c) for c in cells]).astype(np.int64)
+    return ptr, np.concatenate([np.asarray(c, np.int64) for c in cells])
+
+
+def test_panel_area_definition(orc):
+    """Panel area = |sum of the fan cross products| / 2, the magnitude of the vector area.
+
+    Reference panels are triangles (every body holds a gt.GridTriangleSurface,
+    src/FLOWPanel_abstractbody.jl:905-919), where all definitions agree with
+    |(p2-p1) x (p3-p1)|/2.  A polygon of more vertices takes the magnitude of its
+    vector area, which depends only on its boundary; a planted mistake (the sum of
+    the triangle magnitudes, or a dropped fan triangle) fails one of the cases.
+    """
+    h = 0.6
+    nodes = np.asfortranarray(np.array([
+        [0, 0, 0], [1, 0, 0], [1, 1, h], [0, 1, 0],                          # non-planar quad
+        [0, 0, 5], [2, 0, 5], [2.5, 1.5, 5], [1, 2.5, 5], [-0.5, 1, 5],       # planar pentagon (z = 5)
+        [3, 0, 0], [4, 0.5, 1], [3.2, 2, -1],                                 # triangle
+    ], float).T)
+    ptr, idx = _cells([0, 1, 2, 3], [4, 5, 6, 7, 8], [9, 10, 11])
+    _, _, areas = orc.geometry(nodes, ptr, idx)
+    # non-planar quad: fan vectors (0,-h,1)/2 + (-h,0,1)/2 -> |.| = sqrt(1 + h^2/2)
+    assert abs(areas[0] - np.sqrt(1 + h * h / 2)) < 1e-15
+    assert abs(areas[0] - np.sqrt(1 + h * h)) > 0.05      # the sum of triangle magnitudes differs
+    # planar pentagon: shoelace formula in its plane
+    x, y = nodes[0, 4:9], nodes[1, 4:9]
+    shoelace = 0.5 * abs(np.dot(x, np.roll(y, -1)) - np.dot(y, np.roll(x, -1)))
+    assert abs(areas[1] - shoelace) < 1e-14
+    # triangle: |cross| / 2
+    p = nodes[:, 9:12]
+    assert abs(areas[2] - 0.5 * np.linalg.norm(np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]))) < 1e-15
+    # a quad made of the reference's two triangles (p1,p2,p3), (p1,p3,p4) carries, at
+    # one pressure, the force of its vector area: A n summed over the pair
+    t1 = 0.5 * np.cross(nodes[:, 1] - nodes[:, 0], nodes[:, 2] - nodes[:, 0])
+    t2 = 0.5 * np.cross(nodes[:, 2] - nodes[:, 0], nodes[:, 3] - nodes[:, 0])
+    assert abs(np.linalg.norm(t1 + t2) - areas[0]) < 1e-15

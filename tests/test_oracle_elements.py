@@ -270,3 +270,86 @@ def test_offset_cutoff_regularisation(orc):
     U = np.zeros((3, 1), order="F")
     orc.U_boundvortex(np.zeros(3), np.array([1.0, 0, 0]), 1.0, np.array([[2.0], [0.0], [0.0]]), U)
     assert np.all(U == 0)
+
+
+@pytest.fixture(scope="module")
+def te_nonplanar(te_panel):
+    # the same trailing edge with db turned 0.3 rad out of the da-plane: the
+    # non-planar semi-infinite sheet (src/FLOWPanel_elements.jl:1110-1244)
+    R = rot(30, 10, 40)
+    pa, pb, da = te_panel
+    db = R.T @ np.array([np.cos(0.3), 0.0, np.sin(0.3)])
+    return pa, pb, da / np.linalg.norm(da), db / np.linalg.norm(db)
+
+
+def _phi2(orc, pa, pb, da, db):
+    def f(Y):
+        out = np.zeros(1)
+        orc.phi_semiinfinite_doublet2(pa, pb, da, db, 1.0, Y.reshape(3, 1), out)
+        return out
+    return f
+
+
+def test_nonplanar_semiinfinite_U_is_grad_phi(orc, te_nonplanar):
+    # A constant doublet sheet equals the vortex loop on its boundary, so the
+    # gradient of the non-planar sheet (planar strip along da plus the triangle
+    # (pb, pb + L da, pb + L db)) is the horseshoe velocity with legs da and db
+    # (U_semiinfinite_doublet == U_semiinfinite_horseshoe, src/FLOWPanel_elements.jl:948).
+    pa, pb, da, db = te_nonplanar
+    assert np.dot(da, db) > 0
+    rng = np.random.default_rng(5)
+    phi = _phi2(orc, pa, pb, da, db)
+    for _ in range(4):
+        X = 2 * rng.random(3)
+        U = np.zeros((3, 1), order="F")
+        orc.U_semiinfinite_horseshoe(pa, pb, da, db, 1.0, X.reshape(3, 1), U, cutoff=0.0, offset=0.0)
+        g = jac(phi, X, h=1e-4)[0]
+        assert np.abs(g - U[:, 0]).max() <= 1e-9 * np.abs(U).max()
+
+
+def test_nonplanar_semiinfinite_check_has_teeth(orc, te_nonplanar):
+    # A planted mistake, built from the oracle on the CPU: the correction triangle
+    # with reversed orientation, phi_planar - triangle = 2 phi_planar - phi2.  The
+    # gradient check above must reject it.
+    pa, pb, da, db = te_nonplanar
+    X = np.array([0.7, 1.3, 0.4])
+    good = _phi2(orc, pa, pb, da, db)
+
+    def planar(Y):
+        out = np.zeros(1)
+        orc.phi_semiinfinite_doublet(pa, pb, da, 1.0, Y.reshape(3, 1), out)
+        return out
+
+    U = np.zeros((3, 1), order="F")
+    orc.U_semiinfinite_horseshoe(pa, pb, da, db, 1.0, X.reshape(3, 1), U, cutoff=0.0, offset=0.0)
+    bad = jac(lambda Y: 2 * planar(Y) - good(Y), X, h=1e-4)[0]
+    assert np.abs(bad - U[:, 0]).max() > 1e-3 * np.abs(U).max()
+    # and a planar-only sheet (the triangle dropped) fails as well
+    drop = jac(planar, X, h=1e-4)[0]
+    assert np.abs(drop - U[:, 0]).max() > 1e-3 * np.abs(U).max()
+
+
+@pytest.mark.parametrize("hsign", [1.0, -1.0])
+def test_nonplanar_semiinfinite_jump_on_the_triangle(orc, hsign):
+    # phi(+-0) = -+mu/2 across the correction triangle, far from the strip:
+    # da = x, db in the x-z plane, the TE along y; a point above the triangle
+    # (pb, pb + L da, pb + L db) at distance h from it.
+    pa, pb = np.array([0.0, -1.0, 0.0]), np.array([0.0, 1.0, 0.0])
+    da = np.array([1.0, 0.0, 0.0])
+    db = np.array([np.cos(0.4), 0.0, np.sin(0.4)])
+    # the triangle's unit normal (da x db) is -y: a point on the triangle's plane
+    # (y = 1) between the legs, then offset along -y by h
+    nz = np.cross(da, db)
+    nz /= np.linalg.norm(nz)
+    P = pb + 5.0 * (np.cos(0.2) * np.array([1.0, 0, 0]) + np.sin(0.2) * np.array([0, 0, 1.0]))
+    h = hsign * 1e-8
+    T = (P + h * nz).reshape(3, 1)
+    out = np.zeros(1)
+    orc.phi_semiinfinite_doublet2(pa, pb, da, db, 1.0, np.asfortranarray(T), out)
+    # the planar strip along da lies at y in [-1, 1], z = 0: its solid angle at P
+    # (distance ~ 5 sin(0.2) off its plane, outside it) is smooth across the
+    # triangle; the jump of the sheet's potential across the triangle is mu
+    out2 = np.zeros(1)
+    T2 = (P - h * nz).reshape(3, 1)
+    orc.phi_semiinfinite_doublet2(pa, pb, da, db, 1.0, np.asfortranarray(T2), out2)
+    assert abs((out[0] - out2[0]) - np.sign(h) * 1.0) < 1e-6
