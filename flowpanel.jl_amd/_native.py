@@ -36,6 +36,9 @@ FP_RESID_STORED = 1
 FP_KUTTA_NONE = 0
 FP_KUTTA_REFERENCE = 1
 FP_KUTTA_PAIR = 2
+FP_SHARPTE_OFF = 0
+FP_SHARPTE_PAIRS = 1
+FP_SHARPTE_CELLS = 2
 CHARLEN = {"unitary": 0, "bbox": 1, "maxdist": 2, "sqrtarea": 3}
 
 EXPORTS = [
@@ -167,8 +170,8 @@ def lib():
     L.fp_solve_bodies_lsq.argtypes = [_vp, pp, cint, _i64, _d, i64, cint, ctypes.POINTER(SolveOpts),
                                       ctypes.POINTER(SolveStats)]
     L.fp_calcfield_ugradmu.argtypes = [_vp, pp, cint, cint, _d]
-    L.fp_calcfield_ugradmu_node.argtypes = [_vp, pp, cint, _d, _d]
-    L.fp_calcfield_ugradmu_combined.argtypes = [_vp, pp, cint, cint, ctypes.c_double, _d]
+    L.fp_calcfield_ugradmu_node.argtypes = [_vp, pp, cint, cint, _d, _d]
+    L.fp_calcfield_ugradmu_combined.argtypes = [_vp, pp, cint, cint, ctypes.c_double, cint, _d]
     L.fp_body_nnodes.argtypes = [_vp]
     L.fp_body_nnodes.restype = i64
     L.fp_body_set_nodes.argtypes = [_vp, _vp, _d]

@@ -30,7 +30,11 @@ class VortexRing:
     field = "Gamma"
 
 
-ConstantDoublet = VortexRing
+class ConstantDoublet(VortexRing):
+    """Constant-strength doublet panel: for the velocity a vortex ring of the same
+    strength (U_constant_doublet, src/FLOWPanel_elements.jl:740), with the doublet
+    potential; its strength field is "mu"."""
+    field = "mu"
 
 
 def shedding_zero_based(shedding, index_base=0):
@@ -120,11 +124,14 @@ class AbstractBody:
 
 
 class NonLiftingBody(AbstractBody):
-    """NonLiftingBody{ConstantSource}(grid)."""
+    """NonLiftingBody{ConstantSource}(grid) (src/FLOWPanel_nonliftingbody.jl:32-130) or
+    NonLiftingBody{ConstantDoublet}(grid) (src/FLOWPanel_nonliftingbody.jl:258-352): a body
+    without wake; the doublet body assembles U_constant_doublet . n (a vortex ring without
+    shedding edges) and solves G mu = -Uinf . n."""
 
     def __init__(self, grid, element=ConstantSource, **kw):
-        if element is not ConstantSource:
-            raise NotImplementedError("NonLiftingBody supports ConstantSource elements")
+        if element not in (ConstantSource, ConstantDoublet):
+            raise NotImplementedError("NonLiftingBody supports ConstantSource and ConstantDoublet elements")
         super().__init__(grid, element, None, **kw)
 
 

@@ -2727,16 +2727,66 @@ struct NodeData {
     }
 };
 
+// Sharp trailing edge: side 1 = the shedding panels pi (with pi + 1 when `pairs`,
+// the two triangles of a structured triangulated grid's last quad), side 2 = the
+// partners pj (with pj - 1); the TE nodes are the shedding edges' nodes.
+struct TeTables {
+    DBuf side, te, node_te, vals;
+    TeSides view;
+    int build(fp_ctx *ctx, const fp_body *b, int64_t nn, bool pairs)
+    {
+        const int64_t N = b->N;
+        std::vector<int> side(N, 0), node_te(std::max<int64_t>(1, nn), -1), te;
+        auto mark = [&](int64_t c, int s) { if (c >= 0 && c < N) side[c] = s; };
+        for (int s = 1; s <= 2; ++s)   // side 2 last: it wins where the two overlap
+            for (int64_t e = 0; e < b->ns; ++e) {
+                const int64_t p = b->shedding[6 * e + 3 * (s - 1)];
+                if (p < 0) continue;
+                mark(p, s);
+                if (pairs) mark(s == 1 ? p + 1 : p - 1, s);
+            }
+        for (int64_t e = 0; e < b->ns; ++e)
+            for (int sd = 0; sd < 2; ++sd) {
+                const int64_t p = b->shedding[6 * e + 3 * sd];
+                if (p < 0) continue;
+                for (int q = 1; q <= 2; ++q) {
+                    const int64_t v = b->cell_idx[b->cell_ptr[p] + b->shedding[6 * e + 3 * sd + q]];
+                    if (node_te[v] < 0) { node_te[v] = (int)te.size(); te.push_back((int)v); }
+                }
+            }
+        int st;
+        const int nte = (int)te.size();
+        if ((st = dalloc(ctx, this->side, N * sizeof(int))) || (st = dalloc(ctx, this->te, std::max(1, nte) * sizeof(int))) ||
+            (st = dalloc(ctx, this->node_te, node_te.size() * sizeof(int))) ||
+            (st = dalloc(ctx, vals, 2 * std::max(1, nte) * sizeof(double))))
+            return st;
+        FP_HIP(ctx, hipMemcpyAsync(this->side.p, side.data(), N * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        if (nte)
+            FP_HIP(ctx, hipMemcpyAsync(this->te.p, te.data(), nte * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        FP_HIP(ctx, hipMemcpyAsync(this->node_te.p, node_te.data(), node_te.size() * sizeof(int), hipMemcpyHostToDevice,
+                                   ctx->stream));
+        FP_HIP(ctx, hipStreamSynchronize(ctx->stream));   // the host vectors go out of scope
+        view.cell_side = this->side.as<int>();
+        view.te_nodes = this->te.as<int>();
+        view.node_te = this->node_te.as<int>();
+        view.sidev = vals.as<double>();
+        view.nte = nte;
+        return FP_OK;
+    }
+};
+
 }  // namespace
 
 extern "C" {
 
-int fp_calcfield_ugradmu_node(fp_ctx *ctx, fp_body *const *bodies, int nb, double *out, double *nodal_out)
+int fp_calcfield_ugradmu_node(fp_ctx *ctx, fp_body *const *bodies, int nb, int sharp_te, double *out,
+                              double *nodal_out)
 {
     int64_t M = 0;
     int st = ugradmu_inputs(ctx, bodies, nb, &M);
     if (st) return st;
     if (!out) return fp_fail(ctx, FP_ERR_ARG, "null out");
+    if (sharp_te < FP_SHARPTE_OFF || sharp_te > FP_SHARPTE_CELLS) return fp_fail(ctx, FP_ERR_ARG, "unknown sharp_te mode");
     FP_HIP(ctx, hipSetDevice(ctx->device));
     DBuf o;
     if ((st = dalloc(ctx, o, 3 * M * sizeof(double)))) return st;
@@ -2746,8 +2796,11 @@ int fp_calcfield_ugradmu_node(fp_ctx *ctx, fp_body *const *bodies, int nb, doubl
         fp_body *b = bodies[k];
         NodeData nd;
         if ((st = nd.build(ctx, b))) return st;
+        TeTables tt;
+        const bool sharp = sharp_te != FP_SHARPTE_OFF && b->ns > 0;
+        if (sharp && (st = tt.build(ctx, b, nd.nn, sharp_te == FP_SHARPTE_PAIRS))) return st;
         if ((st = fp_launch_ugradmu_node(ctx, b, nd.nptr.as<int64_t>(), nd.ncell.as<int>(), nd.nn, nd.cn.as<int>(),
-                                         nd.nodal.as<double>(), o.as<double>() + 3 * off)))
+                                         nd.nodal.as<double>(), o.as<double>() + 3 * off, sharp ? &tt.view : nullptr)))
             return st;
         if (nodal_out && nd.nn > 0)
             FP_HIP(ctx, hipMemcpyAsync(nodal_out + noff, nd.nodal.p, nd.nn * sizeof(double), hipMemcpyDeviceToHost,
@@ -2762,12 +2815,13 @@ int fp_calcfield_ugradmu_node(fp_ctx *ctx, fp_body *const *bodies, int nb, doubl
 }
 
 int fp_calcfield_ugradmu_combined(fp_ctx *ctx, fp_body *const *bodies, int nb, int force_cell_te,
-                                  double anglecrit_deg, double *out)
+                                  double anglecrit_deg, int sharp_te, double *out)
 {
     int64_t M = 0;
     int st = ugradmu_inputs(ctx, bodies, nb, &M);
     if (st) return st;
     if (!out) return fp_fail(ctx, FP_ERR_ARG, "null out");
+    if (sharp_te < FP_SHARPTE_OFF || sharp_te > FP_SHARPTE_CELLS) return fp_fail(ctx, FP_ERR_ARG, "unknown sharp_te mode");
     FP_HIP(ctx, hipSetDevice(ctx->device));
     DBuf o, cellv, nodev;
     if ((st = dalloc(ctx, o, 3 * M * sizeof(double)))) return st;
@@ -2788,10 +2842,13 @@ int fp_calcfield_ugradmu_combined(fp_ctx *ctx, fp_body *const *bodies, int nb, i
         FP_HIP(ctx, hipMemsetAsync(nodev.p, 0, 3 * N * sizeof(double), ctx->stream));
         NodeData nd;
         if ((st = nd.build(ctx, b))) return st;
+        TeTables tt;
+        const bool sharp = sharp_te != FP_SHARPTE_OFF && b->ns > 0;
+        if (sharp && (st = tt.build(ctx, b, nd.nn, sharp_te == FP_SHARPTE_PAIRS))) return st;
         // cell-centred values with the trailing-edge treatment, node-based values, then the choice per cell
         if ((st = fp_launch_ugradmu(ctx, b, b->d_nbr, b->d_sh, 1, cellv.as<double>()))) return st;
         if ((st = fp_launch_ugradmu_node(ctx, b, nd.nptr.as<int64_t>(), nd.ncell.as<int>(), nd.nn, nd.cn.as<int>(),
-                                         nd.nodal.as<double>(), nodev.as<double>())))
+                                         nd.nodal.as<double>(), nodev.as<double>(), sharp ? &tt.view : nullptr)))
             return st;
         if ((st = fp_launch_ugradmu_combine(ctx, b, b->d_nbr, b->d_sh, force_cell_te, cosc, cellv.as<double>(),
                                             nodev.as<double>(), o.as<double>() + 3 * off)))

@@ -295,8 +295,17 @@ int fp_arnoldi_step2(fp_ctx *ctx, const double *part1, const double *part2, cons
 // out (3 x N point-major) += -1/2 grad(mu) of one solved body, cell-centred
 // Green-Gauss; d_nbr: neighbour across edge k of cell j at [k*N + j]
 int fp_launch_ugradmu(fp_ctx *ctx, const fp_body *b, const int *d_nbr, const int64_t *d_sh, int te, double *out);
+// sharp trailing edge of the node-based scheme: side (1, 2 or 0) of every cell,
+// the TE node list, node -> TE index (-1 elsewhere), and room for the 2 x nte side values
+struct TeSides {
+    const int *cell_side = nullptr;
+    const int *te_nodes = nullptr;
+    const int *node_te = nullptr;
+    double *sidev = nullptr;
+    int nte = 0;
+};
 int fp_launch_ugradmu_node(fp_ctx *ctx, const fp_body *b, const int64_t *d_nptr, const int *d_ncell, int64_t nn,
-                           const int *d_cn, double *d_nodal, double *out);
+                           const int *d_cn, double *d_nodal, double *out, const TeSides *te = nullptr);
 int fp_launch_ugradmu_combine(fp_ctx *ctx, const fp_body *b, const int *d_nbr, const int64_t *d_sh, int force_cell_te,
                               double cos_crit, double *cellv, double *nodev, double *out);
 

@@ -103,9 +103,14 @@ __global__ __launch_bounds__(256) void k_nodal_mean(const int64_t *__restrict__ 
 // out (3 x N point-major) += -1/2 grad: e1 along the first edge, e2 completing
 // the plane of the first three vertices, the plane f = f1 + gx x + gy y through
 // (0, 0), (L, 0), (x3, y3).  cn holds the first three node indices of each cell.
+// Sharp trailing edge (cell_side != nullptr): a cell on side 1 or 2 of the
+// trailing edge takes, at the TE nodes (node_te[v] = k >= 0), the value averaged
+// over its own side's cells, sidev[(side - 1) * nte + k].
 __global__ __launch_bounds__(256) void k_ugradmu_node(const double *__restrict__ pv, int64_t N,
                                                       const int *__restrict__ cn, const double *__restrict__ nodal,
-                                                      double sgn, double *__restrict__ out)
+                                                      double sgn, const int *__restrict__ cell_side,
+                                                      const int *__restrict__ node_te, const double *__restrict__ sidev,
+                                                      int nte, double *__restrict__ out)
 {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= N) return;
@@ -114,7 +119,19 @@ __global__ __launch_bounds__(256) void k_ugradmu_node(const double *__restrict__
     for (int k = 0; k < 3; ++k)
 #pragma unroll
         for (int c = 0; c < 3; ++c) v[k][c] = pv[(3 * k + c) * N + j];
-    const double f1 = sgn * nodal[cn[3 * j]], f2 = sgn * nodal[cn[3 * j + 1]], f3 = sgn * nodal[cn[3 * j + 2]];
+    const int side = cell_side ? cell_side[j] : 0;
+    double f[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int node = cn[3 * j + k];
+        double val = nodal[node];
+        if (side > 0) {
+            const int t = node_te[node];
+            if (t >= 0) val = sidev[(int64_t)(side - 1) * nte + t];
+        }
+        f[k] = sgn * val;
+    }
+    const double f1 = f[0], f2 = f[1], f3 = f[2];
     double e1[3], w[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) { e1[c] = v[1][c] - v[0][c]; w[c] = v[2][c] - v[0][c]; }
@@ -131,6 +148,37 @@ __global__ __launch_bounds__(256) void k_ugradmu_node(const double *__restrict__
     const double gx = (f2 - f1) / L, gy = (f3 - f1 - gx * x3) / y3;
 #pragma unroll
     for (int c = 0; c < 3; ++c) out[3 * j + c] += -0.5 * (gx * e1[c] + gy * e2[c]);
+}
+
+// Sharp trailing edge, the side values: for TE node te[k] and side s = 1, 2 the
+// area-weighted mean of Gamma over the cells holding it that lie on side s
+// (kept at the plain nodal value when none does).  One lane per (node, side).
+__global__ void k_te_side_means(const int *__restrict__ te, int nte, const int64_t *__restrict__ nptr,
+                                const int *__restrict__ ncell, const int *__restrict__ cell_side,
+                                const double *__restrict__ geo, int64_t N, const double *__restrict__ G,
+                                const double *__restrict__ nodal, double *__restrict__ sidev)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * nte) return;
+    const int s = t / nte + 1, k = t % nte, v = te[k];
+    double acc = 0, w = 0;
+    for (int64_t q = nptr[v]; q < nptr[v + 1]; ++q) {
+        const int c = ncell[q];
+        if (cell_side[c] != s) continue;
+        const double a = geo[6 * N + c];
+        acc += a * G[c];
+        w += a;
+    }
+    sidev[t] = w > 0 ? acc / w : nodal[v];
+}
+
+// the nodal values left behind hold side 2's values at the TE nodes (the
+// reference's sharpTE pass overwrites them with the second side's last)
+__global__ void k_te_store_side2(const int *__restrict__ te, int nte, const double *__restrict__ sidev,
+                                 double *__restrict__ nodal)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nte) nodal[te[k]] = sidev[nte + k];
 }
 
 // trailing-edge override of the node-based values (force_cellTE): cells pi and
@@ -175,7 +223,7 @@ __global__ __launch_bounds__(256) void k_ugradmu_combine(const int *__restrict__
 }  // namespace
 
 int fp_launch_ugradmu_node(fp_ctx *ctx, const fp_body *b, const int64_t *d_nptr, const int *d_ncell, int64_t nn,
-                           const int *d_cn, double *d_nodal, double *out)
+                           const int *d_cn, double *d_nodal, double *out, const TeSides *te)
 {
     const int64_t N = b->N;
     if (N == 0) return FP_OK;
@@ -184,8 +232,18 @@ int fp_launch_ugradmu_node(fp_ctx *ctx, const fp_body *b, const int64_t *d_nptr,
     if (nn > 0)
         hipLaunchKernelGGL(k_nodal_mean, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, ctx->stream, d_nptr,
                            d_ncell, nn, b->d_geo, N, b->d_strength, d_nodal);
+    const bool sharp = te && te->nte > 0;
+    if (sharp)
+        hipLaunchKernelGGL(k_te_side_means, dim3((unsigned)((2 * te->nte + 63) / 64)), dim3(64), 0, ctx->stream,
+                           te->te_nodes, te->nte, d_nptr, d_ncell, te->cell_side, b->d_geo, N, b->d_strength, d_nodal,
+                           te->sidev);
     hipLaunchKernelGGL(k_ugradmu_node, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream, b->d_pv, N, d_cn,
-                       d_nodal, sgn, out);
+                       d_nodal, sgn, sharp ? te->cell_side : (const int *)nullptr,
+                       sharp ? te->node_te : (const int *)nullptr, sharp ? te->sidev : (const double *)nullptr,
+                       sharp ? te->nte : 0, out);
+    if (sharp)
+        hipLaunchKernelGGL(k_te_store_side2, dim3((unsigned)((te->nte + 63) / 64)), dim3(64), 0, ctx->stream,
+                           te->te_nodes, te->nte, (const double *)te->sidev, d_nodal);
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }

@@ -280,13 +280,28 @@ function calcfield_Cp!(out::AbstractVector{Float64}, body, Us::AbstractMatrix{Fl
     return out
 end
 
-"""calcfield_Ugradmu!(out, body; force_cellTE, anglecrit): combined -1/2 grad(mu) (3 x N, +=)."""
-function calcfield_Ugradmu!(out::AbstractMatrix{Float64}, body; force_cellTE=true, anglecrit=30.0)
+const SHARPTE_OFF = Cint(0)
+const SHARPTE_PAIRS = Cint(1)
+const SHARPTE_CELLS = Cint(2)
+
+"""calcfield_Ugradmu!(out, body; force_cellTE, anglecrit, sharpTE): combined -1/2 grad(mu) (3 x N, +=).
+`sharpTE=true` uses the reference's structured-grid layout (the two triangles of each trailing-edge quad)."""
+function calcfield_Ugradmu!(out::AbstractMatrix{Float64}, body; force_cellTE=true, anglecrit=30.0, sharpTE=false)
     hs = handles(body)
     check(ccall((:fp_calcfield_ugradmu_combined, lib), Cint,
-                (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Cint, Cint, Cdouble, Ptr{Float64}),
-                ctx(), hs, length(hs), force_cellTE, anglecrit, out))
+                (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Cint, Cint, Cdouble, Cint, Ptr{Float64}),
+                ctx(), hs, length(hs), force_cellTE, anglecrit, sharpTE ? SHARPTE_PAIRS : SHARPTE_OFF, out))
     return out
+end
+
+"""calcfield_Ugradmu_node!(out, body; sharpTE): node-based -1/2 grad(mu) (3 x N, +=); returns the nodal values."""
+function calcfield_Ugradmu_node!(out::AbstractMatrix{Float64}, body; sharpTE=false)
+    hs = handles(body)
+    nodal = zeros(sum(b.nnodes for b in components(body)))
+    check(ccall((:fp_calcfield_ugradmu_node, lib), Cint,
+                (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Cint, Cint, Ptr{Float64}, Ptr{Float64}),
+                ctx(), hs, length(hs), sharpTE ? SHARPTE_PAIRS : SHARPTE_OFF, out, nodal))
+    return nodal
 end
 
 # ---- the solver-function protocol on caller matrices ---------------------------

@@ -49,21 +49,65 @@ def phi(body, targets, out):
     return out
 
 
-def calcfield_U(sourcebody, targetbody=None, fieldname="U", addfield=True):
-    """U = Uinf + induced velocity at the control points (the source body is the target body)."""
-    if targetbody is not None and targetbody is not sourcebody:
-        raise NotImplementedError("calcfield_U supports sourcebody == targetbody; use Uind for other targets")
+def _charlen(body, kind):
+    """Characteristic length of every panel (src/FLOWPanel_abstractbody.jl:571-633)."""
+    g = body.grid
+    if kind == "unitary":
+        return np.ones(g.ncells)
+    if kind == "sqrtarea":
+        return np.sqrt(calc_areas(body))
+    out = np.zeros(g.ncells)
+    for i in range(g.ncells):
+        P = g.nodes[:, g.cell_idx[g.cell_ptr[i]:g.cell_ptr[i + 1]]]
+        if kind == "bbox":
+            out[i] = np.linalg.norm(P.max(axis=1) - P.min(axis=1))
+        elif kind == "maxdist":
+            out[i] = np.linalg.norm(P - P[:, :1], axis=0).max()
+        else:
+            raise ValueError("unknown characteristic length %r" % (kind,))
+    return out
+
+
+def _controlpoints(body, offset, characteristiclength):
+    """calc_controlpoints(body, normals; off, characteristiclength) (src/FLOWPanel_abstractbody.jl:635-711):
+    the vertex mean of every panel plus off * l * n."""
+    cps = []
+    for b in body.bodies:
+        g = b.grid
+        counts = np.diff(g.cell_ptr)
+        sums = np.add.reduceat(g.nodes[:, g.cell_idx], g.cell_ptr[:-1], axis=1)
+        off = b.CPoffset if offset is None else float(offset)
+        kind = b.characteristiclength if characteristiclength is None else characteristiclength
+        cps.append(sums / counts + off * _charlen(b, kind)[None, :] * calc_normals(b))
+    return np.asfortranarray(np.hstack(cps))
+
+
+def calcfield_U(sourcebody, targetbody=None, offset=None, characteristiclength=None, fieldname="U",
+                addfield=True):
+    """calcfield_U(sourcebody, targetbody; offset, characteristiclength) (src/FLOWPanel_postprocess.jl:28-110):
+    U = Uinf + velocity induced by ``sourcebody`` at the control points of ``targetbody``, 3 x N.
+
+    The target's freestream is its field "Uinf" (set by ``solve``).  ``offset`` and
+    ``characteristiclength`` ("unitary", "bbox", "maxdist", "sqrtarea") move the control points
+    along the normals as the reference does; by default they are the target body's own.
+    """
+    if targetbody is None:
+        targetbody = sourcebody
     comps = sourcebody.bodies
     ctx = comps[0].ctx
-    n = sourcebody.ncells
+    n = targetbody.ncells
     out = np.zeros((3, n), order="F")
-    ctx.check(nat.lib().fp_calcfield_U(ctx.handle, nat.handles(comps), len(comps), nat.dptr(out)))
+    if targetbody is sourcebody and offset is None and characteristiclength is None:
+        ctx.check(nat.lib().fp_calcfield_U(ctx.handle, nat.handles(comps), len(comps), nat.dptr(out)))
+    else:
+        try:
+            U = np.hstack([np.asarray(b.get_field("Uinf")).reshape(3, -1) for b in targetbody.bodies])
+        except KeyError:
+            raise nat.FPError(nat.FP_ERR_ARG, "Target body doesn't have freestream field `Uinf`")
+        out += U
+        Uind(sourcebody, _controlpoints(targetbody, offset, characteristiclength), out)
     if addfield:
-        sourcebody.add_field(fieldname, out)
-        col = 0
-        for b in comps:
-            b.add_field(fieldname, out[:, col:col + b.ncells])
-            col += b.ncells
+        _store(targetbody, fieldname, out)
     return out
 
 
@@ -91,40 +135,56 @@ def _store(body, fieldname, out):
         col += b.ncells
 
 
+def _sharp_mode(b, sharpTE):
+    """FP_SHARPTE_* of one component: the reference's structured triangulated layout (pairs of
+    trailing-edge triangles, Kutta mode "reference") or single trailing-edge cells."""
+    if not sharpTE:
+        return nat.FP_SHARPTE_OFF
+    if sharpTE in ("pairs", "cells"):
+        return nat.FP_SHARPTE_PAIRS if sharpTE == "pairs" else nat.FP_SHARPTE_CELLS
+    return nat.FP_SHARPTE_PAIRS if getattr(b, "kutta", "pair") == "reference" else nat.FP_SHARPTE_CELLS
+
+
 def calcfield_Ugradmu_node(body, fieldname="Ugradmu", addfield=True, sharpTE=False):
     """-1/2 grad(mu) from nodal values of Gamma (src/FLOWPanel_postprocess.jl:622-748), 3 x N.
 
     The nodal values (area-weighted means over the cells holding each node) are
-    stored as the field "gamma_node".  The sharp trailing-edge variant
-    (``sharpTE=True``, structured grids only) is not provided.
+    stored as the field "gamma_node".  ``sharpTE`` (True, or "pairs"/"cells")
+    averages the trailing-edge nodal values over each side's cells separately
+    (:692-743): side cells (pi, pi+1)/(pj, pj-1) on triangulated structured grids,
+    pi/pj otherwise.
     """
-    if sharpTE:
-        raise NotImplementedError("calcfield_Ugradmu_node: sharpTE=True is not provided")
-    comps = body.bodies
-    ctx = comps[0].ctx
     out = np.zeros((3, body.ncells), order="F")
-    nn = sum(int(b.nnodes) for b in comps)
-    nodal = np.zeros(max(nn, 1))
-    ctx.check(nat.lib().fp_calcfield_ugradmu_node(ctx.handle, nat.handles(comps), len(comps), nat.dptr(out),
-                                                  nat.dptr(nodal)))
+    nodals = []
+    col = 0
+    for b in body.bodies:
+        o = np.zeros((3, b.ncells), order="F")
+        nodal = np.zeros(max(int(b.nnodes), 1))
+        b.ctx.check(nat.lib().fp_calcfield_ugradmu_node(b.ctx.handle, nat.handles([b]), 1, _sharp_mode(b, sharpTE),
+                                                        nat.dptr(o), nat.dptr(nodal)))
+        out[:, col:col + b.ncells] = o
+        nodals.append(nodal[:b.nnodes])
+        col += b.ncells
     if addfield:
         _store(body, fieldname, out)
-        body.add_field("gamma_node", nodal[:nn])
+        body.add_field("gamma_node", np.concatenate(nodals))
     return out
 
 
 def calcfield_Ugradmu(body, fieldname="Ugradmu", addfield=True, sharpTE=False, force_cellTE=True, anglecrit=30):
     """Combined -1/2 grad(mu) (src/FLOWPanel_postprocess.jl:756-1000), 3 x N: the
     cell-centred value, except on cells at an open edge or a sharp edge (normals
-    more than ``anglecrit`` degrees apart), which take the node-based value;
-    with ``force_cellTE`` the trailing-edge panels keep the cell-centred value."""
-    if sharpTE:
-        raise NotImplementedError("calcfield_Ugradmu: sharpTE=True is not provided")
-    comps = body.bodies
-    ctx = comps[0].ctx
+    more than ``anglecrit`` degrees apart), which take the node-based value (with
+    ``sharpTE`` as in ``calcfield_Ugradmu_node``); with ``force_cellTE`` the
+    trailing-edge panels keep the cell-centred value."""
     out = np.zeros((3, body.ncells), order="F")
-    ctx.check(nat.lib().fp_calcfield_ugradmu_combined(ctx.handle, nat.handles(comps), len(comps),
-                                                      int(bool(force_cellTE)), float(anglecrit), nat.dptr(out)))
+    col = 0
+    for b in body.bodies:
+        o = np.zeros((3, b.ncells), order="F")
+        b.ctx.check(nat.lib().fp_calcfield_ugradmu_combined(b.ctx.handle, nat.handles([b]), 1, int(bool(force_cellTE)),
+                                                            float(anglecrit), _sharp_mode(b, sharpTE), nat.dptr(o)))
+        out[:, col:col + b.ncells] = o
+        col += b.ncells
     if addfield:
         _store(body, fieldname, out)
     return out

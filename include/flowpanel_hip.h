@@ -248,19 +248,32 @@ int fp_calcfield_cp(fp_ctx *ctx, const double *Us, int64_t n, double uref, const
  * neighbour terms (the RigidWakeBody variant).  Cells are neighbours when they
  * share an edge. */
 int fp_calcfield_ugradmu(fp_ctx *ctx, fp_body *const *bodies, int nbodies, int te_correction, double *out);
-/* calcfield_Ugradmu_node!(out, body) (src/FLOWPanel_postprocess.jl:622-748, sharpTE = false):
- * out (3 x N, +=) -1/2 grad(mu) from nodal values of Gamma, each the area-weighted mean over the cells
- * holding the node, and per cell the in-plane gradient of the plane through its first three vertices
- * (sign flipped with CPoffset < 0).  nodal_out (optional): the nodal values ("gamma_node"), the bodies'
- * nodes one after the other. */
-int fp_calcfield_ugradmu_node(fp_ctx *ctx, fp_body *const *bodies, int nbodies, double *out, double *nodal_out);
-/* calcfield_Ugradmu!(out, out_cell, out_node, body) (src/FLOWPanel_postprocess.jl:756-1000, sharpTE = false):
+/* Sharp trailing edge of the node-based scheme (sharpTE, src/FLOWPanel_postprocess.jl:692-743): the cells on
+ * either side of the trailing edge take, at the TE nodes, the area-weighted mean of Gamma over their own side's
+ * cells only, so the jump of Gamma across the trailing edge is not averaged away.  Side 1 is the shedding
+ * panels pi, side 2 their partners pj; FP_SHARPTE_PAIRS adds pi + 1 and pj - 1 (the two triangles of a
+ * structured triangulated grid's trailing-edge quad, the reference's structured-grid layout), FP_SHARPTE_CELLS
+ * takes pi and pj alone (quad grids, unstructured meshes).  The TE nodes are the shedding edges' nodes. */
+#define FP_SHARPTE_OFF 0
+#define FP_SHARPTE_PAIRS 1
+#define FP_SHARPTE_CELLS 2
+
+/* calcfield_Ugradmu_node!(out, body; sharpTE) (src/FLOWPanel_postprocess.jl:622-748):
+ * out (3 x N, +=) -1/2 grad(mu) from nodal values of Gamma, each the area-weighted mean over the cells holding
+ * the node, and per cell the in-plane gradient of the plane through its first three vertices (sign flipped
+ * with CPoffset < 0).  nodal_out (optional): the nodal values ("gamma_node"), the bodies' nodes one after the
+ * other; with a sharp trailing edge they hold side 2's values at the TE nodes, as the reference leaves them. */
+int fp_calcfield_ugradmu_node(fp_ctx *ctx, fp_body *const *bodies, int nbodies, int sharp_te, double *out,
+                              double *nodal_out);
+/* calcfield_Ugradmu!(out, out_cell, out_node, body; sharpTE, force_cellTE, anglecrit)
+ * (src/FLOWPanel_postprocess.jl:756-1000):
  * out (3 x N, +=) the cell-centred value (with the trailing-edge treatment) of every cell except those with
  * an edge without neighbour or a neighbour whose normal is more than anglecrit_deg away, which take the
- * node-based value; with force_cell_te the shedding panels pi, pi+1, pj, pj-1 keep cell-centred values.
- * All edges of a cell are examined (the reference loops over three, every edge of a triangle). */
+ * node-based value (with sharp_te as in fp_calcfield_ugradmu_node); with force_cell_te the shedding panels
+ * pi, pi+1, pj, pj-1 keep cell-centred values.  All edges of a cell are examined (the reference loops over
+ * three, every edge of a triangle). */
 int fp_calcfield_ugradmu_combined(fp_ctx *ctx, fp_body *const *bodies, int nbodies, int force_cell_te,
-                                  double anglecrit_deg, double *out);
+                                  double anglecrit_deg, int sharp_te, double *out);
 
 /* ---- shape updates ------------------------------------------------------- */
 int64_t fp_body_nnodes(const fp_body *body);

@@ -395,11 +395,63 @@ def nodal_mean(body, strength, areas=None):
     return np.where(w > 0, s / np.where(w > 0, w, 1.0), 0.0)
 
 
-def ugradmu_node(body, strength):
-    """Node-based -1/2 grad(mu) (src/FLOWPanel_postprocess.jl:622-748, sharpTE = false), 3 x n,
-    and the nodal values.  Per cell: the plane through the first three vertices
-    carrying the (CPoffset-signed) nodal values, solved as a 3 x 3 system in an
-    in-plane frame (np.linalg.solve), its gradient mapped back to 3-D."""
+def te_sides(body, shedding, pairs):
+    """Trailing-edge nodes and the cells on either side of the trailing edge.
+
+    Side A holds the shedding panels pi (with pi+1 when ``pairs``: the two
+    triangles of a structured triangulated grid's last quad), side B the
+    partners pj (with pj-1).  The TE nodes are the shedding edges' nodes (one
+    set when the two sides share their trailing-edge nodes, as a grid closed
+    around the section loop does).
+    This is the sharp-trailing-edge split of src/FLOWPanel_postprocess.jl:692-748
+    (upper/lower TE cells of the loop dimension) written on the shedding matrix.
+    """
+    cell_ptr, cell_idx = np.asarray(body.cell_ptr), np.asarray(body.cell_idx)
+    n = len(cell_ptr) - 1
+    A, B, te = [], [], []
+    for pi, nia, nib, pj, nja, njb in np.asarray(shedding, dtype=np.int64).T:
+        for c in ((pi, pi + 1) if pairs else (pi,)):
+            if 0 <= c < n and c not in A:
+                A.append(int(c))
+        if pj >= 0:
+            for c in ((pj, pj - 1) if pairs else (pj,)):
+                if 0 <= c < n and c not in B:
+                    B.append(int(c))
+        ends = [cell_idx[cell_ptr[pi] + nia], cell_idx[cell_ptr[pi] + nib]]
+        if pj >= 0:
+            ends += [cell_idx[cell_ptr[pj] + nja], cell_idx[cell_ptr[pj] + njb]]
+        for v in ends:
+            if int(v) not in te:
+                te.append(int(v))
+    return A, B, te
+
+
+def _node_grad(nodes, cell_ptr, cell_idx, c, vals, sgn):
+    v = cell_idx[cell_ptr[c]:cell_ptr[c] + 3]
+    X = nodes[:, v]
+    e1 = X[:, 1] - X[:, 0]
+    e1 /= np.linalg.norm(e1)
+    w = X[:, 2] - X[:, 0]
+    e2 = w - w.dot(e1) * e1
+    e2 /= np.linalg.norm(e2)
+    t = np.array([[0.0, 0.0], [(X[:, 1] - X[:, 0]).dot(e1), 0.0], [w.dot(e1), w.dot(e2)]])
+    t0 = t.mean(axis=0)
+    M = np.column_stack([np.ones(3), t - t0])
+    a = np.linalg.solve(M, sgn * vals[v])
+    return -0.5 * (a[1] * e1 + a[2] * e2)
+
+
+def ugradmu_node(body, strength, shedding=None, sharp_te=0):
+    """Node-based -1/2 grad(mu) (src/FLOWPanel_postprocess.jl:622-748), 3 x n, and the nodal values.
+
+    Per cell: the plane through the first three vertices carrying the
+    (CPoffset-signed) nodal values, solved as a 3 x 3 system in an in-plane frame
+    (np.linalg.solve), its gradient mapped back to 3-D.  ``sharp_te`` (1: side
+    cells (pi, pi+1) / (pj, pj-1), 2: pi / pj, see ``te_sides``): the cells on
+    each side of the trailing edge take TE nodal values averaged over that side's
+    cells only, as the reference's sharpTE pass (:692-743); the nodal values
+    returned then hold side B's values at the TE nodes, as the reference leaves them.
+    """
     normals, cps, areas = _body_geometry(body)
     nodal = nodal_mean(body, strength, areas)
     sgn = -1.0 if body.CPoffset < 0 else 1.0
@@ -408,29 +460,34 @@ def ugradmu_node(body, strength):
     n = len(cell_ptr) - 1
     out = np.zeros((3, n), order="F")
     for c in range(n):
-        v = cell_idx[cell_ptr[c]:cell_ptr[c] + 3]
-        X = nodes[:, v]
-        e1 = X[:, 1] - X[:, 0]
-        e1 /= np.linalg.norm(e1)
-        w = X[:, 2] - X[:, 0]
-        e2 = w - w.dot(e1) * e1
-        e2 /= np.linalg.norm(e2)
-        t = np.array([[0.0, 0.0], [(X[:, 1] - X[:, 0]).dot(e1), 0.0], [w.dot(e1), w.dot(e2)]])
-        t0 = t.mean(axis=0)
-        M = np.column_stack([np.ones(3), t - t0])
-        a = np.linalg.solve(M, sgn * nodal[v])
-        out[:, c] = -0.5 * (a[1] * e1 + a[2] * e2)
+        out[:, c] = _node_grad(nodes, cell_ptr, cell_idx, c, nodal, sgn)
+    if sharp_te and shedding is not None and np.asarray(shedding).size:
+        A, B, te = te_sides(body, shedding, pairs=sharp_te == 1)
+        G = np.asarray(strength, dtype=np.float64)
+        for cells in (A, B):
+            side = nodal.copy()
+            for v in te:
+                s = w = 0.0
+                for c in cells:
+                    if v in cell_idx[cell_ptr[c]:cell_ptr[c + 1]]:
+                        s += areas[c] * G[c]
+                        w += areas[c]
+                if w > 0:
+                    side[v] = s / w
+            for c in cells:
+                out[:, c] = _node_grad(nodes, cell_ptr, cell_idx, c, side, sgn)
+        nodal = side
     return out, nodal
 
 
-def ugradmu_combined(body, strength, shedding=None, force_cell_te=True, anglecrit=30.0):
-    """Combined -1/2 grad(mu) (src/FLOWPanel_postprocess.jl:756-921, sharpTE = false), 3 x n:
+def ugradmu_combined(body, strength, shedding=None, force_cell_te=True, anglecrit=30.0, sharp_te=0):
+    """Combined -1/2 grad(mu) (src/FLOWPanel_postprocess.jl:756-921), 3 x n:
     node-based values on cells with an edge without neighbour or a neighbour normal
     more than ``anglecrit`` degrees away, cell-centred values (trailing-edge variant)
     elsewhere; with ``force_cell_te`` the shedding panels pi, pi+1, pj, pj-1 keep the
     cell-centred values."""
     cell = ugradmu_cell(body, strength, shedding, te=True)
-    node, _ = ugradmu_node(body, strength)
+    node, _ = ugradmu_node(body, strength, shedding, sharp_te)
     n = cell.shape[1]
     if force_cell_te and shedding is not None and np.asarray(shedding).size:
         for pi, nia, nib, pj, nja, njb in np.asarray(shedding, dtype=np.int64).T:

@@ -202,3 +202,51 @@ def test_nan_matrix_is_an_error(fp, solver):
     with pytest.raises(fp.FPError) as e:
         getattr(fp, solver)(x, A, rng.standard_normal(n), **({"precond_block": 50} if "gmres" in solver else {}))
     assert e.value.code in (6, 7), e.value
+
+
+@pytest.mark.parametrize("offset,charlen,kind", [(0.02, "sqrtarea", 3), (0.05, "bbox", 1), (-0.01, "maxdist", 2)])
+def test_calcfield_U_offset_charlen_matches_oracle(fp, geo, orc, offset, charlen, kind):
+    # calcfield_U!(out, body, body; offset, characteristiclength) (src/FLOWPanel_postprocess.jl:77-99)
+    g, sh = geo.loft_wing(n_chord=14, n_span=8, span=4.0, sweep_deg=10.0)
+    body = fp.RigidWakeBody(g, sh)
+    V = fp.direction(3.0)
+    U = np.tile(V[:, None], (1, body.ncells))
+    D = np.tile(V[:, None], (1, body.nsheddings))
+    fp.solve(body, U, D, D)
+    out = fp.calcfield_U(body, body, offset=offset, characteristiclength=charlen)
+    _, cps, _ = orc.geometry(g.nodes, g.cell_ptr, g.cell_idx, offset, kind)
+    ref = U + orc.uind(_obody(body), body.strength, cps, D, D)
+    assert rel(out, ref) < 1e-10
+
+
+def test_calcfield_U_other_target_body(fp, geo, orc):
+    # velocity of a wing at the control points of a separate sphere (source != target)
+    g, sh = geo.loft_wing(n_chord=12, n_span=6, span=3.0)
+    wing = fp.RigidWakeBody(g, sh)
+    V = fp.direction(2.0)
+    U = np.tile(V[:, None], (1, wing.ncells))
+    D = np.tile(V[:, None], (1, wing.nsheddings))
+    fp.solve(wing, U, D, D)
+    sp = fp.NonLiftingBody(geo.sphere(n_meridional=6, n_azimuthal=10, center=(0.5, 0.0, -1.5), radius=0.3))
+    Us = np.tile(V[:, None], (1, sp.ncells))
+    fp.solve(sp, Us)
+    out = fp.calcfield_U(wing, sp)
+    _, cps, _ = orc.geometry(sp.grid.nodes, sp.grid.cell_ptr, sp.grid.cell_idx, sp.CPoffset, 0)
+    ref = Us + orc.uind(_obody(wing), wing.strength, cps, D, D)
+    assert rel(out, ref) < 1e-10
+
+
+def test_nonlifting_constant_doublet_body_matches_oracle(fp, geo, orc):
+    # NonLiftingBody{ConstantDoublet} (src/FLOWPanel_nonliftingbody.jl:258-352): vortex rings without
+    # wake on an open shell (a closed one has the uniform-doublet null mode)
+    g, _ = geo.loft_wing(n_chord=12, n_span=6, span=3.0)
+    body = fp.NonLiftingBody(g, fp.ConstantDoublet)
+    V = fp.direction(5.0)
+    U = np.tile(V[:, None], (1, body.ncells))
+    fp.solve(body, U)
+    ob = _obody(body)
+    ob.element = "vortexring"
+    Go, normals, cps = orc.assemble([ob], [np.zeros((3, 0))], [np.zeros((3, 0))])
+    mu = orc.solve_lu(Go, orc.rhs(U, normals))
+    assert rel(body.strength, mu) < 1e-10
+    assert "mu" in body.fields

@@ -62,5 +62,37 @@ def test_ugradmu_node_and_combined_match_oracle(fp, geo, orc, tri):
             outc = fp.calcfield_Ugradmu(body, force_cellTE=force, anglecrit=ang)
             refc = orc.ugradmu_combined(ob, body.strength, sh, force_cell_te=force, anglecrit=ang)
             assert rel(outc, refc) < 1e-12, (force, ang)
-    with pytest.raises(NotImplementedError):
-        fp.calcfield_Ugradmu(body, sharpTE=True)
+
+
+def _closed_loop(fp, g):
+    """The same grid with coincident nodes merged: the section loop closes at the
+    trailing edge (upper and lower TE cells share their TE nodes)."""
+    X = np.asarray(g.nodes)
+    _, first, inv = np.unique(np.round(X.T, 12), axis=0, return_index=True, return_inverse=True)
+    rep = first[inv.ravel()]
+    return fp.Grid(X, (np.asarray(g.cell_ptr), rep[np.asarray(g.cell_idx)]), dims=g.dims)
+
+
+@pytest.mark.parametrize("tri", [False, True])
+def test_ugradmu_sharp_te_matches_oracle(fp, geo, orc, tri):
+    """sharpTE (src/FLOWPanel_postprocess.jl:692-743) of the node-based and combined schemes."""
+    g, sh = geo.loft_wing(airfoil="2412", n_chord=20, n_span=10, span=5.0, sweep_deg=20.0)
+    if tri:
+        g, sh = geo.triangulate(g, sh)
+    g = _closed_loop(fp, g)
+    body = fp.RigidWakeBody(g, sh)
+    V = fp.direction(5.0)
+    fp.solve(body, np.tile(V[:, None], (1, body.ncells)), np.tile(V[:, None], (1, body.nsheddings)),
+             np.tile(V[:, None], (1, body.nsheddings)))
+    ob = types.SimpleNamespace(nodes=g.nodes, cell_ptr=g.cell_ptr, cell_idx=g.cell_idx, ncells=g.ncells,
+                               CPoffset=body.CPoffset, charlen_kind=0)
+    mode = 1 if tri else 2
+    out = fp.calcfield_Ugradmu_node(body, sharpTE=True)
+    ref, nodal = orc.ugradmu_node(ob, body.strength, sh, sharp_te=mode)
+    assert rel(out, ref) < 1e-12
+    assert rel(body.get_field("gamma_node"), nodal) < 1e-13
+    plain, _ = orc.ugradmu_node(ob, body.strength)
+    assert rel(plain, ref) > 1e-6            # the trailing-edge jump is no longer averaged away
+    outc = fp.calcfield_Ugradmu(body, sharpTE=True, force_cellTE=False)
+    refc = orc.ugradmu_combined(ob, body.strength, sh, force_cell_te=False, anglecrit=30.0, sharp_te=mode)
+    assert rel(outc, refc) < 1e-12

@@ -26,7 +26,8 @@ ELEM = {"double": {"Float64", "Cdouble"}, "int64_t": {"Int64"}, "int": {"Cint", 
 
 def _strip_c_comments(s):
     s = re.sub(r"/\*.*?\*/", " ", s, flags=re.S)
-    return re.sub(r"//[^\n]*", " ", s)
+    s = re.sub(r"//[^\n]*", " ", s)
+    return re.sub(r"^\s*#[^\n]*", " ", s, flags=re.M)      # preprocessor lines
 
 
 def header_prototypes(text=None):

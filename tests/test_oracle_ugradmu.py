@@ -146,3 +146,75 @@ def test_combined_scheme_selects_node_values_on_open_and_sharp_edges(orc):
     for c in range(body.ncells):
         if not (nb[c] < 0).any():
             assert np.array_equal(comb95[:, c], cell[:, c])
+
+
+# ---- sharp trailing edge (src/FLOWPanel_postprocess.jl:692-748) ----
+#
+# A strength that is one constant on the cells of the lower half of the section
+# loop and another on the upper half jumps across the trailing edge, as a lifting
+# solution does.  The plain nodal mean mixes both sides at the TE nodes and puts
+# a spurious gradient into the TE cells; the sharp-TE pass averages each side's
+# TE nodal values over that side's cells only, so every TE cell sees a uniform
+# field and gets exactly zero.  Cells away from the TE are untouched.
+
+def _split_strength(g, nc, cl=-0.8, cu=1.7):
+    i = np.arange(g.ncells) % nc            # chordwise position of each cell in its strip
+    return np.where(i < nc // 2, cl, cu)
+
+
+def _wing_body(g):
+    """The wing with coincident nodes merged: the section loop closes at the trailing
+    edge, so upper and lower TE cells share their TE nodes (a grid closed in its loop
+    dimension, the case the reference's sharpTE pass is written for)."""
+    X = np.asarray(g.nodes)
+    _, first, inv = np.unique(np.round(X.T, 12), axis=0, return_index=True, return_inverse=True)
+    rep = first[inv.ravel()]                    # every node -> its first coincident copy
+    return types.SimpleNamespace(nodes=g.nodes, cell_ptr=g.cell_ptr, cell_idx=rep[np.asarray(g.cell_idx)],
+                                 ncells=g.ncells, CPoffset=1e-14, charlen_kind=0)
+
+
+def test_sharp_te_quads_zero_gradient_at_the_trailing_edge(orc, geo):
+    nc = 12
+    g, sh = geo.loft_wing(n_chord=nc, n_span=5, span=2.5, sweep_deg=10.0)
+    body = _wing_body(g)
+    G = _split_strength(g, nc)
+    plain, nod0 = orc.ugradmu_node(body, G)
+    sharp, nod1 = orc.ugradmu_node(body, G, sh, sharp_te=2)
+    A, B, te = orc.te_sides(body, sh, pairs=False)
+    assert sorted(A) == sorted(sh[0].tolist()) and sorted(B) == sorted(sh[3].tolist())
+    for c in A + B:
+        assert np.abs(sharp[:, c]).max() < 1e-12
+        assert np.abs(plain[:, c]).max() > 1e-2          # the mixing the pass removes
+    rest = [c for c in range(g.ncells) if c not in A + B]
+    assert np.array_equal(sharp[:, rest], plain[:, rest])
+    # the nodal values left behind carry side B's values at the TE nodes
+    assert np.allclose(nod1[te], G[sh[3][0]]) and np.array_equal(np.delete(nod1, te), np.delete(nod0, te))
+
+
+def test_sharp_te_triangle_pairs(orc, geo):
+    nc = 12
+    g, sh = geo.loft_wing(n_chord=nc, n_span=4, span=2.0)
+    gt, sht = geo.triangulate(g, sh)
+    body = _wing_body(gt)
+    G = _split_strength(gt, 2 * nc)
+    sharp, _ = orc.ugradmu_node(body, G, sht, sharp_te=1)
+    plain, _ = orc.ugradmu_node(body, G)
+    A, B, _ = orc.te_sides(body, sht, pairs=True)
+    assert len(A) == 2 * sh.shape[1] and len(B) == 2 * sh.shape[1]
+    for c in A + B:
+        assert np.abs(sharp[:, c]).max() < 1e-12
+    assert max(np.abs(plain[:, c]).max() for c in A + B) > 1e-2
+
+
+def test_sharp_te_combined_uses_it(orc, geo):
+    # the combined scheme hands sharpTE to its node-based part: with force_cellTE off, the TE cells
+    # (open-edge or sharp-angle cells take node values) see the sharp-TE values
+    nc = 12
+    g, sh = geo.loft_wing(n_chord=nc, n_span=5, span=2.5)
+    body = _wing_body(g)
+    G = _split_strength(g, nc)
+    node, _ = orc.ugradmu_node(body, G, sh, sharp_te=2)
+    comb = orc.ugradmu_combined(body, G, sh, force_cell_te=False, anglecrit=30.0, sharp_te=2)
+    A, B, _ = orc.te_sides(body, sh, pairs=False)
+    for c in A + B:     # a TE cell's neighbour across the sharp edge is more than 30 degrees away
+        assert np.array_equal(comb[:, c], node[:, c])
