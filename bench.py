@@ -56,13 +56,13 @@ def build_body(fp, nc, ns, alpha):
     return body, U, D
 
 
-def cpu_baseline(fp, body, U, D, seconds_hint=20.0):
-    """Restated reference CPU path (the oracle) on a bounded sample.
+def cpu_baseline(fp, body, U, D, full=False):
+    """Restated reference CPU path (the oracle) on the host cores.
 
-    Assembly: oracle `_G_Uvortexring!` restatement (OpenMP over panel columns)
-    for the first R control points of the same body, scaled to all rows.
-    Solve: LAPACK getrf+getrs (scipy/OpenBLAS, all host threads) on an n x n
-    slice, scaled by (N/n)^3.
+    Assembly: the oracle's `_G_Uvortexring!` restatement (OpenMP over panel
+    columns) of the whole 30,000 x 30,000 FP64 AIC with its wake columns.
+    Solve: LAPACK getrf + getrs (scipy/OpenBLAS, all host threads) on an n x n
+    leading block of that AIC, scaled by (N/n)^3 (n = N with ``full``).
     """
     import types
     import scipy.linalg as sla
@@ -72,47 +72,150 @@ def cpu_baseline(fp, body, U, D, seconds_hint=20.0):
                               nsheddings=body.nsheddings, kerneloffset=body.kerneloffset,
                               kernelcutoff=body.kernelcutoff, CPoffset=body.CPoffset, charlen_kind=0)
     N = body.ncells
-    normals, cps, _ = orc.geometry(b.nodes, b.cell_ptr, b.cell_idx, b.CPoffset, 0)
-    R = 1200
     t = time.perf_counter()
-    Gs, _, _ = orc.assemble([b], [D], [D], targets=cps[:, :R], normals=normals[:, :R])
-    t_asm = (time.perf_counter() - t) * N / R
-    n = 5000
-    A = np.asfortranarray(np.vstack([Gs[:, :n]] * ((n + R - 1) // R))[:n, :n] + np.eye(n))
+    G, normals, _ = orc.assemble([b], [D], [D])
+    t_asm = time.perf_counter() - t
+    n = N if full else min(N, 10000)
+    A = np.asfortranarray(G[:n, :n]) if n < N else G
+    del G
+    rhs = -np.einsum("ij,ij->j", U[:, :n], normals[:, :n])
     t = time.perf_counter()
     lu = sla.lu_factor(A, overwrite_a=True, check_finite=False)
-    sla.lu_solve(lu, np.ones(n), check_finite=False)
+    sla.lu_solve(lu, rhs, check_finite=False)
     t_lu = (time.perf_counter() - t) * (N / n) ** 3
-    cores = os.cpu_count()
-    return {"value": (t_asm + t_lu) * 1e3, "unit": "ms", "cores": cores, "kind": "port",
-            "sample": "oracle AIC of the first %d of %d control points (scaled x%.1f) + LAPACK getrf/getrs "
-                      "(OpenBLAS) on a %dx%d slice (scaled (N/n)^3); %d host threads"
-                      % (R, N, N / R, n, n, cores),
-            "assembly_ms_est": t_asm * 1e3, "lu_ms_est": t_lu * 1e3}
+    del A, lu
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    sample = ("oracle AIC of all %d control points (measured) + LAPACK getrf/getrs (OpenBLAS) on the %dx%d "
+              "leading block%s; %d host threads" % (N, n, n, "" if n == N else " (scaled (N/n)^3)", cores))
+    return {"value": (t_asm + t_lu) * 1e3, "unit": "ms", "cores": cores, "kind": "port", "sample": sample,
+            "assembly_ms": t_asm * 1e3, "lu_ms" if n == N else "lu_ms_est": t_lu * 1e3}
 
 
-def vendor_lu(fp, body, N):
-    """rocSOLVER FP64 getrf+getrs (through torch) on the same FP64 AIC: an external anchor for the FP64 path."""
-    try:
-        import torch
-        if not torch.cuda.is_available():
-            return {"error": "torch sees no GPU (device_count %d)" % torch.cuda.device_count()}
-        G = fp.assemble(body, "f64")
-        A = torch.from_numpy(G).cuda()
-        del G
-        b = torch.ones(N, 1, dtype=torch.float64, device="cuda")
-        torch.linalg.lu_factor(A[:2048, :2048].contiguous())
-        torch.cuda.synchronize()
+VENDOR_CHILD = r"""
+import ctypes, json, sys, time
+import numpy as np
+sys.path.insert(0, %(root)r)
+import __graft_entry__
+fp = __graft_entry__.load_package()
+hip = ctypes.CDLL("libamdhip64.so")
+rb = ctypes.CDLL("librocblas.so")
+rs = ctypes.CDLL("librocsolver.so")
+vp = ctypes.c_void_p
+hip.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+hip.hipMemcpy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+hip.hipFree.argtypes = [vp]
+rb.rocblas_create_handle.argtypes = [ctypes.POINTER(vp)]
+for f in (rs.rocsolver_dgetrf, rs.rocsolver_sgetrf):
+    f.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, vp]
+for f in (rs.rocsolver_dgetrs, rs.rocsolver_sgetrs):
+    f.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, vp, ctypes.c_int]
+h = vp()
+assert rb.rocblas_create_handle(ctypes.byref(h)) == 0
+
+def dmalloc(nbytes):
+    p = vp()
+    assert hip.hipMalloc(ctypes.byref(p), nbytes) == 0
+    return p
+
+def run(A, dtype):
+    n = A.shape[0]
+    A = np.asfortranarray(A, dtype=dtype)
+    es = A.itemsize
+    dA, dB, dp, di = dmalloc(A.nbytes), dmalloc(n * es), dmalloc(4 * n), dmalloc(4)
+    b = np.ones(n, dtype=dtype)
+    getrf = rs.rocsolver_dgetrf if dtype == np.float64 else rs.rocsolver_sgetrf
+    getrs = rs.rocsolver_dgetrs if dtype == np.float64 else rs.rocsolver_sgetrs
+    best = None
+    for rep in range(2):
+        assert hip.hipMemcpy(dA, A.ctypes.data, A.nbytes, 1) == 0
+        assert hip.hipMemcpy(dB, b.ctypes.data, b.nbytes, 1) == 0
+        hip.hipDeviceSynchronize()
         t = time.perf_counter()
-        LU, piv = torch.linalg.lu_factor(A)
-        x = torch.linalg.lu_solve(LU, piv, b)
-        torch.cuda.synchronize()
-        ms = (time.perf_counter() - t) * 1e3
-        del A, LU, piv, x
-        torch.cuda.empty_cache()
-        return {"ms": round(ms, 2), "what": "torch.linalg.lu_factor+lu_solve (rocSOLVER) FP64, matrix resident"}
+        assert getrf(h, n, n, dA, n, dp, di) == 0
+        hip.hipDeviceSynchronize()
+        t1 = time.perf_counter()
+        assert getrs(h, 111, n, 1, dA, n, dp, dB, n) == 0
+        hip.hipDeviceSynchronize()
+        t2 = time.perf_counter()
+        if best is None or t1 - t < best[0]:
+            best = (t1 - t, t2 - t1)
+    info = ctypes.c_int(0)
+    hip.hipMemcpy(ctypes.addressof(info), di, 4, 2)
+    x = np.empty(n, dtype=dtype)
+    hip.hipMemcpy(x.ctypes.data, dB, x.nbytes, 2)
+    for p in (dA, dB, dp, di):
+        hip.hipFree(p)
+    return {"getrf_ms": round(best[0] * 1e3, 2), "getrs_ms": round(best[1] * 1e3, 2), "info": info.value,
+            "getrf_tflops": round(2.0 / 3.0 * n ** 3 / best[0] / 1e12, 2)}, x
+
+out = {"what": "rocSOLVER getrf + getrs (1 RHS), matrix resident in HBM, best of 2; a separate process"}
+for name, nc, ns, kind in ((%(bwb)r, %(nc)d, %(ns)d, "bwb"), ("wing4800", 60, 80, "wing")):
+    if kind == "bwb":
+        grid, sh = fp.geometry.bwb(seed=5, n_chord=nc, n_span=ns)
+    else:
+        grid, sh = fp.geometry.loft_wing(airfoil="0012", n_chord=60, n_span=80, span=10.0, root_chord=1.0,
+                                         taper=0.5, sweep_deg=30.0)
+    body = fp.RigidWakeBody(grid, sh)
+    V = fp.direction(5.0 if kind == "bwb" else 4.2)
+    fp.solvers.set_freestream(body, np.tile(V[:, None], (1, body.ncells)), np.tile(V[:, None], (1, body.nsheddings)),
+                              np.tile(V[:, None], (1, body.nsheddings)))
+    G = fp.assemble(body, "f64")
+    out[name + "_f64"], _ = run(G, np.float64)
+    if kind == "wing":
+        out[name + "_f32"], _ = run(G, np.float32)
+    del G
+print(json.dumps(out), flush=True)
+"""
+
+
+def vendor_lu(nc, ns):
+    """rocSOLVER FP64 (and FP32 at 4,800) getrf + getrs on the same AICs, in a child process started before
+    this process touches the GPU: an external anchor for the device LU."""
+    import subprocess
+    code = VENDOR_CHILD % {"root": ROOT, "bwb": "bwb%dx%d" % (nc, ns), "nc": nc, "ns": ns}
+    try:
+        p = subprocess.run([sys.executable, "-c", code], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                           timeout=240)
+        if p.returncode != 0:
+            return {"error": (p.stderr or "")[-300:]}
+        return json.loads(p.stdout.strip().splitlines()[-1])
     except Exception as e:  # the anchor is optional
         return {"error": str(e)[:200]}
+
+
+def c2_leg(fp, ctx, reps=3):
+    """The 4,800-panel swept wing (NACA 0012, AR 10, sweep 30, taper 0.5, 60 x 80 quads, AoA 4.2) with one
+    freestream: FP32 LU (randomized, no pivoting) + FP64 refinement against the plain FP64 LU."""
+    g, sh = fp.geometry.loft_wing(airfoil="0012", n_chord=60, n_span=80, span=10.0, root_chord=1.0, taper=0.5,
+                                  sweep_deg=30.0)
+    wing = fp.RigidWakeBody(g, sh)
+    V = fp.direction(4.2)
+    U = np.tile(V[:, None], (1, wing.ncells))
+    D = np.tile(V[:, None], (1, wing.nsheddings))
+    rec = {"workload": "wing NACA 0012, AR 10, sweep 30, taper 0.5, 60 x 80 quads (4,800 panels) + 80 wake strips, "
+                       "AoA 4.2, one freestream", "panels": wing.ncells}
+    res = {}
+    for name, solver, opts in (("lu32_ir", fp.solve_lu32_ir, {"tol": 1e-10}), ("lu64", fp.solve_ludiv, {})):
+        fp.solve(wing, U, D, D, solver=solver, solver_optargs=opts)
+        walls, st = [], None
+        for _ in range(reps):
+            ctx.synchronize()
+            t = time.perf_counter()
+            st = fp.solve(wing, U, D, D, solver=solver, solver_optargs=opts)
+            ctx.synchronize()
+            walls.append((time.perf_counter() - t) * 1e3)
+        fp.calcfield_U(wing, wing)
+        res[name] = (min(walls), st, wing.strength.copy(), fp.calcfield_Cp(wing, 1.0))
+    ms32, st32, g32, cp32 = res["lu32_ir"]
+    ms64, st64, g64, cp64 = res["lu64"]
+    flop = 2.0 / 3.0 * wing.ncells ** 3
+    rec.update({"lu32_ir_ms": round(ms32, 3), "lu64_ms": round(ms64, 3), "speedup": round(ms64 / ms32, 2),
+                "lu32_ir_stats": st32, "lu64_stats": st64,
+                "lu32_factor_tflops": round(flop / (st32["ms_factor"] * 1e-3) / 1e12, 2),
+                "lu64_factor_tflops": round(flop / (st64["ms_factor"] * 1e-3) / 1e12, 2),
+                "cp_err_max": float(np.abs(cp32 - cp64).max()),
+                "gamma_rel_err": float(np.abs(g32 - g64).max() / np.abs(g64).max())})
+    return rec
 
 
 def field_leg(fp, ctx, body, U, n=256):
@@ -331,10 +434,13 @@ def main():
                     help="FP64 relative residual target of the mixed solve (max |dCp| ~1e-8 against FP64 LU, "
                          "100x inside the 1e-6 bound)")
     ap.add_argument("--inner-tol", type=float, default=1e-6, help="relative tolerance of each inner FP32 GMRES")
-    ap.add_argument("--vendor", action="store_true", help="also time rocSOLVER FP64 getrf through torch")
+    ap.add_argument("--no-vendor", action="store_true",
+                    help="skip the rocSOLVER getrf/getrs anchor on the same AICs (a child process started first)")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 4,800-panel single-freestream LU leg")
+    ap.add_argument("--cpu-full", action="store_true", help="CPU baseline LU at full size (about a minute)")
     ap.add_argument("--mixed-only", action="store_true", help="profiling runs: mixed steps only (no FP64 LU, no CPU)")
     ap.add_argument("--copy-calib", action="store_true",
-                    help="profiling runs: also run a plain 1 GiB device copy (torch) to calibrate HBM counters")
+                    help="profiling runs: also run a plain 1 GiB device copy to calibrate HBM counters")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 10,000-panel adjoint leg")
@@ -348,6 +454,10 @@ def main():
         print("bench.py measures one GPU: the library has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
 
+    vendor = None
+    if not args.no_vendor and not (args.adjoint_only or args.batch_only or args.mixed_only):
+        log("bench: rocSOLVER anchor (child process)")
+        vendor = vendor_lu(args.chord, args.span)
     import __graft_entry__
     fp = __graft_entry__.load_package()
     ctx = fp.context()
@@ -450,9 +560,7 @@ def main():
     kernels = {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in prof.items()}   # one solve
     kernels_fp64 = {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in prof64.items()}
 
-    vendor = None
-    if args.vendor:
-        vendor = vendor_lu(fp, body, N)
+    c2 = None if args.no_c2 else c2_leg(fp, ctx)
     adj = None
     if not args.no_adjoint:
         adj = adjoint_leg(fp, ctx, fd_coords=args.adjoint_fd)
@@ -462,7 +570,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")
-        cpu = cpu_baseline(fp, body, U, D)
+        cpu = cpu_baseline(fp, body, U, D, full=args.cpu_full)
 
     line = {
         "metric": "AIC+solve time at 30k panels, mixed+refine vs FP64 @ Cp err<=1e-6; adjoint ms",
@@ -475,8 +583,9 @@ def main():
         "fp64_lu_ms": round(t_fp64, 3), "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
         "cp_err_max": cp_err, "gamma_rel_err": gam_err, "tol": args.tol, "inner_tol": args.inner_tol,
         "mixed_stats": stats[-1], "fp64_stats": st64, "mixed_tol_1e-10": strict,
-        "adjoint_ms": adj["adjoint_total_ms"] if adj else None, "adjoint": adj, "batch": bat, "field": field,
-        "kernels": kernels, "kernels_fp64": kernels_fp64, "vendor_fp64_getrf": vendor, "roofline": roof,
+        "adjoint_ms": adj["adjoint_total_ms"] if adj else None, "adjoint": adj, "c2_wing4800": c2, "batch": bat,
+        "field": field,
+        "kernels": kernels, "kernels_fp64": kernels_fp64, "vendor_rocsolver": vendor, "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

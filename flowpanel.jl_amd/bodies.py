@@ -84,8 +84,15 @@ class AbstractBody:
             nat.CHARLEN[characteristiclength], self.kerneloffset, self.kernelcutoff, ctypes.byref(h)))
         self.handle = h
         if kutta is None:
+            # the reference's Kutta pattern (pi, pi+1, pj, pj-1) holds on structured triangulated grids
+            # (src/FLOWPanel_postprocess.jl:1075-1097); on quads and on unstructured meshes -- where the
+            # reference warns that pattern is wrong (:1077-1080) -- the two cells sharing each shedding
+            # edge, pi and pj, are averaged
             nv = grid.nvertices()
-            kutta = "reference" if (len(nv) and np.all(nv == 3)) else "pair"
+            structured = getattr(grid, "dims", None) is not None
+            kutta = "reference" if (structured and len(nv) and np.all(nv == 3)) else "pair"
+        if kutta not in ("reference", "pair", "none"):
+            raise ValueError("kutta must be 'reference', 'pair' or 'none'")
         self.kutta = kutta
 
     # --- reference-style properties

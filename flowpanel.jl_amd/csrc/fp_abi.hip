@@ -12,6 +12,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -517,6 +518,10 @@ struct SolveArgs {
 
 // A solver prepared once for a matrix (factorisation or preconditioner) and
 // then applied to any number of right-hand sides.
+// Seed of the random butterflies of the FP32 no-pivot LU (any fixed value: the
+// refinement, not the butterflies, decides the result).
+constexpr uint64_t RBT_SEED = 0x46504C5533324952ull;
+
 struct Solver {
     fp_ctx *ctx;
     SolveArgs a;
@@ -525,7 +530,78 @@ struct Solver {
     std::unique_ptr<Gmres> g;
     std::unique_ptr<BJPrecond> M;
     std::unique_ptr<DenseOp<float>> A32op;
+    // FP32 LU of the randomized matrix U^T A V without pivoting (FP_SOLVER_LU32_IR):
+    // Ar (n4 x ld4), the butterflies, identity row order for getrs, an n4 x k work vector
+    bool rbt = false;
+    int64_t n4 = 0, ld4 = 0;
+    DBuf Ar, Wrbt, perm4, work4, t4;
     Solver(fp_ctx *c, const SolveArgs &args, const fp_solve_opts &opts) : ctx(c), a(args), o(opts) {}
+
+    // Randomize and factor without pivoting; false (and nothing kept) when a
+    // diagonal block meets a zero or non-finite pivot: the caller then pivots.
+    int rbt_setup(bool *ok)
+    {
+        const int64_t n = a.n;
+        int st;
+        *ok = false;
+        n4 = fp_rbt_size(n);
+        ld4 = round_ld(n4);
+        DBuf nw;
+        if ((st = dalloc(ctx, Ar, (size_t)n4 * ld4 * sizeof(float))) ||
+            (st = dalloc(ctx, Wrbt, fp_rbt_weights_bytes(n))) || (st = dalloc(ctx, perm4, n4 * sizeof(int))) ||
+            (st = dalloc(ctx, work4, fp_getrs_work_bytes(n4, 1))) || (st = dalloc(ctx, t4, n4 * sizeof(double))) ||
+            (st = dalloc(ctx, nw, fp_getrf_nopiv_work_bytes())) || (st = dalloc(ctx, info, sizeof(int))))
+            return st;
+        if ((st = fp_rbt_weights(ctx, n, RBT_SEED, Wrbt.as<double>()))) return st;
+        st = a.R64 ? fp_rbt_matrix<double>(ctx, a.R64, n, a.ld, Ar.as<float>(), ld4, Wrbt.as<double>())
+                   : fp_rbt_matrix<float>(ctx, a.A32, n, a.ld, Ar.as<float>(), ld4, Wrbt.as<double>());
+        if (st) return st;
+        if ((st = fp_getrf_nopiv_f32(ctx, Ar.as<float>(), n4, ld4, info.as<int>(), nw.p))) return st;
+        std::vector<int> id(n4);
+        for (int64_t i = 0; i < n4; ++i) id[i] = (int)i;
+        int hinfo = 0;
+        FP_HIP(ctx, hipMemcpyAsync(perm4.p, id.data(), n4 * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        FP_HIP(ctx, hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (hinfo) {
+            Ar.reset();
+            return FP_OK;
+        }
+        *ok = true;
+        rbt = true;
+        return FP_OK;
+    }
+
+    // the pivoted FP32 LU of the refinement path
+    int lu32_pivoted(fp_solve_stats &stats)
+    {
+        int st;
+        rbt = false;
+        Ar.reset();
+        if ((st = dalloc(ctx, perm, a.n * sizeof(int)))) return st;
+        if ((st = factor<float>(ctx, a.A32, a.n, a.ld, perm.as<int>(), &stats.info))) return st;
+        return dalloc(ctx, work, fp_getrs_work_bytes(a.n, 1));
+    }
+
+    // d (n x k, ld n) ~ A^-1 r (n x k, ld n) by the FP32 factors
+    int lu32_correct(const double *rr, double *dd, int64_t k, void *wk, size_t wbytes)
+    {
+        const int64_t n = a.n;
+        int st;
+        if (rbt) {
+            DBuf tk;
+            double *t = t4.as<double>();
+            if (k > 1) {
+                if ((st = dalloc(ctx, tk, (size_t)n4 * k * sizeof(double)))) return st;
+                t = tk.as<double>();
+            }
+            if ((st = fp_rbt_vec(ctx, rr, n, t, n4, n, k, Wrbt.as<double>(), 1))) return st;
+            if ((st = fp_getrs<float>(ctx, Ar.as<float>(), n4, ld4, perm4.as<int>(), t, k, n4, wk, wbytes))) return st;
+            return fp_rbt_vec(ctx, t, n4, dd, n, n, k, Wrbt.as<double>(), 0);
+        }
+        FP_HIP(ctx, hipMemcpyAsync(dd, rr, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        return fp_getrs<float>(ctx, a.A32, n, a.ld, perm.as<int>(), dd, k, n, wk, wbytes);
+    }
 
     int bj_setup(const void *A, int prec, int blk)
     {
@@ -581,9 +657,10 @@ struct Solver {
         } else if (a.solver == FP_SOLVER_LU32_IR || a.solver == FP_SOLVER_GMRES_IR) {
             if (!a.A32) return fp_fail(ctx, FP_ERR_ARG, "internal: no FP32 matrix");
             if (a.solver == FP_SOLVER_LU32_IR) {
-                if ((st = dalloc(ctx, perm, n * sizeof(int)))) return st;
-                if ((st = factor<float>(ctx, a.A32, n, a.ld, perm.as<int>(), &stats.info))) return st;
-                if ((st = dalloc(ctx, work, fp_getrs_work_bytes(n, 1)))) return st;
+                bool ok = false;
+                const char *pv = getenv("FLOWPANEL_LU32_PIVOTED");
+                if (!(pv && pv[0] == '1') && (st = rbt_setup(&ok))) return st;
+                if (!ok && (st = lu32_pivoted(stats))) return st;
             } else {
                 const int blk = (int)std::min<int64_t>(o.precond_block > 0 ? o.precond_block : 512, n);
                 if ((st = bj_setup(a.A32, FP_F32, blk))) return st;
@@ -646,13 +723,12 @@ struct Solver {
         FP_HIP(ctx, hipMemcpyAsync(rp, b, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         double rel = bn > 0 ? 1.0 : (bn == 0 ? 0.0 : NAN), rnorm = bn;   // ||r|| of the current residual (r = b at first)
         int k = 0, its_total = 0;
+        for (;;) {
         while (!converged(rel, o.tol) && std::isfinite(rel) && k < o.max_refine) {
             // correction d ~ A^-1 r in the low-precision solver
             if (a.solver == FP_SOLVER_LU32_IR) {
-                FP_HIP(ctx, hipMemcpyAsync(dp, rp, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-                if ((st = fp_getrs<float>(ctx, a.A32, n, a.ld, perm.as<int>(), dp, 1, n, work.p,
-                                          fp_getrs_work_bytes(n, 1))))
-                    return st;
+                void *wk = rbt ? work4.p : work.p;
+                if ((st = lu32_correct(rp, dp, 1, wk, fp_getrs_work_bytes(rbt ? n4 : n, 1)))) return st;
             } else {
                 int its = 0;
                 double irel = 0;
@@ -672,6 +748,17 @@ struct Solver {
             rel = rn / bn;
             rnorm = rn;
             ++k;
+        }
+        // the randomized no-pivot factors failed to converge: start over with the pivoted LU
+        if (rbt && !converged(rel, o.tol) && std::isfinite(bn)) {
+            if ((st = lu32_pivoted(stats))) return st;
+            FP_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
+            FP_HIP(ctx, hipMemcpyAsync(rp, b, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            rel = bn > 0 ? 1.0 : 0.0;
+            rnorm = bn;
+            continue;
+        }
+        break;
         }
         stats.ms_solve += tm.stop();
         stats.refinements += k;
@@ -719,9 +806,13 @@ struct Solver {
         for (int64_t c = 0; c < k; ++c)
             if (!std::isfinite(bn[c])) rel = NAN;
         int it = 0;
+        DBuf wk4;
+        const size_t wbytes4 = rbt ? fp_getrs_work_bytes(n4, k) : 0;
+        if (rbt && (st = dalloc(ctx, wk4, wbytes4))) return st;
+        const double rel0 = rel;
+        for (;;) {
         while (!converged(rel, o.tol) && std::isfinite(rel) && it < o.max_refine) {
-            FP_HIP(ctx, hipMemcpyAsync(D.p, R.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-            if ((st = fp_getrs<float>(ctx, a.A32, n, a.ld, perm.as<int>(), D.as<double>(), k, n, wk.p, wbytes)))
+            if ((st = lu32_correct(R.as<double>(), D.as<double>(), k, rbt ? wk4.p : wk.p, rbt ? wbytes4 : wbytes)))
                 return st;
             if ((st = fp_axpby(ctx, 1.0, D.as<double>(), 1.0, X, n * k))) return st;
             if ((st = fp_gemv_multi<double>(ctx, a.R64, n, a.ld, X, n, k, R.as<double>(), n))) return st;
@@ -733,6 +824,16 @@ struct Solver {
             for (int64_t c = 0; c < k; ++c)
                 rel = worst_residual(rel, bn[c] > 0 ? std::sqrt(rn[c] / bn[c]) : (rn[c] == 0 ? 0.0 : NAN));
             ++it;
+        }
+        // the randomized no-pivot factors failed to converge: start over with the pivoted LU
+        if (rbt && !converged(rel, o.tol) && std::isfinite(rel0)) {
+            if ((st = lu32_pivoted(stats))) return st;
+            FP_HIP(ctx, hipMemsetAsync(X, 0, (size_t)n * k * sizeof(double), ctx->stream));
+            FP_HIP(ctx, hipMemcpyAsync(R.p, B, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            rel = rel0;
+            continue;
+        }
+        break;
         }
         stats.ms_solve += tm.stop();
         stats.refinements += it;
@@ -859,6 +960,11 @@ struct BatchBody {
     Targets t;
     DBuf A64, A32, rhs, x, r, d, perm, info, fwork, swork;
     std::unique_ptr<MatrixFreeOp> mf;
+    // FP32 refinement: LU of the randomized matrix without pivoting (`pivoted` once it
+    // met a bad pivot or did not converge: the tournament-pivoted LU of A32 from then on)
+    bool pivoted = false;
+    int64_t n4 = 0, ld4 = 0;
+    DBuf Ar, W, perm4, work4, nwork, t4;
 };
 
 struct BatchImpl {
@@ -875,6 +981,7 @@ struct BatchImpl {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_nodes = 0;
+    bool recapture = false;   // a body switched to the pivoted LU since the last capture
     bool lu32() const { return o.solver == FP_SOLVER_LU32_IR; }
     bool stored() const { return o.solver == FP_SOLVER_LU || o.residual_mode == FP_RESID_STORED; }
 
@@ -913,6 +1020,20 @@ struct BatchImpl {
             const int64_t n = bb->n, ld = bb->ld;
             if (stored() && (st = dalloc(ctx, bb->A64, (size_t)n * ld * sizeof(double)))) return st;
             if (lu32() && (st = dalloc(ctx, bb->A32, (size_t)n * ld * sizeof(float)))) return st;
+            if (lu32()) {
+                bb->n4 = fp_rbt_size(n);
+                bb->ld4 = round_ld(bb->n4);
+                if ((st = dalloc(ctx, bb->Ar, (size_t)bb->n4 * bb->ld4 * sizeof(float))) ||
+                    (st = dalloc(ctx, bb->W, fp_rbt_weights_bytes(n))) || (st = dalloc(ctx, bb->perm4, bb->n4 * sizeof(int))) ||
+                    (st = dalloc(ctx, bb->work4, fp_getrs_work_bytes(bb->n4, 1))) ||
+                    (st = dalloc(ctx, bb->nwork, fp_getrf_nopiv_work_bytes())) ||
+                    (st = dalloc(ctx, bb->t4, bb->n4 * sizeof(double))))
+                    return st;
+                if ((st = fp_rbt_weights(ctx, n, RBT_SEED, bb->W.as<double>()))) return st;
+                std::vector<int> id(bb->n4);
+                for (int64_t i = 0; i < bb->n4; ++i) id[i] = (int)i;
+                FP_HIP(ctx, hipMemcpy(bb->perm4.p, id.data(), bb->n4 * sizeof(int), hipMemcpyHostToDevice));
+            }
             if ((st = dalloc(ctx, bb->rhs, n * sizeof(double))) || (st = dalloc(ctx, bb->x, n * sizeof(double))) ||
                 (st = dalloc(ctx, bb->r, n * sizeof(double))) || (st = dalloc(ctx, bb->d, n * sizeof(double))) ||
                 (st = dalloc(ctx, bb->perm, n * sizeof(int))) || (st = dalloc(ctx, bb->info, sizeof(int))) ||
@@ -997,7 +1118,8 @@ struct BatchImpl {
         } else {
             if ((st = fp_launch_assemble_wake<float>(ctx, b, bb.t.tv, bb.A32.as<float>(), ld, 0))) return st;
         }
-        if (lu32() && stored() && (st = fp_mat_narrow(ctx, bb.A64.as<double>(), bb.A32.as<float>(), n, ld))) return st;
+        if (lu32() && stored() && bb.pivoted && (st = fp_mat_narrow(ctx, bb.A64.as<double>(), bb.A32.as<float>(), n, ld)))
+            return st;
         if ((st = fp_rhs_kernel(ctx, b, rhs))) return st;
         if ((st = fp_colnorm2(ctx, rhs, n, n, 1, nb + k))) return st;
         if (!lu32()) {
@@ -1012,9 +1134,13 @@ struct BatchImpl {
             if ((st = bb.mf->apply(x, r))) return st;
             if ((st = fp_residual_update(ctx, rhs, r, n))) return st;
         } else {
-            if ((st = fp_getrf<float>(ctx, bb.A32.as<float>(), n, ld, bb.perm.as<int>(), bb.info.as<int>(), bb.fwork.p,
-                                      fp_getrf_work_bytes(n, sizeof(float)))))
+            if (bb.pivoted) {
+                if ((st = fp_getrf<float>(ctx, bb.A32.as<float>(), n, ld, bb.perm.as<int>(), bb.info.as<int>(),
+                                          bb.fwork.p, fp_getrf_work_bytes(n, sizeof(float)))))
+                    return st;
+            } else if ((st = rbt_factor(bb))) {
                 return st;
+            }
             if ((st = fp_fill(ctx, x, n, 0.0))) return st;
             FP_HIP(ctx, hipMemcpyAsync(r, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
             for (int s = 0; s < sweeps; ++s)
@@ -1025,21 +1151,75 @@ struct BatchImpl {
         return FP_OK;
     }
 
+    // U^T A V of the FP64 (stored) or FP32 AIC, then its LU without pivoting
+    int rbt_factor(BatchBody &bb)
+    {
+        int st = stored() ? fp_rbt_matrix<double>(ctx, bb.A64.as<double>(), bb.n, bb.ld, bb.Ar.as<float>(), bb.ld4,
+                                                  bb.W.as<double>())
+                          : fp_rbt_matrix<float>(ctx, bb.A32.as<float>(), bb.n, bb.ld, bb.Ar.as<float>(), bb.ld4,
+                                                 bb.W.as<double>());
+        if (st) return st;
+        return fp_getrf_nopiv_f32(ctx, bb.Ar.as<float>(), bb.n4, bb.ld4, bb.info.as<int>(), bb.nwork.p);
+    }
+
     // one refinement sweep: d = A32^-1 r, x += d, r = b - A x (FP64)
     int sweep(BatchBody &bb)
     {
         const int64_t n = bb.n, ld = bb.ld;
         double *x = bb.x.as<double>(), *r = bb.r.as<double>(), *dd = bb.d.as<double>();
         int st;
-        FP_HIP(ctx, hipMemcpyAsync(dd, r, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-        if ((st = fp_getrs<float>(ctx, bb.A32.as<float>(), n, ld, bb.perm.as<int>(), dd, 1, n, bb.swork.p,
-                                  fp_getrs_work_bytes(n, 1))))
-            return st;
+        if (!bb.pivoted) {
+            double *t = bb.t4.as<double>();
+            if ((st = fp_rbt_vec(ctx, r, n, t, bb.n4, n, 1, bb.W.as<double>(), 1))) return st;
+            if ((st = fp_getrs<float>(ctx, bb.Ar.as<float>(), bb.n4, bb.ld4, bb.perm4.as<int>(), t, 1, bb.n4,
+                                      bb.work4.p, fp_getrs_work_bytes(bb.n4, 1))))
+                return st;
+            if ((st = fp_rbt_vec(ctx, t, bb.n4, dd, n, n, 1, bb.W.as<double>(), 0))) return st;
+        } else {
+            FP_HIP(ctx, hipMemcpyAsync(dd, r, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            if ((st = fp_getrs<float>(ctx, bb.A32.as<float>(), n, ld, bb.perm.as<int>(), dd, 1, n, bb.swork.p,
+                                      fp_getrs_work_bytes(n, 1))))
+                return st;
+        }
         if ((st = fp_axpby(ctx, 1.0, dd, 1.0, x, n))) return st;
         if (stored()) st = fp_gemv<double>(ctx, bb.A64.as<double>(), n, ld, x, r, 1.0, 0.0);
         else st = bb.mf->apply(x, r);
         if (st) return st;
         return fp_residual_update(ctx, bb.rhs.as<double>(), r, n);
+    }
+
+    // eager re-solve of one body with the pivoted FP32 LU (stays pivoted; the graph is
+    // re-captured with it at the next solve)
+    int fallback(BatchBody &bb)
+    {
+        const int64_t n = bb.n, ld = bb.ld;
+        int st;
+        bb.pivoted = true;
+        recapture = true;
+        size_t k = 0;
+        while (bodies[k].get() != &bb) ++k;
+        if (stored() && (st = fp_mat_narrow(ctx, bb.A64.as<double>(), bb.A32.as<float>(), n, ld))) return st;
+        if (!stored()) {
+            // the FP32 AIC was assembled in place; the randomized copy left it intact
+        }
+        if ((st = fp_getrf<float>(ctx, bb.A32.as<float>(), n, ld, bb.perm.as<int>(), bb.info.as<int>(), bb.fwork.p,
+                                  fp_getrf_work_bytes(n, sizeof(float)))))
+            return st;
+        if ((st = fp_fill(ctx, bb.x.as<double>(), n, 0.0))) return st;
+        FP_HIP(ctx, hipMemcpyAsync(bb.r.p, bb.rhs.p, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        for (int s = 0; s < sweeps; ++s)
+            if ((st = sweep(bb))) return st;
+        if ((st = fp_colnorm2(ctx, bb.r.as<double>(), n, n, 1, norms.as<double>() + bodies.size() + k))) return st;
+        FP_HIP(ctx, hipMemcpyAsync(bb.b->d_strength, bb.x.p, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        int info = 0;
+        FP_HIP(ctx, hipMemcpyAsync(&info, bb.info.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (info) {
+            char buf[128];
+            snprintf(buf, sizeof buf, "singular matrix: zero pivot at column %d", info);
+            return fp_fail(ctx, FP_ERR_SINGULAR, buf);
+        }
+        return FP_OK;
     }
 
     int capture()
@@ -1074,6 +1254,7 @@ struct BatchImpl {
         if (st) { if (g) (void)hipGraphDestroy(g); return st; }
         if (e != hipSuccess) return fp_hip_fail(ctx, e, "hipStreamEndCapture");
         graph = g;
+        recapture = false;
         size_t nn = 0;
         if (hipGraphGetNodes(graph, nullptr, &nn) == hipSuccess) graph_nodes = (int)nn;
         FP_HIP(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -1522,6 +1703,162 @@ int fp_solve_bodies(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_solve_
 // G^T G (one FP64 GEMM of the transposed AIC with the AIC) with the prescribed
 // rows and columns replaced by the identity, which gives the same x and the
 // prescribed values in place (set_solution, :290-316).
+// The same least-squares solution without the normal equations.  With k
+// prescribed strengths the reduced matrix G_red (n x (n-k), full column rank)
+// has a range of codimension k, spanned in its complement by Z = null(G_red^T).
+// B = G with the prescribed columns replaced by G[:, p] + w_p (w_p: the panel
+// areas of p's body, scaled to |G_pp|, for the first prescribed strength of each
+// body -- a vector outside the range of G also for a closed body, whose uniform
+// strengths G annihilates) is square and nonsingular, and
+//   B^T Z = E_P      (E_P the unit vectors of the prescribed strengths)
+// gives Z exactly: its rows j not in P say G[:, j]^T Z = 0.  The least-squares
+// residual is the projection r* = Z (Z^T Z)^-1 Z^T b_red of b_red onto that
+// complement, and  B y = b_red - r*  is consistent with y_P = 0 and y = the
+// least-squares solution elsewhere.  Two square solves with the chosen solver
+// (one transposed, k right-hand sides): condition kappa(G), not kappa(G)^2, and
+// (4/3) n^3 flop for LU against 2 n^3 + (2/3) n^3 through G^T G.
+static int lsq_bordered(fp_ctx *ctx, fp_body *const *bodies, int nb, const std::vector<int64_t> &pidx,
+                        const double *pres_val, const fp_solve_opts &o, fp_solve_stats &s, double *x_out)
+{
+    const int np = (int)pidx.size();
+    int st;
+    Timer tm(ctx->stream);
+    Targets t;
+    if ((st = gather_targets(ctx, bodies, nb, t))) return st;
+    const int64_t n = t.M, ld = round_ld(n);
+    DBuf G, B, Bt, rhs, dp, dv, w, Z, E, part, dots, x;
+    if ((st = dalloc(ctx, G, (size_t)n * ld * sizeof(double))) || (st = dalloc(ctx, B, (size_t)n * ld * sizeof(double))) ||
+        (st = dalloc(ctx, Bt, (size_t)n * ld * sizeof(double))) || (st = dalloc(ctx, rhs, n * sizeof(double))) ||
+        (st = dalloc(ctx, dp, std::max(1, np) * sizeof(int64_t))) || (st = dalloc(ctx, dv, std::max(1, np) * sizeof(double))) ||
+        (st = dalloc(ctx, w, (size_t)std::max(1, np) * n * sizeof(double))) ||
+        (st = dalloc(ctx, Z, (size_t)std::max(1, np) * n * sizeof(double))) ||
+        (st = dalloc(ctx, E, (size_t)std::max(1, np) * n * sizeof(double))) ||
+        (st = dalloc(ctx, part, (size_t)512 * (np + 2) * sizeof(double))) ||
+        (st = dalloc(ctx, dots, (size_t)(np + 1) * (np + 1) * sizeof(double))) || (st = dalloc(ctx, x, n * sizeof(double))))
+        return st;
+    if (np) {
+        FP_HIP(ctx, hipMemcpyAsync(dp.p, pidx.data(), np * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+        FP_HIP(ctx, hipMemcpyAsync(dv.p, pres_val, np * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    }
+    int64_t col = 0;
+    for (int b = 0; b < nb; ++b) {
+        if ((st = fp_rhs_kernel(ctx, bodies[b], rhs.as<double>() + col))) return st;
+        col += bodies[b]->N;
+    }
+    if ((st = assemble_into(ctx, bodies, nb, t.tv, FP_F64, G.p, ld))) return st;
+    if ((st = fp_lsq_rhs(ctx, G.as<double>(), n, ld, dp.as<int64_t>(), dv.as<double>(), np, rhs.as<double>())))
+        return st;
+    // border vectors: the areas of the body holding p, scaled to |G_pp|, for the first p of every body
+    std::vector<double> areas(n), gpp(np);
+    col = 0;
+    for (int b = 0; b < nb; ++b) {
+        FP_HIP(ctx, hipMemcpyAsync(areas.data() + col, bodies[b]->d_geo + 6 * bodies[b]->N, bodies[b]->N * sizeof(double),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+        col += bodies[b]->N;
+    }
+    for (int q = 0; q < np; ++q)
+        FP_HIP(ctx, hipMemcpyAsync(&gpp[q], G.as<double>() + pidx[q] * ld + pidx[q], sizeof(double),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<double> wh((size_t)std::max(1, np) * n, 0.0), eh((size_t)std::max(1, np) * n, 0.0);
+    std::vector<char> seen(nb, 0);
+    for (int q = 0; q < np; ++q) {
+        int64_t lo = 0;
+        int b = 0;
+        while (b < nb && pidx[q] >= lo + bodies[b]->N) { lo += bodies[b]->N; ++b; }
+        eh[(size_t)q * n + pidx[q]] = 1.0;
+        if (seen[b]) continue;
+        seen[b] = 1;
+        double amean = 0;
+        for (int64_t i = lo; i < lo + bodies[b]->N; ++i) amean += areas[i];
+        amean /= (double)bodies[b]->N;
+        const double sc = (std::fabs(gpp[q]) > 0 ? std::fabs(gpp[q]) : 1.0) / (amean > 0 ? amean : 1.0);
+        for (int64_t i = lo; i < lo + bodies[b]->N; ++i) wh[(size_t)q * n + i] = sc * areas[i];
+    }
+    if (np) {
+        FP_HIP(ctx, hipMemcpyAsync(w.p, wh.data(), (size_t)np * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        FP_HIP(ctx, hipMemcpyAsync(E.p, eh.data(), (size_t)np * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    }
+    FP_HIP(ctx, hipMemcpyAsync(B.p, G.p, (size_t)n * ld * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    if ((st = fp_lsq_border(ctx, B.as<double>(), n, ld, dp.as<int64_t>(), np, w.as<double>()))) return st;
+    if ((st = fp_mat_transpose(ctx, B.as<double>(), n, ld, Bt.p, FP_F64))) return st;
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));   // the host vectors go out of scope
+    s.ms_assemble = tm.stop();
+    // solve B^T Z = E_P, then B y = b_red - Z (Z^T Z)^-1 Z^T b_red, with the chosen solver
+    auto square_solve = [&](DBuf &M, const double *rhs_d, int64_t k, double *out) -> int {
+        SolveArgs a;
+        a.solver = o.solver;
+        a.n = n;
+        a.ld = ld;
+        DBuf W, W32;
+        std::unique_ptr<DenseOp<double>> op(new DenseOp<double>(ctx, M.as<double>(), n, ld));
+        a.resid = op.get();
+        a.R64 = M.as<double>();
+        int s2;
+        if (o.solver == FP_SOLVER_LU) {
+            if ((s2 = dalloc(ctx, W, (size_t)n * ld * sizeof(double)))) return s2;
+            FP_HIP(ctx, hipMemcpyAsync(W.p, M.p, (size_t)n * ld * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            a.A64 = W.as<double>();
+        } else if (o.solver == FP_SOLVER_GMRES) {
+            a.A64 = M.as<double>();
+        } else {
+            if ((s2 = dalloc(ctx, W32, (size_t)n * ld * sizeof(float)))) return s2;
+            if ((s2 = fp_mat_narrow(ctx, M.as<double>(), W32.as<float>(), n, ld))) return s2;
+            a.A32 = W32.as<float>();
+        }
+        Solver solver(ctx, a, o);
+        if ((s2 = solver.setup(s))) return s2;
+        return solver.solve_multi(rhs_d, out, k, s);
+    };
+    if (np) {
+        if ((st = square_solve(Bt, E.as<double>(), np, Z.as<double>()))) return st;
+        // c = (Z^T Z)^-1 Z^T b_red on the host (np x np), then rhs -= Z c
+        std::vector<double> ztz((size_t)np * np), ztb(np);
+        double *dd = dots.as<double>();
+        for (int q = 0; q < np; ++q)
+            if ((st = fp_mdot(ctx, Z.as<double>(), n, np, Z.as<double>() + (size_t)q * n, n, part.as<double>(),
+                              dd + (size_t)q * np)))
+                return st;
+        if ((st = fp_mdot(ctx, Z.as<double>(), n, np, rhs.as<double>(), n, part.as<double>(), dd + (size_t)np * np)))
+            return st;
+        FP_HIP(ctx, hipMemcpyAsync(ztz.data(), dd, (size_t)np * np * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        FP_HIP(ctx, hipMemcpyAsync(ztb.data(), dd + (size_t)np * np, np * sizeof(double), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        // Gaussian elimination with partial pivoting (np is small)
+        std::vector<double> c = ztb;
+        for (int k = 0; k < np; ++k) {
+            int pr = k;
+            for (int r = k + 1; r < np; ++r)
+                if (std::fabs(ztz[(size_t)r * np + k]) > std::fabs(ztz[(size_t)pr * np + k])) pr = r;
+            if (!(std::fabs(ztz[(size_t)pr * np + k]) > 0))
+                return fp_fail(ctx, FP_ERR_SINGULAR, "least squares: the complement of the reduced range is degenerate");
+            for (int cc = 0; cc < np; ++cc) std::swap(ztz[(size_t)k * np + cc], ztz[(size_t)pr * np + cc]);
+            std::swap(c[k], c[pr]);
+            for (int r = k + 1; r < np; ++r) {
+                const double f = ztz[(size_t)r * np + k] / ztz[(size_t)k * np + k];
+                for (int cc = k; cc < np; ++cc) ztz[(size_t)r * np + cc] -= f * ztz[(size_t)k * np + cc];
+                c[r] -= f * c[k];
+            }
+        }
+        for (int k = np - 1; k >= 0; --k) {
+            for (int cc = k + 1; cc < np; ++cc) c[k] -= ztz[(size_t)k * np + cc] * c[cc];
+            c[k] /= ztz[(size_t)k * np + k];
+        }
+        for (int q = 0; q < np; ++q)
+            if ((st = fp_axpby(ctx, -c[q], Z.as<double>() + (size_t)q * n, 1.0, rhs.as<double>(), n))) return st;
+    }
+    Bt.reset();
+    if ((st = square_solve(B, rhs.as<double>(), 1, x.as<double>()))) return st;
+    // the prescribed values in place (y_P is the border's coefficient, zero up to rounding)
+    std::vector<double> xp(n);
+    FP_HIP(ctx, hipMemcpyAsync(xp.data(), x.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int q = 0; q < np; ++q) xp[pidx[q]] = pres_val[q];
+    std::memcpy(x_out, xp.data(), n * sizeof(double));
+    return FP_OK;
+}
+
 int fp_solve_bodies_lsq(fp_ctx *ctx, fp_body *const *bodies, int nb, const int64_t *pres_idx, const double *pres_val,
                         int64_t npres, int index_base, const fp_solve_opts *opts, fp_solve_stats *stats)
 {
@@ -1544,6 +1881,22 @@ int fp_solve_bodies_lsq(fp_ctx *ctx, fp_body *const *bodies, int nb, const int64
             if (pidx[r] == pidx[q]) return fp_fail(ctx, FP_ERR_ARG, "element prescribed twice");
     }
     fp_solve_stats s{};
+    const char *ne = getenv("FLOWPANEL_LSQ_NORMAL");
+    if (!(ne && ne[0] == '1')) {
+        std::vector<double> xh(n);
+        st = lsq_bordered(ctx, bodies, nb, pidx, pres_val, o, s, xh.data());
+        if (stats) *stats = s;
+        if (st) return st;
+        int64_t off = 0;
+        for (int b = 0; b < nb; ++b) {
+            FP_HIP(ctx, hipMemcpyAsync(bodies[b]->d_strength, xh.data() + off, bodies[b]->N * sizeof(double),
+                                       hipMemcpyHostToDevice, ctx->stream));
+            bodies[b]->solved = true;
+            off += bodies[b]->N;
+        }
+        FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return FP_OK;
+    }
     Timer tm(ctx->stream);
     Targets t;
     if ((st = gather_targets(ctx, bodies, nb, t))) return st;
@@ -1957,6 +2310,124 @@ int fp_calcfield_cp(fp_ctx *ctx, const double *Us, int64_t n, double uref, const
     FP_HIP(ctx, hipGetLastError());
     FP_HIP(ctx, hipMemcpyAsync(out, o.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FP_OK;
+}
+
+static int ensure_shedding_on_device(fp_ctx *ctx, fp_body *b)
+{
+    if (b->ns > 0 && !b->d_sh) {
+        FP_HIP(ctx, hipMalloc((void **)&b->d_sh, 6 * b->ns * sizeof(int64_t)));
+        FP_HIP(ctx, hipMemcpy(b->d_sh, b->shedding.data(), 6 * b->ns * sizeof(int64_t), hipMemcpyHostToDevice));
+    }
+    return FP_OK;
+}
+
+// calcfield_F! with calcfield_Ftot! and calcfield_Mtot! (src/FLOWPanel_postprocess.jl:1199-1307, 1543-1784)
+int fp_calcfield_F(fp_ctx *ctx, fp_body *const *bodies, int nb, const double *Cps, double Uinf, double rho,
+                   int kutta_mode, const double *Xac, const double *X, double *F, double *Ftot, double *Mtot)
+{
+    int st = check_bodies(ctx, bodies, nb);
+    if (st) return st;
+    if (!Cps) return fp_fail(ctx, FP_ERR_ARG, "null Cps");
+    if (kutta_mode < FP_KUTTA_NONE || kutta_mode > FP_KUTTA_PAIR) return fp_fail(ctx, FP_ERR_ARG, "unknown Kutta mode");
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    int64_t M = 0;
+    size_t pb = 0;
+    for (int k = 0; k < nb; ++k) {
+        M += bodies[k]->N;
+        pb = std::max(pb, fp_loads_partial_bytes(bodies[k]->N));
+    }
+    DBuf cp, Fd, part, o6, Xd;
+    if ((st = dalloc(ctx, cp, M * sizeof(double))) || (st = dalloc(ctx, Fd, 3 * M * sizeof(double))) ||
+        (st = dalloc(ctx, part, pb)) || (st = dalloc(ctx, o6, 6 * nb * sizeof(double))))
+        return st;
+    FP_HIP(ctx, hipMemcpyAsync(cp.p, Cps, M * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if (X) {
+        if ((st = dalloc(ctx, Xd, 3 * M * sizeof(double)))) return st;
+        int64_t off = 0;
+        for (int k = 0; k < nb; ++k) {
+            if ((st = upload_soa3(ctx, X + 3 * off, bodies[k]->N, Xd.as<double>() + 3 * off))) return st;
+            off += bodies[k]->N;
+        }
+    }
+    const double zero[3] = {0, 0, 0};
+    const double q = 0.5 * rho * Uinf * Uinf;
+    std::vector<char> te(M, 0);
+    int64_t off = 0;
+    for (int k = 0; k < nb; ++k) {
+        fp_body *b = bodies[k];
+        const bool kutta = kutta_mode != FP_KUTTA_NONE && b->element == FP_ELEM_VORTEXRING && b->ns > 0;
+        if (kutta) {
+            // the Kutta override of the force: the averaged Cp of the trailing-edge cells replaces theirs
+            if ((st = ensure_shedding_on_device(ctx, b))) return st;
+            hipLaunchKernelGGL(k_kutta, dim3(1), dim3(64), 0, ctx->stream, (const int64_t *)b->d_sh, b->ns, kutta_mode,
+                               cp.as<double>() + off);
+            for (int64_t e = 0; e < b->ns; ++e) {
+                const int64_t pi = b->shedding[6 * e], pj = b->shedding[6 * e + 3];
+                if (kutta_mode == FP_KUTTA_REFERENCE) {
+                    te[off + pi] = te[off + pi + 1] = 1;
+                    if (pj != -1) te[off + pj] = te[off + pj - 1] = 1;
+                } else if (pj != -1) {
+                    te[off + pi] = te[off + pj] = 1;
+                }
+            }
+        }
+        if ((st = fp_launch_loads(ctx, b, cp.as<double>() + off, q, X ? Xd.as<double>() + 3 * off : nullptr,
+                                  Xac ? Xac : zero, Fd.as<double>() + 3 * off, part.as<double>(),
+                                  o6.as<double>() + 6 * k)))
+            return st;
+        off += b->N;
+    }
+    std::vector<double> h6(6 * nb), hF(F ? 3 * M : 0);
+    FP_HIP(ctx, hipMemcpyAsync(h6.data(), o6.p, h6.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (F) FP_HIP(ctx, hipMemcpyAsync(hF.data(), Fd.p, hF.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (F)
+        for (int64_t i = 0; i < M; ++i)
+            for (int d = 0; d < 3; ++d) F[3 * i + d] = te[i] ? hF[3 * i + d] : F[3 * i + d] + hF[3 * i + d];
+    for (int k = 0; k < nb; ++k)
+        for (int d = 0; d < 3; ++d) {
+            if (Ftot) Ftot[d] += h6[6 * k + d];
+            if (Mtot) Mtot[d] += h6[6 * k + 3 + d];
+        }
+    return FP_OK;
+}
+
+// calcfield_sectionalforce! (src/FLOWPanel_postprocess.jl:1411-1542)
+int fp_calcfield_sectionalforce(fp_ctx *ctx, const fp_body *b, const double *F, const double *X, int64_t nchord,
+                                int64_t nspan, const double *spandir, double *outF, double *outpos)
+{
+    if (!ctx) return FP_ERR_ARG;
+    if (!b || b->ctx != ctx) return fp_fail(ctx, FP_ERR_ARG, "invalid body handle");
+    if (!F || !spandir || !outF || !outpos) return fp_fail(ctx, FP_ERR_ARG, "null argument");
+    if (nchord <= 0 || nspan < 2 || nchord * nspan != b->N)
+        return fp_fail(ctx, FP_ERR_SHAPE, "sectional loads need a structured grid of nchord x nspan >= 2 cells");
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t N = b->N;
+    DBuf Fd, Xd, of, op;
+    int st;
+    if ((st = dalloc(ctx, Fd, 3 * N * sizeof(double))) || (st = dalloc(ctx, of, 3 * nspan * sizeof(double))) ||
+        (st = dalloc(ctx, op, nspan * sizeof(double))))
+        return st;
+    FP_HIP(ctx, hipMemcpyAsync(Fd.p, F, 3 * N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    const double *Xs = b->d_geo + 3 * N;   // the control points (SoA)
+    if (X) {
+        if ((st = dalloc(ctx, Xd, 3 * N * sizeof(double))) || (st = upload_soa3(ctx, X, N, Xd.as<double>()))) return st;
+        Xs = Xd.as<double>();
+    }
+    if ((st = fp_launch_sectional(ctx, Fd.as<double>(), Xs, N, nchord, nspan, spandir, of.as<double>(),
+                                  op.as<double>())))
+        return st;
+    std::vector<double> hf(3 * nspan), hp(nspan);
+    FP_HIP(ctx, hipMemcpyAsync(hf.data(), of.p, hf.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipMemcpyAsync(hp.data(), op.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // per unit span: central differences of the station positions, one-sided at the ends
+    for (int64_t j = 0; j < nspan; ++j) {
+        const double dp = j == 0 ? hp[1] - hp[0] : (j == nspan - 1 ? hp[j] - hp[j - 1] : (hp[j + 1] - hp[j - 1]) / 2);
+        for (int d = 0; d < 3; ++d) outF[3 * j + d] = hf[3 * j + d] / std::fabs(dp);
+        outpos[j] = hp[j];
+    }
     return FP_OK;
 }
 
@@ -2530,7 +3001,7 @@ int fp_batch_solve(fp_ctx *ctx, fp_batch *h, fp_batch_stats *stats)
     FP_HIP(ctx, hipSetDevice(ctx->device));
     fp_batch_stats S{};
     int st;
-    bool stale = false;
+    bool stale = B.recapture;
     for (auto &bb : B.bodies) stale = stale || bb->version != bb->b->version || !bb->b->has_uinf;
     if (stale) {
         for (auto &bb : B.bodies) {
@@ -2553,14 +3024,26 @@ int fp_batch_solve(fp_ctx *ctx, fp_batch *h, fp_batch_stats *stats)
     std::vector<double> nr(2 * nb);
     FP_HIP(ctx, hipMemcpyAsync(nr.data(), B.norms.p, nr.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    bool refall = false;
     for (auto &bb : B.bodies) {
         int info = 0;
         FP_HIP(ctx, hipMemcpy(&info, bb->info.p, sizeof(int), hipMemcpyDeviceToHost));
+        if (info && B.lu32() && !bb->pivoted) {
+            // the randomized no-pivot LU met a bad pivot: this body switches to the pivoted LU
+            if ((st = B.fallback(*bb))) return st;
+            refall = true;
+            continue;
+        }
         if (info) {
             char buf[128];
             snprintf(buf, sizeof buf, "singular matrix: zero pivot at column %d", info);
             return fp_fail(ctx, FP_ERR_SINGULAR, buf);
         }
+    }
+    if (refall) {
+        FP_HIP(ctx, hipMemcpyAsync(nr.data(), B.norms.p, nr.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
     // bodies that need more refinement continue eagerly on the context stream
     Timer te(ctx->stream);
@@ -2570,6 +3053,7 @@ int fp_batch_solve(fp_ctx *ctx, fp_batch *h, fp_batch_stats *stats)
         BatchBody &bb = *B.bodies[k];
         double rel = nr[k] > 0 ? std::sqrt(nr[nb + k] / nr[k]) : (nr[nb + k] == 0 ? 0.0 : NAN);
         int steps = B.lu32() ? B.sweeps : 0;
+        for (;;) {
         while (B.lu32() && !converged(rel, B.o.tol) && std::isfinite(rel) && steps < B.o.max_refine) {
             if ((st = B.sweep(bb))) return st;
             if ((st = fp_colnorm2(ctx, bb.r.as<double>(), bb.n, bb.n, 1, B.norms.as<double>() + nb + k))) return st;
@@ -2582,6 +3066,19 @@ int fp_batch_solve(fp_ctx *ctx, fp_batch *h, fp_batch_stats *stats)
             ++extra;
             FP_HIP(ctx, hipMemcpyAsync(bb.b->d_strength, bb.x.p, bb.n * sizeof(double), hipMemcpyDeviceToDevice,
                                        ctx->stream));
+        }
+        if (B.lu32() && !bb.pivoted && !converged(rel, B.o.tol) && std::isfinite(nr[k])) {
+            // the randomized factors did not bring the residual down: pivot this body and refine again
+            if ((st = B.fallback(bb))) return st;
+            double v = 0;
+            FP_HIP(ctx, hipMemcpyAsync(&v, B.norms.as<double>() + nb + k, sizeof(double), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+            FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            rel = nr[k] > 0 ? std::sqrt(v / nr[k]) : (v == 0 ? 0.0 : NAN);
+            steps = B.sweeps;
+            continue;
+        }
+        break;
         }
         worst = worst_residual(worst, rel);
         S.panels += bb.n;

@@ -340,6 +340,25 @@ int fp_lsq_rhs(fp_ctx *ctx, const double *G, int64_t n, int64_t ld, const int64_
     return FP_OK;
 }
 
+// Bordered matrix of the least-squares solve: column p_q of B gets the border
+// vector w_q added (w: np x n, row q contiguous).
+__global__ void k_lsq_border(double *__restrict__ B, int64_t n, int64_t ld, const int64_t *__restrict__ pidx, int np,
+                             const double *__restrict__ w)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int q = 0; q < np; ++q) B[i * ld + pidx[q]] += w[(int64_t)q * n + i];
+}
+
+int fp_lsq_border(fp_ctx *ctx, double *B, int64_t n, int64_t ld, const int64_t *pidx, int np, const double *w)
+{
+    if (n <= 0 || np <= 0) return FP_OK;
+    hipLaunchKernelGGL(k_lsq_border, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, B, n, ld, pidx, np,
+                       w);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
 // ----------------------------------------------------------------------------
 // X (nb x ncols, row-major ld) := L^-1 X with L (nb x nb, ld) unit lower, nb <= 256.
 // One workgroup per 32 columns; X in LDS; diagonal 32 x 32 blocks solved by one

@@ -219,6 +219,22 @@ int fp_launch_assemble_queue(fp_ctx *ctx, const AssemblyItem *d_items, int nitem
 int fp_launch_field(fp_ctx *ctx, const fp_body *b, const double *strength, const TargetView &tv, int mode,
                     double *out, double *partial, size_t partial_bytes);
 size_t fp_field_partial_bytes(int64_t M, int64_t N);
+// element forces, integrated force and moment, sectional loads (fp_post.hip)
+int fp_launch_loads(fp_ctx *ctx, const fp_body *b, const double *cp, double q, const double *X, const double *Xac,
+                    double *F, double *part, double *out6);
+size_t fp_loads_partial_bytes(int64_t N);
+int fp_launch_sectional(fp_ctx *ctx, const double *F, const double *X, int64_t N, int64_t nchord, int64_t nspan,
+                        const double *spandir, double *outF, double *pos);
+// FP32 LU of a randomized matrix without pivoting (fp_lu_nopiv.hip)
+int64_t fp_rbt_size(int64_t n);
+size_t fp_rbt_weights_bytes(int64_t n);
+int fp_rbt_weights(fp_ctx *ctx, int64_t n, uint64_t seed, double *d_W);
+template <typename Tin>
+int fp_rbt_matrix(fp_ctx *ctx, const Tin *A, int64_t n, int64_t lda, float *Ar, int64_t ldr, const double *d_W);
+int fp_rbt_vec(fp_ctx *ctx, const double *in, int64_t ldin, double *out, int64_t ldout, int64_t n, int64_t k,
+               const double *d_W, int transpose_u);
+size_t fp_getrf_nopiv_work_bytes();
+int fp_getrf_nopiv_f32(fp_ctx *ctx, float *A, int64_t n, int64_t ld, int *info_dev, void *work);
 
 // ---- dense linear algebra (fp_dense.hip) --------------------------------
 // P A = L U in place (row-major); perm[i] = original row now at row i.
@@ -340,5 +356,7 @@ int fp_adj_nodes(fp_ctx *ctx, const fp_body *b, const int64_t *ptr, const int64_
                  const double *vbar, double *out);
 // B = A^T (row-major n x n, leading dimension ld for both), FP64 or narrowed to FP32
 int fp_mat_transpose(fp_ctx *ctx, const double *A, int64_t n, int64_t ld, void *B, int dst_prec);
+// column p_q of B += w_q (w: np x n) -- the border of the least-squares solve
+int fp_lsq_border(fp_ctx *ctx, double *B, int64_t n, int64_t ld, const int64_t *pidx, int np, const double *w);
 
 // ---- Krylov (fp_abi.hip) ---------------------------------------------------

@@ -1,0 +1,383 @@
+// fp_lu_nopiv.hip -- FP32 LU of a randomized matrix without pivoting, for the
+// mixed-precision refinement path (FP_SOLVER_LU32_IR).
+//
+// The refinement needs only an FP32 approximate inverse; the FP64 residual
+// r = b - A x decides the result.  Partial pivoting is what makes the blocked
+// LU latency-bound at the sizes of a wing (4k-12k unknowns): every 32-column
+// sub-panel runs a tournament of pivot searches.  A random butterfly
+// transformation (RBT, Parker 1995; Baboulin, Dongarra, Herrmann, Tomov 2013)
+// A_r = U^T A V with two depth-2 recursive butterflies U, V makes pivoting
+// unnecessary with probability close to one, and the LU of A_r is then a
+// plain blocked right-looking factorisation:
+//
+//   per 64-column step:  k_nopiv_diag   the 64 x 64 diagonal block factored in
+//                                       registers (one row per lane, row k
+//                                       broadcast by v_readlane), then the
+//                                       explicit inverses of L11 and U11 (two
+//                                       waves, one each);
+//                        k_nopiv_panel  L21 = A21 U11^-1 and U12 = L11^-1 A12 as
+//                                       64 x 64 x 64 products on MFMA, in place;
+//                        fp_gemm_sub    A22 -= L21 U12 (the MFMA trailing update).
+//
+// A x = b becomes  A_r y = U^T b,  x = V y.  A depth-2 butterfly couples only
+// the four indices {p, p + m, p + 2m, p + 3m} (m = n4 / 4, n4 = n rounded up
+// to a multiple of 4, the pad being the identity), so U^T A V is one pass of
+// 4 x 4 . 4 x 4 . 4 x 4 products over the (p, q) orbit blocks, and the vector
+// transforms are 4 x 4 products per orbit.  A zero or non-finite pivot is
+// reported through `info` and the caller falls back to the pivoted LU.
+#include "fp_internal.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace {
+
+constexpr int NPB = 64;   // block of the no-pivot LU
+
+typedef float nf4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bcast(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Diagonal block at (k0, k0): LU without pivoting in place (rows beyond n are
+// the identity), then Linv = L11^-1 (unit lower) and Uinv = U11^-1, 64 x 64
+// row-major.  Wave 0 holds row `lane` of the block in registers and factors
+// it; the broadcast of row j is one v_readlane per entry (j is a compile-time
+// index, every loop is unrolled).  Then wave 0 inverts U from its registers
+// while wave 1 inverts L from the copy in LDS.
+__global__ __launch_bounds__(128) void k_nopiv_diag(float *__restrict__ A, int64_t ld, int64_t n, int64_t k0,
+                                                   float *__restrict__ Linv, float *__restrict__ Uinv,
+                                                   int *__restrict__ info)
+{
+    __shared__ float lu[NPB][NPB + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int bs = (int)min((int64_t)NPB, n - k0);
+    float a[NPB], x[NPB];
+    if (wave == 0) {
+        float *row = A + (k0 + lane) * ld + k0;
+        if (bs == NPB) {
+#pragma unroll
+            for (int c = 0; c < NPB; c += 4) {
+                const float4 v = *reinterpret_cast<const float4 *>(row + c);
+                a[c] = v.x; a[c + 1] = v.y; a[c + 2] = v.z; a[c + 3] = v.w;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < NPB; ++c) a[c] = (lane < bs && c < bs) ? row[c] : (lane == c ? 1.f : 0.f);
+        }
+        bool bad = false;
+#pragma unroll
+        for (int j = 0; j < NPB; ++j) {
+            const float piv = bcast(a[j], j);
+            bad = bad || !(fabsf(piv) > 0.f) || !(fabsf(piv) <= 3.4e38f);
+            const float l = a[j] / piv;
+            const bool below = lane > j;
+            a[j] = below ? l : a[j];
+#pragma unroll
+            for (int c = j + 1; c < NPB; ++c) {
+                const float s = bcast(a[c], j);
+                a[c] = below ? fmaf(-l, s, a[c]) : a[c];
+            }
+        }
+        if (bad && lane == 0) atomicCAS(info, 0, (int)(k0 + 1));
+        if (lane < bs) {
+            if (bs == NPB) {
+#pragma unroll
+                for (int c = 0; c < NPB; c += 4)
+                    *reinterpret_cast<float4 *>(row + c) = make_float4(a[c], a[c + 1], a[c + 2], a[c + 3]);
+            } else {
+#pragma unroll
+                for (int c = 0; c < NPB; ++c)
+                    if (c < bs) row[c] = a[c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NPB; ++c) lu[lane][c] = a[c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NPB; ++c) x[c] = c == lane ? 1.f : 0.f;
+    if (wave == 0) {
+        // U^-1: rows bottom-up; Y_i -= (u_ic / u_cc) Y_c for i < c, then X_i = Y_i / u_ii
+        float dii = 1.f;
+#pragma unroll
+        for (int c = NPB - 1; c >= 0; --c) {
+            const float rc = 1.f / bcast(a[c], c);
+            const float coef = lane < c ? a[c] * rc : 0.f;
+            dii = lane == c ? a[c] : dii;
+#pragma unroll
+            for (int q = c; q < NPB; ++q) x[q] = fmaf(-coef, bcast(x[q], c), x[q]);
+        }
+        const float rd = 1.f / dii;
+#pragma unroll
+        for (int c = 0; c < NPB; c += 4)
+            *reinterpret_cast<float4 *>(Uinv + lane * NPB + c) =
+                make_float4(x[c] * rd, x[c + 1] * rd, x[c + 2] * rd, x[c + 3] * rd);
+    } else {
+        // L^-1 (unit lower): rows top-down; X_i -= l_ic X_c for i > c
+#pragma unroll
+        for (int c = 0; c < NPB; ++c) a[c] = c < lane ? lu[lane][c] : 0.f;
+#pragma unroll
+        for (int c = 0; c < NPB; ++c) {
+            const float coef = a[c];
+#pragma unroll
+            for (int q = 0; q <= c; ++q) x[q] = fmaf(-coef, bcast(x[q], c), x[q]);
+        }
+#pragma unroll
+        for (int c = 0; c < NPB; c += 4)
+            *reinterpret_cast<float4 *>(Linv + lane * NPB + c) = make_float4(x[c], x[c + 1], x[c + 2], x[c + 3]);
+    }
+}
+
+__global__ void k_fill_info(int *p, int v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// One 64 x 64 tile of the block row / column of step k0, in place:
+// blocks [0, nrt): L21 tile rows off..off+63 := A21 U11^-1;
+// blocks [nrt, 2 nrt): U12 tile columns off..off+63 := L11^-1 A12.
+// Both operands in LDS; wave w computes output rows 16w..16w+15 with
+// v_mfma_f32_16x16x4 (lane l supplies P(l%16, k + l/16) and Q(k + l/16, l%16);
+// accumulator element r holds C(4 (l/16) + r, l%16)).
+__global__ __launch_bounds__(256) void k_nopiv_panel(float *__restrict__ A, int64_t ld, int64_t n, int64_t k0,
+                                                    const float *__restrict__ Linv, const float *__restrict__ Uinv,
+                                                    int nrt)
+{
+    __shared__ float P[NPB][NPB + 4];
+    __shared__ float Q[NPB][NPB + 4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int b = blockIdx.x;
+    const bool lpanel = b < nrt;
+    const int64_t off = k0 + NPB + (int64_t)NPB * (lpanel ? b : b - nrt);
+    float *tile = lpanel ? A + off * ld + k0 : A + k0 * ld + off;
+    const int tr = (int)min((int64_t)NPB, lpanel ? n - off : n - k0);
+    const int tc = (int)min((int64_t)NPB, lpanel ? n - k0 : n - off);
+    // the tile is the left operand of L21 (times U11^-1), the right one of U12 (L11^-1 times it)
+    const float *inv = lpanel ? Uinv : Linv;
+    const int r = tid >> 2, c0 = (tid & 3) * 16;
+    const bool full = tr == NPB && tc == NPB;
+#pragma unroll
+    for (int e = 0; e < 16; e += 4) {
+        float4 v;
+        if (full) {
+            v = *reinterpret_cast<const float4 *>(tile + r * ld + c0 + e);
+        } else {
+            v.x = (r < tr && c0 + e < tc) ? tile[r * ld + c0 + e] : 0.f;
+            v.y = (r < tr && c0 + e + 1 < tc) ? tile[r * ld + c0 + e + 1] : 0.f;
+            v.z = (r < tr && c0 + e + 2 < tc) ? tile[r * ld + c0 + e + 2] : 0.f;
+            v.w = (r < tr && c0 + e + 3 < tc) ? tile[r * ld + c0 + e + 3] : 0.f;
+        }
+        const float4 w = *reinterpret_cast<const float4 *>(inv + r * NPB + c0 + e);
+        if (lpanel) {
+            P[r][c0 + e] = v.x; P[r][c0 + e + 1] = v.y; P[r][c0 + e + 2] = v.z; P[r][c0 + e + 3] = v.w;
+            Q[r][c0 + e] = w.x; Q[r][c0 + e + 1] = w.y; Q[r][c0 + e + 2] = w.z; Q[r][c0 + e + 3] = w.w;
+        } else {
+            Q[r][c0 + e] = v.x; Q[r][c0 + e + 1] = v.y; Q[r][c0 + e + 2] = v.z; Q[r][c0 + e + 3] = v.w;
+            P[r][c0 + e] = w.x; P[r][c0 + e + 1] = w.y; P[r][c0 + e + 2] = w.z; P[r][c0 + e + 3] = w.w;
+        }
+    }
+    __syncthreads();
+    nf4 acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = nf4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < NPB; ks += 4) {
+        const float pa = P[16 * wave + (lane & 15)][ks + (lane >> 4)];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(pa, Q[ks + (lane >> 4)][16 * q + (lane & 15)], acc[q], 0, 0,
+                                                          0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int col = 16 * q + (lane & 15);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int row = 16 * wave + 4 * (lane >> 4) + e;
+            if (row < tr && col < tc) tile[row * ld + col] = acc[q][e];
+        }
+    }
+}
+
+// U^T A V over the orbit blocks: thread (p, q) reads the 4 x 4 block of A at rows
+// p + i m, columns q + j m (the identity beyond n), forms U_p^T B V_q in FP64 and
+// stores it rounded to FP32.  W holds the 4 x 4 butterfly of every orbit,
+// row-major: U's at W[16 p], V's at W[16 (m + q)].
+template <typename Tin>
+__global__ __launch_bounds__(256) void k_rbt_matrix(const Tin *__restrict__ A, int64_t n, int64_t lda,
+                                                   float *__restrict__ Ar, int64_t m, int64_t ldr,
+                                                   const double *__restrict__ W)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t p = blockIdx.y;
+    if (q >= m) return;
+    double B[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t gr = p + i * m;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t gc = q + j * m;
+            B[i][j] = (gr < n && gc < n) ? (double)A[gr * lda + gc] : (gr == gc ? 1.0 : 0.0);
+        }
+    }
+    const double *U = W + 16 * p, *V = W + 16 * (m + q);
+    double T[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) T[i][j] = B[i][0] * V[j] + B[i][1] * V[4 + j] + B[i][2] * V[8 + j] + B[i][3] * V[12 + j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float *dst = Ar + (p + i * m) * ldr + q;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            dst[j * m] = (float)(U[i] * T[0][j] + U[4 + i] * T[1][j] + U[8 + i] * T[2][j] + U[12 + i] * T[3][j]);
+    }
+}
+
+// t (n4 x k) = U^T b (n x k, zero-padded), or x (n x k) = V y (y n4 x k, the pad dropped)
+__global__ void k_rbt_vec(const double *__restrict__ in, int64_t ldin, double *__restrict__ out, int64_t ldout,
+                          int64_t n, int64_t m, int64_t k, const double *__restrict__ W, int transpose_u)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t col = blockIdx.y;
+    if (p >= m || col >= k) return;
+    const double *Wp = W + 16 * (transpose_u ? p : m + p);
+    const double *src = in + col * ldin;
+    double *dst = out + col * ldout;
+    double v[4], o[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t g = p + i * m;
+        v[i] = (transpose_u && g >= n) ? 0.0 : src[g];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        o[i] = transpose_u ? Wp[i] * v[0] + Wp[4 + i] * v[1] + Wp[8 + i] * v[2] + Wp[12 + i] * v[3]
+                           : Wp[4 * i] * v[0] + Wp[4 * i + 1] * v[1] + Wp[4 * i + 2] * v[2] + Wp[4 * i + 3] * v[3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t g = p + i * m;
+        if (transpose_u || g < n) dst[g] = o[i];
+    }
+}
+
+// splitmix64: deterministic random weights
+uint64_t mix64(uint64_t &s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// One depth-2 recursive butterfly of size 4m as its m orbit blocks (row-major
+// 4 x 4, acting on (x_p, x_p+m, x_p+2m, x_p+3m)): the outer butterfly pairs
+// p with p + 2m, the two inner ones pair p with p + m in each half; every
+// butterfly is (1/sqrt 2) [R S; R -S] with random diagonals exp(r / 10),
+// r uniform in [-1/2, 1/2].
+void butterfly_blocks(int64_t m, uint64_t seed, double *out)
+{
+    uint64_t s = seed;
+    auto rnd = [&]() { return std::exp((((double)(mix64(s) >> 11) * 0x1.0p-53) - 0.5) / 10.0); };
+    const double h = std::sqrt(0.5);
+    for (int64_t p = 0; p < m; ++p) {
+        const double R0 = rnd(), S0 = rnd(), R1 = rnd(), S1 = rnd();   // outer: positions p, p + m
+        const double Ra = rnd(), Sa = rnd(), Rb = rnd(), Sb = rnd();   // inner: first half, second half
+        // z = outer x: z_a = h (R0 x_a + S0 x_c), z_c = h (R0 x_a - S0 x_c),
+        //              z_b = h (R1 x_b + S1 x_d), z_d = h (R1 x_b - S1 x_d)
+        const double Z[4][4] = {{h * R0, 0, h * S0, 0}, {0, h * R1, 0, h * S1},
+                                {h * R0, 0, -h * S0, 0}, {0, h * R1, 0, -h * S1}};
+        // y = inner z: y_a = h (Ra z_a + Sa z_b), y_b = h (Ra z_a - Sa z_b),
+        //              y_c = h (Rb z_c + Sb z_d), y_d = h (Rb z_c - Sb z_d)
+        const double Y[4][4] = {{h * Ra, h * Sa, 0, 0}, {h * Ra, -h * Sa, 0, 0},
+                                {0, 0, h * Rb, h * Sb}, {0, 0, h * Rb, -h * Sb}};
+        double *o = out + 16 * p;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) {
+                double acc = 0;
+                for (int k = 0; k < 4; ++k) acc += Y[i][k] * Z[k][j];
+                o[4 * i + j] = acc;
+            }
+    }
+}
+
+}  // namespace
+
+int64_t fp_rbt_size(int64_t n) { return (n + 3) / 4 * 4; }
+
+size_t fp_rbt_weights_bytes(int64_t n) { return (size_t)32 * (fp_rbt_size(n) / 4) * sizeof(double); }
+
+// U's orbit blocks then V's, for a matrix of order n (padded to n4)
+int fp_rbt_weights(fp_ctx *ctx, int64_t n, uint64_t seed, double *d_W)
+{
+    const int64_t m = fp_rbt_size(n) / 4;
+    std::vector<double> W((size_t)32 * m);
+    butterfly_blocks(m, seed, W.data());
+    butterfly_blocks(m, seed ^ 0x5DEECE66Dull, W.data() + 16 * m);
+    FP_HIP(ctx, hipMemcpyAsync(d_W, W.data(), W.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FP_OK;
+}
+
+template <typename Tin>
+int fp_rbt_matrix(fp_ctx *ctx, const Tin *A, int64_t n, int64_t lda, float *Ar, int64_t ldr, const double *d_W)
+{
+    const int64_t m = fp_rbt_size(n) / 4;
+    if (m == 0) return FP_OK;
+    ProfScope ps(ctx, "rbt");
+    hipLaunchKernelGGL(k_rbt_matrix<Tin>, dim3((unsigned)((m + 255) / 256), (unsigned)m), dim3(256), 0, ctx->stream,
+                       A, n, lda, Ar, m, ldr, d_W);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+template int fp_rbt_matrix<double>(fp_ctx *, const double *, int64_t, int64_t, float *, int64_t, const double *);
+template int fp_rbt_matrix<float>(fp_ctx *, const float *, int64_t, int64_t, float *, int64_t, const double *);
+
+// transpose_u: out (n4 x k, ldout) = U^T in (n x k, ldin); else out (n x k) = V in (n4 x k)
+int fp_rbt_vec(fp_ctx *ctx, const double *in, int64_t ldin, double *out, int64_t ldout, int64_t n, int64_t k,
+               const double *d_W, int transpose_u)
+{
+    const int64_t m = fp_rbt_size(n) / 4;
+    if (m == 0 || k == 0) return FP_OK;
+    hipLaunchKernelGGL(k_rbt_vec, dim3((unsigned)((m + 255) / 256), (unsigned)k), dim3(256), 0, ctx->stream, in, ldin,
+                       out, ldout, n, m, k, d_W, transpose_u);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+size_t fp_getrf_nopiv_work_bytes() { return (size_t)2 * NPB * NPB * sizeof(float); }
+
+// LU without pivoting of the n x n FP32 matrix A (row-major, ld a multiple of 4);
+// info_dev receives 1 + the first row of the diagonal block whose pivot was zero
+// or not finite (0 if none).
+int fp_getrf_nopiv_f32(fp_ctx *ctx, float *A, int64_t n, int64_t ld, int *info_dev, void *work)
+{
+    if (n <= 0) return FP_OK;
+    if (ld % 4) return fp_fail(ctx, FP_ERR_ARG, "internal: no-pivot LU needs a leading dimension divisible by 4");
+    float *Linv = (float *)work, *Uinv = Linv + NPB * NPB;
+    hipLaunchKernelGGL(k_fill_info, dim3(1), dim3(64), 0, ctx->stream, info_dev, 0);
+    for (int64_t k0 = 0; k0 < n; k0 += NPB) {
+        {
+            ProfScope ps(ctx, "lu_nopiv_diag");
+            hipLaunchKernelGGL(k_nopiv_diag, dim3(1), dim3(128), 0, ctx->stream, A, ld, n, k0, Linv, Uinv, info_dev);
+        }
+        const int64_t rest = n - k0 - NPB;
+        if (rest <= 0) break;
+        const int nrt = (int)((rest + NPB - 1) / NPB);
+        {
+            ProfScope ps(ctx, "lu_nopiv_panel");
+            hipLaunchKernelGGL(k_nopiv_panel, dim3((unsigned)(2 * nrt)), dim3(256), 0, ctx->stream, A, ld, n, k0,
+                               (const float *)Linv, (const float *)Uinv, nrt);
+        }
+        FP_HIP(ctx, hipGetLastError());
+        int st = fp_gemm_sub<float>(ctx, rest, rest, NPB, A + (k0 + NPB) * ld + k0, ld, A + k0 * ld + k0 + NPB, ld,
+                                    A + (k0 + NPB) * ld + k0 + NPB, ld);
+        if (st) return st;
+    }
+    return FP_OK;
+}

@@ -220,7 +220,122 @@ __global__ __launch_bounds__(256) void k_ugradmu_combine(const int *__restrict__
     for (int d = 0; d < 3; ++d) out[3 * j + d] += src[3 * j + d];
 }
 
+// Element forces (calcfield_F!, src/FLOWPanel_postprocess.jl:1199-1307):
+// F_i = -Cp_i q A_i s n_i with s the sign of CPoffset (1 if 0: the normals
+// flipped by CPoffset, calc_normals(body; flipbyCPoffset=true)), point-major
+// 3 x N; and per workgroup the partial sums of F and of (X_i - Xac) x F_i
+// (calcfield_Ftot!, calcfield_Mtot!, :1543-1784), X the control points (geo)
+// or the caller's points (SoA).
+__global__ __launch_bounds__(256) void k_loads(const double *__restrict__ geo, int64_t N, const double *__restrict__ cp,
+                                               double q, double s, const double *__restrict__ X, double xa, double ya,
+                                               double za, double *__restrict__ F, double *__restrict__ part)
+{
+    __shared__ double red[4][6];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    if (i < N) {
+        const double c = -cp[i] * q * geo[6 * N + i] * s;
+        v[0] = c * geo[i];
+        v[1] = c * geo[N + i];
+        v[2] = c * geo[2 * N + i];
+        if (F) { F[3 * i] = v[0]; F[3 * i + 1] = v[1]; F[3 * i + 2] = v[2]; }
+        const double x = (X ? X[i] : geo[3 * N + i]) - xa, y = (X ? X[N + i] : geo[4 * N + i]) - ya,
+                     z = (X ? X[2 * N + i] : geo[5 * N + i]) - za;
+        v[3] = y * v[2] - z * v[1];
+        v[4] = z * v[0] - x * v[2];
+        v[5] = x * v[1] - y * v[0];
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        double a = v[k];
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+        if (lane == 0) red[wave][k] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6)
+        part[6 * (int64_t)blockIdx.x + threadIdx.x] =
+            red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+// out6 = sum of the nblk partial 6-vectors (one workgroup)
+__global__ __launch_bounds__(256) void k_loads_reduce(const double *__restrict__ part, int64_t nblk,
+                                                      double *__restrict__ out6)
+{
+    __shared__ double red[4][6];
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t b = threadIdx.x; b < nblk; b += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v[k] += part[6 * b + k];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        double a = v[k];
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+        if (lane == 0) red[wave][k] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) out6[threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] +
+                                             red[3][threadIdx.x];
+}
+
+// Sectional loads of a structured grid (calcfield_sectionalforce!, :1411-1542):
+// station j sums F over the chordwise cells i + nchord j and averages the
+// span coordinate spandir . X of their points; one wave per station.
+__global__ __launch_bounds__(64) void k_sectional(const double *__restrict__ F, const double *__restrict__ X, int64_t N,
+                                                  int64_t nchord, int64_t nspan, double sx, double sy, double sz,
+                                                  double *__restrict__ outF, double *__restrict__ pos)
+{
+    const int64_t j = blockIdx.x;
+    if (j >= nspan) return;
+    double a[4] = {0, 0, 0, 0};
+    for (int64_t i = threadIdx.x; i < nchord; i += 64) {
+        const int64_t c = i + nchord * j;
+        a[0] += F[3 * c];
+        a[1] += F[3 * c + 1];
+        a[2] += F[3 * c + 2];
+        a[3] += sx * X[c] + sy * X[N + c] + sz * X[2 * N + c];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        for (int o = 32; o > 0; o >>= 1) a[k] += __shfl_xor(a[k], o);
+    if (threadIdx.x == 0) {
+        outF[3 * j] = a[0];
+        outF[3 * j + 1] = a[1];
+        outF[3 * j + 2] = a[2];
+        pos[j] = a[3] / (double)nchord;
+    }
+}
+
 }  // namespace
+
+int fp_launch_loads(fp_ctx *ctx, const fp_body *b, const double *cp, double q, const double *X, const double *Xac,
+                    double *F, double *part, double *out6)
+{
+    const int64_t N = b->N;
+    if (N == 0) return FP_OK;
+    const double s = b->cp_offset < 0 ? -1.0 : 1.0;
+    const int64_t nblk = (N + 255) / 256;
+    ProfScope ps(ctx, "loads");
+    hipLaunchKernelGGL(k_loads, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, b->d_geo, N, cp, q, s, X, Xac[0],
+                       Xac[1], Xac[2], F, part);
+    hipLaunchKernelGGL(k_loads_reduce, dim3(1), dim3(256), 0, ctx->stream, (const double *)part, nblk, out6);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+size_t fp_loads_partial_bytes(int64_t N) { return (size_t)6 * ((N + 255) / 256 + 1) * sizeof(double); }
+
+int fp_launch_sectional(fp_ctx *ctx, const double *F, const double *X, int64_t N, int64_t nchord, int64_t nspan,
+                        const double *spandir, double *outF, double *pos)
+{
+    if (nspan <= 0 || nchord <= 0) return FP_OK;
+    ProfScope ps(ctx, "loads");
+    hipLaunchKernelGGL(k_sectional, dim3((unsigned)nspan), dim3(64), 0, ctx->stream, F, X, N, nchord, nspan, spandir[0],
+                       spandir[1], spandir[2], outF, pos);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
 
 int fp_launch_ugradmu_node(fp_ctx *ctx, const fp_body *b, const int64_t *d_nptr, const int *d_ncell, int64_t nn,
                            const int *d_cn, double *d_nodal, double *out, const TeSides *te)

@@ -342,6 +342,60 @@ def calc_shedding(grid, trailingedge, tolerance=None, periodic=False):
     return np.array(cols, dtype=np.int64).T.reshape(6, -1)
 
 
+def transform(grid, rotation=None, translation=(0.0, 0.0, 0.0), scaling=1.0):
+    """A copy of the grid with its nodes translated, then rotated (3 x 3), then scaled
+    (the Meshes.Translate |> Rotate |> Scale chain of examples/blendedwingbody.jl:78)."""
+    X = np.asarray(grid.nodes, dtype=np.float64) + np.asarray(translation, dtype=np.float64).reshape(3, 1)
+    if rotation is not None:
+        X = np.asarray(rotation, dtype=np.float64) @ X
+    return Grid(X * scaling, (grid.cell_ptr, grid.cell_idx), dims=grid.dims)
+
+
+def mirror(grid, coordinate, tol=1e-12):
+    """The grid joined with its mirror image across the plane ``x[coordinate] = 0`` (0-based coordinate),
+    the mirrored cells' vertex order reversed so that their normals keep pointing out, and the nodes that
+    lie on the plane shared by both halves (the role of gt.mirror in examples/blendedwingbody.jl:82)."""
+    X = np.asarray(grid.nodes, dtype=np.float64)
+    nn = X.shape[1]
+    Xm = X.copy()
+    Xm[coordinate] *= -1
+    on = np.abs(X[coordinate]) <= tol * max(1.0, np.abs(X).max())
+    # mirrored node i -> itself when on the plane, else a new node
+    new_index = np.where(on, np.arange(nn), nn + np.cumsum(~on) - 1)
+    nodes = np.hstack([X, Xm[:, ~on]])
+    cells = [grid.cell(c) for c in range(grid.ncells)]
+    cells += [new_index[np.asarray(c)][::-1] for c in cells]
+    return Grid(nodes, cells)
+
+
+def spatial_order(grid, shedding=None, bits=10):
+    """Cells reordered along a Morton (Z-order) curve of their centroids, so that every run of consecutive
+    indices is a compact patch of the surface: the block-Jacobi blocks and coarse chunks of the Krylov
+    solvers, which take consecutive index ranges, then group neighbouring panels on unstructured meshes.
+    Returns ``(grid, shedding, perm)``: new cell k is old cell ``perm[k]`` (strengths map back with
+    ``old[perm] = new``)."""
+    n = grid.ncells
+    counts = np.diff(grid.cell_ptr)
+    cent = np.add.reduceat(grid.nodes[:, grid.cell_idx], grid.cell_ptr[:-1], axis=1) / counts
+    lo, hi = cent.min(axis=1, keepdims=True), cent.max(axis=1, keepdims=True)
+    q = np.floor((cent - lo) / np.maximum(hi - lo, 1e-300) * ((1 << bits) - 1)).astype(np.uint64)
+    key = np.zeros(n, dtype=np.uint64)
+    for b in range(bits):
+        for d in range(3):
+            key |= ((q[d] >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + d)
+    perm = np.argsort(key, kind="stable")
+    inv = np.empty(n, dtype=np.int64)
+    inv[perm] = np.arange(n)
+    cells = [grid.cell(c) for c in perm]
+    g = Grid(grid.nodes, cells)
+    if shedding is None:
+        return g, None, perm
+    sh = np.array(shedding, dtype=np.int64)
+    sh[0] = inv[sh[0]]
+    sh[3] = np.where(sh[3] >= 0, inv[np.maximum(sh[3], 0)], -1)
+    return g, sh, perm
+
+
 def direction(alpha_deg, beta_deg=0.0):
     """Freestream unit vector (cos a cos b, -sin b, sin a cos b)
     (src/FLOWPanel_utils.jl:34-37)."""

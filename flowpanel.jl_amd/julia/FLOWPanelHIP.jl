@@ -304,6 +304,38 @@ function calcfield_Ugradmu_node!(out::AbstractMatrix{Float64}, body; sharpTE=fal
     return nodal
 end
 
+"""
+    calcfield_F!(out, body, Cps, Uinf, rho; Xac, correct_kuttacondition) -> (Ftot, Mtot)
+
+Element forces (3 x N; the trailing-edge cells overwritten by the Kutta override, the others added to),
+with the integrated force and the moment about `Xac` (src/FLOWPanel_postprocess.jl:1199-1307, 1543-1784).
+"""
+function calcfield_F!(out::AbstractMatrix{Float64}, body, Cps::AbstractVector{Float64}, Uinf::Real, rho::Real;
+                      Xac=zeros(3), correct_kuttacondition=true, kutta_mode=KUTTA_REFERENCE)
+    hs = handles(body)
+    Ftot = zeros(3)
+    Mtot = zeros(3)
+    check(ccall((:fp_calcfield_F, lib), Cint,
+                (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Cint, Ptr{Float64}, Cdouble, Cdouble, Cint, Ptr{Float64}, Ptr{Float64},
+                 Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                ctx(), hs, length(hs), Cps, Uinf, rho, correct_kuttacondition ? kutta_mode : KUTTA_NONE,
+                Vector{Float64}(Xac), C_NULL, out, Ftot, Mtot))
+    return Ftot, Mtot
+end
+
+"""calcfield_sectionalforce(body, Fs, nchord, nspan; spandirection): force per unit span and station positions
+of a structured grid (src/FLOWPanel_postprocess.jl:1411-1542)."""
+function calcfield_sectionalforce(body, Fs::AbstractMatrix{Float64}, nchord::Integer, nspan::Integer;
+                                  spandirection=[0.0, 1.0, 0.0])
+    outf = zeros(3, nspan)
+    pos = zeros(nspan)
+    check(ccall((:fp_calcfield_sectionalforce, lib), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Int64, Ptr{Float64}, Ptr{Float64},
+                 Ptr{Float64}), ctx(), handle(body), Fs, C_NULL, nchord, nspan, Vector{Float64}(spandirection),
+                outf, pos))
+    return outf, pos
+end
+
 # ---- the solver-function protocol on caller matrices ---------------------------
 function solve_matrix!(y::AbstractVecOrMat{Float64}, A::AbstractMatrix{Float64}, b::AbstractVecOrMat{Float64},
                        solver::Integer; opts...)

@@ -9,6 +9,9 @@ here they become a ``Grid`` (nodes 3 x nn, CSR cells, 0-based) directly.
   VTK types triangle (5), quad (9) and polygon (7).
 * ``read_msh``  -- Gmsh 4.1 ASCII: $Nodes and $Elements blocks, 3-node
   triangles (type 2) and 4-node quads (type 3) of the 2-D entities.
+* ``read_msh_nodes`` -- every node of a Gmsh 4.1 ASCII file, 3 x n (a
+  trailing-edge line mesh read as points, gt.vertices2nodes of the reference's
+  examples/blendedwingbody.jl:84).
 """
 import re
 import zlib
@@ -68,6 +71,30 @@ def read_vtu(path):
     return Grid(pts, cells)
 
 
+def _msh_nodes(lines, k):
+    nblocks = int(lines[k + 1].split()[0])
+    k += 2
+    tags, xyz = [], []
+    for _ in range(nblocks):
+        _, _, _, nn = (int(v) for v in lines[k].split())
+        k += 1
+        tags += [int(lines[k + i]) for i in range(nn)]
+        xyz += [[float(v) for v in lines[k + nn + i].split()[:3]] for i in range(nn)]
+        k += 2 * nn
+    return tags, xyz, k
+
+
+def read_msh_nodes(path):
+    """All nodes of a Gmsh 4.1 ASCII file, 3 x n, in tag order."""
+    lines = open(path).read().split("\n")
+    for k, ln in enumerate(lines):
+        if ln.strip() == "$Nodes":
+            tags, xyz, _ = _msh_nodes(lines, k)
+            order = np.argsort(tags)
+            return np.asfortranarray(np.asarray(xyz, dtype=np.float64)[order].T)
+    raise ValueError("no $Nodes block")
+
+
 def read_msh(path):
     """2-D elements (triangles, quads) of a Gmsh 4.1 ASCII file as a ``Grid``."""
     lines = open(path).read().split("\n")
@@ -81,16 +108,9 @@ def read_msh(path):
                 raise ValueError("only Gmsh 4.x ASCII meshes are supported")
             k += 2
         elif ln == "$Nodes":
-            nblocks = int(lines[k + 1].split()[0])
-            k += 2
-            for _ in range(nblocks):
-                _, _, _, nn = (int(v) for v in lines[k].split())
-                k += 1
-                t = [int(lines[k + i]) for i in range(nn)]
-                c = [[float(v) for v in lines[k + nn + i].split()[:3]] for i in range(nn)]
-                tags += t
-                xyz += c
-                k += 2 * nn
+            t, c, k = _msh_nodes(lines, k)
+            tags += t
+            xyz += c
         elif ln == "$Elements":
             nblocks = int(lines[k + 1].split()[0])
             k += 2

@@ -232,33 +232,148 @@ def calcfield_Cp(body, Uref, Us=None, correct_kuttacondition=True, clip=None, fi
     return out
 
 
+def _kutta_mode(b, correct):
+    if not correct or not b.nsheddings or b.element.code != nat.FP_ELEM_VORTEXRING:
+        return nat.FP_KUTTA_NONE
+    return {"reference": nat.FP_KUTTA_REFERENCE, "pair": nat.FP_KUTTA_PAIR, "none": nat.FP_KUTTA_NONE}[b.kutta]
+
+
+def loads(body, Cps, Uinf, rho, Xac=(0.0, 0.0, 0.0), X=None, correct_kuttacondition=True):
+    """Element forces and integrated loads on the device (fp_calcfield_F): returns (F 3 x N, Ftot 3, Mtot 3),
+    Mtot about ``Xac`` with every force applied at ``X`` (3 x N, default the control points)."""
+    Cps = np.ascontiguousarray(Cps, dtype=np.float64)
+    if Cps.shape != (body.ncells,):
+        raise ValueError("Cps must have length %d" % body.ncells)
+    Xh = None if X is None else nat.f64(X)
+    if Xh is not None and Xh.shape != (3, body.ncells):
+        raise ValueError("X must be 3 x %d" % body.ncells)
+    F = np.zeros((3, body.ncells), order="F")
+    Ftot, Mtot = np.zeros(3), np.zeros(3)
+    xac = np.ascontiguousarray(Xac, dtype=np.float64)
+    col = 0
+    for b in body.bodies:
+        n = b.ncells
+        Fb = np.zeros((3, n), order="F")
+        b.ctx.check(nat.lib().fp_calcfield_F(b.ctx.handle, nat.handles([b]), 1, nat.dptr(Cps[col:col + n]),
+                                             float(Uinf), float(rho), _kutta_mode(b, correct_kuttacondition),
+                                             nat.dptr(xac), nat.dptr(None if Xh is None else nat.f64(Xh[:, col:col + n])),
+                                             nat.dptr(Fb), nat.dptr(Ftot), nat.dptr(Mtot)))
+        F[:, col:col + n] = Fb
+        col += n
+    return F, Ftot, Mtot
+
+
 def calcfield_F(body, Uinf, rho, Cps=None, areas=None, normals=None, correct_kuttacondition=True, fieldname="F",
-                addfield=True):
-    """F_i = -Cp_i q A_i n_i with the Kutta override at trailing-edge panels."""
+                addfield=True, Cp_fieldname="Cp"):
+    """calcfield_F (src/FLOWPanel_postprocess.jl:1199-1392): F_i = -Cp_i q A_i n_i, 3 x N, with the Kutta
+    override of the trailing-edge cells.  Without ``areas``/``normals`` it runs on the device with the
+    body's areas and its normals flipped by the sign of CPoffset (calc_normals(body; flipbyCPoffset=true));
+    with them, on the host as the reference's calcfield_F!(out, body, areas, normals, Cps, Uinf, rho)."""
     if Cps is None:
-        Cps = body.get_field("Cp")
-    if areas is None:
-        areas = calc_areas(body)
-    if normals is None:
-        normals = calc_normals(body)
-    q = 0.5 * rho * Uinf ** 2
-    F = -(np.asarray(Cps) * q * np.asarray(areas))[None, :] * np.asarray(normals)
-    if correct_kuttacondition:
-        col = 0
-        for b in body.bodies:
-            for e in range(b.nsheddings):
-                pi, pj = b.shedding[0, e] + col, b.shedding[3, e]
-                if b.kutta == "reference":
-                    ids = [pi, pi + 1] + ([pj + col, pj + col - 1] if pj != -1 else [])
-                else:
-                    ids = [pi] + ([pj + col] if pj != -1 else [])
-                ave = np.mean([Cps[i] for i in ids])
-                for i in ids:
-                    F[:, i] = -ave * q * areas[i] * normals[:, i]
-            col += b.ncells
+        Cps = body.get_field(Cp_fieldname)
+    Cps = np.asarray(Cps, dtype=np.float64)
+    if areas is None and normals is None:
+        F, Ftot, _ = loads(body, Cps, Uinf, rho, correct_kuttacondition=correct_kuttacondition)
+    else:
+        if areas is None:
+            areas = calc_areas(body)
+        if normals is None:
+            normals = calc_normals(body)
+        q = 0.5 * rho * Uinf ** 2
+        F = -(Cps * q * np.asarray(areas))[None, :] * np.asarray(normals)
+        if correct_kuttacondition:
+            col = 0
+            for b in body.bodies:
+                for e in range(b.nsheddings):
+                    pi, pj = b.shedding[0, e] + col, b.shedding[3, e]
+                    if b.kutta == "reference":
+                        ids = [pi, pi + 1] + ([pj + col, pj + col - 1] if pj != -1 else [])
+                    else:
+                        ids = [pi] + ([pj + col] if pj != -1 else [])
+                    if len(ids) < 2:
+                        continue
+                    ave = np.mean([Cps[i] for i in ids])
+                    for i in ids:
+                        F[:, i] = -ave * q * areas[i] * normals[:, i]
+                col += b.ncells
+        Ftot = F.sum(axis=1)
     if addfield:
-        body.add_field(fieldname, F)
+        _store(body, fieldname, F)
+        body.add_field("Ftot", Ftot)
     return F
+
+
+def calcfield_Ftot(body, Fs=None, F_fieldname="F", fieldname="Ftot", addfield=True):
+    """Integrated force (src/FLOWPanel_postprocess.jl:1543-1590), 3."""
+    F = np.asarray(body.get_field(F_fieldname) if Fs is None else Fs)
+    out = F.sum(axis=1)
+    if addfield:
+        body.add_field(fieldname, out)
+    return out
+
+
+def _moment(body, Xac, F_fieldname, offset, characteristiclength, Fs):
+    F = np.asarray(body.get_field(F_fieldname) if Fs is None else Fs)
+    X = _controlpoints(body, offset, characteristiclength)
+    return np.cross(X - np.asarray(Xac, dtype=np.float64).reshape(3, 1), F, axis=0).sum(axis=1)
+
+
+def calcfield_Mtot(body, Xac, F_fieldname="F", offset=None, characteristiclength=None, Fs=None, fieldname="Mtot",
+                   addfield=True):
+    """Integrated moment about Xac (src/FLOWPanel_postprocess.jl:1715-1784): sum of (X_i - Xac) x F_i over
+    the control points X (moved by ``offset``/``characteristiclength`` as calc_controlpoints does)."""
+    out = _moment(body, Xac, F_fieldname, offset, characteristiclength, Fs)
+    if addfield:
+        body.add_field(fieldname, out)
+    return out
+
+
+def calcfield_lmn(body, Xac, lhat, mhat, nhat=None, F_fieldname="F", offset=None, characteristiclength=None,
+                  Fs=None, addfield=True):
+    """Rolling, pitching and yawing moments (src/FLOWPanel_postprocess.jl:1804-1904): 3 x 3, columns the
+    projections of Mtot on lhat, mhat, nhat (default lhat x mhat)."""
+    lhat, mhat = np.asarray(lhat, float), np.asarray(mhat, float)
+    nhat = np.cross(lhat, mhat) if nhat is None else np.asarray(nhat, float)
+    for v, nm in ((lhat, "lhat"), (mhat, "mhat"), (nhat, "nhat")):
+        if abs(np.linalg.norm(v) - 1) > 2 * np.finfo(float).eps:
+            raise ValueError("%s=%s is not a unitary vector" % (nm, v))
+    M = _moment(body, Xac, F_fieldname, offset, characteristiclength, Fs)
+    out = np.stack([lhat * M.dot(lhat), mhat * M.dot(mhat), nhat * M.dot(nhat)], axis=1)
+    if addfield:
+        body.add_field("Mroll", out[:, 0])
+        body.add_field("Mpitch", out[:, 1])
+        body.add_field("Myaw", out[:, 2])
+    return out
+
+
+def calcfield_sectionalforce(body, F_fieldname="F", Fs=None, dimspan=2, dimchord=1, spandirection=(0.0, 1.0, 0.0),
+                             offset=None, characteristiclength=None, fieldname="sectionalforce", addfield=True):
+    """Sectional force per unit span (src/FLOWPanel_postprocess.jl:1411-1542) of a structured grid, on the
+    device: returns (3 x nspan, nspan positions).  The grid's ``dims`` = (cells along dimension 1, along 2)."""
+    if len(body.bodies) != 1:
+        raise ValueError("sectional loads take a single structured body")
+    b = body.bodies[0]
+    dims = getattr(b.grid, "dims", None)
+    if dims is None:
+        raise ValueError("sectional loads need a structured grid (grid.dims)")
+    n1, n2 = dims
+    F = nat.f64(body.get_field(F_fieldname) if Fs is None else Fs)
+    X = _controlpoints(body, offset, characteristiclength)
+    if (dimchord, dimspan) == (2, 1):      # chordwise index slow: reorder to chord-fastest
+        order = np.arange(n1 * n2).reshape(n2, n1).T.ravel()
+        F, X = nat.f64(F[:, order]), nat.f64(X[:, order])
+        n1, n2 = n2, n1
+    elif (dimchord, dimspan) != (1, 2):
+        raise ValueError("dimchord/dimspan must be (1, 2) or (2, 1)")
+    outF = np.zeros((3, n2), order="F")
+    pos = np.zeros(n2)
+    sd = np.ascontiguousarray(spandirection, dtype=np.float64)
+    b.ctx.check(nat.lib().fp_calcfield_sectionalforce(b.ctx.handle, b.handle, nat.dptr(F), nat.dptr(nat.f64(X)),
+                                                      n1, n2, nat.dptr(sd), nat.dptr(outF), nat.dptr(pos)))
+    if addfield:
+        body.add_field(fieldname, outF)
+        body.add_field(fieldname + "-pos", pos)
+    return outF, pos
 
 
 def calcfield_LDS(body, Lhat, Dhat, Shat=None, F=None):

@@ -241,6 +241,24 @@ int fp_calcfield_U(fp_ctx *ctx, fp_body *const *bodies, int nbodies, double *out
 int fp_calcfield_cp(fp_ctx *ctx, const double *Us, int64_t n, double uref, const int64_t *shedding,
                     int64_t nsheddings, int index_base, int kutta_mode, double *out);
 
+/* calcfield_F!(out, body, areas, normals, Cps, Uinf, rho) (src/FLOWPanel_postprocess.jl:1199-1307) with the
+ * integrated loads calcfield_Ftot! (:1543-1590) and calcfield_Mtot! (:1715-1784), on the device:
+ * F_i = -Cp_i (rho Uinf^2 / 2) A_i n_i, the normals flipped by the sign of CPoffset (the reference's
+ * calc_normals(body; flipbyCPoffset=true)).  Cps: N (all bodies one after the other).  kutta_mode: the
+ * trailing-edge cells of each vortex-ring body take the average Cp of their group (FP_KUTTA_*), and their F
+ * overwrites what `F` held (the other cells add to it), as the reference does.  Outputs may be NULL:
+ * F (3 x N), Ftot (3, +=), Mtot (3, +=) the moment about Xac (NULL: the origin) with the force of each cell
+ * applied at X (3 x N; NULL: the bodies' control points). */
+int fp_calcfield_F(fp_ctx *ctx, fp_body *const *bodies, int nbodies, const double *Cps, double Uinf, double rho,
+                   int kutta_mode, const double *Xac, const double *X, double *F, double *Ftot, double *Mtot);
+/* calcfield_sectionalforce!(outf, outpos, body, controlpoints, Fs; spandirection) (src/FLOWPanel_postprocess.jl:
+ * 1411-1542) for a structured grid of nchord x nspan cells (cell i + nchord j: chordwise index fastest): the
+ * force F (3 x N) summed over each spanwise station j, divided by the station spacing (central differences,
+ * one-sided at the ends), and the station position, the mean of spandir . X over its cells (X: 3 x N, NULL
+ * the control points).  outF: 3 x nspan, outpos: nspan. */
+int fp_calcfield_sectionalforce(fp_ctx *ctx, const fp_body *body, const double *F, const double *X, int64_t nchord,
+                                int64_t nspan, const double *spandir, double *outF, double *outpos);
+
 /* calcfield_Ugradmu_cell!(out, body) (src/FLOWPanel_postprocess.jl:139-497):
  * out (3 x N, += ) -= 1/2 grad(mu), the surface velocity due to the gradient of
  * the doublet strength, by the cell-centred Green-Gauss scheme with inverse-

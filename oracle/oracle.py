@@ -333,6 +333,34 @@ def forces(areas, normals, cps_, uinf, rho, shedding=None, kutta=True):
     return out
 
 
+def mtot(X, F, Xac):
+    """Integrated moment about Xac of forces F applied at X (src/FLOWPanel_postprocess.jl:1715-1745)."""
+    X, F = np.asarray(X, float), np.asarray(F, float)
+    out = np.zeros(3)
+    for i in range(F.shape[1]):
+        r = X[:, i] - np.asarray(Xac, float)
+        out += [r[1] * F[2, i] - r[2] * F[1, i], r[2] * F[0, i] - r[0] * F[2, i], r[0] * F[1, i] - r[1] * F[0, i]]
+    return out
+
+
+def sectionalforce(F, X, nchord, nspan, spandir):
+    """Sectional force per unit span of a chord-fastest structured grid (src/FLOWPanel_postprocess.jl:1411-1475):
+    station sums of F, station position = mean of spandir . X, divided by the central-difference spacing
+    (one-sided at the two ends)."""
+    F, X = np.asarray(F, float), np.asarray(X, float)
+    outf = np.zeros((3, nspan))
+    pos = np.zeros(nspan)
+    for j in range(nspan):
+        cells = [i + nchord * j for i in range(nchord)]
+        for c in cells:
+            outf[:, j] += F[:, c]
+        pos[j] = np.mean([np.dot(spandir, X[:, c]) for c in cells])
+    for j in range(nspan):
+        d = pos[1] - pos[0] if j == 0 else (pos[j] - pos[j - 1] if j == nspan - 1 else (pos[j + 1] - pos[j - 1]) / 2)
+        outf[:, j] /= abs(d)
+    return outf, pos
+
+
 def lds(F, Lhat, Dhat, Shat=None):
     """Lift/drag/sideslip projections (src/FLOWPanel_postprocess.jl:1615-1652)."""
     Lhat, Dhat = np.asarray(Lhat, float), np.asarray(Dhat, float)

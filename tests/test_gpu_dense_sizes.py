@@ -124,3 +124,46 @@ def test_coarse_space_above_128_vectors(fp, coarse):
     it1 = fp.solvers.last_stats["iterations"]
     assert rel(y, np.linalg.solve(A, b)) < 1e-10
     assert it2 <= it1, (it2, it1)
+
+
+# ---- FP32 LU of the randomized matrix without pivoting (the LU32 refinement path) ----
+# The no-pivot LU works in 64-column steps on U^T A V, n padded to a multiple of 4:
+# sizes straddle the pad and the 64-row blocks.
+
+@pytest.mark.parametrize("n", [1, 3, 4, 5, 63, 64, 65, 127, 130, 700, 1031])
+def test_randomized_nopivot_lu32_refinement(fp, n):
+    rng = np.random.default_rng(4000 + n)
+    A = rng.standard_normal((n, n)) + 2.0 * np.sqrt(n) * np.eye(n)
+    b = rng.standard_normal(n)
+    x = np.zeros(n)
+    fp.solve_lu32_ir(x, A, b, tol=1e-12)
+    assert rel(x, np.linalg.solve(A, b)) < 1e-11, n
+    # an accurate FP32 factorisation: a handful of refinement steps
+    assert fp.last_stats["refinements"] <= 4, fp.last_stats
+
+
+def test_randomized_nopivot_lu32_needs_no_pivots(fp):
+    # a zero diagonal (plain LU without pivoting breaks down at the first step);
+    # the butterflies make every leading block nonsingular
+    rng = np.random.default_rng(7)
+    n = 300
+    P = np.eye(n)[rng.permutation(n)]
+    A = 5.0 * P + 0.1 * rng.standard_normal((n, n))
+    np.fill_diagonal(A, 0.0)
+    b = rng.standard_normal(n)
+    x = np.zeros(n)
+    fp.solve_lu32_ir(x, A, b, tol=1e-12)
+    assert rel(x, np.linalg.solve(A, b)) < 1e-10
+
+
+def test_lu32_pivoted_and_randomized_agree(fp, geo, monkeypatch):
+    g, sh = geo.loft_wing(n_chord=30, n_span=20, span=6.0, sweep_deg=25.0, taper=0.6)
+    body = fp.RigidWakeBody(g, sh)
+    V = fp.direction(4.2)
+    U = np.tile(V[:, None], (1, body.ncells))
+    D = np.tile(V[:, None], (1, body.nsheddings))
+    st_r = fp.solve(body, U, D, D, solver=fp.solve_lu32_ir, solver_optargs={"tol": 1e-12})
+    g_r = body.strength.copy()
+    monkeypatch.setenv("FLOWPANEL_LU32_PIVOTED", "1")
+    st_p = fp.solve(body, U, D, D, solver=fp.solve_lu32_ir, solver_optargs={"tol": 1e-12})
+    assert rel(body.strength, g_r) < 1e-10, (st_r, st_p)

@@ -96,3 +96,41 @@ def test_ugradmu_sharp_te_matches_oracle(fp, geo, orc, tri):
     outc = fp.calcfield_Ugradmu(body, sharpTE=True, force_cellTE=False)
     refc = orc.ugradmu_combined(ob, body.strength, sh, force_cell_te=False, anglecrit=30.0, sharp_te=mode)
     assert rel(outc, refc) < 1e-12
+
+
+@pytest.mark.parametrize("tri", [False, True])
+def test_loads_match_oracle(fp, geo, orc, tri):
+    """calcfield_F (normals flipped by the sign of CPoffset, Kutta override), Ftot, Mtot
+    (src/FLOWPanel_postprocess.jl:1199-1307, 1543-1784) and the sectional force (:1411-1542) on the
+    device against the oracle, 1e-12."""
+    g, sh = geo.loft_wing(airfoil="2412", n_chord=20, n_span=10, span=5.0, sweep_deg=20.0)
+    if tri:
+        g, sh = geo.triangulate(g, sh)
+    body = fp.RigidWakeBody(g, sh, CPoffset=-1e-14 if tri else 1e-14)
+    V = fp.direction(5.0)
+    fp.solve(body, np.tile(V[:, None], (1, body.ncells)), np.tile(V[:, None], (1, body.nsheddings)),
+             np.tile(V[:, None], (1, body.nsheddings)))
+    fp.calcfield_U(body, body)
+    Cp = fp.calcfield_Cp(body, 1.0)
+    normals, cps, areas = orc.geometry(g.nodes, g.cell_ptr, g.cell_idx, body.CPoffset, 0)
+    s = -1.0 if body.CPoffset < 0 else 1.0
+    Xac = np.array([0.25, 0.1, -0.05])
+    kutta = tri                        # the oracle's Kutta pattern is the triangulated one (pi, pi+1, pj, pj-1)
+    F, Ftot, Mtot = fp.loads(body, Cp, 1.0, 1.2, Xac=Xac, correct_kuttacondition=kutta)
+    Fo = orc.forces(areas, s * normals, Cp, 1.0, 1.2, sh, kutta=kutta)
+    assert rel(F, Fo) < 1e-12
+    assert rel(Ftot, Fo.sum(axis=1)) < 1e-12
+    assert rel(Mtot, orc.mtot(cps, Fo, Xac)) < 1e-12
+    # moments at caller points (the control points moved by an offset)
+    X = cps + 0.05 * normals
+    _, _, M2 = fp.loads(body, Cp, 1.0, 1.2, Xac=Xac, X=X, correct_kuttacondition=kutta)
+    assert rel(M2, orc.mtot(X, Fo, Xac)) < 1e-12
+    # the field-based mirrors agree with the device sums
+    fp.calcfield_F(body, 1.0, 1.2, correct_kuttacondition=kutta)
+    assert rel(fp.calcfield_Ftot(body), Ftot) < 1e-12
+    assert rel(fp.calcfield_Mtot(body, Xac), Mtot) < 1e-10
+    if not tri:
+        n1, n2 = g.dims
+        f, pos = fp.calcfield_sectionalforce(body)
+        fo, po = orc.sectionalforce(F, cps, n1, n2, [0.0, 1.0, 0.0])
+        assert rel(pos, po) < 1e-13 and rel(f, fo) < 1e-12

@@ -144,3 +144,29 @@ def test_panel_area_definition(orc):
     t1 = 0.5 * np.cross(nodes[:, 1] - nodes[:, 0], nodes[:, 2] - nodes[:, 0])
     t2 = 0.5 * np.cross(nodes[:, 2] - nodes[:, 0], nodes[:, 3] - nodes[:, 0])
     assert abs(np.linalg.norm(t1 + t2) - areas[0]) < 1e-15
+
+
+def test_moment_and_sectional_force_hand_cases(orc):
+    # one force at a known arm: M = r x F
+    X = np.array([[1.0], [2.0], [0.5]])
+    F = np.array([[0.0], [0.0], [3.0]])
+    assert np.allclose(orc.mtot(X, F, [1.0, 0.0, 0.0]), np.cross([0.0, 2.0, 0.5], [0.0, 0.0, 3.0]))
+    # a pure couple: equal and opposite forces give the same moment about any point
+    X = np.array([[0.0, 1.0], [0.0, 0.0], [0.0, 0.0]])
+    F = np.array([[0.0, 0.0], [1.0, -1.0], [0.0, 0.0]])
+    assert np.allclose(orc.mtot(X, F, [0, 0, 0]), orc.mtot(X, F, [3.0, -2.0, 7.0]))
+    assert np.allclose(orc.mtot(X, F, [0, 0, 0]), [0, 0, -1.0])
+    # uniform load w per unit span on stations 0.5 apart: every station reads w per unit span
+    nchord, nspan, w, dy = 4, 6, 2.5, 0.5
+    Xc = np.zeros((3, nchord * nspan))
+    Fc = np.zeros((3, nchord * nspan))
+    for j in range(nspan):
+        for i in range(nchord):
+            c = i + nchord * j
+            Xc[:, c] = [0.1 * i, dy * j, 0.0]
+            Fc[2, c] = w * dy / nchord
+    f, pos = orc.sectionalforce(Fc, Xc, nchord, nspan, [0.0, 1.0, 0.0])
+    assert np.allclose(pos, dy * np.arange(nspan))
+    assert np.allclose(f[2], w) and np.allclose(f[:2], 0.0)
+    # a mistaken spacing (the full distance between neighbours) halves the interior stations
+    assert not np.allclose(f[2] / 2, w)
