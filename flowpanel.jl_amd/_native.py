@@ -50,7 +50,7 @@ EXPORTS = [
     "fp_ctx_profile_only", "fp_ctx_profile_get", "fp_ctx_copy_calibrate", "fp_body_nnodes", "fp_body_set_nodes",
     "fp_liftdrag_opts_default", "fp_lift_drag", "fp_lift_drag_adjoint", "fp_solve_bodies_multi",
     "fp_batch_create", "fp_batch_solve", "fp_batch_destroy", "fp_batch_autotune_get", "fp_solve_bodies_lsq",
-    "fp_calcfield_F", "fp_calcfield_sectionalforce", "fp_calcfield_ugradmu", "fp_calcfield_ugradmu_node", "fp_calcfield_ugradmu_combined",
+    "fp_calcfield_F", "fp_calcfield_sectionalforce", "fp_calcfield_ugradmu", "fp_calcfield_ugradmu_node", "fp_calcfield_ugradmu_combined", "fp_uind_noalloc",
 ]
 
 
@@ -174,6 +174,7 @@ def lib():
     L.fp_calcfield_sectionalforce.argtypes = [_vp, _vp, _d, _d, i64, i64, _d, _d, _d]
     L.fp_calcfield_ugradmu_node.argtypes = [_vp, pp, cint, cint, _d, _d]
     L.fp_calcfield_ugradmu_combined.argtypes = [_vp, pp, cint, cint, ctypes.c_double, cint, _d]
+    L.fp_uind_noalloc.argtypes = [_vp, pp, cint, _vp, i64, _vp]
     L.fp_body_nnodes.argtypes = [_vp]
     L.fp_body_nnodes.restype = i64
     L.fp_body_set_nodes.argtypes = [_vp, _vp, _d]

@@ -1320,6 +1320,7 @@ int fp_ctx_create(int device, fp_ctx **out)
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return FP_ERR_DEVICE;
     hipDeviceProp_t p;
     if (hipGetDeviceProperties(&p, device) == hipSuccess) c->num_cu = p.multiProcessorCount;
+    fp_noalloc_init(c.get());
     *out = c.release();
     return FP_OK;
 }

@@ -235,6 +235,8 @@ int fp_rbt_vec(fp_ctx *ctx, const double *in, int64_t ldin, double *out, int64_t
                const double *d_W, int transpose_u);
 size_t fp_getrf_nopiv_work_bytes();
 int fp_getrf_nopiv_f32(fp_ctx *ctx, float *A, int64_t n, int64_t ld, int *info_dev, void *work);
+// device address of fp_uind_noalloc's scratch array, queried once per GPU (fp_noalloc.hip)
+void fp_noalloc_init(fp_ctx *ctx);
 
 // ---- dense linear algebra (fp_dense.hip) --------------------------------
 // P A = L U in place (row-major); perm[i] = original row now at row i.

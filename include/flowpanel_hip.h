@@ -349,6 +349,22 @@ int fp_lift_drag_adjoint(fp_ctx *ctx, fp_body *const *bodies, int nbodies, const
                          const fp_liftdrag_opts *opts, double *dCL_dnodes, double *dCDi_dnodes,
                          double *dCL_dUinf, double *dCDi_dUinf, fp_liftdrag_stats *stats);
 
+/* ---- field evaluation that allocates nothing ----------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP stream/graph capture, tight time-stepping loops that
+ * evaluate Uind! every step, harnesses that forbid hipMalloc.  The same as
+ * fp_uind, but on device memory of the context's GPU: d_targets (3 x nt,
+ * column-major, as fp_uind's targets) and d_out (3 x nt, +=).  The work is
+ * queued on the context stream and the call returns without synchronising.
+ * Target staging and the partial sums of the split launch live in one 64 MiB
+ * device array per GPU, set up by fp_ctx_create and held until the process
+ * ends; nt is limited to FP_UIND_NOALLOC_MAX_TARGETS (larger: FP_ERR_ARG).
+ * Every call on one GPU uses that GPU's array: calls that overlap on one GPU
+ * (two contexts or streams at once) race. */
+#define FP_UIND_NOALLOC_MAX_TARGETS ((int64_t)1 << 20)
+int fp_uind_noalloc(fp_ctx *ctx, fp_body *const *bodies, int nbodies, const double *d_targets, int64_t nt,
+                    double *d_out);
+
 /* ---- instrumentation (no counterpart in the reference) ------------------
  * Per-kernel device time measured with HIP events on the context stream, for
  * roofline reporting.  fp_ctx_profile(ctx, 1) clears the counters and starts
