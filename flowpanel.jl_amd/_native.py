@@ -33,6 +33,8 @@ FP_SOLVER_GMRES_IR = 2
 FP_SOLVER_GMRES = 3
 FP_RESID_MATRIX_FREE = 0
 FP_RESID_STORED = 1
+FP_ORDER_INDEX = 0
+FP_ORDER_SPATIAL = 1
 FP_KUTTA_NONE = 0
 FP_KUTTA_REFERENCE = 1
 FP_KUTTA_PAIR = 2
@@ -51,13 +53,15 @@ EXPORTS = [
     "fp_liftdrag_opts_default", "fp_lift_drag", "fp_lift_drag_adjoint", "fp_solve_bodies_multi",
     "fp_batch_create", "fp_batch_solve", "fp_batch_destroy", "fp_batch_autotune_get", "fp_solve_bodies_lsq",
     "fp_calcfield_F", "fp_calcfield_sectionalforce", "fp_calcfield_ugradmu", "fp_calcfield_ugradmu_node", "fp_calcfield_ugradmu_combined", "fp_uind_noalloc",
+    "fp_assembly_blocks", "fp_field_trails",
 ]
 
 
 class SolveOpts(ctypes.Structure):
     _fields_ = [("solver", ctypes.c_int), ("tol", ctypes.c_double), ("inner_tol", ctypes.c_double),
                 ("restart", ctypes.c_int), ("max_iters", ctypes.c_int), ("max_refine", ctypes.c_int),
-                ("precond_block", ctypes.c_int), ("coarse_block", ctypes.c_int), ("residual_mode", ctypes.c_int)]
+                ("precond_block", ctypes.c_int), ("coarse_block", ctypes.c_int), ("residual_mode", ctypes.c_int),
+                ("precond_order", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):
@@ -175,6 +179,11 @@ def lib():
     L.fp_calcfield_ugradmu_node.argtypes = [_vp, pp, cint, cint, _d, _d]
     L.fp_calcfield_ugradmu_combined.argtypes = [_vp, pp, cint, cint, ctypes.c_double, cint, _d]
     L.fp_uind_noalloc.argtypes = [_vp, pp, cint, _vp, i64, _vp]
+    L.fp_assembly_blocks.argtypes = [_d, i64, _i64, _i64, i64, cint, ctypes.POINTER(ctypes.c_int32), i64, _i64]
+    L.fp_assembly_blocks.restype = i64
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    L.fp_field_trails.argtypes = [_i64, _i64, i64, i64, cint, i32p, i32p, i32p, i64, i64, _i64]
+    L.fp_field_trails.restype = i64
     L.fp_body_nnodes.argtypes = [_vp]
     L.fp_body_nnodes.restype = i64
     L.fp_body_set_nodes.argtypes = [_vp, _vp, _d]
@@ -267,5 +276,7 @@ def opts(solver, **kw):
             continue
         if not hasattr(o, k):
             raise TypeError("unknown solver option %r" % k)
+        if k == "precond_order" and isinstance(v, str):
+            v = {"index": FP_ORDER_INDEX, "spatial": FP_ORDER_SPATIAL}[v]
         setattr(o, k, v)
     return o

@@ -13,7 +13,7 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libflowpanel_hip.so")
-SOURCES = ["fp_influence.hip", "fp_dense.hip", "fp_adjoint.hip", "fp_post.hip", "fp_abi.hip", "fp_lu_nopiv.hip", "fp_noalloc.hip"]
+SOURCES = ["fp_influence.hip", "fp_dense.hip", "fp_adjoint.hip", "fp_post.hip", "fp_abi.hip", "fp_lu_nopiv.hip", "fp_assemble_edges.hip", "fp_field_trails.hip", "fp_noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -507,9 +507,102 @@ int factor(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *d_perm, int *info_out)
 
 // Core solver over an operator for the FP64 residual and (optionally) a
 // stored matrix of the right precision.  b, x device FP64 vectors.
+// ---- preconditioner blocks along a spatial order --------------------------
+// The block-Jacobi blocks and coarse chunks are consecutive index ranges.  On
+// an unstructured mesh consecutive panels need not be neighbours, so with
+// FP_ORDER_SPATIAL the preconditioner is built for P A P^T, P the permutation
+// that sorts the control points along a Morton curve, and applied as
+// P^T M(P A P^T) P: the vector is gathered into that order, preconditioned and
+// scattered back.  The operator and the FP64 residual keep the bodies' order.
+__global__ void k_perm_gather(const double *__restrict__ in, const int *__restrict__ perm, int64_t n,
+                              double *__restrict__ out)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) out[q] = in[perm[q]];
+}
+
+__global__ void k_perm_scatter(const double *__restrict__ in, const int *__restrict__ perm, int64_t n,
+                               double *__restrict__ out)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) out[perm[q]] = in[q];
+}
+
+// Ap[q][c] = A[perm[q]][perm[c]] (row-major, padding columns zero): one grid row
+// per output row (rows beyond gridDim.y in a loop), a row of A read from L2.
+template <typename T>
+__global__ __launch_bounds__(256) void k_perm_matrix(const T *__restrict__ A, int64_t ld, const int *__restrict__ perm,
+                                                     int64_t n, T *__restrict__ Ap)
+{
+    for (int64_t q = blockIdx.y; q < n; q += gridDim.y) {
+        const T *row = A + (int64_t)perm[q] * ld;
+        T *out = Ap + q * ld;
+        for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ld; c += (int64_t)gridDim.x * blockDim.x)
+            out[c] = c < n ? row[perm[c]] : T(0);
+    }
+}
+
+// Control points sorted along a Morton curve (10 bits per axis, each axis
+// scaled to its own extent; ties keep the index order): perm[q] = the panel at
+// position q.
+static int spatial_perm(fp_ctx *ctx, const TargetView &tv, DBuf &perm)
+{
+    const int64_t n = tv.M;
+    std::vector<double> X((size_t)3 * n);
+    FP_HIP(ctx, hipMemcpyAsync(X.data(), tv.x, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipMemcpyAsync(X.data() + n, tv.y, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipMemcpyAsync(X.data() + 2 * n, tv.z, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = *std::min_element(X.begin() + d * n, X.begin() + (d + 1) * n);
+        hi[d] = *std::max_element(X.begin() + d * n, X.begin() + (d + 1) * n);
+    }
+    std::vector<uint32_t> key(n);
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t k = 0;
+        for (int d = 0; d < 3; ++d) {
+            const double u = (X[d * n + i] - lo[d]) / std::max(hi[d] - lo[d], 1e-300);
+            uint32_t q = (uint32_t)std::min(1023.0, std::max(0.0, std::floor(u * 1023.0)));
+            for (int bit = 0; bit < 10; ++bit) k |= ((q >> bit) & 1u) << (3 * bit + d);
+        }
+        key[i] = k;
+    }
+    std::vector<int> p(n);
+    for (int64_t i = 0; i < n; ++i) p[i] = (int)i;
+    std::stable_sort(p.begin(), p.end(), [&](int x, int y) { return key[x] < key[y]; });
+    int st;
+    if ((st = dalloc(ctx, perm, n * sizeof(int)))) return st;
+    FP_HIP(ctx, hipMemcpyAsync(perm.p, p.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));   // p goes out of scope
+    return FP_OK;
+}
+
+struct PermPrecond : Precond {
+    fp_ctx *ctx;
+    Precond *inner;
+    const int *perm;
+    int64_t n;
+    DBuf xp, yp;
+    PermPrecond(fp_ctx *c, Precond *m, const int *p, int64_t nn) : ctx(c), inner(m), perm(p), n(nn) {}
+    int init() { int st = dalloc(ctx, xp, n * sizeof(double)); return st ? st : dalloc(ctx, yp, n * sizeof(double)); }
+    int apply(const double *x, double *y) override
+    {
+        const dim3 g((unsigned)((n + 255) / 256));
+        hipLaunchKernelGGL(k_perm_gather, g, dim3(256), 0, ctx->stream, x, perm, n, xp.as<double>());
+        FP_HIP(ctx, hipGetLastError());
+        int st = inner->apply(xp.as<double>(), yp.as<double>());
+        if (st) return st;
+        hipLaunchKernelGGL(k_perm_scatter, g, dim3(256), 0, ctx->stream, (const double *)yp.as<double>(), perm, n, y);
+        FP_HIP(ctx, hipGetLastError());
+        return FP_OK;
+    }
+};
+
 struct SolveArgs {
     int solver;
     int64_t n, ld;
+    const TargetView *geo = nullptr;   // control points (FP_ORDER_SPATIAL)
     double *A64 = nullptr;   // FP64 matrix (LU path, or residual for uploaded matrices)
     float *A32 = nullptr;    // FP32 matrix (LU32_IR / GMRES_IR)
     Operator *resid = nullptr;  // FP64 residual operator
@@ -529,7 +622,10 @@ struct Solver {
     DBuf perm, work, info, r, d, part, dout;
     std::unique_ptr<Gmres> g;
     std::unique_ptr<BJPrecond> M;
+    std::unique_ptr<PermPrecond> PM;   // M in a spatial order (FP_ORDER_SPATIAL)
+    DBuf order, Aperm;
     std::unique_ptr<DenseOp<float>> A32op;
+    Precond *prec() { return PM ? (Precond *)PM.get() : (Precond *)M.get(); }
     // FP32 LU of the randomized matrix U^T A V without pivoting (FP_SOLVER_LU32_IR):
     // Ar (n4 x ld4), the butterflies, identity row order for getrs, an n4 x k work vector
     bool rbt = false;
@@ -607,6 +703,25 @@ struct Solver {
     {
         int st;
         if ((st = dalloc(ctx, info, sizeof(int)))) return st;
+        if (o.precond_order == FP_ORDER_SPATIAL) {
+            if (!a.geo || a.geo->M != a.n)
+                return fp_fail(ctx, FP_ERR_ARG, "a spatial preconditioner order needs the bodies' control points");
+            if ((st = spatial_perm(ctx, *a.geo, order))) return st;
+            const size_t es = prec == FP_F64 ? sizeof(double) : sizeof(float);
+            if ((st = dalloc(ctx, Aperm, (size_t)a.n * a.ld * es))) return st;
+            const dim3 g((unsigned)std::min<int64_t>((a.ld + 255) / 256, 8), (unsigned)std::min<int64_t>(a.n, 65535));
+            if (prec == FP_F64)
+                hipLaunchKernelGGL(k_perm_matrix<double>, g, dim3(256), 0, ctx->stream, (const double *)A, a.ld,
+                                   (const int *)order.as<int>(), a.n, Aperm.as<double>());
+            else
+                hipLaunchKernelGGL(k_perm_matrix<float>, g, dim3(256), 0, ctx->stream, (const float *)A, a.ld,
+                                   (const int *)order.as<int>(), a.n, Aperm.as<float>());
+            FP_HIP(ctx, hipGetLastError());
+            A = Aperm.p;
+            PM.reset();
+        } else if (o.precond_order != FP_ORDER_INDEX) {
+            return fp_fail(ctx, FP_ERR_ARG, "unknown preconditioner order");
+        }
         M.reset(new BJPrecond(ctx, a.n, blk));
         if ((st = M->setup(A, prec, a.ld, o.coarse_block, info.as<int>()))) return st;
         // zero pivots of the block inverses (main stream) and of the coarse Galerkin
@@ -628,6 +743,10 @@ struct Solver {
             snprintf(buf, sizeof buf, "coarse correction: zero pivot in the Galerkin matrix at row %d of %lld",
                      hinfo[1] - 1, (long long)((a.n + o.coarse_block - 1) / o.coarse_block));
             return fp_fail(ctx, FP_ERR_SINGULAR, buf);
+        }
+        if (o.precond_order == FP_ORDER_SPATIAL) {
+            PM.reset(new PermPrecond(ctx, M.get(), order.as<int>(), a.n));
+            if ((st = PM->init())) return st;
         }
         return FP_OK;
     }
@@ -708,7 +827,7 @@ struct Solver {
         if (a.solver == FP_SOLVER_GMRES) {
             int its = 0;
             double rel = 0;
-            if ((st = g->solve(*a.resid, M.get(), b, x, o.tol, o.max_iters, &its, &rel))) return st;
+            if ((st = g->solve(*a.resid, prec(), b, x, o.tol, o.max_iters, &its, &rel))) return st;
             stats.ms_solve += tm.stop();
             stats.iterations += its;
             stats.residual = worst_residual(stats.residual, rel);
@@ -736,7 +855,7 @@ struct Solver {
                 // the correction only has to bring the residual below tol: once
                 // rel is small the inner solve may stop earlier
                 const double itol = std::max(o.inner_tol, std::min(0.5, 0.3 * o.tol / std::max(rel, 1e-300)));
-                if ((st = g->solve(*A32op, M.get(), rp, dp, itol, left, &its, &irel, rnorm))) return st;
+                if ((st = g->solve(*A32op, prec(), rp, dp, itol, left, &its, &irel, rnorm))) return st;
                 its_total += its;
             }
             if ((st = fp_axpby(ctx, 1.0, dp, 1.0, x, n))) return st;
@@ -1298,6 +1417,7 @@ void fp_solve_opts_default(fp_solve_opts *o, int solver)
     o->precond_block = 0;
     o->coarse_block = 0;
     o->residual_mode = FP_RESID_STORED;
+    o->precond_order = FP_ORDER_INDEX;
     o->inner_tol = 1e-5;
     o->tol = 1e-12;
     if (solver == FP_SOLVER_GMRES) {
@@ -1511,6 +1631,8 @@ int fp_body_create(fp_ctx *ctx, const double *nodes, int64_t nnodes, const int64
     FP_HIP(ctx, hipMemset(b->d_strength, 0, N * sizeof(double)));
     int st = fp_launch_geometry(ctx, b.get());
     if (st) return st;
+    if ((st = fp_body_build_edge_blocks(ctx, b.get()))) return st;
+    if ((st = fp_body_build_trails(ctx, b.get()))) return st;
     FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *out = b.release();
     return FP_OK;
@@ -1522,7 +1644,8 @@ void fp_body_destroy(fp_body *b)
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
     for (void *p : {(void *)b->d_pv, (void *)b->d_nv, (void *)b->d_hs, (void *)b->d_geo, (void *)b->d_uinf,
-                    (void *)b->d_hh, (void *)b->d_hcol, (void *)b->d_strength, (void *)b->d_nbr, (void *)b->d_sh})
+                    (void *)b->d_hh, (void *)b->d_hcol, (void *)b->d_strength, (void *)b->d_nbr, (void *)b->d_sh,
+                    (void *)b->d_eb, (void *)b->d_tr_tab, (void *)b->d_tr})
         if (p) (void)hipFree(p);
     delete b;
 }
@@ -1620,6 +1743,7 @@ static int solve_system(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_so
     a.solver = o.solver;
     a.n = n;
     a.ld = ld;
+    a.geo = &t.tv;
     a.resid = &mf;
     Timer tm(ctx->stream);
     const bool f64 = o.solver == FP_SOLVER_LU || o.solver == FP_SOLVER_GMRES;
@@ -1791,6 +1915,7 @@ static int lsq_bordered(fp_ctx *ctx, fp_body *const *bodies, int nb, const std::
         a.solver = o.solver;
         a.n = n;
         a.ld = ld;
+        a.geo = &t.tv;
         DBuf W, W32;
         std::unique_ptr<DenseOp<double>> op(new DenseOp<double>(ctx, M.as<double>(), n, ld));
         a.resid = op.get();
@@ -1935,6 +2060,7 @@ int fp_solve_bodies_lsq(fp_ctx *ctx, fp_body *const *bodies, int nb, const int64
     a.solver = o.solver;
     a.n = n;
     a.ld = ld;
+    a.geo = &t.tv;
     DBuf W, W32;
     std::unique_ptr<DenseOp<double>> op;
     op.reset(new DenseOp<double>(ctx, C.as<double>(), n, ld));
@@ -2158,12 +2284,16 @@ int fp_solve_mat(fp_ctx *ctx, fp_mat *m, const double *bh, int64_t k, double *xh
         if ((st = mf->init())) return st;
     }
     Operator *R = resid ? resid.get() : (Operator *)mf.get();
+    if (o.precond_order == FP_ORDER_SPATIAL && !m->bodies.empty() && t.M == 0 &&
+        (st = gather_targets(ctx, m->bodies.data(), (int)m->bodies.size(), t)))
+        return st;
     // one factorisation / preconditioner for all k right-hand sides
     fp_solve_stats s{};
     SolveArgs a;
     a.solver = o.solver;
     a.n = n;
     a.ld = ld;
+    a.geo = m->bodies.empty() ? nullptr : &t.tv;
     a.resid = R;
     if (o.solver == FP_SOLVER_LU) {
         if (m->precision != FP_F64) return fp_fail(ctx, FP_ERR_UNSUPPORTED, "FP64 LU of an FP32 matrix");
@@ -2613,6 +2743,7 @@ static int liftdrag_run(fp_ctx *ctx, fp_body *const *bodies, int nb, const doubl
     a.solver = so.solver;
     a.n = n;
     a.ld = ld;
+    a.geo = &t.tv;
     a.resid = &mf;
     if ((st = dalloc(ctx, A, (size_t)n * ld * (stored ? sizeof(double) : sizeof(float))))) return st;
     if (!f64 && stored && (st = dalloc(ctx, A32, (size_t)n * ld * sizeof(float)))) return st;
@@ -2766,6 +2897,7 @@ static int liftdrag_run(fp_ctx *ctx, fp_body *const *bodies, int nb, const doubl
         at.solver = so.solver;
         at.n = n;
         at.ld = ld;
+        at.geo = &t.tv;
         at.A64 = f64 ? AT.as<double>() : nullptr;
         if (!f64) {
             if ((st = dalloc(ctx, AT32, (size_t)n * ld * sizeof(float)))) return st;

@@ -1,0 +1,434 @@
+// fp_assemble_edges.hip -- vortex-ring AIC assembly with node and edge terms
+// shared between the panels of a block (gfx950).
+//
+// G(i, j) = n_i . sum over the edges (a -> b) of panel j of the regularised
+// Biot-Savart segment at control point i (src/FLOWPanel_elements.jl:345-467,
+// assembled as src/FLOWPanel_liftingbody.jl:455-572).  The segment splits into
+//   node terms   r = X - p,  iv = 1 / (|r| + offset)            (per mesh node)
+//   edge terms   c = r_a x r_b,  proj = (p_b - p_a) . (r_a iv_a - r_b iv_b),
+//                s = proj / (4 pi (|c|^2 + offset^2)) (c . n_i)    (per mesh edge)
+// and a panel only adds its edges' s with the sign of its traversal (the
+// segment b -> a gives -s exactly in exact arithmetic).  A quad mesh has about
+// one node and two edges per panel against the four and four a panel-wise
+// kernel evaluates, a triangle mesh a half and one and a half against three
+// and three.
+//
+// Columns are grouped on the host into blocks of at most 256 panels, 512
+// nodes and 768 edges: runs of up to 16 consecutive columns that share nodes
+// (so each row of a block is written as a few contiguous segments), a block
+// seeded by the first unplaced run along a Morton curve of the run centroids
+// and grown by the run sharing the most nodes with it (a compact patch: on a
+// structured loft, runs stacked across strips), else by the next unplaced runs
+// along the curve.  A
+// workgroup takes one block and a tile of TM targets; for TT targets at a time
+// it evaluates the block's node terms (lane = node slot) into LDS, then the
+// edge terms (lane = edge slot) into LDS, then each lane sums its panel's up to
+// four signed edge terms and stores G(i, j) (and its FP32 rounding for the
+// mixed path).  Panels with more than four vertices use the panel-wise kernel.
+#include "fp_internal.h"
+#include "fp_elements.h"
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <unordered_map>
+
+namespace {
+
+constexpr int VRE_P = 256;                 // panels (lanes) per block
+constexpr int VRE_NS = 2, VRE_ES = 3;      // node and edge slots per lane
+constexpr int VRE_NMAX = VRE_NS * VRE_P;   // 512 nodes per block
+constexpr int VRE_EMAX = VRE_ES * VRE_P;   // 768 edges per block
+constexpr int VRE_RUN = 16;                // columns per run
+// per block (int32): node refs (panel*8 + vertex), edges (a | b << 16), columns, edge refs (2 per panel)
+constexpr int VRE_OFF_NODE = 0, VRE_OFF_EDGE = VRE_NMAX, VRE_OFF_COL = VRE_NMAX + VRE_EMAX,
+              VRE_OFF_REF = VRE_NMAX + VRE_EMAX + VRE_P;
+constexpr int VRE_BLOCK_INTS = VRE_OFF_REF + 2 * VRE_P;
+constexpr unsigned VRE_NOREF = 0xFFFFu;
+
+struct __align__(32) NodeTerm {
+    double x, y, z, iv;
+};
+
+uint32_t spread10(uint32_t v)
+{
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+}  // namespace
+
+// Host: the block tables of a body whose panels have at most four vertices
+// (cell_ptr/cell_idx 0-based CSR, nodes 3 x nn).  Appends VRE_BLOCK_INTS int32
+// per block to `tab`; returns the number of blocks and, in sums[0..1], the
+// distinct nodes and edges summed over the blocks (the terms per target).
+static int64_t build_edge_blocks(const double *nodes, int64_t nn, const int64_t *cell_ptr, const int64_t *cell_idx,
+                                 int64_t N, std::vector<int32_t> &tab, int64_t sums[2])
+{
+    auto cell = [&](int64_t j, int k) { return cell_idx[cell_ptr[j] + k]; };
+    auto nv = [&](int64_t j) { return (int)(cell_ptr[j + 1] - cell_ptr[j]); };
+    auto shares_node = [&](int64_t j0, int64_t j1) {
+        for (int k = 0; k < nv(j0); ++k)
+            for (int q = 0; q < nv(j1); ++q)
+                if (cell(j0, k) == cell(j1, q)) return true;
+        return false;
+    };
+    // runs of up to VRE_RUN consecutive columns, broken where neighbours share no node
+    std::vector<int64_t> run0;
+    for (int64_t j = 0; j < N;) {
+        int64_t e = j + 1;
+        while (e < N && e - j < VRE_RUN && shares_node(e - 1, e)) ++e;
+        run0.push_back(j);
+        j = e;
+    }
+    const int64_t nr = (int64_t)run0.size();
+    run0.push_back(N);
+    // Morton order of the run centroids
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    std::vector<double> cen(3 * nr, 0.0);
+    for (int64_t r = 0; r < nr; ++r) {
+        int cnt = 0;
+        for (int64_t j = run0[r]; j < run0[r + 1]; ++j)
+            for (int k = 0; k < nv(j); ++k, ++cnt)
+                for (int d = 0; d < 3; ++d) cen[3 * r + d] += nodes[3 * cell(j, k) + d];
+        for (int d = 0; d < 3; ++d) {
+            cen[3 * r + d] /= cnt;
+            lo[d] = std::min(lo[d], cen[3 * r + d]);
+            hi[d] = std::max(hi[d], cen[3 * r + d]);
+        }
+    }
+    const double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
+    std::vector<uint32_t> code(nr);
+    for (int64_t r = 0; r < nr; ++r) {
+        uint32_t c = 0;
+        for (int d = 0; d < 3; ++d) {
+            const double u = (cen[3 * r + d] - lo[d]) / ext;
+            c |= spread10((uint32_t)std::min(1023.0, std::max(0.0, u * 1023.0))) << d;
+        }
+        code[r] = c;
+    }
+    std::vector<int64_t> order(nr);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return code[x] < code[y]; });
+
+    std::vector<int64_t> rank(nr);
+    for (int64_t q = 0; q < nr; ++q) rank[order[q]] = q;
+    // nodes of each run, runs of each node
+    std::vector<std::vector<int64_t>> rnodes(nr);
+    std::vector<std::vector<int64_t>> noderuns(nn);
+    for (int64_t r = 0; r < nr; ++r) {
+        auto &v = rnodes[r];
+        for (int64_t j = run0[r]; j < run0[r + 1]; ++j)
+            for (int k = 0; k < nv(j); ++k) v.push_back(cell(j, k));
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        for (int64_t u : v) noderuns[u].push_back(r);
+    }
+
+    // blocks grown greedily: seeded by the first unplaced run along the curve, then the
+    // unplaced run sharing the most nodes with the block (ties: earlier on the curve)
+    // while the panel, node and edge limits hold
+    std::vector<int64_t> stamp(nn, -1);
+    std::vector<int32_t> local(nn, 0);
+    std::vector<char> placed(nr, 0);
+    std::unordered_map<uint64_t, int> emap, enew;
+    std::unordered_map<int64_t, int> score;
+    int64_t nb = -1;
+    int np = 0, nnod = 0, ned = 0;
+    sums[0] = sums[1] = 0;
+    auto ekey = [](int64_t u, int64_t w) {
+        return u < w ? ((uint64_t)u << 32) | (uint64_t)w : ((uint64_t)w << 32) | (uint64_t)u;
+    };
+    const size_t base = tab.size();
+    auto fits = [&](int64_t r) {
+        const int len = (int)(run0[r + 1] - run0[r]);
+        if (np + len > VRE_P) return false;
+        int add_n = 0, add_e = 0;
+        for (int64_t u : rnodes[r]) add_n += stamp[u] != nb;
+        if (nnod + add_n > VRE_NMAX) return false;
+        enew.clear();
+        for (int64_t j = run0[r]; j < run0[r + 1]; ++j)
+            for (int k = 0; k < nv(j); ++k) {
+                const int64_t u = cell(j, k), w = cell(j, (k + 1) % nv(j));
+                if (u != w) {
+                    const uint64_t key = ekey(u, w);
+                    if (!emap.count(key) && enew.emplace(key, 1).second) ++add_e;
+                }
+            }
+        return ned + add_e <= VRE_EMAX;
+    };
+    auto place = [&](int64_t r) {
+        placed[r] = 1;
+        score.erase(r);
+        int32_t *T = &tab[base + (size_t)nb * VRE_BLOCK_INTS];
+        for (int64_t j = run0[r]; j < run0[r + 1]; ++j) {
+            const int lane = np++;
+            T[VRE_OFF_COL + lane] = (int32_t)j;
+            unsigned ref[4] = {VRE_NOREF, VRE_NOREF, VRE_NOREF, VRE_NOREF};
+            for (int k = 0; k < nv(j); ++k) {
+                const int64_t u = cell(j, k), w = cell(j, (k + 1) % nv(j));
+                for (int64_t v : {u, w})
+                    if (stamp[v] != nb) {
+                        stamp[v] = nb;
+                        local[v] = nnod;
+                        int kk = 0;
+                        while (cell(j, kk) != v) ++kk;
+                        T[VRE_OFF_NODE + nnod] = (int32_t)(j * 8 + kk);
+                        ++nnod;
+                        for (int64_t r2 : noderuns[v])
+                            if (!placed[r2]) ++score[r2];
+                    }
+                if (u == w) continue;  // repeated vertex: a zero-length edge adds nothing
+                const uint64_t key = ekey(u, w);
+                auto it = emap.find(key);
+                int e;
+                unsigned neg = 0;
+                if (it == emap.end()) {
+                    e = ned++;
+                    emap.emplace(key, e);
+                    T[VRE_OFF_EDGE + e] = (int32_t)((uint32_t)local[u] | ((uint32_t)local[w] << 16));
+                } else {
+                    e = it->second;
+                    neg = ((uint32_t)T[VRE_OFF_EDGE + e] & 0xFFFFu) == (uint32_t)local[u] ? 0u : 1u;
+                }
+                ref[k] = ((unsigned)e << 1) | neg;
+            }
+            T[VRE_OFF_REF + 2 * lane] = (int32_t)(ref[0] | (ref[1] << 16));
+            T[VRE_OFF_REF + 2 * lane + 1] = (int32_t)(ref[2] | (ref[3] << 16));
+        }
+    };
+    std::vector<std::pair<int64_t, int64_t>> cand;  // (-score, rank)
+    int64_t next = 0;
+    for (;;) {
+        while (next < nr && placed[order[next]]) ++next;
+        if (next == nr) break;
+        sums[0] += nnod;
+        sums[1] += ned;
+        ++nb;
+        tab.resize(base + (size_t)(nb + 1) * VRE_BLOCK_INTS, -1);
+        np = nnod = ned = 0;
+        emap.clear();
+        score.clear();
+        place(order[next]);
+        for (;;) {
+            cand.clear();
+            for (const auto &kv : score) cand.emplace_back(-(int64_t)kv.second, rank[kv.first]);
+            std::sort(cand.begin(), cand.end());
+            bool added = false;
+            for (const auto &c : cand)
+                if (fits(order[c.second])) {
+                    place(order[c.second]);
+                    added = true;
+                    break;
+                }
+            // nothing adjacent fits (or nothing is adjacent): the next unplaced runs along the curve
+            for (int64_t q = next, tries = 0; !added && q < nr && tries < 8; ++q) {
+                if (placed[order[q]]) continue;
+                ++tries;
+                if (fits(order[q])) {
+                    place(order[q]);
+                    added = true;
+                }
+            }
+            if (!added) break;
+        }
+    }
+    sums[0] += nnod;
+    sums[1] += ned;
+    return nb + 1;
+}
+
+// Build and upload the block tables of a vortex-ring body of triangles and quads.
+int fp_body_build_edge_blocks(fp_ctx *ctx, fp_body *b)
+{
+    if (b->element != FP_ELEM_VORTEXRING || b->nvp != 4 || b->N == 0) return FP_OK;
+    std::vector<int32_t> tab;
+    int64_t sums[2];
+    const int64_t nblocks = build_edge_blocks(b->nodes.data(), (int64_t)b->nodes.size() / 3, b->cell_ptr.data(),
+                                              b->cell_idx.data(), b->N, tab, sums);
+    FP_HIP(ctx, hipMalloc((void **)&b->d_eb, tab.size() * sizeof(int32_t)));
+    FP_HIP(ctx, hipMemcpy(b->d_eb, tab.data(), tab.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    b->eb_nblocks = nblocks;
+    b->eb_nodes = sums[0];
+    b->eb_edges = sums[1];
+    return FP_OK;
+}
+
+extern "C" int64_t fp_assembly_blocks(const double *nodes, int64_t nnodes, const int64_t *cell_ptr,
+                                      const int64_t *cell_idx, int64_t ncells, int index_base, int32_t *tab,
+                                      int64_t cap_blocks, int64_t *terms)
+{
+    if (!nodes || !cell_ptr || !cell_idx || ncells <= 0 || nnodes <= 0) return -FP_ERR_ARG;
+    std::vector<int64_t> cp(ncells + 1), ci((size_t)(cell_ptr[ncells] - cell_ptr[0]));
+    for (int64_t c = 0; c <= ncells; ++c) cp[c] = cell_ptr[c] - cell_ptr[0];
+    for (int64_t c = 0; c < ncells; ++c)
+        if (cp[c + 1] - cp[c] < 3 || cp[c + 1] - cp[c] > 4) return -FP_ERR_UNSUPPORTED;
+    for (size_t k = 0; k < ci.size(); ++k) {
+        ci[k] = cell_idx[cell_ptr[0] + (int64_t)k] - index_base;
+        if (ci[k] < 0 || ci[k] >= nnodes) return -FP_ERR_ARG;
+    }
+    std::vector<int32_t> t;
+    int64_t sums[2];
+    const int64_t nb = build_edge_blocks(nodes, nnodes, cp.data(), ci.data(), ncells, t, sums);
+    if (terms) { terms[0] = sums[0]; terms[1] = sums[1]; }
+    if (tab && cap_blocks >= nb) std::memcpy(tab, t.data(), t.size() * sizeof(int32_t));
+    return nb;
+}
+
+template <typename TO, int TM, int TT>
+__global__ __launch_bounds__(256) void k_assemble_vre(const int32_t *__restrict__ tab, const double *__restrict__ pv,
+                                                      int64_t N, TargetView tv, TO *__restrict__ G, int64_t ld,
+                                                      int64_t col0, float *__restrict__ G2, double off, double cut)
+{
+    __shared__ double st[6][TM];
+    __shared__ NodeTerm sn[TT][VRE_NMAX];
+    __shared__ double se[TT][VRE_EMAX];
+    const int tid = threadIdx.x;
+    const int32_t *T = tab + (int64_t)blockIdx.x * VRE_BLOCK_INTS;
+    const int64_t i0 = (int64_t)blockIdx.y * TM;
+    for (int t = tid; t < TM; t += blockDim.x) {
+        const int64_t i = i0 + t;
+        const bool ok = i < tv.M;
+        st[0][t] = ok ? tv.x[i] : 0.0;
+        st[1][t] = ok ? tv.y[i] : 0.0;
+        st[2][t] = ok ? tv.z[i] : 0.0;
+        st[3][t] = ok ? tv.nx[i] : 0.0;
+        st[4][t] = ok ? tv.ny[i] : 0.0;
+        st[5][t] = ok ? tv.nz[i] : 0.0;
+    }
+    auto node_xyz = [&](int32_t ref, double &x, double &y, double &z) {
+        const int64_t j = ref >> 3;
+        const int k = ref & 7;
+        x = pv[(3 * k + 0) * N + j];
+        y = pv[(3 * k + 1) * N + j];
+        z = pv[(3 * k + 2) * N + j];
+    };
+    double px[VRE_NS], py[VRE_NS], pz[VRE_NS];
+    bool nok[VRE_NS];
+#pragma unroll
+    for (int s = 0; s < VRE_NS; ++s) {
+        const int32_t ref = T[VRE_OFF_NODE + s * VRE_P + tid];
+        nok[s] = ref >= 0;
+        px[s] = py[s] = pz[s] = 0.0;
+        if (nok[s]) node_xyz(ref, px[s], py[s], pz[s]);
+    }
+    int ea[VRE_ES], eb[VRE_ES];
+    double ex[VRE_ES], ey[VRE_ES], ez[VRE_ES];
+    bool eok[VRE_ES];
+#pragma unroll
+    for (int s = 0; s < VRE_ES; ++s) {
+        const int32_t ab = T[VRE_OFF_EDGE + s * VRE_P + tid];
+        eok[s] = ab >= 0;
+        ea[s] = ab & 0xFFFF;
+        eb[s] = (ab >> 16) & 0xFFFF;
+        ex[s] = ey[s] = ez[s] = 0.0;
+        if (eok[s]) {
+            double ax, ay, az, bx, by, bz;
+            node_xyz(T[VRE_OFF_NODE + ea[s]], ax, ay, az);
+            node_xyz(T[VRE_OFF_NODE + eb[s]], bx, by, bz);
+            ex[s] = bx - ax; ey[s] = by - ay; ez[s] = bz - az;
+        } else {
+            ea[s] = eb[s] = 0;
+        }
+    }
+    const int32_t col = T[VRE_OFF_COL + tid];
+    int ri[4];
+    double rs[4];
+    {
+        const uint32_t r01 = (uint32_t)T[VRE_OFF_REF + 2 * tid], r23 = (uint32_t)T[VRE_OFF_REF + 2 * tid + 1];
+        const uint32_t r[4] = {r01 & 0xFFFFu, r01 >> 16, r23 & 0xFFFFu, r23 >> 16};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool ok = col >= 0 && r[k] != VRE_NOREF;
+            ri[k] = ok ? (int)(r[k] >> 1) : 0;
+            rs[k] = ok ? ((r[k] & 1u) ? -1.0 : 1.0) : 0.0;
+        }
+    }
+    const double off2 = off * off, cut2 = cut * cut;
+    const int iend = (int)min((int64_t)TM, tv.M - i0);
+    __syncthreads();
+    for (int t0 = 0; t0 < iend; t0 += TT) {
+        // node terms
+#pragma unroll
+        for (int s = 0; s < VRE_NS; ++s) {
+            if (!nok[s]) continue;
+#pragma unroll
+            for (int tt = 0; tt < TT; ++tt) {
+                const int t = t0 + tt;
+                if (t >= iend) break;
+                const double rx = st[0][t] - px[s], ry = st[1][t] - py[s], rz = st[2][t] - pz[s];
+                NodeTerm q;
+                q.x = rx; q.y = ry; q.z = rz;
+                q.iv = fp_recip(fp_sqrt(rx * rx + ry * ry + rz * rz) + off);
+                sn[tt][s * VRE_P + tid] = q;
+            }
+        }
+        __syncthreads();
+        // edge terms
+#pragma unroll
+        for (int s = 0; s < VRE_ES; ++s) {
+            if (!eok[s]) continue;
+#pragma unroll
+            for (int tt = 0; tt < TT; ++tt) {
+                const int t = t0 + tt;
+                if (t >= iend) break;
+                const NodeTerm A = sn[tt][ea[s]], B = sn[tt][eb[s]];
+                const double cx = A.y * B.z - A.z * B.y;
+                const double cy = A.z * B.x - A.x * B.z;
+                const double cz = A.x * B.y - A.y * B.x;
+                const double c2 = cx * cx + cy * cy + cz * cz;
+                const double proj = ex[s] * (A.x * A.iv - B.x * B.iv) + ey[s] * (A.y * A.iv - B.y * B.iv) +
+                                    ez[s] * (A.z * A.iv - B.z * B.iv);
+                double v = 0.0;
+                if (c2 > cut2)
+                    v = proj * fp_recip(4.0 * FP_PI * (c2 + off2)) * (cx * st[3][t] + cy * st[4][t] + cz * st[5][t]);
+                se[tt][s * VRE_P + tid] = v;
+            }
+        }
+        __syncthreads();
+        // panel sums
+        if (col >= 0) {
+#pragma unroll
+            for (int tt = 0; tt < TT; ++tt) {
+                const int t = t0 + tt;
+                if (t >= iend) break;
+                double v = 0.0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) v += rs[k] != 0.0 ? rs[k] * se[tt][ri[k]] : 0.0;
+                const int64_t at = (i0 + t) * ld + col0 + col;
+                G[at] = TO(v);
+                if (G2) G2[at] = float(v);
+            }
+        }
+    }
+}
+
+// Launch the shared-term kernel for a body with block tables; returns 1 when the
+// body has none (the caller then uses the panel-wise kernel).
+template <typename T>
+int fp_launch_assemble_edges(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G, int64_t ld, int64_t col0,
+                             float *G32, int *used)
+{
+    constexpr int TM = 64, TT = 2;
+    *used = 0;
+    if (!b->d_eb || b->eb_nblocks == 0) return FP_OK;
+    const char *env = getenv("FLOWPANEL_ASSEMBLY");
+    if (env && strcmp(env, "panel") == 0) return FP_OK;
+    *used = 1;
+    if (tv.M == 0) return FP_OK;
+    dim3 grid((unsigned)b->eb_nblocks, (unsigned)((tv.M + TM - 1) / TM));
+    ProfScope ps(ctx, sizeof(T) == 8 ? "assemble_vre_f64" : "assemble_vre_f32");
+    hipLaunchKernelGGL((k_assemble_vre<T, TM, TT>), grid, dim3(VRE_P), 0, ctx->stream, b->d_eb, b->d_pv, b->N, tv, G,
+                       ld, col0, G32, b->kernel_offset, b->kernel_cutoff);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+template int fp_launch_assemble_edges<double>(fp_ctx *, const fp_body *, const TargetView &, double *, int64_t,
+                                              int64_t, float *, int *);
+template int fp_launch_assemble_edges<float>(fp_ctx *, const fp_body *, const TargetView &, float *, int64_t, int64_t,
+                                             float *, int *);

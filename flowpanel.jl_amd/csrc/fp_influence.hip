@@ -275,8 +275,12 @@ int fp_launch_assemble(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G
     if (tv.M == 0 || b->N == 0) return FP_OK;
     dim3 grid((unsigned)((b->N + 255) / 256), (unsigned)((tv.M + TM - 1) / TM));
     if (b->element == FP_ELEM_VORTEXRING) {
-        ProfScope ps(ctx, sizeof(T) == 8 ? "assemble_vr_f64" : "assemble_vr_f32");
-        if (b->nvp == 4)
+        int shared = 0;
+        int st = fp_launch_assemble_edges<T>(ctx, b, tv, G, ld, col0, G32, &shared);
+        if (st) return st;
+        ProfScope ps(ctx, shared ? nullptr : sizeof(T) == 8 ? "assemble_vr_f64" : "assemble_vr_f32");
+        if (shared) {
+        } else if (b->nvp == 4)
             hipLaunchKernelGGL((k_assemble_vr<double, T, 4, TM>), grid, dim3(256), 0, ctx->stream, pan, tv, G, ld, col0,
                                G32);
         else
@@ -553,6 +557,11 @@ int fp_launch_field(fp_ctx *ctx, const fp_body *b, const double *strength, const
     pchunk = (pchunk + FIELD_TILE - 1) / FIELD_TILE * FIELD_TILE;
     splits = (int)((b->N + pchunk - 1) / pchunk);
     double *part = splits > 1 ? partial : nullptr;
+    if (b->element == FP_ELEM_VORTEXRING) {
+        int used = 0;
+        int st = fp_launch_field_trails(ctx, b, strength, tv, mode, out, part, splits, &used);
+        if (st || used) return st;
+    }
     ProfScope ps(ctx, mode == 1 ? "field_matvec_f64" : (mode == 0 ? "field_velocity_f64" : "field_potential_f64"));
     if (b->nvp == 4) launch_field_nvp<4>(ctx, b, strength, tv, mode, out, part, splits, pchunk);
     else launch_field_nvp<8>(ctx, b, strength, tv, mode, out, part, splits, pchunk);

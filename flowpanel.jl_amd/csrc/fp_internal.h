@@ -78,7 +78,7 @@ struct ProfScope {
     int idx = -1;
     ProfScope(fp_ctx *ctx, const char *name) : c(ctx)
     {
-        if (!c->prof || (!c->prof_only.empty() && c->prof_only != name)) return;
+        if (!name || !c->prof || (!c->prof_only.empty() && c->prof_only != name)) return;
         fp_prof_rec r;
         r.name = name;
         if (!take(r.a)) return;
@@ -125,6 +125,11 @@ struct fp_body {
     uint64_t version = 0;  // bumped whenever device geometry or wake buffers change
     int *d_nbr = nullptr;         // cell neighbours across each edge (built on first use)
     int64_t *d_sh = nullptr;      // shedding table on the device (built on first use)
+    int32_t *d_eb = nullptr;      // shared node/edge assembly blocks (triangles and quads)
+    int64_t eb_nblocks = 0, eb_nodes = 0, eb_edges = 0;
+    int32_t *d_tr_tab = nullptr;  // edge trails of the mesh (node refs, contributor CSR)
+    double *d_tr = nullptr;       // trail stream filled from the strengths before each field evaluation
+    int64_t tr_S = 0;
 };
 
 struct fp_mat {
@@ -202,6 +207,13 @@ struct AssemblyItem {
 int fp_launch_geometry(fp_ctx *ctx, fp_body *b);
 // AIC column block of one body into a row-major matrix (T = float|double); with
 // G32 set, the same entries rounded to FP32 go to G32 (same ld) as well
+int fp_body_build_edge_blocks(fp_ctx *ctx, fp_body *b);
+int fp_body_build_trails(fp_ctx *ctx, fp_body *b);
+int fp_launch_field_trails(fp_ctx *ctx, const fp_body *b, const double *strength, const TargetView &tv, int mode,
+                           double *out, double *partial, int splits, int *used);
+template <typename T>
+int fp_launch_assemble_edges(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G, int64_t ld, int64_t col0,
+                             float *G32, int *used);
 template <typename T>
 int fp_launch_assemble(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G, int64_t ld, int64_t col0,
                        float *G32 = nullptr);

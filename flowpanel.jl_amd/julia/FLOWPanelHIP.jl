@@ -73,6 +73,7 @@ struct SolveOpts
     precond_block::Cint
     coarse_block::Cint
     residual_mode::Cint
+    precond_order::Cint
 end
 
 struct SolveStats

@@ -91,10 +91,16 @@ typedef struct fp_solve_opts {
     int residual_mode; /* refinement residual r = b - A x in FP64 from: FP_RESID_STORED (the
                           FP64 AIC kept beside the FP32 copy) or FP_RESID_MATRIX_FREE
                           (influences recomputed in FP64, no FP64 matrix in memory)          */
+    int precond_order; /* which panels form the block-Jacobi blocks and coarse chunks:
+                          FP_ORDER_INDEX (consecutive indices: structured grids) or
+                          FP_ORDER_SPATIAL (consecutive along a Morton curve of the control
+                          points: unstructured meshes; bodies only, not fp_solve_mat)          */
 } fp_solve_opts;
 
 #define FP_RESID_MATRIX_FREE 0
 #define FP_RESID_STORED 1
+#define FP_ORDER_INDEX 0
+#define FP_ORDER_SPATIAL 1
 
 typedef struct fp_solve_stats {
     int refinements;   /* FP64 refinement steps taken                                      */
@@ -207,6 +213,26 @@ int fp_batch_autotune_get(const fp_batch *batch, int k, int *tile_targets, int *
  * src/FLOWPanel_nonliftingbody.jl:141-186, src/FLOWPanel_multibody.jl:502-555):
  * device-resident AIC of the bodies with their stored wake directions. */
 int fp_assemble(fp_ctx *ctx, fp_body *const *bodies, int nbodies, int precision, fp_mat **out);
+/* Host only, no device: the column blocks the vortex-ring assembly of a triangle/quad body uses (mesh
+ * node and edge terms computed once per target and block, shared by the block's panels).  Returns the
+ * number of blocks (negative FP_ERR_* on bad input); when tab is not null and cap_blocks is at least
+ * that number, writes 2048 int32 per block: 512 node references (panel * 8 + vertex, -1 = none), 768
+ * edges (local node a | b << 16, -1 = none), 256 columns (-1 = none), 256 pairs of packed 16-bit edge
+ * references ((edge << 1) | reversed, 0xFFFF = none); terms (2 values, may be null) = the nodes and the
+ * edges summed over the blocks.  No counterpart in the reference (a layout of this library, exported
+ * so that it can be inspected). */
+int64_t fp_assembly_blocks(const double *nodes, int64_t nnodes, const int64_t *cell_ptr, const int64_t *cell_idx,
+                           int64_t ncells, int index_base, int32_t *tab, int64_t cap_blocks, int64_t *terms);
+/* Host only, no device: the edge trails the velocity of a vortex-ring body is summed over (every mesh edge
+ * once, gamma_e = the signed strengths of the panels that hold it).  Returns the number S of stream
+ * entries (negative FP_ERR_* on bad input) and, in *ncontrib, the number of (panel, sign) contributors;
+ * when the arrays are given and large enough (cap_entries >= S, cap_contrib >= *ncontrib) writes
+ * ref[S] = panel * 8 + vertex of each entry's node, cptr[S + 1] and cpan[*ncontrib] = (panel << 1) |
+ * reversed, the contributors of the edge from entry m - 1's node to entry m's (none: a trail starts).
+ * No counterpart in the reference (a layout of this library, exported so that it can be inspected). */
+int64_t fp_field_trails(const int64_t *cell_ptr, const int64_t *cell_idx, int64_t ncells, int64_t nnodes,
+                        int index_base, int32_t *ref, int32_t *cptr, int32_t *cpan, int64_t cap_entries,
+                        int64_t cap_contrib, int64_t *ncontrib);
 /* Upload a caller matrix (column-major n x n) for the solver protocol. */
 int fp_mat_upload(fp_ctx *ctx, int64_t n, const double *A, fp_mat **out);
 int64_t fp_mat_size(const fp_mat *mat);
