@@ -603,6 +603,7 @@ struct SolveArgs {
     int solver;
     int64_t n, ld;
     const TargetView *geo = nullptr;   // control points (FP_ORDER_SPATIAL)
+    int64_t nrhs = 1;                  // right-hand sides solved together (choice of the LU32 correction)
     double *A64 = nullptr;   // FP64 matrix (LU path, or residual for uploaded matrices)
     float *A32 = nullptr;    // FP32 matrix (LU32_IR / GMRES_IR)
     Operator *resid = nullptr;  // FP64 residual operator
@@ -631,6 +632,9 @@ struct Solver {
     bool rbt = false;
     int64_t n4 = 0, ld4 = 0;
     DBuf Ar, Wrbt, perm4, work4, t4;
+    // explicit inverses of the no-pivot factors (-L^-1, -U^-1): a correction is two GEMVs
+    bool inv = false;
+    DBuf YL, YU, w4;
     Solver(fp_ctx *c, const SolveArgs &args, const fp_solve_opts &opts) : ctx(c), a(args), o(opts) {}
 
     // Randomize and factor without pivoting; false (and nothing kept) when a
@@ -652,7 +656,20 @@ struct Solver {
         st = a.R64 ? fp_rbt_matrix<double>(ctx, a.R64, n, a.ld, Ar.as<float>(), ld4, Wrbt.as<double>())
                    : fp_rbt_matrix<float>(ctx, a.A32, n, a.ld, Ar.as<float>(), ld4, Wrbt.as<double>());
         if (st) return st;
-        if ((st = fp_getrf_nopiv_f32(ctx, Ar.as<float>(), n4, ld4, info.as<int>(), nw.p))) return st;
+        // the inverses cost about twice the factorisation ((4/3) n^3 against (2/3) n^3 on MFMA) and
+        // pay off when many right-hand sides are corrected together; FLOWPANEL_LU32_INVERSE=1/0 forces them
+        const char *gi = getenv("FLOWPANEL_LU32_INVERSE");
+        inv = gi && (gi[0] == '0' || gi[0] == '1') ? gi[0] == '1' : a.nrhs >= 4;
+        if (inv) {
+            if ((st = dalloc(ctx, YL, (size_t)n4 * ld4 * sizeof(float))) ||
+                (st = dalloc(ctx, YU, (size_t)n4 * ld4 * sizeof(float))) || (st = dalloc(ctx, w4, n4 * sizeof(double))) ||
+                (st = fp_fill_f32(ctx, YL.as<float>(), n4 * ld4, 0.f)) ||
+                (st = fp_fill_f32(ctx, YU.as<float>(), n4 * ld4, 0.f)))
+                return st;
+        }
+        if ((st = fp_getrf_nopiv_f32(ctx, Ar.as<float>(), n4, ld4, info.as<int>(), nw.p,
+                                     inv ? YL.as<float>() : nullptr, inv ? YU.as<float>() : nullptr)))
+            return st;
         std::vector<int> id(n4);
         for (int64_t i = 0; i < n4; ++i) id[i] = (int)i;
         int hinfo = 0;
@@ -661,7 +678,18 @@ struct Solver {
         FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
         if (hinfo) {
             Ar.reset();
+            YL.reset();
+            YU.reset();
+            inv = false;
             return FP_OK;
+        }
+        if (inv) {
+            // the triangular inverses take Ar's place; the scratch T reuses Ar after them
+            DBuf T;
+            if ((st = dalloc(ctx, T, (size_t)n4 * ld4 * sizeof(float)))) return st;
+            if ((st = fp_trtri_nopiv_f32(ctx, Ar.as<float>(), n4, ld4, YL.as<float>(), YU.as<float>(), T.as<float>())))
+                return st;
+            Ar.reset();
         }
         *ok = true;
         rbt = true;
@@ -673,7 +701,10 @@ struct Solver {
     {
         int st;
         rbt = false;
+        inv = false;
         Ar.reset();
+        YL.reset();
+        YU.reset();
         if ((st = dalloc(ctx, perm, a.n * sizeof(int)))) return st;
         if ((st = factor<float>(ctx, a.A32, a.n, a.ld, perm.as<int>(), &stats.info))) return st;
         return dalloc(ctx, work, fp_getrs_work_bytes(a.n, 1));
@@ -692,7 +723,22 @@ struct Solver {
                 t = tk.as<double>();
             }
             if ((st = fp_rbt_vec(ctx, rr, n, t, n4, n, k, Wrbt.as<double>(), 1))) return st;
-            if ((st = fp_getrs<float>(ctx, Ar.as<float>(), n4, ld4, perm4.as<int>(), t, k, n4, wk, wbytes))) return st;
+            if (inv) {
+                // t <- (-U^-1)(-L^-1) t
+                DBuf wk2;
+                double *w = w4.as<double>();
+                if (k > 1) {
+                    if ((st = dalloc(ctx, wk2, (size_t)n4 * k * sizeof(double)))) return st;
+                    w = wk2.as<double>();
+                    if ((st = fp_gemv_multi<float>(ctx, YL.as<float>(), n4, ld4, t, n4, k, w, n4))) return st;
+                    if ((st = fp_gemv_multi<float>(ctx, YU.as<float>(), n4, ld4, w, n4, k, t, n4))) return st;
+                } else {
+                    if ((st = fp_gemv<float>(ctx, YL.as<float>(), n4, ld4, t, w, 1.0, 0.0))) return st;
+                    if ((st = fp_gemv<float>(ctx, YU.as<float>(), n4, ld4, w, t, 1.0, 0.0))) return st;
+                }
+            } else if ((st = fp_getrs<float>(ctx, Ar.as<float>(), n4, ld4, perm4.as<int>(), t, k, n4, wk, wbytes))) {
+                return st;
+            }
             return fp_rbt_vec(ctx, t, n4, dd, n, n, k, Wrbt.as<double>(), 0);
         }
         FP_HIP(ctx, hipMemcpyAsync(dd, rr, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
@@ -1772,6 +1818,7 @@ static int solve_system(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_so
         op64.reset(new DenseOp<double>(ctx, a.A64, n, ld));
         a.resid = op64.get();
     }
+    a.nrhs = k;
     Solver solver(ctx, a, o);
     if ((st = solver.setup(s))) return st;
     return solver.solve_multi(rhs, x, k, s);
@@ -1932,6 +1979,7 @@ static int lsq_bordered(fp_ctx *ctx, fp_body *const *bodies, int nb, const std::
             if ((s2 = fp_mat_narrow(ctx, M.as<double>(), W32.as<float>(), n, ld))) return s2;
             a.A32 = W32.as<float>();
         }
+        a.nrhs = k;
         Solver solver(ctx, a, o);
         if ((s2 = solver.setup(s))) return s2;
         return solver.solve_multi(rhs_d, out, k, s);
@@ -2315,6 +2363,7 @@ int fp_solve_mat(fp_ctx *ctx, fp_mat *m, const double *bh, int64_t k, double *xh
         a.A32 = W.as<float>();
     }
     if (m->precision == FP_F64) a.R64 = (const double *)m->A;
+    a.nrhs = k;
     Solver solver(ctx, a, o);
     st = solver.setup(s);
     if (!st) st = solver.solve_multi(b.as<double>(), x.as<double>(), k, s);

@@ -38,8 +38,11 @@ namespace {
 
 constexpr int VRE_P = 256;                 // panels (lanes) per block
 constexpr int VRE_NS = 2, VRE_ES = 3;      // node and edge slots per lane
-constexpr int VRE_NMAX = VRE_NS * VRE_P;   // 512 nodes per block
-constexpr int VRE_EMAX = VRE_ES * VRE_P;   // 768 edges per block
+constexpr int VRE_NMAX = VRE_NS * VRE_P;   // node slots per block in the table (512)
+constexpr int VRE_EMAX = VRE_ES * VRE_P;   // edge slots per block in the table (768)
+// blocks are packed to at most VRE_NPACK nodes and VRE_EPACK edges (the LDS the
+// kernel holds per target: 4 workgroups fit a CU at two targets per pass)
+constexpr int VRE_NPACK = 384, VRE_EPACK = 640;
 constexpr int VRE_RUN = 16;                // columns per run
 // per block (int32): node refs (panel*8 + vertex), edges (a | b << 16), columns, edge refs (2 per panel)
 constexpr int VRE_OFF_NODE = 0, VRE_OFF_EDGE = VRE_NMAX, VRE_OFF_COL = VRE_NMAX + VRE_EMAX,
@@ -150,7 +153,7 @@ static int64_t build_edge_blocks(const double *nodes, int64_t nn, const int64_t 
         if (np + len > VRE_P) return false;
         int add_n = 0, add_e = 0;
         for (int64_t u : rnodes[r]) add_n += stamp[u] != nb;
-        if (nnod + add_n > VRE_NMAX) return false;
+        if (nnod + add_n > VRE_NPACK) return false;
         enew.clear();
         for (int64_t j = run0[r]; j < run0[r + 1]; ++j)
             for (int k = 0; k < nv(j); ++k) {
@@ -160,7 +163,7 @@ static int64_t build_edge_blocks(const double *nodes, int64_t nn, const int64_t 
                     if (!emap.count(key) && enew.emplace(key, 1).second) ++add_e;
                 }
             }
-        return ned + add_e <= VRE_EMAX;
+        return ned + add_e <= VRE_EPACK;
     };
     auto place = [&](int64_t r) {
         placed[r] = 1;
@@ -286,8 +289,8 @@ __global__ __launch_bounds__(256) void k_assemble_vre(const int32_t *__restrict_
                                                       int64_t col0, float *__restrict__ G2, double off, double cut)
 {
     __shared__ double st[6][TM];
-    __shared__ NodeTerm sn[TT][VRE_NMAX];
-    __shared__ double se[TT][VRE_EMAX];
+    __shared__ NodeTerm sn[TT][VRE_NPACK];
+    __shared__ double se[TT][VRE_EPACK];
     const int tid = threadIdx.x;
     const int32_t *T = tab + (int64_t)blockIdx.x * VRE_BLOCK_INTS;
     const int64_t i0 = (int64_t)blockIdx.y * TM;
@@ -308,32 +311,34 @@ __global__ __launch_bounds__(256) void k_assemble_vre(const int32_t *__restrict_
         y = pv[(3 * k + 1) * N + j];
         z = pv[(3 * k + 2) * N + j];
     };
+    // node slots of this lane (slot s = node s * 256 + tid of the block)
     double px[VRE_NS], py[VRE_NS], pz[VRE_NS];
     bool nok[VRE_NS];
 #pragma unroll
     for (int s = 0; s < VRE_NS; ++s) {
-        const int32_t ref = T[VRE_OFF_NODE + s * VRE_P + tid];
+        const int n = s * VRE_P + tid;
+        const int32_t ref = n < VRE_NPACK ? T[VRE_OFF_NODE + n] : -1;
         nok[s] = ref >= 0;
         px[s] = py[s] = pz[s] = 0.0;
         if (nok[s]) node_xyz(ref, px[s], py[s], pz[s]);
     }
+    // edge slots: local end nodes and the edge vector b - a in FP64 coordinates
     int ea[VRE_ES], eb[VRE_ES];
     double ex[VRE_ES], ey[VRE_ES], ez[VRE_ES];
     bool eok[VRE_ES];
 #pragma unroll
     for (int s = 0; s < VRE_ES; ++s) {
-        const int32_t ab = T[VRE_OFF_EDGE + s * VRE_P + tid];
+        const int e = s * VRE_P + tid;
+        const int32_t ab = e < VRE_EPACK ? T[VRE_OFF_EDGE + e] : -1;
         eok[s] = ab >= 0;
-        ea[s] = ab & 0xFFFF;
-        eb[s] = (ab >> 16) & 0xFFFF;
+        ea[s] = eok[s] ? (ab & 0xFFFF) : 0;
+        eb[s] = eok[s] ? ((ab >> 16) & 0xFFFF) : 0;
         ex[s] = ey[s] = ez[s] = 0.0;
         if (eok[s]) {
             double ax, ay, az, bx, by, bz;
             node_xyz(T[VRE_OFF_NODE + ea[s]], ax, ay, az);
             node_xyz(T[VRE_OFF_NODE + eb[s]], bx, by, bz);
             ex[s] = bx - ax; ey[s] = by - ay; ez[s] = bz - az;
-        } else {
-            ea[s] = eb[s] = 0;
         }
     }
     const int32_t col = T[VRE_OFF_COL + tid];
@@ -349,34 +354,38 @@ __global__ __launch_bounds__(256) void k_assemble_vre(const int32_t *__restrict_
             rs[k] = ok ? ((r[k] & 1u) ? -1.0 : 1.0) : 0.0;
         }
     }
-    const double off2 = off * off, cut2 = cut * cut;
+    const double off2 = off * off, cut2 = cut * cut, k4 = 4.0 * FP_PI;
     const int iend = (int)min((int64_t)TM, tv.M - i0);
+    TO *g = G + i0 * ld + col0 + (col >= 0 ? col : 0);
+    float *g2 = G2 ? G2 + i0 * ld + col0 + (col >= 0 ? col : 0) : nullptr;
     __syncthreads();
     for (int t0 = 0; t0 < iend; t0 += TT) {
-        // node terms
+        // node terms (a partial last pass repeats the last target: same values, never stored twice);
+        // per slot, the TT targets are independent chains and are stored together
 #pragma unroll
         for (int s = 0; s < VRE_NS; ++s) {
-            if (!nok[s]) continue;
+            NodeTerm q[TT];
 #pragma unroll
             for (int tt = 0; tt < TT; ++tt) {
-                const int t = t0 + tt;
-                if (t >= iend) break;
+                const int t = min(t0 + tt, iend - 1);
                 const double rx = st[0][t] - px[s], ry = st[1][t] - py[s], rz = st[2][t] - pz[s];
-                NodeTerm q;
-                q.x = rx; q.y = ry; q.z = rz;
-                q.iv = fp_recip(fp_sqrt(rx * rx + ry * ry + rz * rz) + off);
-                sn[tt][s * VRE_P + tid] = q;
+                q[tt].x = rx; q[tt].y = ry; q[tt].z = rz;
+                q[tt].iv = fp_recip(fp_sqrt_nb(rx * rx + ry * ry + rz * rz) + off);
+            }
+            if (nok[s]) {
+#pragma unroll
+                for (int tt = 0; tt < TT; ++tt) sn[tt][s * VRE_P + tid] = q[tt];
             }
         }
         __syncthreads();
-        // edge terms
+        // edge terms, branch-free (the regularised factor is finite for c2 = 0 when off > 0;
+        // the select drops it below the cutoff either way)
 #pragma unroll
         for (int s = 0; s < VRE_ES; ++s) {
-            if (!eok[s]) continue;
+            double v[TT];
 #pragma unroll
             for (int tt = 0; tt < TT; ++tt) {
-                const int t = t0 + tt;
-                if (t >= iend) break;
+                const int t = min(t0 + tt, iend - 1);
                 const NodeTerm A = sn[tt][ea[s]], B = sn[tt][eb[s]];
                 const double cx = A.y * B.z - A.z * B.y;
                 const double cy = A.z * B.x - A.x * B.z;
@@ -384,10 +393,12 @@ __global__ __launch_bounds__(256) void k_assemble_vre(const int32_t *__restrict_
                 const double c2 = cx * cx + cy * cy + cz * cz;
                 const double proj = ex[s] * (A.x * A.iv - B.x * B.iv) + ey[s] * (A.y * A.iv - B.y * B.iv) +
                                     ez[s] * (A.z * A.iv - B.z * B.iv);
-                double v = 0.0;
-                if (c2 > cut2)
-                    v = proj * fp_recip(4.0 * FP_PI * (c2 + off2)) * (cx * st[3][t] + cy * st[4][t] + cz * st[5][t]);
-                se[tt][s * VRE_P + tid] = v;
+                const double w = proj * fp_recip(k4 * (c2 + off2)) * (cx * st[3][t] + cy * st[4][t] + cz * st[5][t]);
+                v[tt] = c2 > cut2 ? w : 0.0;
+            }
+            if (eok[s]) {
+#pragma unroll
+                for (int tt = 0; tt < TT; ++tt) se[tt][s * VRE_P + tid] = v[tt];
             }
         }
         __syncthreads();
@@ -396,13 +407,15 @@ __global__ __launch_bounds__(256) void k_assemble_vre(const int32_t *__restrict_
 #pragma unroll
             for (int tt = 0; tt < TT; ++tt) {
                 const int t = t0 + tt;
-                if (t >= iend) break;
+                // every stored edge term is finite (the select above), so the unused references
+                // (sign 0, slot 0) add exactly zero
                 double v = 0.0;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) v += rs[k] != 0.0 ? rs[k] * se[tt][ri[k]] : 0.0;
-                const int64_t at = (i0 + t) * ld + col0 + col;
-                G[at] = TO(v);
-                if (G2) G2[at] = float(v);
+                for (int k = 0; k < 4; ++k) v += rs[k] * se[tt][ri[k]];
+                if (t < iend) {
+                    g[(int64_t)t * ld] = TO(v);
+                    if (g2) g2[(int64_t)t * ld] = float(v);
+                }
             }
         }
     }

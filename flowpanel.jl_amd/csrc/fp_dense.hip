@@ -72,9 +72,14 @@ __device__ __forceinline__ void ld16(const T *__restrict__ p, bool full, bool ve
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t k, const T *__restrict__ A,
                                                   int64_t lda, const T *__restrict__ B, int64_t ldb,
-                                                  T *__restrict__ C, int64_t ldc, int vec)
+                                                  T *__restrict__ C, int64_t ldc, int vec, int64_t sA = 0,
+                                                  int64_t sB = 0, int64_t sC = 0)
 {
     typedef typename MfmaOp<T>::acc acc_t;
+    // strided batch: grid row y runs product y (all products the same shape)
+    A += (int64_t)blockIdx.y * sA;
+    B += (int64_t)blockIdx.y * sB;
+    C += (int64_t)blockIdx.y * sC;
     __shared__ T As[GB_K][GB_M + GB_PAD];
     __shared__ T Bs[GB_K][GB_N + GB_PAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -281,6 +286,24 @@ int fp_gemm_sub(fp_ctx *ctx, int64_t m, int64_t n, int64_t k, const T *A, int64_
                 int64_t ldc)
 {
     return gemm_sub<T>(ctx, m, n, k, A, lda, B, ldb, C, ldc);
+}
+
+// C_b -= A_b B_b for b < batch, operand b at element offset b * s? from the
+// first (FP32): one launch for a level of small products.
+int fp_gemm_sub_batched_f32(fp_ctx *ctx, int64_t m, int64_t n, int64_t k, const float *A, int64_t lda, int64_t sA,
+                            const float *B, int64_t ldb, int64_t sB, float *C, int64_t ldc, int64_t sC, int batch)
+{
+    if (m <= 0 || n <= 0 || k <= 0 || batch <= 0) return FP_OK;
+    if (batch > 65535) return fp_fail(ctx, FP_ERR_UNSUPPORTED, "batched gemm: too many products");
+    const int64_t ntiles = ((m + GB_M - 1) / GB_M) * ((n + GB_N - 1) / GB_N);
+    const int64_t grid = (ntiles + GB_XCD - 1) / GB_XCD * GB_XCD;
+    const int vec = (((uintptr_t)A | (uintptr_t)B) & 15) == 0 && (lda * 4) % 16 == 0 && (ldb * 4) % 16 == 0 &&
+                    (sA * 4) % 16 == 0 && (sB * 4) % 16 == 0;
+    ProfScope ps(ctx, "gemm_f32");
+    hipLaunchKernelGGL(k_gemm_sub<float>, dim3((unsigned)grid, (unsigned)batch), dim3(256), 0, ctx->stream, m, n, k,
+                       A, lda, B, ldb, C, ldc, vec, sA, sB, sC);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
 }
 template int fp_gemm_sub<double>(fp_ctx *, int64_t, int64_t, int64_t, const double *, int64_t, const double *, int64_t,
                                  double *, int64_t);
@@ -1311,6 +1334,21 @@ int fp_fill(fp_ctx *ctx, double *x, int64_t n, double v)
 {
     if (n <= 0) return FP_OK;
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, x, n, v);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+__global__ void k_fill_f32(float *__restrict__ x, int64_t n, float v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = v;
+}
+
+int fp_fill_f32(fp_ctx *ctx, float *x, int64_t n, float v)
+{
+    if (n <= 0) return FP_OK;
+    const int64_t g = std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)g), dim3(256), 0, ctx->stream, x, n, v);
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }

@@ -36,6 +36,20 @@ __device__ __forceinline__ double fp_sqrt(double x)
 }
 __device__ __forceinline__ float fp_sqrt(float x) { return sqrtf(x); }
 
+// sqrt(x) for 0 <= x < 2^1000 without a branch, for distances to which a
+// kernel offset is added: the hardware reciprocal square root of
+// max(x, 2^-1000) refined by two Newton steps, times x (a few ulp for
+// x > 2^-1000; 0 for x = 0; below 2^-1000 a value under 2^-500).
+__device__ __forceinline__ double fp_sqrt_nb(double x)
+{
+    const double xs = fmax(x, 0x1p-1000);
+    double y = __builtin_amdgcn_rsq(xs);
+    const double h = 0.5 * xs;
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return x * y;
+}
+
 // Regularised Biot-Savart segment a -> b (src/FLOWPanel_elements.jl:345-428),
 // velocity per unit strength added to u.  e = b - a.
 template <typename T>

@@ -223,22 +223,22 @@ __global__ __launch_bounds__(256) void k_field_trail(const double *__restrict__ 
             if (q < cnt) st[c][q] = tr[(int64_t)c * S + t0 + q];
         }
         __syncthreads();
+        // straight-line body (a select instead of branches, so that the unrolled entries
+        // interleave): a trail start (gamma = 0) and a segment below the cutoff add zero
+#pragma unroll 2
         for (int q = 0; q < cnt; ++q) {
             const double rx = X - st[0][q], ry = Y - st[1][q], rz = Z - st[2][q];
-            const double iv = fp_recip(fp_sqrt(rx * rx + ry * ry + rz * rz) + off);
+            const double iv = fp_recip(fp_sqrt_nb(rx * rx + ry * ry + rz * rz) + off);
             const double g = st[6][q];
-            if (g != 0.0) {   // the same entry for every lane: a uniform branch
-                const double cx = rpy * rz - rpz * ry;
-                const double cy = rpz * rx - rpx * rz;
-                const double cz = rpx * ry - rpy * rx;
-                const double c2 = cx * cx + cy * cy + cz * cz;
-                const double proj = st[3][q] * (rpx * ivp - rx * iv) + st[4][q] * (rpy * ivp - ry * iv) +
-                                    st[5][q] * (rpz * ivp - rz * iv);
-                if (c2 > cut2) {
-                    const double aux = g * proj * fp_recip(4.0 * FP_PI * (c2 + off2));
-                    ux += aux * cx; uy += aux * cy; uz += aux * cz;
-                }
-            }
+            const double cx = rpy * rz - rpz * ry;
+            const double cy = rpz * rx - rpx * rz;
+            const double cz = rpx * ry - rpy * rx;
+            const double c2 = cx * cx + cy * cy + cz * cz;
+            const double proj = st[3][q] * (rpx * ivp - rx * iv) + st[4][q] * (rpy * ivp - ry * iv) +
+                                st[5][q] * (rpz * ivp - rz * iv);
+            const double a = g * proj * fp_recip(4.0 * FP_PI * (c2 + off2));
+            const double aux = (g != 0.0 && c2 > cut2) ? a : 0.0;
+            ux += aux * cx; uy += aux * cy; uz += aux * cz;
             rpx = rx; rpy = ry; rpz = rz; ivp = iv;
         }
     }

@@ -207,6 +207,8 @@ struct AssemblyItem {
 int fp_launch_geometry(fp_ctx *ctx, fp_body *b);
 // AIC column block of one body into a row-major matrix (T = float|double); with
 // G32 set, the same entries rounded to FP32 go to G32 (same ld) as well
+int fp_gemm_sub_batched_f32(fp_ctx *ctx, int64_t m, int64_t n, int64_t k, const float *A, int64_t lda, int64_t sA,
+                            const float *B, int64_t ldb, int64_t sB, float *C, int64_t ldc, int64_t sC, int batch);
 int fp_body_build_edge_blocks(fp_ctx *ctx, fp_body *b);
 int fp_body_build_trails(fp_ctx *ctx, fp_body *b);
 int fp_launch_field_trails(fp_ctx *ctx, const fp_body *b, const double *strength, const TargetView &tv, int mode,
@@ -246,7 +248,9 @@ int fp_rbt_matrix(fp_ctx *ctx, const Tin *A, int64_t n, int64_t lda, float *Ar, 
 int fp_rbt_vec(fp_ctx *ctx, const double *in, int64_t ldin, double *out, int64_t ldout, int64_t n, int64_t k,
                const double *d_W, int transpose_u);
 size_t fp_getrf_nopiv_work_bytes();
-int fp_getrf_nopiv_f32(fp_ctx *ctx, float *A, int64_t n, int64_t ld, int *info_dev, void *work);
+int fp_getrf_nopiv_f32(fp_ctx *ctx, float *A, int64_t n, int64_t ld, int *info_dev, void *work, float *YL = nullptr,
+                       float *YU = nullptr);
+int fp_trtri_nopiv_f32(fp_ctx *ctx, const float *A, int64_t n, int64_t ld, float *YL, float *YU, float *T);
 // device address of fp_uind_noalloc's scratch array, queried once per GPU (fp_noalloc.hip)
 void fp_noalloc_init(fp_ctx *ctx);
 
@@ -305,6 +309,7 @@ int fp_residual_update(fp_ctx *ctx, const double *b, double *r, int64_t n);   //
 int fp_dot(fp_ctx *ctx, const double *x, const double *y, int64_t n, double *d_out);
 int fp_axpy(fp_ctx *ctx, double a, const double *x, double *y, int64_t n);
 int fp_fill(fp_ctx *ctx, double *x, int64_t n, double v);   // x[:] = v (a kernel: safe inside graphs)
+int fp_fill_f32(fp_ctx *ctx, float *x, int64_t n, float v);
 
 // multi-dot out[c] = sum_i V[c*ldv + i] w[i] (part: 256*nv doubles)
 int fp_mdot(fp_ctx *ctx, const double *V, int64_t ldv, int nv, const double *w, int64_t n, double *part,

@@ -44,17 +44,22 @@ __device__ __forceinline__ float bcast(float v, int lane)
 // Diagonal block at (k0, k0): LU without pivoting in place (rows beyond n are
 // the identity), then Linv = L11^-1 (unit lower) and Uinv = U11^-1, 64 x 64
 // row-major.  Wave 0 holds row `lane` of the block in registers and factors
-// it; the broadcast of row j is one v_readlane per entry (j is a compile-time
-// index, every loop is unrolled).  Then wave 0 inverts U from its registers
-// while wave 1 inverts L from the copy in LDS.
+// it; at step j lane j writes its row to LDS and every lane reads it back as a
+// broadcast (LDS operations of one wave complete in order, so no barrier is
+// needed inside the wave; the loops are unrolled, j is a compile-time index).
+// Then wave 0 inverts U from its registers while wave 1 inverts L from the
+// copy in LDS, each broadcasting the finished row of its inverse the same way.
 __global__ __launch_bounds__(128) void k_nopiv_diag(float *__restrict__ A, int64_t ld, int64_t n, int64_t k0,
                                                    float *__restrict__ Linv, float *__restrict__ Uinv,
-                                                   int *__restrict__ info)
+                                                   int *__restrict__ info, float *__restrict__ YL,
+                                                   float *__restrict__ YU)
 {
     __shared__ float lu[NPB][NPB + 1];
+    __shared__ __align__(16) float brow[2][NPB];   // per-wave broadcast row
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int bs = (int)min((int64_t)NPB, n - k0);
     float a[NPB], x[NPB];
+    float *br = brow[wave];
     if (wave == 0) {
         float *row = A + (k0 + lane) * ld + k0;
         if (bs == NPB) {
@@ -70,16 +75,19 @@ __global__ __launch_bounds__(128) void k_nopiv_diag(float *__restrict__ A, int64
         bool bad = false;
 #pragma unroll
         for (int j = 0; j < NPB; ++j) {
-            const float piv = bcast(a[j], j);
-            bad = bad || !(fabsf(piv) > 0.f) || !(fabsf(piv) <= 3.4e38f);
-            const float l = a[j] / piv;
-            const bool below = lane > j;
-            a[j] = below ? l : a[j];
+            if (lane == j) {
 #pragma unroll
-            for (int c = j + 1; c < NPB; ++c) {
-                const float s = bcast(a[c], j);
-                a[c] = below ? fmaf(-l, s, a[c]) : a[c];
+                for (int c = j; c < NPB; ++c) br[c] = a[c];
             }
+            __builtin_amdgcn_wave_barrier();
+            const float piv = br[j];
+            bad = bad || !(fabsf(piv) > 0.f) || !(fabsf(piv) <= 3.4e38f);
+            // rows at and above j take a zero multiplier (unconditional updates, no branches)
+            const float l = lane > j ? a[j] / piv : 0.f;
+            a[j] = lane > j ? l : a[j];
+#pragma unroll
+            for (int c = j + 1; c < NPB; ++c) a[c] = fmaf(-l, br[c], a[c]);
+            __builtin_amdgcn_wave_barrier();
         }
         if (bad && lane == 0) atomicCAS(info, 0, (int)(k0 + 1));
         if (lane < bs) {
@@ -104,17 +112,28 @@ __global__ __launch_bounds__(128) void k_nopiv_diag(float *__restrict__ A, int64
         float dii = 1.f;
 #pragma unroll
         for (int c = NPB - 1; c >= 0; --c) {
-            const float rc = 1.f / bcast(a[c], c);
+            const float rc = 1.f / lu[c][c];
             const float coef = lane < c ? a[c] * rc : 0.f;
             dii = lane == c ? a[c] : dii;
+            if (lane == c) {
 #pragma unroll
-            for (int q = c; q < NPB; ++q) x[q] = fmaf(-coef, bcast(x[q], c), x[q]);
+                for (int q = c; q < NPB; ++q) br[q] = x[q];
+            }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int q = c; q < NPB; ++q) x[q] = fmaf(-coef, br[q], x[q]);
+            __builtin_amdgcn_wave_barrier();
         }
         const float rd = 1.f / dii;
 #pragma unroll
         for (int c = 0; c < NPB; c += 4)
             *reinterpret_cast<float4 *>(Uinv + lane * NPB + c) =
                 make_float4(x[c] * rd, x[c + 1] * rd, x[c + 2] * rd, x[c + 3] * rd);
+        if (YU && lane < bs) {   // -U11^-1 into the diagonal block of the full inverse (ld as A)
+#pragma unroll
+            for (int c = 0; c < NPB; ++c)
+                if (c < bs) YU[(k0 + lane) * ld + k0 + c] = -(x[c] * rd);
+        }
     } else {
         // L^-1 (unit lower): rows top-down; X_i -= l_ic X_c for i > c
 #pragma unroll
@@ -122,12 +141,23 @@ __global__ __launch_bounds__(128) void k_nopiv_diag(float *__restrict__ A, int64
 #pragma unroll
         for (int c = 0; c < NPB; ++c) {
             const float coef = a[c];
+            if (lane == c) {
 #pragma unroll
-            for (int q = 0; q <= c; ++q) x[q] = fmaf(-coef, bcast(x[q], c), x[q]);
+                for (int q = 0; q <= c; ++q) br[q] = x[q];
+            }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int q = 0; q <= c; ++q) x[q] = fmaf(-coef, br[q], x[q]);
+            __builtin_amdgcn_wave_barrier();
         }
 #pragma unroll
         for (int c = 0; c < NPB; c += 4)
             *reinterpret_cast<float4 *>(Linv + lane * NPB + c) = make_float4(x[c], x[c + 1], x[c + 2], x[c + 3]);
+        if (YL && lane < bs) {   // -L11^-1 likewise
+#pragma unroll
+            for (int c = 0; c < NPB; ++c)
+                if (c < bs) YL[(k0 + lane) * ld + k0 + c] = -x[c];
+        }
     }
 }
 
@@ -355,7 +385,8 @@ size_t fp_getrf_nopiv_work_bytes() { return (size_t)2 * NPB * NPB * sizeof(float
 // LU without pivoting of the n x n FP32 matrix A (row-major, ld a multiple of 4);
 // info_dev receives 1 + the first row of the diagonal block whose pivot was zero
 // or not finite (0 if none).
-int fp_getrf_nopiv_f32(fp_ctx *ctx, float *A, int64_t n, int64_t ld, int *info_dev, void *work)
+int fp_getrf_nopiv_f32(fp_ctx *ctx, float *A, int64_t n, int64_t ld, int *info_dev, void *work, float *YL,
+                       float *YU)
 {
     if (n <= 0) return FP_OK;
     if (ld % 4) return fp_fail(ctx, FP_ERR_ARG, "internal: no-pivot LU needs a leading dimension divisible by 4");
@@ -364,7 +395,8 @@ int fp_getrf_nopiv_f32(fp_ctx *ctx, float *A, int64_t n, int64_t ld, int *info_d
     for (int64_t k0 = 0; k0 < n; k0 += NPB) {
         {
             ProfScope ps(ctx, "lu_nopiv_diag");
-            hipLaunchKernelGGL(k_nopiv_diag, dim3(1), dim3(128), 0, ctx->stream, A, ld, n, k0, Linv, Uinv, info_dev);
+            hipLaunchKernelGGL(k_nopiv_diag, dim3(1), dim3(128), 0, ctx->stream, A, ld, n, k0, Linv, Uinv, info_dev,
+                               YL, YU);
         }
         const int64_t rest = n - k0 - NPB;
         if (rest <= 0) break;
@@ -378,6 +410,58 @@ int fp_getrf_nopiv_f32(fp_ctx *ctx, float *A, int64_t n, int64_t ld, int *info_d
         int st = fp_gemm_sub<float>(ctx, rest, rest, NPB, A + (k0 + NPB) * ld + k0, ld, A + k0 * ld + k0 + NPB, ld,
                                     A + (k0 + NPB) * ld + k0 + NPB, ld);
         if (st) return st;
+    }
+    return FP_OK;
+}
+
+// Explicit inverses of the no-pivot factors, for refinement sweeps that then cost
+// two GEMVs instead of two blocked triangular solves.  YL = -L^-1 and YU = -U^-1
+// (row-major, ld as A; zero outside their triangles on entry, the diagonal 64 x 64
+// blocks written by fp_getrf_nopiv_f32).  With Y = -X the block recurrences
+//   lower:  Y_JI = Y_JJ L_JI Y_II,   upper:  Y_IJ = Y_II U_IJ Y_JJ   (I before J)
+// are two "C -= A B" products each through T = -L_JI Y_II (resp. -U_IJ Y_JJ);
+// blocks merge pairwise, 64 -> 128 -> ..., one batched launch per product and
+// level.  U^-1 L^-1 = YU YL: the signs cancel in the solve.  T: n x ld scratch.
+int fp_trtri_nopiv_f32(fp_ctx *ctx, const float *A, int64_t n, int64_t ld, float *YL, float *YU, float *T)
+{
+    ProfScope ps(ctx, "lu_nopiv_trtri");
+    int st;
+    for (int64_t s = NPB; s < n; s *= 2) {
+        const int64_t P = n / (2 * s), rem = n - 2 * s * P;
+        const int64_t step = 2 * s * ld + 2 * s;   // between consecutive pairs
+        if (P > 0) {
+            if ((st = fp_fill_f32(ctx, T, P * s * ld, 0.f))) return st;
+            // lower: T_p = -L_JI Y_II, Y_JI -= Y_JJ T_p
+            if ((st = fp_gemm_sub_batched_f32(ctx, s, s, s, A + s * ld, ld, step, YL, ld, step, T, ld, s * ld, (int)P)))
+                return st;
+            if ((st = fp_gemm_sub_batched_f32(ctx, s, s, s, YL + s * ld + s, ld, step, T, ld, s * ld, YL + s * ld, ld,
+                                              step, (int)P)))
+                return st;
+            if ((st = fp_fill_f32(ctx, T, P * s * ld, 0.f))) return st;
+            // upper: T_p = -U_IJ Y_JJ, Y_IJ -= Y_II T_p
+            if ((st = fp_gemm_sub_batched_f32(ctx, s, s, s, A + s, ld, step, YU + s * ld + s, ld, step, T, ld, s * ld,
+                                              (int)P)))
+                return st;
+            if ((st = fp_gemm_sub_batched_f32(ctx, s, s, s, YU, ld, step, T, ld, s * ld, YU + s, ld, step, (int)P)))
+                return st;
+        }
+        if (rem > s) {   // the last pair: I = [r, r + s), J = [r + s, n)
+            const int64_t r = 2 * s * P, sj = rem - s, rj = r + s;
+            if ((st = fp_fill_f32(ctx, T, s * ld, 0.f))) return st;
+            if ((st = fp_gemm_sub_batched_f32(ctx, sj, s, s, A + rj * ld + r, ld, 0, YL + r * ld + r, ld, 0, T, ld, 0,
+                                              1)))
+                return st;
+            if ((st = fp_gemm_sub_batched_f32(ctx, sj, s, sj, YL + rj * ld + rj, ld, 0, T, ld, 0, YL + rj * ld + r, ld,
+                                              0, 1)))
+                return st;
+            if ((st = fp_fill_f32(ctx, T, s * ld, 0.f))) return st;
+            if ((st = fp_gemm_sub_batched_f32(ctx, s, sj, sj, A + r * ld + rj, ld, 0, YU + rj * ld + rj, ld, 0, T, ld,
+                                              0, 1)))
+                return st;
+            if ((st = fp_gemm_sub_batched_f32(ctx, s, sj, s, YU + r * ld + r, ld, 0, T, ld, 0, YU + r * ld + rj, ld, 0,
+                                              1)))
+                return st;
+        }
     }
     return FP_OK;
 }

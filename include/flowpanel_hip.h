@@ -214,7 +214,8 @@ int fp_batch_autotune_get(const fp_batch *batch, int k, int *tile_targets, int *
  * device-resident AIC of the bodies with their stored wake directions. */
 int fp_assemble(fp_ctx *ctx, fp_body *const *bodies, int nbodies, int precision, fp_mat **out);
 /* Host only, no device: the column blocks the vortex-ring assembly of a triangle/quad body uses (mesh
- * node and edge terms computed once per target and block, shared by the block's panels).  Returns the
+ * node and edge terms computed once per target and block, shared by the block's panels; at most 256
+ * panels, 384 nodes and 640 edges per block).  Returns the
  * number of blocks (negative FP_ERR_* on bad input); when tab is not null and cap_blocks is at least
  * that number, writes 2048 int32 per block: 512 node references (panel * 8 + vertex, -1 = none), 768
  * edges (local node a | b << 16, -1 = none), 256 columns (-1 = none), 256 pairs of packed 16-bit edge

@@ -8,7 +8,8 @@ restatement of the kernel's arithmetic over those tables must reproduce the
 oracle's AIC (src/FLOWPanel_liftingbody.jl:455-572, panel by panel through
 U_vortexring, src/FLOWPanel_elements.jl:442-467) to 1e-12 of max|G| on quads,
 triangles, a mix, a sphere with collapsed pole edges and disjoint panels (the
-node limit); every column sits in exactly one block and the block limits hold.
+node limit); every column sits in exactly one block and the packing limits
+(256 panels, 384 nodes, 640 edges) hold.
 A planted mistake in the tables (a flipped traversal sign, a swapped edge end,
 a dropped edge) fails the comparison.
 """
@@ -131,8 +132,8 @@ def test_blocks_reproduce_the_oracle_aic(fp, geo, orc, case):
           (case, G.shape[1], len(tab), terms[0] / G.shape[1], terms[1] / G.shape[1], err))
     assert err < 1e-12
     if case == "disjoint":
-        # four nodes per panel: the 512-node limit packs 128 panels per block
-        assert len(tab) == int(np.ceil(G.shape[1] / 128))
+        # four nodes per panel: the 384-node packing limit gives 96 panels per block
+        assert len(tab) == int(np.ceil(G.shape[1] / 96))
 
 
 def test_block_limits_and_sharing_on_a_large_loft(fp, geo, orc):
@@ -141,7 +142,8 @@ def test_block_limits_and_sharing_on_a_large_loft(fp, geo, orc):
     n = len(g.cell_ptr) - 1
     cols = tab[:, OFF_COL:OFF_COL + P]
     assert np.array_equal(np.sort(cols[cols >= 0]), np.arange(n))
-    assert (tab[:, OFF_NODE:OFF_NODE + NMAX] >= 0).sum(1).max() <= NMAX
+    assert (tab[:, OFF_NODE:OFF_NODE + NMAX] >= 0).sum(1).max() <= 384
+    assert (tab[:, OFF_EDGE:OFF_EDGE + EMAX] >= 0).sum(1).max() <= 640
     # the point of the blocks: far fewer node and edge terms than 4 + 4 per panel
     assert terms[0] / n < 1.5 and terms[1] / n < 2.6, terms / n
     # a sample of targets against the oracle

@@ -167,3 +167,46 @@ def test_lu32_pivoted_and_randomized_agree(fp, geo, monkeypatch):
     monkeypatch.setenv("FLOWPANEL_LU32_PIVOTED", "1")
     st_p = fp.solve(body, U, D, D, solver=fp.solve_lu32_ir, solver_optargs={"tol": 1e-12})
     assert rel(body.strength, g_r) < 1e-10, (st_r, st_p)
+
+
+# With four or more right-hand sides the refinement corrections of the randomized
+# path apply explicit inverses of the no-pivot factors (-L^-1, -U^-1 built level by
+# level on MFMA): two GEMVs per correction (FLOWPANEL_LU32_INVERSE=1 forces them,
+# =0 forces the blocked triangular solves, the reference point here).  Sizes
+# straddle the 64-row blocks, the pairwise merges (ragged last pairs) and the pad
+# to a multiple of 4.
+@pytest.mark.parametrize("n", [5, 64, 130, 300, 1031, 2500])
+def test_lu32_explicit_inverses_match_triangular_solves(fp, n, monkeypatch):
+    rng = np.random.default_rng(9100 + n)
+    A = rng.standard_normal((n, n)) + 2.0 * np.sqrt(n) * np.eye(n)
+    B = rng.standard_normal((n, 3))
+    X = np.zeros((n, 3))
+    monkeypatch.setenv("FLOWPANEL_LU32_INVERSE", "1")
+    fp.solve_lu32_ir(X, A, B, tol=1e-12)
+    st_inv = dict(fp.last_stats)
+    monkeypatch.setenv("FLOWPANEL_LU32_INVERSE", "0")
+    X2 = np.zeros((n, 3))
+    fp.solve_lu32_ir(X2, A, B, tol=1e-12)
+    st_trs = dict(fp.last_stats)
+    ref = np.linalg.solve(A, B)
+    assert rel(X, ref) < 1e-11 and rel(X2, ref) < 1e-11, (st_inv, st_trs)
+    # the inverses are as good a preconditioner as the substitutions
+    assert st_inv["refinements"] <= st_trs["refinements"] + 1, (st_inv, st_trs)
+
+
+def test_lu32_explicit_inverses_on_a_wing(fp, geo, orc, monkeypatch):
+    monkeypatch.setenv("FLOWPANEL_LU32_INVERSE", "1")
+    g, sh = geo.loft_wing(n_chord=40, n_span=30, span=6.0, sweep_deg=25.0, taper=0.6)
+    body = fp.RigidWakeBody(g, sh)
+    V = fp.direction(4.2)
+    U = np.tile(V[:, None], (1, body.ncells))
+    D = np.tile(V[:, None], (1, body.nsheddings))
+    st = fp.solve(body, U, D, D, solver=fp.solve_lu32_ir, solver_optargs={"tol": 1e-12})
+    import types
+    ob = types.SimpleNamespace(nodes=g.nodes, cell_ptr=g.cell_ptr, cell_idx=g.cell_idx, ncells=body.ncells,
+                               element="vortexring", shedding=body.shedding, nsheddings=body.nsheddings,
+                               kerneloffset=body.kerneloffset, kernelcutoff=body.kernelcutoff, CPoffset=body.CPoffset,
+                               charlen_kind=0)
+    Go, normals, _ = orc.assemble([ob], [D], [D])
+    xo = orc.solve_lu(Go, orc.rhs(U, normals))
+    assert rel(body.strength, xo) < 1e-10, st
