@@ -61,8 +61,9 @@ def cpu_baseline(fp, body, U, D, full=False):
 
     Assembly: the oracle's `_G_Uvortexring!` restatement (OpenMP over panel
     columns) of the whole 30,000 x 30,000 FP64 AIC with its wake columns.
-    Solve: LAPACK getrf + getrs (scipy/OpenBLAS, all host threads) on an n x n
-    leading block of that AIC, scaled by (N/n)^3 (n = N with ``full``).
+    Solve: LAPACK getrf + getrs (scipy/OpenBLAS, all host threads) of that AIC
+    (``full``, the default: about 10-15 s on 16 threads), or of its 10,000 x
+    10,000 leading block scaled by (N/n)^3.
     """
     import types
     import scipy.linalg as sla
@@ -437,7 +438,8 @@ def main():
     ap.add_argument("--no-vendor", action="store_true",
                     help="skip the rocSOLVER getrf/getrs anchor on the same AICs (a child process started first)")
     ap.add_argument("--no-c2", action="store_true", help="skip the 4,800-panel single-freestream LU leg")
-    ap.add_argument("--cpu-full", action="store_true", help="CPU baseline LU at full size (about a minute)")
+    ap.add_argument("--cpu-sample", action="store_true",
+                    help="CPU baseline LU on a 10,000 x 10,000 block scaled by (N/n)^3 instead of the full 30,000")
     ap.add_argument("--mixed-only", action="store_true", help="profiling runs: mixed steps only (no FP64 LU, no CPU)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="profiling runs: also run a plain 1 GiB device copy to calibrate HBM counters")
@@ -498,8 +500,8 @@ def main():
     ctx.profile(True)
     run(fp.solve_gmres_ir, mixed_opts)
     ctx.synchronize()
-    prof = {k: ctx.profile_get(k) for k in ("gemv_f32", "gemv_f64", "assemble_vr_f64", "assemble_wake",
-                                           "bj_invert", "bj_apply", "coarse_apply")}
+    prof = {k: ctx.profile_get(k) for k in ("gemv_f32", "gemv_f64", "assemble_vre_f64", "assemble_vr_f64",
+                                           "assemble_wake", "bj_invert", "bj_apply", "coarse_apply")}
     ctx.profile(False)
     gam_mixed = body.strength.copy()
     Us_m = fp.calcfield_U(body, body)
@@ -525,7 +527,8 @@ def main():
         t64.append((time.perf_counter() - t) * 1e3)
         log("bench: fp64 step %d: %.1f ms %s" % (k, t64[-1], st64))
     t_fp64 = min(t64[1:]) if len(t64) > 1 else t64[0]
-    prof64 = {k: ctx.profile_get(k) for k in ("assemble_vr_f64", "gemm_f64", "panel_tslu", "trsm", "getrs")}
+    prof64 = {k: ctx.profile_get(k) for k in ("assemble_vre_f64", "assemble_vr_f64", "gemm_f64", "panel_tslu", "trsm",
+                                             "getrs")}
     ctx.profile(False)
     gam64 = body.strength.copy()
     fp.calcfield_U(body, body)
@@ -545,6 +548,22 @@ def main():
               "cp_err_max": float(np.abs(fp.calcfield_Cp(body, 1.0) - cp64).max()),
               "speedup_vs_fp64": round(t_fp64 / t_strict, 3)}
     log("bench: mixed at tol 1e-10: %.1f ms" % t_strict)
+    # the mixed solve holding only the FP32 AIC (3.6 GB): FP64 residuals recomputed matrix-free (edge trails)
+    mf_opts = dict(mixed_opts, residual_mode=0)
+    run(fp.solve_gmres_ir, mf_opts)
+    ctx.synchronize()
+    ctx.profile(True)
+    t = time.perf_counter()
+    st_mf = run(fp.solve_gmres_ir, mf_opts)
+    ctx.synchronize()
+    t_mf = (time.perf_counter() - t) * 1e3
+    mf_prof = {k: ctx.profile_get(k) for k in ("field_matvec_f64", "assemble_vre_f64", "gemv_f32")}
+    ctx.profile(False)
+    fp.calcfield_U(body, body)
+    matrix_free = {"ms": round(t_mf, 3), "stats": st_mf, "kernels": mf_prof, "aic_bytes": 4 * N * N,
+                   "cp_err_max": float(np.abs(fp.calcfield_Cp(body, 1.0) - cp64).max()),
+                   "speedup_vs_fp64": round(t_fp64 / t_mf, 3)}
+    log("bench: mixed with matrix-free FP64 residuals: %.1f ms" % t_mf)
     field = field_leg(fp, ctx, body, U, args.field) if args.field > 0 else None
 
     # roofline of the dominant kernel of the mixed step (FP32 GEMV of the Krylov matvec), timed in the timed steps
@@ -570,7 +589,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")
-        cpu = cpu_baseline(fp, body, U, D, full=args.cpu_full)
+        cpu = cpu_baseline(fp, body, U, D, full=not args.cpu_sample)
 
     line = {
         "metric": "AIC+solve time at 30k panels, mixed+refine vs FP64 @ Cp err<=1e-6; adjoint ms",
@@ -582,7 +601,7 @@ def main():
                    "panels": N, "shedding_edges": body.nsheddings},
         "fp64_lu_ms": round(t_fp64, 3), "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
         "cp_err_max": cp_err, "gamma_rel_err": gam_err, "tol": args.tol, "inner_tol": args.inner_tol,
-        "mixed_stats": stats[-1], "fp64_stats": st64, "mixed_tol_1e-10": strict,
+        "mixed_stats": stats[-1], "fp64_stats": st64, "mixed_tol_1e-10": strict, "mixed_matrix_free": matrix_free,
         "adjoint_ms": adj["adjoint_total_ms"] if adj else None, "adjoint": adj, "c2_wing4800": c2, "batch": bat,
         "field": field,
         "kernels": kernels, "kernels_fp64": kernels_fp64, "vendor_rocsolver": vendor, "roofline": roof,

@@ -196,6 +196,19 @@ class MultiBody:
         return self.fields[name]
 
 
+def generate_revolution_liftbody(bodytype, points, n_theta, axis_angle=270.0, **bodyoptargs):
+    """generate_revolution_liftbody(bodytype, points, NDIVS_theta; bodyoptargs)
+    (src/FLOWPanel_abstractliftingbody.jl:124-167): the contour ``points`` (k x 2:
+    axial position, radius; closed, first point at the trailing edge) revolved
+    about the x axis, split into triangles, every azimuthal strip shedding at the
+    trailing edge.  ``bodyoptargs`` go to ``bodytype`` (e.g. ``formulation=2`` for
+    the watertight least-squares RigidWakeBody{VortexRing, 2})."""
+    from .geometry import revolution_liftbody, triangulate
+    grid, sh = revolution_liftbody(points, n_theta, axis_angle)
+    grid, sh = triangulate(grid, sh)
+    return bodytype(grid, sh, **bodyoptargs)
+
+
 def _components(body):
     return body.bodies
 

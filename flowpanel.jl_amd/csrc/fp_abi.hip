@@ -229,18 +229,87 @@ struct Precond {
 // c = (Z^T A Z)^-1 Z^T v.
 struct BJPrecond : Precond {
     fp_ctx *ctx; int64_t n; int block;
-    DBuf inv, gjs, AZ, Einv, work, tc, r2, cinfo;
-    hipEvent_t fork = nullptr, join = nullptr;
+    DBuf inv, gjs, AZ, Einv, work, tc, r2, cinfo, binfo;
+    hipEvent_t fork = nullptr, join = nullptr, pre = nullptr, bj_done = nullptr, asm_done = nullptr;
     int cs = 0;
+    bool early = false;   // the block inverses were started beside the assembly (setup_early)
     BJPrecond(fp_ctx *c, int64_t n_, int bl) : ctx(c), n(n_), block(bl) {}
     ~BJPrecond()
     {
-        // side-stream work (coarse set-up, look-ahead Gauss-Jordan passes) must be
-        // over before its buffers can be handed out again
-        if (fork && ctx->side) (void)hipStreamSynchronize(ctx->side);
+        // side-stream work (coarse set-up, look-ahead Gauss-Jordan passes, early
+        // inverses) must be over before its buffers can be handed out again
+        if ((fork || asm_done) && ctx->side) (void)hipStreamSynchronize(ctx->side);
         if (ctx->side2) (void)hipStreamSynchronize(ctx->side2);
-        if (fork) (void)hipEventDestroy(fork);
-        if (join) (void)hipEventDestroy(join);
+        if (early && ctx->side3) (void)hipStreamSynchronize(ctx->side3);
+        for (hipEvent_t e : {fork, join, pre, bj_done, asm_done})
+            if (e) (void)hipEventDestroy(e);
+    }
+    // Block inverses beside the assembly: the diagonal blocks of the body's AIC are
+    // assembled into `inv` first (a few per cent of the entries, on the context
+    // stream), then inverted on the fourth stream while the context stream
+    // assembles the whole matrix.  finish_early adds the coarse space (it needs the
+    // whole matrix) and joins both streams into the context stream.
+    int setup_early(const fp_body *b, const TargetView &tv)
+    {
+        int st;
+        const int64_t nb = (n + block - 1) / block;
+        if ((st = dalloc(ctx, inv, (size_t)nb * block * block * sizeof(float)))) return st;
+        const size_t sf = fp_bj_gj_scratch_floats(n, block);
+        if (sf && (st = dalloc(ctx, gjs, sf * sizeof(float)))) return st;
+        if ((st = dalloc(ctx, binfo, sizeof(int)))) return st;
+        if ((st = fp_launch_assemble_diag(ctx, b, tv, block, inv.as<float>()))) return st;
+        if (!ctx->side3) FP_HIP(ctx, hipStreamCreateWithFlags(&ctx->side3, hipStreamNonBlocking));
+        if (!ctx->side2) FP_HIP(ctx, hipStreamCreateWithFlags(&ctx->side2, hipStreamNonBlocking));
+        FP_HIP(ctx, hipEventCreateWithFlags(&pre, hipEventDisableTiming));
+        FP_HIP(ctx, hipEventCreateWithFlags(&bj_done, hipEventDisableTiming));
+        FP_HIP(ctx, hipEventRecord(pre, ctx->stream));
+        FP_HIP(ctx, hipStreamWaitEvent(ctx->side3, pre, 0));
+        fp_ctx sc;
+        sc.device = ctx->device;
+        sc.num_cu = ctx->num_cu;
+        sc.stream = ctx->side3;
+        sc.side2 = ctx->side2;
+        st = fp_bj_gj_inverse<float>(&sc, inv.as<float>(), n, block, block, inv.as<float>(), binfo.as<int>(),
+                                     sf ? gjs.as<float>() : nullptr, nullptr, true, true);
+        sc.side2 = nullptr;
+        sc.stream = nullptr;
+        if (st) return fp_fail(ctx, st, sc.err.c_str());
+        FP_HIP(ctx, hipEventRecord(bj_done, ctx->side3));
+        early = true;
+        return FP_OK;
+    }
+    int finish_early(const void *A, int prec, int64_t ld, int coarse)
+    {
+        int st;
+        if (coarse > 0) {
+            cs = coarse;
+            const int nc = (int)((n + cs - 1) / cs);
+            if ((st = dalloc(ctx, AZ, (size_t)nc * n * sizeof(double))) ||
+                (st = dalloc(ctx, Einv, (size_t)nc * nc * sizeof(double))) ||
+                (st = dalloc(ctx, work, (size_t)2 * nc * nc * sizeof(double))) ||
+                (st = dalloc(ctx, tc, 2 * nc * sizeof(double))) || (st = dalloc(ctx, r2, n * sizeof(double))) ||
+                (st = dalloc(ctx, cinfo, sizeof(int))))
+                return st;
+            if (!ctx->side) FP_HIP(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+            FP_HIP(ctx, hipEventCreateWithFlags(&asm_done, hipEventDisableTiming));
+            FP_HIP(ctx, hipEventCreateWithFlags(&join, hipEventDisableTiming));
+            FP_HIP(ctx, hipEventRecord(asm_done, ctx->stream));
+            FP_HIP(ctx, hipStreamWaitEvent(ctx->side, asm_done, 0));
+            fp_ctx sc;
+            sc.device = ctx->device;
+            sc.num_cu = ctx->num_cu;
+            sc.stream = ctx->side;
+            st = prec == FP_F64 ? fp_coarse_setup<double>(&sc, (const double *)A, n, ld, cs, AZ.as<double>(),
+                                                          Einv.as<double>(), work.as<double>(), cinfo.as<int>())
+                                : fp_coarse_setup<float>(&sc, (const float *)A, n, ld, cs, AZ.as<double>(),
+                                                         Einv.as<double>(), work.as<double>(), cinfo.as<int>());
+            sc.stream = nullptr;
+            if (st) return fp_fail(ctx, st, sc.err.c_str());
+            FP_HIP(ctx, hipEventRecord(join, ctx->side));
+            FP_HIP(ctx, hipStreamWaitEvent(ctx->stream, join, 0));
+        }
+        FP_HIP(ctx, hipStreamWaitEvent(ctx->stream, bj_done, 0));
+        return FP_OK;
     }
     int setup(const void *A, int prec, int64_t ld, int coarse, int *info_dev)
     {
@@ -627,6 +696,8 @@ struct Solver {
     DBuf order, Aperm;
     std::unique_ptr<DenseOp<float>> A32op;
     Precond *prec() { return PM ? (Precond *)PM.get() : (Precond *)M.get(); }
+    // block inverses started beside the assembly (BJPrecond::setup_early)
+    void adopt(std::unique_ptr<BJPrecond> pre) { M = std::move(pre); }
     // FP32 LU of the randomized matrix U^T A V without pivoting (FP_SOLVER_LU32_IR):
     // Ar (n4 x ld4), the butterflies, identity row order for getrs, an n4 x k work vector
     bool rbt = false;
@@ -748,6 +819,10 @@ struct Solver {
     int bj_setup(const void *A, int prec, int blk)
     {
         int st;
+        if (M && M->early) {
+            if ((st = M->finish_early(A, prec, a.ld, o.coarse_block))) return st;
+            return bj_check(M->binfo.as<int>(), blk);
+        }
         if ((st = dalloc(ctx, info, sizeof(int)))) return st;
         if (o.precond_order == FP_ORDER_SPATIAL) {
             if (!a.geo || a.geo->M != a.n)
@@ -770,10 +845,15 @@ struct Solver {
         }
         M.reset(new BJPrecond(ctx, a.n, blk));
         if ((st = M->setup(A, prec, a.ld, o.coarse_block, info.as<int>()))) return st;
-        // zero pivots of the block inverses (main stream) and of the coarse Galerkin
-        // matrix (side stream, joined into the main stream by setup)
+        return bj_check(info.as<int>(), blk);
+    }
+    // zero pivots of the block inverses and of the coarse Galerkin matrix (side
+    // streams, joined into the main stream by the set-up)
+    int bj_check(const int *binfo_dev, int blk)
+    {
+        int st;
         int hinfo[2] = {0, 0};
-        FP_HIP(ctx, hipMemcpyAsync(&hinfo[0], info.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        FP_HIP(ctx, hipMemcpyAsync(&hinfo[0], binfo_dev, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         if (M->cinfo.p)
             FP_HIP(ctx, hipMemcpyAsync(&hinfo[1], M->cinfo.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1498,6 +1578,7 @@ int fp_ctx_trim(fp_ctx *ctx)
     FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->side) FP_HIP(ctx, hipStreamSynchronize(ctx->side));
     if (ctx->side2) FP_HIP(ctx, hipStreamSynchronize(ctx->side2));
+    if (ctx->side3) FP_HIP(ctx, hipStreamSynchronize(ctx->side3));
     fp_pool_trim(ctx);
     return FP_OK;
 }
@@ -1510,7 +1591,7 @@ void fp_ctx_destroy(fp_ctx *ctx)
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);
     }
-    for (hipStream_t sd : {ctx->side, ctx->side2})
+    for (hipStream_t sd : {ctx->side, ctx->side2, ctx->side3})
         if (sd) {
             (void)hipStreamSynchronize(sd);
             (void)hipStreamDestroy(sd);
@@ -1771,6 +1852,17 @@ static int assemble_into(fp_ctx *ctx, fp_body *const *bodies, int nb, const Targ
     return FP_OK;
 }
 
+// The block-Jacobi inverses of GMRES-IR can start before the AIC is whole when
+// their blocks are index ranges of one vortex-ring body: its diagonal blocks are
+// assembled on their own first (FLOWPANEL_BJ_EARLY=0 turns this off).
+static bool early_block_jacobi(const fp_solve_opts &o, fp_body *const *bodies, int nb)
+{
+    const char *e = getenv("FLOWPANEL_BJ_EARLY");
+    if (e && e[0] == '0') return false;
+    return o.solver == FP_SOLVER_GMRES_IR && o.precond_order == FP_ORDER_INDEX && nb == 1 &&
+           bodies[0]->element == FP_ELEM_VORTEXRING && o.precond_block <= 1024;
+}
+
 // Assembly of the bodies' AIC and solution for k right-hand sides (device,
 // column-major n x k).  The matrix storage follows the solver: FP64 for LU and
 // GMRES, FP64 + its FP32 narrowing for the refinement paths with a stored
@@ -1798,6 +1890,12 @@ static int solve_system(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_so
     const bool dual = !f64 && o.residual_mode == FP_RESID_STORED;
     DBuf A, A32buf;
     std::unique_ptr<DenseOp<double>> op64;
+    std::unique_ptr<BJPrecond> pre;
+    if (early_block_jacobi(o, bodies, nb)) {
+        // the block inverses run on their own stream beside the assembly
+        pre.reset(new BJPrecond(ctx, n, (int)std::min<int64_t>(o.precond_block > 0 ? o.precond_block : 512, n)));
+        if ((st = pre->setup_early(bodies[0], t.tv))) return st;
+    }
     if ((st = dalloc(ctx, A, (size_t)n * ld * ((f64 || dual) ? sizeof(double) : sizeof(float))))) return st;
     if (dual && (st = dalloc(ctx, A32buf, (size_t)n * ld * sizeof(float)))) return st;
     if ((st = assemble_into(ctx, bodies, nb, t.tv, (f64 || dual) ? FP_F64 : FP_F32, A.p, ld,
@@ -1820,6 +1918,7 @@ static int solve_system(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_so
     }
     a.nrhs = k;
     Solver solver(ctx, a, o);
+    if (pre) solver.adopt(std::move(pre));
     if ((st = solver.setup(s))) return st;
     return solver.solve_multi(rhs, x, k, s);
 }

@@ -32,6 +32,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <string>
 #include <unordered_map>
 
 namespace {
@@ -427,19 +428,27 @@ template <typename T>
 int fp_launch_assemble_edges(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G, int64_t ld, int64_t col0,
                              float *G32, int *used)
 {
-    constexpr int TM = 64, TT = 2;
     *used = 0;
     if (!b->d_eb || b->eb_nblocks == 0) return FP_OK;
     const char *env = getenv("FLOWPANEL_ASSEMBLY");
     if (env && strcmp(env, "panel") == 0) return FP_OK;
     *used = 1;
     if (tv.M == 0) return FP_OK;
-    dim3 grid((unsigned)b->eb_nblocks, (unsigned)((tv.M + TM - 1) / TM));
+    // targets per workgroup tile x targets per pass (FLOWPANEL_VRE_SHAPE, for measurement)
+    const char *shape = getenv("FLOWPANEL_VRE_SHAPE");
+    const std::string sh = shape ? shape : "64x2";
     ProfScope ps(ctx, sizeof(T) == 8 ? "assemble_vre_f64" : "assemble_vre_f32");
-    hipLaunchKernelGGL((k_assemble_vre<T, TM, TT>), grid, dim3(VRE_P), 0, ctx->stream, b->d_eb, b->d_pv, b->N, tv, G,
-                       ld, col0, G32, b->kernel_offset, b->kernel_cutoff);
-    FP_HIP(ctx, hipGetLastError());
-    return FP_OK;
+#define FP_VRE(TMV, TTV)                                                                                       \
+    if (sh == #TMV "x" #TTV) {                                                                                   \
+        dim3 grid((unsigned)b->eb_nblocks, (unsigned)((tv.M + TMV - 1) / TMV));                              \
+        hipLaunchKernelGGL((k_assemble_vre<T, TMV, TTV>), grid, dim3(VRE_P), 0, ctx->stream, b->d_eb, b->d_pv, \
+                           b->N, tv, G, ld, col0, G32, b->kernel_offset, b->kernel_cutoff);                    \
+        FP_HIP(ctx, hipGetLastError());                                                                        \
+        return FP_OK;                                                                                          \
+    }
+    FP_VRE(64, 2) FP_VRE(128, 2) FP_VRE(32, 2) FP_VRE(64, 4) FP_VRE(128, 4) FP_VRE(64, 1)
+#undef FP_VRE
+    return fp_fail(ctx, FP_ERR_ARG, "FLOWPANEL_VRE_SHAPE: one of 64x2, 128x2, 32x2, 64x4, 128x4, 64x1");
 }
 template int fp_launch_assemble_edges<double>(fp_ctx *, const fp_body *, const TargetView &, double *, int64_t,
                                               int64_t, float *, int *);

@@ -1650,7 +1650,7 @@ size_t fp_bj_gj_scratch_floats(int64_t n, int block)
 // everything in order on the context stream).
 template <typename T>
 int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *inv, int *info_dev,
-                     float *scratch, hipEvent_t copied, bool lookahead)
+                     float *scratch, hipEvent_t copied, bool lookahead, bool prefilled)
 {
     if (block > 1024) return fp_fail(ctx, FP_ERR_ARG, "block-Jacobi block larger than 1024");
     const int nb = (int)((n + block - 1) / block);
@@ -1659,8 +1659,12 @@ int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, 
     ProfScope ps(ctx, "bj_invert");
     if (block <= GJ_CHUNK) {
         if (copied) FP_HIP(ctx, hipEventRecord(copied, ctx->stream));
-        hipLaunchKernelGGL((k_bj_gj<T, GJ_TG>), dim3(nb), dim3(GJ_TG * GJ_TG), 0, ctx->stream, A, (int64_t)block * (ld + 1), ld,
-                           inv, bb, (int64_t)block, n, block, 0, -1, info_dev);
+        if (prefilled)
+            hipLaunchKernelGGL((k_bj_gj<float, GJ_TG>), dim3(nb), dim3(GJ_TG * GJ_TG), 0, ctx->stream, (const float *)inv,
+                               bb, (int64_t)block, inv, bb, (int64_t)block, n, block, 0, -1, info_dev);
+        else
+            hipLaunchKernelGGL((k_bj_gj<T, GJ_TG>), dim3(nb), dim3(GJ_TG * GJ_TG), 0, ctx->stream, A,
+                               (int64_t)block * (ld + 1), ld, inv, bb, (int64_t)block, n, block, 0, -1, info_dev);
         FP_HIP(ctx, hipGetLastError());
         return FP_OK;
     }
@@ -1682,7 +1686,8 @@ int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, 
         FP_HIP(ctx, hipEventCreateWithFlags(&ev.fork, hipEventDisableTiming));
         FP_HIP(ctx, hipEventCreateWithFlags(&ev.join, hipEventDisableTiming));
     }
-    hipLaunchKernelGGL(k_bj_copy<T>, dim3(block, nb), dim3(256), 0, ctx->stream, A, n, ld, block, inv);
+    // prefilled: inv already holds the identity-padded diagonal blocks (A unused)
+    if (!prefilled) hipLaunchKernelGGL(k_bj_copy<T>, dim3(block, nb), dim3(256), 0, ctx->stream, A, n, ld, block, inv);
     if (copied) FP_HIP(ctx, hipEventRecord(copied, ctx->stream));
     auto gj = [&](int j, hipStream_t s) {
         const int j0 = j * h, hj = min(h, block - j0);
@@ -1729,9 +1734,9 @@ int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, 
     return FP_OK;
 }
 template int fp_bj_gj_inverse<double>(fp_ctx *, const double *, int64_t, int64_t, int, float *, int *, float *,
-                                      hipEvent_t, bool);
+                                      hipEvent_t, bool, bool);
 template int fp_bj_gj_inverse<float>(fp_ctx *, const float *, int64_t, int64_t, int, float *, int *, float *,
-                                     hipEvent_t, bool);
+                                     hipEvent_t, bool, bool);
 
 // y_b = inv_b x_b for every diagonal block.
 template <int KM>

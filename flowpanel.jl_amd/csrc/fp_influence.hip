@@ -149,6 +149,63 @@ __global__ __launch_bounds__(256) void k_assemble_vr(PanelView pan, TargetView t
     vr_tile<T, TO, NVP, TM>(pan, tv, G, ld, col0, blockIdx.x, i0, s, G2);
 }
 
+// The diagonal blocks of one vortex-ring body's AIC for the block-Jacobi
+// inverses, rounded to FP32 into a compact identity-padded array (block b at
+// X + b block^2, row-major, leading dimension block): grid (column tiles of 256,
+// target tiles of TM, blocks); one lane per panel as in k_assemble_vr.
+template <int NVP, int TM>
+__global__ __launch_bounds__(256) void k_assemble_diag_vr(PanelView pan, TargetView tv, int block,
+                                                          float *__restrict__ X)
+{
+    __shared__ double s[6][TM];
+    const int64_t b0 = (int64_t)blockIdx.z * block;
+    const int bs = (int)min((int64_t)block, pan.N - b0);
+    const int r0 = blockIdx.y * TM;
+    for (int t = threadIdx.x; t < TM; t += blockDim.x) {
+        const bool ok = r0 + t < bs;
+        const int64_t i = b0 + r0 + t;
+        s[0][t] = ok ? tv.x[i] : 0.0;
+        s[1][t] = ok ? tv.y[i] : 0.0;
+        s[2][t] = ok ? tv.z[i] : 0.0;
+        s[3][t] = ok ? tv.nx[i] : 0.0;
+        s[4][t] = ok ? tv.ny[i] : 0.0;
+        s[5][t] = ok ? tv.nz[i] : 0.0;
+    }
+    __syncthreads();
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= block) return;
+    float *xb = X + (int64_t)blockIdx.z * block * block;
+    const int rend = min(TM, block - r0);
+    if (c >= bs) {
+        for (int t = 0; t < rend; ++t) xb[(int64_t)(r0 + t) * block + c] = r0 + t == c ? 1.f : 0.f;
+        return;
+    }
+    const int64_t N = pan.N, j = b0 + c;
+    double px[NVP], py[NVP], pz[NVP], ex[NVP], ey[NVP], ez[NVP];
+    double q[NVP][3];
+#pragma unroll
+    for (int k = 0; k < NVP; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) q[k][d] = pan.pv[(3 * k + d) * N + j];
+#pragma unroll
+    for (int k = 0; k < NVP; ++k) {
+        const int kk = (k + 1) % NVP;
+        px[k] = q[k][0]; py[k] = q[k][1]; pz[k] = q[k][2];
+        ex[k] = q[kk][0] - q[k][0]; ey[k] = q[kk][1] - q[k][1]; ez[k] = q[kk][2] - q[k][2];
+    }
+    const double off = pan.off, off2 = pan.off * pan.off, cut2 = pan.cut * pan.cut;
+    for (int t = 0; t < rend; ++t) {
+        const int r = r0 + t;
+        float v = r == c ? 1.f : 0.f;
+        if (r < bs) {
+            double ux = 0, uy = 0, uz = 0;
+            ring_u<double, NVP>(px, py, pz, ex, ey, ez, s[0][t], s[1][t], s[2][t], off, off2, cut2, ux, uy, uz);
+            v = float(ux * s[3][t] + uy * s[4][t] + uz * s[5][t]);
+        }
+        xb[(int64_t)r * block + c] = v;
+    }
+}
+
 // Batched assembly of independent bodies (each its own matrix): persistent
 // workgroups pull (body, panel block, target tile) items from a device queue
 // (one atomic counter) until the queue is empty.  Every body's items are the
@@ -228,10 +285,13 @@ __global__ __launch_bounds__(256) void k_assemble_source(PanelView pan, TargetVi
 }
 
 // Wake horseshoes into their columns: one lane per target, shedding list in order.
+// dblk > 0: G is the compact diagonal-block array of block size dblk (block b at
+// G + b dblk^2, row-major with leading dimension dblk) and only the columns in
+// the target's own block are updated.
 template <typename T, typename TO>
 __global__ __launch_bounds__(256) void k_assemble_wake(HorseView hv, double off_, double cut_, TargetView tv,
                                                        TO *__restrict__ G, int64_t ld, int64_t col0,
-                                                       float *__restrict__ G2)
+                                                       float *__restrict__ G2, int dblk)
 {
     __shared__ T sh[64][12];
     __shared__ int sc[64];
@@ -252,14 +312,42 @@ __global__ __launch_bounds__(256) void k_assemble_wake(HorseView hv, double off_
         if (ok) {
             for (int hh = 0; hh < cnt; ++hh) {
                 T ux = 0, uy = 0, uz = 0;
+                TO *g;
+                if (dblk > 0) {
+                    const int64_t bb = i / dblk, c = sc[hh] - bb * dblk;
+                    if (c < 0 || c >= dblk) continue;
+                    g = G + bb * dblk * dblk + (i - bb * dblk) * dblk + c;
+                } else {
+                    g = G + i * ld + col0 + sc[hh];
+                }
                 horseshoe_u<T>(sh[hh], X, Y, Z, T(1), off, off2, cut2, ux, uy, uz);
-                TO *g = G + i * ld + col0 + sc[hh];
                 const TO v = TO(T(*g) + (ux * nx + uy * ny + uz * nz));
                 *g = v;
-                if (G2) G2[i * ld + col0 + sc[hh]] = float(v);
+                if (G2 && dblk == 0) G2[i * ld + col0 + sc[hh]] = float(v);
             }
         }
     }
+}
+
+int fp_launch_assemble_diag(fp_ctx *ctx, const fp_body *b, const TargetView &tv, int block, float *X)
+{
+    constexpr int TM = 64;
+    if (b->element != FP_ELEM_VORTEXRING || tv.M != b->N || block <= 0)
+        return fp_fail(ctx, FP_ERR_ARG, "internal: diagonal-block assembly of one vortex-ring body");
+    const int64_t nb = (b->N + block - 1) / block;
+    dim3 grid((unsigned)((block + 255) / 256), (unsigned)((block + TM - 1) / TM), (unsigned)nb);
+    if (b->nvp == 4)
+        hipLaunchKernelGGL((k_assemble_diag_vr<4, TM>), grid, dim3(256), 0, ctx->stream, panel_view(b), tv, block, X);
+    else
+        hipLaunchKernelGGL((k_assemble_diag_vr<8, TM>), grid, dim3(256), 0, ctx->stream, panel_view(b), tv, block, X);
+    FP_HIP(ctx, hipGetLastError());
+    if (b->nh > 0) {
+        dim3 g2((unsigned)((tv.M + 255) / 256));
+        hipLaunchKernelGGL((k_assemble_wake<double, float>), g2, dim3(256), 0, ctx->stream, horse_view(b),
+                           b->kernel_offset, b->kernel_cutoff, tv, X, (int64_t)0, (int64_t)0, (float *)nullptr, block);
+        FP_HIP(ctx, hipGetLastError());
+    }
+    return FP_OK;
 }
 
 // The entries are always computed in FP64 arithmetic and rounded once to the
@@ -291,7 +379,7 @@ int fp_launch_assemble(fp_ctx *ctx, const fp_body *b, const TargetView &tv, T *G
             ProfScope pw(ctx, "assemble_wake");
             dim3 g2((unsigned)((tv.M + 255) / 256));
             hipLaunchKernelGGL((k_assemble_wake<double, T>), g2, dim3(256), 0, ctx->stream, horse_view(b),
-                               b->kernel_offset, b->kernel_cutoff, tv, G, ld, col0, G32);
+                               b->kernel_offset, b->kernel_cutoff, tv, G, ld, col0, G32, 0);
             FP_HIP(ctx, hipGetLastError());
         }
     } else {
@@ -316,7 +404,7 @@ int fp_launch_assemble_wake(fp_ctx *ctx, const fp_body *b, const TargetView &tv,
     ProfScope pw(ctx, "assemble_wake");
     dim3 g2((unsigned)((tv.M + 255) / 256));
     hipLaunchKernelGGL((k_assemble_wake<double, T>), g2, dim3(256), 0, ctx->stream, horse_view(b), b->kernel_offset,
-                       b->kernel_cutoff, tv, G, ld, col0, (float *)nullptr);
+                       b->kernel_cutoff, tv, G, ld, col0, (float *)nullptr, 0);
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }

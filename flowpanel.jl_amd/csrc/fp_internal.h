@@ -54,6 +54,8 @@ struct fp_ctx {
     hipStream_t side = nullptr;
     // third stream: look-ahead Gauss-Jordan chunks of the block-Jacobi inverses
     hipStream_t side2 = nullptr;
+    // fourth stream: block-Jacobi inverses formed beside the assembly
+    hipStream_t side3 = nullptr;
     // cached scratch (grown on demand, freed with the context)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -210,6 +212,9 @@ int fp_launch_geometry(fp_ctx *ctx, fp_body *b);
 int fp_gemm_sub_batched_f32(fp_ctx *ctx, int64_t m, int64_t n, int64_t k, const float *A, int64_t lda, int64_t sA,
                             const float *B, int64_t ldb, int64_t sB, float *C, int64_t ldc, int64_t sC, int batch);
 int fp_body_build_edge_blocks(fp_ctx *ctx, fp_body *b);
+// The diagonal blocks (block x block, identity-padded) of a vortex-ring body's FP32 AIC with its wake
+// columns, compact (block b at X + b * block^2), for the block-Jacobi inverses.
+int fp_launch_assemble_diag(fp_ctx *ctx, const fp_body *b, const TargetView &tv, int block, float *X);
 int fp_body_build_trails(fp_ctx *ctx, fp_body *b);
 int fp_launch_field_trails(fp_ctx *ctx, const fp_body *b, const double *strength, const TargetView &tv, int mode,
                            double *out, double *partial, int splits, int *used);
@@ -289,7 +294,7 @@ int fp_colnorm2(fp_ctx *ctx, const double *R, int64_t n, int64_t ld, int64_t k, 
 size_t fp_bj_gj_scratch_floats(int64_t n, int block);
 template <typename T>
 int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *inv, int *info_dev,
-                     float *scratch, hipEvent_t copied = nullptr, bool lookahead = true);
+                     float *scratch, hipEvent_t copied = nullptr, bool lookahead = true, bool prefilled = false);
 int fp_bj_apply_inv(fp_ctx *ctx, const float *inv, int64_t n, int block, const double *x, double *y, int accumulate);
 // Galerkin coarse space of chunk indicators: AZ = A Z (n x nc), Einv = (Z^T A Z)^-1
 template <typename T>

@@ -396,6 +396,100 @@ def spatial_order(grid, shedding=None, bits=10):
     return g, sh, perm
 
 
+def _sections_spacing(sections):
+    """Normalised node positions in [0, 1] of a multi-section discretisation: each
+    section ``(fraction, ncells, ratio, central)`` covers ``fraction`` of the length
+    with ``ncells`` cells whose lengths grow geometrically by ``ratio`` (last over
+    first) from its start to its end, or with ``central`` from both ends to its
+    middle (the (sec, ndivs, expratio, central) convention the reference's
+    examples pass to GeometricTools, restated; that package is not vendored)."""
+    pos = [0.0]
+    for frac, n, ratio, central in sections:
+        n = int(n)
+        if central:
+            h1 = n // 2
+            halves = [(h1, ratio), (n - h1, 1.0 / ratio)]
+            lens = []
+            for nh, r in halves:
+                if nh == 0:
+                    continue
+                q = r ** (1.0 / (nh - 1)) if nh > 1 else 1.0
+                lens += [q ** i for i in range(nh)]
+            lens = np.array(lens)
+        else:
+            q = ratio ** (1.0 / (n - 1)) if n > 1 else 1.0
+            lens = q ** np.arange(n)
+        lens = frac * lens / lens.sum()
+        pos += list(pos[-1] + np.cumsum(lens))
+    pos = np.array(pos)
+    return pos / pos[-1]
+
+
+def rediscretize_airfoil(x, y, ndivs_up, ndivs_lo):
+    """The closed airfoil contour (x, y) (a loop from the trailing edge round the
+    leading edge, the point of least x, back to the trailing edge) re-sampled on
+    each side by arc length with the multi-section spacing ``ndivs_*`` (see
+    ``_sections_spacing``) through a cubic spline, as gt.rediscretize_airfoil in
+    examples/duct.jl:78.  Returns (xs, ys), the first side first, leading-edge
+    point shared."""
+    from scipy.interpolate import CubicSpline
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    le = int(np.argmin(x))
+    sides = []
+    for k, (sx, sy, nd) in enumerate(((x[:le + 1], y[:le + 1], ndivs_up), (x[le:], y[le:], ndivs_lo))):
+        # each side runs from the leading edge to the trailing edge
+        if k == 0:
+            sx, sy = sx[::-1], sy[::-1]
+        t = np.concatenate([[0.0], np.cumsum(np.hypot(np.diff(sx), np.diff(sy)))])
+        keep = np.concatenate([[True], np.diff(t) > 0])
+        t, sx, sy = t[keep], sx[keep], sy[keep]
+        u = _sections_spacing(nd) * t[-1]
+        sides.append((CubicSpline(t, sx)(u), CubicSpline(t, sy)(u)))
+    (ux, uy), (lx, ly) = sides
+    xs = np.concatenate([ux[::-1], lx[1:]])
+    ys = np.concatenate([uy[::-1], ly[1:]])
+    return xs, ys
+
+
+def revolution_liftbody(points, n_theta, axis_angle=270.0):
+    """Lifting body of revolution about the x axis (generate_revolution_liftbody,
+    src/FLOWPanel_abstractliftingbody.jl:124-167): the closed contour ``points``
+    (k x 2: axial position, radius; first point = last = the trailing edge)
+    revolved in ``n_theta`` azimuthal steps from ``axis_angle`` degrees.  A
+    (contour cells) x (n_theta) quad grid like the lofts (contour dimension first,
+    the trailing-edge nodes of the two sides separate), periodic in azimuth, normals
+    pointing out of the closed contour's solid, the trailing edge of every azimuthal
+    strip shedding between the last and the first contour cells.  Returns
+    ``(grid, shedding)``; ``triangulate`` gives the reference's triangle grid."""
+    P = np.asarray(points, dtype=np.float64)
+    if P.ndim != 2 or P.shape[1] != 2:
+        raise ValueError("points must be k x 2 (x, r)")
+    nc1 = P.shape[0]
+    nc = nc1 - 1
+    nt = int(n_theta)
+    th = np.deg2rad(axis_angle) + 2 * np.pi * np.arange(nt) / nt
+    # orientation: the contour's signed area in the (x, r) plane decides the sense of revolution
+    area = 0.5 * np.sum(P[:-1, 0] * P[1:, 1] - P[1:, 0] * P[:-1, 1])
+    if area < 0:
+        th = th[::-1].copy()
+    nodes = np.zeros((3, nc1 * nt))
+    for j in range(nt):
+        sl = slice(j * nc1, (j + 1) * nc1)
+        nodes[0, sl] = P[:, 0]
+        nodes[1, sl] = P[:, 1] * np.cos(th[j])
+        nodes[2, sl] = P[:, 1] * np.sin(th[j])
+    cells = np.zeros((nc * nt, 4), dtype=np.int64)
+    shedding = np.zeros((6, nt), dtype=np.int64)
+    for j in range(nt):
+        jn = (j + 1) % nt
+        for i in range(nc):
+            cells[i + nc * j] = [i + nc1 * j, i + 1 + nc1 * j, i + 1 + nc1 * jn, i + nc1 * jn]
+        shedding[:, j] = [(nc - 1) + nc * j, 2, 1, nc * j, 0, 3]
+    ptr = np.arange(0, 4 * nc * nt + 1, 4, dtype=np.int64)
+    return Grid(nodes, (ptr, cells.reshape(-1)), dims=(nc, nt)), shedding
+
+
 def direction(alpha_deg, beta_deg=0.0):
     """Freestream unit vector (cos a cos b, -sin b, sin a cos b)
     (src/FLOWPanel_utils.jl:34-37)."""

@@ -9,6 +9,8 @@
   formulation (Gmsh 4.1 ASCII).
 * examples/data/zeroebwb.msh and zeroebwb-TE.msh are the blended-wing-body mesh
   and its trailing-edge line of examples/blendedwingbody.jl.
+* examples/data/naca662015.csv is the duct contour of test/runtests_solvers2.jl:98
+  (x, y columns of the NACA 66-2015 section).
 
 They are data, copied byte for byte; run this script in a checkout that has the
 reference at /root/reference.
@@ -21,6 +23,6 @@ SRC = "/root/reference/examples/data"
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 for name in ("sweptwing-left000.vtu", "sweptwing-right000.vtu", "sweptwing-openvsp.msh", "zeroebwb.msh",
-             "zeroebwb-TE.msh"):
+             "zeroebwb-TE.msh", "naca662015.csv"):
     shutil.copyfile(os.path.join(SRC, name), os.path.join(HERE, name))
     print(name, hashlib.sha256(open(os.path.join(HERE, name), "rb").read()).hexdigest())

@@ -9,10 +9,11 @@ AoA 10 deg, 30 m/s; surface velocity U + U_gradmu, Cp clipped at
 1 - 342/30, forces and CL/CD on Sref = 3.23^2 / 8.  The reference publishes no
 CL for it (docs/src/examples/blendedwingbody-aero.md:207-208 gives the panel
 count and ~10 s "with GPU"), so the checks are: the FP64 LU and the mixed
-GMRES-IR least-squares solutions (the latter to a 1e-13 residual: the system's condition is about 5e6)
+GMRES-IR least-squares solutions (the latter to a 3e-13 residual, near the FP64 floor of this system: the LU leaves 5e-13; its condition is about 5e6)
 give the same Cp (5e-5) and CL, CD (1e-6 of CL), the panel order does not change
-the answer (a Morton-ordered copy of the mesh, mapped back, 1e-9), the lift is
-positive and the drag small.  GMRES-IR runs with its block-Jacobi blocks and
+the answer (a Morton-ordered copy of the mesh, mapped back, 1e-9 on the strengths
+and 1e-7 on Cp away from the cells that force_cellTE picks by index, pi+1 and pj-1,
+which are other cells in the other order), the lift is positive and the drag small.  GMRES-IR runs with its block-Jacobi blocks and
 coarse chunks along the library's spatial order (FP_ORDER_SPATIAL) on the mesh
 as read, and along index ranges on the Morton-ordered copy; with index ranges
 on the mesh as read, the preconditioner groups panels that are not neighbours
@@ -71,10 +72,12 @@ def _aero(fp, body, Vinf, magV, rho):
 def test_zeroebwb_example_least_squares(fp):
     aoa, magV, rho = 10.0, 30.0, 1.225
     Vinf = magV * np.array([np.cos(np.deg2rad(aoa)), 0.0, np.sin(np.deg2rad(aoa))])
-    krylov = {"precond_block": 1024, "coarse_block": 50, "tol": 1e-13, "restart": 100}
+    krylov = {"precond_block": 1024, "coarse_block": 50, "tol": 3e-13, "restart": 100}
     res = {}
+    index_neighbours = []
     for order in (False, True):
         g, sh, perm = _zeroebwb(fp, order)
+        index_neighbours.append((sh, perm))
         body = fp.RigidWakeBody(g, sh, formulation=2, CPoffset=1e-14)
         assert body.ncells == 22250 and body.nsheddings > 100
         U = np.tile(Vinf[:, None], (1, body.ncells))
@@ -106,7 +109,7 @@ def test_zeroebwb_example_least_squares(fp):
                   ("Morton-ordered" if order else "as read", solver, opts.get("precond_order", ""), dt, CL, CD,
                    st.get("iterations"), st.get("residual")), flush=True)
         # the bordered least-squares system of this mesh is ill-conditioned (a residual of 1e-10 left
-        # strength errors near 4e-4, condition about 5e6), so GMRES-IR is driven to a 1e-13 residual and
+        # strength errors near 4e-4, condition about 5e6), so GMRES-IR is driven to a 3e-13 residual and
         # the solvers are compared on what the strengths produce: Cp at the control points, CL and CD
         (xl, CLl, CDl, _, cpl), (xg, CLg, CDg, _, cpg) = out["solve_ludiv"], out["solve_gmres_ir"]
         print("  LU vs GMRES-IR: max|dGamma|/max|Gamma| %.1e, max|dCp| %.1e, dCL %.1e, dCD %.1e" %
@@ -116,7 +119,20 @@ def test_zeroebwb_example_least_squares(fp):
         res[order] = out
     x0, x1 = res[False]["solve_ludiv"][0], res[True]["solve_ludiv"][0]
     assert np.abs(x0 - x1).max() / np.abs(x0).max() < 1e-9
-    assert np.abs(res[False]["solve_ludiv"][4] - res[True]["solve_ludiv"][4]).max() < 1e-9
+    # force_cellTE keeps the cell-centred gradient on the shedding panels and on their index
+    # neighbours pi+1, pj-1 (src/FLOWPanel_postprocess.jl:875-882); on an unstructured mesh those
+    # neighbours are other cells in the other order, so Cp is compared everywhere else
+    keep = np.ones(len(x0), bool)
+    for sh, perm in index_neighbours:
+        cells = np.concatenate([sh[0] + 1, sh[3][sh[3] >= 0] - 1])
+        cells = cells[(cells >= 0) & (cells < len(x0))]
+        keep[cells if perm is None else perm[cells]] = False
+    print("  Morton vs as read: max|dCp| %.1e off the %d index-neighbour cells" %
+          (np.abs(res[False]["solve_ludiv"][4] - res[True]["solve_ludiv"][4])[keep].max(), (~keep).sum()), flush=True)
+    assert (~keep).sum() <= 4 * len(sh[0])
+    # the two orders are two LU factorisations of a system of condition about 5e6: strengths agree to
+    # 1e-9 of their maximum and Cp (|Cp| up to about 10) to 1e-7
+    assert np.abs(res[False]["solve_ludiv"][4] - res[True]["solve_ludiv"][4])[keep].max() < 1e-7
     CL, CD = res[False]["solve_ludiv"][1:3]
     assert 0.2 < CL < 2.0 and abs(CD) < 0.2 * CL
 

@@ -181,3 +181,43 @@ $EndElements
     g = fp.read_msh(str(p))
     assert g.nnodes == 4 and g.ncells == 2
     assert list(g.cell(0)) == [0, 1, 2] and list(g.cell(1)) == [0, 2, 3]
+
+
+def test_multisection_spacing_and_airfoil_rediscretisation(fp):
+    g = fp.geometry
+    nd = [(0.25, 10, 10.0, False), (0.50, 10, 1.0, True), (0.25, 10, 0.1, False)]
+    u = g._sections_spacing(nd)
+    assert len(u) == 31 and u[0] == 0.0 and abs(u[-1] - 1.0) < 1e-15 and np.all(np.diff(u) > 0)
+    d = np.diff(u)
+    # section 1 grows by 10 from its first to its last cell, section 3 shrinks by 10, the middle is uniform
+    assert abs(d[9] / d[0] - 10.0) < 1e-9 and abs(d[29] / d[20] - 0.1) < 1e-9
+    assert np.allclose(d[10:20], d[10]) and abs(u[10] - 0.25) < 1e-12 and abs(u[20] - 0.75) < 1e-12
+    # a circle through the rediscretisation stays on the circle and keeps its end points
+    t = np.linspace(0, 2 * np.pi, 81)
+    x, y = 0.5 + 0.5 * np.cos(t), 0.5 * np.sin(t)
+    xs, ys = g.rediscretize_airfoil(x, y, nd, nd)
+    assert len(xs) == 61 and abs(xs[0] - 1.0) < 1e-12 and abs(xs[-1] - 1.0) < 1e-12
+    assert np.abs(np.hypot(xs - 0.5, ys) - 0.5).max() < 1e-4
+    assert abs(xs.min()) < 1e-12                               # the leading edge is kept
+
+
+def test_revolution_liftbody_is_closed_and_outward(fp, orc):
+    g = fp.geometry
+    t = np.linspace(0, 2 * np.pi, 41)
+    pts = np.c_[0.5 + 0.5 * np.cos(t), 1.0 + 0.1 * np.sin(t)]       # a torus-like duct section
+    grid, sh = g.revolution_liftbody(pts, 24)
+    tg, tsh = g.triangulate(grid, sh)
+    for gg in (grid, tg):
+        n, c, a = orc.geometry(gg.nodes, gg.cell_ptr, gg.cell_idx)
+        # closed and outward: the vector areas cancel, sum(n . x A) = 3 V > 0 (Pappus: V = 2 pi R (pi a b) for the
+        # elliptic section of semi-axes a = 0.5, b = 0.1 at radius R = 1; the polygonal surface falls a little short)
+        assert np.abs((n * a).sum(axis=1)).max() < 1e-12
+        vol = (np.einsum("ij,ij->j", n, c) * a).sum() / 3
+        exact = 2 * np.pi ** 2 * 1.0 * 0.5 * 0.1
+        assert 0.95 * exact < vol < exact
+    # every azimuthal strip sheds: coincident trailing-edge nodes of the first and last contour cells
+    assert sh.shape[1] == 24 and tsh.shape[1] == 24
+    X = tg.nodes
+    for pi, nia, nib, pj, nja, njb in tsh.T:
+        ci, cj = tg.cell(pi), tg.cell(pj)
+        assert np.allclose(X[:, ci[nia]], X[:, cj[njb]]) and np.allclose(X[:, ci[nib]], X[:, cj[nja]])
