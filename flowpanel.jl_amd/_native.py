@@ -53,7 +53,7 @@ EXPORTS = [
     "fp_liftdrag_opts_default", "fp_lift_drag", "fp_lift_drag_adjoint", "fp_solve_bodies_multi",
     "fp_batch_create", "fp_batch_solve", "fp_batch_destroy", "fp_batch_autotune_get", "fp_solve_bodies_lsq",
     "fp_calcfield_F", "fp_calcfield_sectionalforce", "fp_calcfield_ugradmu", "fp_calcfield_ugradmu_node", "fp_calcfield_ugradmu_combined", "fp_uind_noalloc",
-    "fp_assembly_blocks", "fp_field_trails",
+    "fp_assembly_blocks", "fp_assembly_pairs", "fp_field_trails",
 ]
 
 
@@ -181,6 +181,8 @@ def lib():
     L.fp_uind_noalloc.argtypes = [_vp, pp, cint, _vp, i64, _vp]
     L.fp_assembly_blocks.argtypes = [_d, i64, _i64, _i64, i64, cint, ctypes.POINTER(ctypes.c_int32), i64, _i64]
     L.fp_assembly_blocks.restype = i64
+    L.fp_assembly_pairs.argtypes = [_d, i64, _i64, _i64, i64, cint, ctypes.POINTER(ctypes.c_int32), i64, _i64]
+    L.fp_assembly_pairs.restype = i64
     i32p = ctypes.POINTER(ctypes.c_int32)
     L.fp_field_trails.argtypes = [_i64, _i64, i64, i64, cint, i32p, i32p, i32p, i64, i64, _i64]
     L.fp_field_trails.restype = i64

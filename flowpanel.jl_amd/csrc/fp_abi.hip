@@ -232,7 +232,7 @@ struct BJPrecond : Precond {
     DBuf inv, gjs, AZ, Einv, work, tc, r2, cinfo, binfo;
     hipEvent_t fork = nullptr, join = nullptr, pre = nullptr, bj_done = nullptr, asm_done = nullptr;
     int cs = 0;
-    bool early = false;   // the block inverses were started beside the assembly (setup_early)
+    bool early = false;   // the block inverses were started beside the assembly (assemble_early, invert_early)
     BJPrecond(fp_ctx *c, int64_t n_, int bl) : ctx(c), n(n_), block(bl) {}
     ~BJPrecond()
     {
@@ -246,10 +246,12 @@ struct BJPrecond : Precond {
     }
     // Block inverses beside the assembly: the diagonal blocks of the body's AIC are
     // assembled into `inv` first (a few per cent of the entries, on the context
-    // stream), then inverted on the fourth stream while the context stream
-    // assembles the whole matrix.  finish_early adds the coarse space (it needs the
-    // whole matrix) and joins both streams into the context stream.
-    int setup_early(const fp_body *b, const TargetView &tv)
+    // stream: assemble_early), then, once the whole matrix's assembly is queued
+    // behind them, inverted on the fourth stream while it runs (invert_early: the
+    // host queues the Gauss-Jordan passes while the assembly already runs).
+    // finish_early adds the coarse space (it needs the whole matrix) and joins
+    // both streams into the context stream.
+    int assemble_early(const fp_body *b, const TargetView &tv)
     {
         int st;
         const int64_t nb = (n + block - 1) / block;
@@ -263,19 +265,30 @@ struct BJPrecond : Precond {
         FP_HIP(ctx, hipEventCreateWithFlags(&pre, hipEventDisableTiming));
         FP_HIP(ctx, hipEventCreateWithFlags(&bj_done, hipEventDisableTiming));
         FP_HIP(ctx, hipEventRecord(pre, ctx->stream));
+        early = true;
+        return FP_OK;
+    }
+    int invert_early()
+    {
+        int st;
+        const size_t sf = fp_bj_gj_scratch_floats(n, block);
         FP_HIP(ctx, hipStreamWaitEvent(ctx->side3, pre, 0));
-        fp_ctx sc;
+        if (!ctx->gj_fork) FP_HIP(ctx, hipEventCreateWithFlags(&ctx->gj_fork, hipEventDisableTiming));
+        if (!ctx->gj_join) FP_HIP(ctx, hipEventCreateWithFlags(&ctx->gj_join, hipEventDisableTiming));
+        fp_ctx sc;   // borrows the context's streams and events; owns none of them
         sc.device = ctx->device;
         sc.num_cu = ctx->num_cu;
         sc.stream = ctx->side3;
         sc.side2 = ctx->side2;
+        sc.gj_fork = ctx->gj_fork;
+        sc.gj_join = ctx->gj_join;
         st = fp_bj_gj_inverse<float>(&sc, inv.as<float>(), n, block, block, inv.as<float>(), binfo.as<int>(),
                                      sf ? gjs.as<float>() : nullptr, nullptr, true, true);
         sc.side2 = nullptr;
         sc.stream = nullptr;
+        sc.gj_fork = sc.gj_join = nullptr;
         if (st) return fp_fail(ctx, st, sc.err.c_str());
         FP_HIP(ctx, hipEventRecord(bj_done, ctx->side3));
-        early = true;
         return FP_OK;
     }
     int finish_early(const void *A, int prec, int64_t ld, int coarse)
@@ -284,7 +297,7 @@ struct BJPrecond : Precond {
         if (coarse > 0) {
             cs = coarse;
             const int nc = (int)((n + cs - 1) / cs);
-            if ((st = dalloc(ctx, AZ, (size_t)nc * n * sizeof(double))) ||
+            if ((st = dalloc(ctx, AZ, (size_t)nc * n * sizeof(float))) ||
                 (st = dalloc(ctx, Einv, (size_t)nc * nc * sizeof(double))) ||
                 (st = dalloc(ctx, work, (size_t)2 * nc * nc * sizeof(double))) ||
                 (st = dalloc(ctx, tc, 2 * nc * sizeof(double))) || (st = dalloc(ctx, r2, n * sizeof(double))) ||
@@ -299,9 +312,9 @@ struct BJPrecond : Precond {
             sc.device = ctx->device;
             sc.num_cu = ctx->num_cu;
             sc.stream = ctx->side;
-            st = prec == FP_F64 ? fp_coarse_setup<double>(&sc, (const double *)A, n, ld, cs, AZ.as<double>(),
+            st = prec == FP_F64 ? fp_coarse_setup<double>(&sc, (const double *)A, n, ld, cs, AZ.as<float>(),
                                                           Einv.as<double>(), work.as<double>(), cinfo.as<int>())
-                                : fp_coarse_setup<float>(&sc, (const float *)A, n, ld, cs, AZ.as<double>(),
+                                : fp_coarse_setup<float>(&sc, (const float *)A, n, ld, cs, AZ.as<float>(),
                                                          Einv.as<double>(), work.as<double>(), cinfo.as<int>());
             sc.stream = nullptr;
             if (st) return fp_fail(ctx, st, sc.err.c_str());
@@ -323,7 +336,7 @@ struct BJPrecond : Precond {
         if (coarse > 0) {
             cs = coarse;
             nc = (int)((n + cs - 1) / cs);
-            if ((st = dalloc(ctx, AZ, (size_t)nc * n * sizeof(double)))) return st;
+            if ((st = dalloc(ctx, AZ, (size_t)nc * n * sizeof(float)))) return st;
             if ((st = dalloc(ctx, Einv, (size_t)nc * nc * sizeof(double)))) return st;
             if ((st = dalloc(ctx, work, (size_t)2 * nc * nc * sizeof(double)))) return st;
             if ((st = dalloc(ctx, tc, 2 * nc * sizeof(double)))) return st;
@@ -347,9 +360,9 @@ struct BJPrecond : Precond {
             sc.device = ctx->device;
             sc.num_cu = ctx->num_cu;
             sc.stream = ctx->side;
-            st = prec == FP_F64 ? fp_coarse_setup<double>(&sc, (const double *)A, n, ld, cs, AZ.as<double>(),
+            st = prec == FP_F64 ? fp_coarse_setup<double>(&sc, (const double *)A, n, ld, cs, AZ.as<float>(),
                                                           Einv.as<double>(), work.as<double>(), cinfo.as<int>())
-                                : fp_coarse_setup<float>(&sc, (const float *)A, n, ld, cs, AZ.as<double>(),
+                                : fp_coarse_setup<float>(&sc, (const float *)A, n, ld, cs, AZ.as<float>(),
                                                          Einv.as<double>(), work.as<double>(), cinfo.as<int>());
             if (st) return fp_fail(ctx, st, sc.err.c_str());
             FP_HIP(ctx, hipEventRecord(join, ctx->side));
@@ -360,7 +373,7 @@ struct BJPrecond : Precond {
     int apply(const double *x, double *y) override
     {
         if (cs > 0) {
-            int st = fp_coarse_apply(ctx, x, n, cs, AZ.as<double>(), Einv.as<double>(), tc.as<double>(), y,
+            int st = fp_coarse_apply(ctx, x, n, cs, AZ.as<float>(), Einv.as<double>(), tc.as<double>(), y,
                                      r2.as<double>());
             if (st) return st;
             return fp_bj_apply_inv(ctx, inv.as<float>(), n, block, r2.as<double>(), y, 1);
@@ -1596,6 +1609,8 @@ void fp_ctx_destroy(fp_ctx *ctx)
             (void)hipStreamSynchronize(sd);
             (void)hipStreamDestroy(sd);
         }
+    for (hipEvent_t e : {ctx->gj_fork, ctx->gj_join})
+        if (e) (void)hipEventDestroy(e);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     for (auto &r : ctx->prof_pending) {
         (void)hipEventDestroy(r.a);
@@ -1772,7 +1787,7 @@ void fp_body_destroy(fp_body *b)
     (void)hipStreamSynchronize(b->ctx->stream);
     for (void *p : {(void *)b->d_pv, (void *)b->d_nv, (void *)b->d_hs, (void *)b->d_geo, (void *)b->d_uinf,
                     (void *)b->d_hh, (void *)b->d_hcol, (void *)b->d_strength, (void *)b->d_nbr, (void *)b->d_sh,
-                    (void *)b->d_eb, (void *)b->d_tr_tab, (void *)b->d_tr})
+                    (void *)b->d_eb, (void *)b->d_pb, (void *)b->d_tr_tab, (void *)b->d_tr})
         if (p) (void)hipFree(p);
     delete b;
 }
@@ -1854,11 +1869,13 @@ static int assemble_into(fp_ctx *ctx, fp_body *const *bodies, int nb, const Targ
 
 // The block-Jacobi inverses of GMRES-IR can start before the AIC is whole when
 // their blocks are index ranges of one vortex-ring body: its diagonal blocks are
-// assembled on their own first (FLOWPANEL_BJ_EARLY=0 turns this off).
+// assembled on their own first and inverted beside the assembly.  Off unless
+// FLOWPANEL_BJ_EARLY=1: on the 30,000-panel loft the Gauss-Jordan products slow
+// the assembly by as much as they save (39.1 ms per solve either way).
 static bool early_block_jacobi(const fp_solve_opts &o, fp_body *const *bodies, int nb)
 {
     const char *e = getenv("FLOWPANEL_BJ_EARLY");
-    if (e && e[0] == '0') return false;
+    if (!(e && e[0] == '1')) return false;
     return o.solver == FP_SOLVER_GMRES_IR && o.precond_order == FP_ORDER_INDEX && nb == 1 &&
            bodies[0]->element == FP_ELEM_VORTEXRING && o.precond_block <= 1024;
 }
@@ -1894,13 +1911,14 @@ static int solve_system(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_so
     if (early_block_jacobi(o, bodies, nb)) {
         // the block inverses run on their own stream beside the assembly
         pre.reset(new BJPrecond(ctx, n, (int)std::min<int64_t>(o.precond_block > 0 ? o.precond_block : 512, n)));
-        if ((st = pre->setup_early(bodies[0], t.tv))) return st;
+        if ((st = pre->assemble_early(bodies[0], t.tv))) return st;
     }
     if ((st = dalloc(ctx, A, (size_t)n * ld * ((f64 || dual) ? sizeof(double) : sizeof(float))))) return st;
     if (dual && (st = dalloc(ctx, A32buf, (size_t)n * ld * sizeof(float)))) return st;
     if ((st = assemble_into(ctx, bodies, nb, t.tv, (f64 || dual) ? FP_F64 : FP_F32, A.p, ld,
                             dual ? A32buf.as<float>() : nullptr)))
         return st;
+    if (pre && (st = pre->invert_early())) return st;
     if (dual) {
         a.A32 = A32buf.as<float>();
         op64.reset(new DenseOp<double>(ctx, A.as<double>(), n, ld));
@@ -2803,8 +2821,10 @@ static int transpose_solves_concurrent(fp_ctx *ctx, const SolveArgs &at, const f
     }
     worker.join();
     (void)hipEventDestroy(ready);
-    if (child.side) (void)hipStreamDestroy(child.side);
-    if (child.side2) (void)hipStreamDestroy(child.side2);
+    for (hipStream_t sd : {child.side, child.side2, child.side3})
+        if (sd) (void)hipStreamDestroy(sd);
+    for (hipEvent_t ev : {child.gj_fork, child.gj_join})
+        if (ev) (void)hipEventDestroy(ev);
     (void)hipStreamDestroy(child.stream);
     child.stream = nullptr;
     if (st0) return st0;

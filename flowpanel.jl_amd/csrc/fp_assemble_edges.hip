@@ -72,7 +72,8 @@ uint32_t spread10(uint32_t v)
 // per block to `tab`; returns the number of blocks and, in sums[0..1], the
 // distinct nodes and edges summed over the blocks (the terms per target).
 static int64_t build_edge_blocks(const double *nodes, int64_t nn, const int64_t *cell_ptr, const int64_t *cell_idx,
-                                 int64_t N, std::vector<int32_t> &tab, int64_t sums[2])
+                                 int64_t N, std::vector<int32_t> &tab, int64_t sums[2], int npack = VRE_NPACK,
+                                 int epack = VRE_EPACK)
 {
     auto cell = [&](int64_t j, int k) { return cell_idx[cell_ptr[j] + k]; };
     auto nv = [&](int64_t j) { return (int)(cell_ptr[j + 1] - cell_ptr[j]); };
@@ -154,7 +155,7 @@ static int64_t build_edge_blocks(const double *nodes, int64_t nn, const int64_t 
         if (np + len > VRE_P) return false;
         int add_n = 0, add_e = 0;
         for (int64_t u : rnodes[r]) add_n += stamp[u] != nb;
-        if (nnod + add_n > VRE_NPACK) return false;
+        if (nnod + add_n > npack) return false;
         enew.clear();
         for (int64_t j = run0[r]; j < run0[r + 1]; ++j)
             for (int k = 0; k < nv(j); ++k) {
@@ -164,7 +165,7 @@ static int64_t build_edge_blocks(const double *nodes, int64_t nn, const int64_t 
                     if (!emap.count(key) && enew.emplace(key, 1).second) ++add_e;
                 }
             }
-        return ned + add_e <= VRE_EPACK;
+        return ned + add_e <= epack;
     };
     auto place = [&](int64_t r) {
         placed[r] = 1;
@@ -247,6 +248,9 @@ static int64_t build_edge_blocks(const double *nodes, int64_t nn, const int64_t 
     return nb + 1;
 }
 
+static int64_t build_pair_blocks(const double *nodes, int64_t nn, const int64_t *cell_ptr, const int64_t *cell_idx,
+                                 int64_t N, std::vector<int32_t> &out, int64_t sums[2]);
+
 // Build and upload the block tables of a vortex-ring body of triangles and quads.
 int fp_body_build_edge_blocks(fp_ctx *ctx, fp_body *b)
 {
@@ -260,6 +264,14 @@ int fp_body_build_edge_blocks(fp_ctx *ctx, fp_body *b)
     b->eb_nblocks = nblocks;
     b->eb_nodes = sums[0];
     b->eb_edges = sums[1];
+    std::vector<int32_t> pt;
+    const int64_t npb = build_pair_blocks(b->nodes.data(), (int64_t)b->nodes.size() / 3, b->cell_ptr.data(),
+                                          b->cell_idx.data(), b->N, pt, sums);
+    if (npb > 0) {
+        FP_HIP(ctx, hipMalloc((void **)&b->d_pb, pt.size() * sizeof(int32_t)));
+        FP_HIP(ctx, hipMemcpy(b->d_pb, pt.data(), pt.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        b->pb_nblocks = npb;
+    }
     return FP_OK;
 }
 
@@ -279,6 +291,177 @@ extern "C" int64_t fp_assembly_blocks(const double *nodes, int64_t nnodes, const
     std::vector<int32_t> t;
     int64_t sums[2];
     const int64_t nb = build_edge_blocks(nodes, nnodes, cp.data(), ci.data(), ncells, t, sums);
+    if (terms) { terms[0] = sums[0]; terms[1] = sums[1]; }
+    if (tab && cap_blocks >= nb) std::memcpy(tab, t.data(), t.size() * sizeof(int32_t));
+    return nb;
+}
+
+// ---------------------------------------------------------------------------
+// Edge-pair tables.  The same column blocks (at most 504 distinct edges), each
+// block's edges paired at shared nodes: lane l of the workgroup owns a path
+// n0 - n1 - n2 of two edges (n0 -> n1 in edge slot l, n1 -> n2 in slot 256 + l)
+// or a single edge n0 -> n1 (n2 = -1).  The lane evaluates the three node terms
+// in registers, the edge terms go through LDS to the panels' sums.  A depth-first
+// pass pairs every edge of a connected block except at most one (each vertex
+// pairs the edges left to it by its subtree and its non-tree edges, and hands an
+// odd one up with the edge to its parent).
+// Per block (int32): columns, column edge refs (2 per panel: slot << 1 | sign, 16
+// bits each), node refs (panel * 8 + vertex) of n0, n1, n2 of every lane.
+namespace {
+constexpr int VRC_OFF_COL = 0, VRC_OFF_REF = VRE_P, VRC_OFF_NODE = 3 * VRE_P;
+constexpr int VRC_BLOCK_INTS = 6 * VRE_P;
+constexpr int VRC_EPACK = 504;     // first distinct-edge limit per block
+constexpr int VRC_NPACK = VRE_NMAX;
+
+// Pairs the edges of one block (local node pairs ea/eb, nl nodes); lanes get
+// (n0, n1, n2) local nodes, n2 = -1 for a single edge; slot_of[e] = the slot of
+// edge e and fwd[e] = whether that slot runs a -> b.  Returns the lanes used.
+int pair_block_edges(int nl, const std::vector<int> &ea, const std::vector<int> &eb, std::vector<int> &l0,
+                     std::vector<int> &l1, std::vector<int> &l2, std::vector<int> &slot_of, std::vector<char> &fwd)
+{
+    const int ne = (int)ea.size();
+    std::vector<std::vector<int>> inc(nl);
+    for (int e = 0; e < ne; ++e) {
+        inc[ea[e]].push_back(e);
+        inc[eb[e]].push_back(e);
+    }
+    std::vector<char> visited(nl, 0), claimed(ne, 0);
+    l0.clear(); l1.clear(); l2.clear();
+    slot_of.assign(ne, -1);
+    fwd.assign(ne, 1);
+    auto other = [&](int e, int v) { return ea[e] == v ? eb[e] : ea[e]; };
+    auto lane_pair = [&](int e1, int e2, int v) {   // path other(e1) - v - other(e2)
+        const int lane = (int)l0.size();
+        const int n0 = other(e1, v), n2 = other(e2, v);
+        l0.push_back(n0); l1.push_back(v); l2.push_back(n2);
+        slot_of[e1] = lane;                    // n0 -> v
+        fwd[e1] = ea[e1] == n0;
+        slot_of[e2] = VRE_P + lane;            // v -> n2
+        fwd[e2] = ea[e2] == v;
+    };
+    auto lane_single = [&](int e) {
+        const int lane = (int)l0.size();
+        l0.push_back(ea[e]); l1.push_back(eb[e]); l2.push_back(-1);
+        slot_of[e] = lane;
+        fwd[e] = 1;
+    };
+    // iterative depth-first search: frame = (vertex, parent edge, next incidence, pending edges)
+    struct Frame { int v, pe; size_t next; std::vector<int> pend; };
+    for (int root = 0; root < nl; ++root) {
+        if (visited[root] || inc[root].empty()) continue;
+        std::vector<Frame> st;
+        visited[root] = 1;
+        st.push_back({root, -1, 0, {}});
+        while (!st.empty()) {
+            Frame &f = st.back();
+            if (f.next < inc[f.v].size()) {
+                const int e = inc[f.v][f.next++];
+                if (e == f.pe || claimed[e]) continue;
+                claimed[e] = 1;
+                const int u = other(e, f.v);
+                if (!visited[u]) {
+                    visited[u] = 1;
+                    st.push_back({u, e, 0, {}});   // tree edge: settled when the child returns
+                } else {
+                    f.pend.push_back(e);           // non-tree edge, paired at this vertex
+                }
+                continue;
+            }
+            // all incidences seen: pair what is pending here, an odd one with the parent edge
+            while (f.pend.size() >= 2) {
+                const int e1 = f.pend.back(); f.pend.pop_back();
+                const int e2 = f.pend.back(); f.pend.pop_back();
+                lane_pair(e1, e2, f.v);
+            }
+            const int v = f.v, pe = f.pe;
+            const bool odd = !f.pend.empty();
+            const int left = odd ? f.pend[0] : -1;
+            st.pop_back();
+            if (pe < 0) {
+                if (odd) lane_single(left);
+            } else if (odd) {
+                lane_pair(left, pe, v);            // the parent edge is used here
+            } else {
+                st.back().pend.push_back(pe);      // the parent pairs it
+            }
+        }
+    }
+    return (int)l0.size();
+}
+}  // namespace
+
+// Host: edge-pair tables of a body (cell lists as build_edge_blocks).  Appends
+// VRC_BLOCK_INTS int32 per block; returns the number of blocks (-1 if a block
+// cannot be paired into 256 lanes at any edge limit) and, in sums[0..1], the
+// lanes and the distinct edges summed over the blocks.
+static int64_t build_pair_blocks(const double *nodes, int64_t nn, const int64_t *cell_ptr, const int64_t *cell_idx,
+                                 int64_t N, std::vector<int32_t> &out, int64_t sums[2])
+{
+    for (int epack = VRC_EPACK; epack >= 96; epack -= 48) {
+        std::vector<int32_t> tab;
+        int64_t s2[2];
+        const int64_t nb = build_edge_blocks(nodes, nn, cell_ptr, cell_idx, N, tab, s2, VRC_NPACK, epack);
+        out.assign((size_t)nb * VRC_BLOCK_INTS, -1);
+        sums[0] = sums[1] = 0;
+        bool ok = true;
+        std::vector<int> ea, eb, l0, l1, l2, slot_of;
+        std::vector<char> fwd;
+        for (int64_t b = 0; b < nb && ok; ++b) {
+            const int32_t *T = &tab[(size_t)b * VRE_BLOCK_INTS];
+            int32_t *O = &out[(size_t)b * VRC_BLOCK_INTS];
+            int nl = 0;
+            while (nl < VRE_NMAX && T[VRE_OFF_NODE + nl] >= 0) ++nl;
+            ea.clear(); eb.clear();
+            for (int e = 0; e < VRE_EMAX && T[VRE_OFF_EDGE + e] >= 0; ++e) {
+                ea.push_back(T[VRE_OFF_EDGE + e] & 0xFFFF);
+                eb.push_back((T[VRE_OFF_EDGE + e] >> 16) & 0xFFFF);
+            }
+            const int lanes = pair_block_edges(nl, ea, eb, l0, l1, l2, slot_of, fwd);
+            if (lanes > VRE_P) { ok = false; break; }
+            sums[0] += lanes;
+            sums[1] += (int64_t)ea.size();
+            for (int l = 0; l < lanes; ++l) {
+                O[VRC_OFF_NODE + l] = T[VRE_OFF_NODE + l0[l]];
+                O[VRC_OFF_NODE + VRE_P + l] = T[VRE_OFF_NODE + l1[l]];
+                O[VRC_OFF_NODE + 2 * VRE_P + l] = l2[l] >= 0 ? T[VRE_OFF_NODE + l2[l]] : -1;
+            }
+            for (int lane = 0; lane < VRE_P; ++lane) {
+                O[VRC_OFF_COL + lane] = T[VRE_OFF_COL + lane];
+                uint32_t r[4];
+                const uint32_t r01 = (uint32_t)T[VRE_OFF_REF + 2 * lane], r23 = (uint32_t)T[VRE_OFF_REF + 2 * lane + 1];
+                const uint32_t in[4] = {r01 & 0xFFFFu, r01 >> 16, r23 & 0xFFFFu, r23 >> 16};
+                for (int k = 0; k < 4; ++k) {
+                    if (T[VRE_OFF_COL + lane] < 0 || in[k] == VRE_NOREF) { r[k] = VRE_NOREF; continue; }
+                    const int e = (int)(in[k] >> 1);
+                    const uint32_t neg = (in[k] & 1u) ^ (fwd[e] ? 0u : 1u);
+                    r[k] = ((uint32_t)slot_of[e] << 1) | neg;
+                }
+                O[VRC_OFF_REF + 2 * lane] = (int32_t)(r[0] | (r[1] << 16));
+                O[VRC_OFF_REF + 2 * lane + 1] = (int32_t)(r[2] | (r[3] << 16));
+            }
+        }
+        if (ok) return nb;
+    }
+    return -1;
+}
+
+extern "C" int64_t fp_assembly_pairs(const double *nodes, int64_t nnodes, const int64_t *cell_ptr,
+                                     const int64_t *cell_idx, int64_t ncells, int index_base, int32_t *tab,
+                                     int64_t cap_blocks, int64_t *terms)
+{
+    if (!nodes || !cell_ptr || !cell_idx || ncells <= 0 || nnodes <= 0) return -FP_ERR_ARG;
+    std::vector<int64_t> cp(ncells + 1), ci((size_t)(cell_ptr[ncells] - cell_ptr[0]));
+    for (int64_t c = 0; c <= ncells; ++c) cp[c] = cell_ptr[c] - cell_ptr[0];
+    for (int64_t c = 0; c < ncells; ++c)
+        if (cp[c + 1] - cp[c] < 3 || cp[c + 1] - cp[c] > 4) return -FP_ERR_UNSUPPORTED;
+    for (size_t k = 0; k < ci.size(); ++k) {
+        ci[k] = cell_idx[cell_ptr[0] + (int64_t)k] - index_base;
+        if (ci[k] < 0 || ci[k] >= nnodes) return -FP_ERR_ARG;
+    }
+    std::vector<int32_t> t;
+    int64_t sums[2];
+    const int64_t nb = build_pair_blocks(nodes, nnodes, cp.data(), ci.data(), ncells, t, sums);
+    if (nb < 0) return -FP_ERR_UNSUPPORTED;
     if (terms) { terms[0] = sums[0]; terms[1] = sums[1]; }
     if (tab && cap_blocks >= nb) std::memcpy(tab, t.data(), t.size() * sizeof(int32_t));
     return nb;
@@ -422,6 +605,117 @@ __global__ __launch_bounds__(256) void k_assemble_vre(const int32_t *__restrict_
     }
 }
 
+// Edge-pair kernel: lane l evaluates the node terms of its path n0 - n1 - n2 for
+// TT targets at a time in registers and writes the two edge terms to LDS (double
+// buffered, one barrier per pass); lane l then sums the signed edge terms of
+// column l.  A missing n2 (single edge) or an idle lane repeats a node, whose
+// zero-length edge gives c = 0 and a dropped term.
+template <typename TO, int TM, int TT>
+__global__ __launch_bounds__(256) void k_assemble_vrc(const int32_t *__restrict__ tab, const double *__restrict__ pv,
+                                                      int64_t N, TargetView tv, TO *__restrict__ G, int64_t ld,
+                                                      int64_t col0, float *__restrict__ G2, double off, double cut)
+{
+    __shared__ double st[6][TM];
+    __shared__ double se[2][TT][2 * VRE_P];
+    const int tid = threadIdx.x;
+    const int32_t *T = tab + (int64_t)blockIdx.x * VRC_BLOCK_INTS;
+    const int64_t i0 = (int64_t)blockIdx.y * TM;
+    for (int t = tid; t < TM; t += blockDim.x) {
+        const int64_t i = i0 + t;
+        const bool ok = i < tv.M;
+        st[0][t] = ok ? tv.x[i] : 0.0;
+        st[1][t] = ok ? tv.y[i] : 0.0;
+        st[2][t] = ok ? tv.z[i] : 0.0;
+        st[3][t] = ok ? tv.nx[i] : 0.0;
+        st[4][t] = ok ? tv.ny[i] : 0.0;
+        st[5][t] = ok ? tv.nz[i] : 0.0;
+    }
+    auto node_xyz = [&](int32_t ref, double &x, double &y, double &z) {
+        const int64_t j = ref >> 3;
+        const int k = ref & 7;
+        x = pv[(3 * k + 0) * N + j];
+        y = pv[(3 * k + 1) * N + j];
+        z = pv[(3 * k + 2) * N + j];
+    };
+    // the lane's path (idle lanes take lane 0's first node, or vertex 0 of panel 0)
+    const int32_t q0 = T[VRC_OFF_NODE + tid], q1 = T[VRC_OFF_NODE + VRE_P + tid],
+                  q2 = T[VRC_OFF_NODE + 2 * VRE_P + tid];
+    const int32_t f0 = q0 >= 0 ? q0 : (T[VRC_OFF_NODE] >= 0 ? T[VRC_OFF_NODE] : 0);
+    const int32_t f1 = q1 >= 0 ? q1 : f0, f2 = q2 >= 0 ? q2 : f1;
+    double p0x, p0y, p0z, p1x, p1y, p1z, p2x, p2y, p2z;
+    node_xyz(f0, p0x, p0y, p0z);
+    node_xyz(f1, p1x, p1y, p1z);
+    node_xyz(f2, p2x, p2y, p2z);
+    const double e0x = p1x - p0x, e0y = p1y - p0y, e0z = p1z - p0z;
+    const double e1x = p2x - p1x, e1y = p2y - p1y, e1z = p2z - p1z;
+    const int32_t col = T[VRC_OFF_COL + tid];
+    int ri[4];
+    double rs[4];
+    {
+        const uint32_t r01 = (uint32_t)T[VRC_OFF_REF + 2 * tid], r23 = (uint32_t)T[VRC_OFF_REF + 2 * tid + 1];
+        const uint32_t r[4] = {r01 & 0xFFFFu, r01 >> 16, r23 & 0xFFFFu, r23 >> 16};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool ok = col >= 0 && r[k] != VRE_NOREF;
+            ri[k] = ok ? (int)(r[k] >> 1) : 0;
+            rs[k] = ok ? ((r[k] & 1u) ? -1.0 : 1.0) : 0.0;
+        }
+    }
+    const double off2 = off * off, cut2 = cut * cut, k4 = 4.0 * FP_PI;
+    const int iend = (int)min((int64_t)TM, tv.M - i0);
+    TO *g = G + i0 * ld + col0 + (col >= 0 ? col : 0);
+    float *g2 = G2 ? G2 + i0 * ld + col0 + (col >= 0 ? col : 0) : nullptr;
+    __syncthreads();
+    int buf = 0;
+    for (int t0 = 0; t0 < iend; t0 += TT, buf ^= 1) {
+        // node and edge terms (a partial last pass repeats the last target: never stored twice)
+#pragma unroll
+        for (int tt = 0; tt < TT; ++tt) {
+            const int t = min(t0 + tt, iend - 1);
+            const double X = st[0][t], Y = st[1][t], Z = st[2][t];
+            const double nx = st[3][t], ny = st[4][t], nz = st[5][t];
+            const double ax = X - p0x, ay = Y - p0y, az = Z - p0z;
+            const double bx = X - p1x, by = Y - p1y, bz = Z - p1z;
+            const double cx_ = X - p2x, cy_ = Y - p2y, cz_ = Z - p2z;
+            const double ia = fp_recip(fp_sqrt_nb(ax * ax + ay * ay + az * az) + off);
+            const double ib = fp_recip(fp_sqrt_nb(bx * bx + by * by + bz * bz) + off);
+            const double ic = fp_recip(fp_sqrt_nb(cx_ * cx_ + cy_ * cy_ + cz_ * cz_) + off);
+            // edge n0 -> n1
+            {
+                const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+                const double c2 = cx * cx + cy * cy + cz * cz;
+                const double proj = e0x * (ax * ia - bx * ib) + e0y * (ay * ia - by * ib) + e0z * (az * ia - bz * ib);
+                const double w = proj * fp_recip(k4 * (c2 + off2)) * (cx * nx + cy * ny + cz * nz);
+                se[buf][tt][tid] = c2 > cut2 ? w : 0.0;
+            }
+            // edge n1 -> n2
+            {
+                const double cx = by * cz_ - bz * cy_, cy = bz * cx_ - bx * cz_, cz = bx * cy_ - by * cx_;
+                const double c2 = cx * cx + cy * cy + cz * cz;
+                const double proj = e1x * (bx * ib - cx_ * ic) + e1y * (by * ib - cy_ * ic) + e1z * (bz * ib - cz_ * ic);
+                const double w = proj * fp_recip(k4 * (c2 + off2)) * (cx * nx + cy * ny + cz * nz);
+                se[buf][tt][VRE_P + tid] = c2 > cut2 ? w : 0.0;
+            }
+        }
+        __syncthreads();
+        // panel sums; the next pass writes the other buffer, and the pass after it
+        // comes after the next barrier, which every lane reaches after these reads
+        if (col >= 0) {
+#pragma unroll
+            for (int tt = 0; tt < TT; ++tt) {
+                const int t = t0 + tt;
+                double v = 0.0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) v += rs[k] * se[buf][tt][ri[k]];
+                if (t < iend) {
+                    g[(int64_t)t * ld] = TO(v);
+                    if (g2) g2[(int64_t)t * ld] = float(v);
+                }
+            }
+        }
+    }
+}
+
 // Launch the shared-term kernel for a body with block tables; returns 1 when the
 // body has none (the caller then uses the panel-wise kernel).
 template <typename T>
@@ -434,6 +728,23 @@ int fp_launch_assemble_edges(fp_ctx *ctx, const fp_body *b, const TargetView &tv
     if (env && strcmp(env, "panel") == 0) return FP_OK;
     *used = 1;
     if (tv.M == 0) return FP_OK;
+    if (b->d_pb && !(env && strcmp(env, "nodes") == 0)) {
+        // edge pairs (default); FLOWPANEL_VRC_SHAPE for measurement
+        const char *shape = getenv("FLOWPANEL_VRC_SHAPE");
+        const std::string sh = shape ? shape : "64x2";
+        ProfScope ps(ctx, sizeof(T) == 8 ? "assemble_vrc_f64" : "assemble_vrc_f32");
+#define FP_VRC(TMV, TTV)                                                                                       \
+        if (sh == #TMV "x" #TTV) {                                                                               \
+            dim3 grid((unsigned)b->pb_nblocks, (unsigned)((tv.M + TMV - 1) / TMV));                            \
+            hipLaunchKernelGGL((k_assemble_vrc<T, TMV, TTV>), grid, dim3(VRE_P), 0, ctx->stream, b->d_pb,       \
+                               b->d_pv, b->N, tv, G, ld, col0, G32, b->kernel_offset, b->kernel_cutoff);      \
+            FP_HIP(ctx, hipGetLastError());                                                                    \
+            return FP_OK;                                                                                      \
+        }
+        FP_VRC(64, 2) FP_VRC(64, 1) FP_VRC(64, 4) FP_VRC(128, 2) FP_VRC(32, 2)
+#undef FP_VRC
+        return fp_fail(ctx, FP_ERR_ARG, "FLOWPANEL_VRC_SHAPE: one of 64x2, 64x1, 64x4, 128x2, 32x2");
+    }
     // targets per workgroup tile x targets per pass (FLOWPANEL_VRE_SHAPE, for measurement)
     const char *shape = getenv("FLOWPANEL_VRE_SHAPE");
     const std::string sh = shape ? shape : "64x2";

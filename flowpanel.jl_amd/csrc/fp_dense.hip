@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 typedef float fp_f4 __attribute__((ext_vector_type(4)));
 
@@ -1674,18 +1675,11 @@ int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, 
     // look-ahead: once X_{J+1,J+1} has its update from chunk J, its Gauss-Jordan
     // pass runs on a second stream beside the rest of chunk J's products
     if (lookahead && !ctx->side2) FP_HIP(ctx, hipStreamCreateWithFlags(&ctx->side2, hipStreamNonBlocking));
-    struct Events {
-        hipEvent_t fork = nullptr, join = nullptr;
-        ~Events()
-        {
-            if (fork) (void)hipEventDestroy(fork);
-            if (join) (void)hipEventDestroy(join);
-        }
-    } ev;
-    if (lookahead) {
-        FP_HIP(ctx, hipEventCreateWithFlags(&ev.fork, hipEventDisableTiming));
-        FP_HIP(ctx, hipEventCreateWithFlags(&ev.join, hipEventDisableTiming));
-    }
+    if (lookahead && !ctx->gj_fork) FP_HIP(ctx, hipEventCreateWithFlags(&ctx->gj_fork, hipEventDisableTiming));
+    if (lookahead && !ctx->gj_join) FP_HIP(ctx, hipEventCreateWithFlags(&ctx->gj_join, hipEventDisableTiming));
+    struct {
+        hipEvent_t fork, join;
+    } ev{ctx->gj_fork, ctx->gj_join};
     // prefilled: inv already holds the identity-padded diagonal blocks (A unused)
     if (!prefilled) hipLaunchKernelGGL(k_bj_copy<T>, dim3(block, nb), dim3(256), 0, ctx->stream, A, n, ld, block, inv);
     if (copied) FP_HIP(ctx, hipEventRecord(copied, ctx->stream));
@@ -1777,12 +1771,80 @@ __global__ __launch_bounds__(256) void k_bj_apply_inv(const float *__restrict__ 
     }
 }
 
+// The same with 16-byte loads (block % 4 == 0, 16-byte aligned inv and x):
+// lane l takes entries 4 l .. 4 l + 3 of every 256 of a row, KM4 steps per row.
+template <int KM4>
+__global__ __launch_bounds__(256) void k_bj_apply_inv4(const float *__restrict__ inv, int64_t n, int block,
+                                                       const double *__restrict__ x, double *__restrict__ y,
+                                                       int accumulate)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 4;
+    if (r0 >= n) return;
+    double acc[4];
+    float4 v[4][KM4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int64_t r = min(r0 + q, n - 1);
+        const int64_t b = r / block, b0 = b * block;
+        const float *row = inv + b * block * block + (r - b0) * block;
+#pragma unroll
+        for (int k = 0; k < KM4; ++k) {
+            const int c = 4 * lane + 256 * k;
+            v[q][k] = c < block ? *reinterpret_cast<const float4 *>(row + c) : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int64_t r = min(r0 + q, n - 1);
+        const int64_t b = r / block, b0 = b * block;
+        const int bs = (int)min((int64_t)block, n - b0);
+        double a = 0;
+#pragma unroll
+        for (int k = 0; k < KM4; ++k) {
+            const int c = 4 * lane + 256 * k;
+            // entries at or beyond bs of a short last block are the identity padding's zeros
+            if (c + 3 < bs) {
+                const double2 xa = *reinterpret_cast<const double2 *>(x + b0 + c);
+                const double2 xb = *reinterpret_cast<const double2 *>(x + b0 + c + 2);
+                a += (double)v[q][k].x * xa.x + (double)v[q][k].y * xa.y + (double)v[q][k].z * xb.x +
+                     (double)v[q][k].w * xb.y;
+            } else {
+                const float e[4] = {v[q][k].x, v[q][k].y, v[q][k].z, v[q][k].w};
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if (c + t < bs) a += (double)e[t] * x[b0 + c + t];
+            }
+        }
+        acc[q] = a;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o);
+    if (lane < 4 && r0 + lane < n) {
+        double a = acc[0];
+#pragma unroll
+        for (int q = 1; q < 4; ++q)
+            if (lane == q) a = acc[q];
+        const int64_t r = r0 + lane;
+        y[r] = accumulate ? y[r] + a : a;
+    }
+}
+
 int fp_bj_apply_inv(fp_ctx *ctx, const float *inv, int64_t n, int block, const double *x, double *y, int accumulate)
 {
     if (block > 1024) return fp_fail(ctx, FP_ERR_ARG, "block-Jacobi block larger than 1024");
     ProfScope ps(ctx, "bj_apply");
     const dim3 g((unsigned)((n + 15) / 16));
-    if (block <= 512)
+    const char *sc = getenv("FLOWPANEL_BJ_APPLY_SCALAR");
+    const bool vec = block % 4 == 0 && (uintptr_t)inv % 16 == 0 && (uintptr_t)x % 16 == 0 && !(sc && sc[0] == '1');
+    if (vec && block <= 256)
+        hipLaunchKernelGGL(k_bj_apply_inv4<1>, g, dim3(256), 0, ctx->stream, inv, n, block, x, y, accumulate);
+    else if (vec && block <= 512)
+        hipLaunchKernelGGL(k_bj_apply_inv4<2>, g, dim3(256), 0, ctx->stream, inv, n, block, x, y, accumulate);
+    else if (vec)
+        hipLaunchKernelGGL(k_bj_apply_inv4<4>, g, dim3(256), 0, ctx->stream, inv, n, block, x, y, accumulate);
+    else if (block <= 512)
         hipLaunchKernelGGL(k_bj_apply_inv<8>, g, dim3(256), 0, ctx->stream, inv, n, block, x, y, accumulate);
     else
         hipLaunchKernelGGL(k_bj_apply_inv<16>, g, dim3(256), 0, ctx->stream, inv, n, block, x, y, accumulate);
@@ -1795,7 +1857,7 @@ int fp_bj_apply_inv(fp_ctx *ctx, const float *inv, int64_t n, int block, const d
 // AZ = A Z: segmented row sums of A, one wave per row.
 template <typename T>
 __global__ __launch_bounds__(256) void k_chunk_rowsum(const T *__restrict__ A, int64_t n, int64_t ld, int cs, int nc,
-                                                      double *__restrict__ AZ)
+                                                      float *__restrict__ AZ)
 {
     // four rows per wave: four independent loads and reductions in flight
     const int lane = threadIdx.x & 63;
@@ -1820,7 +1882,7 @@ __global__ __launch_bounds__(256) void k_chunk_rowsum(const T *__restrict__ A, i
 #pragma unroll
             for (int q = 1; q < 4; ++q)
                 if (lane == q) a = acc[q];
-            AZ[(int64_t)k * n + r0 + lane] = a;
+            AZ[(int64_t)k * n + r0 + lane] = (float)a;
         }
     }
 }
@@ -1830,7 +1892,7 @@ __global__ __launch_bounds__(256) void k_chunk_rowsum(const T *__restrict__ A, i
 // loads).
 template <typename T>
 __global__ __launch_bounds__(256) void k_chunk_rowsum_lane(const T *__restrict__ A, int64_t n, int64_t ld, int cs,
-                                                           int nc, double *__restrict__ AZ)
+                                                           int nc, float *__restrict__ AZ)
 {
     const int lane = threadIdx.x & 63;
     const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -1843,12 +1905,70 @@ __global__ __launch_bounds__(256) void k_chunk_rowsum_lane(const T *__restrict__
         double acc = 0;
 #pragma unroll 8
         for (int64_t c = c0; c < c1; ++c) acc += (double)a[c];
-        AZ[(int64_t)k * n + r] = acc;
+        AZ[(int64_t)k * n + r] = (float)acc;
     }
 }
 
+// Chunk sums of rows for chunks shorter than a wave, read coalesced:
+// out[k * nrows + r] = sum of A[r * ld + c] over the columns c of chunk k
+// (columns k cs .. k cs + cs - 1, the last chunk cut at ncols), summed in FP64 in
+// column order.  Each wave takes one row and walks it in segments of 64 chunks:
+// the segment goes into the wave's slice of LDS with 16-byte loads, then lane j
+// sums chunk j of the segment out of LDS.  The four waves of a workgroup take
+// the same number of segments (the barriers are uniform); dynamic LDS of
+// 4 * 64 * cs elements.
+template <typename T, typename OT>
+__global__ __launch_bounds__(256) void k_chunk_rowsum_seg(const T *__restrict__ A, int64_t nrows, int64_t ncols,
+                                                          int64_t ld, int cs, int vec, OT *__restrict__ out)
+{
+    extern __shared__ __align__(16) unsigned char seg_raw[];
+    constexpr int V = 16 / sizeof(T);
+    using VT = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t r = (int64_t)blockIdx.x * 4 + wave;
+    const bool live = r < nrows;
+    T *seg = reinterpret_cast<T *>(seg_raw) + (int64_t)wave * 64 * cs;
+    const int64_t span = (int64_t)64 * cs;
+    for (int64_t s0 = 0; s0 < ncols; s0 += span) {
+        const int L = (int)min(span, ncols - s0);
+        if (live) {
+            const T *a = A + r * ld + s0;
+            const int LV = vec ? L / V * V : 0;   // 16-byte loads where the rows are 16-byte aligned
+            for (int i = lane * V; i < LV; i += 64 * V)
+                *reinterpret_cast<VT *>(seg + i) = *reinterpret_cast<const VT *>(a + i);
+            for (int i = LV + lane; i < L; i += 64) seg[i] = a[i];
+        }
+        __syncthreads();
+        const int c0 = lane * cs;
+        if (live && c0 < L) {
+            const int e = min(cs, L - c0);
+            double acc = 0;
+            for (int c = 0; c < e; ++c) acc += (double)seg[c0 + c];
+            out[(s0 / cs + lane) * nrows + r] = (OT)acc;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, typename OT>
+static int chunk_rowsum_seg(fp_ctx *ctx, const T *A, int64_t nrows, int64_t ncols, int64_t ld, int cs, OT *out)
+{
+    const size_t lds = (size_t)4 * 64 * cs * sizeof(T);
+    static bool raised = false;   // per instantiation
+    if (lds > 65536 && !raised) {
+        FP_HIP(ctx, hipFuncSetAttribute((const void *)k_chunk_rowsum_seg<T, OT>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        raised = true;
+    }
+    const int vec = ((uintptr_t)A % 16 == 0 && (ld * sizeof(T)) % 16 == 0) ? 1 : 0;
+    hipLaunchKernelGGL((k_chunk_rowsum_seg<T, OT>), dim3((unsigned)((nrows + 3) / 4)), dim3(256), lds, ctx->stream, A,
+                       nrows, ncols, ld, cs, vec, out);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
 // E = Z^T AZ (nc x nc, row-major): E[k'][k] = sum over chunk k' of AZ[:, k]
-__global__ void k_coarse_galerkin(const double *__restrict__ AZ, int64_t n, int cs, int nc, double *__restrict__ E)
+__global__ void k_coarse_galerkin(const float *__restrict__ AZ, int64_t n, int cs, int nc, double *__restrict__ E)
 {
     const int kp = blockIdx.x, k = blockIdx.y;
     __shared__ double red[4];
@@ -1955,20 +2075,30 @@ __global__ __launch_bounds__(256) void k_widen_f32(const float *__restrict__ x, 
 }
 
 template <typename T>
-int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, double *AZ, double *Einv, double *work,
+int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, float *AZ, double *Einv, double *work,
                     int *info_dev)
 {
     const int nc = (int)((n + cs - 1) / cs);
     if (nc > 1024) return fp_fail(ctx, FP_ERR_ARG, "coarse space larger than 1024 vectors");
     ProfScope ps(ctx, "coarse_setup");
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, info_dev, (int64_t)1, 0);
-    if (cs < 64)
-        hipLaunchKernelGGL(k_chunk_rowsum_lane<T>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, A, n, ld,
-                           cs, nc, AZ);
-    else
-        hipLaunchKernelGGL(k_chunk_rowsum<T>, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, ctx->stream, A, n, ld, cs,
-                           nc, AZ);
-    hipLaunchKernelGGL(k_coarse_galerkin, dim3(nc, nc), dim3(256), 0, ctx->stream, (const double *)AZ, n, cs, nc, Einv);
+    // AZ: chunk sums of the rows of A; E = Z^T AZ: chunk sums of the columns of AZ
+    // (AZ column k is contiguous, so E[k'][k] is chunk k' of "row" k of AZ)
+    const char *lanev = getenv("FLOWPANEL_COARSE_LANE");
+    if (cs < 64 && !(lanev && lanev[0] == '1')) {
+        int st = chunk_rowsum_seg<T, float>(ctx, A, n, n, ld, cs, AZ);
+        if (!st) st = chunk_rowsum_seg<float, double>(ctx, AZ, nc, n, n, cs, Einv);
+        if (st) return st;
+    } else {
+        if (cs < 64)
+            hipLaunchKernelGGL(k_chunk_rowsum_lane<T>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, A, n,
+                               ld, cs, nc, AZ);
+        else
+            hipLaunchKernelGGL(k_chunk_rowsum<T>, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, ctx->stream, A, n, ld,
+                               cs, nc, AZ);
+        hipLaunchKernelGGL(k_coarse_galerkin, dim3(nc, nc), dim3(256), 0, ctx->stream, (const float *)AZ, n, cs, nc,
+                           Einv);
+    }
     if (nc <= SMALL_INV_LDS) {
         hipLaunchKernelGGL(k_small_inverse_lds, dim3(1), dim3(1024), 0, ctx->stream, Einv, nc, info_dev);
     } else {
@@ -1986,8 +2116,8 @@ int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, doub
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }
-template int fp_coarse_setup<double>(fp_ctx *, const double *, int64_t, int64_t, int, double *, double *, double *, int *);
-template int fp_coarse_setup<float>(fp_ctx *, const float *, int64_t, int64_t, int, double *, double *, double *, int *);
+template int fp_coarse_setup<double>(fp_ctx *, const double *, int64_t, int64_t, int, float *, double *, double *, int *);
+template int fp_coarse_setup<float>(fp_ctx *, const float *, int64_t, int64_t, int, float *, double *, double *, int *);
 
 // t = Z^T v: one wave per chunk (chunk sums of v)
 __global__ __launch_bounds__(256) void k_coarse_restrict(const double *__restrict__ v, int64_t n, int cs, int nc,
@@ -2018,7 +2148,7 @@ __global__ __launch_bounds__(256) void k_coarse_solve(const double *__restrict__
 
 // y = Z c, r2 = v - AZ c: 64 rows per workgroup, the nc columns of AZ split
 // over the 4 waves (coalesced rows of AZ, summed through LDS)
-__global__ __launch_bounds__(256) void k_coarse_resid(const double *__restrict__ v, const double *__restrict__ AZ,
+__global__ __launch_bounds__(256) void k_coarse_resid(const double *__restrict__ v, const float *__restrict__ AZ,
                                                       int64_t n, int cs, int nc, const double *__restrict__ c,
                                                       double *__restrict__ y, double *__restrict__ r2)
 {
@@ -2029,13 +2159,13 @@ __global__ __launch_bounds__(256) void k_coarse_resid(const double *__restrict__
     if (i < n) {
         int k = wave;
         for (; k + 28 < nc; k += 32) {   // 8 independent loads per step
-            double a[8];
+            float a[8];
 #pragma unroll
             for (int q = 0; q < 8; ++q) a[q] = AZ[(int64_t)(k + 4 * q) * n + i];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) acc += a[q] * c[k + 4 * q];
+            for (int q = 0; q < 8; ++q) acc += (double)a[q] * c[k + 4 * q];
         }
-        for (; k < nc; k += 4) acc += AZ[(int64_t)k * n + i] * c[k];
+        for (; k < nc; k += 4) acc += (double)AZ[(int64_t)k * n + i] * c[k];
     }
     part[wave][lane] = acc;
     __syncthreads();
@@ -2045,7 +2175,7 @@ __global__ __launch_bounds__(256) void k_coarse_resid(const double *__restrict__
     }
 }
 
-int fp_coarse_apply(fp_ctx *ctx, const double *v, int64_t n, int cs, const double *AZ, const double *Einv, double *tc,
+int fp_coarse_apply(fp_ctx *ctx, const double *v, int64_t n, int cs, const float *AZ, const double *Einv, double *tc,
                     double *y, double *r2)
 {
     const int nc = (int)((n + cs - 1) / cs);

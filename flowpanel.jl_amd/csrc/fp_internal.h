@@ -56,6 +56,9 @@ struct fp_ctx {
     hipStream_t side2 = nullptr;
     // fourth stream: block-Jacobi inverses formed beside the assembly
     hipStream_t side3 = nullptr;
+    // fork/join events of the look-ahead Gauss-Jordan passes, kept for the
+    // context's life (destroying an event that is still pending holds the host)
+    hipEvent_t gj_fork = nullptr, gj_join = nullptr;
     // cached scratch (grown on demand, freed with the context)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -129,6 +132,8 @@ struct fp_body {
     int64_t *d_sh = nullptr;      // shedding table on the device (built on first use)
     int32_t *d_eb = nullptr;      // shared node/edge assembly blocks (triangles and quads)
     int64_t eb_nblocks = 0, eb_nodes = 0, eb_edges = 0;
+    int32_t *d_pb = nullptr;      // the same blocks with their edges paired per lane (edge-pair assembly)
+    int64_t pb_nblocks = 0;
     int32_t *d_tr_tab = nullptr;  // edge trails of the mesh (node refs, contributor CSR)
     double *d_tr = nullptr;       // trail stream filled from the strengths before each field evaluation
     int64_t tr_S = 0;
@@ -296,12 +301,13 @@ template <typename T>
 int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *inv, int *info_dev,
                      float *scratch, hipEvent_t copied = nullptr, bool lookahead = true, bool prefilled = false);
 int fp_bj_apply_inv(fp_ctx *ctx, const float *inv, int64_t n, int block, const double *x, double *y, int accumulate);
-// Galerkin coarse space of chunk indicators: AZ = A Z (n x nc), Einv = (Z^T A Z)^-1
+// Galerkin coarse space of chunk indicators: AZ = A Z (n x nc, FP32 rounding of FP64 sums),
+// Einv = (Z^T AZ)^-1 from that same AZ (so that Z^T (v - AZ Einv Z^T v) = 0)
 template <typename T>
-int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, double *AZ, double *Einv, double *work,
+int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, float *AZ, double *Einv, double *work,
                     int *info_dev);
 // y = Z Einv Z^T v and r2 = v - AZ Einv Z^T v (tc: 2*nc scratch)
-int fp_coarse_apply(fp_ctx *ctx, const double *v, int64_t n, int cs, const double *AZ, const double *Einv, double *tc,
+int fp_coarse_apply(fp_ctx *ctx, const double *v, int64_t n, int cs, const float *AZ, const double *Einv, double *tc,
                     double *y, double *r2);
 // small vector kernels
 int fp_vec_copy_convert(fp_ctx *ctx, const double *src, void *dst, int64_t n, int dst_prec);

@@ -224,6 +224,17 @@ int fp_assemble(fp_ctx *ctx, fp_body *const *bodies, int nbodies, int precision,
  * so that it can be inspected). */
 int64_t fp_assembly_blocks(const double *nodes, int64_t nnodes, const int64_t *cell_ptr, const int64_t *cell_idx,
                            int64_t ncells, int index_base, int32_t *tab, int64_t cap_blocks, int64_t *terms);
+/* Host only, no device: the edge-pair tables the default vortex-ring assembly uses (the same kind of
+ * column blocks, at most 504 distinct edges, each block's edges paired at shared nodes so that a lane
+ * owns a path n0 - n1 - n2: edge n0 -> n1 in slot lane, n1 -> n2 in slot 256 + lane).  Returns the
+ * number of blocks (negative FP_ERR_* on bad input); when tab is not null and cap_blocks is at least
+ * that number, writes 1536 int32 per block: 256 columns (-1 = none), 256 pairs of packed 16-bit slot
+ * references ((slot << 1) | reversed, 0xFFFF = none), then the node references (panel * 8 + vertex,
+ * -1 = none) of n0, n1 and n2, 256 each; terms (2 values, may be null) = the lanes used and the edges,
+ * summed over the blocks.  No counterpart in the reference (a layout of this library, exported so that
+ * it can be inspected). */
+int64_t fp_assembly_pairs(const double *nodes, int64_t nnodes, const int64_t *cell_ptr, const int64_t *cell_idx,
+                          int64_t ncells, int index_base, int32_t *tab, int64_t cap_blocks, int64_t *terms);
 /* Host only, no device: the edge trails the velocity of a vortex-ring body is summed over (every mesh edge
  * once, gamma_e = the signed strengths of the panels that hold it).  Returns the number S of stream
  * entries (negative FP_ERR_* on bad input) and, in *ncontrib, the number of (panel, sign) contributors;

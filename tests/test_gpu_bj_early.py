@@ -6,7 +6,8 @@ rounded to FP32, the last block padded with the identity) and inverts them on a
 stream of their own while the whole matrix is assembled; the coarse space
 follows once the matrix is whole.  The solution must match the oracle's LU
 (1e-10 of max|Gamma|) and the solve with the inverses taken from the assembled
-matrix (FLOWPANEL_BJ_EARLY=0), with the iteration counts within one of each
+matrix (the default; FLOWPANEL_BJ_EARLY=1 selects the early set-up), with the
+iteration counts within one of each
 other, on quads with a wake, triangles with a wake and a ragged last block,
 blocks in registers (<= 256 rows) and blocked Gauss-Jordan (> 256 rows), with and
 without the coarse space, for the matrix-free and the stored FP64 residual and
@@ -53,6 +54,7 @@ def test_early_block_jacobi_matches_oracle_and_late(fp, geo, orc, case, block, c
     opts = {"precond_block": block, "coarse_block": coarse, "tol": 1e-12,
             "residual_mode": fp._native.FP_RESID_STORED if resid == "stored" else fp._native.FP_RESID_MATRIX_FREE}
     print("case %s, %d panels, block %d, coarse %d, %s" % (case, body.ncells, block, coarse, resid), flush=True)
+    monkeypatch.setenv("FLOWPANEL_BJ_EARLY", "1")
     st_e = fp.solve(body, U, D, D, solver=fp.solve_gmres_ir, solver_optargs=opts)
     x_e = body.strength.copy()
     monkeypatch.setenv("FLOWPANEL_BJ_EARLY", "0")
@@ -66,7 +68,8 @@ def test_early_block_jacobi_matches_oracle_and_late(fp, geo, orc, case, block, c
     assert abs(st_e["iterations"] - st_l["iterations"]) <= max(1, st_l["refinements"])
 
 
-def test_early_block_jacobi_several_freestreams(fp, geo, orc):
+def test_early_block_jacobi_several_freestreams(fp, geo, orc, monkeypatch):
+    monkeypatch.setenv("FLOWPANEL_BJ_EARLY", "1")
     body = _body(fp, geo, "quads")
     aoas = (0.0, 4.0, 8.0)
     Us = [np.tile(fp.direction(a)[:, None], (1, body.ncells)) for a in aoas]

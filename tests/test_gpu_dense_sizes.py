@@ -126,6 +126,47 @@ def test_coarse_space_above_128_vectors(fp, coarse):
     assert it2 <= it1, (it2, it1)
 
 
+@pytest.mark.parametrize("n,coarse", [(1001, 7), (1200, 50), (999, 63), (640, 1)])
+def test_coarse_chunk_sums_coalesced_match_lane_sums(fp, n, coarse, monkeypatch):
+    """The coarse space's chunk sums (A Z and Z^T A Z) read through LDS segments
+    against the one-lane-per-chunk kernels (FLOWPANEL_COARSE_LANE=1): the same
+    iterations and the same solution on a nonsymmetric matrix with strong
+    coupling between chunks, rows of odd and even length (16-byte and 8-byte
+    aligned rows of A Z), a last chunk cut short and chunks of one entry."""
+    rng = np.random.default_rng(7000 + n)
+    A = rng.standard_normal((n, n)) / np.sqrt(n) + 2.0 * np.eye(n)
+    A += np.kron(rng.uniform(0.0, 0.4, (n // coarse + 1, n // coarse + 1)), np.ones((coarse, coarse)))[:n, :n] / coarse
+    b = rng.standard_normal(n)
+    y1 = fp.solve_gmres_ir(None, A, b, precond_block=100, coarse_block=coarse, tol=1e-12)
+    it1 = fp.solvers.last_stats["iterations"]
+    monkeypatch.setenv("FLOWPANEL_COARSE_LANE", "1")
+    y2 = fp.solve_gmres_ir(None, A, b, precond_block=100, coarse_block=coarse, tol=1e-12)
+    it2 = fp.solvers.last_stats["iterations"]
+    x = np.linalg.solve(A, b)
+    print("n %d chunk %d: iterations %d / %d, %.1e / %.1e" % (n, coarse, it1, it2, rel(y1, x), rel(y2, x)))
+    assert rel(y1, x) < 1e-10 and rel(y2, x) < 1e-10
+    assert abs(it1 - it2) <= 1
+
+
+@pytest.mark.parametrize("n,block", [(1001, 48), (1001, 100), (1001, 300), (2050, 1000), (1001, 250), (777, 1024)])
+def test_block_jacobi_apply_vector_loads(fp, n, block, monkeypatch):
+    """The block-Jacobi product with 16-byte loads (block % 4 == 0) against the
+    one-float-per-lane product (FLOWPANEL_BJ_APPLY_SCALAR=1): both solves reach
+    the FP64 target with the same iterations, with a short last block and a
+    block of 1024; block 250 takes the one-float path either way."""
+    rng = np.random.default_rng(7100 + block)
+    A = rng.standard_normal((n, n)) * 0.4 / np.sqrt(n) + 2.0 * np.eye(n)
+    b = rng.standard_normal(n)
+    y1 = fp.solve_gmres_ir(None, A, b, precond_block=block, tol=1e-12)
+    it1 = fp.solvers.last_stats["iterations"]
+    monkeypatch.setenv("FLOWPANEL_BJ_APPLY_SCALAR", "1")
+    y2 = fp.solve_gmres_ir(None, A, b, precond_block=block, tol=1e-12)
+    it2 = fp.solvers.last_stats["iterations"]
+    x = np.linalg.solve(A, b)
+    assert rel(y1, x) < 1e-10 and rel(y2, x) < 1e-10
+    assert abs(it1 - it2) <= 1, (it1, it2)
+
+
 # ---- FP32 LU of the randomized matrix without pivoting (the LU32 refinement path) ----
 # The no-pivot LU works in 64-column steps on U^T A V, n padded to a multiple of 4:
 # sizes straddle the pad and the 64-row blocks.
