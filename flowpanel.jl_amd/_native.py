@@ -47,7 +47,8 @@ EXPORTS = [
     "fp_solve_opts_default", "fp_ctx_create", "fp_ctx_destroy", "fp_ctx_last_error", "fp_ctx_synchronize", "fp_ctx_trim",
     "fp_body_create", "fp_body_destroy", "fp_body_ncells", "fp_body_nsheddings", "fp_body_geometry",
     "fp_body_set_freestream", "fp_body_set_strength", "fp_body_get_strength", "fp_solve_bodies",
-    "fp_assemble", "fp_mat_upload", "fp_mat_size", "fp_mat_download", "fp_mat_destroy", "fp_solve_mat",
+    "fp_assemble", "fp_mat_upload", "fp_mat_size", "fp_mat_block_inverses", "fp_mat_download", "fp_mat_destroy",
+    "fp_solve_mat",
     "fp_mat_getrf", "fp_uind", "fp_phi", "fp_calcfield_U", "fp_calcfield_cp", "fp_ctx_profile",
     "fp_ctx_profile_only", "fp_ctx_profile_get", "fp_ctx_copy_calibrate", "fp_body_nnodes", "fp_body_set_nodes",
     "fp_liftdrag_opts_default", "fp_lift_drag", "fp_lift_drag_adjoint", "fp_solve_bodies_multi",
@@ -151,6 +152,8 @@ def lib():
     L.fp_mat_upload.argtypes = [_vp, i64, _d, pp]
     L.fp_mat_size.argtypes = [_vp]
     L.fp_mat_size.restype = i64
+    L.fp_mat_block_inverses.argtypes = [_vp, _vp, cint, ctypes.POINTER(ctypes.c_float)]
+    L.fp_mat_block_inverses.restype = cint
     L.fp_mat_download.argtypes = [_vp, _vp, _d]
     L.fp_mat_destroy.argtypes = [_vp]
     L.fp_mat_destroy.restype = None

@@ -346,15 +346,11 @@ struct BJPrecond : Precond {
             FP_HIP(ctx, hipEventCreateWithFlags(&fork, hipEventDisableTiming));
             FP_HIP(ctx, hipEventCreateWithFlags(&join, hipEventDisableTiming));
         }
-        st = prec == FP_F64 ? fp_bj_gj_inverse<double>(ctx, (const double *)A, n, ld, block, inv.as<float>(), info_dev,
-                                                       sp, fork)
-                            : fp_bj_gj_inverse<float>(ctx, (const float *)A, n, ld, block, inv.as<float>(), info_dev,
-                                                      sp, fork);
-        if (st) return st;
         if (coarse > 0) {
             // the coarse space does not depend on the block inverses: it is built on
-            // the side stream beside the Gauss-Jordan steps, once the block copy has
-            // read its share of A
+            // the side stream beside them, and queued first (the host takes a few
+            // microseconds per launch, and the block inverses are about a hundred)
+            FP_HIP(ctx, hipEventRecord(fork, ctx->stream));
             FP_HIP(ctx, hipStreamWaitEvent(ctx->side, fork, 0));
             fp_ctx sc;
             sc.device = ctx->device;
@@ -366,8 +362,13 @@ struct BJPrecond : Precond {
                                                          Einv.as<double>(), work.as<double>(), cinfo.as<int>());
             if (st) return fp_fail(ctx, st, sc.err.c_str());
             FP_HIP(ctx, hipEventRecord(join, ctx->side));
-            FP_HIP(ctx, hipStreamWaitEvent(ctx->stream, join, 0));
         }
+        st = prec == FP_F64 ? fp_bj_gj_inverse<double>(ctx, (const double *)A, n, ld, block, inv.as<float>(), info_dev,
+                                                       sp)
+                            : fp_bj_gj_inverse<float>(ctx, (const float *)A, n, ld, block, inv.as<float>(), info_dev,
+                                                      sp);
+        if (st) return st;
+        if (coarse > 0) FP_HIP(ctx, hipStreamWaitEvent(ctx->stream, join, 0));
         return FP_OK;
     }
     int apply(const double *x, double *y) override
@@ -2347,6 +2348,42 @@ int fp_mat_upload(fp_ctx *ctx, int64_t n, const double *Ah, fp_mat **out)
 }
 
 int64_t fp_mat_size(const fp_mat *m) { return m ? m->n : -1; }
+
+int fp_mat_block_inverses(fp_ctx *ctx, const fp_mat *m, int block, float *out)
+{
+    if (!ctx || !m || !out || m->ctx != ctx || block <= 0) return FP_ERR_ARG;
+    if (m->factored != -1) return fp_fail(ctx, FP_ERR_ARG, "block inverses need a matrix that is not factorised");
+    if (block > 1024) return fp_fail(ctx, FP_ERR_ARG, "block-Jacobi block larger than 1024");
+    FP_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t n = m->n;
+    const int blk = (int)std::min<int64_t>(block, n);
+    const int64_t nb = (n + blk - 1) / blk;
+    DBuf inv, scr, info;
+    int st;
+    if ((st = dalloc(ctx, inv, (size_t)nb * blk * blk * sizeof(float)))) return st;
+    const size_t sf = fp_bj_gj_scratch_floats(n, blk);
+    if (sf && (st = dalloc(ctx, scr, sf * sizeof(float)))) return st;
+    if ((st = dalloc(ctx, info, sizeof(int)))) return st;
+    st = m->precision == FP_F64
+             ? fp_bj_gj_inverse<double>(ctx, (const double *)m->A, n, m->ld, blk, inv.as<float>(), info.as<int>(),
+                                        sf ? scr.as<float>() : nullptr)
+             : fp_bj_gj_inverse<float>(ctx, (const float *)m->A, n, m->ld, blk, inv.as<float>(), info.as<int>(),
+                                       sf ? scr.as<float>() : nullptr);
+    if (st) return st;
+    if (ctx->side2) FP_HIP(ctx, hipStreamSynchronize(ctx->side2));
+    int hinfo = 0;
+    FP_HIP(ctx, hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    FP_HIP(ctx, hipMemcpyAsync(out, inv.p, (size_t)nb * blk * blk * sizeof(float), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    FP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (hinfo) {
+        char buf[128];
+        snprintf(buf, sizeof buf, "block inverses: zero pivot in diagonal block %lld (global row %lld)",
+                 (long long)((hinfo - 1) / blk), (long long)(hinfo - 1));
+        return fp_fail(ctx, FP_ERR_SINGULAR, buf);
+    }
+    return FP_OK;
+}
 
 int fp_mat_download(fp_ctx *ctx, const fp_mat *m, double *Ah)
 {

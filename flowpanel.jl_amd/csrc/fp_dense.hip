@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstring>
 #include <type_traits>
 
 typedef float fp_f4 __attribute__((ext_vector_type(4)));
@@ -1627,13 +1628,135 @@ __global__ __launch_bounds__(256) void k_bj_panels(float *__restrict__ X, int bl
     }
 }
 
+// Wave-wide maximum of v (lane 63 after row prefix maxima and the two row
+// broadcasts of the DPP network), returned to every lane.
+__device__ __forceinline__ int wave_max_i32(int v)
+{
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x111, 0xf, 0xf, false));   // row_shr:1
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x112, 0xf, 0xf, false));   // row_shr:2
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x114, 0xf, 0xf, false));   // row_shr:4
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x118, 0xf, 0xf, false));   // row_shr:8
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x142, 0xa, 0xf, false));   // row_bcast:15
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x143, 0xc, 0xf, false));   // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Gauss-Jordan with partial pivoting of a chunk of at most 64 rows in one wave:
+// lane r holds row r of the chunk in 64 registers (the same element layout and
+// result as k_bj_gj: source element (r, c) at A[b * a_bstride + (off + r) * a_ld +
+// off + c], rows never swapped, inverse(pinv(r), p_c) = W(r, c) scattered by the
+// final store).  Step k: the pivot is the unused row with the largest |a(r, k)|
+// (NaN counted as 0, the lowest row on ties): a DPP wave maximum of the FP32 bit
+// patterns (monotonic for non-negative values) and the lowest lane that holds
+// it (ballot); the pivot row reaches every lane through LDS (written by its lane,
+// read back as broadcasts); k and the columns are compile-time indices; no
+// barrier.  Rows at or beyond the chunk's size are never stored.
+// One pivot step k of k_bj_gj64 (k a compile-time value once inlined and unrolled).
+// The pivot row is not divided by its pivot: it is eliminated from the other rows
+// as it stands (multiplier a(r, k) / pivot) and keeps 1 in column k, so that
+// every row that has been a pivot row holds its final values times its own
+// pivot (later eliminations are linear in it) and is divided once at the end;
+// the pivot lane takes a zero multiplier, so one FMA per entry serves every lane.
+__device__ __forceinline__ void gj64_step(float (&a)[64], const int k, const int lane, const int bs, bool &used,
+                                          int &step, float &mypiv, int *prow_of, float4 *prow, int *__restrict__ info,
+                                          int64_t b0, int off)
+{
+    float av = fabsf(a[k]);
+    if (!(av == av)) av = 0.f;
+    const int key = used ? -1 : __float_as_int(av);
+    const int kmax = wave_max_i32(key);
+    const uint64_t hold = __builtin_amdgcn_ballot_w64(key == kmax);
+    const int p = (int)__builtin_ctzll(hold);
+    const float piv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a[k]), p));
+    if (piv == 0.f && lane == 0 && k < bs) atomicCAS(info, 0, (int)(b0 + off + k + 1));
+    const bool isp = lane == p;
+    const float g = isp ? 0.f : a[k] / piv;
+    // the pivot row to every lane: lane p writes it to LDS, every lane reads it back
+    // as broadcasts (the LDS operations of one wave complete in order)
+    if (isp) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) prow[c] = make_float4(a[4 * c], a[4 * c + 1], a[4 * c + 2], a[4 * c + 3]);
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int c4 = 0; c4 < 16; ++c4) {
+        const float4 v = prow[c4];
+        const float e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int c = 4 * c4 + t;
+            if (c != k) a[c] = fmaf(-g, e[t], a[c]);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    a[k] = isp ? 1.f : -g;
+    used = used || isp;
+    step = isp ? k : step;
+    mypiv = isp ? piv : mypiv;
+    if (lane == 0) prow_of[k] = p;
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_bj_gj64(const T *A, int64_t a_bstride, int64_t a_ld, float *inv,
+                                                int64_t i_bstride, int64_t i_ld, int64_t n, int block, int off,
+                                                int hs, int *__restrict__ info)
+{
+    __shared__ int prow_of[64];
+    __shared__ float4 prow[16];
+    const int b = blockIdx.x, lane = threadIdx.x;
+    const int64_t b0 = (int64_t)b * block;
+    const int bs = hs < 0 ? (int)min((int64_t)block, n - b0) : hs;
+    const T *src = A + (int64_t)b * a_bstride + (int64_t)off * a_ld + off;
+    // rows and columns at or beyond bs: the identity's, so that steps k >= bs pivot on
+    // row k with 1 and change nothing (every step runs, the row stays in registers)
+    float a[64];
+#pragma unroll
+    for (int c = 0; c < 64; ++c)
+        a[c] = (lane < bs && c < bs) ? (float)src[(int64_t)lane * a_ld + c] : (lane == c ? 1.f : 0.f);
+    bool used = false;
+    int step = 0;
+    float mypiv = 1.f;
+    // two halves, each within the compiler's limit for a fully unrolled loop
+#pragma unroll
+    for (int k = 0; k < 32; ++k) gj64_step(a, k, lane, bs, used, step, mypiv, prow_of, prow, info, b0, off);
+#pragma unroll
+    for (int k = 32; k < 64; ++k) gj64_step(a, k, lane, bs, used, step, mypiv, prow_of, prow, info, b0, off);
+    const float rp = 1.f / mypiv;
+#pragma unroll
+    for (int c = 0; c < 64; ++c) a[c] *= rp;
+    __syncthreads();
+    float *I = inv + (int64_t)b * i_bstride + (int64_t)off * i_ld + off;
+    if (lane < bs) {
+        float *row = I + (int64_t)step * i_ld;
+#pragma unroll
+        for (int c = 0; c < 64; ++c)
+            if (c < bs) row[prow_of[c]] = a[c];
+    }
+    if (hs < 0 && bs < block && lane < block) {   // a short whole block: the rest is the identity's
+        if (lane < bs) {
+            float *row = I + (int64_t)step * i_ld;
+            for (int c = bs; c < block; ++c) row[c] = 0.f;
+        } else {
+            float *row = I + (int64_t)lane * i_ld;
+            for (int c = 0; c < block; ++c) row[c] = c == lane ? 1.f : 0.f;
+        }
+    }
+}
+
 constexpr int GJ_TG = 16;               // Gauss-Jordan thread grid: 16 x 16 threads, 8 x 8 values each
-constexpr int GJ_CHUNK = 8 * GJ_TG;     // rows per Gauss-Jordan chunk
-static int bj_chunks(int block) { return (block + GJ_CHUNK - 1) / GJ_CHUNK; }
+// Rows per Gauss-Jordan chunk: 64 (one wave, k_bj_gj64), or 128 with the
+// 256-thread register grid of k_bj_gj (FLOWPANEL_BJ_GJ=grid).
+static bool gj_grid()
+{
+    const char *e = getenv("FLOWPANEL_BJ_GJ");
+    return e && strcmp(e, "grid") == 0;
+}
+static int gj_chunk() { return gj_grid() ? 8 * GJ_TG : 64; }
+static int bj_chunks(int block) { return (block + gj_chunk() - 1) / gj_chunk(); }
 
 size_t fp_bj_gj_scratch_floats(int64_t n, int block)
 {
-    if (block <= GJ_CHUNK) return 0;
+    if (block <= gj_chunk()) return 0;
     const int64_t nb = (n + block - 1) / block, h = (block + bj_chunks(block) - 1) / bj_chunks(block);
     return (size_t)(2 * nb * h * block);
 }
@@ -1658,14 +1781,21 @@ int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, 
     const int64_t bb = (int64_t)block * block;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, info_dev, (int64_t)1, 0);
     ProfScope ps(ctx, "bj_invert");
-    if (block <= GJ_CHUNK) {
+    const bool grid = gj_grid();
+    if (block <= gj_chunk()) {
         if (copied) FP_HIP(ctx, hipEventRecord(copied, ctx->stream));
-        if (prefilled)
+        if (prefilled && grid)
             hipLaunchKernelGGL((k_bj_gj<float, GJ_TG>), dim3(nb), dim3(GJ_TG * GJ_TG), 0, ctx->stream, (const float *)inv,
                                bb, (int64_t)block, inv, bb, (int64_t)block, n, block, 0, -1, info_dev);
-        else
+        else if (prefilled)
+            hipLaunchKernelGGL((k_bj_gj64<float>), dim3(nb), dim3(64), 0, ctx->stream, (const float *)inv, bb,
+                               (int64_t)block, inv, bb, (int64_t)block, n, block, 0, -1, info_dev);
+        else if (grid)
             hipLaunchKernelGGL((k_bj_gj<T, GJ_TG>), dim3(nb), dim3(GJ_TG * GJ_TG), 0, ctx->stream, A,
                                (int64_t)block * (ld + 1), ld, inv, bb, (int64_t)block, n, block, 0, -1, info_dev);
+        else
+            hipLaunchKernelGGL((k_bj_gj64<T>), dim3(nb), dim3(64), 0, ctx->stream, A, (int64_t)block * (ld + 1), ld,
+                               inv, bb, (int64_t)block, n, block, 0, -1, info_dev);
         FP_HIP(ctx, hipGetLastError());
         return FP_OK;
     }
@@ -1685,8 +1815,12 @@ int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, 
     if (copied) FP_HIP(ctx, hipEventRecord(copied, ctx->stream));
     auto gj = [&](int j, hipStream_t s) {
         const int j0 = j * h, hj = min(h, block - j0);
-        hipLaunchKernelGGL((k_bj_gj<float, GJ_TG>), dim3(nb), dim3(GJ_TG * GJ_TG), 0, s, (const float *)inv, bb, (int64_t)block, inv,
-                           bb, (int64_t)block, n, block, j0, hj, info_dev);
+        if (grid)
+            hipLaunchKernelGGL((k_bj_gj<float, GJ_TG>), dim3(nb), dim3(GJ_TG * GJ_TG), 0, s, (const float *)inv, bb,
+                               (int64_t)block, inv, bb, (int64_t)block, n, block, j0, hj, info_dev);
+        else
+            hipLaunchKernelGGL((k_bj_gj64<float>), dim3(nb), dim3(64), 0, s, (const float *)inv, bb, (int64_t)block,
+                               inv, bb, (int64_t)block, n, block, j0, hj, info_dev);
     };
     const int tb = (block + 63) / 64;
     gj(0, ctx->stream);

@@ -281,6 +281,27 @@ def calc_bc_noflowthrough(Us, normals):
     return -np.einsum("ij,ij->j", Us, normals)
 
 
+def block_inverses(A, block):
+    """The FP32 block-Jacobi inverses the Krylov solvers apply to A (blocks of
+    ``block`` rows along the index order, the last one padded with the identity):
+    array (nblocks, b, b), b = min(block, n)."""
+    A = np.asarray(A, dtype=np.float64)
+    n = A.shape[0]
+    b = min(int(block), n)
+    nb = (n + b - 1) // b
+    ctx = nat.context()
+    Af = nat.f64(A)
+    h = ctypes.c_void_p()
+    ctx.check(nat.lib().fp_mat_upload(ctx.handle, n, nat.dptr(Af), ctypes.byref(h)))
+    try:
+        out = np.zeros((nb, b, b), dtype=np.float32)
+        ctx.check(nat.lib().fp_mat_block_inverses(ctx.handle, h, int(block),
+                                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+    finally:
+        nat.lib().fp_mat_destroy(h)
+    return out
+
+
 def getrf(A, precision="f64"):
     """Device LU factorisation of a caller matrix: (LU, ipiv) in LAPACK form (1-based ipiv)."""
     A = np.asarray(A, dtype=np.float64)

@@ -224,7 +224,7 @@ int fp_assemble(fp_ctx *ctx, fp_body *const *bodies, int nbodies, int precision,
  * so that it can be inspected). */
 int64_t fp_assembly_blocks(const double *nodes, int64_t nnodes, const int64_t *cell_ptr, const int64_t *cell_idx,
                            int64_t ncells, int index_base, int32_t *tab, int64_t cap_blocks, int64_t *terms);
-/* Host only, no device: the edge-pair tables the default vortex-ring assembly uses (the same kind of
+/* Host only, no device: the edge-pair tables of the alternative vortex-ring assembly (the same kind of
  * column blocks, at most 504 distinct edges, each block's edges paired at shared nodes so that a lane
  * owns a path n0 - n1 - n2: edge n0 -> n1 in slot lane, n1 -> n2 in slot 256 + lane).  Returns the
  * number of blocks (negative FP_ERR_* on bad input); when tab is not null and cap_blocks is at least
@@ -248,6 +248,12 @@ int64_t fp_field_trails(const int64_t *cell_ptr, const int64_t *cell_idx, int64_
 /* Upload a caller matrix (column-major n x n) for the solver protocol. */
 int fp_mat_upload(fp_ctx *ctx, int64_t n, const double *A, fp_mat **out);
 int64_t fp_mat_size(const fp_mat *mat);
+/* The FP32 block-Jacobi inverses the Krylov solvers apply (blocks of `block` rows along the index order,
+ * Gauss-Jordan with partial pivoting in chunks of at most 64 rows, the last block padded with the
+ * identity) of a matrix that is not factorised: out (host) = ceil(n / b) row-major b x b blocks,
+ * b = min(block, n), block <= 1024.  FP_ERR_SINGULAR on a zero pivot.  No counterpart in the reference
+ * (the preconditioner of this library, exported so that it can be inspected). */
+int fp_mat_block_inverses(fp_ctx *ctx, const fp_mat *mat, int block, float *out);
 /* Column-major n x n copy of the (unfactorised) matrix. */
 int fp_mat_download(fp_ctx *ctx, const fp_mat *mat, double *A);
 void fp_mat_destroy(fp_mat *mat);

@@ -1,7 +1,7 @@
 """Edge-pair assembly tables of the vortex-ring AIC, checked on the CPU.
 
-The device's default vortex-ring assembly takes the shared-term column blocks
-with each block's distinct edges paired at shared nodes: lane l evaluates the
+The device's edge-pair vortex-ring assembly (FLOWPANEL_ASSEMBLY=pairs) takes the
+shared-term column blocks with each block's distinct edges paired at shared nodes: lane l evaluates the
 node terms of its path n0 - n1 - n2 in registers and writes the segment terms of
 n0 -> n1 (edge slot l) and n1 -> n2 (slot 256 + l); each column sums its edges'
 slots with the sign of its traversal.  The host builds the tables

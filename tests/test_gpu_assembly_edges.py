@@ -1,10 +1,10 @@
 """The shared-term vortex-ring assemblies on the device.
 
-Triangle and quad bodies assemble their AIC with the edge-pair kernel (each
-lane's path of two mesh edges evaluated in registers, the edge terms shared by
-the block's panels through LDS; the default) or the kernel with node terms in
-LDS (FLOWPANEL_ASSEMBLY=nodes) (src/FLOWPanel_liftingbody.jl:455-572 restated
-edge by edge).  Both are checked against the oracle (1e-12 of max|G| in FP64,
+Triangle and quad bodies assemble their AIC with the kernel that shares each
+mesh node's and mesh edge's terms between the block's panels through LDS (the
+default) or with the edge-pair kernel (each lane's path of two mesh edges
+evaluated in registers, only the edge terms through LDS; FLOWPANEL_ASSEMBLY=pairs)
+(src/FLOWPanel_liftingbody.jl:455-572 restated edge by edge).  Both are checked against the oracle (1e-12 of max|G| in FP64,
 2e-5 in FP32 storage) and against the panel-wise kernel (FLOWPANEL_ASSEMBLY=panel,
 1e-13) on quads with a wake,
 triangles, a quad/triangle mix, a sphere with collapsed pole edges, disjoint
@@ -77,11 +77,11 @@ def test_shared_edge_assembly_matches_oracle_and_panelwise(fp, geo, orc, case, m
     G = fp.assemble(body, "f64")
     G32 = fp.assemble(body, "f32")
     Go, _, _ = orc.assemble([_obody(body)], [D], [D])
-    monkeypatch.setenv("FLOWPANEL_ASSEMBLY", "nodes")
+    monkeypatch.setenv("FLOWPANEL_ASSEMBLY", "pairs")
     Gn = fp.assemble(body, "f64")
     monkeypatch.setenv("FLOWPANEL_ASSEMBLY", "panel")
     Gp = fp.assemble(body, "f64")
-    print("  vs oracle %.1e, vs panel-wise %.1e, vs node terms in LDS %.1e, f32 %.1e" %
+    print("  vs oracle %.1e, vs panel-wise %.1e, vs edge pairs %.1e, f32 %.1e" %
           (rel(G, Go), rel(G, Gp), rel(G, Gn), rel(G32, Go)), flush=True)
     assert rel(G, Go) < 1e-12 and rel(Gn, Go) < 1e-12
     assert rel(G, Gp) < 1e-13 and rel(Gn, Gp) < 1e-13

@@ -167,6 +167,81 @@ def test_block_jacobi_apply_vector_loads(fp, n, block, monkeypatch):
     assert abs(it1 - it2) <= 1, (it1, it2)
 
 
+def _blocks_matrix(n, block, rng, permute):
+    """Weakly coupled diagonal blocks: well-conditioned random blocks, or (permute) blocks
+    whose rows are permuted so that an elimination without row exchanges would meet tiny
+    pivots (blocks of at most one chunk: the blocked Gauss-Jordan pivots within chunks)."""
+    A = rng.standard_normal((n, n)) * 0.01
+    for b0 in range(0, n, block):
+        m = min(block, n - b0)
+        B = rng.standard_normal((m, m)) * 0.5 / np.sqrt(m) + 2.0 * np.eye(m)
+        if permute:
+            B[np.arange(m), np.arange(m)] *= 1e-3
+            B = B[rng.permutation(m)]
+        A[b0:b0 + m, b0:b0 + m] = B
+    return A
+
+
+def _check_block_inverses(A, X, block, tol):
+    n = A.shape[0]
+    b = min(block, n)
+    worst = 0.0
+    for k in range(X.shape[0]):
+        b0, m = k * b, min(b, n - k * b)
+        ref = np.linalg.inv(A[b0:b0 + m, b0:b0 + m])
+        worst = max(worst, np.abs(X[k, :m, :m] - ref).max() / np.abs(ref).max())
+        if m < b:                                        # padding: diag(inverse, I)
+            pad = np.eye(b, dtype=np.float32)
+            pad[:m, :m] = X[k, :m, :m]
+            assert np.array_equal(X[k], pad)
+    assert worst < tol, worst
+    return worst
+
+
+@pytest.mark.parametrize("n,block,permute", [(130, 1, True), (130, 7, True), (130, 64, True), (200, 65, False),
+                                             (1001, 100, False), (1001, 250, False), (2050, 1000, False),
+                                             (1030, 1024, False)])
+def test_block_inverses_against_numpy(fp, n, block, permute, monkeypatch):
+    """The block-Jacobi inverses (one-wave Gauss-Jordan on chunks of at most 64 rows,
+    blocked above 64) against numpy's inverse of each diagonal block, FP32 accuracy:
+    blocks that need row exchanges, blocks of many chunks, a short last block padded
+    with the identity; the 256-thread register grid (FLOWPANEL_BJ_GJ=grid, chunks of
+    128 rows) likewise."""
+    rng = np.random.default_rng(7300 + n + block)
+    A = _blocks_matrix(n, block, rng, permute)
+    w1 = _check_block_inverses(A, fp.solvers.block_inverses(A, block), block, 1e-4)
+    monkeypatch.setenv("FLOWPANEL_BJ_GJ", "grid")
+    X = fp.solvers.block_inverses(A, block)
+    b = min(block, n)
+    w2 = max(np.abs(X[k, :min(b, n - k * b), :min(b, n - k * b)] -
+                    np.linalg.inv(A[k * b:k * b + min(b, n - k * b), k * b:k * b + min(b, n - k * b)])).max() /
+             np.abs(np.linalg.inv(A[k * b:k * b + min(b, n - k * b), k * b:k * b + min(b, n - k * b)])).max()
+             for k in range(X.shape[0]))
+    print("n %d block %d: worst block error %.1e (grid %.1e)" % (n, block, w1, w2))
+    assert w2 < 1e-4
+
+
+@pytest.mark.parametrize("block", [250, 1000])
+def test_block_inverses_of_an_aic(fp, geo, block):
+    """Diagonal blocks of a wing's AIC (1,440 panels with its wake) against numpy."""
+    g, sh = geo.loft_wing(n_chord=48, n_span=30, span=6.0, sweep_deg=20.0)
+    body = fp.RigidWakeBody(g, sh)
+    V = fp.direction(5.0)
+    fp.solvers.set_freestream(body, np.tile(V[:, None], (1, body.ncells)), np.tile(V[:, None], (1, body.nsheddings)),
+                              np.tile(V[:, None], (1, body.nsheddings)))
+    A = fp.assemble(body, "f64")
+    w = _check_block_inverses(A, fp.solvers.block_inverses(A, block), block, 1e-4)
+    print("AIC block %d: worst block error %.1e" % (block, w))
+
+
+def test_block_inverses_report_a_zero_pivot(fp):
+    A = np.eye(300) * 3.0
+    A[130:140, 130:140] = 0.0                            # a singular 10 x 10 piece of block 1
+    with pytest.raises(fp._native.FPError) as e:
+        fp.solvers.block_inverses(A, 100)
+    assert "zero pivot in diagonal block 1" in str(e.value)
+
+
 # ---- FP32 LU of the randomized matrix without pivoting (the LU32 refinement path) ----
 # The no-pivot LU works in 64-column steps on U^T A V, n padded to a multiple of 4:
 # sizes straddle the pad and the 64-row blocks.
