@@ -254,6 +254,7 @@ struct BJPrecond : Precond {
     int assemble_early(const fp_body *b, const TargetView &tv)
     {
         int st;
+        if (block > 1024) return fp_fail(ctx, FP_ERR_ARG, "block-Jacobi block larger than 1024");
         const int64_t nb = (n + block - 1) / block;
         if ((st = dalloc(ctx, inv, (size_t)nb * block * block * sizeof(float)))) return st;
         const size_t sf = fp_bj_gj_scratch_floats(n, block);
@@ -327,6 +328,7 @@ struct BJPrecond : Precond {
     int setup(const void *A, int prec, int64_t ld, int coarse, int *info_dev)
     {
         int st;
+        if (block > 1024) return fp_fail(ctx, FP_ERR_ARG, "block-Jacobi block larger than 1024");
         const int64_t nb = (n + block - 1) / block;
         if ((st = dalloc(ctx, inv, (size_t)nb * block * block * sizeof(float)))) return st;
         const size_t sf = fp_bj_gj_scratch_floats(n, block);

@@ -297,6 +297,7 @@ int fp_colnorm2(fp_ctx *ctx, const double *R, int64_t n, int64_t ld, int64_t k, 
 // up to 256 rows, above that blocked Gauss-Jordan over chunks of at most 256
 // rows (scratch of fp_bj_gj_scratch_floats floats), and their application
 size_t fp_bj_gj_scratch_floats(int64_t n, int block);
+constexpr int FP_COARSE_MAX = 4096;   // largest Gauss-Jordan block (the coarse-space inverse)
 template <typename T>
 int fp_bj_gj_inverse(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int block, float *inv, int *info_dev,
                      float *scratch, hipEvent_t copied = nullptr, bool lookahead = true, bool prefilled = false);

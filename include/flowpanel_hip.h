@@ -87,7 +87,7 @@ typedef struct fp_solve_opts {
     int precond_block; /* block-Jacobi block size, at most 1024 (GMRES_IR, GMRES)         */
     int coarse_block;  /* > 0: add a Galerkin coarse correction on the indicator vectors of
                           consecutive chunks of this many panels (e.g. a fifth of a
-                          chordwise strip); at most 1024 chunks                              */
+                          chordwise strip); at most 4096 chunks                              */
     int residual_mode; /* refinement residual r = b - A x in FP64 from: FP_RESID_STORED (the
                           FP64 AIC kept beside the FP32 copy) or FP_RESID_MATRIX_FREE
                           (influences recomputed in FP64, no FP64 matrix in memory)          */

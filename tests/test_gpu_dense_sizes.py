@@ -108,14 +108,13 @@ def test_buffer_pool_reuse_and_trim(fp):
     assert np.array_equal(x1, x2)
 
 
-@pytest.mark.parametrize("coarse", [8, 5, 3])
-def test_coarse_space_above_128_vectors(fp, coarse):
-    """Coarse spaces of 126, 201 and 334 chunk vectors (1001 rows): above 128 the
+@pytest.mark.parametrize("coarse,n", [(8, 1001), (5, 1001), (3, 1001), (1, 1300), (2, 2601)])
+def test_coarse_space_above_128_vectors(fp, coarse, n):
+    """Coarse spaces of 126, 201, 334, 1,300 and 1,301 chunk vectors: above 128 the
     Galerkin matrix is inverted by the blocked Gauss-Jordan of the block-Jacobi
-    path; the two-level solve must still reach its FP64 target and take no more
-    iterations than block-Jacobi alone."""
+    path (one block of nc rows, above 1,024 too); the two-level solve must still
+    reach its FP64 target and take no more iterations than block-Jacobi alone."""
     rng = np.random.default_rng(6000 + coarse)
-    n = 1001
     A = rng.standard_normal((n, n)) * 0.3 / np.sqrt(n) + 3.0 * np.eye(n) + 0.5 / n
     b = rng.standard_normal(n)
     y = fp.solve_gmres_ir(None, A, b, precond_block=100, coarse_block=coarse, tol=1e-12)
