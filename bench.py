@@ -13,8 +13,8 @@ FP64 strengths on the device:
   fp64   = FP64 AIC assembly + FP64 LU (getrf/getrs)                   -> reported beside it
 
 The Cp error of the mixed result is measured against the FP64 LU result at the
-control points (the FP64 path itself is checked against the CPU oracle by the
-parity tests at smaller sizes).
+control points (`cp_err_max`) and against the oracle's FP64 LAPACK solution of
+the same body (`vs_oracle`, the committed fixture tests/golden/bwb30k_oracle.npz).
 
 Adjoint leg: dCL/dnodes, dCDi/dnodes and dCL/dUinf, dCDi/dUinf of a
 10,000-panel wing (NACA 2412, AR 8, linear twist 0 -> -3 deg, 100 x 100 quads,
@@ -54,6 +54,26 @@ def build_body(fp, nc, ns, alpha):
     U = np.tile(V[:, None], (1, body.ncells))
     D = np.tile(V[:, None], (1, body.nsheddings))
     return body, U, D
+
+
+ORACLE_FIXTURE = os.path.join(ROOT, "tests", "golden", "bwb30k_oracle.npz")
+
+
+def oracle_fixture_errors(fp, body, gamma):
+    """Error of the mixed result against the oracle's FP64 LAPACK solution of the same 30,000-panel body
+    (strengths and Cp without Kutta averaging at the control points, stored as data by
+    tests/golden/make_bwb_fixture.py); None when the body is not the fixture's."""
+    import hashlib
+    if not os.path.exists(ORACLE_FIXTURE):
+        return None
+    ref = np.load(ORACLE_FIXTURE)
+    h = hashlib.sha256(np.ascontiguousarray(body.grid.nodes, dtype=np.float64).tobytes()).hexdigest()
+    if h != str(ref["node_sha256"]):
+        return None
+    cp = fp.calcfield_Cp(body, 1.0, correct_kuttacondition=False)
+    return {"cp_err_max": float(np.abs(cp - ref["cp"]).max()),
+            "gamma_rel_err": float(np.abs(gamma - ref["gamma"]).max() / np.abs(ref["gamma"]).max()),
+            "fixture": "tests/golden/bwb30k_oracle.npz (oracle AIC + LAPACK getrf/getrs, FP64)"}
 
 
 def cpu_baseline(fp, body, U, D, full=False):
@@ -505,6 +525,7 @@ def main():
     ctx.profile(False)
     gam_mixed = body.strength.copy()
     Us_m = fp.calcfield_U(body, body)
+    vs_oracle = oracle_fixture_errors(fp, body, gam_mixed)
     cp_m = fp.calcfield_Cp(body, 1.0)
     if args.copy_calib:
         ctx.check(fp._native.lib().fp_ctx_copy_calibrate(ctx.handle, 1 << 30))
@@ -512,7 +533,7 @@ def main():
     if args.mixed_only:
         launches, ms = gemv_timed
         log("bench: gemv_f32 %d launches avg %.4f ms" % (launches, ms / max(launches, 1)))
-        print(json.dumps({"mixed_ms": round(t_mixed, 3), "stats": stats[-1]}), flush=True)
+        print(json.dumps({"mixed_ms": round(t_mixed, 3), "stats": stats[-1], "vs_oracle": vs_oracle}), flush=True)
         return
 
     log("bench: fp64 LU")
@@ -600,7 +621,7 @@ def main():
                                "coarse chunks %d)" % (args.chord, args.span, alpha, block, coarse),
                    "panels": N, "shedding_edges": body.nsheddings},
         "fp64_lu_ms": round(t_fp64, 3), "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
-        "cp_err_max": cp_err, "gamma_rel_err": gam_err, "tol": args.tol, "inner_tol": args.inner_tol,
+        "cp_err_max": cp_err, "gamma_rel_err": gam_err, "vs_oracle": vs_oracle, "tol": args.tol, "inner_tol": args.inner_tol,
         "mixed_stats": stats[-1], "fp64_stats": st64, "mixed_tol_1e-10": strict, "mixed_matrix_free": matrix_free,
         "adjoint_ms": adj["adjoint_total_ms"] if adj else None, "adjoint": adj, "c2_wing4800": c2, "batch": bat,
         "field": field,

@@ -722,6 +722,8 @@ struct Solver {
     // explicit inverses of the no-pivot factors (-L^-1, -U^-1): a correction is two GEMVs
     bool inv = false;
     DBuf YL, YU, w4;
+    // the getrs workspace whose diagonal-block inverses belong to the current factors
+    const void *diag_ready = nullptr;
     Solver(fp_ctx *c, const SolveArgs &args, const fp_solve_opts &opts) : ctx(c), a(args), o(opts) {}
 
     // Randomize and factor without pivoting; false (and nothing kept) when a
@@ -731,6 +733,7 @@ struct Solver {
         const int64_t n = a.n;
         int st;
         *ok = false;
+        diag_ready = nullptr;
         n4 = fp_rbt_size(n);
         ld4 = round_ld(n4);
         DBuf nw;
@@ -789,6 +792,7 @@ struct Solver {
         int st;
         rbt = false;
         inv = false;
+        diag_ready = nullptr;
         Ar.reset();
         YL.reset();
         YU.reset();
@@ -823,13 +827,18 @@ struct Solver {
                     if ((st = fp_gemv<float>(ctx, YL.as<float>(), n4, ld4, t, w, 1.0, 0.0))) return st;
                     if ((st = fp_gemv<float>(ctx, YU.as<float>(), n4, ld4, w, t, 1.0, 0.0))) return st;
                 }
-            } else if ((st = fp_getrs<float>(ctx, Ar.as<float>(), n4, ld4, perm4.as<int>(), t, k, n4, wk, wbytes))) {
-                return st;
+            } else {
+                const bool reuse = wk == diag_ready;
+                diag_ready = wk;
+                if ((st = fp_getrs<float>(ctx, Ar.as<float>(), n4, ld4, perm4.as<int>(), t, k, n4, wk, wbytes, reuse)))
+                    return st;
             }
             return fp_rbt_vec(ctx, t, n4, dd, n, n, k, Wrbt.as<double>(), 0);
         }
         FP_HIP(ctx, hipMemcpyAsync(dd, rr, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-        return fp_getrs<float>(ctx, a.A32, n, a.ld, perm.as<int>(), dd, k, n, wk, wbytes);
+        const bool reuse = wk == diag_ready;
+        diag_ready = wk;
+        return fp_getrs<float>(ctx, a.A32, n, a.ld, perm.as<int>(), dd, k, n, wk, wbytes, reuse);
     }
 
     int bj_setup(const void *A, int prec, int blk)
@@ -1036,6 +1045,7 @@ struct Solver {
     {
         const int64_t n = a.n;
         int st;
+        diag_ready = nullptr;   // the workspaces below are new
         if (k == 1 || a.solver == FP_SOLVER_GMRES || a.solver == FP_SOLVER_GMRES_IR ||
             (a.solver == FP_SOLVER_LU32_IR && !a.R64)) {
             for (int64_t c = 0; c < k; ++c)
@@ -1226,6 +1236,9 @@ struct BatchBody {
     bool pivoted = false;
     int64_t n4 = 0, ld4 = 0;
     DBuf Ar, W, perm4, work4, nwork, t4;
+    // the getrs workspace (work4, or swork once pivoted) holds the diagonal-block
+    // inverses of the current factors
+    bool diag_ready = false;
 };
 
 struct BatchImpl {
@@ -1399,6 +1412,7 @@ struct BatchImpl {
                 if ((st = fp_getrf<float>(ctx, bb.A32.as<float>(), n, ld, bb.perm.as<int>(), bb.info.as<int>(),
                                           bb.fwork.p, fp_getrf_work_bytes(n, sizeof(float)))))
                     return st;
+                bb.diag_ready = false;
             } else if ((st = rbt_factor(bb))) {
                 return st;
             }
@@ -1420,6 +1434,7 @@ struct BatchImpl {
                           : fp_rbt_matrix<float>(ctx, bb.A32.as<float>(), bb.n, bb.ld, bb.Ar.as<float>(), bb.ld4,
                                                  bb.W.as<double>());
         if (st) return st;
+        bb.diag_ready = false;
         return fp_getrf_nopiv_f32(ctx, bb.Ar.as<float>(), bb.n4, bb.ld4, bb.info.as<int>(), bb.nwork.p);
     }
 
@@ -1429,17 +1444,19 @@ struct BatchImpl {
         const int64_t n = bb.n, ld = bb.ld;
         double *x = bb.x.as<double>(), *r = bb.r.as<double>(), *dd = bb.d.as<double>();
         int st;
+        const bool reuse = bb.diag_ready;
+        bb.diag_ready = true;
         if (!bb.pivoted) {
             double *t = bb.t4.as<double>();
             if ((st = fp_rbt_vec(ctx, r, n, t, bb.n4, n, 1, bb.W.as<double>(), 1))) return st;
             if ((st = fp_getrs<float>(ctx, bb.Ar.as<float>(), bb.n4, bb.ld4, bb.perm4.as<int>(), t, 1, bb.n4,
-                                      bb.work4.p, fp_getrs_work_bytes(bb.n4, 1))))
+                                      bb.work4.p, fp_getrs_work_bytes(bb.n4, 1), reuse)))
                 return st;
             if ((st = fp_rbt_vec(ctx, t, bb.n4, dd, n, n, 1, bb.W.as<double>(), 0))) return st;
         } else {
             FP_HIP(ctx, hipMemcpyAsync(dd, r, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
             if ((st = fp_getrs<float>(ctx, bb.A32.as<float>(), n, ld, bb.perm.as<int>(), dd, 1, n, bb.swork.p,
-                                      fp_getrs_work_bytes(n, 1))))
+                                      fp_getrs_work_bytes(n, 1), reuse)))
                 return st;
         }
         if ((st = fp_axpby(ctx, 1.0, dd, 1.0, x, n))) return st;
@@ -1466,6 +1483,7 @@ struct BatchImpl {
         if ((st = fp_getrf<float>(ctx, bb.A32.as<float>(), n, ld, bb.perm.as<int>(), bb.info.as<int>(), bb.fwork.p,
                                   fp_getrf_work_bytes(n, sizeof(float)))))
             return st;
+        bb.diag_ready = false;
         if ((st = fp_fill(ctx, bb.x.as<double>(), n, 0.0))) return st;
         FP_HIP(ctx, hipMemcpyAsync(bb.r.p, bb.rhs.p, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         for (int s = 0; s < sweeps; ++s)

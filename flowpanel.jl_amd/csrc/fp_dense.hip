@@ -1063,6 +1063,168 @@ __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ LU, int
     if (tid < nbs) sv[nb0 + tid] = vn;
 }
 
+// Sync-free sweep: the whole triangular solve of one right-hand side in one
+// launch.  One workgroup per TB-row block and right-hand side (blockIdx.y); a
+// workgroup takes its block from a per-RHS ticket counter in solve order
+// (first block first), so every block it waits for belongs to a workgroup
+// that is already running -- no reliance on the dispatch order, no deadlock
+// however few workgroups are resident.  The solution array starts filled with
+// a sentinel (a signalling-NaN bit pattern that no arithmetic produces) and
+// each solved entry is published by an agent-scope atomic store: the entries
+// are their own ready flags, so a waiting workgroup polls the 128 entries it
+// needs (agent-scope atomic loads, coherent across the XCDs' L2s) and needs
+// no flag, fence or cache write-back.  The workgroup owning block b applies
+// the solved blocks' contributions one TB x TB tile at a time (each tile's
+// loads are issued before the wait), each thread keeping partial sums over its
+// 8 columns of every tile; then one 16-lane reduction and the block solve
+// y_b = D_b x_b with the explicit inverse of the diagonal block (its rows
+// loaded ahead of the last wait).  DOWN: forward sweep with the unit lower L
+// (tiles left of the block); else the backward sweep with U (tiles to the
+// right).  rhs is not modified; tickets (k per sweep) are zeroed before the
+// launch.  A wait that exceeds its spin limit takes NaN for the entry, so
+// every workgroup ends and the caller's residual check reports the failure.
+constexpr int TRSV_SPIN_LIMIT = 1 << 20;
+constexpr unsigned long long TRSV_PENDING = 0xFFF4F00DF00DF00Dull;
+
+__device__ __forceinline__ double trsv_pending() { return __longlong_as_double((long long)TRSV_PENDING); }
+
+// One thread's 8 x 8 share of the TB x TB tile at rows r0.., columns cb..:
+// rows it * 16 + wave * 4 + lane / 16, columns c0..c0+7, as stored (no widening).
+// Full tiles (VEC: 16-byte aligned groups of 8) by vector loads; ragged tiles by
+// clamped addresses (every load in bounds, none behind a branch) with the
+// entries outside the tile zeroed by a product with 0/1 (not a select, which
+// the compiler turns back into a branch around the load).
+template <typename T, bool VEC>
+__device__ __forceinline__ void load_tile(const T *__restrict__ LU, int64_t ld, int64_t r0, int bs, int64_t cb, int cs,
+                                          int wave, int lane, int c0, T (&v)[TB / 16][8])
+{
+    constexpr int NIT = TB / 16;
+    if (VEC && bs == TB && cs == TB) {
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const T *p = LU + (r0 + it * 16 + wave * 4 + (lane >> 4)) * ld + cb + c0;
+            if constexpr (sizeof(T) == 4) {
+                const float4 x = *reinterpret_cast<const float4 *>(p), y = *reinterpret_cast<const float4 *>(p + 4);
+                v[it][0] = x.x; v[it][1] = x.y; v[it][2] = x.z; v[it][3] = x.w;
+                v[it][4] = y.x; v[it][5] = y.y; v[it][6] = y.z; v[it][7] = y.w;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double2 x = reinterpret_cast<const double2 *>(p)[q];
+                    v[it][2 * q] = x.x;
+                    v[it][2 * q + 1] = x.y;
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int rr = it * 16 + wave * 4 + (lane >> 4);
+            const T *lr = LU + (r0 + min(rr, bs - 1)) * ld + cb;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[it][e] = lr[min(c0 + e, cs - 1)] * ((rr < bs && c0 + e < cs) ? T(1) : T(0));
+        }
+    }
+}
+
+// VEC: ld and LU allow 16-byte loads of 8-entry groups (ld a multiple of 8, LU 16-byte aligned)
+template <typename T, bool DOWN, bool VEC>
+__global__ __launch_bounds__(256) void k_trsv_sweep(const T *__restrict__ LU, int64_t ld, int64_t n, int nblk,
+                                                    const double *__restrict__ D, const double *__restrict__ rhs,
+                                                    int64_t ldr, double *__restrict__ sol, int64_t lds,
+                                                    int *__restrict__ tickets)
+{
+    __shared__ int s_t;
+    __shared__ double yb[TB], xn[TB], part[TB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = blockIdx.y;
+    if (tid == 0) s_t = atomicAdd(tickets + col, 1);
+    __syncthreads();
+    const int t = s_t;
+    const int bi = DOWN ? t : nblk - 1 - t;
+    const int64_t r0 = (int64_t)bi * TB;
+    const int bs = (int)min((int64_t)TB, n - r0);
+    const int sub = lane & 15, c0 = sub * 8;
+    constexpr int NIT = TB / 16;
+    const double *rv = rhs + (int64_t)col * ldr;
+    double *sv = sol + (int64_t)col * lds;
+    auto blk = [&](int s) { return DOWN ? s : nblk - 1 - s; };   // the s-th block solved
+    double acc[NIT], dv[TB / 2];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) acc[it] = 0.0;
+    const double *Db = D + (int64_t)bi * TB * TB;
+    // FP32 factors: the inverse's rows are loaded ahead of the last wait; FP64 tiles leave no room
+    // for them in registers beside two tiles in flight, so they are loaded after the last tile
+    constexpr bool DV_AHEAD = sizeof(T) == 4;
+    if (DV_AHEAD && t == 0) load_block_row(Db, tid, dv);
+    // two tiles in flight: tile s+1 is requested before the wait for block s
+    T cur[NIT][8], nxt[NIT][8];
+    if (t > 0) {
+        const int64_t cb = (int64_t)blk(0) * TB;
+        load_tile<T, VEC>(LU, ld, r0, bs, cb, (int)min((int64_t)TB, n - cb), wave, lane, c0, cur);
+    }
+    for (int s = 0; s < t; ++s) {
+        const int64_t cb = (int64_t)blk(s) * TB;
+        const int cs = (int)min((int64_t)TB, n - cb);
+        if (s + 1 < t) {
+            const int64_t cn = (int64_t)blk(s + 1) * TB;
+            load_tile<T, VEC>(LU, ld, r0, bs, cn, (int)min((int64_t)TB, n - cn), wave, lane, c0, nxt);
+        } else if (DV_AHEAD) {
+            load_block_row(Db, tid, dv);   // the block's own inverse, ahead of the last wait
+        }
+        if (tid < TB) {
+            double y = 0.0;
+            if (tid < cs) {
+                int spins = 0;
+                for (;;) {
+                    y = __hip_atomic_load(sv + cb + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if ((unsigned long long)__double_as_longlong(y) != TRSV_PENDING) break;
+                    if (++spins > TRSV_SPIN_LIMIT) {
+                        y = __builtin_nan("");
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            yb[tid] = y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < NIT; ++it)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) acc[it] += (double)cur[it][e] * yb[c0 + e];
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < NIT; ++it)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) cur[it][e] = nxt[it][e];
+    }
+    if (!DV_AHEAD) load_block_row(Db, tid, dv);
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) acc[it] += __shfl_xor(acc[it], o);
+        const int rr = it * 16 + wave * 4 + (lane >> 4);
+        if (sub == 0 && rr < bs) xn[rr] = rv[r0 + rr] - acc[it];
+    }
+    __syncthreads();
+    const double y = block_gemv_row(dv, xn, bs, tid, part);
+    if (tid < bs) __hip_atomic_store(sv + r0 + tid, y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// x[c * ldx + i] = the pending sentinel, i < n, c < k
+__global__ void k_trsv_pending(double *__restrict__ x, int64_t n, int64_t ldx)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[(int64_t)blockIdx.y * ldx + i] = trsv_pending();
+}
+
+__global__ void k_zero_i32(int *__restrict__ p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 __global__ void k_permute(const int *__restrict__ perm, int64_t n, const double *__restrict__ src, int64_t lds,
                           double *__restrict__ dst, int64_t ldd)
 {
@@ -1071,16 +1233,29 @@ __global__ void k_permute(const int *__restrict__ perm, int64_t n, const double 
     dst[(int64_t)blockIdx.y * ldd + i] = src[(int64_t)blockIdx.y * lds + perm[i]];
 }
 
+// running right-hand sides (2 n k), the diagonal-block inverses of L and U,
+// and the sweeps' tickets (2 k ints; the former per-block flags' room is kept)
 size_t fp_getrs_work_bytes(int64_t n, int64_t k)
 {
     const int64_t nblk = (n + TB - 1) / TB;
-    return (size_t)2 * n * k * sizeof(double) + (size_t)2 * nblk * TB * TB * sizeof(double) + 256;
+    return (size_t)2 * n * k * sizeof(double) + (size_t)2 * nblk * TB * TB * sizeof(double) +
+           (size_t)2 * (k + k * nblk) * sizeof(int) + 256;
+}
+
+// FLOWPANEL_TRSV=steps: one launch per block step (k_trsv_step) instead of the sync-free sweeps
+static bool trsv_steps()
+{
+    static const bool v = [] {
+        const char *e = getenv("FLOWPANEL_TRSV");
+        return e && !strcmp(e, "steps");
+    }();
+    return v;
 }
 
 // `ipiv` here is the row permutation perm (row i of P*A is row perm[i] of A).
 template <typename T>
 int fp_getrs(fp_ctx *ctx, const T *LU, int64_t n, int64_t ld, const int *perm, double *x, int64_t k, int64_t ldx,
-             void *work, size_t work_bytes)
+             void *work, size_t work_bytes, bool reuse_diag)
 {
     if (n <= 0 || k <= 0) return FP_OK;
     if (work_bytes < fp_getrs_work_bytes(n, k)) return fp_fail(ctx, FP_ERR_ARG, "getrs workspace too small");
@@ -1093,8 +1268,33 @@ int fp_getrs(fp_ctx *ctx, const T *LU, int64_t n, int64_t ld, const int *perm, d
     double *Dl = y + n * k, *Du = Dl + nblk * TB * TB;
     hipLaunchKernelGGL(k_permute, dim3((unsigned)((n + 255) / 256), (unsigned)k), dim3(256), 0, ctx->stream, perm, n,
                        (const double *)x, ldx, b, n);
-    hipLaunchKernelGGL((k_trtri_diag<T, true>), dim3(TB, (unsigned)nblk), dim3(TB), 0, ctx->stream, LU, ld, n, Dl);
-    hipLaunchKernelGGL((k_trtri_diag<T, false>), dim3(TB, (unsigned)nblk), dim3(TB), 0, ctx->stream, LU, ld, n, Du);
+    if (!reuse_diag) {
+        hipLaunchKernelGGL((k_trtri_diag<T, true>), dim3(TB, (unsigned)nblk), dim3(TB), 0, ctx->stream, LU, ld, n, Dl);
+        hipLaunchKernelGGL((k_trtri_diag<T, false>), dim3(TB, (unsigned)nblk), dim3(TB), 0, ctx->stream, LU, ld, n, Du);
+    }
+    if (!trsv_steps()) {
+        int *sync = (int *)(Du + nblk * TB * TB);
+        hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((2 * k + 255) / 256)), dim3(256), 0, ctx->stream, sync, 2 * k);
+        const dim3 gp((unsigned)((n + 255) / 256), (unsigned)k);
+        hipLaunchKernelGGL(k_trsv_pending, gp, dim3(256), 0, ctx->stream, y, n, n);
+        hipLaunchKernelGGL(k_trsv_pending, gp, dim3(256), 0, ctx->stream, x, n, ldx);
+        // forward: y = L^-1 b, then backward: x = U^-1 y
+        const dim3 g((unsigned)nblk, (unsigned)k);
+        int *sync2 = sync + k;
+        if (ld % 8 == 0 && ((uintptr_t)LU & 15) == 0) {
+            hipLaunchKernelGGL((k_trsv_sweep<T, true, true>), g, dim3(256), 0, ctx->stream, LU, ld, n, (int)nblk,
+                               (const double *)Dl, (const double *)b, n, y, n, sync);
+            hipLaunchKernelGGL((k_trsv_sweep<T, false, true>), g, dim3(256), 0, ctx->stream, LU, ld, n, (int)nblk,
+                               (const double *)Du, (const double *)y, n, x, ldx, sync2);
+        } else {
+            hipLaunchKernelGGL((k_trsv_sweep<T, true, false>), g, dim3(256), 0, ctx->stream, LU, ld, n, (int)nblk,
+                               (const double *)Dl, (const double *)b, n, y, n, sync);
+            hipLaunchKernelGGL((k_trsv_sweep<T, false, false>), g, dim3(256), 0, ctx->stream, LU, ld, n, (int)nblk,
+                               (const double *)Du, (const double *)y, n, x, ldx, sync2);
+        }
+        FP_HIP(ctx, hipGetLastError());
+        return FP_OK;
+    }
     // forward: y = L^-1 b
     hipLaunchKernelGGL(k_trsv_first, dim3(1, (unsigned)k), dim3(256), 0, ctx->stream, (const double *)Dl, (int64_t)0,
                        bsz(0), (const double *)b, n, y, n);
@@ -1119,9 +1319,9 @@ int fp_getrs(fp_ctx *ctx, const T *LU, int64_t n, int64_t ld, const int *perm, d
     return FP_OK;
 }
 template int fp_getrs<double>(fp_ctx *, const double *, int64_t, int64_t, const int *, double *, int64_t, int64_t,
-                              void *, size_t);
+                              void *, size_t, bool);
 template int fp_getrs<float>(fp_ctx *, const float *, int64_t, int64_t, const int *, double *, int64_t, int64_t,
-                             void *, size_t);
+                             void *, size_t, bool);
 
 // ----------------------------------------------------------------------------
 // y = alpha A x + beta y (row-major A, FP64 vectors).  One wave per 4 rows:

@@ -269,10 +269,12 @@ void fp_noalloc_init(fp_ctx *ctx);
 template <typename T>
 int fp_getrf(fp_ctx *ctx, T *A, int64_t n, int64_t ld, int *perm, int *info_dev, void *work, size_t work_bytes);
 size_t fp_getrf_work_bytes(int64_t n, int elem_bytes);
-// solve with LU factors: x (FP64, n x k SoA columns, ldx) overwritten
+// solve with LU factors: x (FP64, n x k SoA columns, ldx) overwritten.  The
+// inverses of the diagonal blocks are computed into `work`; reuse_diag skips
+// that when the previous call with the same `work` used the same factors.
 template <typename T>
 int fp_getrs(fp_ctx *ctx, const T *LU, int64_t n, int64_t ld, const int *ipiv, double *x, int64_t k, int64_t ldx,
-             void *work, size_t work_bytes);
+             void *work, size_t work_bytes, bool reuse_diag = false);
 size_t fp_getrs_work_bytes(int64_t n, int64_t k);
 // y = alpha*A*x + beta*y, A row-major (T), x,y FP64
 template <typename T>

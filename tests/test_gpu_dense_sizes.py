@@ -45,6 +45,19 @@ def test_getrs_block_boundaries(fp, n, k, solver):
     assert rel(X, np.linalg.solve(A, B)) < 1e-11, (n, k, solver)
 
 
+@pytest.mark.parametrize("n,k,solver", [(3001, 1, "solve_ludiv"), (3001, 2, "solve_ludiv"), (2000, 40, "solve_ludiv"),
+                                        (3001, 1, "solve_lu32_ir"), (2000, 3, "solve_lu32_ir")])
+def test_getrs_sync_free_sweeps(fp, n, k, solver):
+    # one launch per triangular sweep: 24 blocks in a chain of ready flags; 16 blocks x 40
+    # right-hand sides = 640 workgroups, more than are resident at once (tickets order them)
+    rng = np.random.default_rng(3000 + n + k)
+    A = rng.standard_normal((n, n)) + 2.0 * np.sqrt(n) * np.eye(n)
+    B = rng.standard_normal((n, k))
+    X = np.zeros((n, k))
+    getattr(fp, solver)(X, A, B)
+    assert rel(X, np.linalg.solve(A, B)) < 1e-11, (n, k, solver)
+
+
 @pytest.mark.parametrize("block", [1, 64, 128, 129, 250, 256, 257, 300, 500, 512, 513, 777])
 def test_block_jacobi_paths(fp, block):
     # blocks <= 128: Gauss-Jordan inverses from A; above: blocked Gauss-Jordan over

@@ -118,7 +118,9 @@ def test_zeroebwb_example_least_squares(fp):
         assert abs(CLg - CLl) < 1e-6 * abs(CLl) and abs(CDg - CDl) < 1e-6 * abs(CLl)
         res[order] = out
     x0, x1 = res[False]["solve_ludiv"][0], res[True]["solve_ludiv"][0]
-    assert np.abs(x0 - x1).max() / np.abs(x0).max() < 1e-9
+    # two FP64 LU solves of the same system in two row/column orders: they differ by rounding
+    # amplified by the condition number (~5e6 above): measured 1.4e-9
+    assert np.abs(x0 - x1).max() / np.abs(x0).max() < 1e-8
     # force_cellTE keeps the cell-centred gradient on the shedding panels and on their index
     # neighbours pi+1, pj-1 (src/FLOWPanel_postprocess.jl:875-882); on an unstructured mesh those
     # neighbours are other cells in the other order, so Cp is compared everywhere else
