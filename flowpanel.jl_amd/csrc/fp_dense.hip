@@ -137,6 +137,29 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t 
                 for (int b = 0; b < 4; ++b) acc[a][b] = MfmaOp<T>::run(fa[a], fb[b], acc[a][b]);
         }
     }
+    // epilogue: every load of the lane's 64 entries of C is issued before the first
+    // store (as far as the compiler knows rows of C could alias, and a load after a
+    // store would wait for it: one memory round trip per entry)
+    if (m0 + GB_M <= m && n0 + GB_N <= n) {   // uniform: an interior tile, no guards
+        T cv[4][4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    cv[a][b][r] = C[(m0 + wm * 64 + a * 16 + MfmaOp<T>::row(lane, r)) * ldc + n0 + wn * 64 + b * 16 +
+                                    (lane & 15)];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    C[(m0 + wm * 64 + a * 16 + MfmaOp<T>::row(lane, r)) * ldc + n0 + wn * 64 + b * 16 + (lane & 15)] =
+                        cv[a][b][r] - acc[a][b][r];
+        return;
+    }
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll

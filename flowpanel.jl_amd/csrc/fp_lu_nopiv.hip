@@ -16,8 +16,12 @@
 //                                       explicit inverses of L11 and U11 (two
 //                                       waves, one each);
 //                        k_nopiv_panel  L21 = A21 U11^-1 and U12 = L11^-1 A12 as
-//                                       64 x 64 x 64 products on MFMA, in place;
-//                        fp_gemm_sub    A22 -= L21 U12 (the MFMA trailing update).
+//                                       64 x 64 x 64 products on MFMA, in place
+//                                       (U12 within the 256-column outer block);
+//                        fp_gemm_sub    the update of the outer block's columns;
+//   per 256-column outer block: the U rows right of it (blocked forward
+//   substitution: MFMA products and k_nopiv_panel), then the rank-256 MFMA
+//   trailing update.
 //
 // A x = b becomes  A_r y = U^T b,  x = V y.  A depth-2 butterfly couples only
 // the four indices {p, p + m, p + 2m, p + 3m} (m = n4 / 4, n4 = n rounded up
@@ -148,21 +152,21 @@ __global__ void k_fill_info(int *p, int v)
 }
 
 // One 64 x 64 tile of the block row / column of step k0, in place:
-// blocks [0, nrt): L21 tile rows off..off+63 := A21 U11^-1;
-// blocks [nrt, 2 nrt): U12 tile columns off..off+63 := L11^-1 A12.
+// blocks [0, nl): L21 tile rows off = k0 + 64 (1 + b) .. off+63 := A21 U11^-1;
+// blocks [nl, nl + nu): U12 tile columns off = u_off + 64 (b - nl) .. off+63 := L11^-1 A12.
 // Both operands in LDS; wave w computes output rows 16w..16w+15 with
 // v_mfma_f32_16x16x4 (lane l supplies P(l%16, k + l/16) and Q(k + l/16, l%16);
 // accumulator element r holds C(4 (l/16) + r, l%16)).
 __global__ __launch_bounds__(256) void k_nopiv_panel(float *__restrict__ A, int64_t ld, int64_t n, int64_t k0,
                                                     const float *__restrict__ Linv, const float *__restrict__ Uinv,
-                                                    int nrt)
+                                                    int nl, int64_t u_off)
 {
     __shared__ float P[NPB][NPB + 4];
     __shared__ float Q[NPB][NPB + 4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int b = blockIdx.x;
-    const bool lpanel = b < nrt;
-    const int64_t off = k0 + NPB + (int64_t)NPB * (lpanel ? b : b - nrt);
+    const bool lpanel = b < nl;
+    const int64_t off = lpanel ? k0 + NPB + (int64_t)NPB * b : u_off + (int64_t)NPB * (b - nl);
     float *tile = lpanel ? A + off * ld + k0 : A + k0 * ld + off;
     const int tr = (int)min((int64_t)NPB, lpanel ? n - off : n - k0);
     const int tc = (int)min((int64_t)NPB, lpanel ? n - k0 : n - off);
@@ -211,6 +215,70 @@ __global__ __launch_bounds__(256) void k_nopiv_panel(float *__restrict__ A, int6
             if (row < tr && col < tc) tile[row * ld + col] = acc[q][e];
         }
     }
+}
+
+// C -= A B on 64 x 64 output tiles (FP32, row-major), for the small products of
+// the outer blocks (the in-block trailing updates, the U rows' substitution),
+// where the 128 x 128 tiles of fp_gemm_sub leave most of the chip idle: blockIdx.x
+// = tile (row-tile major), K in chunks of 64 through LDS, v_mfma_f32_16x16x4 as in
+// k_nopiv_panel (wave w: output rows 16w..16w+15).  Loads of partial tiles use
+// clamped addresses and a 0/1 factor (every load in bounds and unconditional).
+__global__ __launch_bounds__(256) void k_gemm64_sub(int64_t m, int64_t n, int64_t kk, const float *__restrict__ A,
+                                                   int64_t lda, const float *__restrict__ B, int64_t ldb,
+                                                   float *__restrict__ C, int64_t ldc)
+{
+    __shared__ float P[NPB][NPB + 4];
+    __shared__ float Q[NPB][NPB + 4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t tn_count = (n + NPB - 1) / NPB;
+    const int64_t i0 = (int64_t)(blockIdx.x / tn_count) * NPB, j0 = (int64_t)(blockIdx.x % tn_count) * NPB;
+    const int tr = (int)min((int64_t)NPB, m - i0), tc = (int)min((int64_t)NPB, n - j0);
+    const int r = tid >> 2, c0 = (tid & 3) * 16;
+    nf4 acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = nf4{0.f, 0.f, 0.f, 0.f};
+    for (int64_t k0 = 0; k0 < kk; k0 += NPB) {
+        const int kc = (int)min((int64_t)NPB, kk - k0);
+        const float *ar = A + (i0 + min(r, tr - 1)) * lda + k0;
+        const float *br = B + (k0 + min(r, kc - 1)) * ldb + j0;
+        const float am = r < tr ? 1.f : 0.f, bm = r < kc ? 1.f : 0.f;
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int c = c0 + e;
+            P[r][c] = ar[min(c, kc - 1)] * (c < kc ? am : 0.f);
+            Q[r][c] = br[min(c, tc - 1)] * (c < tc ? bm : 0.f);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < NPB; ks += 4) {
+            const float pa = P[16 * wave + (lane & 15)][ks + (lane >> 4)];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(pa, Q[ks + (lane >> 4)][16 * q + (lane & 15)], acc[q], 0,
+                                                              0, 0);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int col = 16 * q + (lane & 15);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int row = 16 * wave + 4 * (lane >> 4) + e;
+            if (row < tr && col < tc) C[(i0 + row) * ldc + j0 + col] -= acc[q][e];
+        }
+    }
+}
+
+static int gemm64_sub(fp_ctx *ctx, int64_t m, int64_t n, int64_t k, const float *A, int64_t lda, const float *B,
+                      int64_t ldb, float *C, int64_t ldc)
+{
+    if (m <= 0 || n <= 0 || k <= 0) return FP_OK;
+    const int64_t tiles = ((m + NPB - 1) / NPB) * ((n + NPB - 1) / NPB);
+    ProfScope ps(ctx, "gemm_f32");
+    hipLaunchKernelGGL(k_gemm64_sub, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, m, n, k, A, lda, B, ldb, C, ldc);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
 }
 
 // U^T A V over the orbit blocks: thread (p, q) reads the 4 x 4 block of A at rows
@@ -361,7 +429,17 @@ int fp_rbt_vec(fp_ctx *ctx, const double *in, int64_t ldin, double *out, int64_t
     return FP_OK;
 }
 
-size_t fp_getrf_nopiv_work_bytes() { return (size_t)2 * NPB * NPB * sizeof(float); }
+// Inner blocks of 64 columns per outer block of 256: the inner steps factor the
+// outer block's columns only (their trailing updates stop at its last column),
+// then the outer block's U rows right of it come from one blocked forward
+// substitution with the four inner L11^-1, and the trailing matrix takes one
+// rank-256 update: a rank-64 update per 64 columns reads and writes the whole
+// trailing matrix four times as often (at 4,800 unknowns it bound the
+// factorisation: 41 us per step against 32 us for the diagonal block).
+constexpr int NOB = 4 * NPB;
+
+// Linv and Uinv of the NOB / NPB inner diagonal blocks
+size_t fp_getrf_nopiv_work_bytes() { return (size_t)2 * (NOB / NPB) * NPB * NPB * sizeof(float); }
 
 // LU without pivoting of the n x n FP32 matrix A (row-major, ld a multiple of 4);
 // info_dev receives 1 + the first row of the diagonal block whose pivot was zero
@@ -371,26 +449,52 @@ int fp_getrf_nopiv_f32(fp_ctx *ctx, float *A, int64_t n, int64_t ld, int *info_d
 {
     if (n <= 0) return FP_OK;
     if (ld % 4) return fp_fail(ctx, FP_ERR_ARG, "internal: no-pivot LU needs a leading dimension divisible by 4");
-    float *Linv = (float *)work, *Uinv = Linv + NPB * NPB;
     hipLaunchKernelGGL(k_fill_info, dim3(1), dim3(64), 0, ctx->stream, info_dev, 0);
-    for (int64_t k0 = 0; k0 < n; k0 += NPB) {
-        {
-            ProfScope ps(ctx, "lu_nopiv_diag");
-            hipLaunchKernelGGL(k_nopiv_diag, dim3(1), dim3(128), 0, ctx->stream, A, ld, n, k0, Linv, Uinv, info_dev,
-                               YL, YU);
+    int st;
+    for (int64_t k = 0; k < n; k += NOB) {
+        const int64_t ob = std::min<int64_t>(NOB, n - k), kend = k + ob;
+        const int T = (int)((ob + NPB - 1) / NPB);
+        for (int t = 0; t < T; ++t) {
+            const int64_t c = k + (int64_t)NPB * t;
+            float *Linv = (float *)work + (int64_t)2 * t * NPB * NPB, *Uinv = Linv + NPB * NPB;
+            {
+                ProfScope ps(ctx, "lu_nopiv_diag");
+                hipLaunchKernelGGL(k_nopiv_diag, dim3(1), dim3(128), 0, ctx->stream, A, ld, n, c, Linv, Uinv, info_dev,
+                                   YL, YU);
+            }
+            const int64_t below = n - c - NPB, inpanel = kend - c - NPB;
+            if (below <= 0) break;
+            const int nl = (int)((below + NPB - 1) / NPB), nu = inpanel > 0 ? (int)((inpanel + NPB - 1) / NPB) : 0;
+            {
+                ProfScope ps(ctx, "lu_nopiv_panel");
+                hipLaunchKernelGGL(k_nopiv_panel, dim3((unsigned)(nl + nu)), dim3(256), 0, ctx->stream, A, ld, n, c,
+                                   (const float *)Linv, (const float *)Uinv, nl, c + NPB);
+            }
+            FP_HIP(ctx, hipGetLastError());
+            if (inpanel > 0 &&
+                (st = gemm64_sub(ctx, below, inpanel, NPB, A + (c + NPB) * ld + c, ld, A + c * ld + c + NPB, ld,
+                                 A + (c + NPB) * ld + c + NPB, ld)))
+                return st;
         }
-        const int64_t rest = n - k0 - NPB;
-        if (rest <= 0) break;
-        const int nrt = (int)((rest + NPB - 1) / NPB);
-        {
+        const int64_t right = n - kend;
+        if (right <= 0) continue;
+        // U12 = L11^-1 A12 for the columns right of the outer block: U_t = L_tt^-1 (A_t - sum_{s<t} L_ts U_s)
+        const int nu = (int)((right + NPB - 1) / NPB);
+        for (int t = 0; t < T; ++t) {
+            const int64_t c = k + (int64_t)NPB * t;
+            const float *Linv = (const float *)work + (int64_t)2 * t * NPB * NPB, *Uinv = Linv + NPB * NPB;
+            if (t > 0 && (st = gemm64_sub(ctx, NPB, right, c - k, A + c * ld + k, ld, A + k * ld + kend, ld,
+                                          A + c * ld + kend, ld)))
+                return st;
             ProfScope ps(ctx, "lu_nopiv_panel");
-            hipLaunchKernelGGL(k_nopiv_panel, dim3((unsigned)(2 * nrt)), dim3(256), 0, ctx->stream, A, ld, n, k0,
-                               (const float *)Linv, (const float *)Uinv, nrt);
+            hipLaunchKernelGGL(k_nopiv_panel, dim3((unsigned)nu), dim3(256), 0, ctx->stream, A, ld, n, c, Linv, Uinv, 0,
+                               kend);
+            FP_HIP(ctx, hipGetLastError());
         }
-        FP_HIP(ctx, hipGetLastError());
-        int st = fp_gemm_sub<float>(ctx, rest, rest, NPB, A + (k0 + NPB) * ld + k0, ld, A + k0 * ld + k0 + NPB, ld,
-                                    A + (k0 + NPB) * ld + k0 + NPB, ld);
-        if (st) return st;
+        // the rank-ob trailing update
+        if ((st = fp_gemm_sub<float>(ctx, right, right, ob, A + kend * ld + k, ld, A + k * ld + kend, ld,
+                                     A + kend * ld + kend, ld)))
+            return st;
     }
     return FP_OK;
 }
