@@ -450,7 +450,8 @@ def main():
     ap.add_argument("--chord", type=int, default=250)
     ap.add_argument("--span", type=int, default=120)
     ap.add_argument("--block", type=int, default=-1, help="block-Jacobi block size (default: four chordwise strips)")
-    ap.add_argument("--coarse", type=int, default=-1, help="coarse chunk (default: a fifth of a chordwise strip)")
+    ap.add_argument("--coarse", type=int, default=-1,
+                    help="coarse chunk (default: a twenty-fifth of a chordwise strip)")
     ap.add_argument("--tol", type=float, default=1e-8,
                     help="FP64 relative residual target of the mixed solve (max |dCp| ~1e-8 against FP64 LU, "
                          "100x inside the 1e-6 bound)")
@@ -495,7 +496,7 @@ def main():
     N = body.ncells
     fp.solvers.set_freestream(body, U, D, D)
     block = 4 * args.chord if args.block < 0 else args.block
-    coarse = args.chord // 5 if args.coarse < 0 else args.coarse
+    coarse = max(1, args.chord // 25) if args.coarse < 0 else args.coarse
     mixed_opts = {"precond_block": block, "coarse_block": coarse, "tol": args.tol, "inner_tol": args.inner_tol,
                   "restart": 100, "residual_mode": 1}
 

@@ -300,7 +300,7 @@ struct BJPrecond : Precond {
             const int nc = (int)((n + cs - 1) / cs);
             if ((st = dalloc(ctx, AZ, (size_t)nc * n * sizeof(float))) ||
                 (st = dalloc(ctx, Einv, (size_t)nc * nc * sizeof(double))) ||
-                (st = dalloc(ctx, work, (size_t)2 * nc * nc * sizeof(double))) ||
+                (st = dalloc(ctx, work, fp_coarse_work_bytes(nc))) ||
                 (st = dalloc(ctx, tc, 2 * nc * sizeof(double))) || (st = dalloc(ctx, r2, n * sizeof(double))) ||
                 (st = dalloc(ctx, cinfo, sizeof(int))))
                 return st;
@@ -340,7 +340,7 @@ struct BJPrecond : Precond {
             nc = (int)((n + cs - 1) / cs);
             if ((st = dalloc(ctx, AZ, (size_t)nc * n * sizeof(float)))) return st;
             if ((st = dalloc(ctx, Einv, (size_t)nc * nc * sizeof(double)))) return st;
-            if ((st = dalloc(ctx, work, (size_t)2 * nc * nc * sizeof(double)))) return st;
+            if ((st = dalloc(ctx, work, fp_coarse_work_bytes(nc)))) return st;
             if ((st = dalloc(ctx, tc, 2 * nc * sizeof(double)))) return st;
             if ((st = dalloc(ctx, r2, n * sizeof(double)))) return st;
             if ((st = dalloc(ctx, cinfo, sizeof(int)))) return st;

@@ -2433,6 +2433,22 @@ __global__ __launch_bounds__(256) void k_widen_f32(const float *__restrict__ x, 
     if (i < m) y[i] = (double)x[i];
 }
 
+// FLOWPANEL_COARSE_INV=lu: coarse spaces from this size up are inverted through the
+// randomized no-pivot LU and the inverses of its factors instead of the blocked
+// Gauss-Jordan.  Measured on the 30,000-panel BWB with 3,000 coarse vectors: 7.5 ms
+// on the side stream (about 4 nc^3 flop of FP32 MFMA products at ~50 TFLOP/s, and
+// ~280 launches whose host time delays the block inverses queued after them)
+// against 8.3 ms for the 47 Gauss-Jordan passes, and no shorter set-up; the pivoted
+// Gauss-Jordan also survives Galerkin matrices too close to singular for a
+// factorisation without pivoting, so it stays the default.
+constexpr int COARSE_INV_LU_MIN = 512;
+constexpr uint64_t COARSE_RBT_SEED = 0x434F415253455242ull;
+
+size_t fp_coarse_work_bytes(int64_t nc)
+{
+    return std::max<size_t>((size_t)2 * nc * nc * sizeof(double), fp_inverse_rbt_work_bytes(nc));
+}
+
 template <typename T>
 int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, float *AZ, double *Einv, double *work,
                     int *info_dev)
@@ -2458,8 +2474,15 @@ int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, floa
         hipLaunchKernelGGL(k_coarse_galerkin, dim3(nc, nc), dim3(256), 0, ctx->stream, (const float *)AZ, n, cs, nc,
                            Einv);
     }
+    const char *ci = getenv("FLOWPANEL_COARSE_INV");
+    const bool gj = !(ci && !strcmp(ci, "lu"));
     if (nc <= SMALL_INV_LDS) {
         hipLaunchKernelGGL(k_small_inverse_lds, dim3(1), dim3(1024), 0, ctx->stream, Einv, nc, info_dev);
+    } else if (nc >= COARSE_INV_LU_MIN && !gj) {
+        // large coarse spaces: the randomized no-pivot FP32 LU and the inverses of its
+        // factors (the blocked Gauss-Jordan below is a chain of nc / 64 passes)
+        int st = fp_inverse_rbt_f32(ctx, Einv, nc, nc, work, fp_coarse_work_bytes(nc), info_dev, COARSE_RBT_SEED);
+        if (st) return st;
     } else {
         // larger coarse spaces: FP32 blocked Gauss-Jordan of E (it only feeds the
         // preconditioner), widened back in place; work (2 nc^2 doubles) holds the

@@ -261,6 +261,13 @@ size_t fp_getrf_nopiv_work_bytes();
 int fp_getrf_nopiv_f32(fp_ctx *ctx, float *A, int64_t n, int64_t ld, int *info_dev, void *work, float *YL = nullptr,
                        float *YU = nullptr);
 int fp_trtri_nopiv_f32(fp_ctx *ctx, const float *A, int64_t n, int64_t ld, float *YL, float *YU, float *T);
+// E^-1 in place (FP64, n x n) through the randomized no-pivot FP32 LU and the explicit
+// inverses of its factors; allocates nothing, never synchronises the host
+size_t fp_inverse_rbt_work_bytes(int64_t n);
+int fp_inverse_rbt_f32(fp_ctx *ctx, double *E, int64_t n, int64_t lde, void *work, size_t work_bytes, int *info_dev,
+                       uint64_t seed);
+// workspace of fp_coarse_setup for nc coarse vectors
+size_t fp_coarse_work_bytes(int64_t nc);
 // device address of fp_uind_noalloc's scratch array, queried once per GPU (fp_noalloc.hip)
 void fp_noalloc_init(fp_ctx *ctx);
 

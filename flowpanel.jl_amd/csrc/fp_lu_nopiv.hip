@@ -385,6 +385,75 @@ void butterfly_blocks(int64_t m, uint64_t seed, double *out)
     }
 }
 
+// The butterflies' orbit blocks on the device (the same construction as
+// butterfly_blocks; the i-th draw of a splitmix64 stream is mix64 of seed +
+// (i + 1) * golden, so every orbit computes its own eight draws).
+__device__ __forceinline__ uint64_t mix64_at(uint64_t seed, uint64_t i)
+{
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_rbt_weights(int64_t m, uint64_t seed, double *__restrict__ W)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= 2 * m) return;
+    const bool v = p >= m;   // U's orbit blocks first, then V's
+    const int64_t q = v ? p - m : p;
+    const uint64_t sd = v ? seed ^ 0x5DEECE66Dull : seed;
+    double r[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = exp((((double)(mix64_at(sd, 8 * (uint64_t)q + j) >> 11) * 0x1.0p-53) - 0.5) / 10.0);
+    const double h = sqrt(0.5);
+    const double Z[4][4] = {{h * r[0], 0, h * r[1], 0}, {0, h * r[2], 0, h * r[3]},
+                            {h * r[0], 0, -h * r[1], 0}, {0, h * r[2], 0, -h * r[3]}};
+    const double Y[4][4] = {{h * r[4], h * r[5], 0, 0}, {h * r[4], -h * r[5], 0, 0},
+                            {0, 0, h * r[6], h * r[7]}, {0, 0, h * r[6], -h * r[7]}};
+    double *o = W + 16 * p;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) o[4 * i + j] = Y[i][0] * Z[0][j] + Y[i][1] * Z[1][j] + Y[i][2] * Z[2][j] + Y[i][3] * Z[3][j];
+}
+
+// E (n x n, FP64, lde) = -(V C U^T) restricted to n x n, C (4m x 4m, FP32, ldc):
+// thread (p, q) forms -V_p B U_q^T of the orbit block B of C at rows p + i m,
+// columns q + j m (the undoing of U^T A V for an inverse: A^-1 = V A_r^-1 U^T).
+__global__ __launch_bounds__(256) void k_rbt_unmatrix(const float *__restrict__ C, int64_t ldc, int64_t m,
+                                                     double *__restrict__ E, int64_t n, int64_t lde,
+                                                     const double *__restrict__ W)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t p = blockIdx.y;
+    if (q >= m) return;
+    double B[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) B[i][j] = (double)C[(p + i * m) * ldc + q + j * m];
+    const double *Vp = W + 16 * (m + p), *Uq = W + 16 * q;
+    double T[4][4];   // B U_q^T
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            T[i][j] = B[i][0] * Uq[4 * j] + B[i][1] * Uq[4 * j + 1] + B[i][2] * Uq[4 * j + 2] + B[i][3] * Uq[4 * j + 3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t gr = p + i * m;
+        if (gr >= n) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t gc = q + j * m;
+            if (gc < n)
+                E[gr * lde + gc] =
+                    -(Vp[4 * i] * T[0][j] + Vp[4 * i + 1] * T[1][j] + Vp[4 * i + 2] * T[2][j] + Vp[4 * i + 3] * T[3][j]);
+        }
+    }
+}
+
 }  // namespace
 
 int64_t fp_rbt_size(int64_t n) { return (n + 3) / 4 * 4; }
@@ -548,5 +617,46 @@ int fp_trtri_nopiv_f32(fp_ctx *ctx, const float *A, int64_t n, int64_t ld, float
                 return st;
         }
     }
+    return FP_OK;
+}
+
+// Workspace of fp_inverse_rbt_f32 for order n: the randomized copy (then the
+// product), -L^-1, -U^-1, the triangular-inverse scratch (FP32, n4 x ld4 each),
+// the butterflies and the LU's own work.
+static int64_t inv_ld(int64_t n4) { return std::max<int64_t>(64, (n4 + 63) / 64 * 64); }
+size_t fp_inverse_rbt_work_bytes(int64_t n)
+{
+    const int64_t n4 = fp_rbt_size(n), ld4 = inv_ld(n4);
+    return (size_t)4 * n4 * ld4 * sizeof(float) + fp_rbt_weights_bytes(n) + fp_getrf_nopiv_work_bytes() + 1024;
+}
+
+// E^-1 in place (n x n, FP64, lde) by the LU without pivoting of the randomized
+// FP32 matrix U^T E V, the explicit inverses of its factors and one product:
+// E^-1 = V (U_r^-1 L_r^-1) U^T.  No chain of n / 64 Gauss-Jordan passes with
+// their products: the factorisation's own chain is a few launches per 64
+// columns and the rest are large MFMA products.  info_dev as fp_getrf_nopiv_f32.
+// Nothing is allocated and the host is not synchronised (a side stream may run it).
+int fp_inverse_rbt_f32(fp_ctx *ctx, double *E, int64_t n, int64_t lde, void *work, size_t work_bytes, int *info_dev,
+                       uint64_t seed)
+{
+    if (n <= 0) return FP_OK;
+    if (work_bytes < fp_inverse_rbt_work_bytes(n)) return fp_fail(ctx, FP_ERR_ARG, "internal: inverse workspace too small");
+    ProfScope ps(ctx, "inverse_rbt");
+    const int64_t n4 = fp_rbt_size(n), m = n4 / 4, ld4 = inv_ld(n4), sz = n4 * ld4;
+    float *Ar = (float *)work, *YL = Ar + sz, *YU = YL + sz, *T = YU + sz;
+    double *W = (double *)(T + sz);
+    void *nwork = (char *)W + fp_rbt_weights_bytes(n);
+    hipLaunchKernelGGL(k_rbt_weights, dim3((unsigned)((2 * m + 127) / 128)), dim3(128), 0, ctx->stream, m, seed, W);
+    int st;
+    if ((st = fp_rbt_matrix<double>(ctx, E, n, lde, Ar, ld4, W))) return st;
+    if ((st = fp_fill_f32(ctx, YL, sz, 0.f)) || (st = fp_fill_f32(ctx, YU, sz, 0.f))) return st;
+    if ((st = fp_getrf_nopiv_f32(ctx, Ar, n4, ld4, info_dev, nwork, YL, YU))) return st;
+    if ((st = fp_trtri_nopiv_f32(ctx, Ar, n4, ld4, YL, YU, T))) return st;
+    // Ar <- -(YU YL) = -U^-1 L^-1 (the factors are no longer needed)
+    if ((st = fp_fill_f32(ctx, Ar, sz, 0.f))) return st;
+    if ((st = fp_gemm_sub<float>(ctx, n4, n4, n4, YU, ld4, YL, ld4, Ar, ld4))) return st;
+    hipLaunchKernelGGL(k_rbt_unmatrix, dim3((unsigned)((m + 255) / 256), (unsigned)m), dim3(256), 0, ctx->stream,
+                       (const float *)Ar, ld4, m, E, n, lde, (const double *)W);
+    FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }

@@ -2,8 +2,8 @@
 
 * The large-body workload (blended-wing-body loft, seed 5) at a reduced size with
   the benchmark's own preconditioner layout (blocks of four chordwise strips,
-  coarse chunks of a fifth of a strip: chunks shorter than a wave and more than
-  128 coarse vectors) against the oracle's LAPACK solve; and at its full size
+  coarse chunks of a twenty-fifth of a strip: chunks shorter than a wave and
+  more than 512 coarse vectors) against the oracle's LAPACK solve; and at its full size
   (250 x 120 = 30,000 panels) against the committed oracle fixture
   (tests/golden/make_bwb_fixture.py).
 * Every body of an uneven body batch against its own oracle LAPACK solve.
@@ -49,8 +49,8 @@ def _bwb(fp, nc, ns, alpha=5.0):
 
 def _bench_opts(nc, tol, inner_tol=1e-6):
     # the benchmark's layout: block-Jacobi blocks of four chordwise strips,
-    # Galerkin coarse chunks of a fifth of a strip
-    return {"precond_block": 4 * nc, "coarse_block": nc // 5, "tol": tol, "inner_tol": inner_tol,
+    # Galerkin coarse chunks of a twenty-fifth of a strip
+    return {"precond_block": 4 * nc, "coarse_block": max(1, nc // 25), "tol": tol, "inner_tol": inner_tol,
             "restart": 100, "residual_mode": 1}
 
 
@@ -60,7 +60,7 @@ def _cp_nokutta(fp, body):
 
 
 def test_bwb_reduced_bench_layout_matches_oracle(fp, orc):
-    nc, ns = 50, 40                     # 2,000 panels; chunks of 10 -> 200 coarse vectors
+    nc, ns = 50, 40                     # 2,000 panels; chunks of 2 -> 1,000 coarse vectors
     body, U, D = _bwb(fp, nc, ns)
     Go, normals, cps = orc.assemble([_obody(body)], [D], [D])
     gam = orc.solve_lu(Go, orc.rhs(U, normals))

@@ -138,6 +138,36 @@ def test_coarse_space_above_128_vectors(fp, coarse, n):
     assert it2 <= it1, (it2, it1)
 
 
+@pytest.mark.parametrize("n,coarse", [(2600, 2), (2601, 5), (3000, 1)])
+def test_coarse_inverse_lu_matches_gauss_jordan(fp, n, coarse, monkeypatch):
+    """Coarse spaces of 1,300, 521 and 3,000 vectors inverted through the randomized
+    no-pivot FP32 LU (FLOWPANEL_COARSE_INV=lu: U^T E V, explicit inverses of its
+    factors, V (.) U^T back) against the blocked Gauss-Jordan with pivoting, on a
+    nonsymmetric matrix with strong coupling between chunks: both inverses are FP32
+    approximations of the same E^-1, so the two-level GMRES takes the same number of
+    iterations (within one) and reaches the same FP64 solution.  A transposed or
+    mis-indexed inverse (a wrong butterfly side) would not converge in as few.
+    (The diagonal shift keeps E's eigenvalues away from zero: with 2 I instead of
+    8 I the chunk coupling puts them in a disk around 4 of radius ~8, and E is too
+    close to singular for any FP32 factorisation.)"""
+    rng = np.random.default_rng(8000 + n)
+    A = rng.standard_normal((n, n)) / np.sqrt(n) + 8.0 * np.eye(n)
+    A += np.kron(rng.uniform(0.0, 0.4, (n // coarse + 1, n // coarse + 1)), np.ones((coarse, coarse)))[:n, :n] / coarse
+    b = rng.standard_normal(n)
+    monkeypatch.setenv("FLOWPANEL_COARSE_INV", "lu")
+    y_lu = fp.solve_gmres_ir(None, A, b, precond_block=100, coarse_block=coarse, tol=1e-12)
+    it_lu = fp.solvers.last_stats["iterations"]
+    monkeypatch.delenv("FLOWPANEL_COARSE_INV")
+    y_gj = fp.solve_gmres_ir(None, A, b, precond_block=100, coarse_block=coarse, tol=1e-12)
+    it_gj = fp.solvers.last_stats["iterations"]
+    fp.solve_gmres_ir(None, A, b, precond_block=100, tol=1e-12)
+    it_bj = fp.solvers.last_stats["iterations"]
+    x = np.linalg.solve(A, b)
+    assert rel(y_lu, x) < 1e-10 and rel(y_gj, x) < 1e-10
+    assert abs(it_lu - it_gj) <= 1, (it_lu, it_gj)
+    assert it_lu < it_bj, (it_lu, it_bj)
+
+
 @pytest.mark.parametrize("n,coarse", [(1001, 7), (1200, 50), (999, 63), (640, 1)])
 def test_coarse_chunk_sums_coalesced_match_lane_sums(fp, n, coarse, monkeypatch):
     """The coarse space's chunk sums (A Z and Z^T A Z) read through LDS segments
