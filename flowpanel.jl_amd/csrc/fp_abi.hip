@@ -354,14 +354,25 @@ struct BJPrecond : Precond {
             // microseconds per launch, and the block inverses are about a hundred)
             FP_HIP(ctx, hipEventRecord(fork, ctx->stream));
             FP_HIP(ctx, hipStreamWaitEvent(ctx->side, fork, 0));
+            // the coarse inverse's look-ahead passes on the fourth stream, with events of their own
+            // (the block inverses' look-ahead uses the third stream and gj_fork / gj_join meanwhile)
+            if (!ctx->side3) FP_HIP(ctx, hipStreamCreateWithFlags(&ctx->side3, hipStreamNonBlocking));
+            if (!ctx->cgj_fork) FP_HIP(ctx, hipEventCreateWithFlags(&ctx->cgj_fork, hipEventDisableTiming));
+            if (!ctx->cgj_join) FP_HIP(ctx, hipEventCreateWithFlags(&ctx->cgj_join, hipEventDisableTiming));
             fp_ctx sc;
             sc.device = ctx->device;
             sc.num_cu = ctx->num_cu;
             sc.stream = ctx->side;
+            sc.side2 = ctx->side3;
+            sc.gj_fork = ctx->cgj_fork;
+            sc.gj_join = ctx->cgj_join;
             st = prec == FP_F64 ? fp_coarse_setup<double>(&sc, (const double *)A, n, ld, cs, AZ.as<float>(),
-                                                          Einv.as<double>(), work.as<double>(), cinfo.as<int>())
+                                                          Einv.as<double>(), work.as<double>(), cinfo.as<int>(), true)
                                 : fp_coarse_setup<float>(&sc, (const float *)A, n, ld, cs, AZ.as<float>(),
-                                                         Einv.as<double>(), work.as<double>(), cinfo.as<int>());
+                                                         Einv.as<double>(), work.as<double>(), cinfo.as<int>(), true);
+            sc.stream = nullptr;
+            sc.side2 = nullptr;
+            sc.gj_fork = sc.gj_join = nullptr;
             if (st) return fp_fail(ctx, st, sc.err.c_str());
             FP_HIP(ctx, hipEventRecord(join, ctx->side));
         }
@@ -1630,7 +1641,7 @@ void fp_ctx_destroy(fp_ctx *ctx)
             (void)hipStreamSynchronize(sd);
             (void)hipStreamDestroy(sd);
         }
-    for (hipEvent_t e : {ctx->gj_fork, ctx->gj_join})
+    for (hipEvent_t e : {ctx->gj_fork, ctx->gj_join, ctx->cgj_fork, ctx->cgj_join})
         if (e) (void)hipEventDestroy(e);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     for (auto &r : ctx->prof_pending) {
@@ -2880,7 +2891,7 @@ static int transpose_solves_concurrent(fp_ctx *ctx, const SolveArgs &at, const f
     (void)hipEventDestroy(ready);
     for (hipStream_t sd : {child.side, child.side2, child.side3})
         if (sd) (void)hipStreamDestroy(sd);
-    for (hipEvent_t ev : {child.gj_fork, child.gj_join})
+    for (hipEvent_t ev : {child.gj_fork, child.gj_join, child.cgj_fork, child.cgj_join})
         if (ev) (void)hipEventDestroy(ev);
     (void)hipStreamDestroy(child.stream);
     child.stream = nullptr;

@@ -2449,9 +2449,11 @@ size_t fp_coarse_work_bytes(int64_t nc)
     return std::max<size_t>((size_t)2 * nc * nc * sizeof(double), fp_inverse_rbt_work_bytes(nc));
 }
 
+// lookahead: each next Gauss-Jordan chunk's pass runs on ctx->side2 (with the
+// ctx->gj_fork / gj_join events) beside the current chunk's products
 template <typename T>
 int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, float *AZ, double *Einv, double *work,
-                    int *info_dev)
+                    int *info_dev, bool lookahead)
 {
     const int nc = (int)((n + cs - 1) / cs);
     if (nc > FP_COARSE_MAX) return fp_fail(ctx, FP_ERR_ARG, "coarse space larger than 4096 vectors");
@@ -2490,7 +2492,7 @@ int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, floa
         float *inv32 = reinterpret_cast<float *>(work), *scr = inv32 + (int64_t)nc * nc;
         if (fp_bj_gj_scratch_floats(nc, nc) > (size_t)3 * nc * nc)
             return fp_fail(ctx, FP_ERR_ARG, "internal: coarse inverse scratch too small");
-        int st = fp_bj_gj_inverse<double>(ctx, Einv, nc, nc, nc, inv32, info_dev, scr, nullptr, false);
+        int st = fp_bj_gj_inverse<double>(ctx, Einv, nc, nc, nc, inv32, info_dev, scr, nullptr, lookahead);
         if (st) return st;
         hipLaunchKernelGGL(k_widen_f32, dim3((unsigned)(((int64_t)nc * nc + 255) / 256)), dim3(256), 0, ctx->stream,
                            (const float *)inv32, (int64_t)nc * nc, Einv);
@@ -2498,8 +2500,10 @@ int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, floa
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
 }
-template int fp_coarse_setup<double>(fp_ctx *, const double *, int64_t, int64_t, int, float *, double *, double *, int *);
-template int fp_coarse_setup<float>(fp_ctx *, const float *, int64_t, int64_t, int, float *, double *, double *, int *);
+template int fp_coarse_setup<double>(fp_ctx *, const double *, int64_t, int64_t, int, float *, double *, double *, int *,
+                                     bool);
+template int fp_coarse_setup<float>(fp_ctx *, const float *, int64_t, int64_t, int, float *, double *, double *, int *,
+                                    bool);
 
 // t = Z^T v: one wave per chunk (chunk sums of v)
 __global__ __launch_bounds__(256) void k_coarse_restrict(const double *__restrict__ v, int64_t n, int cs, int nc,

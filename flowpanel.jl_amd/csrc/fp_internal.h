@@ -59,6 +59,8 @@ struct fp_ctx {
     // fork/join events of the look-ahead Gauss-Jordan passes, kept for the
     // context's life (destroying an event that is still pending holds the host)
     hipEvent_t gj_fork = nullptr, gj_join = nullptr;
+    // the same for the coarse-space inverse's look-ahead (on the fourth stream)
+    hipEvent_t cgj_fork = nullptr, cgj_join = nullptr;
     // cached scratch (grown on demand, freed with the context)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -315,7 +317,7 @@ int fp_bj_apply_inv(fp_ctx *ctx, const float *inv, int64_t n, int block, const d
 // Einv = (Z^T AZ)^-1 from that same AZ (so that Z^T (v - AZ Einv Z^T v) = 0)
 template <typename T>
 int fp_coarse_setup(fp_ctx *ctx, const T *A, int64_t n, int64_t ld, int cs, float *AZ, double *Einv, double *work,
-                    int *info_dev);
+                    int *info_dev, bool lookahead = false);
 // y = Z Einv Z^T v and r2 = v - AZ Einv Z^T v (tc: 2*nc scratch)
 int fp_coarse_apply(fp_ctx *ctx, const double *v, int64_t n, int cs, const float *AZ, const double *Einv, double *tc,
                     double *y, double *r2);
