@@ -71,19 +71,45 @@ __device__ __forceinline__ void ld16(const T *__restrict__ p, bool full, bool ve
     }
 }
 
+// FP32: K steps of GB_KF through two LDS buffers, one barrier per step (a step's
+// stores go to the buffer the previous step did not read; a wave reaching the
+// store after next has passed the barrier that every wave reaches only after
+// finishing its reads of that buffer).  Measured on 4,608^2 x k products:
+// 87 against 80 TFLOP/s at k = 256 and 107 against 94 at k = 1,024 for K steps
+// of 32 through one buffer (register-only 16x16x4 rate: 125 TFLOP/s).
+constexpr int GB_KF = 16;
+
+// n_ entries from p (16-byte aligned groups when vec and full), zero beyond `valid`
+template <int N_>
+__device__ __forceinline__ void ldf(const float *__restrict__ p, bool full, bool vec, int valid, float *out)
+{
+    if (full && vec) {
+#pragma unroll
+        for (int e = 0; e < N_; e += 4) {
+            const float4 v = *reinterpret_cast<const float4 *>(p + e);
+            out[e] = v.x; out[e + 1] = v.y; out[e + 2] = v.z; out[e + 3] = v.w;
+        }
+    } else {
+        // clamped addresses: every load in bounds and unconditional, the rest zeroed by a product
+#pragma unroll
+        for (int e = 0; e < N_; ++e) out[e] = valid > 0 ? p[min(e, valid - 1)] * (e < valid ? 1.f : 0.f) : 0.f;
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t k, const T *__restrict__ A,
                                                   int64_t lda, const T *__restrict__ B, int64_t ldb,
                                                   T *__restrict__ C, int64_t ldc, int vec, int64_t sA = 0,
                                                   int64_t sB = 0, int64_t sC = 0)
 {
+    static_assert(sizeof(T) == 4, "FP64 products run k_gemm_sub_f64");
     typedef typename MfmaOp<T>::acc acc_t;
     // strided batch: grid row y runs product y (all products the same shape)
     A += (int64_t)blockIdx.y * sA;
     B += (int64_t)blockIdx.y * sB;
     C += (int64_t)blockIdx.y * sC;
-    __shared__ T As[GB_K][GB_M + GB_PAD];
-    __shared__ T Bs[GB_K][GB_N + GB_PAD];
+    __shared__ T As[2][GB_KF][GB_M + GB_PAD];
+    __shared__ T Bs[2][GB_KF][GB_N + GB_PAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int64_t tiles_m = (m + GB_M - 1) / GB_M, tiles_n = (n + GB_N - 1) / GB_N;
@@ -103,74 +129,68 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t 
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
 
-    // A: row tid/2, k (tid%2)*16..+16 ; B: k row tid/8, cols (tid%8)*16..+16
-    const int arow = tid >> 1, ak = (tid & 1) * 16;
-    const int bk = tid >> 3, bc = (tid & 7) * 16;
-    T ra[16], rb[16];
+    // A: row tid/2, k (tid%2)*8..+8 ; B: k row tid/16, cols (tid%16)*8..+8
+    constexpr int AE = GB_M * GB_KF / 256, BE = GB_KF * GB_N / 256;
+    const int arow = tid / (GB_KF / AE), ak = (tid % (GB_KF / AE)) * AE;
+    const int bk = tid / (GB_N / BE), bc = (tid % (GB_N / BE)) * BE;
+    T ra[AE], rb[BE];
     auto load = [&](int64_t k0) {
         const int64_t gr = m0 + arow, gk = k0 + ak;
-        const int av = gr < m ? (int)max((int64_t)0, min((int64_t)16, k - gk)) : 0;
-        ld16<T>(A + gr * lda + gk, av == 16, vec, av, ra);
+        const int av = gr < m ? (int)max((int64_t)0, min((int64_t)AE, k - gk)) : 0;
+        ldf<AE>(A + min(gr, m - 1) * lda + gk, av == AE, vec, av, ra);
         const int64_t gkb = k0 + bk, gc = n0 + bc;
-        const int bv = gkb < k ? (int)max((int64_t)0, min((int64_t)16, n - gc)) : 0;
-        ld16<T>(B + gkb * ldb + gc, bv == 16, vec, bv, rb);
+        const int bv = gkb < k ? (int)max((int64_t)0, min((int64_t)BE, n - gc)) : 0;
+        ldf<BE>(B + min(gkb, k - 1) * ldb + gc, bv == BE, vec, bv, rb);
     };
     load(0);
-    for (int64_t k0 = 0; k0 < k; k0 += GB_K) {
+    int buf = 0;
+    for (int64_t k0 = 0; k0 < k; k0 += GB_KF) {
+#pragma unroll
+        for (int e = 0; e < AE; ++e) As[buf][ak + e][arow] = ra[e];
+#pragma unroll
+        for (int e = 0; e < BE; ++e) Bs[buf][bk][bc + e] = rb[e];
         __syncthreads();
+        if (k0 + GB_KF < k) load(k0 + GB_KF);
 #pragma unroll
-        for (int e = 0; e < 16; ++e) As[ak + e][arow] = ra[e];
-#pragma unroll
-        for (int e = 0; e < 16; ++e) Bs[bk][bc + e] = rb[e];
-        __syncthreads();
-        if (k0 + GB_K < k) load(k0 + GB_K);
-#pragma unroll
-        for (int ks = 0; ks < GB_K; ks += 4) {
+        for (int ks = 0; ks < GB_KF; ks += 4) {
             T fa[4], fb[4];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) fa[a] = As[ks + (lane >> 4)][wm * 64 + a * 16 + (lane & 15)];
+            for (int a = 0; a < 4; ++a) fa[a] = As[buf][ks + (lane >> 4)][wm * 64 + a * 16 + (lane & 15)];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) fb[b] = Bs[ks + (lane >> 4)][wn * 64 + b * 16 + (lane & 15)];
+            for (int b = 0; b < 4; ++b) fb[b] = Bs[buf][ks + (lane >> 4)][wn * 64 + b * 16 + (lane & 15)];
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int b = 0; b < 4; ++b) acc[a][b] = MfmaOp<T>::run(fa[a], fb[b], acc[a][b]);
         }
+        buf ^= 1;
     }
     // epilogue: every load of the lane's 64 entries of C is issued before the first
     // store (as far as the compiler knows rows of C could alias, and a load after a
-    // store would wait for it: one memory round trip per entry)
-    if (m0 + GB_M <= m && n0 + GB_N <= n) {   // uniform: an interior tile, no guards
-        T cv[4][4][4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b)
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    cv[a][b][r] = C[(m0 + wm * 64 + a * 16 + MfmaOp<T>::row(lane, r)) * ldc + n0 + wn * 64 + b * 16 +
-                                    (lane & 15)];
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b)
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    C[(m0 + wm * 64 + a * 16 + MfmaOp<T>::row(lane, r)) * ldc + n0 + wn * 64 + b * 16 + (lane & 15)] =
-                        cv[a][b][r] - acc[a][b][r];
-        return;
-    }
+    // store would wait for it: one memory round trip per entry); edge tiles clamp
+    // their addresses and store only inside C
+    const bool interior = m0 + GB_M <= m && n0 + GB_N <= n;   // uniform
+    T cv[4][4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int64_t col = n0 + wn * 64 + b * 16 + (lane & 15);
+        for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t row = m0 + wm * 64 + a * 16 + MfmaOp<T>::row(lane, r);
-                if (row < m && col < n) C[row * ldc + col] -= acc[a][b][r];
+                const int64_t col = n0 + wn * 64 + b * 16 + (lane & 15);
+                cv[a][b][r] = interior ? C[row * ldc + col] : C[min(row, m - 1) * ldc + min(col, n - 1)];
             }
-        }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = m0 + wm * 64 + a * 16 + MfmaOp<T>::row(lane, r);
+                const int64_t col = n0 + wn * 64 + b * 16 + (lane & 15);
+                if (interior || (row < m && col < n)) C[row * ldc + col] = cv[a][b][r] - acc[a][b][r];
+            }
 }
 
 // FP64 tile kernel on v_mfma_f64_4x4x4_4b.  Operand layout of that instruction
