@@ -1813,12 +1813,20 @@ __global__ __launch_bounds__(256) void k_bj_gemm(BjOp Ao, BjOp Bo, BjOp Co, int 
         for (int j = 0; j < 2; ++j) acc[i][j] = fp_f4{0.f, 0.f, 0.f, 0.f};
     const int ar = tid >> 2, ak = (tid & 3) * 4, bk = tid >> 4, bc = (tid & 15) * 4;
     for (int k0 = 0; k0 < kk; k0 += 16) {
+        // clamped addresses (every load in bounds and unconditional, so the eight loads are
+        // in flight together) and a 0/1 factor for the entries outside the operands
+        float av[4], bv[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int r = m0 + ar, kx = k0 + ak + e;
-            As[ak + e][ar] = (r < m && kx < kk) ? Ap[(int64_t)r * Ao.ld + kx] : 0.f;
+            av[e] = Ap[(int64_t)min(r, m - 1) * Ao.ld + min(kx, kk - 1)] * ((r < m && kx < kk) ? 1.f : 0.f);
             const int kb = k0 + bk, c = n0 + bc + e;
-            Bs[bk][bc + e] = (kb < kk && c < nn) ? Bp[(int64_t)kb * Bo.ld + c] : 0.f;
+            bv[e] = Bp[(int64_t)min(kb, kk - 1) * Bo.ld + min(c, nn - 1)] * ((kb < kk && c < nn) ? 1.f : 0.f);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            As[ak + e][ar] = av[e];
+            Bs[bk][bc + e] = bv[e];
         }
         __syncthreads();
 #pragma unroll
@@ -1835,6 +1843,20 @@ __global__ __launch_bounds__(256) void k_bj_gemm(BjOp Ao, BjOp Bo, BjOp Co, int 
         }
         __syncthreads();
     }
+    // every load of C (beta) is issued before the first store: a load after a store to a
+    // row of C that might alias would wait for it, one memory round trip per entry
+    float cv[2][2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int c = n0 + wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = m0 + wm * 32 + i * 16 + MfmaOp<float>::row(lane, q);
+                cv[i][j][q] = beta ? Cp[(int64_t)min(r, m - 1) * Co.ld + min(c, nn - 1)] : 0.f;
+            }
+        }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -1845,10 +1867,7 @@ __global__ __launch_bounds__(256) void k_bj_gemm(BjOp Ao, BjOp Bo, BjOp Co, int 
                 const int r = m0 + wm * 32 + i * 16 + MfmaOp<float>::row(lane, q);
                 const bool skip = (r >= skip0 && r < skip1) || (c >= skip0 && c < skip1) ||
                                   (r >= sq0 && r < sq1 && c >= sq0 && c < sq1);
-                if (r < m && c < nn && !skip) {
-                    float *cp = Cp + (int64_t)r * Co.ld + c;
-                    *cp = alpha * acc[i][j][q] + (beta ? *cp : 0.f);
-                }
+                if (r < m && c < nn && !skip) Cp[(int64_t)r * Co.ld + c] = alpha * acc[i][j][q] + cv[i][j][q];
             }
         }
 }

@@ -259,13 +259,20 @@ __global__ __launch_bounds__(256) void k_gemm64_sub(int64_t m, int64_t n, int64_
                                                               0, 0);
         }
     }
+    // every load of C is issued before the first store (clamped addresses), then the stores
+    float cv[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            cv[q][e] = C[(i0 + min(16 * wave + 4 * (lane >> 4) + e, tr - 1)) * ldc + j0 + min(16 * q + (lane & 15), tc - 1)];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int col = 16 * q + (lane & 15);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int row = 16 * wave + 4 * (lane >> 4) + e;
-            if (row < tr && col < tc) C[(i0 + row) * ldc + j0 + col] -= acc[q][e];
+            if (row < tr && col < tc) C[(i0 + row) * ldc + j0 + col] = cv[q][e] - acc[q][e];
         }
     }
 }
