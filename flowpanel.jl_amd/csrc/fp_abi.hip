@@ -112,6 +112,17 @@ struct Timer {
         return ms;
     }
     void restart() { (void)hipEventRecord(a, s); }
+    // stop without waiting: the elapsed time is read by elapsed() once the stream has
+    // passed b (a stop() would hold the host until the GPU gets there, and the launches
+    // queued after it would reach an idle GPU at the host's pace)
+    void mark() { (void)hipEventRecord(b, s); }
+    double elapsed()
+    {
+        (void)hipEventSynchronize(b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, a, b);
+        return ms;
+    }
     ~Timer()
     {
         if (a) (void)hipEventDestroy(a);
@@ -227,6 +238,9 @@ struct Precond {
 // With coarse_block > 0 a Galerkin coarse correction on chunk indicators Z is
 // applied first (two-level, multiplicative):  y = Z c + B (v - A Z c),
 // c = (Z^T A Z)^-1 Z^T v.
+// coarse spaces up to this many vectors are queued ahead of the block inverses (BJPrecond::setup)
+constexpr int COARSE_FIRST_MAX = 1024;
+
 struct BJPrecond : Precond {
     fp_ctx *ctx; int64_t n; int block;
     DBuf inv, gjs, AZ, Einv, work, tc, r2, cinfo, binfo;
@@ -348,12 +362,13 @@ struct BJPrecond : Precond {
             FP_HIP(ctx, hipEventCreateWithFlags(&fork, hipEventDisableTiming));
             FP_HIP(ctx, hipEventCreateWithFlags(&join, hipEventDisableTiming));
         }
-        if (coarse > 0) {
-            // the coarse space does not depend on the block inverses: it is built on
-            // the side stream beside them, and queued first (the host takes a few
-            // microseconds per launch, and the block inverses are about a hundred)
-            FP_HIP(ctx, hipEventRecord(fork, ctx->stream));
-            FP_HIP(ctx, hipStreamWaitEvent(ctx->side, fork, 0));
+        // the coarse space does not depend on the block inverses: it is built on the side
+        // stream beside them.  The host spends a few microseconds per launch: a short coarse
+        // chain (up to COARSE_FIRST_MAX vectors) is queued first so that it starts at once;
+        // a long one (nc / 64 Gauss-Jordan chunks of five launches and two events each) is
+        // queued after the block inverses, which would otherwise wait for the host to finish
+        // queueing it (3,000 vectors: the block inverses began 4.5 ms late)
+        auto queue_coarse = [&]() -> int {
             // the coarse inverse's look-ahead passes on the fourth stream, with events of their own
             // (the block inverses' look-ahead uses the third stream and gj_fork / gj_join meanwhile)
             if (!ctx->side3) FP_HIP(ctx, hipStreamCreateWithFlags(&ctx->side3, hipStreamNonBlocking));
@@ -366,21 +381,31 @@ struct BJPrecond : Precond {
             sc.side2 = ctx->side3;
             sc.gj_fork = ctx->cgj_fork;
             sc.gj_join = ctx->cgj_join;
-            st = prec == FP_F64 ? fp_coarse_setup<double>(&sc, (const double *)A, n, ld, cs, AZ.as<float>(),
-                                                          Einv.as<double>(), work.as<double>(), cinfo.as<int>(), true)
-                                : fp_coarse_setup<float>(&sc, (const float *)A, n, ld, cs, AZ.as<float>(),
-                                                         Einv.as<double>(), work.as<double>(), cinfo.as<int>(), true);
+            int cst = prec == FP_F64
+                          ? fp_coarse_setup<double>(&sc, (const double *)A, n, ld, cs, AZ.as<float>(), Einv.as<double>(),
+                                                    work.as<double>(), cinfo.as<int>(), true)
+                          : fp_coarse_setup<float>(&sc, (const float *)A, n, ld, cs, AZ.as<float>(), Einv.as<double>(),
+                                                   work.as<double>(), cinfo.as<int>(), true);
             sc.stream = nullptr;
             sc.side2 = nullptr;
             sc.gj_fork = sc.gj_join = nullptr;
-            if (st) return fp_fail(ctx, st, sc.err.c_str());
+            if (cst) return fp_fail(ctx, cst, sc.err.c_str());
             FP_HIP(ctx, hipEventRecord(join, ctx->side));
+            return FP_OK;
+        };
+        bool coarse_first = nc <= COARSE_FIRST_MAX;
+        if (const char *cf = getenv("FLOWPANEL_COARSE_FIRST")) coarse_first = cf[0] == '1';
+        if (coarse > 0) {
+            FP_HIP(ctx, hipEventRecord(fork, ctx->stream));
+            FP_HIP(ctx, hipStreamWaitEvent(ctx->side, fork, 0));
+            if (coarse_first && (st = queue_coarse())) return st;
         }
         st = prec == FP_F64 ? fp_bj_gj_inverse<double>(ctx, (const double *)A, n, ld, block, inv.as<float>(), info_dev,
                                                        sp)
                             : fp_bj_gj_inverse<float>(ctx, (const float *)A, n, ld, block, inv.as<float>(), info_dev,
                                                       sp);
         if (st) return st;
+        if (coarse > 0 && !coarse_first && (st = queue_coarse())) return st;
         if (coarse > 0) FP_HIP(ctx, hipStreamWaitEvent(ctx->stream, join, 0));
         return FP_OK;
     }
@@ -1961,7 +1986,7 @@ static int solve_system(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_so
     } else {
         a.A32 = A.as<float>();
     }
-    s.ms_assemble = tm.stop();
+    tm.mark();   // read after the solve: the set-up is queued while the assembly runs
     if (o.solver == FP_SOLVER_GMRES) {
         op64.reset(new DenseOp<double>(ctx, a.A64, n, ld));
         a.resid = op64.get();
@@ -1970,7 +1995,9 @@ static int solve_system(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_so
     Solver solver(ctx, a, o);
     if (pre) solver.adopt(std::move(pre));
     if ((st = solver.setup(s))) return st;
-    return solver.solve_multi(rhs, x, k, s);
+    st = solver.solve_multi(rhs, x, k, s);
+    s.ms_assemble = tm.elapsed();
+    return st;
 }
 
 static int check_solve_inputs(fp_ctx *ctx, fp_body *const *bodies, int nb, const fp_solve_opts *opts,
