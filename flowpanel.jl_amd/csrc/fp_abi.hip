@@ -315,7 +315,7 @@ struct BJPrecond : Precond {
             if ((st = dalloc(ctx, AZ, (size_t)nc * n * sizeof(float))) ||
                 (st = dalloc(ctx, Einv, (size_t)nc * nc * sizeof(double))) ||
                 (st = dalloc(ctx, work, fp_coarse_work_bytes(nc))) ||
-                (st = dalloc(ctx, tc, 2 * nc * sizeof(double))) || (st = dalloc(ctx, r2, n * sizeof(double))) ||
+                (st = dalloc(ctx, tc, fp_coarse_tc_bytes(n, nc))) || (st = dalloc(ctx, r2, n * sizeof(double))) ||
                 (st = dalloc(ctx, cinfo, sizeof(int))))
                 return st;
             if (!ctx->side) FP_HIP(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
@@ -355,7 +355,7 @@ struct BJPrecond : Precond {
             if ((st = dalloc(ctx, AZ, (size_t)nc * n * sizeof(float)))) return st;
             if ((st = dalloc(ctx, Einv, (size_t)nc * nc * sizeof(double)))) return st;
             if ((st = dalloc(ctx, work, fp_coarse_work_bytes(nc)))) return st;
-            if ((st = dalloc(ctx, tc, 2 * nc * sizeof(double)))) return st;
+            if ((st = dalloc(ctx, tc, fp_coarse_tc_bytes(n, nc)))) return st;
             if ((st = dalloc(ctx, r2, n * sizeof(double)))) return st;
             if ((st = dalloc(ctx, cinfo, sizeof(int)))) return st;
             if (!ctx->side) FP_HIP(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));

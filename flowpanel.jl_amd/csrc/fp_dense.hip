@@ -2646,6 +2646,71 @@ __global__ __launch_bounds__(256) void k_coarse_resid4(const double *__restrict_
     }
 }
 
+// k_coarse_resid4 with the coarse vectors split over grid.y = ks slices: at 30,000
+// rows one slice is 118 workgroups for 256 CUs (360 MB of A Z at 3 TB/s); the
+// slices' partial sums go to part[s * n + i] and k_coarse_finish adds them.
+__global__ __launch_bounds__(256) void k_coarse_resid4s(const float *__restrict__ AZ, int64_t n, int nc,
+                                                        const double *__restrict__ c, double *__restrict__ part)
+{
+    __shared__ double ps[4][4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i0 = (int64_t)blockIdx.x * 256 + 4 * lane;
+    const int ks = gridDim.y, sl = blockIdx.y;
+    const int kb = (int)((int64_t)nc * sl / ks), ke = (int)((int64_t)nc * (sl + 1) / ks);
+    double acc[4] = {0, 0, 0, 0};
+    if (i0 < n) {
+        int k = kb + wave;
+        for (; k + 12 < ke; k += 16) {   // 4 independent 16-byte loads per step
+            float4 a[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a[q] = *reinterpret_cast<const float4 *>(AZ + (int64_t)(k + 4 * q) * n + i0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double cq = c[k + 4 * q];
+                acc[0] += (double)a[q].x * cq;
+                acc[1] += (double)a[q].y * cq;
+                acc[2] += (double)a[q].z * cq;
+                acc[3] += (double)a[q].w * cq;
+            }
+        }
+        for (; k < ke; k += 4) {
+            const float4 a = *reinterpret_cast<const float4 *>(AZ + (int64_t)k * n + i0);
+            const double ck = c[k];
+            acc[0] += (double)a.x * ck;
+            acc[1] += (double)a.y * ck;
+            acc[2] += (double)a.z * ck;
+            acc[3] += (double)a.w * ck;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) ps[wave][t][lane] = acc[t];
+    __syncthreads();
+    const int64_t i = i0 + wave;
+    if (i0 < n && i < n) part[(int64_t)sl * n + i] = ps[0][wave][lane] + ps[1][wave][lane] + ps[2][wave][lane] + ps[3][wave][lane];
+}
+
+__global__ __launch_bounds__(256) void k_coarse_finish(const double *__restrict__ v, int64_t n, int cs, int ks,
+                                                       const double *__restrict__ c, const double *__restrict__ part,
+                                                       double *__restrict__ y, double *__restrict__ r2)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a = 0.0;
+    for (int s = 0; s < ks; ++s) a += part[(int64_t)s * n + i];
+    r2[i] = v[i] - a;
+    y[i] = c[i / cs];
+}
+
+// slices of k_coarse_resid4s for n rows: about 1,024 workgroups, at most COARSE_KS
+constexpr int COARSE_KS = 8;
+static int coarse_slices(int64_t n, int nc)
+{
+    const int64_t rb = (n + 255) / 256;
+    return (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)COARSE_KS, (1024 + rb - 1) / rb, (int64_t)nc / 16}));
+}
+
+size_t fp_coarse_tc_bytes(int64_t n, int nc) { return ((size_t)2 * nc + (size_t)COARSE_KS * n) * sizeof(double); }
+
 int fp_coarse_apply(fp_ctx *ctx, const double *v, int64_t n, int cs, const float *AZ, const double *Einv, double *tc,
                     double *y, double *r2)
 {
@@ -2654,7 +2719,14 @@ int fp_coarse_apply(fp_ctx *ctx, const double *v, int64_t n, int cs, const float
     const unsigned g4 = (unsigned)((nc + 3) / 4);
     hipLaunchKernelGGL(k_coarse_restrict, dim3(g4), dim3(256), 0, ctx->stream, v, n, cs, nc, tc);
     hipLaunchKernelGGL(k_coarse_solve, dim3(g4), dim3(256), 0, ctx->stream, Einv, nc, (const double *)tc, tc + nc);
-    if (n % 4 == 0 && (uintptr_t)AZ % 16 == 0)
+    const int ks = coarse_slices(n, nc);
+    if (n % 4 == 0 && (uintptr_t)AZ % 16 == 0 && ks > 1) {
+        double *part = tc + 2 * nc;   // tc holds fp_coarse_tc_bytes
+        hipLaunchKernelGGL(k_coarse_resid4s, dim3((unsigned)((n + 255) / 256), (unsigned)ks), dim3(256), 0, ctx->stream,
+                           AZ, n, nc, (const double *)(tc + nc), part);
+        hipLaunchKernelGGL(k_coarse_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, v, n, cs, ks,
+                           (const double *)(tc + nc), (const double *)part, y, r2);
+    } else if (n % 4 == 0 && (uintptr_t)AZ % 16 == 0)
         hipLaunchKernelGGL(k_coarse_resid4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, v, AZ, n, cs,
                            nc, (const double *)(tc + nc), y, r2);
     else

@@ -270,6 +270,8 @@ int fp_inverse_rbt_f32(fp_ctx *ctx, double *E, int64_t n, int64_t lde, void *wor
                        uint64_t seed);
 // workspace of fp_coarse_setup for nc coarse vectors
 size_t fp_coarse_work_bytes(int64_t nc);
+// the coarse apply's vector workspace (tc) for n rows and nc coarse vectors
+size_t fp_coarse_tc_bytes(int64_t n, int nc);
 // device address of fp_uind_noalloc's scratch array, queried once per GPU (fp_noalloc.hip)
 void fp_noalloc_init(fp_ctx *ctx);
 
