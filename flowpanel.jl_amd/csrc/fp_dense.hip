@@ -1770,15 +1770,24 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_bj_copy(const T *__restrict__ A, int64_t n, int64_t ld, int block,
                                                  float *__restrict__ X)
 {
+    // every thread four consecutive columns per pass: all loads of a pass issued together
+    // (clamped addresses, so each is in bounds and unconditional), then the stores
     const int b = blockIdx.y, r = blockIdx.x;
     const int64_t b0 = (int64_t)b * block;
     const int bs = (int)min((int64_t)block, n - b0);
     float *x = X + (int64_t)b * block * block + (int64_t)r * block;
-    if (r < bs) {
-        const T *a = A + (b0 + r) * ld + b0;
-        for (int c = threadIdx.x; c < block; c += blockDim.x) x[c] = c < bs ? (float)a[c] : 0.f;
-    } else {
+    if (r >= bs) {   // uniform: a row of the identity padding
         for (int c = threadIdx.x; c < block; c += blockDim.x) x[c] = c == r ? 1.f : 0.f;
+        return;
+    }
+    const T *a = A + (b0 + r) * ld + b0;
+    for (int c0 = 4 * threadIdx.x; c0 < block; c0 += 4 * blockDim.x) {
+        T w[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) w[e] = a[min(c0 + e, bs - 1)];
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if (c0 + e < block) x[c0 + e] = (float)w[e] * (c0 + e < bs ? 1.f : 0.f);
     }
 }
 
