@@ -32,6 +32,7 @@
 #include "fp_internal.h"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace {
@@ -512,10 +513,21 @@ int fp_rbt_vec(fp_ctx *ctx, const double *in, int64_t ldin, double *out, int64_t
 // rank-256 update: a rank-64 update per 64 columns reads and writes the whole
 // trailing matrix four times as often (at 4,800 unknowns it bound the
 // factorisation: 41 us per step against 32 us for the diagonal block).
-constexpr int NOB = 4 * NPB;
+constexpr int NOB = 4 * NPB, NOB_MAX = 8 * NPB;
 
-// Linv and Uinv of the NOB / NPB inner diagonal blocks
-size_t fp_getrf_nopiv_work_bytes() { return (size_t)2 * (NOB / NPB) * NPB * NPB * sizeof(float); }
+// outer block: NOB columns, or FLOWPANEL_LU32_OUTER (a multiple of 64 up to NOB_MAX)
+static int outer_block()
+{
+    static const int v = [] {
+        const char *e = getenv("FLOWPANEL_LU32_OUTER");
+        const int o = e ? atoi(e) : NOB;
+        return (o >= NPB && o <= NOB_MAX && o % NPB == 0) ? o : NOB;
+    }();
+    return v;
+}
+
+// Linv and Uinv of up to NOB_MAX / NPB inner diagonal blocks
+size_t fp_getrf_nopiv_work_bytes() { return (size_t)2 * (NOB_MAX / NPB) * NPB * NPB * sizeof(float); }
 
 // LU without pivoting of the n x n FP32 matrix A (row-major, ld a multiple of 4);
 // info_dev receives 1 + the first row of the diagonal block whose pivot was zero
@@ -527,8 +539,9 @@ int fp_getrf_nopiv_f32(fp_ctx *ctx, float *A, int64_t n, int64_t ld, int *info_d
     if (ld % 4) return fp_fail(ctx, FP_ERR_ARG, "internal: no-pivot LU needs a leading dimension divisible by 4");
     hipLaunchKernelGGL(k_fill_info, dim3(1), dim3(64), 0, ctx->stream, info_dev, 0);
     int st;
-    for (int64_t k = 0; k < n; k += NOB) {
-        const int64_t ob = std::min<int64_t>(NOB, n - k), kend = k + ob;
+    const int nob = outer_block();
+    for (int64_t k = 0; k < n; k += nob) {
+        const int64_t ob = std::min<int64_t>(nob, n - k), kend = k + ob;
         const int T = (int)((ob + NPB - 1) / NPB);
         for (int t = 0; t < T; ++t) {
             const int64_t c = k + (int64_t)NPB * t;
