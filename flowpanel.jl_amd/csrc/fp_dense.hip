@@ -1972,7 +1972,7 @@ __global__ __launch_bounds__(64) void k_bj_gj64(const T *A, int64_t a_bstride, i
                                                 int64_t i_bstride, int64_t i_ld, int64_t n, int block, int off,
                                                 int hs, int *__restrict__ info)
 {
-    __shared__ __align__(16) int prow_of[64];
+    __shared__ int prow_of[64];
     __shared__ float4 prow[16];
     const int b = blockIdx.x, lane = threadIdx.x;
     const int64_t b0 = (int64_t)b * block;
@@ -1980,18 +1980,10 @@ __global__ __launch_bounds__(64) void k_bj_gj64(const T *A, int64_t a_bstride, i
     const T *src = A + (int64_t)b * a_bstride + (int64_t)off * a_ld + off;
     // rows and columns at or beyond bs: the identity's, so that steps k >= bs pivot on
     // row k with 1 and change nothing (every step runs, the row stays in registers)
-    // (clamped addresses and a 0/1 factor: every load unconditional and all 64 in flight
-    // together; a select would put each load behind a branch of its own)
     float a[64];
-    {
-        const T *srow = src + (int64_t)min(lane, bs - 1) * a_ld;
-        const float rin = lane < bs ? 1.f : 0.f;
 #pragma unroll
-        for (int c = 0; c < 64; ++c) {
-            const float m = c < bs ? rin : 0.f;
-            a[c] = (float)srow[min(c, bs - 1)] * m + (lane == c ? 1.f - m : 0.f);
-        }
-    }
+    for (int c = 0; c < 64; ++c)
+        a[c] = (lane < bs && c < bs) ? (float)src[(int64_t)lane * a_ld + c] : (lane == c ? 1.f : 0.f);
     bool used = false;
     int step = 0;
     float mypiv = 1.f;
@@ -2005,18 +1997,11 @@ __global__ __launch_bounds__(64) void k_bj_gj64(const T *A, int64_t a_bstride, i
     for (int c = 0; c < 64; ++c) a[c] *= rp;
     __syncthreads();
     float *I = inv + (int64_t)b * i_bstride + (int64_t)off * i_ld + off;
-    // the pivot rows of all columns first (16 broadcast reads), then the stores back to back
-    int po[64];
-#pragma unroll
-    for (int c = 0; c < 64; c += 4) {
-        const int4 v = reinterpret_cast<const int4 *>(prow_of)[c / 4];
-        po[c] = v.x; po[c + 1] = v.y; po[c + 2] = v.z; po[c + 3] = v.w;
-    }
     if (lane < bs) {
         float *row = I + (int64_t)step * i_ld;
 #pragma unroll
         for (int c = 0; c < 64; ++c)
-            if (c < bs) row[po[c]] = a[c];
+            if (c < bs) row[prow_of[c]] = a[c];
     }
     if (hs < 0 && bs < block && lane < block) {   // a short whole block: the rest is the identity's
         if (lane < bs) {
