@@ -1292,6 +1292,11 @@ struct BatchImpl {
     hipGraphExec_t exec = nullptr;
     int graph_nodes = 0;
     bool recapture = false;   // a body switched to the pivoted LU since the last capture
+    // batched FP32 LU of the randomized bodies (FLOWPANEL_BATCH_LU=bodies: each body its own)
+    bool batched_lu = true;
+    DBuf lu_tab;
+    std::vector<int64_t> lu_n;
+    hipEvent_t fork2 = nullptr;
     bool lu32() const { return o.solver == FP_SOLVER_LU32_IR; }
     bool stored() const { return o.solver == FP_SOLVER_LU || o.residual_mode == FP_RESID_STORED; }
 
@@ -1302,6 +1307,7 @@ struct BatchImpl {
         for (auto &s : streams)
             if (s) (void)hipStreamDestroy(s);
         if (fork) (void)hipEventDestroy(fork);
+        if (fork2) (void)hipEventDestroy(fork2);
         for (auto &e : join)
             if (e) (void)hipEventDestroy(e);
     }
@@ -1414,7 +1420,11 @@ struct BatchImpl {
     }
 
     // the work of one body after the queue assembly, on the current ctx->stream
-    int issue_body(BatchBody &bb)
+    // phase ISSUE_ALL: a body's whole solve; with the batched LU, ISSUE_PRE stops a
+    // randomized body after U^T A V (its factorisation joins the batched one) and
+    // ISSUE_POST issues its refinement (pivoted bodies do everything in ISSUE_PRE)
+    enum { ISSUE_ALL, ISSUE_PRE, ISSUE_POST };
+    int issue_body(BatchBody &bb, int phase = ISSUE_ALL)
     {
         fp_body *b = bb.b;
         const int64_t n = bb.n, ld = bb.ld;
@@ -1423,6 +1433,11 @@ struct BatchImpl {
         double *nb = norms.as<double>();
         size_t k = 0;
         while (bodies[k].get() != &bb) ++k;
+        const bool joint = lu32() && !bb.pivoted;   // takes part in the batched LU
+        if (phase == ISSUE_POST) {
+            if (!joint) return FP_OK;
+            return refine_body(bb, k);
+        }
         if (stored()) {
             if ((st = fp_launch_assemble_wake<double>(ctx, b, bb.t.tv, bb.A64.as<double>(), ld, 0))) return st;
         } else {
@@ -1449,29 +1464,69 @@ struct BatchImpl {
                                           bb.fwork.p, fp_getrf_work_bytes(n, sizeof(float)))))
                     return st;
                 bb.diag_ready = false;
+            } else if (phase == ISSUE_PRE) {
+                return rbt_matrix(bb);   // factored by the batched LU, refined in ISSUE_POST
             } else if ((st = rbt_factor(bb))) {
                 return st;
             }
-            if ((st = fp_fill(ctx, x, n, 0.0))) return st;
-            FP_HIP(ctx, hipMemcpyAsync(r, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-            for (int s = 0; s < sweeps; ++s)
-                if ((st = sweep(bb))) return st;
+            return refine_body(bb, k);
         }
         if ((st = fp_colnorm2(ctx, r, n, n, 1, nb + bodies.size() + k))) return st;
         FP_HIP(ctx, hipMemcpyAsync(b->d_strength, x, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         return FP_OK;
     }
 
-    // U^T A V of the FP64 (stored) or FP32 AIC, then its LU without pivoting
+    // the FP32 refinement sweeps of body k from x = 0, then its residual norm and strengths
+    int refine_body(BatchBody &bb, size_t k)
+    {
+        const int64_t n = bb.n;
+        int st;
+        double *x = bb.x.as<double>(), *r = bb.r.as<double>(), *rhs = bb.rhs.as<double>();
+        if ((st = fp_fill(ctx, x, n, 0.0))) return st;
+        FP_HIP(ctx, hipMemcpyAsync(r, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        for (int s = 0; s < sweeps; ++s)
+            if ((st = sweep(bb))) return st;
+        if ((st = fp_colnorm2(ctx, r, n, n, 1, norms.as<double>() + bodies.size() + k))) return st;
+        FP_HIP(ctx, hipMemcpyAsync(bb.b->d_strength, x, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        return FP_OK;
+    }
+
+    // U^T A V of the FP64 (stored) or FP32 AIC
+    int rbt_matrix(BatchBody &bb)
+    {
+        bb.diag_ready = false;
+        return stored() ? fp_rbt_matrix<double>(ctx, bb.A64.as<double>(), bb.n, bb.ld, bb.Ar.as<float>(), bb.ld4,
+                                                bb.W.as<double>())
+                        : fp_rbt_matrix<float>(ctx, bb.A32.as<float>(), bb.n, bb.ld, bb.Ar.as<float>(), bb.ld4,
+                                               bb.W.as<double>());
+    }
+
+    // U^T A V, then its LU without pivoting
     int rbt_factor(BatchBody &bb)
     {
-        int st = stored() ? fp_rbt_matrix<double>(ctx, bb.A64.as<double>(), bb.n, bb.ld, bb.Ar.as<float>(), bb.ld4,
-                                                  bb.W.as<double>())
-                          : fp_rbt_matrix<float>(ctx, bb.A32.as<float>(), bb.n, bb.ld, bb.Ar.as<float>(), bb.ld4,
-                                                 bb.W.as<double>());
+        int st = rbt_matrix(bb);
         if (st) return st;
-        bb.diag_ready = false;
         return fp_getrf_nopiv_f32(ctx, bb.Ar.as<float>(), bb.n4, bb.ld4, bb.info.as<int>(), bb.nwork.p);
+    }
+
+    // the device table of the batched LU: every randomized (not pivoted) body's U^T A V
+    int build_lu_table()
+    {
+        std::vector<fp_lu_desc> t;
+        lu_n.clear();
+        for (auto &bb : bodies)
+            if (!bb->pivoted) {
+                t.push_back(fp_lu_desc{bb->Ar.as<float>(), bb->ld4, bb->n4, (float *)bb->nwork.p, bb->info.as<int>()});
+                lu_n.push_back(bb->n4);
+            }
+        if (t.empty()) return FP_OK;
+        int st;
+        if (!lu_tab.p || lu_tab.bytes < t.size() * sizeof(fp_lu_desc)) {
+            lu_tab.reset();
+            if ((st = dalloc(ctx, lu_tab, t.size() * sizeof(fp_lu_desc)))) return st;
+        }
+        FP_HIP(ctx, hipMemcpy(lu_tab.p, t.data(), t.size() * sizeof(fp_lu_desc), hipMemcpyHostToDevice));
+        return FP_OK;
     }
 
     // one refinement sweep: d = A32^-1 r, x += d, r = b - A x (FP64)
@@ -1542,9 +1597,16 @@ struct BatchImpl {
         if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
         if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
         const bool prof = ctx->prof;
-        ctx->prof = false;  // no profiling events inside the graph
         hipStream_t main = ctx->stream;
         int st = FP_OK;
+        const char *bl = getenv("FLOWPANEL_BATCH_LU");
+        batched_lu = !(bl && !strcmp(bl, "bodies"));
+        const bool joint = lu32() && batched_lu;
+        if (joint) {
+            if ((st = build_lu_table())) return st;
+            if (!fork2) FP_HIP(ctx, hipEventCreateWithFlags(&fork2, hipEventDisableTiming));
+        }
+        ctx->prof = false;  // no profiling events inside the graph
         hipError_t e = hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) { ctx->prof = prof; return fp_hip_fail(ctx, e, "hipStreamBeginCapture"); }
         st = assemble_queue();
@@ -1554,8 +1616,30 @@ struct BatchImpl {
         for (auto &bb : bodies) {
             if (st) break;
             ctx->stream = streams[bb->stream];
-            st = issue_body(*bb);
+            st = issue_body(*bb, joint ? ISSUE_PRE : ISSUE_ALL);
             ctx->stream = main;
+        }
+        if (joint && !st && !lu_n.empty()) {
+            // every stream's U^T A V joins the main stream, the batched LU runs there, and the
+            // refinements fork back onto the bodies' streams
+            for (int q = 0; q < BATCH_STREAMS && !st; ++q) {
+                if ((e = hipEventRecord(join[q], streams[q])) != hipSuccess) st = fp_hip_fail(ctx, e, "hipEventRecord");
+                else if ((e = hipStreamWaitEvent(main, join[q], 0)) != hipSuccess)
+                    st = fp_hip_fail(ctx, e, "hipStreamWaitEvent");
+            }
+            // (releasing each body's refinement right after its last outer block, beside the larger
+            // bodies' steps, measured 112.8 against 108.7 ms per replay: the sweeps slow the chain)
+            if (!st) st = fp_getrf_nopiv_f32_batched(ctx, lu_tab.as<fp_lu_desc>(), lu_n.data(), (int)lu_n.size());
+            if (!st && (e = hipEventRecord(fork2, main)) != hipSuccess) st = fp_hip_fail(ctx, e, "hipEventRecord");
+            for (int q = 0; q < BATCH_STREAMS && !st; ++q)
+                if ((e = hipStreamWaitEvent(streams[q], fork2, 0)) != hipSuccess)
+                    st = fp_hip_fail(ctx, e, "hipStreamWaitEvent");
+            for (auto &bb : bodies) {
+                if (st) break;
+                ctx->stream = streams[bb->stream];
+                st = issue_body(*bb, ISSUE_POST);
+                ctx->stream = main;
+            }
         }
         for (int q = 0; q < BATCH_STREAMS && !st; ++q) {
             if ((e = hipEventRecord(join[q], streams[q])) != hipSuccess) st = fp_hip_fail(ctx, e, "hipEventRecord");

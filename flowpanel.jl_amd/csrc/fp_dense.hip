@@ -96,32 +96,38 @@ __device__ __forceinline__ void ldf(const float *__restrict__ p, bool full, bool
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t k, const T *__restrict__ A,
-                                                  int64_t lda, const T *__restrict__ B, int64_t ldb,
-                                                  T *__restrict__ C, int64_t ldc, int vec, int64_t sA = 0,
-                                                  int64_t sB = 0, int64_t sC = 0)
+// Tile of workgroup slot bx in the XCD-grouped order of an m x n product (GB_M x GB_N
+// tiles): false when the slot has no tile (the grid may be larger than the product).
+__device__ __forceinline__ bool gemm_tile_of(int64_t bx, int64_t m, int64_t n, int64_t &m0, int64_t &n0)
 {
-    static_assert(sizeof(T) == 4, "FP64 products run k_gemm_sub_f64");
-    typedef typename MfmaOp<T>::acc acc_t;
-    // strided batch: grid row y runs product y (all products the same shape)
-    A += (int64_t)blockIdx.y * sA;
-    B += (int64_t)blockIdx.y * sB;
-    C += (int64_t)blockIdx.y * sC;
-    __shared__ T As[2][GB_KF][GB_M + GB_PAD];
-    __shared__ T Bs[2][GB_KF][GB_N + GB_PAD];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
     const int64_t tiles_m = (m + GB_M - 1) / GB_M, tiles_n = (n + GB_N - 1) / GB_N;
     const int64_t ntiles = tiles_m * tiles_n;
     const int64_t per = (ntiles + GB_XCD - 1) / GB_XCD;
-    const int64_t t = (int64_t)(blockIdx.x % GB_XCD) * per + blockIdx.x / GB_XCD;
-    if (t >= ntiles) return;
+    if (bx >= per * GB_XCD) return false;
+    const int64_t t = (bx % GB_XCD) * per + bx / GB_XCD;
+    if (t >= ntiles) return false;
     const int64_t in_group = (int64_t)GB_GROUP * tiles_n;
     const int64_t first_m = (t / in_group) * GB_GROUP;
     const int64_t gsize = min((int64_t)GB_GROUP, tiles_m - first_m);
     const int64_t tm = first_m + (t % in_group) % gsize, tn = (t % in_group) / gsize;
-    const int64_t m0 = tm * GB_M, n0 = tn * GB_N;
+    m0 = tm * GB_M;
+    n0 = tn * GB_N;
+    return true;
+}
+
+// One 128 x 128 tile (m0, n0) of C -= A B in FP32 (the body of k_gemm_sub, shared
+// with the batched LU's trailing updates).
+__device__ __forceinline__ void gemm_sub_f32_tile(int64_t m, int64_t n, int64_t k, const float *__restrict__ A,
+                                                  int64_t lda, const float *__restrict__ B, int64_t ldb,
+                                                  float *__restrict__ C, int64_t ldc, int vec, int64_t m0,
+                                                  int64_t n0)
+{
+    typedef float T;
+    typedef typename MfmaOp<T>::acc acc_t;
+    __shared__ T As[2][GB_KF][GB_M + GB_PAD];
+    __shared__ T Bs[2][GB_KF][GB_N + GB_PAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
 
     acc_t acc[4][4];
 #pragma unroll
@@ -191,6 +197,46 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t 
                 const int64_t col = n0 + wn * 64 + b * 16 + (lane & 15);
                 if (interior || (row < m && col < n)) C[row * ldc + col] = cv[a][b][r] - acc[a][b][r];
             }
+}
+
+// The batched LU's trailing update of outer block [kk, kk + ob) for body blockIdx.y:
+// C = A[kend:, kend:] -= A[kend:, kk:kend] A[kk:kend, kend:] (bodies with n <= kend idle).
+__global__ __launch_bounds__(256) void k_gemm_sub_lub(const fp_lu_desc *__restrict__ desc, int64_t kk, int ob)
+{
+    const fp_lu_desc d = desc[blockIdx.y];
+    const int64_t kend = kk + ob, right = d.n - kend;
+    if (right <= 0) return;
+    int64_t m0, n0;
+    if (!gemm_tile_of(blockIdx.x, right, right, m0, n0)) return;
+    const float *Ap = d.A + kend * d.ld + kk, *Bp = d.A + kk * d.ld + kend;
+    float *Cp = d.A + kend * d.ld + kend;
+    const int vec = (((uintptr_t)Ap | (uintptr_t)Bp) & 15) == 0 && (d.ld * sizeof(float)) % 16 == 0;
+    gemm_sub_f32_tile(right, right, ob, Ap, d.ld, Bp, d.ld, Cp, d.ld, vec, m0, n0);
+}
+
+int fp_gemm_sub_lub(fp_ctx *ctx, const fp_lu_desc *d_desc, int nb, int64_t kk, int ob, int64_t max_right)
+{
+    if (max_right <= 0 || nb <= 0) return FP_OK;
+    const int64_t t = (max_right + GB_M - 1) / GB_M;
+    const int64_t grid = (t * t + GB_XCD - 1) / GB_XCD * GB_XCD;
+    ProfScope ps(ctx, "gemm_f32");
+    hipLaunchKernelGGL(k_gemm_sub_lub, dim3((unsigned)grid, (unsigned)nb), dim3(256), 0, ctx->stream, d_desc, kk, ob);
+    FP_HIP(ctx, hipGetLastError());
+    return FP_OK;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_sub(int64_t m, int64_t n, int64_t k, const T *__restrict__ A,
+                                                  int64_t lda, const T *__restrict__ B, int64_t ldb,
+                                                  T *__restrict__ C, int64_t ldc, int vec, int64_t sA = 0,
+                                                  int64_t sB = 0, int64_t sC = 0)
+{
+    static_assert(sizeof(T) == 4, "FP64 products run k_gemm_sub_f64");
+    // strided batch: grid row y runs product y (all products the same shape)
+    int64_t m0, n0;
+    if (!gemm_tile_of(blockIdx.x, m, n, m0, n0)) return;
+    gemm_sub_f32_tile(m, n, k, A + (int64_t)blockIdx.y * sA, lda, B + (int64_t)blockIdx.y * sB, ldb,
+                      C + (int64_t)blockIdx.y * sC, ldc, vec, m0, n0);
 }
 
 // FP64 tile kernel on v_mfma_f64_4x4x4_4b.  Operand layout of that instruction

@@ -263,6 +263,19 @@ size_t fp_getrf_nopiv_work_bytes();
 int fp_getrf_nopiv_f32(fp_ctx *ctx, float *A, int64_t n, int64_t ld, int *info_dev, void *work, float *YL = nullptr,
                        float *YU = nullptr);
 int fp_trtri_nopiv_f32(fp_ctx *ctx, const float *A, int64_t n, int64_t ld, float *YL, float *YU, float *T);
+// One body of the batched FP32 no-pivot LU (a device array of these, one per body):
+// A (n x n, ld, row-major), the LU's work (fp_getrf_nopiv_work_bytes), its info word.
+struct fp_lu_desc {
+    float *A;
+    int64_t ld, n;
+    float *work;
+    int *info;
+};
+// LU without pivoting of every body's matrix, one launch per step for all bodies
+// (n_host: the bodies' orders, as in the device table)
+int fp_getrf_nopiv_f32_batched(fp_ctx *ctx, const fp_lu_desc *d_desc, const int64_t *n_host, int nb);
+// the batched LU's trailing updates of outer block [kk, kk + ob) (fp_dense.hip)
+int fp_gemm_sub_lub(fp_ctx *ctx, const fp_lu_desc *d_desc, int nb, int64_t kk, int ob, int64_t max_right);
 // E^-1 in place (FP64, n x n) through the randomized no-pivot FP32 LU and the explicit
 // inverses of its factors; allocates nothing, never synchronises the host
 size_t fp_inverse_rbt_work_bytes(int64_t n);

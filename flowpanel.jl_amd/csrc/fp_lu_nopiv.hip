@@ -55,10 +55,9 @@ __device__ __forceinline__ float bcast(float v, int lane)
 // (the branchy form cost 37 us per block, a broadcast through LDS 46 us).
 // Then wave 0 inverts U from its registers while wave 1 inverts L from the
 // copy in LDS.
-__global__ __launch_bounds__(128) void k_nopiv_diag(float *__restrict__ A, int64_t ld, int64_t n, int64_t k0,
-                                                   float *__restrict__ Linv, float *__restrict__ Uinv,
-                                                   int *__restrict__ info, float *__restrict__ YL,
-                                                   float *__restrict__ YU)
+__device__ __forceinline__ void nopiv_diag_dev(float *__restrict__ A, int64_t ld, int64_t n, int64_t k0,
+                                               float *__restrict__ Linv, float *__restrict__ Uinv,
+                                               int *__restrict__ info, float *__restrict__ YL, float *__restrict__ YU)
 {
     __shared__ float lu[NPB][NPB + 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -147,6 +146,14 @@ __global__ __launch_bounds__(128) void k_nopiv_diag(float *__restrict__ A, int64
     }
 }
 
+__global__ __launch_bounds__(128) void k_nopiv_diag(float *__restrict__ A, int64_t ld, int64_t n, int64_t k0,
+                                                   float *__restrict__ Linv, float *__restrict__ Uinv,
+                                                   int *__restrict__ info, float *__restrict__ YL,
+                                                   float *__restrict__ YU)
+{
+    nopiv_diag_dev(A, ld, n, k0, Linv, Uinv, info, YL, YU);
+}
+
 __global__ void k_fill_info(int *p, int v)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
@@ -158,14 +165,13 @@ __global__ void k_fill_info(int *p, int v)
 // Both operands in LDS; wave w computes output rows 16w..16w+15 with
 // v_mfma_f32_16x16x4 (lane l supplies P(l%16, k + l/16) and Q(k + l/16, l%16);
 // accumulator element r holds C(4 (l/16) + r, l%16)).
-__global__ __launch_bounds__(256) void k_nopiv_panel(float *__restrict__ A, int64_t ld, int64_t n, int64_t k0,
-                                                    const float *__restrict__ Linv, const float *__restrict__ Uinv,
-                                                    int nl, int64_t u_off)
+__device__ __forceinline__ void nopiv_panel_dev(float *__restrict__ A, int64_t ld, int64_t n, int64_t k0,
+                                                const float *__restrict__ Linv, const float *__restrict__ Uinv, int nl,
+                                                int64_t u_off, int b)
 {
     __shared__ float P[NPB][NPB + 4];
     __shared__ float Q[NPB][NPB + 4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int b = blockIdx.x;
     const bool lpanel = b < nl;
     const int64_t off = lpanel ? k0 + NPB + (int64_t)NPB * b : u_off + (int64_t)NPB * (b - nl);
     float *tile = lpanel ? A + off * ld + k0 : A + k0 * ld + off;
@@ -224,15 +230,15 @@ __global__ __launch_bounds__(256) void k_nopiv_panel(float *__restrict__ A, int6
 // = tile (row-tile major), K in chunks of 64 through LDS, v_mfma_f32_16x16x4 as in
 // k_nopiv_panel (wave w: output rows 16w..16w+15).  Loads of partial tiles use
 // clamped addresses and a 0/1 factor (every load in bounds and unconditional).
-__global__ __launch_bounds__(256) void k_gemm64_sub(int64_t m, int64_t n, int64_t kk, const float *__restrict__ A,
-                                                   int64_t lda, const float *__restrict__ B, int64_t ldb,
-                                                   float *__restrict__ C, int64_t ldc)
+__device__ __forceinline__ void gemm64_dev(int64_t m, int64_t n, int64_t kk, const float *__restrict__ A, int64_t lda,
+                                           const float *__restrict__ B, int64_t ldb, float *__restrict__ C, int64_t ldc,
+                                           int64_t tile)
 {
     __shared__ float P[NPB][NPB + 4];
     __shared__ float Q[NPB][NPB + 4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t tn_count = (n + NPB - 1) / NPB;
-    const int64_t i0 = (int64_t)(blockIdx.x / tn_count) * NPB, j0 = (int64_t)(blockIdx.x % tn_count) * NPB;
+    const int64_t i0 = (tile / tn_count) * NPB, j0 = (tile % tn_count) * NPB;
     const int tr = (int)min((int64_t)NPB, m - i0), tc = (int)min((int64_t)NPB, n - j0);
     const int r = tid >> 2, c0 = (tid & 3) * 16;
     nf4 acc[4];
@@ -278,6 +284,13 @@ __global__ __launch_bounds__(256) void k_gemm64_sub(int64_t m, int64_t n, int64_
     }
 }
 
+__global__ __launch_bounds__(256) void k_gemm64_sub(int64_t m, int64_t n, int64_t kk, const float *__restrict__ A,
+                                                   int64_t lda, const float *__restrict__ B, int64_t ldb,
+                                                   float *__restrict__ C, int64_t ldc)
+{
+    gemm64_dev(m, n, kk, A, lda, B, ldb, C, ldc, blockIdx.x);
+}
+
 static int gemm64_sub(fp_ctx *ctx, int64_t m, int64_t n, int64_t k, const float *A, int64_t lda, const float *B,
                       int64_t ldb, float *C, int64_t ldc)
 {
@@ -287,6 +300,13 @@ static int gemm64_sub(fp_ctx *ctx, int64_t m, int64_t n, int64_t k, const float 
     hipLaunchKernelGGL(k_gemm64_sub, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, m, n, k, A, lda, B, ldb, C, ldc);
     FP_HIP(ctx, hipGetLastError());
     return FP_OK;
+}
+
+__global__ __launch_bounds__(256) void k_nopiv_panel(float *__restrict__ A, int64_t ld, int64_t n, int64_t k0,
+                                                    const float *__restrict__ Linv, const float *__restrict__ Uinv,
+                                                    int nl, int64_t u_off)
+{
+    nopiv_panel_dev(A, ld, n, k0, Linv, Uinv, nl, u_off, blockIdx.x);
 }
 
 // U^T A V over the orbit blocks: thread (p, q) reads the 4 x 4 block of A at rows
@@ -584,6 +604,123 @@ int fp_getrf_nopiv_f32(fp_ctx *ctx, float *A, int64_t n, int64_t ld, int *info_d
         if ((st = fp_gemm_sub<float>(ctx, right, right, ob, A + kend * ld + k, ld, A + k * ld + kend, ld,
                                      A + kend * ld + kend, ld)))
             return st;
+    }
+    return FP_OK;
+}
+
+// ---- the same factorisation for many bodies at once --------------------------
+// One launch per step covers every body (blockIdx.y or blockIdx.x = body): the bodies'
+// serial chains of 64-column diagonal blocks run side by side, and the small products
+// of different bodies fill the chip together.  Global outer blocks [kk, kk + NOB);
+// a body takes part in the steps inside its order.
+
+__global__ void k_fill_info_lub(const fp_lu_desc *__restrict__ desc, int nb)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nb) *desc[b].info = 0;
+}
+
+// inner step t (column c = kk + 64 t) of body blockIdx.x
+__global__ __launch_bounds__(128) void k_nopiv_diag_lub(const fp_lu_desc *__restrict__ desc, int64_t c, int t)
+{
+    const fp_lu_desc d = desc[blockIdx.x];
+    if (c >= d.n) return;   // uniform per workgroup
+    float *Linv = d.work + (int64_t)2 * t * NPB * NPB;
+    nopiv_diag_dev(d.A, d.ld, d.n, c, Linv, Linv + NPB * NPB, d.info, nullptr, nullptr);
+}
+
+// mode 0: the panel tiles of inner step t (L tiles below, U tiles inside the outer block);
+// mode 1: the U-row tiles of inner block t right of the outer block
+__global__ __launch_bounds__(256) void k_nopiv_panel_lub(const fp_lu_desc *__restrict__ desc, int64_t kk, int t, int mode)
+{
+    const fp_lu_desc d = desc[blockIdx.y];
+    const int64_t c = kk + (int64_t)NPB * t;
+    const float *Linv = d.work + (int64_t)2 * t * NPB * NPB, *Uinv = Linv + NPB * NPB;
+    if (mode == 0) {
+        const int64_t below = d.n - c - NPB;
+        if (c >= d.n || below <= 0) return;
+        const int64_t kend = min(kk + (int64_t)NOB, d.n), inpanel = kend - c - NPB;
+        const int nl = (int)((below + NPB - 1) / NPB), nu = inpanel > 0 ? (int)((inpanel + NPB - 1) / NPB) : 0;
+        if ((int)blockIdx.x >= nl + nu) return;
+        nopiv_panel_dev(d.A, d.ld, d.n, c, Linv, Uinv, nl, c + NPB, blockIdx.x);
+    } else {
+        const int64_t kend = kk + NOB, right = d.n - kend;
+        if (right <= 0 || (int64_t)blockIdx.x >= (right + NPB - 1) / NPB) return;
+        nopiv_panel_dev(d.A, d.ld, d.n, c, Linv, Uinv, 0, kend, blockIdx.x);
+    }
+}
+
+// mode 0: the in-block update after inner step t; mode 1: the U rows' update before inner block t
+__global__ __launch_bounds__(256) void k_gemm64_lub(const fp_lu_desc *__restrict__ desc, int64_t kk, int t, int mode)
+{
+    const fp_lu_desc d = desc[blockIdx.y];
+    const int64_t c = kk + (int64_t)NPB * t, ld = d.ld;
+    float *A = d.A;
+    if (mode == 0) {
+        const int64_t below = d.n - c - NPB, kend = min(kk + (int64_t)NOB, d.n), inpanel = kend - c - NPB;
+        if (c >= d.n || below <= 0 || inpanel <= 0) return;
+        if ((int64_t)blockIdx.x >= ((below + NPB - 1) / NPB) * ((inpanel + NPB - 1) / NPB)) return;
+        gemm64_dev(below, inpanel, NPB, A + (c + NPB) * ld + c, ld, A + c * ld + c + NPB, ld,
+                   A + (c + NPB) * ld + c + NPB, ld, blockIdx.x);
+    } else {
+        const int64_t kend = kk + NOB, right = d.n - kend;
+        if (right <= 0 || (int64_t)blockIdx.x >= (right + NPB - 1) / NPB) return;
+        gemm64_dev(NPB, right, c - kk, A + c * ld + kk, ld, A + kk * ld + kend, ld, A + c * ld + kend, ld, blockIdx.x);
+    }
+}
+
+int fp_getrf_nopiv_f32_batched(fp_ctx *ctx, const fp_lu_desc *d_desc, const int64_t *n_host, int nb)
+{
+    if (nb <= 0) return FP_OK;
+    if (nb > 65535) return fp_fail(ctx, FP_ERR_UNSUPPORTED, "batched LU: too many bodies");
+    int64_t nmax = 0;
+    for (int b = 0; b < nb; ++b) nmax = std::max(nmax, n_host[b]);
+    hipLaunchKernelGGL(k_fill_info_lub, dim3((unsigned)((nb + 63) / 64)), dim3(64), 0, ctx->stream, d_desc, nb);
+    auto tiles = [](int64_t x) { return x > 0 ? (x + NPB - 1) / NPB : (int64_t)0; };
+    int st;
+    for (int64_t kk = 0; kk < nmax; kk += NOB) {
+        for (int t = 0; t < NOB / NPB; ++t) {
+            const int64_t c = kk + (int64_t)NPB * t;
+            if (c >= nmax) break;
+            int64_t gp = 0, gg = 0;   // the largest grids over the bodies
+            for (int b = 0; b < nb; ++b) {
+                const int64_t n = n_host[b];
+                if (c >= n) continue;
+                const int64_t below = n - c - NPB, inpanel = std::min(kk + (int64_t)NOB, n) - c - NPB;
+                gp = std::max(gp, tiles(below) + (below > 0 ? tiles(inpanel) : 0));
+                if (below > 0 && inpanel > 0) gg = std::max(gg, tiles(below) * tiles(inpanel));
+            }
+            {
+                ProfScope ps(ctx, "lu_nopiv_diag");
+                hipLaunchKernelGGL(k_nopiv_diag_lub, dim3((unsigned)nb), dim3(128), 0, ctx->stream, d_desc, c, t);
+            }
+            if (gp > 0) {
+                ProfScope ps(ctx, "lu_nopiv_panel");
+                hipLaunchKernelGGL(k_nopiv_panel_lub, dim3((unsigned)gp, (unsigned)nb), dim3(256), 0, ctx->stream,
+                                   d_desc, kk, t, 0);
+            }
+            if (gg > 0) {
+                ProfScope ps(ctx, "gemm_f32");
+                hipLaunchKernelGGL(k_gemm64_lub, dim3((unsigned)gg, (unsigned)nb), dim3(256), 0, ctx->stream, d_desc, kk,
+                                   t, 0);
+            }
+            FP_HIP(ctx, hipGetLastError());
+        }
+        int64_t right = 0;
+        for (int b = 0; b < nb; ++b) right = std::max(right, n_host[b] - kk - NOB);
+        if (right <= 0) continue;
+        for (int t = 0; t < NOB / NPB; ++t) {
+            if (t > 0) {
+                ProfScope ps(ctx, "gemm_f32");
+                hipLaunchKernelGGL(k_gemm64_lub, dim3((unsigned)tiles(right), (unsigned)nb), dim3(256), 0, ctx->stream,
+                                   d_desc, kk, t, 1);
+            }
+            ProfScope ps(ctx, "lu_nopiv_panel");
+            hipLaunchKernelGGL(k_nopiv_panel_lub, dim3((unsigned)tiles(right), (unsigned)nb), dim3(256), 0, ctx->stream,
+                               d_desc, kk, t, 1);
+            FP_HIP(ctx, hipGetLastError());
+        }
+        if ((st = fp_gemm_sub_lub(ctx, d_desc, nb, kk, NOB, right))) return st;
     }
     return FP_OK;
 }

@@ -107,3 +107,27 @@ def test_body_batch_matches_single_solves(fp, geo, solver, resid):
     st = batch.solve()
     assert st["recaptured"] == 1
     assert rel(b.strength, ref) < 1e-10
+
+
+@pytest.mark.parametrize("lu", ["bodies", "batched"])
+def test_body_batch_factorisations_agree(fp, geo, lu, monkeypatch):
+    """The batched FP32 no-pivot LU (one launch per step for every body: bodies of 1,152 to
+    4,560 unknowns, so the steps end at different outer blocks and the last outer blocks are
+    ragged) against each body's own LU launches (FLOWPANEL_BATCH_LU=bodies), through the
+    whole mixed solve with three graph replays: both reach the FP64 LU's strengths to 1e-10."""
+    if lu == "bodies":
+        monkeypatch.setenv("FLOWPANEL_BATCH_LU", "bodies")
+    bodies = _wings(fp, geo, [(48, 24), (60, 76), (36, 32), (20, 60)])
+    refs = []
+    for k, b in enumerate(bodies):
+        V = fp.direction(3.0 + k)
+        U = np.tile(V[:, None], (1, b.ncells))
+        D = np.tile(V[:, None], (1, b.nsheddings))
+        fp.solve(b, U, D, D)
+        refs.append(b.strength.copy())
+    batch = fp.BodyBatch(bodies, solver=fp.solve_lu32_ir, solver_optargs={"residual_mode": 0, "tol": 1e-12})
+    for rep in range(3):
+        st = batch.solve()
+        assert st["residual"] <= 1e-12, st
+        for b, ref in zip(bodies, refs):
+            assert rel(b.strength, ref) < 1e-10, (lu, rep, rel(b.strength, ref))
