@@ -46,15 +46,105 @@ __device__ __forceinline__ float bcast(float v, int lane)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+// Compile-time loop: f(cint<I>{}) for I = B, B + S, ... while I < E (S > 0)
+// or I > E (S < 0), so every register-array index below is a constant.
+template <int V> struct cint {
+    static constexpr int v = V;
+};
+template <int B, int E, int S, typename F> __device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (S > 0 ? B < E : B > E) {
+        f(cint<B>{});
+        static_for<B + S, E, S>(f);
+    }
+}
+
+// v[q] -= f * (v[q] of lane j) for q in [LO, HI): the broadcasts go out in
+// groups of four before their FMAs, so that no v_readlane's result is
+// consumed by the very next instruction (a readlane followed at once by its
+// use costs two wait states, which doubled the length of these loops).
+template <int LO, int HI, int G = 4>
+__device__ __forceinline__ void bcast_axpy(float (&v)[NPB], float f, int j)
+{
+#pragma unroll
+    for (int q0 = LO; q0 < HI; q0 += G) {
+        float t[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+            if (q0 + u < HI) t[u] = bcast(v[q0 + u], j);
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+            if (q0 + u < HI) v[q0 + u] = fmaf(-f, t[u], v[q0 + u]);
+    }
+}
+
+// Columns [Q0, Q1) of X = U11^-1 for row `lane`, from row `lane` of U in u[]:
+// rows bottom-up, Y_i -= (u_ic / u_cc) Y_c for i < c, then X_i = Y_i / u_ii.
+// Column q takes the steps c <= q, so a column range is independent of the rest.
+template <int Q0, int Q1>
+__device__ __forceinline__ void nopiv_uinv_cols(const float (&u)[NPB], const float (*lu)[NPB + 1], int lane, int bs,
+                                                int64_t k0, int64_t ld, float *__restrict__ Uinv,
+                                                float *__restrict__ YU)
+{
+    float x[NPB];
+#pragma unroll
+    for (int q = Q0; q < Q1; ++q) x[q] = q == lane ? 1.f : 0.f;
+    float dii = 1.f;
+    static_for<NPB - 1, -1, -1>([&](auto C) {
+        constexpr int c = decltype(C)::v;
+        dii = lane == c ? u[c] : dii;
+        if constexpr (c < Q1) {   // else no column of the range at or right of c
+            const float coef = lane < c ? u[c] * (1.f / lu[c][c]) : 0.f;
+            bcast_axpy<(c > Q0 ? c : Q0), Q1>(x, coef, c);
+        }
+    });
+    const float rd = 1.f / dii;
+#pragma unroll
+    for (int c = Q0; c < Q1; c += 4)
+        *reinterpret_cast<float4 *>(Uinv + lane * NPB + c) =
+            make_float4(x[c] * rd, x[c + 1] * rd, x[c + 2] * rd, x[c + 3] * rd);
+    if (YU && lane < bs) {   // -U11^-1 into the diagonal block of the full inverse (ld as A)
+#pragma unroll
+        for (int c = Q0; c < Q1; ++c)
+            if (c < bs) YU[(k0 + lane) * ld + k0 + c] = -(x[c] * rd);
+    }
+}
+
+// Columns [Q0, Q1) of X = L11^-1 (unit lower) for row `lane`, the row's
+// multipliers l_lane,c in l[] (zero at and right of the diagonal): rows
+// top-down, X_i -= l_ic X_c for i > c; column q takes the steps c >= q.
+template <int Q0, int Q1>
+__device__ __forceinline__ void nopiv_linv_cols(const float (&l)[NPB], int lane, int bs, int64_t k0, int64_t ld,
+                                                float *__restrict__ Linv, float *__restrict__ YL)
+{
+    float x[NPB];
+#pragma unroll
+    for (int q = Q0; q < Q1; ++q) x[q] = q == lane ? 1.f : 0.f;
+    static_for<Q0, NPB, 1>([&](auto C) {
+        constexpr int c = decltype(C)::v;
+        bcast_axpy<Q0, (c + 1 < Q1 ? c + 1 : Q1)>(x, l[c], c);
+    });
+#pragma unroll
+    for (int c = Q0; c < Q1; c += 4)
+        *reinterpret_cast<float4 *>(Linv + lane * NPB + c) = make_float4(x[c], x[c + 1], x[c + 2], x[c + 3]);
+    if (YL && lane < bs) {   // -L11^-1 likewise
+#pragma unroll
+        for (int c = Q0; c < Q1; ++c)
+            if (c < bs) YL[(k0 + lane) * ld + k0 + c] = -x[c];
+    }
+}
+
 // Diagonal block at (k0, k0): LU without pivoting in place (rows beyond n are
 // the identity), then Linv = L11^-1 (unit lower) and Uinv = U11^-1, 64 x 64
-// row-major.  Wave 0 holds row `lane` of the block in registers and factors
-// it; the broadcast of row j is one v_readlane per entry (j is a compile-time
-// index, every loop is unrolled), and every row takes the update, rows at and
-// above the pivot with a zero multiplier, so no update hides behind a branch
-// (the branchy form cost 37 us per block, a broadcast through LDS 46 us).
-// Then wave 0 inverts U from its registers while wave 1 inverts L from the
-// copy in LDS.
+// row-major; 512 threads.  Wave 0 holds row `lane` of the block in registers
+// and factors it; the broadcast of row j is one v_readlane per entry (j is a
+// compile-time index, every loop is unrolled), and every row takes the update,
+// rows at and above the pivot with a zero multiplier, so no update hides
+// behind a branch (the branchy form cost 37 us per block, a broadcast through
+// LDS 46 us).  Then the eight waves invert: waves 0-3 four column ranges of
+// U^-1, waves 4-7 four of L^-1, the ranges cut so that each wave takes about
+// an eighth of the 2 x 2,080 broadcast-and-FMA steps (U column q costs q + 1,
+// L column q costs 64 - q).
 __device__ __forceinline__ void nopiv_diag_dev(float *__restrict__ A, int64_t ld, int64_t n, int64_t k0,
                                                float *__restrict__ Linv, float *__restrict__ Uinv,
                                                int *__restrict__ info, float *__restrict__ YL, float *__restrict__ YU)
@@ -62,7 +152,7 @@ __device__ __forceinline__ void nopiv_diag_dev(float *__restrict__ A, int64_t ld
     __shared__ float lu[NPB][NPB + 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int bs = (int)min((int64_t)NPB, n - k0);
-    float a[NPB], x[NPB];
+    float a[NPB];
     if (wave == 0) {
         float *row = A + (k0 + lane) * ld + k0;
         if (bs == NPB) {
@@ -76,16 +166,15 @@ __device__ __forceinline__ void nopiv_diag_dev(float *__restrict__ A, int64_t ld
             for (int c = 0; c < NPB; ++c) a[c] = (lane < bs && c < bs) ? row[c] : (lane == c ? 1.f : 0.f);
         }
         bool bad = false;
-#pragma unroll
-        for (int j = 0; j < NPB; ++j) {
+        static_for<0, NPB, 1>([&](auto J) {
+            constexpr int j = decltype(J)::v;
             const float piv = bcast(a[j], j);
             bad = bad || !(fabsf(piv) > 0.f) || !(fabsf(piv) <= 3.4e38f);
             // rows at and above j take a zero multiplier (unconditional updates, no branches)
             const float l = lane > j ? a[j] / piv : 0.f;
             a[j] = lane > j ? l : a[j];
-#pragma unroll
-            for (int c = j + 1; c < NPB; ++c) a[c] = fmaf(-l, bcast(a[c], j), a[c]);
-        }
+            bcast_axpy<j + 1, NPB>(a, l, j);
+        });
         if (bad && lane == 0) atomicCAS(info, 0, (int)(k0 + 1));
         if (lane < bs) {
             if (bs == NPB) {
@@ -102,51 +191,27 @@ __device__ __forceinline__ void nopiv_diag_dev(float *__restrict__ A, int64_t ld
         for (int c = 0; c < NPB; ++c) lu[lane][c] = a[c];
     }
     __syncthreads();
-#pragma unroll
-    for (int c = 0; c < NPB; ++c) x[c] = c == lane ? 1.f : 0.f;
     if (wave == 0) {
-        // U^-1: rows bottom-up; Y_i -= (u_ic / u_cc) Y_c for i < c, then X_i = Y_i / u_ii
-        float dii = 1.f;
-#pragma unroll
-        for (int c = NPB - 1; c >= 0; --c) {
-            const float rc = 1.f / lu[c][c];
-            const float coef = lane < c ? a[c] * rc : 0.f;
-            dii = lane == c ? a[c] : dii;
-#pragma unroll
-            for (int q = c; q < NPB; ++q) x[q] = fmaf(-coef, bcast(x[q], c), x[q]);
-        }
-        const float rd = 1.f / dii;
-#pragma unroll
-        for (int c = 0; c < NPB; c += 4)
-            *reinterpret_cast<float4 *>(Uinv + lane * NPB + c) =
-                make_float4(x[c] * rd, x[c + 1] * rd, x[c + 2] * rd, x[c + 3] * rd);
-        if (YU && lane < bs) {   // -U11^-1 into the diagonal block of the full inverse (ld as A)
-#pragma unroll
-            for (int c = 0; c < NPB; ++c)
-                if (c < bs) YU[(k0 + lane) * ld + k0 + c] = -(x[c] * rd);
-        }
-    } else {
-        // L^-1 (unit lower): rows top-down; X_i -= l_ic X_c for i > c
-#pragma unroll
-        for (int c = 0; c < NPB; ++c) a[c] = c < lane ? lu[lane][c] : 0.f;
-#pragma unroll
-        for (int c = 0; c < NPB; ++c) {
-            const float coef = a[c];
-#pragma unroll
-            for (int q = 0; q <= c; ++q) x[q] = fmaf(-coef, bcast(x[q], c), x[q]);
-        }
-#pragma unroll
-        for (int c = 0; c < NPB; c += 4)
-            *reinterpret_cast<float4 *>(Linv + lane * NPB + c) = make_float4(x[c], x[c + 1], x[c + 2], x[c + 3]);
-        if (YL && lane < bs) {   // -L11^-1 likewise
-#pragma unroll
-            for (int c = 0; c < NPB; ++c)
-                if (c < bs) YL[(k0 + lane) * ld + k0 + c] = -x[c];
-        }
+        nopiv_uinv_cols<0, 32>(a, lu, lane, bs, k0, ld, Uinv, YU);
+        return;
     }
+    if (wave < 4) {   // U part of row `lane` (the strictly lower entries are never read)
+#pragma unroll
+        for (int c = 0; c < NPB; ++c) a[c] = lu[lane][c];
+        if (wave == 1) nopiv_uinv_cols<32, 44>(a, lu, lane, bs, k0, ld, Uinv, YU);
+        else if (wave == 2) nopiv_uinv_cols<44, 56>(a, lu, lane, bs, k0, ld, Uinv, YU);
+        else nopiv_uinv_cols<56, 64>(a, lu, lane, bs, k0, ld, Uinv, YU);
+        return;
+    }
+#pragma unroll
+    for (int c = 0; c < NPB; ++c) a[c] = c < lane ? lu[lane][c] : 0.f;
+    if (wave == 4) nopiv_linv_cols<0, 8>(a, lane, bs, k0, ld, Linv, YL);
+    else if (wave == 5) nopiv_linv_cols<8, 20>(a, lane, bs, k0, ld, Linv, YL);
+    else if (wave == 6) nopiv_linv_cols<20, 32>(a, lane, bs, k0, ld, Linv, YL);
+    else nopiv_linv_cols<32, 64>(a, lane, bs, k0, ld, Linv, YL);
 }
 
-__global__ __launch_bounds__(128) void k_nopiv_diag(float *__restrict__ A, int64_t ld, int64_t n, int64_t k0,
+__global__ __launch_bounds__(512) void k_nopiv_diag(float *__restrict__ A, int64_t ld, int64_t n, int64_t k0,
                                                    float *__restrict__ Linv, float *__restrict__ Uinv,
                                                    int *__restrict__ info, float *__restrict__ YL,
                                                    float *__restrict__ YU)
@@ -568,7 +633,7 @@ int fp_getrf_nopiv_f32(fp_ctx *ctx, float *A, int64_t n, int64_t ld, int *info_d
             float *Linv = (float *)work + (int64_t)2 * t * NPB * NPB, *Uinv = Linv + NPB * NPB;
             {
                 ProfScope ps(ctx, "lu_nopiv_diag");
-                hipLaunchKernelGGL(k_nopiv_diag, dim3(1), dim3(128), 0, ctx->stream, A, ld, n, c, Linv, Uinv, info_dev,
+                hipLaunchKernelGGL(k_nopiv_diag, dim3(1), dim3(512), 0, ctx->stream, A, ld, n, c, Linv, Uinv, info_dev,
                                    YL, YU);
             }
             const int64_t below = n - c - NPB, inpanel = kend - c - NPB;
@@ -621,7 +686,7 @@ __global__ void k_fill_info_lub(const fp_lu_desc *__restrict__ desc, int nb)
 }
 
 // inner step t (column c = kk + 64 t) of body blockIdx.x
-__global__ __launch_bounds__(128) void k_nopiv_diag_lub(const fp_lu_desc *__restrict__ desc, int64_t c, int t)
+__global__ __launch_bounds__(512) void k_nopiv_diag_lub(const fp_lu_desc *__restrict__ desc, int64_t c, int t)
 {
     const fp_lu_desc d = desc[blockIdx.x];
     if (c >= d.n) return;   // uniform per workgroup
@@ -692,7 +757,7 @@ int fp_getrf_nopiv_f32_batched(fp_ctx *ctx, const fp_lu_desc *d_desc, const int6
             }
             {
                 ProfScope ps(ctx, "lu_nopiv_diag");
-                hipLaunchKernelGGL(k_nopiv_diag_lub, dim3((unsigned)nb), dim3(128), 0, ctx->stream, d_desc, c, t);
+                hipLaunchKernelGGL(k_nopiv_diag_lub, dim3((unsigned)nb), dim3(512), 0, ctx->stream, d_desc, c, t);
             }
             if (gp > 0) {
                 ProfScope ps(ctx, "lu_nopiv_panel");
